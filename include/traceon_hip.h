@@ -1,0 +1,222 @@
+/*
+ * traceon_hip.h — C ABI of libtraceon_hip.so, the MI355X (gfx950) implementation of
+ * Traceon's radially symmetric BEM path.
+ *
+ * Two tiers.
+ *
+ * Tier R — the reference's own FFI.  Same symbol names, argument order, array layouts
+ * and accumulate semantics as the C backend that traceon/backend/__init__.py binds
+ * (signature table at traceon/backend/__init__.py:287-328).  Arguments are HOST pointers
+ * (numpy arrays); every call copies its inputs to the GPU, runs there and copies the
+ * results back before returning.  A maintainer switches Traceon to this library by
+ * pointing the loader at it (traceon/backend/__init__.py:23-55); see INTEGRATION.md.
+ * Where the reference has no error channel (void functions, exit(1) on a bad type) the
+ * call records a message readable with tb_last_error() and leaves its outputs untouched.
+ *
+ * Tier B — batched, device-resident entry points the reference lacks (the GPU's way of
+ * doing the same work).  Pointers are DEVICE pointers, `stream` is a hipStream_t (NULL:
+ * the default stream), calls are asynchronous on that stream and return 0 on success or
+ * -1 with tb_last_error() set.
+ *
+ * Units are SI; charges are stored as sigma/epsilon_0 (traceon/__init__.py:35).
+ */
+#ifndef TRACEON_HIP_H
+#define TRACEON_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <stdbool.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------------------ */
+/* Tier R — data symbols read with ctypes in_dll (traceon/backend/__init__.py:58-66)    */
+/* ------------------------------------------------------------------------------------ */
+extern const size_t TRACING_BLOCK_SIZE;   /* tracing.c:7 */
+extern const int N_QUAD_2D_SYM;           /* utilities_2d.c:2 */
+extern const int N_TRIANGLE_QUAD_SYM;     /* utilities_3d.c:4 */
+extern const int DERIV_2D_MAX_SYM;        /* radial_ring.c:140 */
+extern const int NU_MAX_SYM;              /* three_d.c:10 */
+extern const int M_MAX_SYM;               /* three_d.c:11 */
+
+/* Reference structs carried through void* arguments (radial.c:147-152, three_d.c:18-31,
+   radial.c:312-317).  Pointers inside are host pointers. */
+struct effective_point_charges_2d { double *charges; double *jacobians; double *positions; size_t N; };
+struct effective_point_charges_3d { double *charges; double *jacobians; double *positions; size_t N; };
+struct field_evaluation_args { void *elec_charges; void *mag_charges; void *currents; double *bounds; };
+struct field_derivs_args { double *z_interpolation; double *electrostatic_axial_coeffs; double *magnetostatic_axial_coeffs; size_t N_z; };
+
+typedef void (*field_fun)(double *position, double *velocity, void *args, double *elec_out, double *mag_out);
+
+/* elliptic.c:9-75 — Cody (1965) complete elliptic integrals (backend/__init__.py:291-294) */
+double ellipkm1(double p);
+double ellipem1(double p);
+double ellipk(double k);
+double ellipe(double k);
+
+/* utilities_2d.c:18-87 (backend/__init__.py:295-300) */
+void normal_2d(const double *p1, const double *p2, double *normal);
+void higher_order_normal_radial(double alpha, const double *v1, const double *v2, const double *v3, const double *v4, double *normal);
+void position_and_jacobian_radial(double alpha, const double *v1, const double *v2, const double *v3, const double *v4,
+                                  double *pos_out, double *jac);
+void delta_position_and_jacobian_radial(double alpha, const double *v1, const double *v2, const double *v3, const double *v4,
+                                        double *pos_out, double *jac);
+
+/* radial_ring.c:4-107 (backend/__init__.py:302-304, 308, 316, 318) */
+double potential_radial_ring(double r0, double z0, double delta_r, double delta_z, void *unused);
+double dr1_potential_radial_ring(double r0, double z0, double delta_r, double delta_z, void *unused);
+double dz1_potential_radial_ring(double r0, double z0, double delta_r, double delta_z, void *unused);
+double flux_density_to_charge_factor(double K);
+double current_potential_axial_radial_ring(double z0, double r, double z);
+void current_field_radial_ring(double x0, double y0, double x, double y, double *result);
+
+/* radial.c:361-383, three_d.c:34-62 (backend/__init__.py:321, 325) */
+void fill_jacobian_buffer_radial(double *jacobian_buffer, double *pos_buffer, const double *line_points, size_t N_lines);
+void fill_jacobian_buffer_3d(double *jacobian_buffer, double *pos_buffer, const double *triangles, size_t N_triangles);
+
+/* radial.c:386-447 (backend/__init__.py:324): rows [start,end] inclusive, ACCUMULATES into matrix
+   (row stride N_matrix).  An unknown excitation type records an error and changes nothing. */
+void fill_matrix_radial(double *matrix, const double *line_points, const uint8_t *excitation_types,
+                        const double *excitation_values, const double *jacobian_buffer, const double *pos_buffer,
+                        size_t N_lines, size_t N_matrix, int lines_range_start, int lines_range_end);
+
+/* The singular diagonal.  The reference integrates the integrands of radial.c:319-359 with
+   scipy.integrate.quad (backend/__init__.py:688-708); here the whole integral is one call.
+   self_field_dot_normal_radial_integral returns the integral itself (the solver subtracts 1,
+   solver.py:245). */
+double self_potential_radial_integral(const double *line_points);
+double self_field_dot_normal_radial_integral(const double *line_points, double K);
+
+/* radial.c:155-310, 449-521 (backend/__init__.py:305-307, 309-311, 317, 319-320) */
+double potential_radial(const double *point, const double *charges, const double *jacobian_buffer, const double *pos_buffer, size_t N);
+void field_radial(const double *point, double *result, const double *charges, const double *jacobian_buffer, const double *pos_buffer,
+                  size_t N);
+double current_potential_axial(double z0, const double *currents, const double *jacobian_buffer, const double *pos_buffer, size_t N);
+void current_field_radial(const double *point, double *result, const double *currents, const double *jacobian_buffer,
+                          const double *pos_buffer, size_t N);
+void axial_derivatives_radial(double *derivs, const double *charges, const double *jacobian_buffer, const double *pos_buffer,
+                              size_t N_lines, const double *z, size_t N_z);   /* accumulates */
+void current_axial_derivatives_radial(double *derivs, const double *currents, const double *jacobian_buffer,
+                                      const double *pos_buffer, size_t N, const double *z, size_t N_z);   /* accumulates */
+double potential_radial_derivs(const double *point, const double *z_inter, const double *coeff, size_t N_z);
+void field_radial_derivs(const double *point, double *field, const double *z_inter, const double *coeff, size_t N_z);
+double charge_radial(const double *vertices, double charge);
+
+/* tracing.c:60-223 (backend/__init__.py:301, 326).  trace_particle runs on the GPU when `field`
+   is this library's field_radial_traceable or field_radial_derivs_traceable (the two C field
+   functions the reference passes, field.py:814-816, 1064-1066); any other function pointer
+   (a Python callback) records an error and returns 0. */
+size_t trace_particle(double *times_array, double *pos_array, double charge_over_mass, field_fun field,
+                      const double *bounds, double atol, void *args);
+void field_radial_traceable(double *position, double *velocity, void *args, double *elec_out, double *mag_out);
+void field_radial_derivs_traceable(double *position, double *velocity, void *args, double *elec_out, double *mag_out);
+bool plane_intersection(const double *p0, const double *normal, const double *positions, size_t N_p, double *result);
+
+/* ------------------------------------------------------------------------------------ */
+/* Tier B — device-resident batched entry points (no reference counterpart unless noted) */
+/* ------------------------------------------------------------------------------------ */
+const char *tb_last_error(void);
+void tb_clear_error(void);
+int tb_version(void);
+int tb_synchronize(void *stream);
+
+/* Element preparation for the fill (fill_jacobian_buffer_radial + the per-row data of
+   fill_matrix_radial, radial.c:224-310).  src: 3 x (16, N) SoA (r, z, weight*jac);
+   rows: N x tb_rowinfo_bytes(); bad_type (device int, zeroed by caller) is set to 1 if a
+   type code is not 1, 2, 3, 5 or 6.  jac/pos may be NULL. */
+size_t tb_rowinfo_bytes(void);
+int tb_prepare_elements(const double *lines, const uint8_t *types, const double *values, int N, double *jac, double *pos,
+                        double *src, void *rows, int *bad_type, void *stream);
+/* Dense fill of rows [row0,row1]: stores (accumulate = 0) or adds; M32 (nullable) receives an
+   FP32 copy.  Replaces the threaded fill_matrix_radial calls of solver.py:223-236. */
+int tb_fill_matrix(double *M, size_t ld, float *M32, size_t ld32, const void *rows, const double *src, int N, int row0, int row1,
+                   int accumulate, void *stream);
+/* Singular diagonal of rows [row0,row1] (solver.py:239-247): M[i][i] = self potential, or
+   self flux - 1 for dielectric/magnetizable rows.  M, M32, diag_out nullable. */
+int tb_fill_diagonal(double *M, size_t ld, float *M32, size_t ld32, const double *lines, const void *rows, int N, int row0,
+                     int row1, double *diag_out, void *stream);
+
+/* Dense LU with partial pivoting, row-major (numpy.linalg.solve, solver.py:173).
+   ipiv: N ints, 0-based pivot rows; info: device int zeroed by caller, set to 1 + first zero pivot. */
+size_t tb_lu_workspace_bytes(int N, int fp32);
+int tb_lu_factor(double *A, size_t lda, int N, int *ipiv, void *workspace, int *info, void *stream);
+int tb_lu_factor_f32(float *A, size_t lda, int N, int *ipiv, void *workspace, int *info, void *stream);
+/* X (nrhs x N) <- A^{-1} X with the factors (FP64 vectors; FP32 factors promoted on load) */
+int tb_lu_solve(const double *LU, size_t lda, int N, const int *ipiv, double *X, int nrhs, void *stream);
+int tb_lu_solve_f32(const float *LU, size_t lda, int N, const int *ipiv, double *X, int nrhs, void *stream);
+/* R = B - A X in FP64, rmax[r] = bit pattern of max |R[r]| (nonnegative double) */
+int tb_residual(const double *A, size_t lda, int N, const double *B, const double *X, double *R, int nrhs,
+                unsigned long long *rmax, void *stream);
+int tb_axpy(double *X, const double *D, size_t n, void *stream);
+int tb_convert_f64_f32(const double *S, size_t lds, float *D, size_t ldd, int rows, int cols, void *stream);
+/* C = beta C + alpha A B (row-major), the MFMA GEMM used by the factorisation */
+int tb_gemm_f64(int M, int N, int K, double alpha, const double *A, size_t lda, const double *B, size_t ldb, double beta, double *C,
+                size_t ldc, void *stream);
+int tb_gemm_f32(int M, int N, int K, float alpha, const float *A, size_t lda, const float *B, size_t ldb, float beta, float *C,
+                size_t ldc, void *stream);
+
+/* Ring sources for the field sums: w = q * jac, r, z flattened over (element, quad point);
+   nq = 16 (lines, pstride 2, zcol 1) or 12 (coil triangles, pstride 3, zcol 2). */
+int tb_sources(const double *q, const double *jac, const double *pos, int N, int nq, int pstride, int zcol, double *w, double *r,
+               double *z, void *stream);
+/* potential_radial / field_radial at M points (M x 3); pot or field nullable */
+int tb_field_points(const double *pts, int M, const double *w, const double *r, const double *z, int Ns, double *pot, double *field,
+                    int accumulate, void *stream);
+/* current_field_radial at M points */
+int tb_current_points(const double *pts, int M, const double *w, const double *r, const double *z, int Ns, double *field,
+                      int accumulate, void *stream);
+int tb_current_axial_potential(const double *zq, int M, const double *w, const double *r, const double *z, int Ns, double *out,
+                               void *stream);
+/* axial_derivatives_radial / current_axial_derivatives_radial (accumulate into out (Nz, 9)) */
+int tb_axial_derivatives(double *out, const double *zq, int Nz, const double *w, const double *r, const double *z, int Ns,
+                         int current, void *stream);
+/* FieldRadialAxial coefficients (field.py:906-937): C (Nz-1, 9, 6); scratch >= 6*Nz doubles */
+int tb_axial_coefficients(const double *derivs, const double *zq, int Nz, double *C, double *scratch, void *stream);
+int tb_axial_points(const double *pts, int M, const double *zq, int Nz, const double *C, double *pot, double *field, void *stream);
+int tb_jacobian_3d(const double *tri, int N, double *jac, double *pos, void *stream);
+int tb_charge_radial(const double *lines, const double *q, int N, double *out, void *stream);
+
+/* Field programs and batched tracing (tracing.c:60-174 for many particles at once). */
+enum { TB_COMP_BEM = 0, TB_COMP_AXIAL = 1, TB_COMP_UNIFORM = 2 };
+enum { TB_BOUNDS_NONE = 0, TB_BOUNDS_STRICT = 1, TB_BOUNDS_INCLUSIVE = 2 };
+enum { TB_STATUS_LEFT_BOUNDS = 0, TB_STATUS_BLOCK_FULL = 1, TB_STATUS_STEP_CAP = 2 };
+
+typedef struct { const double *w; const double *r; const double *z; int n, pad; } tb_sources_t;
+typedef struct {
+  int kind, transform, bounds_mode, pad;
+  double factor;
+  double inv[12];
+  double basis[9];
+  double bounds[6];
+  tb_sources_t elec, mag, cur;
+  const double *zs; const double *ce; const double *cm;
+  int nz, pad2;
+  double uE[3], uH[3];
+} tb_field_comp_t;
+typedef struct { const tb_field_comp_t *comp; int ncomp, bounds_mode; double bounds[6]; } tb_field_program_t;
+typedef struct {
+  tb_field_program_t field;
+  const double *state0; const double *t0; const double *h0; const double *qm;
+  int qm_stride, n;
+  int *counter;
+  double bounds[6];
+  double atol;
+  long long max_attempts;
+  double plane_p[3], plane_n[3], plane_nn;
+  double *final_state; double *landing; int *status;
+  long long *n_accepted; long long *n_attempted;
+  double *t_final; double *h_final;
+  double *traj; double *times; long long traj_cap; long long *traj_len;
+} tb_trace_params_t;
+
+/* Persistent RK45 tracer over params->n particles (all pointers device pointers).
+   waves: number of 64-lane workgroups (0: 8 per CU). */
+int tb_trace(const tb_trace_params_t *params, int waves, void *stream);
+int tb_eval_program(const tb_field_program_t *program, const double *pts, int M, double *E, double *H, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

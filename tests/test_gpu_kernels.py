@@ -1,0 +1,253 @@
+"""GPU parity of the HIP kernels against the oracle (C restatement of the reference).
+
+Tolerances: the path is FP64 end to end; entries, potentials and fields agree with the
+oracle to rounding (1e-12 relative on the matrix, 1e-10 on sums as the north-star bar),
+the singular diagonal to 1e-13 against a tight QUADPACK run (the reference itself asks
+QUADPACK for only 1e-9, backend/__init__.py:692)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from traceon_amd import device
+    return device
+
+
+@pytest.fixture(scope="module")
+def geo():
+    from traceon_amd import configs
+    mesh, volts, diel = configs.small_geometry(240)
+    return configs.lines_types_values(mesh, volts, diel)
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def test_gemm_mfma_layout(dev):
+    import torch
+    rng = np.random.default_rng(0)
+    for dt, tol in ((torch.float64, 1e-13), (torch.float32, 1e-5)):
+        for (M, N, K) in ((37, 53, 19), (128, 128, 128), (200, 300, 128), (129, 257, 64)):
+            A = rng.standard_normal((M, K))
+            Bm = rng.standard_normal((K, N))
+            C0 = rng.standard_normal((M, N))
+            a = torch.tensor(A, dtype=dt, device="cuda")
+            b = torch.tensor(Bm, dtype=dt, device="cuda")
+            c = torch.tensor(C0, dtype=dt, device="cuda")
+            out = dev.gemm(a, b, c, alpha=-1.0, beta=1.0).double().cpu().numpy()
+            ref = C0 - A @ Bm
+            print("gemm", dt, M, N, K, rel(out, ref), flush=True)
+            assert rel(out, ref) < tol
+
+
+def test_element_buffers(dev, geo, oracle):
+    lines, types, values = geo
+    E = dev.Elements(lines, types, values)
+    jac_o, pos_o = oracle.fill_jacobian_buffer_radial(lines)
+    assert rel(E.jac.cpu().numpy(), jac_o) < 1e-13
+    assert rel(E.pos.cpu().numpy(), pos_o) < 1e-14
+
+
+def test_fill_offdiagonal(dev, geo, oracle):
+    lines, types, values = geo
+    E = dev.Elements(lines, types, values)
+    M, _ = dev.assemble(E, diagonal=False)
+    Mo = oracle.fill_matrix(lines, types, values)
+    M = M.cpu().numpy()
+    off = ~np.eye(len(lines), dtype=bool)
+    err = np.abs(M - Mo)[off] / np.maximum(np.abs(Mo)[off], 1e-300)
+    print("fill max rel", err.max(), flush=True)
+    # entries agree to rounding; the cancellation-prone far-field flux entries are compared absolutely per row
+    scale = np.max(np.abs(Mo), axis=1, keepdims=True)
+    assert np.max(np.abs(M - Mo)[off] / np.broadcast_to(scale, M.shape)[off]) < 1e-12
+    assert np.median(err) < 1e-14
+
+
+def test_fill_fp32_copy(dev, geo):
+    lines, types, values = geo
+    E = dev.Elements(lines, types, values)
+    M, M32 = dev.assemble(E, fp64=True, fp32=True)
+    assert np.array_equal(M.float().cpu().numpy(), M32.cpu().numpy())
+
+
+def test_diagonal_vs_quadpack(dev, geo, oracle):
+    lines, types, values = geo
+    E = dev.Elements(lines, types, values)
+    d = dev.self_integrals(E).cpu().numpy()
+    idx = np.linspace(0, len(lines) - 1, 25).astype(int)
+    for i in idx:
+        if types[i] in (3, 6):
+            tight = oracle.self_field_dot_normal_quad(lines[i], values[i], epsabs=1e-15, epsrel=1e-14, limit=500)
+            ref = oracle.self_field_dot_normal_quad(lines[i], values[i])
+        else:
+            tight = oracle.self_potential_quad(lines[i], epsabs=1e-15, epsrel=1e-14, limit=500)
+            ref = oracle.self_potential_quad(lines[i])
+        print(i, types[i], d[i], tight, ref, flush=True)
+        # potential self terms agree with a tight QUADPACK run to rounding; flux self terms
+        # scatter ~1e-12 between every quadrature (QUADPACK included), far inside the 1e-9
+        # the reference requests
+        tol = 1e-13 if types[i] in (1, 2, 5) else 1e-11
+        assert abs(d[i] - tight) <= tol * abs(tight) + 1e-16
+        assert abs(d[i] - ref) <= 1e-9 * abs(ref) + 1e-15
+
+
+def refined_solution(M, F):
+    """Solution of M x = F to well below cond*eps: LAPACK solve plus refinement with
+    residuals in x87 extended precision."""
+    x = np.linalg.solve(M, F)
+    Ml = M.astype(np.longdouble)
+    for _ in range(4):
+        r = (F.astype(np.longdouble) - Ml @ x.astype(np.longdouble)).astype(np.float64)
+        x = x + np.linalg.solve(M, r)
+    return x
+
+
+def test_solve_vs_numpy(dev, geo, oracle):
+    lines, types, values = geo
+    E = dev.Elements(lines, types, values)
+    M, M32 = dev.assemble(E, fp32=True)
+    F = np.where(np.isin(types, [1, 2, 5]), values, 0.0)
+    Mh = M.cpu().numpy()
+    xstar = refined_solution(Mh, F)
+    err_np = rel(np.linalg.solve(Mh, F), xstar)
+    X0, st0, _ = dev.solve_system(M, F, mode="fp64", refine=False)
+    X, st, _ = dev.solve_system(M, F, mode="fp64")
+    Xm, stm, _ = dev.solve_system(M, F, M32=M32, mode="mixed")
+    e0, e1, em = rel(X0.cpu().numpy()[0], xstar), rel(X.cpu().numpy()[0], xstar), rel(Xm.cpu().numpy()[0], xstar)
+    print("cond", np.linalg.cond(Mh), "lapack", err_np, "lu64", e0, "lu64+dd", e1, st, "mixed", em, stm, flush=True)
+    assert e0 <= max(1e-12, 10 * err_np)      # unrefined LU: LAPACK-class forward error
+    assert e1 <= 1e-14 and em <= 1e-14        # refined: the correctly rounded charges
+
+
+@pytest.mark.parametrize("N", [300, 1000])
+def test_lu_random(dev, N):
+    import torch
+    rng = np.random.default_rng(N)
+    A = rng.standard_normal((N, N))
+    b = rng.standard_normal((2, N))
+    lu = dev.lu_factor(torch.tensor(A, device="cuda"), overwrite=True)
+    X = torch.tensor(b, device="cuda")
+    dev.lu_solve(lu, X)
+    ref = np.linalg.solve(A, b.T).T
+    print("lu random", N, rel(X.cpu().numpy(), ref), flush=True)
+    assert rel(X.cpu().numpy(), ref) < 1e-10
+    # pivots equal LAPACK's on a matrix that needs them
+    import scipy.linalg
+    _, piv = scipy.linalg.lu_factor(A)
+    assert np.array_equal(lu.ipiv.cpu().numpy(), piv)
+
+
+def test_field_points(dev, geo, oracle):
+    import torch
+    lines, types, values = geo
+    jac, pos = oracle.fill_jacobian_buffer_radial(lines)
+    rng = np.random.default_rng(3)
+    q = rng.standard_normal(len(lines))
+    pts = np.stack([rng.uniform(-0.9e-3, 0.9e-3, 200), rng.uniform(-0.5e-3, 0.5e-3, 200), rng.uniform(-3e-3, 3e-3, 200)], 1)
+    pts[:5, :2] = 0.0   # on the axis
+    S = dev.Sources(q, jac, pos)
+    P, Fd = S.field_points(torch.tensor(pts, device="cuda"))
+    P, Fd = P.cpu().numpy(), Fd.cpu().numpy()
+    Po = np.array([oracle.potential_radial(p, q, jac, pos) for p in pts])
+    Fo = np.array([oracle.field_radial(p, q, jac, pos) for p in pts])
+    print("pot", rel(P, Po), "field", rel(Fd, Fo), flush=True)
+    assert rel(P, Po) < 1e-12
+    assert rel(Fd, Fo) < 1e-11
+
+
+def test_axial_derivatives_and_coefficients(dev, geo, oracle):
+    import torch
+    from oracle.oracle import quintic_spline_coefficients
+    lines, types, values = geo
+    jac, pos = oracle.fill_jacobian_buffer_radial(lines)
+    rng = np.random.default_rng(4)
+    q = rng.standard_normal(len(lines))
+    z = np.linspace(-2.5e-3, 2.5e-3, 120)
+    S = dev.Sources(q, jac, pos)
+    zt = torch.tensor(z, device="cuda")
+    d = S.axial_derivatives(zt)
+    do = oracle.axial_derivatives_radial(z, q, jac, pos)
+    for l in range(9):
+        assert rel(d[:, l].cpu().numpy(), do[:, l]) < 1e-11, l
+    C = dev.axial_coefficients(torch.tensor(do, device="cuda"), zt).cpu().numpy()
+    Co = quintic_spline_coefficients(z, do.T)
+    # compare slot k (power 5-k) through its contribution over an interval: |dc| h^(5-k)
+    h = z[1] - z[0]
+    powers = h ** (5 - np.arange(6))
+    for l in range(9):
+        scale = np.max(np.abs(do[:, l]))
+        err = np.max(np.abs(C[:, l, :] - Co[:, l, :]) * powers) / scale
+        print("coeff order", l, err, flush=True)
+        assert err < 1e-11, l
+
+
+def test_trace_bem_vs_oracle(dev, geo, oracle):
+    from traceon_amd import backend as B
+    lines, types, values = geo
+    jac, pos = oracle.fill_jacobian_buffer_radial(lines)
+    M = oracle.matrix(lines, types, values)
+    F = np.where(np.isin(types, [1, 2, 5]), values, 0.0)
+    q = np.linalg.solve(M, F)
+
+    class Eff:
+        def __init__(self, c, j, p):
+            self.charges, self.jacobians, self.positions = c, j, p
+
+        def __len__(self):
+            return len(self.charges)
+
+    elec = Eff(q, jac, pos)
+    empty2 = Eff(np.zeros(0), np.zeros((0, 16)), np.zeros((0, 16, 2)))
+    empty3 = Eff(np.zeros(0), np.zeros((0, 12)), np.zeros((0, 12, 3)))
+    args = B.FieldEvaluationArgsRadial(elec, empty2, empty3, None)
+    import ctypes
+    bounds = np.array([[-0.9e-3, 0.9e-3], [-0.9e-3, 0.9e-3], [-2.5e-3, 2.5e-3]])
+    v = np.sqrt(2 * 50 * 1.602176634e-19 / 9.1093837015e-31)
+    qm = -1.602176634e-19 / 9.1093837015e-31
+    p0 = np.array([0.2e-3, 0.0, -2.4e-3])
+    v0 = np.array([0.0, 0.0, v])
+    t, s = B.trace_particle(p0, v0, qm, B.traceable("field_radial_traceable"), bounds, 1e-8,
+                            ctypes.cast(ctypes.pointer(args), ctypes.c_void_p))
+    to, so = oracle.trace_bem(p0, v0, qm, (q, jac, pos), bounds=bounds, atol=1e-8)
+    print("steps", len(s), len(so), flush=True)
+    assert len(s) == len(so)
+    assert np.max(np.abs(s[:, :3] - so[:, :3])) < 1e-9
+    land = B.plane_intersection(s, np.array([0, 0, 2.0e-3]), np.array([0, 0, 1.0]))
+    land_o = oracle.plane_intersection(so, [0, 0, 2.0e-3], [0, 0, 1.0])
+    assert np.max(np.abs(land[:3] - land_o[:3])) < 1e-9
+
+
+def test_singular_matrix_raises(dev):
+    """A ring of radius zero carries no charge: an element on the axis gives an all-zero
+    column.  The factorisation must report the singular matrix (numpy.linalg.solve raises
+    LinAlgError at solver.py:173) and keep every pivot index inside the matrix."""
+    import torch
+    rng = np.random.default_rng(7)
+    N = 700
+    A = rng.standard_normal((N, N))
+    A[:, 123] = 0.0
+    A[5, :] = np.nan
+    t = torch.tensor(A, device="cuda")
+    with pytest.raises(np.linalg.LinAlgError):
+        dev.lu_factor(t, overwrite=True)
+    torch.cuda.synchronize()
+
+
+def test_mirror_geometry_is_regular(dev, oracle):
+    from traceon_amd import configs
+    mesh, volts = configs.mirror(400)
+    lines, types, values = configs.lines_types_values(mesh, volts)
+    assert len(lines) == 400
+    E = dev.Elements(lines, types, values)
+    M, _ = dev.assemble(E)
+    assert not np.any(np.all(M.cpu().numpy() == 0.0, axis=0))
+    X, st, _ = dev.solve_system(M, np.where(types == 1, values, 0.0))
+    assert np.all(np.isfinite(X.cpu().numpy()))

@@ -1,0 +1,627 @@
+// BEM kernels for gfx950: element buffers, dense FP64 matrix fill, log-singular
+// diagonal, field sums at point batches, axial derivatives and their interpolation
+// coefficients.  Each kernel names the reference function it replaces.
+//
+// Layout in HBM (all FP64 unless stated):
+//   lines    (N,4,3)       reference layout, read once by k_elements
+//   jac,pos  (N,16),(N,16,2) reference layout (EffectivePointCharges), kept for the API
+//   src      3 x (16,N)    SoA (r, z, w) per Gauss point, k-major: lane j of a wave reads
+//                          consecutive j -> fully coalesced in the fill
+//   rows     (N) RowInfo   per target element: (r0, z0, nr, nz, flux factor, kind)
+//   matrix   (N,ld)        row-major, one 64-wide wave = 64 consecutive columns
+#include "radial_math.h"
+#include "bem_kernels.h"
+
+namespace tb {
+
+// ------------------------------------------------------------------------------
+// Element preparation: fill_jacobian_buffer_radial (radial.c:224-246) plus the
+// per-row data of fill_matrix_radial (radial.c:265-292) and the SoA source copy.
+// ------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_elements(const double* __restrict__ lines, const uint8_t* __restrict__ types,
+                                                  const double* __restrict__ values, int N, double* __restrict__ jac,
+                                                  double* __restrict__ pos, double* __restrict__ src, RowInfo* __restrict__ rows,
+                                                  int* __restrict__ bad_type) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N * NQ) return;
+  int i = t / NQ, k = t % NQ;
+  Line4 L = load_line4(lines + 12 * (size_t)i);
+  double r, z, j;
+  line4_point(L, GL_X[k], r, z, j);
+  double w = GL_W[k] * j;
+  if (jac) jac[(size_t)i * NQ + k] = w;
+  if (pos) { pos[(size_t)i * 2 * NQ + 2 * k] = r; pos[(size_t)i * 2 * NQ + 2 * k + 1] = z; }
+  if (src) {
+    src[(size_t)k * N + i] = r;
+    src[(size_t)(NQ + k) * N + i] = z;
+    src[(size_t)(2 * NQ + k) * N + i] = w;
+  }
+  if (rows && k == 0) {
+    RowInfo R;
+    double j0;
+    line4_point(L, 0.0, R.r0, R.z0, j0);
+    line4_normal(L, 0.0, R.nr, R.nz);
+    int ty = types ? types[i] : VOLTAGE_FIXED;
+    if (ty == VOLTAGE_FIXED || ty == VOLTAGE_FUN || ty == MAGNETOSTATIC_POT) { R.kind = 0; R.fac = 0.0; }
+    else if (ty == DIELECTRIC || ty == MAGNETIZABLE) { R.kind = 1; R.fac = flux_factor(values[i]); }
+    else { R.kind = 2; R.fac = 0.0; if (bad_type) atomicOr(bad_type, 1); }
+    rows[i] = R;
+  }
+}
+
+// SoA sources from caller-provided (N,16) jacobian and (N,16,2) position buffers
+__global__ __launch_bounds__(256) void k_src_from_buffers(const double* __restrict__ jac, const double* __restrict__ pos, int N,
+                                                          double* __restrict__ src) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N * NQ) return;
+  int i = t / NQ, k = t % NQ;
+  src[(size_t)k * N + i] = pos[2 * (size_t)t];
+  src[(size_t)(NQ + k) * N + i] = pos[2 * (size_t)t + 1];
+  src[(size_t)(2 * NQ + k) * N + i] = jac[t];
+}
+
+// ------------------------------------------------------------------------------
+// Dense fill, fill_matrix_radial (radial.c:386-447).
+// Block = 256 consecutive columns x ROWS rows; the rows' data are wave-uniform
+// (scalar loads), the source Gauss points are per-lane coalesced loads.  Each
+// entry is the 16-point sum over the source element, written once (or added,
+// reproducing the reference's accumulate semantics).  Optionally also writes an
+// FP32 copy of the same entries for the mixed-precision factorisation.
+// ------------------------------------------------------------------------------
+template <int ROWS>
+__global__ __launch_bounds__(256) void k_fill(double* __restrict__ M, size_t ld, float* __restrict__ M32, size_t ld32,
+                                              const RowInfo* __restrict__ rows, const double* __restrict__ src, int N,
+                                              int row0, int row1, int accumulate) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  const int ib = row0 + blockIdx.y * ROWS;
+  const int jj = j < N ? j : N - 1;
+  const double* sr = src;
+  const double* sz = src + (size_t)NQ * N;
+  const double* sw = src + (size_t)2 * NQ * N;
+  double acc[ROWS];
+#pragma unroll
+  for (int t = 0; t < ROWS; t++) acc[t] = 0.0;
+
+#pragma unroll 2
+  for (int k = 0; k < NQ; k++) {
+    const double r = sr[(size_t)k * N + jj], z = sz[(size_t)k * N + jj], w = sw[(size_t)k * N + jj];
+#pragma unroll
+    for (int t = 0; t < ROWS; t++) {
+      const int i = ib + t;
+      if (i <= row1) {
+        const RowInfo R = rows[i];
+        const double dr = r - R.r0, dz = z - R.z0;
+        if (R.kind == 0) {
+          acc[t] = fma(w, ring_potential(R.r0, dr, dz), acc[t]);
+        } else if (R.kind == 1) {
+          double gr, gz;
+          ring_grad(R.r0, dr, dz, gr, gz);
+          // field_dot_normal_radial: factor * (n . E), E = -grad
+          acc[t] = fma(w, -R.fac * (R.nr * gr + R.nz * gz), acc[t]);
+        } else {
+          acc[t] = __builtin_nan("");
+        }
+      }
+    }
+  }
+  if (j >= N) return;
+#pragma unroll
+  for (int t = 0; t < ROWS; t++) {
+    const int i = ib + t;
+    if (i <= row1) {
+      double v = acc[t];
+      double* dst = M + (size_t)i * ld + j;
+      if (accumulate) v += *dst;
+      *dst = v;
+      if (M32) M32[(size_t)i * ld32 + j] = (float)v;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------
+// Singular diagonal: self_potential_radial / self_field_dot_normal_radial
+// (radial.c:319-359) integrated over alpha in [-1,1] as solver.py:239-247 does with
+// QUADPACK.  Here: split at the singular point alpha = 0, geometric grading towards it
+// (ratio SIGMA, NLEV levels, the innermost panel [0, SIGMA^(NLEV-1)] below 1e-13) and
+// the 16-point Gauss rule on every panel.  A log singularity at distance SIGMA/(1-SIGMA)
+// panel lengths leaves a per-panel relative error below ~1e-17, far inside QUADPACK's
+// 1e-9 request.  One 64-lane wave per element, shuffles for the reduction.
+// ------------------------------------------------------------------------------
+constexpr double SIGMA = 0.3;
+constexpr int NLEV = 26;
+constexpr int NSELF = 2 * NLEV * NQ;   // 832 nodes per element
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__global__ __launch_bounds__(256) void k_self(double* __restrict__ M, size_t ld, float* __restrict__ M32, size_t ld32,
+                                              const double* __restrict__ lines, const RowInfo* __restrict__ rows, int N,
+                                              int row0, int row1, double* __restrict__ diag_out) {
+  const int lane = threadIdx.x & 63;
+  const int i = row0 + blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i > row1) return;
+  const Line4 L = load_line4(lines + 12 * (size_t)i);
+  const RowInfo R = rows[i];
+  double acc = 0.0;
+  for (int n = lane; n < NSELF; n += 64) {
+    const int side = n / (NLEV * NQ);
+    const int m = (n % (NLEV * NQ)) / NQ;
+    const int q = n % NQ;
+    const double b = pow(SIGMA, (double)m);
+    const double a = (m == NLEV - 1) ? 0.0 : b * SIGMA;
+    const double half = 0.5 * (b - a);
+    double alpha = fma(half, GL_X[q], 0.5 * (a + b));
+    if (side) alpha = -alpha;
+    double dr, dz, j;
+    line4_delta(L, alpha, dr, dz, j);
+    double f;
+    if (R.kind == 0) {
+      f = j * ring_potential(R.r0, dr, dz);
+    } else {
+      double gr, gz;
+      ring_grad(R.r0, dr, dz, gr, gz);
+      f = j * -R.fac * (R.nr * gr + R.nz * gz);
+    }
+    acc = fma(half * GL_W[q], f, acc);
+  }
+  acc = wave_sum(acc);
+  if (lane == 0) {
+    double v = (R.kind == 0) ? acc : (R.kind == 1 ? acc - 1.0 : __builtin_nan(""));
+    if (M) M[(size_t)i * ld + i] = v;
+    if (M32) M32[(size_t)i * ld32 + i] = (float)v;
+    if (diag_out) diag_out[i - row0] = acc;   // the integral itself
+  }
+}
+
+// ------------------------------------------------------------------------------
+// Sources for the field sums: w = q * jac, flattened over (element, Gauss point).
+// (the products radial.c:253, 296-297 form inside their loops)
+// ------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_sources(const double* __restrict__ q, const double* __restrict__ jac,
+                                                 const double* __restrict__ pos, int N, int nq, int pstride, int zcol,
+                                                 double* __restrict__ sw, double* __restrict__ sr, double* __restrict__ sz) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N * nq) return;
+  int i = s / nq;
+  sw[s] = q[i] * jac[s];
+  sr[s] = pos[(size_t)s * pstride];
+  sz[s] = pos[(size_t)s * pstride + zcol];
+}
+
+// ------------------------------------------------------------------------------
+// Field sums at a batch of points: potential_radial / field_radial (radial.c:242-310).
+// One lane per point; the loop over sources is wave-uniform, so the source triple
+// comes in through scalar loads and every lane shares it.  FP64 VALU bound.
+// ------------------------------------------------------------------------------
+template <bool POT, bool FIELD>
+__global__ __launch_bounds__(256) void k_field_points(const double* __restrict__ pts, int M, const double* __restrict__ sw,
+                                                      const double* __restrict__ sr, const double* __restrict__ sz, int Ns,
+                                                      double* __restrict__ pot, double* __restrict__ field, int accumulate) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int tt = t < M ? t : M - 1;
+  const double x = pts[3 * (size_t)tt], y = pts[3 * (size_t)tt + 1], z0 = pts[3 * (size_t)tt + 2];
+  const double r0 = sqrt(x * x + y * y);
+  double P = 0.0, Er = 0.0, Ez = 0.0;
+#pragma unroll 2
+  for (int s = 0; s < Ns; s++) {
+    const double w = sw[s], dr = sr[s] - r0, dz = sz[s] - z0;
+    if (POT && FIELD) {
+      double ph, gr, gz;
+      ring_all(r0, dr, dz, ph, gr, gz);
+      P = fma(w, ph, P); Er = fma(-w, gr, Er); Ez = fma(-w, gz, Ez);
+    } else if (POT) {
+      P = fma(w, ring_potential(r0, dr, dz), P);
+    } else {
+      double gr, gz;
+      ring_grad(r0, dr, dz, gr, gz);
+      Er = fma(-w, gr, Er); Ez = fma(-w, gz, Ez);
+    }
+  }
+  if (t >= M) return;
+  if (POT) pot[t] = accumulate ? pot[t] + P : P;
+  if (FIELD) {
+    double fx = 0.0, fy = 0.0;
+    if (r0 >= AXIS_EPS) { fx = x / r0 * Er; fy = y / r0 * Er; }
+    double* f = field + 3 * (size_t)t;
+    if (accumulate) { f[0] += fx; f[1] += fy; f[2] += Ez; }
+    else { f[0] = fx; f[1] = fy; f[2] = Ez; }
+  }
+}
+
+// current_field_radial (radial.c:192-220) at a batch of points; sources (w, r, z)
+__global__ __launch_bounds__(256) void k_current_points(const double* __restrict__ pts, int M, const double* __restrict__ sw,
+                                                        const double* __restrict__ sr, const double* __restrict__ sz, int Ns,
+                                                        double* __restrict__ field, int accumulate) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int tt = t < M ? t : M - 1;
+  const double x = pts[3 * (size_t)tt], y = pts[3 * (size_t)tt + 1], z0 = pts[3 * (size_t)tt + 2];
+  const double r = sqrt(x * x + y * y);
+  double Hr = 0.0, Hz = 0.0;
+  for (int s = 0; s < Ns; s++) {
+    double hr, hz;
+    current_ring_field(r, z0, sr[s], sz[s], hr, hz);
+    Hr = fma(sw[s], hr, Hr); Hz = fma(sw[s], hz, Hz);
+  }
+  if (t >= M) return;
+  double fx = 0.0, fy = 0.0;
+  if (r >= AXIS_EPS) { fx = x / r * Hr; fy = y / r * Hr; }
+  double* f = field + 3 * (size_t)t;
+  if (accumulate) { f[0] += fx; f[1] += fy; f[2] += Hz; }
+  else { f[0] = fx; f[1] = fy; f[2] = Hz; }
+}
+
+// current_potential_axial (radial.c:175-190) at a batch of z values
+__global__ __launch_bounds__(256) void k_current_axial_potential(const double* __restrict__ zq, int M, const double* __restrict__ sw,
+                                                                 const double* __restrict__ sr, const double* __restrict__ sz,
+                                                                 int Ns, double* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= M) return;
+  const double z0 = zq[t];
+  double P = 0.0;
+  for (int s = 0; s < Ns; s++) P = fma(sw[s], current_ring_axial_potential(z0, sr[s], sz[s]), P);
+  out[t] = P;
+}
+
+// ------------------------------------------------------------------------------
+// Axial derivatives: axial_derivatives_radial (radial.c:155-173) and
+// current_axial_derivatives_radial (radial.c:222-240), accumulating into (Nz, 9).
+// One block per z sample, threads stride over the sources, block reduction.
+// ------------------------------------------------------------------------------
+template <bool CURRENT>
+__global__ __launch_bounds__(256) void k_axial_derivs(double* __restrict__ out, const double* __restrict__ zq, int Nz,
+                                                      const double* __restrict__ sw, const double* __restrict__ sr,
+                                                      const double* __restrict__ sz, int Ns) {
+  __shared__ double red[4][NDERIV];
+  const int iz = blockIdx.x;
+  const double z0 = zq[iz];
+  double acc[NDERIV];
+#pragma unroll
+  for (int l = 0; l < NDERIV; l++) acc[l] = 0.0;
+  for (int s = threadIdx.x; s < Ns; s += blockDim.x) {
+    double d[NDERIV];
+    const double r = sr[s];
+    if (CURRENT) {
+      current_ring_axial_derivs(z0, r, sz[s], d);
+#pragma unroll
+      for (int l = 0; l < NDERIV; l++) acc[l] = fma(sw[s], d[l], acc[l]);
+    } else {
+      ring_axial_derivs(z0, r, sz[s], d);
+      const double w = sw[s] * (0.5 * r);
+#pragma unroll
+      for (int l = 0; l < NDERIV; l++) acc[l] = fma(w, d[l], acc[l]);
+    }
+  }
+#pragma unroll
+  for (int l = 0; l < NDERIV; l++) acc[l] = wave_sum(acc[l]);
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int l = 0; l < NDERIV; l++) red[wv][l] = acc[l];
+  }
+  __syncthreads();
+  if (threadIdx.x < NDERIV) {
+    double v = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+    out[(size_t)iz * NDERIV + threadIdx.x] += v;
+  }
+}
+
+// ------------------------------------------------------------------------------
+// Interpolation coefficients (field.py:906-937), layout (Nz-1, 9, 6) highest power first.
+// Orders 0..6: quintic Hermite on each interval matching value, first and second
+// derivative (the samples of orders l, l+1, l+2) at both ends -- the polynomial
+// scipy's BPoly.from_derivatives builds, written in closed form.
+// ------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_quintic(const double* __restrict__ d, const double* __restrict__ zq, int Nz,
+                                                 double* __restrict__ C) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (Nz - 1) * 7) return;
+  const int i = t / 7, l = t % 7;
+  const double h = zq[i + 1] - zq[i];
+  const double y0 = d[(size_t)i * NDERIV + l], y1 = d[(size_t)(i + 1) * NDERIV + l];
+  const double p0 = d[(size_t)i * NDERIV + l + 1], p1 = d[(size_t)(i + 1) * NDERIV + l + 1];
+  const double s0 = d[(size_t)i * NDERIV + l + 2], s1 = d[(size_t)(i + 1) * NDERIV + l + 2];
+  const double D0 = y1 - y0 - p0 * h - 0.5 * s0 * h * h;   // value mismatch of the quadratic Taylor part
+  const double D1 = (p1 - p0 - s0 * h) * h;                 // slope mismatch, scaled by h
+  const double D2 = (s1 - s0) * h * h;                      // curvature mismatch, scaled by h^2
+  const double h3 = h * h * h;
+  double* c = C + ((size_t)i * NDERIV + l) * 6;
+  c[0] = (6.0 * D0 - 3.0 * D1 + 0.5 * D2) / (h3 * h * h);
+  c[1] = (-15.0 * D0 + 7.0 * D1 - D2) / (h3 * h);
+  c[2] = (10.0 * D0 - 4.0 * D1 + 0.5 * D2) / h3;
+  c[3] = 0.5 * s0;
+  c[4] = p0;
+  c[5] = y0;
+}
+
+// Orders 7, 8: not-a-knot cubic spline (scipy CubicSpline default) in slots 2..5.
+// Tridiagonal system for the knot slopes, solved by the Thomas algorithm; one
+// thread per order (Nz is a few thousand at most).  scratch: 3*Nz doubles per order.
+__global__ void k_cubic_notaknot(const double* __restrict__ d, const double* __restrict__ zq, int Nz,
+                                 double* __restrict__ C, double* __restrict__ scratch) {
+  const int l = 7 + threadIdx.x;
+  if (threadIdx.x >= 2) return;
+  double* cp = scratch + (size_t)threadIdx.x * 3 * Nz;   // modified super-diagonal
+  double* dp = cp + Nz;                                   // modified rhs
+  double* s = dp + Nz;                                    // slopes
+  auto y = [&](int i) { return d[(size_t)i * NDERIV + l]; };
+  auto dx = [&](int i) { return zq[i + 1] - zq[i]; };
+  auto sl = [&](int i) { return (y(i + 1) - y(i)) / dx(i); };
+  const int n = Nz;
+  // row 0: dx1 s0 + (dx0+dx1) s1 = ((dx0 + 2(dx0+dx1)) dx1 sl0 + dx0^2 sl1)/(dx0+dx1)
+  {
+    double a0 = dx(1), dd = dx(0) + dx(1);
+    double rhs = ((dx(0) + 2.0 * dd) * dx(1) * sl(0) + dx(0) * dx(0) * sl(1)) / dd;
+    cp[0] = dd / a0;
+    dp[0] = rhs / a0;
+  }
+  for (int i = 1; i < n - 1; i++) {
+    double sub = dx(i), diag = 2.0 * (dx(i - 1) + dx(i)), sup = dx(i - 1);
+    double rhs = 3.0 * (dx(i) * sl(i - 1) + dx(i - 1) * sl(i));
+    double den = diag - sub * cp[i - 1];
+    cp[i] = sup / den;
+    dp[i] = (rhs - sub * dp[i - 1]) / den;
+  }
+  {
+    // last row: (dx_{n-3}+dx_{n-2}) s_{n-2} + dx_{n-3} s_{n-1} = (dx_{n-2}^2 sl_{n-3} + (2 dd + dx_{n-2}) dx_{n-3} sl_{n-2})/dd
+    double dd = dx(n - 3) + dx(n - 2);
+    double sub = dd, diag = dx(n - 3);
+    double rhs = (dx(n - 2) * dx(n - 2) * sl(n - 3) + (2.0 * dd + dx(n - 2)) * dx(n - 3) * sl(n - 2)) / dd;
+    double den = diag - sub * cp[n - 2];
+    dp[n - 1] = (rhs - sub * dp[n - 2]) / den;
+  }
+  s[n - 1] = dp[n - 1];
+  for (int i = n - 2; i >= 0; i--) s[i] = dp[i] - cp[i] * s[i + 1];
+  for (int i = 0; i < n - 1; i++) {
+    double h = dx(i), m = sl(i);
+    double t = (s[i] + s[i + 1] - 2.0 * m) / h;
+    double* c = C + ((size_t)i * NDERIV + l) * 6;
+    c[0] = 0.0; c[1] = 0.0;
+    c[2] = t / h;
+    c[3] = (m - s[i]) / h - t;
+    c[4] = s[i];
+    c[5] = y(i);
+  }
+}
+
+// potential_radial_derivs / field_radial_derivs (radial.c:449-521) at a batch of points
+__global__ __launch_bounds__(256) void k_axial_points(const double* __restrict__ pts, int M, const double* __restrict__ zq, int Nz,
+                                                      const double* __restrict__ C, double* __restrict__ pot,
+                                                      double* __restrict__ field) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= M) return;
+  const double x = pts[3 * (size_t)t], y = pts[3 * (size_t)t + 1], z = pts[3 * (size_t)t + 2];
+  if (pot) pot[t] = axial_potential(zq, Nz, C, x, y, z);
+  if (field) axial_field(zq, Nz, C, x, y, z, field + 3 * (size_t)t);
+}
+
+// fill_jacobian_buffer_3d (three_d.c:34-62): 12-point rule on each triangle
+__global__ __launch_bounds__(256) void k_jac3d(const double* __restrict__ tri, int N, double* __restrict__ jac,
+                                               double* __restrict__ pos) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N * NQ3) return;
+  const int i = t / NQ3, k = t % NQ3;
+  const double* T = tri + 9 * (size_t)i;
+  double e1[3] = {T[3] - T[0], T[4] - T[1], T[5] - T[2]};
+  double e2[3] = {T[6] - T[0], T[7] - T[1], T[8] - T[2]};
+  double c0 = e1[1] * e2[2] - e1[2] * e2[1], c1 = e1[2] * e2[0] - e1[0] * e2[2], c2 = e1[0] * e2[1] - e1[1] * e2[0];
+  double area = 0.5 * sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+  jac[t] = 2.0 * TRI_W[k] * area;
+  for (int c = 0; c < 3; c++) pos[3 * (size_t)t + c] = T[c] + TRI_B1[k] * e1[c] + TRI_B2[k] * e2[c];
+}
+
+// charge_radial (radial.c:260-285): total charge of each element at unit-free density q
+__global__ __launch_bounds__(256) void k_charge(const double* __restrict__ lines, const double* __restrict__ q, int N,
+                                                double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  Line4 L = load_line4(lines + 12 * (size_t)i);
+  double s = 0.0;
+  for (int k = 0; k < NQ; k++) {
+    double r, z, j;
+    line4_point(L, GL_X[k], r, z, j);
+    s += 2.0 * PI * r * GL_W[k] * j * q[i];
+  }
+  out[i] = s;
+}
+
+// Elementwise helpers behind the reference's scalar entry points.
+__global__ void k_scalar(int op, const double* __restrict__ in, int n, double* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double* a = in + 4 * (size_t)t;
+  double* o = out + 4 * (size_t)t;
+  switch (op) {
+    case 0: o[0] = ellipkm1_(a[0]); break;
+    case 1: o[0] = ellipem1_(a[0]); break;
+    case 2: o[0] = ellipk_(a[0]); break;
+    case 3: o[0] = ellipe_(a[0]); break;
+    case 4: o[0] = ring_potential(a[0], a[2], a[3]); break;
+    case 5: { double gr, gz; ring_grad(a[0], a[2], a[3], gr, gz); o[0] = gr; break; }
+    case 6: { double gr, gz; ring_grad(a[0], a[2], a[3], gr, gz); o[0] = gz; break; }
+    case 7: o[0] = flux_factor(a[0]); break;
+    case 8: o[0] = current_ring_axial_potential(a[0], a[1], a[2]); break;
+    case 9: { double hr, hz; current_ring_field(a[0], a[1], a[2], a[3], hr, hz); o[0] = hr; o[1] = hz; break; }
+    case 10: {   // normal_2d (utilities_2d.c:18-30): tangent p1->p2 rotated to (ty, -tx)
+      double tx = a[2] - a[0], ty = a[3] - a[1];
+      double len = sqrt(tx * tx + ty * ty);
+      o[0] = ty / len; o[1] = -tx / len;
+      break;
+    }
+    default: o[0] = __builtin_nan("");
+  }
+}
+
+// Element geometry at one parameter value: position_and_jacobian_radial (op 0),
+// delta_position_and_jacobian_radial (op 1), higher_order_normal_radial (op 2).
+// in: per item (alpha, line[12]) ; out: per item (r, z, jac)
+__global__ void k_line_eval(int op, const double* __restrict__ in, int n, double* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double* a = in + 13 * (size_t)t;
+  Line4 L = load_line4(a + 1);
+  double* o = out + 3 * (size_t)t;
+  if (op == 0) line4_point(L, a[0], o[0], o[1], o[2]);
+  else if (op == 1) line4_delta(L, a[0], o[0], o[1], o[2]);
+  else { line4_normal(L, a[0], o[0], o[1]); o[2] = 0.0; }
+}
+
+// plane_intersection (tracing.c:184-223): the last index i whose signed distance
+// differs in sign from i+1 (zero counts as negative), found in parallel with an atomic max.
+__global__ void k_plane_scan(const double* __restrict__ p0, const double* __restrict__ nrm, const double* __restrict__ S, int Np,
+                             int* __restrict__ found) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i + 1 >= Np) return;
+  const double nn = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+  const double* a = S + 6 * (size_t)i;
+  const double* b = S + 6 * (size_t)(i + 1);
+  double ka = (nrm[0] * (p0[0] - a[0]) + nrm[1] * (p0[1] - a[1]) + nrm[2] * (p0[2] - a[2])) / nn;
+  double kb = (nrm[0] * (p0[0] - b[0]) + nrm[1] * (p0[1] - b[1]) + nrm[2] * (p0[2] - b[2])) / nn;
+  if ((ka > 0) != (kb > 0)) atomicMax(found, i);
+}
+
+__global__ void k_plane_interp(const double* __restrict__ p0, const double* __restrict__ nrm, const double* __restrict__ S,
+                               const int* __restrict__ found, double* __restrict__ out) {
+  const int i = *found;
+  if (threadIdx.x != 0) return;
+  if (i < 0) { for (int c = 0; c < 6; c++) out[c] = __builtin_nan(""); return; }
+  const double nn = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+  const double* a = S + 6 * (size_t)i;
+  const double* b = S + 6 * (size_t)(i + 1);
+  double ka = (nrm[0] * (p0[0] - a[0]) + nrm[1] * (p0[1] - a[1]) + nrm[2] * (p0[2] - a[2])) / nn;
+  double kb = (nrm[0] * (p0[0] - b[0]) + nrm[1] * (p0[1] - b[1]) + nrm[2] * (p0[2] - b[2])) / nn;
+  double diff = ka - kb;
+  double f = -kb / diff, fp = ka / diff;
+  for (int c = 0; c < 6; c++) out[c] = fp * b[c] + f * a[c];
+}
+
+// ------------------------------------------------------------------------------
+// Host launchers
+// ------------------------------------------------------------------------------
+static inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+hipError_t launch_elements(const double* lines, const uint8_t* types, const double* values, int N, double* jac, double* pos,
+                           double* src, RowInfo* rows, int* bad_type, hipStream_t st) {
+  if (N <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_elements, dim3(cdiv((size_t)N * NQ, 256)), dim3(256), 0, st, lines, types, values, N, jac, pos, src, rows,
+                     bad_type);
+  return hipGetLastError();
+}
+
+hipError_t launch_src_from_buffers(const double* jac, const double* pos, int N, double* src, hipStream_t st) {
+  if (N <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_src_from_buffers, dim3(cdiv((size_t)N * NQ, 256)), dim3(256), 0, st, jac, pos, N, src);
+  return hipGetLastError();
+}
+
+constexpr int FILL_ROWS = 8;
+hipError_t launch_fill(double* M, size_t ld, float* M32, size_t ld32, const RowInfo* rows, const double* src, int N, int row0,
+                       int row1, int accumulate, hipStream_t st) {
+  if (N <= 0 || row1 < row0) return hipSuccess;
+  int nrows = row1 - row0 + 1;
+  // grid.y is limited to 65535: tile the row range
+  const int max_y = 65535;
+  for (int r = row0; r <= row1; r += max_y * FILL_ROWS) {
+    int r1 = r + max_y * FILL_ROWS - 1;
+    if (r1 > row1) r1 = row1;
+    dim3 grid(cdiv(N, 256), cdiv(r1 - r + 1, FILL_ROWS));
+    hipLaunchKernelGGL(k_fill<FILL_ROWS>, grid, dim3(256), 0, st, M, ld, M32, ld32, rows, src, N, r, r1, accumulate);
+  }
+  (void)nrows;
+  return hipGetLastError();
+}
+
+hipError_t launch_self(double* M, size_t ld, float* M32, size_t ld32, const double* lines, const RowInfo* rows, int N, int row0,
+                       int row1, double* diag_out, hipStream_t st) {
+  if (N <= 0 || row1 < row0) return hipSuccess;
+  hipLaunchKernelGGL(k_self, dim3(cdiv(row1 - row0 + 1, 4)), dim3(256), 0, st, M, ld, M32, ld32, lines, rows, N, row0, row1,
+                     diag_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_sources(const double* q, const double* jac, const double* pos, int N, int nq, int pstride, int zcol, double* sw,
+                          double* sr, double* sz, hipStream_t st) {
+  if (N <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sources, dim3(cdiv((size_t)N * nq, 256)), dim3(256), 0, st, q, jac, pos, N, nq, pstride, zcol, sw, sr, sz);
+  return hipGetLastError();
+}
+
+hipError_t launch_field_points(const double* pts, int M, const double* sw, const double* sr, const double* sz, int Ns, double* pot,
+                               double* field, int accumulate, hipStream_t st) {
+  if (M <= 0) return hipSuccess;
+  dim3 g(cdiv(M, 256)), b(256);
+  if (pot && field) hipLaunchKernelGGL((k_field_points<true, true>), g, b, 0, st, pts, M, sw, sr, sz, Ns, pot, field, accumulate);
+  else if (pot) hipLaunchKernelGGL((k_field_points<true, false>), g, b, 0, st, pts, M, sw, sr, sz, Ns, pot, field, accumulate);
+  else if (field) hipLaunchKernelGGL((k_field_points<false, true>), g, b, 0, st, pts, M, sw, sr, sz, Ns, pot, field, accumulate);
+  return hipGetLastError();
+}
+
+hipError_t launch_current_points(const double* pts, int M, const double* sw, const double* sr, const double* sz, int Ns,
+                                 double* field, int accumulate, hipStream_t st) {
+  if (M <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_current_points, dim3(cdiv(M, 256)), dim3(256), 0, st, pts, M, sw, sr, sz, Ns, field, accumulate);
+  return hipGetLastError();
+}
+
+hipError_t launch_current_axial_potential(const double* zq, int M, const double* sw, const double* sr, const double* sz, int Ns,
+                                          double* out, hipStream_t st) {
+  if (M <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_current_axial_potential, dim3(cdiv(M, 256)), dim3(256), 0, st, zq, M, sw, sr, sz, Ns, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_axial_derivs(double* out, const double* zq, int Nz, const double* sw, const double* sr, const double* sz, int Ns,
+                               int current, hipStream_t st) {
+  if (Nz <= 0 || Ns <= 0) return hipSuccess;
+  if (current) hipLaunchKernelGGL(k_axial_derivs<true>, dim3(Nz), dim3(256), 0, st, out, zq, Nz, sw, sr, sz, Ns);
+  else hipLaunchKernelGGL(k_axial_derivs<false>, dim3(Nz), dim3(256), 0, st, out, zq, Nz, sw, sr, sz, Ns);
+  return hipGetLastError();
+}
+
+hipError_t launch_axial_coeffs(const double* d, const double* zq, int Nz, double* C, double* scratch, hipStream_t st) {
+  if (Nz < 4) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_quintic, dim3(cdiv((size_t)(Nz - 1) * 7, 256)), dim3(256), 0, st, d, zq, Nz, C);
+  hipLaunchKernelGGL(k_cubic_notaknot, dim3(1), dim3(64), 0, st, d, zq, Nz, C, scratch);
+  return hipGetLastError();
+}
+
+hipError_t launch_axial_points(const double* pts, int M, const double* zq, int Nz, const double* C, double* pot, double* field,
+                               hipStream_t st) {
+  if (M <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_axial_points, dim3(cdiv(M, 256)), dim3(256), 0, st, pts, M, zq, Nz, C, pot, field);
+  return hipGetLastError();
+}
+
+hipError_t launch_jac3d(const double* tri, int N, double* jac, double* pos, hipStream_t st) {
+  if (N <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_jac3d, dim3(cdiv((size_t)N * NQ3, 256)), dim3(256), 0, st, tri, N, jac, pos);
+  return hipGetLastError();
+}
+
+hipError_t launch_charge(const double* lines, const double* q, int N, double* out, hipStream_t st) {
+  if (N <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_charge, dim3(cdiv(N, 256)), dim3(256), 0, st, lines, q, N, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_scalar(int op, const double* in, int n, double* out, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scalar, dim3(cdiv(n, 256)), dim3(256), 0, st, op, in, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_line_eval(int op, const double* in, int n, double* out, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_line_eval, dim3(cdiv(n, 256)), dim3(256), 0, st, op, in, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_plane(const double* p0, const double* nrm, const double* S, int Np, int* found, double* out, hipStream_t st) {
+  if (Np >= 2) hipLaunchKernelGGL(k_plane_scan, dim3(cdiv(Np - 1, 256)), dim3(256), 0, st, p0, nrm, S, Np, found);
+  hipLaunchKernelGGL(k_plane_interp, dim3(1), dim3(64), 0, st, p0, nrm, S, found, out);
+  return hipGetLastError();
+}
+
+}  // namespace tb

@@ -1,0 +1,563 @@
+// Dense LU with partial pivoting and solves, for the BEM charge system
+// (replaces numpy.linalg.solve -> LAPACK dgesv at traceon/solver.py:173).
+//
+// Row-major storage (the fill's layout).  Blocked right-looking factorisation, panel
+// width NB.  Per panel at column k0:
+//   1. k_panel   : cooperative kernel, one workgroup per CU.  The m x NB panel is split
+//                  by rows over the workgroups and held in LDS for the whole panel;
+//                  each column costs ONE grid barrier: every workgroup publishes its
+//                  local pivot candidate together with that row's panel values, and the
+//                  owner of the next diagonal row publishes that row, so after the
+//                  barrier everyone can swap and update without a second exchange.
+//   2. k_laswp   : the panel's row interchanges applied to the other columns.
+//   3. k_trinv   : inverse of the unit-lower NB x NB diagonal block.
+//   4. gemm      : U12 = inv(L11) * A12  (into a workspace), A22 -= L21 * U12 (MFMA),
+//                  U12 copied back.
+// Pivot choice is LAPACK's (largest |a|, first index on ties).
+#include <hip/hip_runtime.h>
+#include <hip/hip_cooperative_groups.h>
+#include <stdint.h>
+#include "lu_kernels.h"
+
+namespace cg = cooperative_groups;
+
+namespace tb {
+
+template <typename T> __device__ __forceinline__ T tabs(T x) { return x < T(0) ? -x : x; }
+
+// ---------------------------------------------------------------------------------
+// Panel factorisation
+// ---------------------------------------------------------------------------------
+template <typename T, int NB>
+__global__ __launch_bounds__(256) void k_panel(T* __restrict__ A, size_t lda, int N, int k0, int* __restrict__ ipiv,
+                                               double* __restrict__ wsd, int* __restrict__ wsi, T* __restrict__ wsr,
+                                               int* __restrict__ info) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* P = reinterpret_cast<T*>(smem_raw);          // local panel rows, R x NB
+  __shared__ T prow[NB], drow[NB];
+  __shared__ double red_v[256];
+  __shared__ int red_i[256], red_s[256];
+  __shared__ int s_p, s_slot;
+
+  cg::grid_group grid = cg::this_grid();
+  const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
+  const int m = N - k0;
+  const int R = (m + G - 1) / G;
+  const int rbeg = k0 + g * R;
+  const int rend = min(rbeg + R, N);
+  const int nloc = rend > rbeg ? rend - rbeg : 0;
+  const int nbw = min(NB, N - k0);   // panel width (last panel may be narrower)
+
+  volatile double* cand_val = wsd;          // [2][G]
+  volatile int* cand_idx = wsi;             // [2][G]
+  volatile T* cand_row = wsr;               // [2][G][NB]
+  volatile T* diag_row = wsr + 2 * (size_t)G * NB;   // [2][NB]
+
+  for (int e = tid; e < nloc * NB; e += 256) {
+    int r = e / NB, c = e % NB;
+    P[e] = c < nbw ? A[(size_t)(rbeg + r) * lda + k0 + c] : T(0);
+  }
+  __syncthreads();
+
+  // publish the local candidate for column j among local rows with global index >= first
+  auto publish = [&](int j, int first, int buf) {
+    double best = -1.0;
+    int bi = 0x7fffffff;
+    for (int r = tid; r < nloc; r += 256) {
+      int gr = rbeg + r;
+      if (gr < first) continue;
+      double v = (double)tabs(P[r * NB + j]);
+      if (v > best || (v == best && gr < bi)) { best = v; bi = gr; }
+    }
+    red_v[tid] = best; red_i[tid] = bi;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (tid < s) {
+        double v2 = red_v[tid + s]; int i2 = red_i[tid + s];
+        if (v2 > red_v[tid] || (v2 == red_v[tid] && i2 < red_i[tid])) { red_v[tid] = v2; red_i[tid] = i2; }
+      }
+      __syncthreads();
+    }
+    int bi0 = red_i[0];
+    if (tid == 0) { cand_val[buf * G + g] = red_v[0]; cand_idx[buf * G + g] = bi0; }
+    if (bi0 != 0x7fffffff && red_v[0] >= 0.0) {
+      for (int c = tid; c < NB; c += 256) cand_row[((size_t)buf * G + g) * NB + c] = P[(bi0 - rbeg) * NB + c];
+    }
+    if (first >= rbeg && first < rend) {
+      for (int c = tid; c < NB; c += 256) diag_row[(size_t)buf * NB + c] = P[(first - rbeg) * NB + c];
+    }
+    __threadfence();
+    __syncthreads();
+  };
+
+  publish(0, k0, 0);
+  grid.sync();
+
+  for (int j = 0; j < nbw; j++) {
+    const int buf = j & 1;
+    const int d = k0 + j;
+    // global pivot: largest candidate, smallest row on ties
+    {
+      double v = -1.0; int i = 0x7fffffff, s = -1;
+      if (tid < G) { v = cand_val[buf * G + tid]; i = cand_idx[buf * G + tid]; s = tid; }
+      red_v[tid] = v; red_i[tid] = i; red_s[tid] = s;
+      __syncthreads();
+      for (int st = 128; st > 0; st >>= 1) {
+        if (tid < st) {
+          double v2 = red_v[tid + st]; int i2 = red_i[tid + st];
+          if (v2 > red_v[tid] || (v2 == red_v[tid] && i2 < red_i[tid])) {
+            red_v[tid] = v2; red_i[tid] = i2; red_s[tid] = red_s[tid + st];
+          }
+        }
+        __syncthreads();
+      }
+      if (tid == 0) { s_p = red_i[0]; s_slot = red_s[0]; }
+      __syncthreads();
+    }
+    // No candidate (every remaining entry of the column is NaN): keep the row in place
+    // (no swap) and report the column as singular; the index stays inside the matrix.
+    const bool none = (s_slot < 0) || (s_p < d) || (s_p >= N);
+    const int p = none ? d : s_p, slot = s_slot;
+    for (int c = tid; c < NB; c += 256) {
+      drow[c] = diag_row[(size_t)buf * NB + c];
+      prow[c] = none ? drow[c] : cand_row[((size_t)buf * G + slot) * NB + c];
+    }
+    __syncthreads();
+    if (g == 0 && tid == 0) {
+      ipiv[d] = p;
+      if ((none || !(tabs(prow[j]) > T(0))) && *info == 0) *info = d + 1;
+    }
+    __syncthreads();
+    if (p != d && p >= rbeg && p < rend)
+      for (int c = tid; c < NB; c += 256) P[(p - rbeg) * NB + c] = drow[c];
+    if (d >= rbeg && d < rend)
+      for (int c = tid; c < NB; c += 256) P[(d - rbeg) * NB + c] = prow[c];
+    __syncthreads();
+    // scale the sub-diagonal part of column j, then rank-1 update of the columns right of j
+    const T inv = T(1) / prow[j];
+    const int first = max(d + 1, rbeg);
+    const int nr = rend > first ? rend - first : 0;
+    for (int r = tid; r < nr; r += 256) P[(first - rbeg + r) * NB + j] *= inv;
+    __syncthreads();
+    const int ncol = nbw - j - 1;
+    if (ncol > 0) {
+      for (int e = tid; e < nr * ncol; e += 256) {
+        int r = first - rbeg + e / ncol, c = j + 1 + e % ncol;
+        P[r * NB + c] -= P[r * NB + j] * prow[c];
+      }
+      __syncthreads();
+      publish(j + 1, d + 1, buf ^ 1);
+      grid.sync();
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < nloc * NB; e += 256) {
+    int r = e / NB, c = e % NB;
+    if (c < nbw) A[(size_t)(rbeg + r) * lda + k0 + c] = P[e];
+  }
+}
+
+// Row interchanges of one panel applied to the columns outside it (LAPACK dlaswp):
+// one thread per column, swaps in panel order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_laswp(T* __restrict__ A, size_t lda, int N, int k0, int nb, const int* __restrict__ ipiv) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  int ncols = N - nb;   // columns outside [k0, k0+nb)
+  if (c >= ncols) return;
+  int col = c < k0 ? c : c + nb;
+  for (int j = 0; j < nb; j++) {
+    int d = k0 + j, p = ipiv[d];
+    if (p != d && p >= 0 && p < N) {
+      T a = A[(size_t)d * lda + col], b = A[(size_t)p * lda + col];
+      A[(size_t)d * lda + col] = b;
+      A[(size_t)p * lda + col] = a;
+    }
+  }
+}
+
+// Inverse of the unit-lower nb x nb block L11 at (k0,k0): Y = inv(L11), Y row-major NB x NB.
+// Y[r][c] = delta_rc - sum_{s<r} L[r][s] Y[s][c]; thread c owns column c and walks the rows
+// in order (no cross-thread dependence), L[r][s] is the same for every thread.
+template <typename T, int NB>
+__global__ __launch_bounds__(NB) void k_trinv(const T* __restrict__ A, size_t lda, int k0, int nb, T* __restrict__ Y) {
+  const int c = threadIdx.x;
+  for (int r = 0; r < NB; r++) {
+    T acc = (r == c) ? T(1) : T(0);
+    if (r < nb && c < nb) {
+      const T* Lr = A + (size_t)(k0 + r) * lda + k0;
+      for (int s = c; s < r; s++) acc -= Lr[s] * Y[(size_t)s * NB + c];
+    } else {
+      acc = T(0);
+    }
+    Y[(size_t)r * NB + c] = acc;
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// GEMM on MFMA: C(MxN) = beta*C + alpha*A(MxK)*B(KxN), all row-major.
+// Block 256 threads = 2x2 waves, block tile 128x128, wave tile 64x64 made of 4x4
+// MFMA 16x16 tiles; K staged through LDS in chunks of BK, A stored k-major in LDS so
+// the 16 rows an MFMA reads are consecutive.  Instruction per element type:
+//   double: v_mfma_f64_16x16x4f64, float: v_mfma_f32_16x16x4f32
+// Operand layout of both (16x16x4, one block): lane l holds A[l%16][l/16] and
+// B[l/16][l%16]; the four accumulator values v = 0..3 of lane l are D[row][l%16] with
+// row = l/16 + 4v for the f64 instruction and row = 4(l/16) + v for the f32 one.
+// ---------------------------------------------------------------------------------
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct MfmaAcc;
+template <> struct MfmaAcc<double> {
+  typedef d4 type;
+  static __device__ __forceinline__ d4 mma(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  static __device__ __forceinline__ int row(int lane, int v) { return (lane >> 4) + 4 * v; }
+};
+template <> struct MfmaAcc<float> {
+  typedef f4 type;
+  static __device__ __forceinline__ f4 mma(float a, float b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
+constexpr int GBM = 128, GBN = 128;
+
+template <typename T, int BK>
+__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, const T* __restrict__ A, size_t lda,
+                                              const T* __restrict__ B, size_t ldb, T beta, T* __restrict__ C, size_t ldc) {
+  typedef typename MfmaAcc<T>::type acc_t;
+  __shared__ T As[2][BK][GBM + 4];
+  __shared__ T Bs[2][BK][GBN + 4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+  const int bm = blockIdx.y * GBM, bn = blockIdx.x * GBN;
+
+  acc_t acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) acc[a][b] = acc_t{0, 0, 0, 0};
+
+  auto load_tile = [&](int buf, int k0) {
+    // A: GBM x BK elements, each thread several; store transposed (k-major)
+    for (int e = tid; e < GBM * BK; e += 256) {
+      int r = e / BK, k = e % BK;
+      int gr = bm + r, gk = k0 + k;
+      As[buf][k][r] = (gr < M && gk < K) ? A[(size_t)gr * lda + gk] : T(0);
+    }
+    for (int e = tid; e < BK * GBN; e += 256) {
+      int k = e / GBN, c = e % GBN;
+      int gk = k0 + k, gc = bn + c;
+      Bs[buf][k][c] = (gk < K && gc < N) ? B[(size_t)gk * ldb + gc] : T(0);
+    }
+  };
+
+  const int nk = (K + BK - 1) / BK;
+  load_tile(0, 0);
+  __syncthreads();
+  for (int kt = 0; kt < nk; kt++) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) load_tile(cur ^ 1, (kt + 1) * BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      T af[4], bf[4];
+      const int kr = kk + (lane >> 4), ci = lane & 15;
+#pragma unroll
+      for (int a = 0; a < 4; a++) af[a] = As[cur][kr][wm + a * 16 + ci];
+#pragma unroll
+      for (int b = 0; b < 4; b++) bf[b] = Bs[cur][kr][wn + b * 16 + ci];
+#pragma unroll
+      for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = MfmaAcc<T>::mma(af[a], bf[b], acc[a][b]);
+    }
+    __syncthreads();
+  }
+  // write back
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      const int col = bn + wn + b * 16 + (lane & 15);
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int row = bm + wm + a * 16 + MfmaAcc<T>::row(lane, v);
+        if (row < M && col < N) {
+          T* dst = C + (size_t)row * ldc + col;
+          T val = alpha * acc[a][b][v];
+          *dst = (beta == T(0)) ? val : fma(beta, *dst, val);
+        }
+      }
+    }
+}
+
+// copy a rows x cols block (row-major, separate leading dims)
+template <typename T>
+__global__ __launch_bounds__(256) void k_copy2d(const T* __restrict__ S, size_t lds, T* __restrict__ D, size_t ldd, int rows, int cols) {
+  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)rows * cols) return;
+  int r = (int)(e / cols), c = (int)(e % cols);
+  D[(size_t)r * ldd + c] = S[(size_t)r * lds + c];
+}
+
+__global__ __launch_bounds__(256) void k_d2f(const double* __restrict__ S, size_t lds, float* __restrict__ D, size_t ldd, int rows,
+                                             int cols) {
+  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)rows * cols) return;
+  int r = (int)(e / cols), c = (int)(e % cols);
+  D[(size_t)r * ldd + c] = (float)S[(size_t)r * lds + c];
+}
+
+// ---------------------------------------------------------------------------------
+// Solves with the factors: x = U^{-1} L^{-1} P b, nrhs right-hand sides stored
+// as (nrhs, N) rows (the reference's F layout).  Vectors in FP64; factor entries
+// are promoted on load (an FP32 factorisation is applied in FP64 arithmetic).
+// ---------------------------------------------------------------------------------
+__global__ void k_apply_pivots(double* __restrict__ X, int N, int nrhs, const int* __restrict__ ipiv) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrhs) return;
+  double* x = X + (size_t)r * N;
+  for (int i = 0; i < N; i++) {
+    int p = ipiv[i];
+    if (p != i && p >= 0 && p < N) { double t = x[i]; x[i] = x[p]; x[p] = t; }
+  }
+}
+
+constexpr int TSB = 64;   // rows per diagonal block of the triangular solves
+
+// diagonal block solve: one workgroup, block [b0, b0+nb)
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(64) void k_trsv_diag(const T* __restrict__ A, size_t lda, int N, int b0, int nb, double* __restrict__ X,
+                                                 int nrhs) {
+  __shared__ double xs[TSB];
+  for (int r = 0; r < nrhs; r++) {
+    double* x = X + (size_t)r * N;
+    if (threadIdx.x < nb) xs[threadIdx.x] = x[b0 + threadIdx.x];
+    __syncthreads();
+    if (LOWER) {
+      for (int i = 1; i < nb; i++) {
+        double part = 0.0;
+        for (int s = threadIdx.x; s < i; s += 64) part += (double)A[(size_t)(b0 + i) * lda + b0 + s] * xs[s];
+        for (int o = 32; o >= 1; o >>= 1) part += __shfl_xor(part, o, 64);
+        if (threadIdx.x == 0) xs[i] -= part;
+        __syncthreads();
+      }
+    } else {
+      for (int i = nb - 1; i >= 0; i--) {
+        double part = 0.0;
+        for (int s = i + 1 + threadIdx.x; s < nb; s += 64) part += (double)A[(size_t)(b0 + i) * lda + b0 + s] * xs[s];
+        for (int o = 32; o >= 1; o >>= 1) part += __shfl_xor(part, o, 64);
+        if (threadIdx.x == 0) xs[i] = (xs[i] - part) / (double)A[(size_t)(b0 + i) * lda + b0 + i];
+        __syncthreads();
+      }
+    }
+    if (threadIdx.x < nb) x[b0 + threadIdx.x] = xs[threadIdx.x];
+    __syncthreads();
+  }
+}
+
+// update after a solved block: x[i] -= A[i, b0:b0+nb] . x[b0:b0+nb] for rows i in [r0, r1); one wave per row
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsv_update(const T* __restrict__ A, size_t lda, int N, int b0, int nb, int r0, int r1,
+                                                    double* __restrict__ X, int nrhs) {
+  const int row = r0 + blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= r1) return;
+  for (int r = 0; r < nrhs; r++) {
+    double* x = X + (size_t)r * N;
+    double part = 0.0;
+    for (int s = lane; s < nb; s += 64) part += (double)A[(size_t)row * lda + b0 + s] * x[b0 + s];
+    for (int o = 32; o >= 1; o >>= 1) part += __shfl_xor(part, o, 64);
+    if (lane == 0) x[row] -= part;
+  }
+}
+
+// residual r = b - A x with the FP64 matrix, accumulated in double-double: each product
+// is split exactly with an FMA (a*x = p + e) and summed with TwoSum, so the residual of
+// an already accurate x is itself accurate and refinement converges to the correctly
+// rounded solution of the assembled system rather than stalling at cond * eps.
+// One wave per row; rmax[r] = bit pattern of max |r| (nonnegative doubles order as integers).
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+  s = a + b;
+  double bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+__device__ __forceinline__ void dd_add(double& hi, double& lo, double a, double alo) {
+  double s, e;
+  two_sum(hi, a, s, e);
+  e += lo + alo;
+  hi = s + e;
+  lo = e - (hi - s);
+}
+
+__global__ __launch_bounds__(256) void k_residual(const double* __restrict__ A, size_t lda, int N, const double* __restrict__ B,
+                                                  const double* __restrict__ X, double* __restrict__ Rv, int nrhs,
+                                                  unsigned long long* __restrict__ rmax) {
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= N) return;
+  const double* a = A + (size_t)row * lda;
+  for (int r = 0; r < nrhs; r++) {
+    const double* x = X + (size_t)r * N;
+    double hi = 0.0, lo = 0.0;
+    for (int s = lane; s < N; s += 64) {
+      const double p = a[s] * x[s];
+      const double e = fma(a[s], x[s], -p);
+      dd_add(hi, lo, p, e);
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+      const double h2 = __shfl_xor(hi, o, 64), l2 = __shfl_xor(lo, o, 64);
+      dd_add(hi, lo, h2, l2);
+    }
+    if (lane == 0) {
+      // b - (hi + lo)
+      double s, e;
+      two_sum(B[(size_t)r * N + row], -hi, s, e);
+      const double v = s + (e - lo);
+      Rv[(size_t)r * N + row] = v;
+      const double av = v < 0 ? -v : v;
+      atomicMax(rmax + r, (unsigned long long)__double_as_longlong(av));
+    }
+  }
+}
+
+__global__ void k_axpy(double* __restrict__ X, const double* __restrict__ D, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) X[i] += D[i];
+}
+
+// ---------------------------------------------------------------------------------
+// Host drivers
+// ---------------------------------------------------------------------------------
+static inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+template <typename T>
+hipError_t gemm(int M, int N, int K, T alpha, const T* A, size_t lda, const T* B, size_t ldb, T beta, T* C, size_t ldc,
+                hipStream_t st) {
+  if (M <= 0 || N <= 0) return hipSuccess;
+  dim3 grid(cdiv(N, GBN), cdiv(M, GBM));
+  constexpr int BK = sizeof(T) == 8 ? 16 : 32;
+  hipLaunchKernelGGL((k_gemm<T, BK>), grid, dim3(256), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+  return hipGetLastError();
+}
+template hipError_t gemm<double>(int, int, int, double, const double*, size_t, const double*, size_t, double, double*, size_t,
+                                 hipStream_t);
+template hipError_t gemm<float>(int, int, int, float, const float*, size_t, const float*, size_t, float, float*, size_t, hipStream_t);
+
+template <typename T>
+size_t lu_workspace_bytes(int N) {
+  // candidates (2G doubles + 2G ints + 2G*NB T + 2NB T), trinv (NB*NB T), U12 staging (NB*N T), info
+  const int G = 1024, NB = LU_NB;
+  size_t b = 2 * G * sizeof(double) + 2 * G * sizeof(int) + (2 * (size_t)G * NB + 2 * NB) * sizeof(T);
+  b = (b + 255) & ~(size_t)255;
+  b += (size_t)NB * NB * sizeof(T);
+  b = (b + 255) & ~(size_t)255;
+  b += (size_t)NB * (size_t)N * sizeof(T);
+  b = (b + 255) & ~(size_t)255;
+  b += 256;
+  return b;
+}
+template size_t lu_workspace_bytes<double>(int);
+template size_t lu_workspace_bytes<float>(int);
+
+template <typename T>
+hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev, hipStream_t st, int panel_groups) {
+  constexpr int NB = LU_NB;
+  if (N <= 0) return hipSuccess;
+  int dev = 0;
+  hipGetDevice(&dev);
+  hipDeviceProp_t prop;
+  hipError_t e = hipGetDeviceProperties(&prop, dev);
+  if (e != hipSuccess) return e;
+  int G = panel_groups > 0 ? panel_groups : prop.multiProcessorCount;
+  if (G > 1024) G = 1024;
+  const size_t max_lds = 160 * 1024 - (2 * NB * sizeof(T) + 256 * (sizeof(double) + 2 * sizeof(int)) + 64);
+  // rows per group must fit in LDS
+  size_t Rmax = ((size_t)N + G - 1) / G;
+  if (Rmax * NB * sizeof(T) > max_lds) return hipErrorInvalidValue;
+  auto kpanel = k_panel<T, NB>;
+  size_t dyn = Rmax * NB * sizeof(T);
+  e = hipFuncSetAttribute((const void*)kpanel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+  if (e != hipSuccess) return e;
+  int occ = 0;
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kpanel, 256, dyn);
+  if (e != hipSuccess) return e;
+  if (occ * prop.multiProcessorCount < G) return hipErrorCooperativeLaunchTooLarge;
+
+  unsigned char* w = (unsigned char*)ws;
+  double* wsd = (double*)w;                       w += 2 * 1024 * sizeof(double);
+  int* wsi = (int*)w;                             w += 2 * 1024 * sizeof(int);
+  T* wsr = (T*)w;                                 w += (2 * (size_t)1024 * NB + 2 * NB) * sizeof(T);
+  w = (unsigned char*)(((uintptr_t)w + 255) & ~(uintptr_t)255);
+  T* Y = (T*)w;                                   w += (size_t)NB * NB * sizeof(T);
+  w = (unsigned char*)(((uintptr_t)w + 255) & ~(uintptr_t)255);
+  T* U12 = (T*)w;
+
+  for (int k0 = 0; k0 < N; k0 += NB) {
+    const int nb = min(NB, N - k0);
+    {
+      T* Ap = A; size_t ldap = lda; int Np = N, k0p = k0;
+      int* ipivp = ipiv; double* wsdp = wsd; int* wsip = wsi; T* wsrp = wsr; int* infop = info_dev;
+      void* args[] = {&Ap, &ldap, &Np, &k0p, &ipivp, &wsdp, &wsip, &wsrp, &infop};
+      const int m = N - k0;
+      size_t R = ((size_t)m + G - 1) / G;
+      e = hipLaunchCooperativeKernel((const void*)kpanel, dim3(G), dim3(256), args, (unsigned)(R * NB * sizeof(T)), st);
+      if (e != hipSuccess) return e;
+    }
+    if (N - nb > 0) hipLaunchKernelGGL(k_laswp<T>, dim3(cdiv(N - nb, 256)), dim3(256), 0, st, A, lda, N, k0, nb, ipiv);
+    const int n2 = N - k0 - nb;
+    if (n2 > 0) {
+      hipLaunchKernelGGL((k_trinv<T, NB>), dim3(1), dim3(NB), 0, st, A, lda, k0, nb, Y);
+      T* A12 = A + (size_t)k0 * lda + k0 + nb;
+      // U12 = inv(L11) * A12
+      e = gemm<T>(nb, n2, nb, T(1), Y, NB, A12, lda, T(0), U12, (size_t)n2, st);
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL(k_copy2d<T>, dim3(cdiv((size_t)nb * n2, 256)), dim3(256), 0, st, U12, (size_t)n2, A12, lda, nb, n2);
+      // A22 -= L21 * U12
+      const T* L21 = A + (size_t)(k0 + nb) * lda + k0;
+      T* A22 = A + (size_t)(k0 + nb) * lda + k0 + nb;
+      e = gemm<T>(n2, n2, nb, T(-1), L21, lda, U12, (size_t)n2, T(1), A22, lda, st);
+      if (e != hipSuccess) return e;
+    }
+  }
+  return hipGetLastError();
+}
+template hipError_t lu_factor<double>(double*, size_t, int, int*, void*, int*, hipStream_t, int);
+template hipError_t lu_factor<float>(float*, size_t, int, int*, void*, int*, hipStream_t, int);
+
+template <typename T>
+hipError_t lu_solve(const T* LU, size_t lda, int N, const int* ipiv, double* X, int nrhs, hipStream_t st) {
+  if (N <= 0 || nrhs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_apply_pivots, dim3(cdiv(nrhs, 64)), dim3(64), 0, st, X, N, nrhs, ipiv);
+  for (int b0 = 0; b0 < N; b0 += TSB) {
+    int nb = min(TSB, N - b0);
+    hipLaunchKernelGGL((k_trsv_diag<T, true>), dim3(1), dim3(64), 0, st, LU, lda, N, b0, nb, X, nrhs);
+    if (b0 + nb < N)
+      hipLaunchKernelGGL(k_trsv_update<T>, dim3(cdiv(N - b0 - nb, 4)), dim3(256), 0, st, LU, lda, N, b0, nb, b0 + nb, N, X, nrhs);
+  }
+  for (int b0 = ((N - 1) / TSB) * TSB; b0 >= 0; b0 -= TSB) {
+    int nb = min(TSB, N - b0);
+    hipLaunchKernelGGL((k_trsv_diag<T, false>), dim3(1), dim3(64), 0, st, LU, lda, N, b0, nb, X, nrhs);
+    if (b0 > 0) hipLaunchKernelGGL(k_trsv_update<T>, dim3(cdiv(b0, 4)), dim3(256), 0, st, LU, lda, N, b0, nb, 0, b0, X, nrhs);
+  }
+  return hipGetLastError();
+}
+template hipError_t lu_solve<double>(const double*, size_t, int, const int*, double*, int, hipStream_t);
+template hipError_t lu_solve<float>(const float*, size_t, int, const int*, double*, int, hipStream_t);
+
+hipError_t residual(const double* A, size_t lda, int N, const double* B, const double* X, double* R, int nrhs,
+                    unsigned long long* rmax, hipStream_t st) {
+  hipMemsetAsync(rmax, 0, sizeof(unsigned long long) * nrhs, st);
+  hipLaunchKernelGGL(k_residual, dim3(cdiv(N, 4)), dim3(256), 0, st, A, lda, N, B, X, R, nrhs, rmax);
+  return hipGetLastError();
+}
+
+hipError_t axpy(double* X, const double* D, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_axpy, dim3(cdiv(n, 256)), dim3(256), 0, st, X, D, n);
+  return hipGetLastError();
+}
+
+hipError_t convert_d2f(const double* S, size_t lds, float* D, size_t ldd, int rows, int cols, hipStream_t st) {
+  hipLaunchKernelGGL(k_d2f, dim3(cdiv((size_t)rows * cols, 256)), dim3(256), 0, st, S, lds, D, ldd, rows, cols);
+  return hipGetLastError();
+}
+
+}  // namespace tb

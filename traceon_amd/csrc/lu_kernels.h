@@ -1,0 +1,27 @@
+// Host-visible drivers of the dense LU and solves (lu.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+namespace tb {
+
+constexpr int LU_NB = 128;   // panel width of the blocked factorisation
+
+template <typename T>
+hipError_t gemm(int M, int N, int K, T alpha, const T* A, size_t lda, const T* B, size_t ldb, T beta, T* C, size_t ldc,
+                hipStream_t st);
+template <typename T> size_t lu_workspace_bytes(int N);
+// In-place LU of the row-major N x N matrix A (leading dimension lda) with partial pivoting.
+// ipiv (device, N ints) receives 0-based pivot rows; info_dev (device int, zeroed by the caller)
+// receives 1 + the first zero pivot column, or stays 0.
+template <typename T>
+hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev, hipStream_t st, int panel_groups);
+// X (nrhs x N, row per right-hand side) <- A^{-1} X using the factors
+template <typename T>
+hipError_t lu_solve(const T* LU, size_t lda, int N, const int* ipiv, double* X, int nrhs, hipStream_t st);
+hipError_t residual(const double* A, size_t lda, int N, const double* B, const double* X, double* R, int nrhs,
+                    unsigned long long* rmax, hipStream_t st);
+hipError_t axpy(double* X, const double* D, size_t n, hipStream_t st);
+hipError_t convert_d2f(const double* S, size_t lds, float* D, size_t ldd, int rows, int cols, hipStream_t st);
+
+}  // namespace tb

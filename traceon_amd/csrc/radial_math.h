@@ -1,0 +1,301 @@
+// Device (and host) arithmetic of Traceon's radially symmetric BEM path for gfx950.
+//
+// Formulas follow the reference C backend (traceon/backend/elliptic.c, utilities_2d.c,
+// radial_ring.c, radial.c, tracing.c at v0.10.0); each function names the lines it
+// replaces.  Differences from the reference's layout are deliberate GPU choices:
+//   * K(m) and E(m) of one pair share a single log(1/p) (the reference calls log
+//     once per integral, three times per field pair);
+//   * 1/sqrt(S) is formed once per pair and reused for the division by S;
+//   * a pair's potential, dPhi/dr0 and dPhi/dz0 come out of one evaluation.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define TB_HD __host__ __device__ __forceinline__
+
+namespace tb {
+
+constexpr int NQ = 16;        // N_QUAD_2D, utilities_2d.c:2
+constexpr int NQ3 = 12;       // N_TRIANGLE_QUAD, utilities_3d.c:4
+constexpr int NDERIV = 9;     // DERIV_2D_MAX, radial_ring.c:2
+constexpr double AXIS_EPS = 1e-10;            // MIN_DISTANCE_AXIS, three_d.c:2
+constexpr double MU0 = 1.25663706212e-06;     // definitions.c:31
+constexpr double INV_PI = 0.318309886183790671537767526745;
+constexpr double PI = 3.14159265358979323846;
+
+// Excitation type codes, definitions.c:34-40
+enum : uint8_t { VOLTAGE_FIXED = 1, VOLTAGE_FUN = 2, DIELECTRIC = 3, CURRENT = 4,
+                 MAGNETOSTATIC_POT = 5, MAGNETIZABLE = 6 };
+
+// 16-point Gauss-Legendre rule with the reference's 16-digit literals (utilities_2d.c:3-5).
+__constant__ static const double GL_X[NQ] = {
+    -0.0950125098376374, 0.0950125098376374, -0.2816035507792589, 0.2816035507792589,
+    -0.4580167776572274, 0.4580167776572274, -0.6178762444026438, 0.6178762444026438,
+    -0.7554044083550030, 0.7554044083550030, -0.8656312023878318, 0.8656312023878318,
+    -0.9445750230732326, 0.9445750230732326, -0.9894009349916499, 0.9894009349916499};
+__constant__ static const double GL_W[NQ] = {
+    0.1894506104550685, 0.1894506104550685, 0.1826034150449236, 0.1826034150449236,
+    0.1691565193950025, 0.1691565193950025, 0.1495959888165767, 0.1495959888165767,
+    0.1246289712555339, 0.1246289712555339, 0.0951585116824928, 0.0951585116824928,
+    0.0622535239386479, 0.0622535239386479, 0.0271524594117541, 0.0271524594117541};
+
+// 12-point triangle rule (utilities_3d.c:6-47).
+__constant__ static const double TRI_W[NQ3] = {
+    0.0254224531851035, 0.0254224531851035, 0.0254224531851035, 0.0583931378631895, 0.0583931378631895,
+    0.0583931378631895, 0.041425537809187, 0.041425537809187, 0.041425537809187, 0.041425537809187,
+    0.041425537809187, 0.041425537809187};
+__constant__ static const double TRI_B1[NQ3] = {
+    0.873821971016996, 0.063089014491502, 0.063089014491502, 0.501426509658179, 0.249286745170910,
+    0.249286745170910, 0.636502499121399, 0.636502499121399, 0.310352451033785, 0.310352451033785,
+    0.053145049844816, 0.053145049844816};
+__constant__ static const double TRI_B2[NQ3] = {
+    0.063089014491502, 0.873821971016996, 0.063089014491502, 0.249286745170910, 0.501426509658179,
+    0.249286745170910, 0.310352451033785, 0.053145049844816, 0.636502499121399, 0.053145049844816,
+    0.636502499121399, 0.310352451033785};
+
+// ---------------------------------------------------------------------------------
+// Complete elliptic integrals from p = 1 - m: Cody (1965) fits, elliptic.c:9-75.
+//   K = PKA(p) + ln(1/p) PKB(p),  E = PEA(p) + ln(1/p) PEB(p)
+// ---------------------------------------------------------------------------------
+TB_HD double horner_ka(double p) {
+  double s = 1.40704915496101e-3;
+  s = fma(s, p, 1.09423810688623e-2); s = fma(s, p, 1.68695685967517e-2);
+  s = fma(s, p, 1.25565693543211e-2); s = fma(s, p, 1.52618320622534e-2);
+  s = fma(s, p, 3.08909633861795e-2); s = fma(s, p, 9.65736020516771e-2);
+  return fma(s, p, 1.3862943611198906);   // ln 4
+}
+TB_HD double horner_kb(double p) {
+  double s = 3.42805719229748e-4;
+  s = fma(s, p, 5.81807961871996e-3); s = fma(s, p, 2.09857677336790e-2);
+  s = fma(s, p, 3.57218443007327e-2); s = fma(s, p, 4.87379510945218e-2);
+  s = fma(s, p, 7.03114105853296e-2); s = fma(s, p, 1.24999998585309e-1);
+  return fma(s, p, 0.5);
+}
+TB_HD double horner_ea(double p) {
+  double s = 1.55618744745296e-3;
+  s = fma(s, p, 1.21819481486695e-2); s = fma(s, p, 1.92284389022977e-2);
+  s = fma(s, p, 1.56847700239786e-2); s = fma(s, p, 2.21862206993846e-2);
+  s = fma(s, p, 5.68115681053803e-2); s = fma(s, p, 4.43147193467733e-1);
+  return fma(s, p, 1.0);
+}
+TB_HD double horner_eb(double p) {
+  double s = 3.78886487349367e-4;
+  s = fma(s, p, 6.45682247315060e-3); s = fma(s, p, 2.35091602564984e-2);
+  s = fma(s, p, 4.09074821593164e-2); s = fma(s, p, 5.84950297066166e-2);
+  s = fma(s, p, 9.37488062098189e-2); s = fma(s, p, 2.49999998448655e-1);
+  return s * p;
+}
+
+TB_HD double ellipkm1_(double p) { double L = -log(p); return fma(L, horner_kb(p), horner_ka(p)); }
+TB_HD double ellipem1_(double p) { double L = -log(p); return fma(L, horner_eb(p), horner_ea(p)); }
+TB_HD void ellip_km1_em1(double p, double& K, double& E) {
+  double L = -log(p);
+  K = fma(L, horner_kb(p), horner_ka(p));
+  E = fma(L, horner_eb(p), horner_ea(p));
+}
+// elliptic.c:37-41, 71-75 (parameter forms used by the current-ring kernel)
+TB_HD double ellipk_(double k) { return k > -1.0 ? ellipkm1_(1.0 - k) : ellipkm1_(1.0 / (1.0 - k)) / sqrt(k); }
+TB_HD double ellipe_(double k) {
+  return (k >= 0.0 && k <= 1.0) ? ellipem1_(1.0 - k) : ellipem1_(-1.0 / (k - 1.0)) * sqrt(1.0 - k);
+}
+
+// ---------------------------------------------------------------------------------
+// Line4 element map (utilities_2d.c:18-87).  Node order in parameter alpha is
+// vertex rows 0, 2, 3, 1 of a (4,3) element (radial.c:265-268); r = x (col 0), z = col 2.
+// ---------------------------------------------------------------------------------
+struct Line4 { double r[4], z[4]; };   // nodes in parameter order
+
+TB_HD Line4 load_line4(const double* line) {   // line: (4,3) row-major
+  Line4 L;
+  const int order[4] = {0, 2, 3, 1};
+#pragma unroll
+  for (int n = 0; n < 4; n++) { L.r[n] = line[3 * order[n]]; L.z[n] = line[3 * order[n] + 2]; }
+  return L;
+}
+
+TB_HD void lagrange4(double a, double B[4], double dB[4]) {
+  double a2 = a * a;
+  B[0] = -(a - 1.0) * (9.0 * a2 - 1.0) * 0.0625;
+  B[1] = 9.0 * (a2 - 1.0) * (3.0 * a - 1.0) * 0.0625;
+  B[2] = -9.0 * (a2 - 1.0) * (3.0 * a + 1.0) * 0.0625;
+  B[3] = (a + 1.0) * (9.0 * a2 - 1.0) * 0.0625;
+  dB[0] = (-27.0 * a2 + 18.0 * a + 1.0) * 0.0625;
+  dB[1] = 9.0 * (9.0 * a2 - 2.0 * a - 3.0) * 0.0625;
+  dB[2] = -9.0 * (9.0 * a2 + 2.0 * a - 3.0) * 0.0625;
+  dB[3] = (27.0 * a2 + 18.0 * a - 1.0) * 0.0625;
+}
+
+// position_and_jacobian_radial, utilities_2d.c:50-68
+TB_HD void line4_point(const Line4& L, double a, double& r, double& z, double& jac) {
+  double B[4], dB[4];
+  lagrange4(a, B, dB);
+  double rr = 0, zz = 0, dr = 0, dz = 0;
+#pragma unroll
+  for (int n = 0; n < 4; n++) {
+    rr = fma(B[n], L.r[n], rr); zz = fma(B[n], L.z[n], zz);
+    dr = fma(dB[n], L.r[n], dr); dz = fma(dB[n], L.z[n], dz);
+  }
+  r = rr; z = zz; jac = sqrt(dr * dr + dz * dz);
+}
+
+// delta_position_and_jacobian_radial, utilities_2d.c:70-87: offset from alpha = 0,
+// basis without constant terms so that small offsets keep their relative precision.
+TB_HD void line4_delta(const Line4& L, double a, double& dr_out, double& dz_out, double& jac) {
+  double B[4], dB[4];
+  lagrange4(a, B, dB);
+  double a2 = a * a;
+  double D[4] = {a * (-9.0 * a2 + 9.0 * a + 1.0) * 0.0625, 9.0 * a * (3.0 * a2 - a - 3.0) * 0.0625,
+                 -9.0 * a * (3.0 * a2 + a - 3.0) * 0.0625, a * (9.0 * a2 + 9.0 * a - 1.0) * 0.0625};
+  double rr = 0, zz = 0, dr = 0, dz = 0;
+#pragma unroll
+  for (int n = 0; n < 4; n++) {
+    rr = fma(D[n], L.r[n], rr); zz = fma(D[n], L.z[n], zz);
+    dr = fma(dB[n], L.r[n], dr); dz = fma(dB[n], L.z[n], dz);
+  }
+  dr_out = rr; dz_out = zz; jac = sqrt(dr * dr + dz * dz);
+}
+
+// higher_order_normal_radial, utilities_2d.c:32-48: tangent (dr,dz) -> (dz,-dr)/|.|
+TB_HD void line4_normal(const Line4& L, double a, double& nr, double& nz) {
+  double B[4], dB[4];
+  lagrange4(a, B, dB);
+  double dr = 0, dz = 0;
+#pragma unroll
+  for (int n = 0; n < 4; n++) { dr = fma(dB[n], L.r[n], dr); dz = fma(dB[n], L.z[n], dz); }
+  double len = sqrt(dr * dr + dz * dz);
+  nr = dz / len; nz = -dr / len;
+}
+
+// flux_density_to_charge_factor, radial_ring.c:4-18
+TB_HD double flux_factor(double K) { return 2.0 * (K - 1.0) / (1.0 + K); }
+
+// ---------------------------------------------------------------------------------
+// Ring kernels (radial_ring.c:21-53).  Target ring (r0, z0), source ring offset
+// (dr, dz); source radius r = r0 + dr.  S = dz^2 + (r + r0)^2, p = (dr^2 + dz^2)/S.
+// ---------------------------------------------------------------------------------
+
+// potential_radial_ring, radial_ring.c:37-43
+TB_HD double ring_potential(double r0, double dr, double dz) {
+  double r = r0 + dr;
+  double s = r + r0;
+  double dz2 = dz * dz;
+  double S = fma(s, s, dz2);
+  double p = fma(dr, dr, dz2) / S;
+  return ellipkm1_(p) * r / (PI * sqrt(S));
+}
+
+// dr1/dz1_potential_radial_ring, radial_ring.c:21-35, 45-53:
+//   d/dz0 = dz r E / (pi d2 sqrt S)
+//   d/dr0 = -r [d2 K + (dr(dr - 2r) - dz^2) E] / (2 pi r0 d2 sqrt S), 0 for r0 < 1e-10
+TB_HD void ring_grad(double r0, double dr, double dz, double& g_r, double& g_z) {
+  double r = r0 + dr;
+  double s = r + r0;
+  double dz2 = dz * dz;
+  double d2 = fma(dr, dr, dz2);
+  double S = fma(s, s, dz2);
+  double K, E;
+  ellip_km1_em1(d2 / S, K, E);
+  double c = r / (PI * d2 * sqrt(S));
+  g_z = dz * E * c;
+  g_r = (r0 < AXIS_EPS) ? 0.0 : -c * (d2 * K + (dr * (dr - 2.0 * r) - dz2) * E) / (2.0 * r0);
+}
+
+// All three of a pair at once: potential, d/dr0, d/dz0 (one log, one sqrt, one division).
+TB_HD void ring_all(double r0, double dr, double dz, double& phi, double& g_r, double& g_z) {
+  double r = r0 + dr;
+  double s = r + r0;
+  double dz2 = dz * dz;
+  double d2 = fma(dr, dr, dz2);
+  double S = fma(s, s, dz2);
+  double iS = 1.0 / S;
+  double p = d2 * iS;
+  double K, E;
+  ellip_km1_em1(p, K, E);
+  double rq = r * sqrt(iS) * INV_PI;   // r / (pi sqrt S)
+  phi = K * rq;
+  double c = rq / d2;
+  g_z = dz * E * c;
+  g_r = (r0 < AXIS_EPS) ? 0.0 : -c * (d2 * K + (dr * (dr - 2.0 * r) - dz2) * E) / (2.0 * r0);
+}
+
+// current_field_radial_ring, radial_ring.c:78-107: H of a unit current ring of radius a
+// at height y, evaluated at (r, z0).  Returns (Hr, Hz).
+TB_HD void current_ring_field(double r, double z0, double a, double y, double& hr, double& hz) {
+  double z = z0 - y;
+  if (a < AXIS_EPS) { hr = 0.0; hz = 0.0; return; }
+  double A = z * z + (a + r) * (a + r);
+  double B = z * z + (r - a) * (r - a);
+  double k = 4.0 * r * a / A;
+  double Kk = ellipk_(k), Ek = ellipe_(k);
+  double sA = sqrt(A);
+  hr = (r < AXIS_EPS) ? 0.0 : z / (2.0 * PI * r * sA) * ((z * z + r * r + a * a) / B * Ek - Kk);
+  hz = 1.0 / (2.0 * PI * sA) * ((a * a - z * z - r * r) / B * Ek + Kk);
+}
+
+// current_potential_axial_radial_ring, radial_ring.c:73-76
+TB_HD double current_ring_axial_potential(double z0, double r, double z) {
+  double dz = z0 - z;
+  return -dz / (2.0 * sqrt(dz * dz + r * r));
+}
+
+// axial_derivatives_radial_ring, radial_ring.c:109-122 (times r/2 applied by caller's weight)
+TB_HD void ring_axial_derivs(double z0, double r, double z, double d[NDERIV]) {
+  double u = z0 - z;
+  double R2 = fma(u, u, r * r);
+  double R = sqrt(R2);
+  double iR2 = 1.0 / R2;
+  d[0] = 1.0 / R;
+  d[1] = -u * d[0] * iR2;
+#pragma unroll
+  for (int n = 1; n < NDERIV - 1; n++) d[n + 1] = -((2 * n + 1) * u * d[n] + double(n * n) * d[n - 1]) * iR2;
+}
+
+// current_axial_derivatives_radial_ring, radial_ring.c:124-136
+TB_HD void current_ring_axial_derivs(double z0, double r, double z, double d[NDERIV]) {
+  double u = z0 - z;
+  double R2 = u * u + r * r;
+  double R = sqrt(R2);
+  double mu = u / R;
+  d[0] = -u / (2.0 * R);
+  d[1] = -r * r / (2.0 * R2 * R);
+#pragma unroll
+  for (int n = 2; n < NDERIV; n++) d[n] = -(2 * n - 1) * mu / R * d[n - 1] - double(n * n - 2 * n) / R2 * d[n - 2];
+}
+
+// ---------------------------------------------------------------------------------
+// Paraxial series evaluation (radial.c:449-521).  coeffs (Nz-1, 9, 6) highest power first.
+// ---------------------------------------------------------------------------------
+TB_HD bool axial_local(const double* zs, int Nz, const double* coeffs, double z, double d[NDERIV]) {
+  double z0 = zs[0], zl = zs[Nz - 1];
+  if (!(z0 <= z && z <= zl)) return false;
+  double dz = zs[1] - zs[0];
+  int idx = int((z - z0) / dz);
+  idx = idx < 0 ? 0 : idx;
+  idx = idx > Nz - 2 ? Nz - 2 : idx;
+  double x = z - zs[idx];
+  const double* C = coeffs + (size_t)idx * (NDERIV * 6);
+#pragma unroll
+  for (int l = 0; l < NDERIV; l++) {
+    const double* c = C + 6 * l;
+    d[l] = fma(fma(fma(fma(fma(c[0], x, c[1]), x, c[2]), x, c[3]), x, c[4]), x, c[5]);
+  }
+  return true;
+}
+
+TB_HD double axial_potential(const double* zs, int Nz, const double* coeffs, double x, double y, double z) {
+  double d[NDERIV];
+  if (!axial_local(zs, Nz, coeffs, z, d)) return 0.0;
+  double r2 = x * x + y * y;
+  return d[0] - r2 / 4.0 * d[2] + r2 * r2 / 64.0 * d[4] - r2 * r2 * r2 / 2304.0 * d[6] + r2 * r2 * r2 * r2 / 147456.0 * d[8];
+}
+
+TB_HD void axial_field(const double* zs, int Nz, const double* coeffs, double x, double y, double z, double out[3]) {
+  double d[NDERIV];
+  if (!axial_local(zs, Nz, coeffs, z, d)) { out[0] = out[1] = out[2] = 0.0; return; }
+  double r2 = x * x + y * y;
+  double fr = 0.5 * (d[2] - r2 / 8.0 * d[4] + r2 * r2 / 192.0 * d[6] - r2 * r2 * r2 / 9216.0 * d[8]);
+  double fz = -d[1] + r2 / 4.0 * d[3] - r2 * r2 / 64.0 * d[5] + r2 * r2 * r2 / 2304.0 * d[7];
+  out[0] = x * fr; out[1] = y * fr; out[2] = fz;
+}
+
+}  // namespace tb

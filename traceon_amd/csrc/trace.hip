@@ -1,0 +1,308 @@
+// RK45 (Fehlberg 4(5)) particle tracing on gfx950 — replaces trace_particle and its
+// field adaptors (traceon/backend/tracing.c:20-174) and the Python block loop
+// (traceon/backend/__init__.py:382-420).
+//
+// Persistent kernel, one lane per particle.  Every lane of a wave runs the same
+// attempted step (6 stages, 6 field evaluations, error norm, controller), so the only
+// divergence is the per-lane accept/reject select.  A lane whose particle leaves the
+// bounds (or reaches its step cap / trajectory capacity) retires: it writes its results
+// and takes the next particle index from a global counter (one atomic per wave, handed
+// out by prefix popcount), so the wave stays full until the queue is empty.  For the
+// direct BEM field the source loop is wave-uniform (scalar loads shared by all lanes).
+//
+// The landing state on a plane is tracked on the fly: after every accepted step the
+// signed distances of the previous and new point are compared exactly as
+// plane_intersection (tracing.c:184-223) does when it scans the stored trajectory
+// backwards, and the last crossing found is kept.
+#include "radial_math.h"
+#include "trace_kernels.h"
+
+namespace tb {
+
+__constant__ static const double RK_B[6][5] = {
+    {0, 0, 0, 0, 0},
+    {2. / 9., 0, 0, 0, 0},
+    {1. / 12., 1. / 4., 0, 0, 0},
+    {69. / 128., -243. / 128., 135. / 64., 0, 0},
+    {-17. / 12., 27. / 4., -27. / 5., 16. / 15., 0},
+    {65. / 432., -5. / 16., 13. / 16., 4. / 27., 5. / 144.}};
+__constant__ static const double RK_CH[6] = {47. / 450., 0., 12. / 25., 32. / 225., 1. / 30., 6. / 25.};
+__constant__ static const double RK_CT[6] = {-1. / 150., 0., 3. / 100., -16. / 75., -1. / 20., 6. / 25.};
+
+// ---------------------------------------------------------------------------------
+// field evaluation
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ void bem_field(const SourceSet& S, double x, double y, double z, double out[3]) {
+  const double r0 = sqrt(x * x + y * y);
+  double Er = 0.0, Ez = 0.0;
+  for (int s = 0; s < S.n; s++) {
+    double gr, gz;
+    ring_grad(r0, S.r[s] - r0, S.z[s] - z, gr, gz);
+    Er = fma(-S.w[s], gr, Er);
+    Ez = fma(-S.w[s], gz, Ez);
+  }
+  if (r0 >= AXIS_EPS) { out[0] = x / r0 * Er; out[1] = y / r0 * Er; }
+  else { out[0] = 0.0; out[1] = 0.0; }
+  out[2] = Ez;
+}
+
+__device__ __forceinline__ void coil_field(const SourceSet& S, double x, double y, double z, double out[3]) {
+  const double r = sqrt(x * x + y * y);
+  double Hr = 0.0, Hz = 0.0;
+  for (int s = 0; s < S.n; s++) {
+    double hr, hz;
+    current_ring_field(r, z, S.r[s], S.z[s], hr, hz);
+    Hr = fma(S.w[s], hr, Hr);
+    Hz = fma(S.w[s], hz, Hz);
+  }
+  if (r >= AXIS_EPS) { out[0] = x / r * Hr; out[1] = y / r * Hr; }
+  else { out[0] = 0.0; out[1] = 0.0; }
+  out[2] = Hz;
+}
+
+__device__ __forceinline__ void to_local(const FieldComp& c, const double p[3], double q[3]) {
+  for (int i = 0; i < 3; i++) q[i] = c.inv[4 * i] * p[0] + c.inv[4 * i + 1] * p[1] + c.inv[4 * i + 2] * p[2] + c.inv[4 * i + 3];
+}
+__device__ __forceinline__ void rotate(const FieldComp& c, double v[3]) {
+  double a = c.basis[0] * v[0] + c.basis[1] * v[1] + c.basis[2] * v[2];
+  double b = c.basis[3] * v[0] + c.basis[4] * v[1] + c.basis[5] * v[2];
+  double d = c.basis[6] * v[0] + c.basis[7] * v[1] + c.basis[8] * v[2];
+  v[0] = a; v[1] = b; v[2] = d;
+}
+__device__ __forceinline__ bool in_bounds(const double* b, int mode, const double q[3]) {
+  if (mode == BOUNDS_NONE) return true;
+  if (mode == BOUNDS_STRICT)   // field_radial_traceable (tracing.c:136-138)
+    return b[0] < q[0] && q[0] < b[1] && b[2] < q[1] && q[1] < b[3] && b[4] < q[2] && q[2] < b[5];
+  // Field._within_field_bounds (field.py:220-221)
+  return b[0] <= q[0] && q[0] <= b[1] && b[2] <= q[1] && q[1] <= b[3] && b[4] <= q[2] && q[2] <= b[5];
+}
+
+__device__ void eval_field(const FieldProgram& F, const double p[3], double E[3], double H[3]) {
+  E[0] = E[1] = E[2] = 0.0;
+  H[0] = H[1] = H[2] = 0.0;
+  if (!in_bounds(F.bounds, F.bounds_mode, p)) return;
+  for (int ci = 0; ci < F.ncomp; ci++) {
+    const FieldComp& c = F.comp[ci];
+    double q[3] = {p[0], p[1], p[2]};
+    if (c.transform) to_local(c, p, q);
+    if (!in_bounds(c.bounds, c.bounds_mode, q)) continue;
+    double e[3] = {0, 0, 0}, h[3] = {0, 0, 0};
+    if (c.kind == COMP_BEM) {
+      if (c.elec.n) bem_field(c.elec, q[0], q[1], q[2], e);
+      if (c.mag.n) bem_field(c.mag, q[0], q[1], q[2], h);
+      if (c.cur.n) {
+        double hc[3];
+        if (c.transform) {
+          // FieldRadialBEM.magnetostatic_field_at_local_point (field.py:672-695) evaluates the
+          // coil field through current_field_at_point on the already local point: that maps
+          // it once more, checks the bounds there and rotates by the basis.
+          double q2[3];
+          to_local(c, q, q2);
+          if (in_bounds(c.bounds, c.bounds_mode, q2)) { coil_field(c.cur, q2[0], q2[1], q2[2], hc); rotate(c, hc); }
+          else { hc[0] = hc[1] = hc[2] = 0.0; }
+        } else {
+          coil_field(c.cur, q[0], q[1], q[2], hc);
+        }
+        h[0] += hc[0]; h[1] += hc[1]; h[2] += hc[2];
+      }
+    } else if (c.kind == COMP_AXIAL) {
+      axial_field(c.zs, c.nz, c.ce, q[0], q[1], q[2], e);
+      axial_field(c.zs, c.nz, c.cm, q[0], q[1], q[2], h);
+    } else {
+      for (int i = 0; i < 3; i++) { e[i] = c.uE[i]; h[i] = c.uH[i]; }
+    }
+    if (c.transform) { rotate(c, e); rotate(c, h); }
+    for (int i = 0; i < 3; i++) { E[i] = fma(c.factor, e[i], E[i]); H[i] = fma(c.factor, h[i], H[i]); }
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// the persistent tracer
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ double signed_dist(const TraceParams& P, const double y[6]) {
+  const double* n = P.plane_n;
+  const double* p0 = P.plane_p;
+  return (n[0] * (p0[0] - y[0]) + n[1] * (p0[1] - y[1]) + n[2] * (p0[2] - y[2])) / P.plane_nn;
+}
+
+__device__ __forceinline__ bool inside_trace_bounds(const double* b, const double y[6]) {
+  // tracing.c:79-81, inclusive
+  return b[0] <= y[0] && y[0] <= b[1] && b[2] <= y[1] && y[1] <= b[3] && b[4] <= y[2] && y[2] <= b[5];
+}
+
+__global__ __launch_bounds__(64) void k_trace(TraceParams P) {
+  const int lane = threadIdx.x & 63;
+  // lane state
+  int pid = -1;
+  double y[6], h = 0.0, hmax = 0.0, t = 0.0, qm = 0.0;
+  long long nacc = 0, natt = 0;
+  double kprev = 0.0;
+  int found = 0;
+  double land[6];
+  long long traj_n = 0;
+
+  auto next_particle = [&](bool need) {
+    // wave-aggregated queue pop for every lane with need == true
+    unsigned long long mask = __ballot(need);
+    int cnt = __popcll(mask);
+    int base = 0;
+    if (cnt) {
+      const int leader = __ffsll((long long)mask) - 1;
+      if (lane == leader) base = atomicAdd(P.counter, cnt);
+      base = __shfl(base, leader, 64);
+    }
+    if (need) {
+      const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+      const int id = base + rank;
+      if (id < P.n) {
+        pid = id;
+        for (int c = 0; c < 6; c++) y[c] = P.state0[6 * (size_t)id + c];
+        t = P.t0 ? P.t0[id] : 0.0;
+        qm = P.qm[P.qm_stride ? id : 0];
+        const double speed = sqrt(y[3] * y[3] + y[4] * y[4] + y[5] * y[5]);
+        hmax = STEP_MAX / speed;   // TRACING_STEP_MAX, tracing.c:5, 68-70
+        h = (P.h0 && P.h0[id] > 0.0) ? P.h0[id] : hmax;
+        nacc = 0; natt = 0; found = 0;
+        kprev = signed_dist(P, y);
+        traj_n = 1;
+        if (P.traj) {
+          double* row = P.traj + (size_t)id * P.traj_cap * 6;
+          for (int c = 0; c < 6; c++) row[c] = y[c];
+          if (P.times) P.times[(size_t)id * P.traj_cap] = t;
+        }
+      } else {
+        pid = -1;
+      }
+    }
+  };
+
+  auto retire = [&](int status) {
+    const size_t id = (size_t)pid;
+    for (int c = 0; c < 6; c++) P.final_state[6 * id + c] = y[c];
+    if (P.landing) {
+      for (int c = 0; c < 6; c++) P.landing[6 * id + c] = found ? land[c] : __builtin_nan("");
+    }
+    if (P.status) P.status[id] = status;
+    if (P.n_accepted) P.n_accepted[id] = nacc;
+    if (P.n_attempted) P.n_attempted[id] = natt;
+    if (P.t_final) P.t_final[id] = t;
+    if (P.h_final) P.h_final[id] = h;
+    if (P.traj_len) P.traj_len[id] = traj_n;
+  };
+
+  for (int c = 0; c < 6; c++) { y[c] = 0.0; land[c] = 0.0; }
+  next_particle(true);
+  while (true) {
+    // a fresh particle that starts outside the bounds takes no step (tracing.c:79 loop not entered)
+    while (true) {
+      const bool out0 = (pid >= 0) && natt == 0 && !inside_trace_bounds(P.bounds, y);
+      if (out0) retire(STATUS_LEFT_BOUNDS);
+      if (__ballot(out0) == 0ull) break;
+      if (out0) pid = -1;
+      next_particle(out0);
+    }
+    if (__ballot(pid >= 0) == 0ull) break;
+    const bool active = pid >= 0;
+    double k[6][6];
+#pragma unroll
+    for (int s = 0; s < 6; s++) {
+      double ys[6];
+#pragma unroll
+      for (int c = 0; c < 6; c++) {
+        double acc = y[c];
+#pragma unroll
+        for (int j = 0; j < s; j++) acc = fma(RK_B[s][j], k[j][c], acc);
+        ys[c] = acc;
+      }
+      double E[3], H[3];
+      eval_field(P.field, ys, E, H);
+      const double vxh0 = ys[4] * H[2] - ys[5] * H[1];
+      const double vxh1 = ys[5] * H[0] - ys[3] * H[2];
+      const double vxh2 = ys[3] * H[1] - ys[4] * H[0];
+      k[s][0] = h * ys[3]; k[s][1] = h * ys[4]; k[s][2] = h * ys[5];
+      k[s][3] = h * qm * (E[0] + MU0 * vxh0);
+      k[s][4] = h * qm * (E[1] + MU0 * vxh1);
+      k[s][5] = h * qm * (E[2] + MU0 * vxh2);
+    }
+    double ep = 0.0, ev = 0.0;
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+      double e = 0.0;
+#pragma unroll
+      for (int s = 0; s < 6; s++) e = fma(RK_CT[s], k[s][c], e);
+      e = fabs(e);
+      if (c < 3) ep = e > ep ? e : ep;
+      else ev = e > ev ? e : ev;
+    }
+    const double err = ep + h * ev;
+    int st = -1;
+    if (active) {
+      natt++;
+      if (err <= P.atol) {
+        double yn[6];
+#pragma unroll
+        for (int c = 0; c < 6; c++)
+          yn[c] = y[c] + (RK_CH[0] * k[0][c] + RK_CH[1] * k[1][c] + RK_CH[2] * k[2][c] + RK_CH[3] * k[3][c] +
+                          RK_CH[4] * k[4][c] + RK_CH[5] * k[5][c]);
+        t += h;
+        nacc++;
+        if (P.landing) {
+          const double kn = signed_dist(P, yn);
+          if ((kn > 0) != (kprev > 0)) {
+            const double diff = kprev - kn;   // kappa(i) - kappa(i+1) with i the previous point
+            const double f = -kn / diff, fp = kprev / diff;
+#pragma unroll
+            for (int c = 0; c < 6; c++) land[c] = fp * yn[c] + f * y[c];
+            found = 1;
+          }
+          kprev = kn;
+        }
+#pragma unroll
+        for (int c = 0; c < 6; c++) y[c] = yn[c];
+        if (P.traj) {
+          double* row = P.traj + ((size_t)pid * P.traj_cap + traj_n) * 6;
+          for (int c = 0; c < 6; c++) row[c] = y[c];
+          if (P.times) P.times[(size_t)pid * P.traj_cap + traj_n] = t;
+        }
+        traj_n++;
+        if (!inside_trace_bounds(P.bounds, y)) st = STATUS_LEFT_BOUNDS;
+        else if (P.traj && traj_n == P.traj_cap) st = STATUS_BLOCK_FULL;
+      }
+      const double hn = 0.9 * h * pow(P.atol / err, 0.2);   // tracing.c:119
+      h = hn < hmax ? hn : hmax;
+      if (st < 0 && natt >= P.max_attempts) st = STATUS_STEP_CAP;
+      if (st >= 0) retire(st);
+    }
+    const bool need = st >= 0;
+    if (need) pid = -1;
+    next_particle(need);
+  }
+}
+
+// single field evaluation at a batch of points through a field program
+__global__ __launch_bounds__(256) void k_eval_program(FieldProgram F, const double* __restrict__ pts, int M, double* __restrict__ E,
+                                                      double* __restrict__ H) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= M) return;
+  double p[3] = {pts[3 * (size_t)t], pts[3 * (size_t)t + 1], pts[3 * (size_t)t + 2]};
+  double e[3], h[3];
+  eval_field(F, p, e, h);
+  for (int i = 0; i < 3; i++) { if (E) E[3 * (size_t)t + i] = e[i]; if (H) H[3 * (size_t)t + i] = h[i]; }
+}
+
+hipError_t launch_trace(const TraceParams& P, int waves, hipStream_t st) {
+  if (P.n <= 0) return hipSuccess;
+  int nw = waves > 0 ? waves : 1;
+  int need = (P.n + 63) / 64;
+  if (nw > need) nw = need;
+  hipLaunchKernelGGL(k_trace, dim3(nw), dim3(64), 0, st, P);
+  return hipGetLastError();
+}
+
+hipError_t launch_eval_program(const FieldProgram& F, const double* pts, int M, double* E, double* H, hipStream_t st) {
+  if (M <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_eval_program, dim3((M + 255) / 256), dim3(256), 0, st, F, pts, M, E, H);
+  return hipGetLastError();
+}
+
+}  // namespace tb

@@ -1,0 +1,73 @@
+// Device field programs and the tracer launch interface (trace.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tb {
+
+constexpr double STEP_MAX = 0.01;   // TRACING_STEP_MAX, tracing.c:5
+
+enum { COMP_BEM = 0, COMP_AXIAL = 1, COMP_UNIFORM = 2 };
+enum { BOUNDS_NONE = 0, BOUNDS_STRICT = 1, BOUNDS_INCLUSIVE = 2 };
+enum { STATUS_LEFT_BOUNDS = 0, STATUS_BLOCK_FULL = 1, STATUS_STEP_CAP = 2 };
+
+// Flattened ring sources: weight (charge * Gauss weight * jacobian, or current density
+// * weight * area), ring radius, ring height.
+struct SourceSet {
+  const double* w;
+  const double* r;
+  const double* z;
+  int n, pad;
+};
+
+// One field of a superposition.  A plain FieldRadialBEM / FieldRadialAxial is one
+// component without transform; FieldSuperposition members carry their affine map.
+struct FieldComp {
+  int kind, transform, bounds_mode, pad;
+  double factor;
+  double inv[12];    // global -> local, 3x4 row-major
+  double basis[9];   // local field vector -> global, row-major
+  double bounds[6];
+  SourceSet elec, mag, cur;
+  const double* zs;
+  const double* ce;
+  const double* cm;
+  int nz, pad2;
+  double uE[3], uH[3];
+};
+
+struct FieldProgram {
+  const FieldComp* comp;   // device array
+  int ncomp, bounds_mode;
+  double bounds[6];
+};
+
+struct TraceParams {
+  FieldProgram field;
+  const double* state0;   // (n, 6)
+  const double* t0;       // (n) or null
+  const double* h0;       // (n) or null: first step, default hmax
+  const double* qm;       // charge / mass, (n) or (1)
+  int qm_stride, n;
+  int* counter;           // zeroed work counter
+  double bounds[6];
+  double atol;
+  long long max_attempts;
+  double plane_p[3], plane_n[3], plane_nn;
+  double* final_state;    // (n, 6)
+  double* landing;        // (n, 6) or null
+  int* status;
+  long long* n_accepted;
+  long long* n_attempted;
+  double* t_final;
+  double* h_final;
+  double* traj;           // (n, traj_cap, 6) or null
+  double* times;          // (n, traj_cap) or null
+  long long traj_cap;
+  long long* traj_len;
+};
+
+hipError_t launch_trace(const TraceParams& P, int waves, hipStream_t st);
+hipError_t launch_eval_program(const FieldProgram& F, const double* pts, int M, double* E, double* H, hipStream_t st);
+
+}  // namespace tb
