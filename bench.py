@@ -1,0 +1,321 @@
+"""Benchmark of the radial BEM hot path on one MI355X.
+
+Workload (one "step"): the 5-electrode electrostatic mirror meshed into N = 20,000 cubic
+line elements, voltages U[-2, 2] kV from seed 1: dense fill (FP64 + FP32 copy) + singular
+diagonal + LU of the FP32 copy on MFMA + FP64 double-double iterative refinement, i.e. the
+BEM fill+solve of the metric.  Reported beside it in the same JSON line:
+
+* the FP64 path (FP64 LU) time and the charge difference of the two paths;
+* the field (potential + E) on a 1000 x 1000 (r, z) grid (10^6 points) from the charges;
+* batched tracing: the 4-electrode lens (N = 8,000) with a -1.1 kV reflecting electrode,
+  10^6 electrons (lognormal energy, median 1 keV, sigma 0.5; disc of 0.2 mm diameter;
+  5 mrad angles) through the FieldRadialAxial interpolation, landing RMS error against
+  the direct FP64 BEM tracer on a seeded sample;
+* roofline of the dominant kernel (HIP events on the launch stream);
+* cpu_baseline: the reference's own C backend (oracle/_ref, compiled from its sources)
+  on bounded samples of the same work, on this host's cores.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-trace] [--no-field]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "hbm": 8.0e12}
+# algorithmic FLOP per unit (reference formulas, +-*/sqrt = 1, FMA = 2, log = 20)
+C_POT, C_FIELD, C_FLUX = 71, 215, 219
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+class Timer:
+    """HIP events recorded on the current (launch) stream."""
+
+    def __init__(self):
+        import torch
+        self.s = torch.cuda.Event(enable_timing=True)
+        self.e = torch.cuda.Event(enable_timing=True)
+
+    def __enter__(self):
+        self.s.record()
+        return self
+
+    def __exit__(self, *a):
+        import torch
+        self.e.record()
+        torch.cuda.synchronize()
+        self.ms = self.s.elapsed_time(self.e)
+
+
+def mirror_problem(N):
+    from traceon_amd import configs
+    mesh, volts = configs.mirror(N, seed=1)
+    return configs.lines_types_values(mesh, volts)
+
+
+def fill_solve(lines, types, values, mode):
+    """One fill+solve step on the device; returns (charges tensor, stats, times dict)."""
+    import torch
+    from traceon_amd import device
+    t = {}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    E = device.Elements(lines, types, values)
+    with Timer() as tf:
+        M, M32 = device.assemble(E, fp64=True, fp32=(mode == "mixed"))
+    F = np.where(np.isin(types, [1, 2, 5]), values, 0.0)
+    with Timer() as ts:
+        X, stats, lu = device.solve_system(M, F, M32=M32, mode=mode)
+    torch.cuda.synchronize()
+    t["total_s"] = time.perf_counter() - t0
+    t["fill_ms"] = tf.ms
+    t["solve_ms"] = ts.ms
+    del M, M32, lu
+    return X, stats, t
+
+
+def fill_kernel_roofline(lines, types, values, reps=3):
+    """Time the fill kernel alone (HIP events on its stream) for the roofline entry."""
+    import torch
+    from traceon_amd import device
+    E = device.Elements(lines, types, values)
+    N = E.N
+    M = torch.empty((N, N), dtype=torch.float64, device="cuda")
+    L = device._L
+    L.tb_fill_matrix(device.ptr(M), N, None, 0, device.ptr(E.rows), device.ptr(E.src), N, 0, N - 1, 0, device.stream_ptr())
+    with Timer() as t:
+        for _ in range(reps):
+            L.tb_fill_matrix(device.ptr(M), N, None, 0, device.ptr(E.rows), device.ptr(E.src), N, 0, N - 1, 0, device.stream_ptr())
+    ms = t.ms / reps
+    npot = int(np.sum(np.isin(types, [1, 2, 5])))
+    flops = (npot * C_POT + (N - npot) * C_FLUX) * N * 16.0
+    del M
+    return ms, flops
+
+
+def field_grid_leg(X, lines, types, values, n_side=1000):
+    import torch
+    from traceon_amd import device
+    E = device.Elements(lines, types, values)
+    S = device.Sources(X[0], E.jac, E.pos)
+    r = np.linspace(0.0, 4.0e-3, n_side)
+    z = np.linspace(-0.2e-3, 10.2e-3, n_side)
+    R, Z = np.meshgrid(r, z, indexing="ij")
+    pts = device.to_dev(np.column_stack([R.ravel(), np.zeros(R.size), Z.ravel()]))
+    S.field_points(pts[:1000])   # warm-up
+    with Timer() as t:
+        P, Fd = S.field_points(pts)
+    pairs = float(pts.shape[0]) * S.n
+    return t.ms, pairs, P, Fd, pts
+
+
+def tracing_leg(n_particles, n_sample, seed=2):
+    """Batched tracing through the 4-electrode lens (field from a GPU solve)."""
+    import torch
+    from traceon_amd import configs, solver, tracing
+    from traceon_amd.excitation import Excitation
+    from traceon_amd.field import FieldRadialAxial
+    mm = 1e-3
+    mesh, volts = configs.einzel_lens(8000, voltages=(0.0, 1800.0, 0.0), fourth=(4.0 * mm, 1.0 * mm, -1100.0))
+    exc = Excitation(mesh)
+    exc.add_voltage(**volts)
+    field = solver.solve_direct(exc)
+    bounds = np.array([[-1.0 * mm, 1.0 * mm], [-1.0 * mm, 1.0 * mm], [-5.0 * mm, 10.0 * mm]])
+    t0 = time.perf_counter()
+    axial = FieldRadialAxial(field, -5.0 * mm, 10.0 * mm, N=3000)
+    torch.cuda.synchronize()
+    t_axial_build = time.perf_counter() - t0
+    rng = np.random.default_rng(seed)
+    n = n_particles
+    energy = 1000.0 * np.exp(0.5 * rng.standard_normal(n))
+    rad = 0.1 * mm * np.sqrt(rng.random(n))       # uniform in a disc of 0.2 mm diameter
+    phi = 2 * np.pi * rng.random(n)
+    tx, ty = rng.normal(0, 5e-3, n), rng.normal(0, 5e-3, n)
+    d = np.column_stack([np.tan(tx), np.tan(ty), np.ones(n)])
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    speed = np.sqrt(2 * energy * tracing.E_CHARGE / tracing.M_E)
+    P = np.column_stack([rad * np.cos(phi), rad * np.sin(phi), np.full(n, -5.0 * mm)])
+    V = d * speed[:, None]
+    plane = (np.array([0.0, 0.0, 9.5 * mm]), np.array([0.0, 0.0, 1.0]))
+    tr_ax = axial.get_tracer(bounds)
+    warm = tr_ax.trace_batch(P[:4096], V[:4096], plane=plane)
+    torch.cuda.synchronize()
+    with Timer() as t:
+        res = tr_ax.trace_batch(P, V, plane=plane)
+    steps = int(res.n_attempted.sum().item())
+    land = res.landing.cpu().numpy()
+    reflected = float(np.mean(res.final[:, 2].cpu().numpy() < -4.999 * mm))
+    nacc = res.n_accepted.cpu().numpy()
+    # error reference: direct FP64 BEM field, same particles (sample)
+    tr_bem = field.get_tracer(bounds)
+    with Timer() as tb:
+        ref = tr_bem.trace_batch(P[:n_sample], V[:n_sample], plane=plane)
+    land_ref = ref.landing.cpu().numpy()
+    ok = np.isfinite(land_ref[:, 0]) & np.isfinite(land[:n_sample, 0])
+    err = np.linalg.norm(land[:n_sample][ok, :2] - land_ref[ok, :2], axis=1)
+    steps_bem = int(ref.n_attempted.sum().item())
+    return {
+        "n": n, "ms": t.ms, "steps": steps, "reflected_fraction": reflected,
+        "len_ratio": float(np.percentile(nacc, 99.9) / max(np.percentile(nacc, 0.1), 1)),
+        "landing_rms_nm": float(np.sqrt(np.mean(err ** 2)) * 1e9) if ok.any() else None,
+        "landing_max_nm": float(np.max(err) * 1e9) if ok.any() else None,
+        "n_sample": int(n_sample), "bem_ms": tb.ms, "bem_steps": steps_bem, "n_axial": len(axial.z),
+        "axial_build_s": t_axial_build, "field": field, "axial": axial, "bounds": bounds, "P": P, "V": V, "plane": plane,
+        "N_elements": len(field.electrostatic_point_charges),
+    }
+
+
+def cpu_baseline(lines, types, values, trace_info, threads):
+    """The reference's C backend (oracle/_ref) on bounded samples, on `threads` host cores.
+    Fill: a contiguous block of rows split over threads (util.split_collect); diagonal:
+    scipy QUADPACK on a sample of elements; solve: numpy.linalg.solve at a smaller N
+    scaled by N^3.  Result: estimated seconds for the full N = 20,000 fill+solve."""
+    from oracle.oracle import reference, restatement
+    lib = reference()
+    kind = "reference"
+    if lib is None:
+        lib, kind = restatement(), "port"
+    N = len(lines)
+    jac, pos = lib.fill_jacobian_buffer_radial(lines)
+    rows = max(threads * 8, 256)
+    M = np.zeros((rows, N))
+    import threading
+    chunks = [c for c in np.array_split(np.arange(rows), threads) if len(c)]
+
+    def work(c):
+        lib.lib.fill_matrix_radial(M, lines, types, values, jac, pos, N, N, int(c[0]), int(c[-1]))
+
+    t0 = time.perf_counter()
+    ts = [threading.Thread(target=work, args=(c,)) for c in chunks]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    t_fill = (time.perf_counter() - t0) * N / rows
+    nd = 200
+    t0 = time.perf_counter()
+    for i in np.linspace(0, N - 1, nd).astype(int):
+        lib.self_potential_quad(lines[i])
+    t_diag = (time.perf_counter() - t0) * N / nd
+    Ns = 6000
+    A = np.random.default_rng(0).standard_normal((Ns, Ns)) + Ns * np.eye(Ns)
+    b = np.ones(Ns)
+    t0 = time.perf_counter()
+    np.linalg.solve(A, b)
+    t_solve = (time.perf_counter() - t0) * (N / Ns) ** 3
+    out = {"fill_solve_s": t_fill + t_diag + t_solve, "fill_s": t_fill, "diag_s": t_diag, "solve_s": t_solve, "kind": kind,
+           "sample": f"fill {rows} of {N} rows over {threads} threads, QUADPACK diagonal {nd} of {N} elements (1 thread, as "
+                     f"solver.py:239-247), numpy.linalg.solve at N={Ns} scaled by (N/{Ns})^3"}
+    if trace_info is not None:
+        # tracing: the reference C tracer through its axial field function, one particle per call
+        ax = trace_info["axial"]
+        P, V = trace_info["P"], trace_info["V"]
+        ns = 64
+        t0 = time.perf_counter()
+
+        def tw(idx):
+            for i in idx:
+                lib.trace_axial(P[i], V[i], -1.602176634e-19 / 9.1093837015e-31, ax.z, ax.electrostatic_coeffs,
+                                ax.magnetostatic_coeffs, trace_info["bounds"])
+
+        ts = [threading.Thread(target=tw, args=(c,)) for c in np.array_split(np.arange(ns), threads) if len(c)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        out["electrons_per_s_axial"] = ns / (time.perf_counter() - t0)
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--N", type=int, default=20000)
+    ap.add_argument("--particles", type=int, default=1_000_000)
+    ap.add_argument("--sample", type=int, default=1024)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-trace", action="store_true")
+    ap.add_argument("--no-field", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md for how it would shard)",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    assert torch.cuda.is_available(), "bench.py needs the GPU"
+    lines, types, values = mirror_problem(args.N)
+    N = len(lines)
+    log(f"bench: mirror N={N}, warmup {args.warmup}, steps {args.steps}")
+    for _ in range(args.warmup):
+        fill_solve(lines, types, values, "mixed")
+    times, tfs = [], []
+    for k in range(args.steps):
+        torch.cuda.synchronize()
+        X, stats, t = fill_solve(lines, types, values, "mixed")
+        times.append(t["total_s"])
+        tfs.append(t)
+        log(f"bench: step {k}: fill+solve {t['total_s']:.3f} s (fill {t['fill_ms']:.1f} ms, solve {t['solve_ms']:.1f} ms) {stats}")
+    X64, stats64, t64 = fill_solve(lines, types, values, "fp64")
+    log(f"bench: fp64 path {t64['total_s']:.3f} s {stats64}")
+    dq = float((X - X64).abs().max().item() / X64.abs().max().item())
+    fill_ms, fill_flops = fill_kernel_roofline(lines, types, values)
+    log(f"bench: fill kernel {fill_ms:.2f} ms, {fill_flops / fill_ms / 1e9:.1f} TFLOP/s")
+    result = {
+        "metric": "electrons traced/s @<=10nm landing error; BEM fill+solve time at N=20k",
+        "value": float(np.median(times)), "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": float(np.median(times) * 1e3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 (fill, residuals) + f32 MFMA (LU)", "data": "synthetic geometry (seeded), no external data",
+        "config": {"workload": f"electrostatic mirror, 5 electrodes, N={N} line elements: fill + FP32-LU + FP64 refinement",
+                   "N": N, "parallelism": "single GPU"},
+        "fill_solve_fp64_s": t64["total_s"], "fill_ms": float(np.median([t["fill_ms"] for t in tfs])),
+        "solve_ms": float(np.median([t["solve_ms"] for t in tfs])), "refine_iterations": stats.iterations,
+        "charge_rel_diff_mixed_vs_fp64": dq,
+    }
+    roof = {"kernel": "k_fill (dense FP64 fill)", "bound": "fp64-valu", "achieved": fill_flops / (fill_ms * 1e-3) / 1e12,
+            "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s", "traffic": None,
+            "flop_per_launch": fill_flops, "launch_ms": fill_ms}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    result["roofline"] = roof
+    if not args.no_field:
+        ms, pairs, P, Fd, pts = field_grid_leg(X, lines, types, values)
+        result["field_grid"] = {"points": int(pts.shape[0]), "ms": ms, "points_per_s": pts.shape[0] / (ms * 1e-3),
+                                "tflops": pairs * (C_POT + C_FIELD) / (ms * 1e-3) / 1e12}
+        log(f"bench: field grid {pts.shape[0]} points in {ms:.1f} ms")
+    trace_info = None
+    if not args.no_trace:
+        trace_info = tracing_leg(args.particles, args.sample)
+        eps = trace_info["n"] / (trace_info["ms"] * 1e-3)
+        result["tracing"] = {k: trace_info[k] for k in ("n", "ms", "steps", "reflected_fraction", "len_ratio", "landing_rms_nm",
+                                                        "landing_max_nm", "n_sample", "n_axial", "N_elements")}
+        result["tracing"]["electrons_per_s"] = eps
+        result["tracing"]["electrons_per_s_direct_bem_fp64"] = trace_info["n_sample"] / (trace_info["bem_ms"] * 1e-3)
+        result["electrons_per_s"] = eps
+        log(f"bench: traced {trace_info['n']} electrons in {trace_info['ms']:.1f} ms ({eps:.3e}/s), "
+            f"landing RMS {trace_info['landing_rms_nm']} nm vs direct BEM")
+    if not args.no_cpu:
+        threads = min(16, len(os.sched_getaffinity(0)))
+        cb = cpu_baseline(lines, types, values, trace_info, threads)
+        result["cpu_baseline"] = {"value": cb["fill_solve_s"], "unit": "s", "cores": threads, "kind": cb["kind"],
+                                  "sample": cb["sample"], "fill_s": cb["fill_s"], "diag_s": cb["diag_s"], "solve_s": cb["solve_s"]}
+        if "electrons_per_s_axial" in cb:
+            result["cpu_baseline"]["electrons_per_s_axial"] = cb["electrons_per_s_axial"]
+        result["speedup_vs_cpu"] = cb["fill_solve_s"] / result["value"]
+        log(f"bench: cpu baseline {cb}")
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

@@ -120,6 +120,8 @@ bool plane_intersection(const double *p0, const double *normal, const double *po
 const char *tb_last_error(void);
 void tb_clear_error(void);
 int tb_version(void);
+/* sizes of tb_field_comp_t, tb_field_program_t, tb_trace_params_t and one row record, for binding checks (host only) */
+void tb_struct_sizes(size_t *out4);
 int tb_synchronize(void *stream);
 
 /* Element preparation for the fill (fill_jacobian_buffer_radial + the per-row data of
@@ -212,8 +214,10 @@ typedef struct {
 } tb_trace_params_t;
 
 /* Persistent RK45 tracer over params->n particles (all pointers device pointers).
-   waves: number of 64-lane workgroups (0: 8 per CU). */
-int tb_trace(const tb_trace_params_t *params, int waves, void *stream);
+   waves: number of 64-lane workgroups (0: 8 per CU).  wave_per_particle = 0: one lane per
+   particle (cheap fields: axial interpolation, uniform); 1: one wave per particle with the
+   ring-source sums split over its 64 lanes (direct BEM fields). */
+int tb_trace(const tb_trace_params_t *params, int waves, int wave_per_particle, void *stream);
 int tb_eval_program(const tb_field_program_t *program, const double *pts, int M, double *E, double *H, void *stream);
 
 #ifdef __cplusplus

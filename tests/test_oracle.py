@@ -1,0 +1,413 @@
+"""Pins of the oracle (CPU restatement of the reference C backend).
+
+Each check is independent of the restated formula: the reference's own known-answer
+tests (tests/test_elliptic.py, test_utilities_2d.py, test_radial_ring.py, test_radial.py,
+test_tracing.py of the reference), closed forms, direct quadrature of the defining
+integrals, and element-wise agreement with the reference's own C code compiled from its
+sources (oracle/_ref) on seeded random inputs.
+"""
+import numpy as np
+import pytest
+from scipy.integrate import quad, dblquad, solve_ivp
+from scipy.interpolate import CubicSpline
+from scipy.special import ellipe as sp_ellipe, ellipk as sp_ellipk, ellipkm1 as sp_ellipkm1
+
+MU0 = 1.25663706212e-06
+EPS0 = 8.8541878128e-12
+E = 1.602176634e-19
+ME = 9.1093837015e-31
+EM = -E / ME
+
+
+def ring_theta(r0, z0, r, z):
+    """Potential factor of a ring at (r, z) on (r0, z0) by direct theta quadrature:
+    (1/4pi) int_0^2pi r / |x - x'| dtheta (the defining integral of potential_radial_ring)."""
+    f = lambda t: r / np.sqrt(r * r + r0 * r0 + (z0 - z) ** 2 - 2 * r * r0 * np.cos(t))
+    return quad(f, 0, 2 * np.pi, epsabs=5e-13, epsrel=5e-13)[0] / (4 * np.pi)
+
+
+def biot_savart_loop(current, p):
+    """mu0 H of a unit-radius loop in the xy plane at p by direct Biot-Savart quadrature."""
+    def integrand(t, axis):
+        rl = np.array([np.cos(t), np.sin(t), 0.0])
+        dl = np.array([-np.sin(t), np.cos(t), 0.0])
+        r = np.asarray(p) - rl
+        return np.cross(dl, r)[axis] / np.linalg.norm(r) ** 3
+    return current * MU0 / (4 * np.pi) * np.array([quad(integrand, 0, 2 * np.pi, args=(a,), epsabs=5e-13)[0] for a in range(3)])
+
+
+# ---------------------------------------------------------------- elliptic (test_elliptic.py:10-30)
+def test_ellipk(oracle):
+    x = np.linspace(-1, 1, 40)[1:-1]
+    assert np.allclose([oracle.lib.ellipk(v) for v in x], sp_ellipk(x), atol=0, rtol=1e-5)
+
+
+def test_ellipe(oracle):
+    x = np.linspace(-1, 1, 40)[1:-1]
+    assert np.allclose([oracle.lib.ellipe(v) for v in x], sp_ellipe(x), atol=0, rtol=1e-12)
+
+
+def test_ellipkm1(oracle):
+    x = np.linspace(0, 1)[1:-1]
+    assert np.allclose(sp_ellipkm1(x), [oracle.lib.ellipkm1(v) for v in x], atol=0, rtol=1e-12)
+    x = 10 ** (-np.linspace(1, 100, 5))
+    assert np.allclose(sp_ellipkm1(x), [oracle.lib.ellipkm1(v) for v in x], atol=0, rtol=1e-12)
+    assert np.allclose(sp_ellipe(1 - x), [oracle.lib.ellipem1(v) for v in x], atol=0, rtol=1e-12)
+
+
+# ---------------------------------------------------------------- element map (test_utilities_2d.py)
+def test_line4_nodes_and_delta(oracle):
+    line = np.array([[0.0, 0.0, 0.0], [0.25, 0.0, 0.0], [0.5, 0.0, 0.0], [1.0, 0.0, 0.0]])
+    # parameter order is rows 0, 2, 3, 1; feed the rows so that the nodes are line[0..3]
+    el = np.array([line[0], line[3], line[1], line[2]])
+    _, mid = oracle.position_and_jacobian_radial(0.0, el)
+    for a, row in zip([-1, -1 + 2 / 3, -1 + 4 / 3, 1], line):
+        _, p = oracle.position_and_jacobian_radial(a, el)
+        _, d = oracle.delta_position_and_jacobian_radial(a, el)
+        assert np.allclose(p, row[[0, 2]]) and np.allclose(p - mid, d)
+
+
+def test_line4_length(oracle):
+    line = np.array([[0.0, 0.0, 0.0], [1.5, 0.0, 0.0], [0.5, 0.0, 0.0], [1.0, 0.0, 0.0]])
+    jac, _ = oracle.fill_jacobian_buffer_radial(np.array([line]))
+    assert np.isclose(jac.sum(), 1.5)
+    length = quad(lambda a: oracle.position_and_jacobian_radial(a, line)[0], -1, 1)[0]
+    assert np.isclose(length, 1.5)
+
+
+def test_field_dot_normal_known_answer(oracle):
+    """test_utilities_2d.py:110-132: KAT -0.01893954382812056."""
+    verts = np.array([[2.1, 0.0, -0.74974852], [2.1, 0.0, -0.77751699], [2.1, 0.0, -0.80528545], [2.1, 0.0, -0.83305391]])
+    el = np.array([verts[0], verts[3], verts[1], verts[2]])
+    length = quad(lambda a: oracle.position_and_jacobian_radial(a, el)[0], -1, 1)[0]
+    assert np.isclose(length, verts[0, 2] - verts[3, 2])
+    normal = np.array([-1.0, 0.0])
+    middle = np.mean(verts, axis=0)
+
+    def integrand(a):
+        jac, pos = oracle.position_and_jacobian_radial(a, el)
+        Er = -oracle.lib.dr1_potential_radial_ring(middle[0], middle[2], pos[0] - middle[0], pos[1] - middle[2], None)
+        Ez = -oracle.lib.dz1_potential_radial_ring(middle[0], middle[2], pos[0] - middle[0], pos[1] - middle[2], None)
+        return jac * (normal[0] * Er + normal[1] * Ez)
+
+    fdn = quad(integrand, -1, 1, points=[-1.0, 0.0, 1.0])[0]
+    assert np.isclose(fdn, -0.01893954382812056)
+
+
+@pytest.mark.parametrize("verts,expected", [
+    ([[1.0, 0, 0.0], [1.0, 0, 1.0], [1.0, 0, 1 / 3], [1.0, 0, 2 / 3]], 2 * np.pi),
+    ([[1.0, 0, 0.0], [4.0, 0, 0.0], [2.0, 0, 0.0], [3.0, 0, 0.0]], 15 * np.pi),
+    ([[0.0, 0, 0.0], [1.0, 0, 1.0], [1 / 3, 0, 1 / 3], [2 / 3, 0, 2 / 3]], np.pi * np.sqrt(2)),
+])
+def test_charge_radial(oracle, verts, expected):
+    """test_radial.py:29-63."""
+    assert np.isclose(oracle.charge_radial(np.array(verts), 1.0), expected)
+
+
+# ---------------------------------------------------------------- ring kernels (test_radial_ring.py)
+def test_ring_potential_vs_theta_quadrature(oracle):
+    rng = np.random.default_rng(11)
+    for r0, z0, r, z in rng.random((20, 4)):
+        assert np.isclose(oracle.lib.potential_radial_ring(r0, z0, r - r0, z - z0, None), ring_theta(r0, z0, r, z), rtol=1e-12)
+
+
+def test_ring_potential_on_axis(oracle):
+    """test_radial_ring.py:100-114: k Q / sqrt(z^2 + r^2) on the axis."""
+    z = np.linspace(-10, 10, 200)
+    r = 1.5
+    ours = np.array([oracle.lib.potential_radial_ring(0.0, z0, r, -z0, None) for z0 in z])
+    assert np.allclose(ours, 0.5 * r / np.sqrt(z ** 2 + r ** 2), rtol=1e-13)
+
+
+def test_ring_gradient_consistency(oracle):
+    """test_radial_ring.py:72-84: derivatives against a spline of the potential, rtol 1e-9."""
+    rs, zs = 1.0, 0.0
+    r = np.linspace(1.1, 2, 10000)
+    pot = [oracle.lib.potential_radial_ring(r_, zs, rs - r_, 0.0, None) for r_ in r]
+    der = [oracle.lib.dr1_potential_radial_ring(r_, zs, rs - r_, 0.0, None) for r_ in r]
+    assert np.allclose(CubicSpline(r, pot)(r, 1), der, atol=0, rtol=1e-9)
+    z = np.linspace(0.1, 1, 10000)
+    pot = [oracle.lib.potential_radial_ring(rs, z_, 0.0, zs - z_, None) for z_ in z]
+    der = [oracle.lib.dz1_potential_radial_ring(rs, z_, 0.0, zs - z_, None) for z_ in z]
+    assert np.allclose(CubicSpline(z, pot)(z, 1), der, atol=0, rtol=1e-9)
+
+
+def test_ring_gradient_vs_finite_difference(oracle):
+    rng = np.random.default_rng(12)
+    for r0, z0, r, z in rng.uniform(0.05, 1.0, (15, 4)):
+        dz = 1e-5
+        fd = (ring_theta(r0, z0 + dz, r, z) - ring_theta(r0, z0 - dz, r, z)) / (2 * dz)
+        assert np.isclose(oracle.lib.dz1_potential_radial_ring(r0, z0, r - r0, z - z0, None), fd, rtol=1e-6, atol=1e-12)
+        fd = (ring_theta(r0 + dz, z0, r, z) - ring_theta(r0 - dz, z0, r, z)) / (2 * dz)
+        assert np.isclose(oracle.lib.dr1_potential_radial_ring(r0, z0, r - r0, z - z0, None), fd, rtol=1e-6, atol=1e-12)
+
+
+def test_current_ring_vs_biot_savart(oracle):
+    """test_radial_ring.py:136-145, 185-205."""
+    rng = np.random.default_rng(13)
+    for x0, y0, x, y in 10 * rng.random((15, 4)):
+        ours = MU0 * oracle.current_field_radial_ring(x0, y0, x, y)
+        ref = biot_savart_loop(1.0, np.array([x0, 0.0, y0 - y]) / x) / x
+        assert np.isclose(ours[0], ref[0], atol=1e-10) and np.isclose(ours[1], ref[2], atol=1e-10)
+    pts = np.array([[0, 0, 0], [0.5, 0, 0], [0, 0, 0.5], [0.5, 0, 0.5], [0.5, 0, -0.5], [1.0, 0, 0.01], [1.0, 0, -0.01],
+                    [1.001, 0, 0], [1 - 1e-3, 0, 0], [1e10, 0, 0.0]])
+    zr = 3.2
+    ours = np.array([MU0 * oracle.current_field_radial_ring(p[0], p[2] + zr, 1.0, zr) for p in pts])
+    ref = np.array([biot_savart_loop(1.0, p) for p in pts])
+    assert np.allclose(ours[:, 0], ref[:, 0], atol=1e-10) and np.allclose(ours[:, 1], ref[:, 2], atol=1e-10)
+
+
+def test_current_ring_closed_forms(oracle):
+    """test_radial_ring.py:116-183: on-axis field, centre field, in-plane field."""
+    z = np.linspace(-5, 5, 250)
+    for R in (2.0, 55.0):
+        fz = np.array([oracle.current_field_radial_ring(0.0, z_, R, 0.0)[1] for z_ in z])
+        assert np.allclose(fz, R ** 2 / (2 * (z ** 2 + R ** 2) ** 1.5))
+    pot = np.array([oracle.lib.current_potential_axial_radial_ring(z_, 2.0, 0.0) for z_ in z])
+    assert np.allclose(pot, -z / (2 * np.sqrt(z ** 2 + 4.0)))
+    R = np.linspace(0.5, 5)
+    assert np.allclose([oracle.current_field_radial_ring(0.0, 0.0, r, 0.0)[1] for r in R], 1 / (2 * R))
+    # known values from an online loop calculator (test_radial_ring.py:221-243): validates the quadrature used above
+    vals = np.array([[0.0, 0.0, 6.283185307179588e-7], [1.6168908407550761e-7, 0.0, 4.3458489359416384e-7],
+                     [-1.5246460125113486e-7, 0.0, 6.48191970028077e-7], [-0.00019999938843237334, 0.0, 7.987195385643937e-7]])
+    pts = np.array([[0.0, 0, 0.0], [0.5, 0, 0.5], [0.5, 0, -0.25], [1.0, 0, -0.001]])
+    assert np.allclose(vals, [biot_savart_loop(1.0, p) for p in pts])
+    ours = np.array([MU0 * oracle.current_field_radial_ring(p[0], p[2], 1.0, 0.0) for p in pts])
+    assert np.allclose(ours[:, [0, 1]], vals[:, [0, 2]], rtol=1e-7, atol=1e-14)
+
+
+def test_single_element_field_vs_quadrature(oracle):
+    """test_radial.py:65-92: one straight element, field vs quadrature of the finite-difference
+    field of theta-integrated rings, rtol 1e-10."""
+    verts = np.array([[1.0, 0.0, 1.0], [2.0, 0.0, 2.0], [1 + 1 / 3, 0.0, 1 + 1 / 3], [1 + 2 / 3, 0.0, 1 + 2 / 3]])
+    r0, z0, d = 2.0, -2.5, 1e-5
+    Er = lambda r, z: -(ring_theta(r0 + d, z0, r, z) - ring_theta(r0 - d, z0, r, z)) / (2 * d)
+    Ez = lambda r, z: -(ring_theta(r0, z0 + d, r, z) - ring_theta(r0, z0 - d, r, z)) / (2 * d)
+    L = np.sqrt(2)
+    er = quad(lambda x: Er(1 + x, 1 + x), 0, 1)[0] * L
+    ez = quad(lambda x: Ez(1 + x, 1 + x), 0, 1)[0] * L
+    jac, pos = oracle.fill_jacobian_buffer_radial(np.array([verts]))
+    f = oracle.field_radial(np.array([r0, 0.0, z0]), np.ones(1), jac, pos)
+    assert np.allclose(f, [er, 0.0, ez], atol=0.0, rtol=1e-10)
+
+
+def test_coil_field_vs_dblquad(oracle):
+    """test_radial.py:218-240: 1 A rectangular coil [2,3]x[2,3], field at (2.5, 4) to rtol 1e-9."""
+    from traceon_amd.mesh import rectangle_surface_mesh
+    m = rectangle_surface_mesh(2, 3, 2, 3, "coil", 10, 10)
+    jac, pos = oracle.fill_jacobian_buffer_3d(m.points[m.triangles.astype(int)])
+    assert np.isclose(jac.sum(), 1.0)
+    I = np.full(len(jac), 1.0 / jac.sum())
+    target = np.array([2.5, 0.0, 4.0])
+    ours = MU0 * oracle.current_field_radial(target, I, jac, pos)
+    cr = dblquad(lambda x, y: MU0 * oracle.current_field_radial_ring(target[0], target[2], x, y)[0], 2, 3, 2, 3, epsrel=1e-4)[0]
+    cz = dblquad(lambda x, y: MU0 * oracle.current_field_radial_ring(target[0], target[2], x, y)[1], 2, 3, 2, 3, epsrel=1e-4)[0]
+    assert np.allclose(ours, [cr, 0.0, cz], atol=0.0, rtol=1e-9)
+
+
+def test_coil_axial_field_fixture(oracle):
+    """test_radial.py:94-114 against the reference's fixture 'axial magnetic field.txt'
+    (copied data: z and B_z in mT for a 5 A coil on [1,2]x[1,2]), rtol 2e-2."""
+    import os
+    from traceon_amd.mesh import rectangle_surface_mesh
+    data = np.loadtxt(os.path.join(os.path.dirname(__file__), "golden", "axial_magnetic_field.txt"), delimiter=",")
+    m = rectangle_surface_mesh(1, 2, 1, 2, "coil", 10, 10)
+    jac, pos = oracle.fill_jacobian_buffer_3d(m.points[m.triangles.astype(int)])
+    I = np.full(len(jac), 5.0 / jac.sum())
+    z = np.linspace(-0.5, 3.5, 300)
+    Hz = np.array([oracle.current_field_radial(np.array([0.0, 0.0, z_]), I, jac, pos)[2] for z_ in z])
+    ref = CubicSpline(data[:, 0], data[:, 1] * 1e-3)(z)
+    assert np.allclose(ref, Hz, atol=0.0, rtol=2e-2)
+
+
+def test_axial_derivative_recursion(oracle):
+    """The on-axis recursion against derivatives of the closed-form on-axis potential
+    r/2 / sqrt((z0-z)^2 + r^2) (radial_ring.c:109-122) by high-order finite differences."""
+    import mpmath as mp
+    r, zs = 0.7, 0.3
+    for z0 in (-1.0, 0.1, 0.8):
+        d = np.zeros(9)
+        oracle.lib.axial_derivatives_radial_ring(z0, r, zs, d)
+        f = lambda x: 0.5 * r / mp.sqrt((x - zs) ** 2 + r ** 2)
+        exact = [float(mp.diff(f, z0, n)) for n in range(9)]
+        assert np.allclose(d, exact, rtol=1e-11, atol=0)
+        c = np.zeros(9)
+        oracle.lib.current_axial_derivatives_radial_ring(z0, r, zs, c)
+        g = lambda x: -(x - zs) / (2 * mp.sqrt((x - zs) ** 2 + r ** 2))
+        exact = [float(mp.diff(g, z0, n)) for n in range(9)]
+        assert np.allclose(c, exact, rtol=1e-11, atol=1e-300)
+
+
+# ---------------------------------------------------------------- tracing (test_tracing.py)
+def _uniform(oracle, E, H):
+    import ctypes as C
+    from oracle.oracle import field_fun
+    buf = (C.c_double * 6)(*E, *H)
+    addr = C.cast(oracle.lib.field_uniform_traceable, C.c_void_p).value
+    return addr, buf
+
+
+def _trace_uniform(oracle, p0, v0, qm, E, H, bounds, atol):
+    import ctypes as C
+    addr, buf = _uniform(oracle, E, H)
+
+    class Wrap(C.Structure):
+        _fields_ = [("d", C.c_double * 6)]
+    w = Wrap()
+    w.d = buf
+    return oracle.trace(p0, v0, qm, addr, w, bounds, atol)
+
+
+def test_trace_constant_acceleration(oracle):
+    """test_tracing.py:26-41: x = 3/2 t^2 under a constant field."""
+    t, s = _trace_uniform(oracle, np.zeros(3), np.array([0.0, 0.0, 3.0]), EM, [3.0 / EM, 0, 0], [0, 0, 0],
+                          [[-2, 2], [-2, 2], [-2, np.sqrt(12) + 1]], 1e-10)
+    assert np.allclose(1.5 * t ** 2, s[:, 0]) and np.allclose(3 * t, s[:, 2])
+
+
+def test_trace_helix_vs_dop853(oracle):
+    """test_tracing.py:74-93: uniform B, against scipy's DOP853 at 1e-10."""
+    p0, v0 = np.zeros(3), np.array([0.0, 1.0, -1.0])
+    t, s = _trace_uniform(oracle, p0, v0, EM, [0, 0, 0], np.array([0, 0, 1]) / (MU0 * EM),
+                          [[-5, 5], [-5, 5], [-40, 10]], 1e-10)
+    sol = solve_ivp(lambda _, y: np.hstack((y[3:], np.cross(y[3:], [0, 0, 1]))), (0, 30), np.hstack((p0, v0)), method="DOP853",
+                    rtol=1e-10, atol=1e-10)
+    interp = CubicSpline(s[::-1, 2], np.array([s[::-1, 0], s[::-1, 1]]).T)
+    assert np.allclose(interp(sol.y[2]), np.array([sol.y[0], sol.y[1]]).T)
+
+
+def test_trace_cyclotron_radius(oracle):
+    """test_tracing.py:269-285."""
+    t, s = _trace_uniform(oracle, np.array([1.0, 0, 0]), np.array([0.0, 1.0, 0]), EM, [0, 0, 0],
+                          np.array([0, 0, -1.0]) / (EM * MU0), [[-2, 2], [0, 2], [-2, 2]], 1e-10)
+    interp = CubicSpline(s[-10:, 1][::-1], s[-10:][::-1])
+    assert np.allclose(interp(0.0), [-1.0, 0, 0, 0, -1, 0])
+
+
+def test_plane_intersection_hand_cases(oracle):
+    """test_tracing.py:196-267."""
+    pi = oracle.plane_intersection
+    p = np.array([[3, 0, 0, 0, 0, 0], [1, 0, 0, 0, 0, 0], [-1, 0, 0, 0, 0, 0.0]])
+    assert np.allclose(pi(p, np.zeros(3), [4.0, 0, 0]), 0.0)
+    p = np.array([[2, 2, 2, 0, 0, 0], [1, 1, 1, 0, 0, 0], [-1, -1, -1, 0, 0, 0.0]])
+    assert np.allclose(pi(p, np.zeros(3), [2.0, 2, 2]), 0.0)
+    assert np.allclose(pi(p, np.zeros(3), [-1.0, -1, -1]), 0.0)
+    p = np.array([[2, 2, 2, 0, 0, 0], [2, 2, 1, 0, 1, 0], [2, 2, -3, 0, 0, 0.0]])
+    assert np.allclose(pi(p, [2.0, 2, 1], [0, 0, 1.0]), [2, 2, 1, 0, 1, 0])
+    assert np.allclose(pi(p, [2.0, 2, 1], [0, 0, -1.0]), [2, 2, 1, 0, 1, 0])
+    p = np.array([[0.0, 0, -3, 0, 0, 0], [0.0, 0, 9, 0, 0, 0]])
+    assert np.allclose(pi(p, [0.0, 1, 0], [1.0, 1, 1]), [0, 0, 1, 0, 0, 0])
+    assert np.allclose(pi(p, [0.0, 1, 0], [-1.0, -1, -1]), [0, 0, 1, 0, 0, 0])
+    for states, z, col, want in (([[1.0, 0, 0, 0, 0, 0], [1.0, 0, 1, 0, 0, 0]], 0.5, 0, 1.0),
+                                 ([[1.0, 0, 0, 0, 0, 0], [1.0, 0, -1, 0, 0, 0]], -0.5, 0, 1.0),
+                                 ([[0.0, 0, 0, 0, 0, 0], [1.0, 0, -1, 0, 0, 0], [2.0, 0, -2, 0, 0, 0]], -1.5, 0, 1.5),
+                                 ([[0.0, 0, 0, 0, 0, 0], [1.0, 0, 1, 0, 0, 0], [4.0, 0, 2, 0, 0, 0]], 1.75, 0, 3.25),
+                                 ([[0.0, 0, 0, 0, 0, 0], [0.0, 1, 1, 0, 0, 0], [0.0, 4, 2, 0, 0, 0]], 1.75, 1, 3.25)):
+        assert np.isclose(pi(np.array(states), [0, 0, z], [0, 0, 1.0])[col], want)
+    with pytest.raises(ValueError):
+        pi(np.array([[0.0, 0, 1, 0, 0, 0], [0.0, 0, 2, 0, 0, 0]]), [0, 0, 0], [0, 0, 1.0])
+
+
+# ---------------------------------------------------------------- restatement vs the reference's own C
+def _small_problem():
+    from traceon_amd import configs
+    mesh, volts, diel = configs.small_geometry(180)
+    return configs.lines_types_values(mesh, volts, diel)
+
+
+def test_against_reference_scalar_kernels(oracle, ref_lib):
+    rng = np.random.default_rng(21)
+    for name in ("ellipkm1", "ellipem1"):
+        x = rng.random(500)
+        a = np.array([getattr(oracle.lib, name)(v) for v in x])
+        b = np.array([getattr(ref_lib.lib, name)(v) for v in x])
+        assert np.max(np.abs(a - b) / np.abs(b)) < 1e-14
+    X = rng.random((500, 4)) + [0.01, 0, 0, 0]
+    for name in ("potential_radial_ring", "dz1_potential_radial_ring"):
+        a = np.array([getattr(oracle.lib, name)(*v, None) for v in X])
+        b = np.array([getattr(ref_lib.lib, name)(*v, None) for v in X])
+        assert np.max(np.abs(a - b) / np.abs(b)) < 1e-13, name
+    # d/dr0 cancels in the far field (both sides lose digits there): compare to the size of its terms
+    a = np.array([oracle.lib.dr1_potential_radial_ring(*v, None) for v in X])
+    b = np.array([ref_lib.lib.dr1_potential_radial_ring(*v, None) for v in X])
+    scale = np.array([abs(oracle.lib.dz1_potential_radial_ring(v[0], v[1], v[2], 1.0, None)) + abs(bb) for v, bb in zip(X, b)])
+    assert np.max(np.abs(a - b) / scale) < 1e-11
+    for x0, y0, x, y in 5 * rng.random((100, 4)):
+        assert np.allclose(oracle.current_field_radial_ring(x0, y0, x, y), ref_lib.current_field_radial_ring(x0, y0, x, y),
+                           rtol=1e-12, atol=1e-15)
+
+
+def test_against_reference_fill_and_diagonal(oracle, ref_lib):
+    lines, types, values = _small_problem()
+    Mo = oracle.fill_matrix(lines, types, values)
+    Mr = ref_lib.fill_matrix(lines, types, values)
+    off = ~np.eye(len(lines), dtype=bool)   # the 16-point diagonal is overwritten by the singular integral
+    scale = np.max(np.abs(Mr), axis=1, keepdims=True)
+    assert np.max((np.abs(Mo - Mr) / scale)[off]) < 1e-12
+    for i in (0, 17, 60, 121, 170):
+        if types[i] in (3, 6):
+            a, b = oracle.self_field_dot_normal_quad(lines[i], values[i]), ref_lib.self_field_dot_normal_quad(lines[i], values[i])
+        else:
+            a, b = oracle.self_potential_quad(lines[i]), ref_lib.self_potential_quad(lines[i])
+        assert np.isclose(a, b, rtol=1e-11, atol=1e-13)   # straight flux elements integrate to ~0
+
+
+def test_against_reference_fields_and_axial(oracle, ref_lib):
+    lines, types, values = _small_problem()
+    jac, pos = oracle.fill_jacobian_buffer_radial(lines)
+    jr, pr = ref_lib.fill_jacobian_buffer_radial(lines)
+    assert np.allclose(jac, jr, rtol=1e-14) and np.allclose(pos, pr, rtol=1e-14)
+    rng = np.random.default_rng(22)
+    q = rng.standard_normal(len(lines))
+    for p in np.column_stack([rng.uniform(0, 0.9e-3, 20), np.zeros(20), rng.uniform(-2e-3, 2e-3, 20)]):
+        assert np.isclose(oracle.potential_radial(p, q, jac, pos), ref_lib.potential_radial(p, q, jac, pos), rtol=1e-12)
+        assert np.allclose(oracle.field_radial(p, q, jac, pos), ref_lib.field_radial(p, q, jac, pos), rtol=1e-10, atol=1e-6)
+    z = np.linspace(-2e-3, 2e-3, 40)
+    a, b = oracle.axial_derivatives_radial(z, q, jac, pos), ref_lib.axial_derivatives_radial(z, q, jac, pos)
+    assert np.allclose(a, b, rtol=1e-11, atol=0)
+    from oracle.oracle import quintic_spline_coefficients
+    C = quintic_spline_coefficients(z, a.T)
+    for p in np.column_stack([rng.uniform(0, 0.2e-3, 20), np.zeros(20), rng.uniform(-1.9e-3, 1.9e-3, 20)]):
+        assert np.isclose(oracle.potential_radial_derivs(p, z, C), ref_lib.potential_radial_derivs(p, z, C), rtol=1e-12)
+        assert np.allclose(oracle.field_radial_derivs(p, z, C), ref_lib.field_radial_derivs(p, z, C), rtol=1e-12, atol=1e-9)
+
+
+def test_against_reference_trace(oracle, ref_lib):
+    lines, types, values = _small_problem()
+    jac, pos = oracle.fill_jacobian_buffer_radial(lines)
+    M = oracle.matrix(lines, types, values)
+    F = np.where(np.isin(types, [1, 2, 5]), values, 0.0)
+    q = np.linalg.solve(M, F)
+    v = np.sqrt(2 * 50 * E / ME)
+    bounds = np.array([[-0.9e-3, 0.9e-3], [-0.9e-3, 0.9e-3], [-2.5e-3, 2.5e-3]])
+    p0, v0 = np.array([0.2e-3, 0.0, -2.4e-3]), np.array([0.0, 0.0, v])
+    t1, s1 = oracle.trace_bem(p0, v0, EM, (q, jac, pos), bounds=bounds)
+    t2, s2 = ref_lib.trace_bem(p0, v0, EM, (q, jac, pos), bounds=bounds)
+    assert len(s1) == len(s2)
+    assert np.max(np.abs(s1[:, :3] - s2[:, :3])) < 1e-12
+    l1 = oracle.plane_intersection(s1, [0, 0, 2e-3], [0, 0, 1.0])
+    l2 = ref_lib.plane_intersection(s2, [0, 0, 2e-3], [0, 0, 1.0])
+    assert np.allclose(l1[:3], l2[:3], rtol=0, atol=1e-12)
+
+
+def test_spline_coefficients_reproduce_samples():
+    """Interpolation identities of the oracle's coefficients: the quintic pieces match value,
+    first and second derivative samples at both knots; the cubic ones pass through the samples."""
+    from oracle.oracle import quintic_spline_coefficients
+    z = np.linspace(0, 1, 30)
+    rng = np.random.default_rng(5)
+    d = rng.standard_normal((9, 30))
+    C = quintic_spline_coefficients(z, d)
+    h = z[1] - z[0]
+    for l in range(7):
+        c = C[:, l, :]
+        p = np.poly1d
+        for i in range(29):
+            poly = p(c[i])
+            assert np.isclose(poly(0), d[l, i]) and np.isclose(poly(h), d[l, i + 1])
+            assert np.isclose(poly.deriv()(0), d[l + 1, i]) and np.isclose(poly.deriv()(h), d[l + 1, i + 1])
+            assert np.isclose(poly.deriv(2)(0), d[l + 2, i]) and np.isclose(poly.deriv(2)(h), d[l + 2, i + 1])
+    for l in (7, 8):
+        for i in range(29):
+            poly = np.poly1d(C[i, l, 2:])
+            assert np.isclose(poly(0), d[l, i]) and np.isclose(poly(h), d[l, i + 1])

@@ -87,6 +87,7 @@ _SIG = {
     "tb_last_error": (C.c_char_p, []),
     "tb_clear_error": (None, []),
     "tb_version": (i32, []),
+    "tb_struct_sizes": (None, [arr(dtype=np.uint64, shape=(4,))]),
     "tb_synchronize": (i32, [vp]),
     "tb_rowinfo_bytes": (sz, []),
     "tb_prepare_elements": (i32, [vp, vp, vp, i32, vp, vp, vp, vp, vp, vp]),
@@ -111,7 +112,7 @@ _SIG = {
     "tb_axial_points": (i32, [vp, i32, vp, i32, vp, vp, vp, vp]),
     "tb_jacobian_3d": (i32, [vp, i32, vp, vp, vp]),
     "tb_charge_radial": (i32, [vp, vp, i32, vp, vp]),
-    "tb_trace": (i32, [vp, i32, vp]),
+    "tb_trace": (i32, [vp, i32, i32, vp]),
     "tb_eval_program": (i32, [vp, vp, i32, vp, vp, vp]),
 }
 

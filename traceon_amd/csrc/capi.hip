@@ -118,6 +118,12 @@ extern "C" {
 const char* tb_last_error(void) { return g_err.c_str(); }
 void tb_clear_error(void) { g_err.clear(); }
 int tb_version(void) { return 1; }
+void tb_struct_sizes(size_t* out4) {
+  out4[0] = sizeof(tb_field_comp_t);
+  out4[1] = sizeof(tb_field_program_t);
+  out4[2] = sizeof(tb_trace_params_t);
+  out4[3] = sizeof(tb::RowInfo);
+}
 int tb_synchronize(void* stream) { return ok(hipStreamSynchronize(S(stream)), "synchronize") ? 0 : -1; }
 
 // ====================================================================================
@@ -394,7 +400,7 @@ size_t trace_particle(double* times, double* states, double qm, field_fun field,
   P.plane_n[2] = 1.0; P.plane_nn = 1.0;
   P.final_state = fin.p; P.landing = nullptr; P.status = status.p; P.n_accepted = nacc.p; P.n_attempted = natt.p;
   P.t_final = tf.p; P.h_final = hf.p; P.traj = traj.p; P.times = tt.p; P.traj_cap = cap; P.traj_len = tlen.p;
-  tb::launch_trace(P, 1, 0);
+  tb::launch_trace(P, 1, 1, 0);
   long long n = 0;
   if (!finish("trace_particle") || !tlen.get(&n, 1)) return 0;
   // row 0 is the caller's initial state; rows 1..n-1 the accepted steps
@@ -498,7 +504,7 @@ int tb_jacobian_3d(const double* tri, int N, double* jac, double* pos, void* str
 int tb_charge_radial(const double* lines, const double* q, int N, double* out, void* stream) {
   TB_CALL(tb::launch_charge(lines, q, N, out, S(stream)), "charge_radial");
 }
-int tb_trace(const tb_trace_params_t* params, int waves, void* stream) {
+int tb_trace(const tb_trace_params_t* params, int waves, int wave_per_particle, void* stream) {
   if (!params) { set_err("tb_trace: null params"); return -1; }
   int nw = waves;
   if (nw <= 0) {
@@ -508,7 +514,7 @@ int tb_trace(const tb_trace_params_t* params, int waves, void* stream) {
     if (hipGetDeviceProperties(&prop, dev) != hipSuccess) { set_err("device properties"); return -1; }
     nw = prop.multiProcessorCount * 8;
   }
-  TB_CALL(tb::launch_trace(*reinterpret_cast<const tb::TraceParams*>(params), nw, S(stream)), "trace");
+  TB_CALL(tb::launch_trace(*reinterpret_cast<const tb::TraceParams*>(params), nw, wave_per_particle, S(stream)), "trace");
 }
 int tb_eval_program(const tb_field_program_t* program, const double* pts, int M, double* E, double* H, void* stream) {
   TB_CALL(tb::launch_eval_program(*reinterpret_cast<const tb::FieldProgram*>(program), pts, M, E, H, S(stream)), "eval_program");

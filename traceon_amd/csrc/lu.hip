@@ -194,6 +194,103 @@ __global__ __launch_bounds__(NB) void k_trinv(const T* __restrict__ A, size_t ld
 }
 
 // ---------------------------------------------------------------------------------
+// No-pivot fast path of one panel.  Partial pivoting leaves every row in place exactly
+// when each multiplier satisfies |l| <= 1 (the diagonal entry is then the largest of its
+// column, ties going to the diagonal as in LAPACK).  So the panel is first factored
+// without interchanges; if every computed |l| <= 1 the result IS the partially pivoted
+// factorisation (ipiv = identity) and no grid-wide pivot search is needed.  Otherwise the
+// panel is restored from its copy and factored by k_panel.  Pure potential (Dirichlet)
+// BEM rows never need a swap; rows of flux type (dielectric / magnetizable) usually do.
+// ---------------------------------------------------------------------------------
+
+// (a) unpivoted LU of the nb x nb top block in LDS (one workgroup); max |l| -> lmax bits
+template <typename T, int NB>
+__global__ __launch_bounds__(256) void k_getf2_nopiv(T* __restrict__ A, size_t lda, int k0, int nb, int* __restrict__ ipiv,
+                                                     unsigned long long* __restrict__ lmax) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* B = reinterpret_cast<T*>(smem_raw);   // nb x NB
+  const int tid = threadIdx.x;
+  for (int e = tid; e < nb * NB; e += 256) {
+    int r = e / NB, c = e % NB;
+    B[e] = c < nb ? A[(size_t)(k0 + r) * lda + k0 + c] : T(0);
+  }
+  __syncthreads();
+  double m = 0.0;
+  for (int j = 0; j < nb; j++) {
+    const T piv = B[j * NB + j];
+    for (int r = j + 1 + tid; r < nb; r += 256) {
+      T l = B[r * NB + j] / piv;
+      B[r * NB + j] = l;
+      double al = (double)tabs(l);
+      m = (al > m || al != al) ? al : m;
+    }
+    __syncthreads();
+    const int n = nb - j - 1;
+    for (int e = tid; e < n * n; e += 256) {
+      int r = j + 1 + e / n, c = j + 1 + e % n;
+      B[r * NB + c] -= B[r * NB + j] * B[j * NB + c];
+    }
+    __syncthreads();
+    if (tid == 0) ipiv[k0 + j] = k0 + j;
+  }
+  for (int e = tid; e < nb * nb; e += 256) {
+    int r = e / nb, c = e % nb;
+    A[(size_t)(k0 + r) * lda + k0 + c] = B[r * NB + c];
+  }
+  if (m != m) m = 1e300;
+  atomicMax(lmax, (unsigned long long)__double_as_longlong(m));
+}
+
+// (b) inverse of the upper-triangular U11 (non-unit diagonal): Z = inv(U11), row-major NB x NB.
+// Column c of Z solves U z = e_c from the bottom up; thread c owns column c.
+template <typename T, int NB>
+__global__ __launch_bounds__(NB) void k_triu_inv(const T* __restrict__ A, size_t lda, int k0, int nb, T* __restrict__ Z) {
+  const int c = threadIdx.x;
+  for (int r = NB - 1; r >= 0; r--) {
+    T acc = T(0);
+    if (r < nb && c < nb && r <= c) {
+      const T* Ur = A + (size_t)(k0 + r) * lda + k0;
+      acc = (r == c) ? T(1) : T(0);
+      for (int s = r + 1; s <= c; s++) acc -= Ur[s] * Z[(size_t)s * NB + c];
+      acc /= Ur[r];
+    }
+    Z[(size_t)r * NB + c] = acc;
+  }
+}
+
+// After the factorisation: info = 1 + first column whose U diagonal is zero or NaN
+// (numpy.linalg.solve reports such a matrix as singular), else unchanged.
+template <typename T>
+__global__ __launch_bounds__(1024) void k_diag_check(const T* __restrict__ A, size_t lda, int N, int* __restrict__ info) {
+  __shared__ int first;
+  if (threadIdx.x == 0) first = 0x7fffffff;
+  __syncthreads();
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    T d = A[(size_t)i * lda + i];
+    if (!(tabs(d) > T(0))) atomicMin(&first, i);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && first != 0x7fffffff && (*info == 0 || *info > first + 1)) *info = first + 1;
+}
+
+// copy a rows x cols block back into the matrix and track max |value| (bits, nonnegative)
+template <typename T>
+__global__ __launch_bounds__(256) void k_copy_back_max(const T* __restrict__ S, size_t lds, T* __restrict__ D, size_t ldd, int rows,
+                                                       int cols, unsigned long long* __restrict__ vmax) {
+  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double m = 0.0;
+  if (e < (size_t)rows * cols) {
+    int r = (int)(e / cols), c = (int)(e % cols);
+    T v = S[(size_t)r * lds + c];
+    D[(size_t)r * ldd + c] = v;
+    m = (double)tabs(v);
+    if (m != m) m = 1e300;
+  }
+  for (int o = 32; o >= 1; o >>= 1) { double x = __shfl_xor(m, o, 64); m = x > m ? x : m; }
+  if ((threadIdx.x & 63) == 0 && m > 0.0) atomicMax(vmax, (unsigned long long)__double_as_longlong(m));
+}
+
+// ---------------------------------------------------------------------------------
 // GEMM on MFMA: C(MxN) = beta*C + alpha*A(MxK)*B(KxN), all row-major.
 // Block 256 threads = 2x2 waves, block tile 128x128, wave tile 64x64 made of 4x4
 // MFMA 16x16 tiles; K staged through LDS in chunks of BK, A stored k-major in LDS so
@@ -444,11 +541,14 @@ template hipError_t gemm<float>(int, int, int, float, const float*, size_t, cons
 
 template <typename T>
 size_t lu_workspace_bytes(int N) {
-  // candidates (2G doubles + 2G ints + 2G*NB T + 2NB T), trinv (NB*NB T), U12 staging (NB*N T), info
+  // candidates (2G doubles + 2G ints + 2G*NB T + 2NB T), two NB x NB inverses,
+  // U12/L21 staging (NB*N T), panel backup (NB*N T), flags
   const int G = 1024, NB = LU_NB;
   size_t b = 2 * G * sizeof(double) + 2 * G * sizeof(int) + (2 * (size_t)G * NB + 2 * NB) * sizeof(T);
   b = (b + 255) & ~(size_t)255;
-  b += (size_t)NB * NB * sizeof(T);
+  b += 2 * (size_t)NB * NB * sizeof(T);
+  b = (b + 255) & ~(size_t)255;
+  b += (size_t)NB * (size_t)N * sizeof(T);
   b = (b + 255) & ~(size_t)255;
   b += (size_t)NB * (size_t)N * sizeof(T);
   b = (b + 255) & ~(size_t)255;
@@ -470,17 +570,18 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
   int G = panel_groups > 0 ? panel_groups : prop.multiProcessorCount;
   if (G > 1024) G = 1024;
   const size_t max_lds = 160 * 1024 - (2 * NB * sizeof(T) + 256 * (sizeof(double) + 2 * sizeof(int)) + 64);
-  // rows per group must fit in LDS
   size_t Rmax = ((size_t)N + G - 1) / G;
   if (Rmax * NB * sizeof(T) > max_lds) return hipErrorInvalidValue;
   auto kpanel = k_panel<T, NB>;
-  size_t dyn = Rmax * NB * sizeof(T);
-  e = hipFuncSetAttribute((const void*)kpanel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+  e = hipFuncSetAttribute((const void*)kpanel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(Rmax * NB * sizeof(T)));
   if (e != hipSuccess) return e;
   int occ = 0;
-  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kpanel, 256, dyn);
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kpanel, 256, Rmax * NB * sizeof(T));
   if (e != hipSuccess) return e;
   if (occ * prop.multiProcessorCount < G) return hipErrorCooperativeLaunchTooLarge;
+  auto kgetf2 = k_getf2_nopiv<T, NB>;
+  e = hipFuncSetAttribute((const void*)kgetf2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(NB * NB * sizeof(T)));
+  if (e != hipSuccess) return e;
 
   unsigned char* w = (unsigned char*)ws;
   double* wsd = (double*)w;                       w += 2 * 1024 * sizeof(double);
@@ -488,12 +589,43 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
   T* wsr = (T*)w;                                 w += (2 * (size_t)1024 * NB + 2 * NB) * sizeof(T);
   w = (unsigned char*)(((uintptr_t)w + 255) & ~(uintptr_t)255);
   T* Y = (T*)w;                                   w += (size_t)NB * NB * sizeof(T);
+  T* Z = (T*)w;                                   w += (size_t)NB * NB * sizeof(T);
   w = (unsigned char*)(((uintptr_t)w + 255) & ~(uintptr_t)255);
-  T* U12 = (T*)w;
+  T* U12 = (T*)w;                                 w += (size_t)NB * (size_t)N * sizeof(T);
+  w = (unsigned char*)(((uintptr_t)w + 255) & ~(uintptr_t)255);
+  T* Pbak = (T*)w;                                w += (size_t)NB * (size_t)N * sizeof(T);
+  w = (unsigned char*)(((uintptr_t)w + 255) & ~(uintptr_t)255);
+  unsigned long long* lmax = (unsigned long long*)w;
 
+  const unsigned long long ONE_BITS = 0x3FF0000000000000ull;   // bit pattern of 1.0
   for (int k0 = 0; k0 < N; k0 += NB) {
     const int nb = min(NB, N - k0);
-    {
+    const int m2 = N - k0 - nb;   // rows below the diagonal block
+    bool pivoted = true;
+    // ---- fast path: unpivoted panel, accepted when every |l| <= 1 ----
+    hipLaunchKernelGGL(k_copy2d<T>, dim3(cdiv((size_t)(N - k0) * nb, 256)), dim3(256), 0, st, A + (size_t)k0 * lda + k0, lda, Pbak,
+                       (size_t)nb, N - k0, nb);
+    hipMemsetAsync(lmax, 0, sizeof(unsigned long long), st);
+    hipLaunchKernelGGL(kgetf2, dim3(1), dim3(256), NB * NB * sizeof(T), st, A, lda, k0, nb, ipiv, lmax);
+    if (m2 > 0) {
+      hipLaunchKernelGGL((k_triu_inv<T, NB>), dim3(1), dim3(NB), 0, st, A, lda, k0, nb, Z);
+      T* A21 = A + (size_t)(k0 + nb) * lda + k0;
+      e = gemm<T>(m2, nb, nb, T(1), A21, lda, Z, NB, T(0), U12, (size_t)nb, st);
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL(k_copy_back_max<T>, dim3(cdiv((size_t)m2 * nb, 256)), dim3(256), 0, st, U12, (size_t)nb, A21, lda, m2, nb,
+                         lmax);
+    }
+    unsigned long long lm = 0;
+    e = hipMemcpyAsync(&lm, lmax, sizeof(lm), hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return e;
+    e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    if (lm <= ONE_BITS) {
+      pivoted = false;
+    } else {
+      // ---- slow path: restore the panel and factor it with partial pivoting ----
+      hipLaunchKernelGGL(k_copy2d<T>, dim3(cdiv((size_t)(N - k0) * nb, 256)), dim3(256), 0, st, Pbak, (size_t)nb,
+                         A + (size_t)k0 * lda + k0, lda, N - k0, nb);
       T* Ap = A; size_t ldap = lda; int Np = N, k0p = k0;
       int* ipivp = ipiv; double* wsdp = wsd; int* wsip = wsi; T* wsrp = wsr; int* infop = info_dev;
       void* args[] = {&Ap, &ldap, &Np, &k0p, &ipivp, &wsdp, &wsip, &wsrp, &infop};
@@ -502,7 +634,7 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
       e = hipLaunchCooperativeKernel((const void*)kpanel, dim3(G), dim3(256), args, (unsigned)(R * NB * sizeof(T)), st);
       if (e != hipSuccess) return e;
     }
-    if (N - nb > 0) hipLaunchKernelGGL(k_laswp<T>, dim3(cdiv(N - nb, 256)), dim3(256), 0, st, A, lda, N, k0, nb, ipiv);
+    if (pivoted && N - nb > 0) hipLaunchKernelGGL(k_laswp<T>, dim3(cdiv(N - nb, 256)), dim3(256), 0, st, A, lda, N, k0, nb, ipiv);
     const int n2 = N - k0 - nb;
     if (n2 > 0) {
       hipLaunchKernelGGL((k_trinv<T, NB>), dim3(1), dim3(NB), 0, st, A, lda, k0, nb, Y);
@@ -518,6 +650,7 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
       if (e != hipSuccess) return e;
     }
   }
+  hipLaunchKernelGGL(k_diag_check<T>, dim3(1), dim3(1024), 0, st, A, lda, N, info_dev);
   return hipGetLastError();
 }
 template hipError_t lu_factor<double>(double*, size_t, int, int*, void*, int*, hipStream_t, int);

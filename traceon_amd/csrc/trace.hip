@@ -32,10 +32,10 @@ __constant__ static const double RK_CT[6] = {-1. / 150., 0., 3. / 100., -16. / 7
 // ---------------------------------------------------------------------------------
 // field evaluation
 // ---------------------------------------------------------------------------------
-__device__ __forceinline__ void bem_field(const SourceSet& S, double x, double y, double z, double out[3]) {
+__device__ __forceinline__ void bem_field(const SourceSet& S, double x, double y, double z, double out[3], int s0, int ds) {
   const double r0 = sqrt(x * x + y * y);
   double Er = 0.0, Ez = 0.0;
-  for (int s = 0; s < S.n; s++) {
+  for (int s = s0; s < S.n; s += ds) {
     double gr, gz;
     ring_grad(r0, S.r[s] - r0, S.z[s] - z, gr, gz);
     Er = fma(-S.w[s], gr, Er);
@@ -46,10 +46,10 @@ __device__ __forceinline__ void bem_field(const SourceSet& S, double x, double y
   out[2] = Ez;
 }
 
-__device__ __forceinline__ void coil_field(const SourceSet& S, double x, double y, double z, double out[3]) {
+__device__ __forceinline__ void coil_field(const SourceSet& S, double x, double y, double z, double out[3], int s0, int ds) {
   const double r = sqrt(x * x + y * y);
   double Hr = 0.0, Hz = 0.0;
-  for (int s = 0; s < S.n; s++) {
+  for (int s = s0; s < S.n; s += ds) {
     double hr, hz;
     current_ring_field(r, z, S.r[s], S.z[s], hr, hz);
     Hr = fma(S.w[s], hr, Hr);
@@ -77,7 +77,12 @@ __device__ __forceinline__ bool in_bounds(const double* b, int mode, const doubl
   return b[0] <= q[0] && q[0] <= b[1] && b[2] <= q[1] && q[1] <= b[3] && b[4] <= q[2] && q[2] <= b[5];
 }
 
-__device__ void eval_field(const FieldProgram& F, const double p[3], double E[3], double H[3]) {
+// Field of a program at p.  Ring-source sums run over sources s0, s0+ds, ...; terms without
+// a source sum (axial, uniform) are added only when `lead` is set, so that partial results
+// of the lanes of a wave add up to the field (wave-per-particle mode) and (0, 1, true)
+// gives the whole field in one lane.
+__device__ void eval_field(const FieldProgram& F, const double p[3], double E[3], double H[3], int s0 = 0, int ds = 1,
+                           bool lead = true) {
   E[0] = E[1] = E[2] = 0.0;
   H[0] = H[1] = H[2] = 0.0;
   if (!in_bounds(F.bounds, F.bounds_mode, p)) return;
@@ -88,8 +93,8 @@ __device__ void eval_field(const FieldProgram& F, const double p[3], double E[3]
     if (!in_bounds(c.bounds, c.bounds_mode, q)) continue;
     double e[3] = {0, 0, 0}, h[3] = {0, 0, 0};
     if (c.kind == COMP_BEM) {
-      if (c.elec.n) bem_field(c.elec, q[0], q[1], q[2], e);
-      if (c.mag.n) bem_field(c.mag, q[0], q[1], q[2], h);
+      if (c.elec.n) bem_field(c.elec, q[0], q[1], q[2], e, s0, ds);
+      if (c.mag.n) bem_field(c.mag, q[0], q[1], q[2], h, s0, ds);
       if (c.cur.n) {
         double hc[3];
         if (c.transform) {
@@ -98,17 +103,19 @@ __device__ void eval_field(const FieldProgram& F, const double p[3], double E[3]
           // it once more, checks the bounds there and rotates by the basis.
           double q2[3];
           to_local(c, q, q2);
-          if (in_bounds(c.bounds, c.bounds_mode, q2)) { coil_field(c.cur, q2[0], q2[1], q2[2], hc); rotate(c, hc); }
+          if (in_bounds(c.bounds, c.bounds_mode, q2)) { coil_field(c.cur, q2[0], q2[1], q2[2], hc, s0, ds); rotate(c, hc); }
           else { hc[0] = hc[1] = hc[2] = 0.0; }
         } else {
-          coil_field(c.cur, q[0], q[1], q[2], hc);
+          coil_field(c.cur, q[0], q[1], q[2], hc, s0, ds);
         }
         h[0] += hc[0]; h[1] += hc[1]; h[2] += hc[2];
       }
     } else if (c.kind == COMP_AXIAL) {
-      axial_field(c.zs, c.nz, c.ce, q[0], q[1], q[2], e);
-      axial_field(c.zs, c.nz, c.cm, q[0], q[1], q[2], h);
-    } else {
+      if (lead) {
+        axial_field(c.zs, c.nz, c.ce, q[0], q[1], q[2], e);
+        axial_field(c.zs, c.nz, c.cm, q[0], q[1], q[2], h);
+      }
+    } else if (lead) {
       for (int i = 0; i < 3; i++) { e[i] = c.uE[i]; h[i] = c.uH[i]; }
     }
     if (c.transform) { rotate(c, e); rotate(c, h); }
@@ -279,6 +286,114 @@ __global__ __launch_bounds__(64) void k_trace(TraceParams P) {
   }
 }
 
+// ---------------------------------------------------------------------------------
+// Wave-per-particle variant for expensive fields (direct BEM sums over 10^5 ring sources):
+// the 64 lanes of a wave split the source sum of every field evaluation and combine it
+// with shuffles; the RK arithmetic is repeated by every lane on identical data, so the
+// step sequence is the same as in the lane-per-particle kernel.  A wave takes the next
+// particle from the counter when its particle retires.
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum64(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return __shfl(v, 0, 64);   // every lane continues with lane 0's bits
+}
+
+__global__ __launch_bounds__(64) void k_trace_wave(TraceParams P) {
+  const int lane = threadIdx.x & 63;
+  while (true) {
+    int id = 0;
+    if (lane == 0) id = atomicAdd(P.counter, 1);
+    id = __shfl(id, 0, 64);
+    if (id >= P.n) break;
+    double y[6];
+    for (int c = 0; c < 6; c++) y[c] = P.state0[6 * (size_t)id + c];
+    double t = P.t0 ? P.t0[id] : 0.0;
+    const double qm = P.qm[P.qm_stride ? id : 0];
+    const double hmax = STEP_MAX / sqrt(y[3] * y[3] + y[4] * y[4] + y[5] * y[5]);
+    double h = (P.h0 && P.h0[id] > 0.0) ? P.h0[id] : hmax;
+    long long nacc = 0, natt = 0, traj_n = 1;
+    double kprev = signed_dist(P, y), land[6] = {0, 0, 0, 0, 0, 0};
+    int found = 0, st = STATUS_LEFT_BOUNDS;
+    if (P.traj && lane == 0) {
+      for (int c = 0; c < 6; c++) P.traj[(size_t)id * P.traj_cap * 6 + c] = y[c];
+      if (P.times) P.times[(size_t)id * P.traj_cap] = t;
+    }
+    while (inside_trace_bounds(P.bounds, y)) {
+      double k[6][6];
+      for (int s = 0; s < 6; s++) {
+        double ys[6];
+        for (int c = 0; c < 6; c++) {
+          double acc = y[c];
+          for (int j = 0; j < s; j++) acc = fma(RK_B[s][j], k[j][c], acc);
+          ys[c] = acc;
+        }
+        double E[3], H[3];
+        eval_field(P.field, ys, E, H, lane, 64, lane == 0);
+        for (int i = 0; i < 3; i++) { E[i] = wave_sum64(E[i]); H[i] = wave_sum64(H[i]); }
+        const double vxh0 = ys[4] * H[2] - ys[5] * H[1];
+        const double vxh1 = ys[5] * H[0] - ys[3] * H[2];
+        const double vxh2 = ys[3] * H[1] - ys[4] * H[0];
+        k[s][0] = h * ys[3]; k[s][1] = h * ys[4]; k[s][2] = h * ys[5];
+        k[s][3] = h * qm * (E[0] + MU0 * vxh0);
+        k[s][4] = h * qm * (E[1] + MU0 * vxh1);
+        k[s][5] = h * qm * (E[2] + MU0 * vxh2);
+      }
+      double ep = 0.0, ev = 0.0;
+      for (int c = 0; c < 6; c++) {
+        double e = 0.0;
+        for (int s = 0; s < 6; s++) e = fma(RK_CT[s], k[s][c], e);
+        e = fabs(e);
+        if (c < 3) ep = e > ep ? e : ep;
+        else ev = e > ev ? e : ev;
+      }
+      const double err = ep + h * ev;
+      natt++;
+      bool stop = false;
+      if (err <= P.atol) {
+        double yn[6];
+        for (int c = 0; c < 6; c++)
+          yn[c] = y[c] + (RK_CH[0] * k[0][c] + RK_CH[1] * k[1][c] + RK_CH[2] * k[2][c] + RK_CH[3] * k[3][c] +
+                          RK_CH[4] * k[4][c] + RK_CH[5] * k[5][c]);
+        t += h;
+        nacc++;
+        if (P.landing) {
+          const double kn = signed_dist(P, yn);
+          if ((kn > 0) != (kprev > 0)) {
+            const double diff = kprev - kn;
+            const double f = -kn / diff, fp = kprev / diff;
+            for (int c = 0; c < 6; c++) land[c] = fp * yn[c] + f * y[c];
+            found = 1;
+          }
+          kprev = kn;
+        }
+        for (int c = 0; c < 6; c++) y[c] = yn[c];
+        if (P.traj && lane == 0) {
+          for (int c = 0; c < 6; c++) P.traj[((size_t)id * P.traj_cap + traj_n) * 6 + c] = y[c];
+          if (P.times) P.times[(size_t)id * P.traj_cap + traj_n] = t;
+        }
+        traj_n++;
+        if (P.traj && traj_n == P.traj_cap && inside_trace_bounds(P.bounds, y)) { st = STATUS_BLOCK_FULL; stop = true; }
+      }
+      const double hn = 0.9 * h * pow(P.atol / err, 0.2);
+      h = hn < hmax ? hn : hmax;
+      if (!stop && natt >= P.max_attempts && inside_trace_bounds(P.bounds, y)) { st = STATUS_STEP_CAP; stop = true; }
+      if (stop) break;
+    }
+    if (lane == 0) {
+      for (int c = 0; c < 6; c++) P.final_state[6 * (size_t)id + c] = y[c];
+      if (P.landing)
+        for (int c = 0; c < 6; c++) P.landing[6 * (size_t)id + c] = found ? land[c] : __builtin_nan("");
+      if (P.status) P.status[id] = st;
+      if (P.n_accepted) P.n_accepted[id] = nacc;
+      if (P.n_attempted) P.n_attempted[id] = natt;
+      if (P.t_final) P.t_final[id] = t;
+      if (P.h_final) P.h_final[id] = h;
+      if (P.traj_len) P.traj_len[id] = traj_n;
+    }
+  }
+}
+
 // single field evaluation at a batch of points through a field program
 __global__ __launch_bounds__(256) void k_eval_program(FieldProgram F, const double* __restrict__ pts, int M, double* __restrict__ E,
                                                       double* __restrict__ H) {
@@ -290,12 +405,17 @@ __global__ __launch_bounds__(256) void k_eval_program(FieldProgram F, const doub
   for (int i = 0; i < 3; i++) { if (E) E[3 * (size_t)t + i] = e[i]; if (H) H[3 * (size_t)t + i] = h[i]; }
 }
 
-hipError_t launch_trace(const TraceParams& P, int waves, hipStream_t st) {
+hipError_t launch_trace(const TraceParams& P, int waves, int wave_per_particle, hipStream_t st) {
   if (P.n <= 0) return hipSuccess;
   int nw = waves > 0 ? waves : 1;
-  int need = (P.n + 63) / 64;
-  if (nw > need) nw = need;
-  hipLaunchKernelGGL(k_trace, dim3(nw), dim3(64), 0, st, P);
+  if (wave_per_particle) {
+    if (nw > P.n) nw = P.n;
+    hipLaunchKernelGGL(k_trace_wave, dim3(nw), dim3(64), 0, st, P);
+  } else {
+    int need = (P.n + 63) / 64;
+    if (nw > need) nw = need;
+    hipLaunchKernelGGL(k_trace, dim3(nw), dim3(64), 0, st, P);
+  }
   return hipGetLastError();
 }
 

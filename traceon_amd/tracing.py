@@ -1,0 +1,340 @@
+"""Particle tracing — counterpart of traceon/tracing.py.
+
+``Tracer.__call__`` (= ``trace_single``) keeps the reference signature and returns the full
+trajectory ``(times (T,), states (T,6))``.  For FieldRadialBEM and FieldRadialAxial it goes
+through the reference-named ``trace_particle`` entry point with the field's C function and
+argument struct, exactly as traceon/tracing.py:141-184 does; for a FieldSuperposition (whose
+reference path is a Python callback per RK stage) it runs the GPU tracer on a device field
+program.  Additions: ``Tracer.trace_batch`` traces many particles in one persistent launch
+and returns final states, landing states on a plane and step counts.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from math import sqrt, sin, cos
+
+import numpy as np
+
+from . import backend
+
+M_E = 9.1093837015e-31
+E_CHARGE = 1.602176634e-19
+
+
+def _convert_velocity_to_SI(velocity, mass):
+    speed_eV = np.linalg.norm(velocity)
+    speed = sqrt(2 * speed_eV * E_CHARGE / mass)
+    return speed * np.array(velocity) / speed_eV
+
+
+def velocity_vec(eV: float, direction, mass: float = M_E):
+    """tracing.py:32-56: velocity (m/s) of a particle with kinetic energy eV along direction."""
+    assert eV > 0.0, "Please provide a positive energy in eV"
+    d = np.array(direction, dtype=np.float64)
+    assert d.shape == (3,)
+    return _convert_velocity_to_SI(eV * d / np.linalg.norm(d), mass)
+
+
+def velocity_vec_spherical(eV: float, theta: float, phi: float):
+    return velocity_vec(eV, [sin(theta) * cos(phi), sin(theta) * sin(phi), cos(theta)])
+
+
+def velocity_vec_xz_plane(eV: float, angle: float, downward: bool = True):
+    sign = -1 if downward else 1
+    return velocity_vec(eV, [sin(angle), 0.0, sign * cos(angle)])
+
+
+# ---------------------------------------------------------------------------------------
+# ctypes mirrors of the Tier B tracer structs (include/traceon_hip.h)
+# ---------------------------------------------------------------------------------------
+class SourcesT(C.Structure):
+    _fields_ = [("w", C.c_void_p), ("r", C.c_void_p), ("z", C.c_void_p), ("n", C.c_int), ("pad", C.c_int)]
+
+
+class FieldCompT(C.Structure):
+    _fields_ = [("kind", C.c_int), ("transform", C.c_int), ("bounds_mode", C.c_int), ("pad", C.c_int), ("factor", C.c_double),
+                ("inv", C.c_double * 12), ("basis", C.c_double * 9), ("bounds", C.c_double * 6),
+                ("elec", SourcesT), ("mag", SourcesT), ("cur", SourcesT),
+                ("zs", C.c_void_p), ("ce", C.c_void_p), ("cm", C.c_void_p), ("nz", C.c_int), ("pad2", C.c_int),
+                ("uE", C.c_double * 3), ("uH", C.c_double * 3)]
+
+
+class FieldProgramT(C.Structure):
+    _fields_ = [("comp", C.c_void_p), ("ncomp", C.c_int), ("bounds_mode", C.c_int), ("bounds", C.c_double * 6)]
+
+
+class TraceParamsT(C.Structure):
+    _fields_ = [("field", FieldProgramT), ("state0", C.c_void_p), ("t0", C.c_void_p), ("h0", C.c_void_p), ("qm", C.c_void_p),
+                ("qm_stride", C.c_int), ("n", C.c_int), ("counter", C.c_void_p), ("bounds", C.c_double * 6), ("atol", C.c_double),
+                ("max_attempts", C.c_longlong), ("plane_p", C.c_double * 3), ("plane_n", C.c_double * 3), ("plane_nn", C.c_double),
+                ("final_state", C.c_void_p), ("landing", C.c_void_p), ("status", C.c_void_p), ("n_accepted", C.c_void_p),
+                ("n_attempted", C.c_void_p), ("t_final", C.c_void_p), ("h_final", C.c_void_p), ("traj", C.c_void_p),
+                ("times", C.c_void_p), ("traj_cap", C.c_longlong), ("traj_len", C.c_void_p)]
+
+
+COMP_KIND = {"bem": 0, "axial": 1, "uniform": 2}
+BOUNDS_NONE, BOUNDS_STRICT, BOUNDS_INCLUSIVE = 0, 1, 2
+STATUS = {0: "left bounds", 1: "block full", 2: "step cap"}
+
+
+def _set_sources(dst: SourcesT, charges):
+    if charges is None or len(charges) == 0:
+        dst.w = dst.r = dst.z = None
+        dst.n = 0
+        return None
+    s = charges.device_sources()
+    dst.w, dst.r, dst.z, dst.n = s.w.data_ptr(), s.r.data_ptr(), s.z.data_ptr(), s.n
+    return s
+
+
+def make_component(field, factor: float, kind: str, in_superposition: bool):
+    """One GPU field component.  A field traced on its own follows its C adaptor
+    (strict bounds, no transform: tracing.c:125-174, field.py:814-816); inside a
+    superposition it follows the Python Field API (affine map, inclusive bounds, basis
+    rotation: field.py:272-329)."""
+    c = FieldCompT()
+    c.kind = COMP_KIND[kind]
+    c.factor = float(factor)
+    keep = []
+    if in_superposition:
+        c.transform = int(field.has_transform())
+        inv = field._inverse_transformation_matrix[:3, :].ravel()
+        for i in range(12):
+            c.inv[i] = inv[i]
+        for i in range(9):
+            c.basis[i] = field._basis.ravel()[i]
+        mode = BOUNDS_INCLUSIVE
+    else:
+        c.transform = 0
+        mode = BOUNDS_STRICT if kind == "bem" else BOUNDS_NONE
+    if field.field_bounds is not None and (in_superposition or kind == "bem"):
+        c.bounds_mode = mode
+        for i, v in enumerate(np.asarray(field.field_bounds, dtype=np.float64).ravel()):
+            c.bounds[i] = v
+    else:
+        c.bounds_mode = BOUNDS_NONE
+    if kind == "bem":
+        keep.append(_set_sources(c.elec, field.electrostatic_point_charges))
+        keep.append(_set_sources(c.mag, field.magnetostatic_point_charges))
+        keep.append(_set_sources(c.cur, field.current_point_charges))
+    elif kind == "axial":
+        z, ce, cm = field.device_arrays()
+        c.zs, c.ce, c.cm, c.nz = z.data_ptr(), ce.data_ptr(), cm.data_ptr(), len(field.z)
+        keep += [z, ce, cm]
+    else:
+        for i in range(3):
+            c.uE[i] = field.E[i]
+            c.uH[i] = field.H[i]
+    return c, keep
+
+
+class DeviceFieldProgram:
+    """A field (or superposition) laid out for the GPU tracer; keeps its buffers alive."""
+
+    def __init__(self, field):
+        import torch
+        from .field import FieldSuperposition
+        comps = field.device_components(1.0)
+        self.keep = [k for _, k in comps]
+        # ring-source sums make a field evaluation cost 10^5 pairs: trace those wave-per-particle
+        self.has_sources = any(c.elec.n or c.mag.n or c.cur.n for c, _ in comps)
+        n = len(comps)
+        raw = (FieldCompT * n)(*[c for c, _ in comps])
+        host = torch.frombuffer(bytearray(bytes(raw)), dtype=torch.uint8)
+        self.comp_buf = host.to("cuda")
+        self.prog = FieldProgramT()
+        self.prog.comp = self.comp_buf.data_ptr()
+        self.prog.ncomp = n
+        # the superposition's own bounds (field.py:285-290 on the top-level object)
+        if isinstance(field, FieldSuperposition) and field.field_bounds is not None:
+            self.prog.bounds_mode = BOUNDS_INCLUSIVE
+            for i, v in enumerate(np.asarray(field.field_bounds).ravel()):
+                self.prog.bounds[i] = v
+        else:
+            self.prog.bounds_mode = BOUNDS_NONE
+
+    def evaluate(self, points):
+        """(E, H) at points (M,3) through the field program."""
+        import torch
+        from . import device
+        P = device.to_dev(np.atleast_2d(points))
+        M = P.shape[0]
+        E = torch.zeros((M, 3), dtype=torch.float64, device=P.device)
+        H = torch.zeros_like(E)
+        backend.check(backend.backend_lib.tb_eval_program(C.byref(self.prog), P.data_ptr(), M, E.data_ptr(), H.data_ptr(),
+                                                          device.stream_ptr()), "eval_program")
+        return E.cpu().numpy(), H.cpu().numpy()
+
+
+class BatchResult:
+    def __init__(self, final, landing, status, n_accepted, n_attempted, t_final):
+        self.final = final
+        self.landing = landing
+        self.status = status
+        self.n_accepted = n_accepted
+        self.n_attempted = n_attempted
+        self.t_final = t_final
+
+
+def run_trace(program: DeviceFieldProgram, state0, qm, bounds, atol=1e-8, plane=None, traj_cap=0, t0=None,
+              max_attempts=1 << 62, waves=0, wave_per_particle=None):
+    """Persistent GPU tracer over many particles (all arrays numpy or device tensors)."""
+    import torch
+    from . import device
+    S0 = state0 if isinstance(state0, torch.Tensor) else device.to_dev(np.atleast_2d(state0))
+    n = S0.shape[0]
+    qm_arr = np.atleast_1d(np.asarray(qm, dtype=np.float64))
+    QM = device.to_dev(qm_arr)
+    dev = S0.device
+    P = TraceParamsT()
+    P.field = program.prog
+    P.state0 = S0.data_ptr()
+    T0 = device.to_dev(np.asarray(t0, dtype=np.float64)) if t0 is not None else None
+    P.t0 = T0.data_ptr() if T0 is not None else None
+    P.h0 = None
+    P.qm = QM.data_ptr()
+    P.qm_stride = 1 if qm_arr.size > 1 else 0
+    P.n = n
+    counter = torch.zeros(1, dtype=torch.int32, device=dev)
+    P.counter = counter.data_ptr()
+    for i, v in enumerate(np.asarray(bounds, dtype=np.float64).ravel()):
+        P.bounds[i] = v
+    P.atol = float(atol)
+    P.max_attempts = int(max_attempts)
+    if plane is not None:
+        p0, nrm = plane
+        for i in range(3):
+            P.plane_p[i] = p0[i]
+            P.plane_n[i] = nrm[i]
+        P.plane_nn = float(np.linalg.norm(nrm))
+    else:
+        P.plane_n[2] = 1.0
+        P.plane_nn = 1.0
+    final = torch.empty((n, 6), dtype=torch.float64, device=dev)
+    landing = torch.empty((n, 6), dtype=torch.float64, device=dev) if plane is not None else None
+    status = torch.empty(n, dtype=torch.int32, device=dev)
+    nacc = torch.empty(n, dtype=torch.int64, device=dev)
+    natt = torch.empty(n, dtype=torch.int64, device=dev)
+    tfin = torch.empty(n, dtype=torch.float64, device=dev)
+    P.final_state = final.data_ptr()
+    P.landing = landing.data_ptr() if landing is not None else None
+    P.status = status.data_ptr()
+    P.n_accepted = nacc.data_ptr()
+    P.n_attempted = natt.data_ptr()
+    P.t_final = tfin.data_ptr()
+    P.h_final = None
+    traj = times = tlen = None
+    if traj_cap:
+        traj = torch.empty((n, traj_cap, 6), dtype=torch.float64, device=dev)
+        times = torch.empty((n, traj_cap), dtype=torch.float64, device=dev)
+        tlen = torch.empty(n, dtype=torch.int64, device=dev)
+        P.traj, P.times, P.traj_cap, P.traj_len = traj.data_ptr(), times.data_ptr(), int(traj_cap), tlen.data_ptr()
+    if wave_per_particle is None:
+        wave_per_particle = program.has_sources
+    backend.check(backend.backend_lib.tb_trace(C.byref(P), int(waves), int(wave_per_particle), device.stream_ptr()), "trace")
+    out = BatchResult(final, landing, status, nacc, natt, tfin)
+    out.traj, out.times, out.traj_len = traj, times, tlen
+    out._keep = (S0, QM, T0, counter)
+    return out
+
+
+class Tracer:
+    """tracing.py:104-245."""
+
+    def __init__(self, field, bounds):
+        self.field = field
+        bounds = np.array(bounds, dtype=np.float64)
+        assert bounds.shape == (3, 2)
+        self.bounds = bounds
+        self._program = None
+        from .field import FieldRadialBEM, FieldRadialAxial
+        if isinstance(field, (FieldRadialBEM, FieldRadialAxial)):
+            self.trace_fun, self.low_level_args = field.get_low_level_trace_function()
+            self.trace_args = C.cast(C.pointer(self.low_level_args), C.c_void_p)
+        else:
+            self.trace_fun = None
+            self.low_level_args = None
+            self.trace_args = None
+
+    def program(self) -> DeviceFieldProgram:
+        if self._program is None:
+            self._program = DeviceFieldProgram(self.field)
+        return self._program
+
+    def __call__(self, *args, **kwargs):
+        return self.trace_single(*args, **kwargs)
+
+    def trace_single(self, position, velocity, mass: float = M_E, charge: float = -E_CHARGE, atol: float = 1e-8):
+        position = np.array(position, dtype=np.float64)
+        velocity = np.array(velocity, dtype=np.float64)
+        qm = charge / mass
+        if self.trace_fun is not None:
+            return backend.trace_particle(position, velocity, qm, self.trace_fun, self.bounds, atol, self.trace_args)
+        return self._trace_program(position, velocity, qm, atol)
+
+    def _trace_program(self, position, velocity, qm, atol):
+        """The reference's block loop (backend/__init__.py:382-420) over the GPU tracer."""
+        prog = self.program()
+        cap = backend.TRACING_BLOCK_SIZE
+        state = np.concatenate([position, velocity])[None, :]
+        t0 = 0.0
+        tb, sb = [], []
+        while True:
+            r = run_trace(prog, state, qm, self.bounds, atol, traj_cap=cap, t0=np.array([t0]), waves=1)
+            n = int(r.traj_len[0].item())
+            s = r.traj[0, :n].cpu().numpy()
+            t = r.times[0, :n].cpu().numpy()
+            sb.append(s[1:] if sb else s)
+            tb.append(t[1:] if tb else t)
+            if n != cap:
+                break
+            state = sb[-1][-1][None, :].copy()
+            t0 = tb[-1][-1]
+        if len(sb) == 1:
+            return tb[0], sb[0]
+        return np.concatenate(tb), np.concatenate(sb)
+
+    def trace_multiple(self, position, velocity, mass=M_E, charge=-E_CHARGE, atol: float = 1e-8):
+        """tracing.py:210-245: one trajectory per particle (numpy broadcasting)."""
+        positions, velocities = np.broadcast_arrays(np.atleast_2d(position), np.atleast_2d(velocity))
+        N = len(positions)
+        masses = np.broadcast_to(np.asarray(mass, dtype=np.float64), (N,))
+        charges = np.broadcast_to(np.asarray(charge, dtype=np.float64), (N,))
+        return [self.trace_single(p, v, mass=float(m), charge=float(c), atol=atol)
+                for p, v, m, c in zip(positions, velocities, masses, charges)]
+
+    def trace_batch(self, positions, velocities, mass=M_E, charge=-E_CHARGE, atol: float = 1e-8, plane=None,
+                    max_attempts=1 << 62, waves=0) -> BatchResult:
+        """Trace many particles in one persistent launch; no trajectories are stored.
+        plane = (p0, normal) records the state where each trajectory last crosses it
+        (what xy_plane_intersection returns on the full trajectory), NaN if never."""
+        positions = np.atleast_2d(np.asarray(positions, dtype=np.float64))
+        velocities = np.atleast_2d(np.asarray(velocities, dtype=np.float64))
+        positions, velocities = np.broadcast_arrays(positions, velocities)
+        state0 = np.concatenate([positions, velocities], axis=1)
+        qm = np.asarray(charge, dtype=np.float64) / np.asarray(mass, dtype=np.float64)
+        if qm.size > 1:
+            qm = np.broadcast_to(qm, (len(state0),))
+        return run_trace(self.program(), state0, qm, self.bounds, atol, plane=plane, max_attempts=max_attempts, waves=waves)
+
+
+def plane_intersection(positions, p0, normal):
+    positions = np.array(positions, dtype=np.float64)
+    assert positions.shape == (len(positions), 6), "The positions array should have shape (N, 6)"
+    return backend.plane_intersection(positions, np.array(p0, dtype=np.float64), np.array(normal, dtype=np.float64))
+
+
+def xy_plane_intersection(positions, z: float):
+    return plane_intersection(positions, [0, 0, z], [0, 0, 1])
+
+
+def xz_plane_intersection(positions, y: float):
+    return plane_intersection(positions, [0, y, 0], [0, 1, 0])
+
+
+def yz_plane_intersection(positions, x: float):
+    return plane_intersection(positions, [x, 0, 0], [1, 0, 0])
+
+
+def axis_intersection(positions) -> float:
+    return yz_plane_intersection(positions, 0)[2]
