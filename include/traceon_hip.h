@@ -145,9 +145,11 @@ int tb_fill_diagonal(double *M, size_t ld, float *M32, size_t ld32, const double
 size_t tb_lu_workspace_bytes(int N, int fp32);
 int tb_lu_factor(double *A, size_t lda, int N, int *ipiv, void *workspace, int *info, void *stream);
 int tb_lu_factor_f32(float *A, size_t lda, int N, int *ipiv, void *workspace, int *info, void *stream);
-/* X (nrhs x N) <- A^{-1} X with the factors (FP64 vectors; FP32 factors promoted on load) */
-int tb_lu_solve(const double *LU, size_t lda, int N, const int *ipiv, double *X, int nrhs, void *stream);
-int tb_lu_solve_f32(const float *LU, size_t lda, int N, const int *ipiv, double *X, int nrhs, void *stream);
+/* X (nrhs x N, nrhs <= 64) <- A^{-1} X with the factors and the workspace of the same
+   factorisation, which holds the diagonal-block inverses and the row permutation
+   (FP64 vectors; FP32 factors promoted on load) */
+int tb_lu_solve(const double *LU, size_t lda, int N, const int *ipiv, double *X, int nrhs, const void *workspace, void *stream);
+int tb_lu_solve_f32(const float *LU, size_t lda, int N, const int *ipiv, double *X, int nrhs, const void *workspace, void *stream);
 /* R = B - A X in FP64, rmax[r] = bit pattern of max |R[r]| (nonnegative double) */
 int tb_residual(const double *A, size_t lda, int N, const double *B, const double *X, double *R, int nrhs,
                 unsigned long long *rmax, void *stream);
@@ -214,10 +216,10 @@ typedef struct {
 } tb_trace_params_t;
 
 /* Persistent RK45 tracer over params->n particles (all pointers device pointers).
-   waves: number of 64-lane workgroups (0: 8 per CU).  wave_per_particle = 0: one lane per
-   particle (cheap fields: axial interpolation, uniform); 1: one wave per particle with the
-   ring-source sums split over its 64 lanes (direct BEM fields). */
-int tb_trace(const tb_trace_params_t *params, int waves, int wave_per_particle, void *stream);
+   One lane per particle; waves: number of 64-lane workgroups that drain the particle
+   queue (0: 8 per CU).  Ring-source sums of direct BEM fields run wave-uniformly, every
+   lane of a wave sharing each source. */
+int tb_trace(const tb_trace_params_t *params, int waves, void *stream);
 int tb_eval_program(const tb_field_program_t *program, const double *pts, int M, double *E, double *H, void *stream);
 
 #ifdef __cplusplus

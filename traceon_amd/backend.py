@@ -96,8 +96,8 @@ _SIG = {
     "tb_lu_workspace_bytes": (sz, [i32, i32]),
     "tb_lu_factor": (i32, [vp, sz, i32, vp, vp, vp, vp]),
     "tb_lu_factor_f32": (i32, [vp, sz, i32, vp, vp, vp, vp]),
-    "tb_lu_solve": (i32, [vp, sz, i32, vp, vp, i32, vp]),
-    "tb_lu_solve_f32": (i32, [vp, sz, i32, vp, vp, i32, vp]),
+    "tb_lu_solve": (i32, [vp, sz, i32, vp, vp, i32, vp, vp]),
+    "tb_lu_solve_f32": (i32, [vp, sz, i32, vp, vp, i32, vp, vp]),
     "tb_residual": (i32, [vp, sz, i32, vp, vp, vp, i32, vp, vp]),
     "tb_axpy": (i32, [vp, vp, sz, vp]),
     "tb_convert_f64_f32": (i32, [vp, sz, vp, sz, i32, i32, vp]),
@@ -112,7 +112,7 @@ _SIG = {
     "tb_axial_points": (i32, [vp, i32, vp, i32, vp, vp, vp, vp]),
     "tb_jacobian_3d": (i32, [vp, i32, vp, vp, vp]),
     "tb_charge_radial": (i32, [vp, vp, i32, vp, vp]),
-    "tb_trace": (i32, [vp, i32, i32, vp]),
+    "tb_trace": (i32, [vp, i32, vp]),
     "tb_eval_program": (i32, [vp, vp, i32, vp, vp, vp]),
 }
 

@@ -400,7 +400,7 @@ size_t trace_particle(double* times, double* states, double qm, field_fun field,
   P.plane_n[2] = 1.0; P.plane_nn = 1.0;
   P.final_state = fin.p; P.landing = nullptr; P.status = status.p; P.n_accepted = nacc.p; P.n_attempted = natt.p;
   P.t_final = tf.p; P.h_final = hf.p; P.traj = traj.p; P.times = tt.p; P.traj_cap = cap; P.traj_len = tlen.p;
-  tb::launch_trace(P, 1, 1, 0);
+  tb::launch_trace(P, 1, 0);
   long long n = 0;
   if (!finish("trace_particle") || !tlen.get(&n, 1)) return 0;
   // row 0 is the caller's initial state; rows 1..n-1 the accepted steps
@@ -449,11 +449,11 @@ int tb_lu_factor(double* A, size_t lda, int N, int* ipiv, void* ws, int* info, v
 int tb_lu_factor_f32(float* A, size_t lda, int N, int* ipiv, void* ws, int* info, void* stream) {
   TB_CALL(tb::lu_factor<float>(A, lda, N, ipiv, ws, info, S(stream), 0), "lu_factor_f32");
 }
-int tb_lu_solve(const double* LU, size_t lda, int N, const int* ipiv, double* X, int nrhs, void* stream) {
-  TB_CALL(tb::lu_solve<double>(LU, lda, N, ipiv, X, nrhs, S(stream)), "lu_solve");
+int tb_lu_solve(const double* LU, size_t lda, int N, const int* ipiv, double* X, int nrhs, const void* ws, void* stream) {
+  TB_CALL(tb::lu_solve<double>(LU, lda, N, ipiv, X, nrhs, ws, S(stream)), "lu_solve");
 }
-int tb_lu_solve_f32(const float* LU, size_t lda, int N, const int* ipiv, double* X, int nrhs, void* stream) {
-  TB_CALL(tb::lu_solve<float>(LU, lda, N, ipiv, X, nrhs, S(stream)), "lu_solve_f32");
+int tb_lu_solve_f32(const float* LU, size_t lda, int N, const int* ipiv, double* X, int nrhs, const void* ws, void* stream) {
+  TB_CALL(tb::lu_solve<float>(LU, lda, N, ipiv, X, nrhs, ws, S(stream)), "lu_solve_f32");
 }
 int tb_residual(const double* A, size_t lda, int N, const double* B, const double* X, double* R, int nrhs, unsigned long long* rmax,
                 void* stream) {
@@ -504,7 +504,7 @@ int tb_jacobian_3d(const double* tri, int N, double* jac, double* pos, void* str
 int tb_charge_radial(const double* lines, const double* q, int N, double* out, void* stream) {
   TB_CALL(tb::launch_charge(lines, q, N, out, S(stream)), "charge_radial");
 }
-int tb_trace(const tb_trace_params_t* params, int waves, int wave_per_particle, void* stream) {
+int tb_trace(const tb_trace_params_t* params, int waves, void* stream) {
   if (!params) { set_err("tb_trace: null params"); return -1; }
   int nw = waves;
   if (nw <= 0) {
@@ -514,7 +514,7 @@ int tb_trace(const tb_trace_params_t* params, int waves, int wave_per_particle, 
     if (hipGetDeviceProperties(&prop, dev) != hipSuccess) { set_err("device properties"); return -1; }
     nw = prop.multiProcessorCount * 8;
   }
-  TB_CALL(tb::launch_trace(*reinterpret_cast<const tb::TraceParams*>(params), nw, wave_per_particle, S(stream)), "trace");
+  TB_CALL(tb::launch_trace(*reinterpret_cast<const tb::TraceParams*>(params), nw, S(stream)), "trace");
 }
 int tb_eval_program(const tb_field_program_t* program, const double* pts, int M, double* E, double* H, void* stream) {
   TB_CALL(tb::launch_eval_program(*reinterpret_cast<const tb::FieldProgram*>(program), pts, M, E, H, S(stream)), "eval_program");

@@ -317,15 +317,25 @@ template <> struct MfmaAcc<float> {
 
 constexpr int GBM = 128, GBN = 128;
 
+// Software pipeline: the global loads of K-chunk kt+1 are issued into registers before the
+// MFMA loop of chunk kt and written to the other LDS buffer after it, so their latency hides
+// behind the matrix instructions.  Loads are 16-byte vectors along the contiguous dimension
+// (k for A, n for B).  A is stored k-major in LDS (the 16 rows an MFMA reads are adjacent).
 template <typename T, int BK>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, const T* __restrict__ A, size_t lda,
                                               const T* __restrict__ B, size_t ldb, T beta, T* __restrict__ C, size_t ldc) {
   typedef typename MfmaAcc<T>::type acc_t;
-  __shared__ T As[2][BK][GBM + 4];
-  __shared__ T Bs[2][BK][GBN + 4];
+  constexpr int VEC = 16 / sizeof(T);                     // elements per 16-byte load
+  constexpr int A_LD = (GBM * BK) / (256 * VEC);          // vector loads per thread for A
+  constexpr int B_LD = (BK * GBN) / (256 * VEC);
+  constexpr int APAD = (sizeof(T) == 8) ? 2 : 4;
+  __shared__ T As[2][BK][GBM + APAD];
+  __shared__ T Bs[2][BK][GBN + APAD];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
   const int bm = blockIdx.y * GBM, bn = blockIdx.x * GBN;
+  const bool vec_ok_a = ((lda % VEC) == 0) && ((((uintptr_t)A) % 16) == 0);
+  const bool vec_ok_b = ((ldb % VEC) == 0) && ((((uintptr_t)B) % 16) == 0);
 
   acc_t acc[4][4];
 #pragma unroll
@@ -333,26 +343,72 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, cons
 #pragma unroll
     for (int b = 0; b < 4; b++) acc[a][b] = acc_t{0, 0, 0, 0};
 
-  auto load_tile = [&](int buf, int k0) {
-    // A: GBM x BK elements, each thread several; store transposed (k-major)
-    for (int e = tid; e < GBM * BK; e += 256) {
-      int r = e / BK, k = e % BK;
-      int gr = bm + r, gk = k0 + k;
-      As[buf][k][r] = (gr < M && gk < K) ? A[(size_t)gr * lda + gk] : T(0);
+  T ra[A_LD][VEC], rb[B_LD][VEC];
+
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < A_LD; i++) {
+      const int e = (tid + i * 256) * VEC;   // element index in the GBM x BK tile, k fastest
+      const int r = e / BK, k = e % BK;
+      const int gr = bm + r, gk = k0 + k;
+      if (vec_ok_a && gr < M && gk + VEC <= K) {
+        const T* src = A + (size_t)gr * lda + gk;
+        if constexpr (sizeof(T) == 8) {
+          double2 v = *reinterpret_cast<const double2*>(src);
+          ra[i][0] = v.x; ra[i][1] = v.y;
+        } else {
+          float4 v = *reinterpret_cast<const float4*>(src);
+          ra[i][0] = v.x; ra[i][1] = v.y; ra[i][2] = v.z; ra[i][3] = v.w;
+        }
+      } else {
+#pragma unroll
+        for (int v = 0; v < VEC; v++) ra[i][v] = (gr < M && gk + v < K) ? A[(size_t)gr * lda + gk + v] : T(0);
+      }
     }
-    for (int e = tid; e < BK * GBN; e += 256) {
-      int k = e / GBN, c = e % GBN;
-      int gk = k0 + k, gc = bn + c;
-      Bs[buf][k][c] = (gk < K && gc < N) ? B[(size_t)gk * ldb + gc] : T(0);
+#pragma unroll
+    for (int i = 0; i < B_LD; i++) {
+      const int e = (tid + i * 256) * VEC;   // element index in the BK x GBN tile, n fastest
+      const int k = e / GBN, c = e % GBN;
+      const int gk = k0 + k, gc = bn + c;
+      if (vec_ok_b && gk < K && gc + VEC <= N) {
+        const T* src = B + (size_t)gk * ldb + gc;
+        if constexpr (sizeof(T) == 8) {
+          double2 v = *reinterpret_cast<const double2*>(src);
+          rb[i][0] = v.x; rb[i][1] = v.y;
+        } else {
+          float4 v = *reinterpret_cast<const float4*>(src);
+          rb[i][0] = v.x; rb[i][1] = v.y; rb[i][2] = v.z; rb[i][3] = v.w;
+        }
+      } else {
+#pragma unroll
+        for (int v = 0; v < VEC; v++) rb[i][v] = (gk < K && gc + v < N) ? B[(size_t)gk * ldb + gc + v] : T(0);
+      }
+    }
+  };
+  auto stash = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < A_LD; i++) {
+      const int e = (tid + i * 256) * VEC;
+      const int r = e / BK, k = e % BK;
+#pragma unroll
+      for (int v = 0; v < VEC; v++) As[buf][k + v][r] = ra[i][v];
+    }
+#pragma unroll
+    for (int i = 0; i < B_LD; i++) {
+      const int e = (tid + i * 256) * VEC;
+      const int k = e / GBN, c = e % GBN;
+#pragma unroll
+      for (int v = 0; v < VEC; v++) Bs[buf][k][c + v] = rb[i][v];
     }
   };
 
   const int nk = (K + BK - 1) / BK;
-  load_tile(0, 0);
+  fetch(0);
+  stash(0);
   __syncthreads();
   for (int kt = 0; kt < nk; kt++) {
     const int cur = kt & 1;
-    if (kt + 1 < nk) load_tile(cur ^ 1, (kt + 1) * BK);
+    if (kt + 1 < nk) fetch((kt + 1) * BK);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       T af[4], bf[4];
@@ -366,6 +422,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, cons
 #pragma unroll
         for (int b = 0; b < 4; b++) acc[a][b] = MfmaAcc<T>::mma(af[a], bf[b], acc[a][b]);
     }
+    if (kt + 1 < nk) stash(cur ^ 1);
     __syncthreads();
   }
   // write back
@@ -404,67 +461,116 @@ __global__ __launch_bounds__(256) void k_d2f(const double* __restrict__ S, size_
 }
 
 // ---------------------------------------------------------------------------------
-// Solves with the factors: x = U^{-1} L^{-1} P b, nrhs right-hand sides stored
-// as (nrhs, N) rows (the reference's F layout).  Vectors in FP64; factor entries
-// are promoted on load (an FP32 factorisation is applied in FP64 arithmetic).
+// Solves with the factors: x = U^{-1} L^{-1} P b for nrhs right-hand sides stored as
+// (nrhs, N) rows (the reference's F layout).  Vectors in FP64; factor entries are
+// promoted on load (an FP32 factorisation is applied in FP64 arithmetic).
+//
+// One kernel per triangle.  Workgroup g claims block row b from a counter (in order, so
+// every block it waits for belongs to a workgroup that is already running), accumulates
+// L[b, j] x_j for the solved blocks j as their completion flags appear, then applies the
+// inverse of the diagonal block stored by the factorisation.  Waits are bounded: past the
+// spin limit the kernel records an error and finishes.
 // ---------------------------------------------------------------------------------
-__global__ void k_apply_pivots(double* __restrict__ X, int N, int nrhs, const int* __restrict__ ipiv) {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nrhs) return;
-  double* x = X + (size_t)r * N;
-  for (int i = 0; i < N; i++) {
-    int p = ipiv[i];
-    if (p != i && p >= 0 && p < N) { double t = x[i]; x[i] = x[p]; x[p] = t; }
+
+// permutation of the row interchanges: perm[i] = original row that ends at position i
+__global__ __launch_bounds__(256) void k_perm(const int* __restrict__ ipiv, int N, int* __restrict__ perm) {
+  extern __shared__ int pl[];
+  for (int i = threadIdx.x; i < N; i += blockDim.x) pl[i] = i;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int d = 0; d < N; d++) {
+      int p = ipiv[d];
+      if (p != d && p >= 0 && p < N) { int t = pl[d]; pl[d] = pl[p]; pl[p] = t; }
+    }
   }
+  __syncthreads();
+  for (int i = threadIdx.x; i < N; i += blockDim.x) perm[i] = pl[i];
 }
 
-constexpr int TSB = 64;   // rows per diagonal block of the triangular solves
+constexpr long long SPIN_LIMIT = 1ll << 26;
 
-// diagonal block solve: one workgroup, block [b0, b0+nb)
+__global__ void k_set_int(int* p, int v) { *p = v; }
+
 template <typename T, bool LOWER>
-__global__ __launch_bounds__(64) void k_trsv_diag(const T* __restrict__ A, size_t lda, int N, int b0, int nb, double* __restrict__ X,
-                                                 int nrhs) {
-  __shared__ double xs[TSB];
-  for (int r = 0; r < nrhs; r++) {
-    double* x = X + (size_t)r * N;
-    if (threadIdx.x < nb) xs[threadIdx.x] = x[b0 + threadIdx.x];
-    __syncthreads();
-    if (LOWER) {
-      for (int i = 1; i < nb; i++) {
-        double part = 0.0;
-        for (int s = threadIdx.x; s < i; s += 64) part += (double)A[(size_t)(b0 + i) * lda + b0 + s] * xs[s];
-        for (int o = 32; o >= 1; o >>= 1) part += __shfl_xor(part, o, 64);
-        if (threadIdx.x == 0) xs[i] -= part;
-        __syncthreads();
-      }
-    } else {
-      for (int i = nb - 1; i >= 0; i--) {
-        double part = 0.0;
-        for (int s = i + 1 + threadIdx.x; s < nb; s += 64) part += (double)A[(size_t)(b0 + i) * lda + b0 + s] * xs[s];
-        for (int o = 32; o >= 1; o >>= 1) part += __shfl_xor(part, o, 64);
-        if (threadIdx.x == 0) xs[i] = (xs[i] - part) / (double)A[(size_t)(b0 + i) * lda + b0 + i];
-        __syncthreads();
+__global__ __launch_bounds__(256) void k_trsv_blocked(const T* __restrict__ LU, size_t lda, int N, const int* __restrict__ perm,
+                                                      const int* __restrict__ pivoted, const double* __restrict__ Bv,
+                                                      double* __restrict__ X, int rhs, const T* __restrict__ Dinv,
+                                                      int* __restrict__ flags, int* __restrict__ counter, int* __restrict__ err) {
+  constexpr int NB = LU_NB;
+  constexpr int RPW = NB / 4;   // rows per wave
+  __shared__ double xs[NB];
+  __shared__ double vs[NB];
+  __shared__ int s_b;
+  const int nblk = (N + NB - 1) / NB;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid == 0) {
+    int t = atomicAdd(counter, 1);
+    s_b = LOWER ? t : nblk - 1 - t;
+  }
+  __syncthreads();
+  const int b = s_b;
+  if (b < 0 || b >= nblk) return;
+  const int r0 = b * NB;
+  const int nr = min(NB, N - r0);
+  double acc[RPW];
+#pragma unroll
+  for (int i = 0; i < RPW; i++) acc[i] = 0.0;
+  const double* xr = X + (size_t)rhs * N;
+  volatile const double* xv = xr;
+  const int jfirst = LOWER ? 0 : b + 1, jlast = LOWER ? b - 1 : nblk - 1;
+  for (int jb = jfirst; jb <= jlast; jb++) {
+    if (tid == 0) {
+      long long spins = 0;
+      while (__atomic_load_n(flags + jb, __ATOMIC_ACQUIRE) == 0) {
+        if (++spins > SPIN_LIMIT) { atomicOr(err, 1); break; }
+        __builtin_amdgcn_s_sleep(1);
       }
     }
-    if (threadIdx.x < nb) x[b0 + threadIdx.x] = xs[threadIdx.x];
     __syncthreads();
+    const int c0 = jb * NB;
+    const int nc = min(NB, N - c0);
+    const double x0 = lane < nc ? xv[c0 + lane] : 0.0;
+    const double x1 = lane + 64 < nc ? xv[c0 + lane + 64] : 0.0;
+#pragma unroll
+    for (int i = 0; i < RPW; i++) {
+      const int row = r0 + wave * RPW + i;
+      if (row < N) {
+        const T* Lr = LU + (size_t)row * lda + c0;
+        double p = lane < nc ? (double)Lr[lane] * x0 : 0.0;
+        if (lane + 64 < nc) p = fma((double)Lr[lane + 64], x1, p);
+        acc[i] += p;
+      }
+    }
   }
-}
-
-// update after a solved block: x[i] -= A[i, b0:b0+nb] . x[b0:b0+nb] for rows i in [r0, r1); one wave per row
-template <typename T>
-__global__ __launch_bounds__(256) void k_trsv_update(const T* __restrict__ A, size_t lda, int N, int b0, int nb, int r0, int r1,
-                                                    double* __restrict__ X, int nrhs) {
-  const int row = r0 + blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (row >= r1) return;
-  for (int r = 0; r < nrhs; r++) {
-    double* x = X + (size_t)r * N;
-    double part = 0.0;
-    for (int s = lane; s < nb; s += 64) part += (double)A[(size_t)row * lda + b0 + s] * x[b0 + s];
-    for (int o = 32; o >= 1; o >>= 1) part += __shfl_xor(part, o, 64);
-    if (lane == 0) x[row] -= part;
+  // v = b - sum, then x_b = inv(diagonal block) v
+#pragma unroll
+  for (int i = 0; i < RPW; i++) {
+    double v = acc[i];
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int lr = wave * RPW + i;
+    if (lane == 0 && lr < NB) {
+      const int row = r0 + lr;
+      double bv = 0.0;
+      const bool use_perm = perm && pivoted && *pivoted;
+      if (row < N) bv = LOWER ? Bv[(size_t)rhs * N + (use_perm ? perm[row] : row)] : Bv[(size_t)rhs * N + row];
+      vs[lr] = (row < N) ? bv - v : 0.0;
+    }
   }
+  __syncthreads();
+  const T* Db = Dinv + (size_t)b * NB * NB;
+#pragma unroll
+  for (int i = 0; i < RPW; i++) {
+    const int lr = wave * RPW + i;
+    double p = (double)Db[(size_t)lr * NB + lane] * vs[lane];
+    p = fma((double)Db[(size_t)lr * NB + lane + 64], vs[lane + 64], p);
+    for (int o = 32; o >= 1; o >>= 1) p += __shfl_xor(p, o, 64);
+    if (lane == 0) xs[lr] = p;
+  }
+  __syncthreads();
+  if (tid < nr) X[(size_t)rhs * N + r0 + tid] = xs[tid];
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) __atomic_store_n(flags + b, 1, __ATOMIC_RELEASE);
 }
 
 // residual r = b - A x with the FP64 matrix, accumulated in double-double: each product
@@ -539,21 +645,42 @@ template hipError_t gemm<double>(int, int, int, double, const double*, size_t, c
                                  hipStream_t);
 template hipError_t gemm<float>(int, int, int, float, const float*, size_t, const float*, size_t, float, float*, size_t, hipStream_t);
 
+// Workspace of one factorisation (kept with the factors: the solves read the block inverses,
+// the permutation and the pivot flag from it).
+template <typename T>
+struct LuWorkspace {
+  static constexpr int G = 1024, NB = LU_NB;
+  double* cand_val; int* cand_idx; T* cand_row;   // pivot candidates of the cooperative panel
+  T* Y; T* Z;                                     // scratch inverses of the current panel
+  T* U12; T* Pbak;                                // trailing-update staging, panel backup
+  T* Linv; T* Uinv;                               // inverses of every diagonal block (nblk x NB x NB)
+  int* perm; int* flags; int* err; int* pivoted;  // permutation, solve flags, error, any-interchange flag
+  double* rhs;                                    // permuted right-hand sides (nrhs x N, nrhs <= 64)
+  unsigned long long* lmax;
+  size_t bytes;
+  LuWorkspace(void* base, int N) {
+    const int nblk = (N + NB - 1) / NB;
+    size_t o = 0;
+    auto take = [&](size_t b) { size_t at = o; o = (o + b + 255) & ~(size_t)255; return at; };
+    unsigned char* p = (unsigned char*)base;
+    size_t a_cv = take(2 * G * sizeof(double)), a_ci = take(2 * G * sizeof(int)), a_cr = take((2 * (size_t)G * NB + 2 * NB) * sizeof(T));
+    size_t a_y = take((size_t)NB * NB * sizeof(T)), a_z = take((size_t)NB * NB * sizeof(T));
+    size_t a_u = take((size_t)NB * N * sizeof(T)), a_pb = take((size_t)NB * N * sizeof(T));
+    size_t a_li = take((size_t)nblk * NB * NB * sizeof(T)), a_ui = take((size_t)nblk * NB * NB * sizeof(T));
+    size_t a_pm = take((size_t)N * sizeof(int)), a_fl = take((2 * (size_t)nblk + 2) * sizeof(int));
+    size_t a_er = take(sizeof(int)), a_pv = take(sizeof(int)), a_rhs = take(64 * (size_t)N * sizeof(double)), a_lm = take(8);
+    bytes = o;
+    if (!base) return;
+    cand_val = (double*)(p + a_cv); cand_idx = (int*)(p + a_ci); cand_row = (T*)(p + a_cr);
+    Y = (T*)(p + a_y); Z = (T*)(p + a_z); U12 = (T*)(p + a_u); Pbak = (T*)(p + a_pb);
+    Linv = (T*)(p + a_li); Uinv = (T*)(p + a_ui); perm = (int*)(p + a_pm); flags = (int*)(p + a_fl);
+    err = (int*)(p + a_er); pivoted = (int*)(p + a_pv); rhs = (double*)(p + a_rhs); lmax = (unsigned long long*)(p + a_lm);
+  }
+};
+
 template <typename T>
 size_t lu_workspace_bytes(int N) {
-  // candidates (2G doubles + 2G ints + 2G*NB T + 2NB T), two NB x NB inverses,
-  // U12/L21 staging (NB*N T), panel backup (NB*N T), flags
-  const int G = 1024, NB = LU_NB;
-  size_t b = 2 * G * sizeof(double) + 2 * G * sizeof(int) + (2 * (size_t)G * NB + 2 * NB) * sizeof(T);
-  b = (b + 255) & ~(size_t)255;
-  b += 2 * (size_t)NB * NB * sizeof(T);
-  b = (b + 255) & ~(size_t)255;
-  b += (size_t)NB * (size_t)N * sizeof(T);
-  b = (b + 255) & ~(size_t)255;
-  b += (size_t)NB * (size_t)N * sizeof(T);
-  b = (b + 255) & ~(size_t)255;
-  b += 256;
-  return b;
+  return LuWorkspace<T>(nullptr, N).bytes;
 }
 template size_t lu_workspace_bytes<double>(int);
 template size_t lu_workspace_bytes<float>(int);
@@ -583,19 +710,15 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
   e = hipFuncSetAttribute((const void*)kgetf2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(NB * NB * sizeof(T)));
   if (e != hipSuccess) return e;
 
-  unsigned char* w = (unsigned char*)ws;
-  double* wsd = (double*)w;                       w += 2 * 1024 * sizeof(double);
-  int* wsi = (int*)w;                             w += 2 * 1024 * sizeof(int);
-  T* wsr = (T*)w;                                 w += (2 * (size_t)1024 * NB + 2 * NB) * sizeof(T);
-  w = (unsigned char*)(((uintptr_t)w + 255) & ~(uintptr_t)255);
-  T* Y = (T*)w;                                   w += (size_t)NB * NB * sizeof(T);
-  T* Z = (T*)w;                                   w += (size_t)NB * NB * sizeof(T);
-  w = (unsigned char*)(((uintptr_t)w + 255) & ~(uintptr_t)255);
-  T* U12 = (T*)w;                                 w += (size_t)NB * (size_t)N * sizeof(T);
-  w = (unsigned char*)(((uintptr_t)w + 255) & ~(uintptr_t)255);
-  T* Pbak = (T*)w;                                w += (size_t)NB * (size_t)N * sizeof(T);
-  w = (unsigned char*)(((uintptr_t)w + 255) & ~(uintptr_t)255);
-  unsigned long long* lmax = (unsigned long long*)w;
+  LuWorkspace<T> W(ws, N);
+  double* wsd = W.cand_val;
+  int* wsi = W.cand_idx;
+  T* wsr = W.cand_row;
+  T* U12 = W.U12;
+  T* Pbak = W.Pbak;
+  unsigned long long* lmax = W.lmax;
+  bool any_pivot = false;
+  hipMemsetAsync(W.err, 0, sizeof(int), st);
 
   const unsigned long long ONE_BITS = 0x3FF0000000000000ull;   // bit pattern of 1.0
   for (int k0 = 0; k0 < N; k0 += NB) {
@@ -607,8 +730,10 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
                        (size_t)nb, N - k0, nb);
     hipMemsetAsync(lmax, 0, sizeof(unsigned long long), st);
     hipLaunchKernelGGL(kgetf2, dim3(1), dim3(256), NB * NB * sizeof(T), st, A, lda, k0, nb, ipiv, lmax);
+    T* Z = W.Uinv + (size_t)(k0 / NB) * NB * NB;
+    T* Y = W.Linv + (size_t)(k0 / NB) * NB * NB;
+    hipLaunchKernelGGL((k_triu_inv<T, NB>), dim3(1), dim3(NB), 0, st, A, lda, k0, nb, Z);
     if (m2 > 0) {
-      hipLaunchKernelGGL((k_triu_inv<T, NB>), dim3(1), dim3(NB), 0, st, A, lda, k0, nb, Z);
       T* A21 = A + (size_t)(k0 + nb) * lda + k0;
       e = gemm<T>(m2, nb, nb, T(1), A21, lda, Z, NB, T(0), U12, (size_t)nb, st);
       if (e != hipSuccess) return e;
@@ -633,11 +758,13 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
       size_t R = ((size_t)m + G - 1) / G;
       e = hipLaunchCooperativeKernel((const void*)kpanel, dim3(G), dim3(256), args, (unsigned)(R * NB * sizeof(T)), st);
       if (e != hipSuccess) return e;
+      any_pivot = true;
+      hipLaunchKernelGGL((k_triu_inv<T, NB>), dim3(1), dim3(NB), 0, st, A, lda, k0, nb, Z);
     }
     if (pivoted && N - nb > 0) hipLaunchKernelGGL(k_laswp<T>, dim3(cdiv(N - nb, 256)), dim3(256), 0, st, A, lda, N, k0, nb, ipiv);
     const int n2 = N - k0 - nb;
+    hipLaunchKernelGGL((k_trinv<T, NB>), dim3(1), dim3(NB), 0, st, A, lda, k0, nb, Y);
     if (n2 > 0) {
-      hipLaunchKernelGGL((k_trinv<T, NB>), dim3(1), dim3(NB), 0, st, A, lda, k0, nb, Y);
       T* A12 = A + (size_t)k0 * lda + k0 + nb;
       // U12 = inv(L11) * A12
       e = gemm<T>(nb, n2, nb, T(1), Y, NB, A12, lda, T(0), U12, (size_t)n2, st);
@@ -651,30 +778,37 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
     }
   }
   hipLaunchKernelGGL(k_diag_check<T>, dim3(1), dim3(1024), 0, st, A, lda, N, info_dev);
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, W.pivoted, any_pivot ? 1 : 0);
+  if (any_pivot) {
+    if ((size_t)N * sizeof(int) > 150 * 1024) return hipErrorInvalidValue;
+    e = hipFuncSetAttribute((const void*)k_perm, hipFuncAttributeMaxDynamicSharedMemorySize, (int)((size_t)N * sizeof(int)));
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_perm, dim3(1), dim3(256), (size_t)N * sizeof(int), st, ipiv, N, W.perm);
+  }
   return hipGetLastError();
 }
 template hipError_t lu_factor<double>(double*, size_t, int, int*, void*, int*, hipStream_t, int);
 template hipError_t lu_factor<float>(float*, size_t, int, int*, void*, int*, hipStream_t, int);
 
 template <typename T>
-hipError_t lu_solve(const T* LU, size_t lda, int N, const int* ipiv, double* X, int nrhs, hipStream_t st) {
+hipError_t lu_solve(const T* LU, size_t lda, int N, const int* ipiv, double* X, int nrhs, const void* ws, hipStream_t st) {
   if (N <= 0 || nrhs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_apply_pivots, dim3(cdiv(nrhs, 64)), dim3(64), 0, st, X, N, nrhs, ipiv);
-  for (int b0 = 0; b0 < N; b0 += TSB) {
-    int nb = min(TSB, N - b0);
-    hipLaunchKernelGGL((k_trsv_diag<T, true>), dim3(1), dim3(64), 0, st, LU, lda, N, b0, nb, X, nrhs);
-    if (b0 + nb < N)
-      hipLaunchKernelGGL(k_trsv_update<T>, dim3(cdiv(N - b0 - nb, 4)), dim3(256), 0, st, LU, lda, N, b0, nb, b0 + nb, N, X, nrhs);
-  }
-  for (int b0 = ((N - 1) / TSB) * TSB; b0 >= 0; b0 -= TSB) {
-    int nb = min(TSB, N - b0);
-    hipLaunchKernelGGL((k_trsv_diag<T, false>), dim3(1), dim3(64), 0, st, LU, lda, N, b0, nb, X, nrhs);
-    if (b0 > 0) hipLaunchKernelGGL(k_trsv_update<T>, dim3(cdiv(b0, 4)), dim3(256), 0, st, LU, lda, N, b0, nb, 0, b0, X, nrhs);
+  if (nrhs > 64) return hipErrorInvalidValue;
+  const LuWorkspace<T> W(const_cast<void*>(ws), N);
+  const int nblk = (N + LU_NB - 1) / LU_NB;
+  // the right-hand sides are read through the permutation into a copy
+  hipMemcpyAsync(W.rhs, X, sizeof(double) * (size_t)nrhs * N, hipMemcpyDeviceToDevice, st);
+  for (int r = 0; r < nrhs; r++) {
+    hipMemsetAsync(W.flags, 0, sizeof(int) * (2 * (size_t)nblk + 2), st);
+    hipLaunchKernelGGL((k_trsv_blocked<T, true>), dim3(nblk), dim3(256), 0, st, LU, lda, N, W.perm, W.pivoted, W.rhs, X, r, W.Linv,
+                       W.flags, W.flags + 2 * nblk, W.err);
+    hipLaunchKernelGGL((k_trsv_blocked<T, false>), dim3(nblk), dim3(256), 0, st, LU, lda, N, (const int*)nullptr,
+                       (const int*)nullptr, X, X, r, W.Uinv, W.flags + nblk, W.flags + 2 * nblk + 1, W.err);
   }
   return hipGetLastError();
 }
-template hipError_t lu_solve<double>(const double*, size_t, int, const int*, double*, int, hipStream_t);
-template hipError_t lu_solve<float>(const float*, size_t, int, const int*, double*, int, hipStream_t);
+template hipError_t lu_solve<double>(const double*, size_t, int, const int*, double*, int, const void*, hipStream_t);
+template hipError_t lu_solve<float>(const float*, size_t, int, const int*, double*, int, const void*, hipStream_t);
 
 hipError_t residual(const double* A, size_t lda, int N, const double* B, const double* X, double* R, int nrhs,
                     unsigned long long* rmax, hipStream_t st) {

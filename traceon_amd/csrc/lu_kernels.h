@@ -16,9 +16,10 @@ template <typename T> size_t lu_workspace_bytes(int N);
 // receives 1 + the first zero pivot column, or stays 0.
 template <typename T>
 hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev, hipStream_t st, int panel_groups);
-// X (nrhs x N, row per right-hand side) <- A^{-1} X using the factors
+// X (nrhs x N, row per right-hand side, nrhs <= 64) <- A^{-1} X using the factors and the
+// workspace of the same factorisation (block inverses, permutation)
 template <typename T>
-hipError_t lu_solve(const T* LU, size_t lda, int N, const int* ipiv, double* X, int nrhs, hipStream_t st);
+hipError_t lu_solve(const T* LU, size_t lda, int N, const int* ipiv, double* X, int nrhs, const void* ws, hipStream_t st);
 hipError_t residual(const double* A, size_t lda, int N, const double* B, const double* X, double* R, int nrhs,
                     unsigned long long* rmax, hipStream_t st);
 hipError_t axpy(double* X, const double* D, size_t n, hipStream_t st);

@@ -286,114 +286,6 @@ __global__ __launch_bounds__(64) void k_trace(TraceParams P) {
   }
 }
 
-// ---------------------------------------------------------------------------------
-// Wave-per-particle variant for expensive fields (direct BEM sums over 10^5 ring sources):
-// the 64 lanes of a wave split the source sum of every field evaluation and combine it
-// with shuffles; the RK arithmetic is repeated by every lane on identical data, so the
-// step sequence is the same as in the lane-per-particle kernel.  A wave takes the next
-// particle from the counter when its particle retires.
-// ---------------------------------------------------------------------------------
-__device__ __forceinline__ double wave_sum64(double v) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-  return __shfl(v, 0, 64);   // every lane continues with lane 0's bits
-}
-
-__global__ __launch_bounds__(64) void k_trace_wave(TraceParams P) {
-  const int lane = threadIdx.x & 63;
-  while (true) {
-    int id = 0;
-    if (lane == 0) id = atomicAdd(P.counter, 1);
-    id = __shfl(id, 0, 64);
-    if (id >= P.n) break;
-    double y[6];
-    for (int c = 0; c < 6; c++) y[c] = P.state0[6 * (size_t)id + c];
-    double t = P.t0 ? P.t0[id] : 0.0;
-    const double qm = P.qm[P.qm_stride ? id : 0];
-    const double hmax = STEP_MAX / sqrt(y[3] * y[3] + y[4] * y[4] + y[5] * y[5]);
-    double h = (P.h0 && P.h0[id] > 0.0) ? P.h0[id] : hmax;
-    long long nacc = 0, natt = 0, traj_n = 1;
-    double kprev = signed_dist(P, y), land[6] = {0, 0, 0, 0, 0, 0};
-    int found = 0, st = STATUS_LEFT_BOUNDS;
-    if (P.traj && lane == 0) {
-      for (int c = 0; c < 6; c++) P.traj[(size_t)id * P.traj_cap * 6 + c] = y[c];
-      if (P.times) P.times[(size_t)id * P.traj_cap] = t;
-    }
-    while (inside_trace_bounds(P.bounds, y)) {
-      double k[6][6];
-      for (int s = 0; s < 6; s++) {
-        double ys[6];
-        for (int c = 0; c < 6; c++) {
-          double acc = y[c];
-          for (int j = 0; j < s; j++) acc = fma(RK_B[s][j], k[j][c], acc);
-          ys[c] = acc;
-        }
-        double E[3], H[3];
-        eval_field(P.field, ys, E, H, lane, 64, lane == 0);
-        for (int i = 0; i < 3; i++) { E[i] = wave_sum64(E[i]); H[i] = wave_sum64(H[i]); }
-        const double vxh0 = ys[4] * H[2] - ys[5] * H[1];
-        const double vxh1 = ys[5] * H[0] - ys[3] * H[2];
-        const double vxh2 = ys[3] * H[1] - ys[4] * H[0];
-        k[s][0] = h * ys[3]; k[s][1] = h * ys[4]; k[s][2] = h * ys[5];
-        k[s][3] = h * qm * (E[0] + MU0 * vxh0);
-        k[s][4] = h * qm * (E[1] + MU0 * vxh1);
-        k[s][5] = h * qm * (E[2] + MU0 * vxh2);
-      }
-      double ep = 0.0, ev = 0.0;
-      for (int c = 0; c < 6; c++) {
-        double e = 0.0;
-        for (int s = 0; s < 6; s++) e = fma(RK_CT[s], k[s][c], e);
-        e = fabs(e);
-        if (c < 3) ep = e > ep ? e : ep;
-        else ev = e > ev ? e : ev;
-      }
-      const double err = ep + h * ev;
-      natt++;
-      bool stop = false;
-      if (err <= P.atol) {
-        double yn[6];
-        for (int c = 0; c < 6; c++)
-          yn[c] = y[c] + (RK_CH[0] * k[0][c] + RK_CH[1] * k[1][c] + RK_CH[2] * k[2][c] + RK_CH[3] * k[3][c] +
-                          RK_CH[4] * k[4][c] + RK_CH[5] * k[5][c]);
-        t += h;
-        nacc++;
-        if (P.landing) {
-          const double kn = signed_dist(P, yn);
-          if ((kn > 0) != (kprev > 0)) {
-            const double diff = kprev - kn;
-            const double f = -kn / diff, fp = kprev / diff;
-            for (int c = 0; c < 6; c++) land[c] = fp * yn[c] + f * y[c];
-            found = 1;
-          }
-          kprev = kn;
-        }
-        for (int c = 0; c < 6; c++) y[c] = yn[c];
-        if (P.traj && lane == 0) {
-          for (int c = 0; c < 6; c++) P.traj[((size_t)id * P.traj_cap + traj_n) * 6 + c] = y[c];
-          if (P.times) P.times[(size_t)id * P.traj_cap + traj_n] = t;
-        }
-        traj_n++;
-        if (P.traj && traj_n == P.traj_cap && inside_trace_bounds(P.bounds, y)) { st = STATUS_BLOCK_FULL; stop = true; }
-      }
-      const double hn = 0.9 * h * pow(P.atol / err, 0.2);
-      h = hn < hmax ? hn : hmax;
-      if (!stop && natt >= P.max_attempts && inside_trace_bounds(P.bounds, y)) { st = STATUS_STEP_CAP; stop = true; }
-      if (stop) break;
-    }
-    if (lane == 0) {
-      for (int c = 0; c < 6; c++) P.final_state[6 * (size_t)id + c] = y[c];
-      if (P.landing)
-        for (int c = 0; c < 6; c++) P.landing[6 * (size_t)id + c] = found ? land[c] : __builtin_nan("");
-      if (P.status) P.status[id] = st;
-      if (P.n_accepted) P.n_accepted[id] = nacc;
-      if (P.n_attempted) P.n_attempted[id] = natt;
-      if (P.t_final) P.t_final[id] = t;
-      if (P.h_final) P.h_final[id] = h;
-      if (P.traj_len) P.traj_len[id] = traj_n;
-    }
-  }
-}
-
 // single field evaluation at a batch of points through a field program
 __global__ __launch_bounds__(256) void k_eval_program(FieldProgram F, const double* __restrict__ pts, int M, double* __restrict__ E,
                                                       double* __restrict__ H) {
@@ -405,17 +297,12 @@ __global__ __launch_bounds__(256) void k_eval_program(FieldProgram F, const doub
   for (int i = 0; i < 3; i++) { if (E) E[3 * (size_t)t + i] = e[i]; if (H) H[3 * (size_t)t + i] = h[i]; }
 }
 
-hipError_t launch_trace(const TraceParams& P, int waves, int wave_per_particle, hipStream_t st) {
+hipError_t launch_trace(const TraceParams& P, int waves, hipStream_t st) {
   if (P.n <= 0) return hipSuccess;
   int nw = waves > 0 ? waves : 1;
-  if (wave_per_particle) {
-    if (nw > P.n) nw = P.n;
-    hipLaunchKernelGGL(k_trace_wave, dim3(nw), dim3(64), 0, st, P);
-  } else {
-    int need = (P.n + 63) / 64;
-    if (nw > need) nw = need;
-    hipLaunchKernelGGL(k_trace, dim3(nw), dim3(64), 0, st, P);
-  }
+  const int need = (P.n + 63) / 64;
+  if (nw > need) nw = need;
+  hipLaunchKernelGGL(k_trace, dim3(nw), dim3(64), 0, st, P);
   return hipGetLastError();
 }
 

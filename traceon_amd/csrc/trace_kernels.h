@@ -67,9 +67,8 @@ struct TraceParams {
   long long* traj_len;
 };
 
-// wave_per_particle = 0: one lane per particle (cheap fields, millions of particles);
-// 1: one 64-lane wave per particle with the source sums split over the lanes (direct BEM).
-hipError_t launch_trace(const TraceParams& P, int waves, int wave_per_particle, hipStream_t st);
+// One lane per particle; `waves` 64-lane workgroups drain the particle queue.
+hipError_t launch_trace(const TraceParams& P, int waves, hipStream_t st);
 hipError_t launch_eval_program(const FieldProgram& F, const double* pts, int M, double* E, double* H, hipStream_t st);
 
 }  // namespace tb

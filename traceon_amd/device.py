@@ -117,6 +117,7 @@ class LU:
     ipiv: torch.Tensor
     info: int
     fp32: bool
+    ws: torch.Tensor   # block inverses and permutation used by the solves
 
 
 def lu_factor(A: torch.Tensor, overwrite: bool = True) -> LU:
@@ -132,7 +133,7 @@ def lu_factor(A: torch.Tensor, overwrite: bool = True) -> LU:
     ws = torch.empty(_L.tb_lu_workspace_bytes(N, int(fp32)), dtype=torch.uint8, device=dev)
     f = _L.tb_lu_factor_f32 if fp32 else _L.tb_lu_factor
     check(f(ptr(A), N, N, ptr(ipiv), ptr(ws), ptr(info), stream_ptr()), "lu_factor")
-    lu = LU(A, ipiv, int(info.item()), fp32)
+    lu = LU(A, ipiv, int(info.item()), fp32, ws)
     if lu.info:
         # numpy.linalg.solve (solver.py:173) raises LinAlgError on an exactly singular matrix
         raise np.linalg.LinAlgError(f"Singular matrix (zero or NaN pivot in column {lu.info - 1})")
@@ -143,9 +144,11 @@ def lu_solve(lu: LU, X: torch.Tensor) -> torch.Tensor:
     """In-place X (nrhs, N) FP64 <- A^{-1} X."""
     assert X.dtype == F64 and X.is_contiguous()
     N = lu.A.shape[0]
-    nrhs = X.shape[0] if X.dim() == 2 else 1
+    X2 = X.view(-1, N)
     f = _L.tb_lu_solve_f32 if lu.fp32 else _L.tb_lu_solve
-    check(f(ptr(lu.A), N, N, ptr(lu.ipiv), ptr(X), nrhs, stream_ptr()), "lu_solve")
+    for r0 in range(0, X2.shape[0], 64):
+        blk = X2[r0:r0 + 64]
+        check(f(ptr(lu.A), N, N, ptr(lu.ipiv), ptr(blk), blk.shape[0], ptr(lu.ws), stream_ptr()), "lu_solve")
     return X
 
 

@@ -177,7 +177,7 @@ class BatchResult:
 
 
 def run_trace(program: DeviceFieldProgram, state0, qm, bounds, atol=1e-8, plane=None, traj_cap=0, t0=None,
-              max_attempts=1 << 62, waves=0, wave_per_particle=None):
+              max_attempts=1 << 62, waves=0):
     """Persistent GPU tracer over many particles (all arrays numpy or device tensors)."""
     import torch
     from . import device
@@ -229,9 +229,7 @@ def run_trace(program: DeviceFieldProgram, state0, qm, bounds, atol=1e-8, plane=
         times = torch.empty((n, traj_cap), dtype=torch.float64, device=dev)
         tlen = torch.empty(n, dtype=torch.int64, device=dev)
         P.traj, P.times, P.traj_cap, P.traj_len = traj.data_ptr(), times.data_ptr(), int(traj_cap), tlen.data_ptr()
-    if wave_per_particle is None:
-        wave_per_particle = program.has_sources
-    backend.check(backend.backend_lib.tb_trace(C.byref(P), int(waves), int(wave_per_particle), device.stream_ptr()), "trace")
+    backend.check(backend.backend_lib.tb_trace(C.byref(P), int(waves), device.stream_ptr()), "trace")
     out = BatchResult(final, landing, status, nacc, natt, tfin)
     out.traj, out.times, out.traj_len = traj, times, tlen
     out._keep = (S0, QM, T0, counter)
