@@ -10,9 +10,11 @@
 //                  owner of the next diagonal row publishes that row, so after the
 //                  barrier everyone can swap and update without a second exchange.
 //   2. k_laswp   : the panel's row interchanges applied to the other columns.
-//   3. k_trinv   : inverse of the unit-lower NB x NB diagonal block.
+//   3. k_diag_block : inverses of the unit-lower and upper triangles of the NB x NB
+//                  diagonal block (they also serve the triangular solves).
 //   4. gemm      : U12 = inv(L11) * A12  (into a workspace), A22 -= L21 * U12 (MFMA),
 //                  U12 copied back.
+// Before step 1 each panel first tries the unpivoted fast path described below.
 // Pivot choice is LAPACK's (largest |a|, first index on ties).
 #include <hip/hip_runtime.h>
 #include <hip/hip_cooperative_groups.h>
@@ -175,86 +177,136 @@ __global__ __launch_bounds__(256) void k_laswp(T* __restrict__ A, size_t lda, in
   }
 }
 
-// Inverse of the unit-lower nb x nb block L11 at (k0,k0): Y = inv(L11), Y row-major NB x NB.
-// Y[r][c] = delta_rc - sum_{s<r} L[r][s] Y[s][c]; thread c owns column c and walks the rows
-// in order (no cross-thread dependence), L[r][s] is the same for every thread.
-template <typename T, int NB>
-__global__ __launch_bounds__(NB) void k_trinv(const T* __restrict__ A, size_t lda, int k0, int nb, T* __restrict__ Y) {
-  const int c = threadIdx.x;
-  for (int r = 0; r < NB; r++) {
-    T acc = (r == c) ? T(1) : T(0);
-    if (r < nb && c < nb) {
-      const T* Lr = A + (size_t)(k0 + r) * lda + k0;
-      for (int s = c; s < r; s++) acc -= Lr[s] * Y[(size_t)s * NB + c];
-    } else {
-      acc = T(0);
-    }
-    Y[(size_t)r * NB + c] = acc;
-  }
-}
-
 // ---------------------------------------------------------------------------------
 // No-pivot fast path of one panel.  Partial pivoting leaves every row in place exactly
 // when each multiplier satisfies |l| <= 1 (the diagonal entry is then the largest of its
 // column, ties going to the diagonal as in LAPACK).  So the panel is first factored
-// without interchanges; if every computed |l| <= 1 the result IS the partially pivoted
-// factorisation (ipiv = identity) and no grid-wide pivot search is needed.  Otherwise the
-// panel is restored from its copy and factored by k_panel.  Pure potential (Dirichlet)
-// BEM rows never need a swap; rows of flux type (dielectric / magnetizable) usually do.
+// without interchanges (the diagonal block by k_diag_block, the rows below it as
+// L21 = A21 * inv(U11) on MFMA); if every computed |l| <= 1 the result is the partially
+// pivoted factorisation (ipiv = identity, up to the rounding of the multipliers) and no
+// grid-wide pivot search is needed.  Otherwise the panel is restored from its copy and
+// factored by k_panel.  Pure potential (Dirichlet) BEM rows never need a swap; rows of
+// flux type (dielectric / magnetizable) usually do.
 // ---------------------------------------------------------------------------------
 
-// (a) unpivoted LU of the nb x nb top block in LDS (one workgroup); max |l| -> lmax bits
-template <typename T, int NB>
-__global__ __launch_bounds__(256) void k_getf2_nopiv(T* __restrict__ A, size_t lda, int k0, int nb, int* __restrict__ ipiv,
-                                                     unsigned long long* __restrict__ lmax) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* B = reinterpret_cast<T*>(smem_raw);   // nb x NB
-  const int tid = threadIdx.x;
-  for (int e = tid; e < nb * NB; e += 256) {
-    int r = e / NB, c = e % NB;
-    B[e] = c < nb ? A[(size_t)(k0 + r) * lda + k0 + c] : T(0);
+// Diagonal block of a panel in one workgroup of 1024 threads holding the NB x NB block in
+// registers (thread t: column t % NB, rows t / NB + 8k, k = 0..15); outside nb x nb the
+// block is padded with the identity.
+//   factor = 1: unpivoted LU of the block (fast path).  Multipliers are formed with the
+//     reciprocal of the pivot, as LAPACK's dgetf2 scales its column; max |l| -> lmax.
+//   Then, always: Y = inv(L11) (unit lower) and Z = inv(U11), row-major NB x NB with zero
+//   padding.  L is inverted forwards and U backwards in the same sweep.
+// Every step publishes one row and one column through LDS, double-buffered by step
+// parity, so a step costs one barrier.
+template <typename T>
+__global__ __launch_bounds__(1024) void k_diag_block(T* __restrict__ A, size_t lda, int k0, int nb, int factor,
+                                                    int* __restrict__ ipiv, T* __restrict__ Y, T* __restrict__ Z,
+                                                    unsigned long long* __restrict__ lmax) {
+  constexpr int NB = LU_NB, RG = 1024 / NB, KR = NB / RG;
+  __shared__ T rowb[2][NB], colb[2][NB], rowc[2][NB], colc[2][NB];
+  __shared__ T dg[NB];
+  __shared__ double wmax[1024 / 64];
+  const int tid = threadIdx.x, c = tid & (NB - 1), rg = tid / NB;
+  T B[KR];
+#pragma unroll
+  for (int k = 0; k < KR; k++) {
+    const int r = rg + RG * k;
+    B[k] = (r < nb && c < nb) ? A[(size_t)(k0 + r) * lda + k0 + c] : (r == c ? T(1) : T(0));
   }
-  __syncthreads();
   double m = 0.0;
-  for (int j = 0; j < nb; j++) {
-    const T piv = B[j * NB + j];
-    for (int r = j + 1 + tid; r < nb; r += 256) {
-      T l = B[r * NB + j] / piv;
-      B[r * NB + j] = l;
-      double al = (double)tabs(l);
-      m = (al > m || al != al) ? al : m;
+  if (factor) {
+    for (int j = 0; j < nb; j++) {
+      const int p = j & 1;
+      if (rg == j % RG) {
+#pragma unroll
+        for (int k = 0; k < KR; k++)
+          if (k == j / RG) rowb[p][c] = B[k];
+      }
+      if (c == j) {
+#pragma unroll
+        for (int k = 0; k < KR; k++) colb[p][rg + RG * k] = B[k];
+      }
+      __syncthreads();
+      const T rp = T(1) / rowb[p][j];
+#pragma unroll
+      for (int k = 0; k < KR; k++) {
+        const int r = rg + RG * k;
+        if (r > j) {
+          const T l = colb[p][r] * rp;
+          if (c == j) {
+            B[k] = l;
+            const double al = (double)tabs(l);
+            m = (al > m || al != al) ? al : m;
+          } else if (c > j) {
+            B[k] = fma(-l, rowb[p][c], B[k]);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < KR; k++) {
+      const int r = rg + RG * k;
+      if (r < nb && c < nb) A[(size_t)(k0 + r) * lda + k0 + c] = B[k];
+    }
+    if (tid < nb) ipiv[k0 + tid] = k0 + tid;
+    for (int o = 32; o >= 1; o >>= 1) {
+      const double x = __shfl_xor(m, o, 64);
+      m = (x > m || x != x) ? x : m;
+    }
+    if ((tid & 63) == 0) wmax[tid >> 6] = m;
+  }
+#pragma unroll
+  for (int k = 0; k < KR; k++)
+    if (rg + RG * k == c) dg[c] = B[k];
+  __syncthreads();
+  if (factor && tid == 0) {
+    double mm = 0.0;
+    for (int w = 0; w < 1024 / 64; w++) mm = (wmax[w] > mm || wmax[w] != wmax[w]) ? wmax[w] : mm;
+    if (mm != mm) mm = 1e300;
+    atomicMax(lmax, (unsigned long long)__double_as_longlong(mm));
+  }
+  T Yv[KR], Zv[KR];
+#pragma unroll
+  for (int k = 0; k < KR; k++) {
+    const int r = rg + RG * k;
+    Yv[k] = (r == c) ? T(1) : T(0);
+    Zv[k] = (r == c) ? T(1) : T(0);
+  }
+  for (int st = 0; st < NB; st++) {
+    const int p = st & 1;
+    const int jl = st, ju = NB - 1 - st;
+    if (rg == jl % RG) {
+#pragma unroll
+      for (int k = 0; k < KR; k++)
+        if (k == jl / RG) rowb[p][c] = Yv[k];
+    }
+    if (c == jl) {
+#pragma unroll
+      for (int k = 0; k < KR; k++) colb[p][rg + RG * k] = B[k];
+    }
+    if (rg == ju % RG) {
+#pragma unroll
+      for (int k = 0; k < KR; k++)
+        if (k == ju / RG) { Zv[k] = Zv[k] / dg[ju]; rowc[p][c] = Zv[k]; }
+    }
+    if (c == ju) {
+#pragma unroll
+      for (int k = 0; k < KR; k++) colc[p][rg + RG * k] = B[k];
     }
     __syncthreads();
-    const int n = nb - j - 1;
-    for (int e = tid; e < n * n; e += 256) {
-      int r = j + 1 + e / n, c = j + 1 + e % n;
-      B[r * NB + c] -= B[r * NB + j] * B[j * NB + c];
+#pragma unroll
+    for (int k = 0; k < KR; k++) {
+      const int r = rg + RG * k;
+      if (r > jl) Yv[k] = fma(-colb[p][r], rowb[p][c], Yv[k]);
+      if (r < ju) Zv[k] = fma(-colc[p][r], rowc[p][c], Zv[k]);
     }
-    __syncthreads();
-    if (tid == 0) ipiv[k0 + j] = k0 + j;
   }
-  for (int e = tid; e < nb * nb; e += 256) {
-    int r = e / nb, c = e % nb;
-    A[(size_t)(k0 + r) * lda + k0 + c] = B[r * NB + c];
-  }
-  if (m != m) m = 1e300;
-  atomicMax(lmax, (unsigned long long)__double_as_longlong(m));
-}
-
-// (b) inverse of the upper-triangular U11 (non-unit diagonal): Z = inv(U11), row-major NB x NB.
-// Column c of Z solves U z = e_c from the bottom up; thread c owns column c.
-template <typename T, int NB>
-__global__ __launch_bounds__(NB) void k_triu_inv(const T* __restrict__ A, size_t lda, int k0, int nb, T* __restrict__ Z) {
-  const int c = threadIdx.x;
-  for (int r = NB - 1; r >= 0; r--) {
-    T acc = T(0);
-    if (r < nb && c < nb && r <= c) {
-      const T* Ur = A + (size_t)(k0 + r) * lda + k0;
-      acc = (r == c) ? T(1) : T(0);
-      for (int s = r + 1; s <= c; s++) acc -= Ur[s] * Z[(size_t)s * NB + c];
-      acc /= Ur[r];
-    }
-    Z[(size_t)r * NB + c] = acc;
+#pragma unroll
+  for (int k = 0; k < KR; k++) {
+    const int r = rg + RG * k;
+    const bool in = r < nb && c < nb;
+    Y[(size_t)r * NB + c] = in ? Yv[k] : T(0);
+    Z[(size_t)r * NB + c] = in ? Zv[k] : T(0);
   }
 }
 
@@ -277,17 +329,28 @@ __global__ __launch_bounds__(1024) void k_diag_check(const T* __restrict__ A, si
 template <typename T>
 __global__ __launch_bounds__(256) void k_copy_back_max(const T* __restrict__ S, size_t lds, T* __restrict__ D, size_t ldd, int rows,
                                                        int cols, unsigned long long* __restrict__ vmax) {
-  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double wm[4];
+  const size_t total = (size_t)rows * cols;
   double m = 0.0;
-  if (e < (size_t)rows * cols) {
-    int r = (int)(e / cols), c = (int)(e % cols);
-    T v = S[(size_t)r * lds + c];
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const int r = (int)(e / cols), c = (int)(e % cols);
+    const T v = S[(size_t)r * lds + c];
     D[(size_t)r * ldd + c] = v;
-    m = (double)tabs(v);
-    if (m != m) m = 1e300;
+    const double a = (double)tabs(v);
+    m = (a > m || a != a) ? a : m;
   }
-  for (int o = 32; o >= 1; o >>= 1) { double x = __shfl_xor(m, o, 64); m = x > m ? x : m; }
-  if ((threadIdx.x & 63) == 0 && m > 0.0) atomicMax(vmax, (unsigned long long)__double_as_longlong(m));
+  for (int o = 32; o >= 1; o >>= 1) {
+    const double x = __shfl_xor(m, o, 64);
+    m = (x > m || x != x) ? x : m;
+  }
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double mm = 0.0;
+    for (int w = 0; w < 4; w++) mm = (wm[w] > mm || wm[w] != wm[w]) ? wm[w] : mm;
+    if (mm != mm) mm = 1e300;
+    if (mm > 0.0) atomicMax(vmax, (unsigned long long)__double_as_longlong(mm));
+  }
 }
 
 // ---------------------------------------------------------------------------------
@@ -487,24 +550,42 @@ __global__ __launch_bounds__(256) void k_perm(const int* __restrict__ ipiv, int 
   for (int i = threadIdx.x; i < N; i += blockDim.x) perm[i] = pl[i];
 }
 
-constexpr long long SPIN_LIMIT = 1ll << 26;
+constexpr long long SPIN_LIMIT = 1ll << 22;   // polls of a flag before a wait gives up (err set)
 
 __global__ void k_set_int(int* p, int v) { *p = v; }
 
+// Blocked triangular solve with the stored diagonal-block inverses, one workgroup per
+// block row of 128.  Workgroups take block rows in dependency order from a ticket
+// counter (so a waiting workgroup never blocks one that has not started) and walk their
+// dependencies in the order they complete: forward for L, backward for U.  Per
+// dependency block: wave w accumulates rows 32w..32w+31, lanes across the 128 columns.
+// A workgroup polls the flag of the next block it needs and, once set, also counts the
+// blocks behind it that are already done, so one acquire fence covers the whole run.
+// The block's inverse is staged in LDS while the dependencies are summed; the row sums
+// are reduced through LDS; x_b = inv(D_b) (b_b - sum) is one LDS matvec.
+constexpr int TRSV_PAD = LU_NB + 1;   // LDS row stride of the staged inverse (conflict-free)
+constexpr int TRSV_RED = 16;          // partial sums per row left after two shuffle steps
+template <typename T>
+constexpr size_t trsv_lds_bytes() {
+  return sizeof(T) * LU_NB * TRSV_PAD + sizeof(double) * LU_NB * (TRSV_RED + 1) + sizeof(double) * LU_NB;
+}
+
 template <typename T, bool LOWER>
 __global__ __launch_bounds__(256) void k_trsv_blocked(const T* __restrict__ LU, size_t lda, int N, const int* __restrict__ perm,
-                                                      const int* __restrict__ pivoted, const double* __restrict__ Bv,
-                                                      double* __restrict__ X, int rhs, const T* __restrict__ Dinv,
+                                                      const int* __restrict__ pivoted, const double* Bv,   // Bv may be X
+                                                      double* X, int rhs, const T* __restrict__ Dinv,
                                                       int* __restrict__ flags, int* __restrict__ counter, int* __restrict__ err) {
   constexpr int NB = LU_NB;
   constexpr int RPW = NB / 4;   // rows per wave
-  __shared__ double xs[NB];
-  __shared__ double vs[NB];
-  __shared__ int s_b;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* Ds = reinterpret_cast<T*>(smem_raw);                                        // NB x TRSV_PAD
+  double* part = reinterpret_cast<double*>(smem_raw + sizeof(T) * NB * TRSV_PAD);   // NB x (TRSV_RED + 1)
+  double* vs = part + NB * (TRSV_RED + 1);                                         // NB
+  __shared__ int s_b, s_ready[2];
   const int nblk = (N + NB - 1) / NB;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (tid == 0) {
-    int t = atomicAdd(counter, 1);
+    const int t = atomicAdd(counter, 1);
     s_b = LOWER ? t : nblk - 1 - t;
   }
   __syncthreads();
@@ -512,25 +593,38 @@ __global__ __launch_bounds__(256) void k_trsv_blocked(const T* __restrict__ LU, 
   if (b < 0 || b >= nblk) return;
   const int r0 = b * NB;
   const int nr = min(NB, N - r0);
+  // stage inv(D_b) (independent of every other block)
+  const T* Db = Dinv + (size_t)b * NB * NB;
+  for (int e = tid; e < NB * NB; e += 256) Ds[(e / NB) * TRSV_PAD + (e % NB)] = Db[e];
   double acc[RPW];
 #pragma unroll
   for (int i = 0; i < RPW; i++) acc[i] = 0.0;
   const double* xr = X + (size_t)rhs * N;
-  volatile const double* xv = xr;
-  const int jfirst = LOWER ? 0 : b + 1, jlast = LOWER ? b - 1 : nblk - 1;
-  for (int jb = jfirst; jb <= jlast; jb++) {
-    if (tid == 0) {
-      long long spins = 0;
-      while (__atomic_load_n(flags + jb, __ATOMIC_ACQUIRE) == 0) {
-        if (++spins > SPIN_LIMIT) { atomicOr(err, 1); break; }
-        __builtin_amdgcn_s_sleep(1);
+  const int ndep = LOWER ? b : nblk - 1 - b;
+  int ready = 0, round = 0;
+  for (int k = 0; k < ndep; k++) {
+    const int jb = LOWER ? k : nblk - 1 - k;
+    if (k >= ready) {
+      if (tid == 0) {
+        long long spins = 0;
+        while (__hip_atomic_load(flags + jb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+          if (++spins > SPIN_LIMIT) { atomicOr(err, 1); break; }
+          __builtin_amdgcn_s_sleep(1);
+        }
+        int rdy = k + 1;
+        while (rdy < ndep && __hip_atomic_load(flags + (LOWER ? rdy : nblk - 1 - rdy), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+          rdy++;
+        s_ready[round & 1] = rdy;
       }
+      __syncthreads();
+      ready = s_ready[round & 1];
+      round++;
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
-    __syncthreads();
     const int c0 = jb * NB;
     const int nc = min(NB, N - c0);
-    const double x0 = lane < nc ? xv[c0 + lane] : 0.0;
-    const double x1 = lane + 64 < nc ? xv[c0 + lane + 64] : 0.0;
+    const double x0 = lane < nc ? xr[c0 + lane] : 0.0;
+    const double x1 = lane + 64 < nc ? xr[c0 + lane + 64] : 0.0;
 #pragma unroll
     for (int i = 0; i < RPW; i++) {
       const int row = r0 + wave * RPW + i;
@@ -542,35 +636,38 @@ __global__ __launch_bounds__(256) void k_trsv_blocked(const T* __restrict__ LU, 
       }
     }
   }
-  // v = b - sum, then x_b = inv(diagonal block) v
+  // row sums: two shuffle steps leave 16 partials per row, summed from LDS in fixed order
 #pragma unroll
   for (int i = 0; i < RPW; i++) {
     double v = acc[i];
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    const int lr = wave * RPW + i;
-    if (lane == 0 && lr < NB) {
-      const int row = r0 + lr;
-      double bv = 0.0;
-      const bool use_perm = perm && pivoted && *pivoted;
-      if (row < N) bv = LOWER ? Bv[(size_t)rhs * N + (use_perm ? perm[row] : row)] : Bv[(size_t)rhs * N + row];
-      vs[lr] = (row < N) ? bv - v : 0.0;
-    }
+    v += __shfl_xor(v, 32, 64);
+    v += __shfl_xor(v, 16, 64);
+    if (lane < TRSV_RED) part[(wave * RPW + i) * (TRSV_RED + 1) + lane] = v;
   }
   __syncthreads();
-  const T* Db = Dinv + (size_t)b * NB * NB;
+  if (tid < NB) {
+    double ssum = 0.0;
 #pragma unroll
-  for (int i = 0; i < RPW; i++) {
-    const int lr = wave * RPW + i;
-    double p = (double)Db[(size_t)lr * NB + lane] * vs[lane];
-    p = fma((double)Db[(size_t)lr * NB + lane + 64], vs[lane + 64], p);
-    for (int o = 32; o >= 1; o >>= 1) p += __shfl_xor(p, o, 64);
-    if (lane == 0) xs[lr] = p;
+    for (int l = 0; l < TRSV_RED; l++) ssum += part[tid * (TRSV_RED + 1) + l];
+    const int row = r0 + tid;
+    double bv = 0.0;
+    if (row < N) {
+      const bool use_perm = LOWER && perm && pivoted && *pivoted;
+      bv = Bv[(size_t)rhs * N + (use_perm ? perm[row] : row)];
+    }
+    vs[tid] = row < N ? bv - ssum : 0.0;
   }
   __syncthreads();
-  if (tid < nr) X[(size_t)rhs * N + r0 + tid] = xs[tid];
+  if (tid < nr) {
+    const T* Dr = Ds + tid * TRSV_PAD;
+    double xv = 0.0;
+#pragma unroll 8
+    for (int c = 0; c < NB; c++) xv = fma((double)Dr[c], vs[c], xv);
+    X[(size_t)rhs * N + r0 + tid] = xv;
+  }
   __threadfence();
   __syncthreads();
-  if (tid == 0) __atomic_store_n(flags + b, 1, __ATOMIC_RELEASE);
+  if (tid == 0) __hip_atomic_store(flags + b, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // residual r = b - A x with the FP64 matrix, accumulated in double-double: each product
@@ -706,9 +803,6 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
   e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kpanel, 256, Rmax * NB * sizeof(T));
   if (e != hipSuccess) return e;
   if (occ * prop.multiProcessorCount < G) return hipErrorCooperativeLaunchTooLarge;
-  auto kgetf2 = k_getf2_nopiv<T, NB>;
-  e = hipFuncSetAttribute((const void*)kgetf2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(NB * NB * sizeof(T)));
-  if (e != hipSuccess) return e;
 
   LuWorkspace<T> W(ws, N);
   double* wsd = W.cand_val;
@@ -729,16 +823,15 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
     hipLaunchKernelGGL(k_copy2d<T>, dim3(cdiv((size_t)(N - k0) * nb, 256)), dim3(256), 0, st, A + (size_t)k0 * lda + k0, lda, Pbak,
                        (size_t)nb, N - k0, nb);
     hipMemsetAsync(lmax, 0, sizeof(unsigned long long), st);
-    hipLaunchKernelGGL(kgetf2, dim3(1), dim3(256), NB * NB * sizeof(T), st, A, lda, k0, nb, ipiv, lmax);
     T* Z = W.Uinv + (size_t)(k0 / NB) * NB * NB;
     T* Y = W.Linv + (size_t)(k0 / NB) * NB * NB;
-    hipLaunchKernelGGL((k_triu_inv<T, NB>), dim3(1), dim3(NB), 0, st, A, lda, k0, nb, Z);
+    hipLaunchKernelGGL(k_diag_block<T>, dim3(1), dim3(1024), 0, st, A, lda, k0, nb, 1, ipiv, Y, Z, lmax);
     if (m2 > 0) {
       T* A21 = A + (size_t)(k0 + nb) * lda + k0;
       e = gemm<T>(m2, nb, nb, T(1), A21, lda, Z, NB, T(0), U12, (size_t)nb, st);
       if (e != hipSuccess) return e;
-      hipLaunchKernelGGL(k_copy_back_max<T>, dim3(cdiv((size_t)m2 * nb, 256)), dim3(256), 0, st, U12, (size_t)nb, A21, lda, m2, nb,
-                         lmax);
+      const size_t nblk_copy = std::min<size_t>(cdiv((size_t)m2 * nb, 256), 1024);
+      hipLaunchKernelGGL(k_copy_back_max<T>, dim3(nblk_copy), dim3(256), 0, st, U12, (size_t)nb, A21, lda, m2, nb, lmax);
     }
     unsigned long long lm = 0;
     e = hipMemcpyAsync(&lm, lmax, sizeof(lm), hipMemcpyDeviceToHost, st);
@@ -759,11 +852,10 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
       e = hipLaunchCooperativeKernel((const void*)kpanel, dim3(G), dim3(256), args, (unsigned)(R * NB * sizeof(T)), st);
       if (e != hipSuccess) return e;
       any_pivot = true;
-      hipLaunchKernelGGL((k_triu_inv<T, NB>), dim3(1), dim3(NB), 0, st, A, lda, k0, nb, Z);
+      hipLaunchKernelGGL(k_diag_block<T>, dim3(1), dim3(1024), 0, st, A, lda, k0, nb, 0, ipiv, Y, Z, lmax);
     }
     if (pivoted && N - nb > 0) hipLaunchKernelGGL(k_laswp<T>, dim3(cdiv(N - nb, 256)), dim3(256), 0, st, A, lda, N, k0, nb, ipiv);
     const int n2 = N - k0 - nb;
-    hipLaunchKernelGGL((k_trinv<T, NB>), dim3(1), dim3(NB), 0, st, A, lda, k0, nb, Y);
     if (n2 > 0) {
       T* A12 = A + (size_t)k0 * lda + k0 + nb;
       // U12 = inv(L11) * A12
@@ -798,11 +890,16 @@ hipError_t lu_solve(const T* LU, size_t lda, int N, const int* ipiv, double* X, 
   const int nblk = (N + LU_NB - 1) / LU_NB;
   // the right-hand sides are read through the permutation into a copy
   hipMemcpyAsync(W.rhs, X, sizeof(double) * (size_t)nrhs * N, hipMemcpyDeviceToDevice, st);
+  constexpr size_t lds = trsv_lds_bytes<T>();
+  hipError_t e = hipFuncSetAttribute((const void*)k_trsv_blocked<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void*)k_trsv_blocked<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
   for (int r = 0; r < nrhs; r++) {
     hipMemsetAsync(W.flags, 0, sizeof(int) * (2 * (size_t)nblk + 2), st);
-    hipLaunchKernelGGL((k_trsv_blocked<T, true>), dim3(nblk), dim3(256), 0, st, LU, lda, N, W.perm, W.pivoted, W.rhs, X, r, W.Linv,
+    hipLaunchKernelGGL((k_trsv_blocked<T, true>), dim3(nblk), dim3(256), lds, st, LU, lda, N, W.perm, W.pivoted, W.rhs, X, r, W.Linv,
                        W.flags, W.flags + 2 * nblk, W.err);
-    hipLaunchKernelGGL((k_trsv_blocked<T, false>), dim3(nblk), dim3(256), 0, st, LU, lda, N, (const int*)nullptr,
+    hipLaunchKernelGGL((k_trsv_blocked<T, false>), dim3(nblk), dim3(256), lds, st, LU, lda, N, (const int*)nullptr,
                        (const int*)nullptr, X, X, r, W.Uinv, W.flags + nblk, W.flags + 2 * nblk + 1, W.err);
   }
   return hipGetLastError();
