@@ -203,44 +203,48 @@ __global__ __launch_bounds__(1024) void k_diag_block(T* __restrict__ A, size_t l
                                                     int* __restrict__ ipiv, T* __restrict__ Y, T* __restrict__ Z,
                                                     unsigned long long* __restrict__ lmax) {
   constexpr int NB = LU_NB, RG = 1024 / NB, KR = NB / RG;
-  __shared__ T rowb[2][NB], colb[2][NB], rowc[2][NB], colc[2][NB];
-  __shared__ T dg[NB];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* Bs = reinterpret_cast<T*>(smem_raw);   // the factored block, NB x NB, for the inverse sweep
+  __shared__ T rowb[2][NB], colb[2][NB], rowc[2][NB];
   __shared__ double wmax[1024 / 64];
   const int tid = threadIdx.x, c = tid & (NB - 1), rg = tid / NB;
+  // The inner loops are written with selects only: the rows a thread holds differ by k,
+  // and a branch per k would serialise one LDS latency per row.
   T B[KR];
 #pragma unroll
   for (int k = 0; k < KR; k++) {
     const int r = rg + RG * k;
     B[k] = (r < nb && c < nb) ? A[(size_t)(k0 + r) * lda + k0 + c] : (r == c ? T(1) : T(0));
   }
+  // v[kk] for a runtime kk, as a chain of selects (keeps v in registers)
+#define TB_PICK(dst, v, kk)                                          \
+  do {                                                               \
+    dst = v[0];                                                      \
+    _Pragma("unroll") for (int q_ = 1; q_ < KR; q_++) dst = (q_ == (kk)) ? v[q_] : dst; \
+  } while (0)
   double m = 0.0;
   if (factor) {
     for (int j = 0; j < nb; j++) {
       const int p = j & 1;
-      if (rg == j % RG) {
-#pragma unroll
-        for (int k = 0; k < KR; k++)
-          if (k == j / RG) rowb[p][c] = B[k];
-      }
+      T own;
+      TB_PICK(own, B, j / RG);
+      if (rg == j % RG) rowb[p][c] = own;
       if (c == j) {
 #pragma unroll
         for (int k = 0; k < KR; k++) colb[p][rg + RG * k] = B[k];
       }
       __syncthreads();
+      const T rv = rowb[p][c];
       const T rp = T(1) / rowb[p][j];
 #pragma unroll
       for (int k = 0; k < KR; k++) {
         const int r = rg + RG * k;
-        if (r > j) {
-          const T l = colb[p][r] * rp;
-          if (c == j) {
-            B[k] = l;
-            const double al = (double)tabs(l);
-            m = (al > m || al != al) ? al : m;
-          } else if (c > j) {
-            B[k] = fma(-l, rowb[p][c], B[k]);
-          }
-        }
+        const T l = colb[p][r] * rp;
+        const T upd = fma(-l, rv, B[k]);
+        const bool below = r > j;
+        const double al = (below && c == j) ? (double)tabs(l) : 0.0;
+        m = (al > m || al != al) ? al : m;
+        B[k] = below ? (c == j ? l : (c > j ? upd : B[k])) : B[k];
       }
     }
 #pragma unroll
@@ -256,8 +260,7 @@ __global__ __launch_bounds__(1024) void k_diag_block(T* __restrict__ A, size_t l
     if ((tid & 63) == 0) wmax[tid >> 6] = m;
   }
 #pragma unroll
-  for (int k = 0; k < KR; k++)
-    if (rg + RG * k == c) dg[c] = B[k];
+  for (int k = 0; k < KR; k++) Bs[(rg + RG * k) * NB + c] = B[k];
   __syncthreads();
   if (factor && tid == 0) {
     double mm = 0.0;
@@ -265,6 +268,7 @@ __global__ __launch_bounds__(1024) void k_diag_block(T* __restrict__ A, size_t l
     if (mm != mm) mm = 1e300;
     atomicMax(lmax, (unsigned long long)__double_as_longlong(mm));
   }
+  // Y = inv(L) forwards and Z = inv(U) backwards, one row of each final per step
   T Yv[KR], Zv[KR];
 #pragma unroll
   for (int k = 0; k < KR; k++) {
@@ -275,30 +279,26 @@ __global__ __launch_bounds__(1024) void k_diag_block(T* __restrict__ A, size_t l
   for (int st = 0; st < NB; st++) {
     const int p = st & 1;
     const int jl = st, ju = NB - 1 - st;
-    if (rg == jl % RG) {
-#pragma unroll
-      for (int k = 0; k < KR; k++)
-        if (k == jl / RG) rowb[p][c] = Yv[k];
-    }
-    if (c == jl) {
-#pragma unroll
-      for (int k = 0; k < KR; k++) colb[p][rg + RG * k] = B[k];
-    }
+    T yown;
+    TB_PICK(yown, Yv, jl / RG);
+    if (rg == jl % RG) rowb[p][c] = yown;
     if (rg == ju % RG) {
+      T zown;
+      TB_PICK(zown, Zv, ju / RG);
+      zown = zown / Bs[ju * NB + ju];
 #pragma unroll
-      for (int k = 0; k < KR; k++)
-        if (k == ju / RG) { Zv[k] = Zv[k] / dg[ju]; rowc[p][c] = Zv[k]; }
-    }
-    if (c == ju) {
-#pragma unroll
-      for (int k = 0; k < KR; k++) colc[p][rg + RG * k] = B[k];
+      for (int k = 0; k < KR; k++) Zv[k] = (k == ju / RG) ? zown : Zv[k];
+      rowc[p][c] = zown;
     }
     __syncthreads();
+    const T yr = rowb[p][c], zr = rowc[p][c];
 #pragma unroll
     for (int k = 0; k < KR; k++) {
       const int r = rg + RG * k;
-      if (r > jl) Yv[k] = fma(-colb[p][r], rowb[p][c], Yv[k]);
-      if (r < ju) Zv[k] = fma(-colc[p][r], rowc[p][c], Zv[k]);
+      const T ny = fma(-Bs[r * NB + jl], yr, Yv[k]);
+      const T nz = fma(-Bs[r * NB + ju], zr, Zv[k]);
+      Yv[k] = r > jl ? ny : Yv[k];
+      Zv[k] = r < ju ? nz : Zv[k];
     }
   }
 #pragma unroll
@@ -308,6 +308,7 @@ __global__ __launch_bounds__(1024) void k_diag_block(T* __restrict__ A, size_t l
     Y[(size_t)r * NB + c] = in ? Yv[k] : T(0);
     Z[(size_t)r * NB + c] = in ? Zv[k] : T(0);
   }
+#undef TB_PICK
 }
 
 // After the factorisation: info = 1 + first column whose U diagonal is zero or NaN
@@ -804,6 +805,8 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
   if (e != hipSuccess) return e;
   if (occ * prop.multiProcessorCount < G) return hipErrorCooperativeLaunchTooLarge;
 
+  e = hipFuncSetAttribute((const void*)k_diag_block<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(NB * NB * sizeof(T)));
+  if (e != hipSuccess) return e;
   LuWorkspace<T> W(ws, N);
   double* wsd = W.cand_val;
   int* wsi = W.cand_idx;
@@ -825,7 +828,7 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
     hipMemsetAsync(lmax, 0, sizeof(unsigned long long), st);
     T* Z = W.Uinv + (size_t)(k0 / NB) * NB * NB;
     T* Y = W.Linv + (size_t)(k0 / NB) * NB * NB;
-    hipLaunchKernelGGL(k_diag_block<T>, dim3(1), dim3(1024), 0, st, A, lda, k0, nb, 1, ipiv, Y, Z, lmax);
+    hipLaunchKernelGGL(k_diag_block<T>, dim3(1), dim3(1024), NB * NB * sizeof(T), st, A, lda, k0, nb, 1, ipiv, Y, Z, lmax);
     if (m2 > 0) {
       T* A21 = A + (size_t)(k0 + nb) * lda + k0;
       e = gemm<T>(m2, nb, nb, T(1), A21, lda, Z, NB, T(0), U12, (size_t)nb, st);
@@ -852,7 +855,7 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
       e = hipLaunchCooperativeKernel((const void*)kpanel, dim3(G), dim3(256), args, (unsigned)(R * NB * sizeof(T)), st);
       if (e != hipSuccess) return e;
       any_pivot = true;
-      hipLaunchKernelGGL(k_diag_block<T>, dim3(1), dim3(1024), 0, st, A, lda, k0, nb, 0, ipiv, Y, Z, lmax);
+      hipLaunchKernelGGL(k_diag_block<T>, dim3(1), dim3(1024), NB * NB * sizeof(T), st, A, lda, k0, nb, 0, ipiv, Y, Z, lmax);
     }
     if (pivoted && N - nb > 0) hipLaunchKernelGGL(k_laswp<T>, dim3(cdiv(N - nb, 256)), dim3(256), 0, st, A, lda, N, k0, nb, ipiv);
     const int n2 = N - k0 - nb;
