@@ -64,8 +64,9 @@ def mirror_problem(N):
     return configs.lines_types_values(mesh, volts)
 
 
-def fill_solve(lines, types, values, mode):
-    """One fill+solve step on the device; returns (charges tensor, stats, times dict)."""
+def fill_solve(lines, types, values, mode, gemm_timing=None):
+    """One fill+solve step on the device; returns (charges tensor, stats, times dict).
+    gemm_timing (device.KernelTiming): HIP events around the LU's MFMA GEMM launches."""
     import torch
     from traceon_amd import device
     t = {}
@@ -76,7 +77,7 @@ def fill_solve(lines, types, values, mode):
         M, M32 = device.assemble(E, fp64=True, fp32=(mode == "mixed"))
     F = np.where(np.isin(types, [1, 2, 5]), values, 0.0)
     with Timer() as ts:
-        X, stats, lu = device.solve_system(M, F, M32=M32, mode=mode)
+        X, stats, lu = device.solve_system(M, F, M32=M32, mode=mode, gemm_timing=gemm_timing)
     torch.cuda.synchronize()
     t["total_s"] = time.perf_counter() - t0
     t["fill_ms"] = tf.ms
@@ -120,7 +121,7 @@ def field_grid_leg(X, lines, types, values, n_side=1000):
     return t.ms, pairs, P, Fd, pts
 
 
-def tracing_leg(n_particles, n_sample, seed=2):
+def tracing_leg(n_particles, n_sample, n_axial=3000, seed=2):
     """Batched tracing through the 4-electrode lens (field from a GPU solve)."""
     import torch
     from traceon_amd import configs, solver, tracing
@@ -133,7 +134,7 @@ def tracing_leg(n_particles, n_sample, seed=2):
     field = solver.solve_direct(exc)
     bounds = np.array([[-1.0 * mm, 1.0 * mm], [-1.0 * mm, 1.0 * mm], [-5.0 * mm, 10.0 * mm]])
     t0 = time.perf_counter()
-    axial = FieldRadialAxial(field, -5.0 * mm, 10.0 * mm, N=3000)
+    axial = FieldRadialAxial(field, -5.0 * mm, 10.0 * mm, N=n_axial)
     torch.cuda.synchronize()
     t_axial_build = time.perf_counter() - t0
     rng = np.random.default_rng(seed)
@@ -246,6 +247,7 @@ def main():
     ap.add_argument("--N", type=int, default=20000)
     ap.add_argument("--particles", type=int, default=1_000_000)
     ap.add_argument("--sample", type=int, default=1024)
+    ap.add_argument("--n-axial", type=int, default=3000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-trace", action="store_true")
     ap.add_argument("--no-field", action="store_true")
@@ -261,10 +263,12 @@ def main():
     log(f"bench: mirror N={N}, warmup {args.warmup}, steps {args.steps}")
     for _ in range(args.warmup):
         fill_solve(lines, types, values, "mixed")
+    from traceon_amd.device import KernelTiming
+    gt = KernelTiming()
     times, tfs = [], []
     for k in range(args.steps):
         torch.cuda.synchronize()
-        X, stats, t = fill_solve(lines, types, values, "mixed")
+        X, stats, t = fill_solve(lines, types, values, "mixed", gemm_timing=gt)
         times.append(t["total_s"])
         tfs.append(t)
         log(f"bench: step {k}: fill+solve {t['total_s']:.3f} s (fill {t['fill_ms']:.1f} ms, solve {t['solve_ms']:.1f} ms) {stats}")
@@ -284,11 +288,20 @@ def main():
         "solve_ms": float(np.median([t["solve_ms"] for t in tfs])), "refine_iterations": stats.iterations,
         "charge_rel_diff_mixed_vs_fp64": dq,
     }
-    roof = {"kernel": "k_fill (dense FP64 fill)", "bound": "fp64-valu", "achieved": fill_flops / (fill_ms * 1e-3) / 1e12,
-            "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s", "traffic": None,
-            "flop_per_launch": fill_flops, "launch_ms": fill_ms}
+    # dominant kernel of the step: the FP32 MFMA GEMM of the LU (trailing updates and panel
+    # products), timed with HIP events around each of its launches on the solve stream
+    gemm_ms_avg = gt.ms / max(gt.launches, 1)
+    gemm_flop_avg = gt.flops / max(gt.launches, 1)
+    roof = {"kernel": "k_gemm<float,32> (LU products, v_mfma_f32_16x16x4f32)", "bound": "mfma",
+            "achieved": gemm_flop_avg / (gemm_ms_avg * 1e-3) / 1e12, "peak": PEAK["fp32"] / 1e12, "unit": "TFLOP/s",
+            "traffic": None, "launches": gt.launches, "flop_per_launch": gemm_flop_avg, "launch_ms": gemm_ms_avg,
+            "share_of_step": gt.ms / max(sum(t["fill_ms"] + t["solve_ms"] for t in tfs), 1e-9)}
     roof["frac"] = roof["achieved"] / roof["peak"]
     result["roofline"] = roof
+    rf = {"kernel": "k_fill (dense FP64 fill)", "bound": "fp64-valu", "achieved": fill_flops / (fill_ms * 1e-3) / 1e12,
+          "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s", "flop_per_launch": fill_flops, "launch_ms": fill_ms}
+    rf["frac"] = rf["achieved"] / rf["peak"]
+    result["roofline_fill"] = rf
     if not args.no_field:
         ms, pairs, P, Fd, pts = field_grid_leg(X, lines, types, values)
         result["field_grid"] = {"points": int(pts.shape[0]), "ms": ms, "points_per_s": pts.shape[0] / (ms * 1e-3),
@@ -296,7 +309,7 @@ def main():
         log(f"bench: field grid {pts.shape[0]} points in {ms:.1f} ms")
     trace_info = None
     if not args.no_trace:
-        trace_info = tracing_leg(args.particles, args.sample)
+        trace_info = tracing_leg(args.particles, args.sample, args.n_axial)
         eps = trace_info["n"] / (trace_info["ms"] * 1e-3)
         result["tracing"] = {k: trace_info[k] for k in ("n", "ms", "steps", "reflected_fraction", "len_ratio", "landing_rms_nm",
                                                         "landing_max_nm", "n_sample", "n_axial", "N_elements")}

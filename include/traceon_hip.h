@@ -145,6 +145,13 @@ int tb_fill_diagonal(double *M, size_t ld, float *M32, size_t ld32, const double
 size_t tb_lu_workspace_bytes(int N, int fp32);
 int tb_lu_factor(double *A, size_t lda, int N, int *ipiv, void *workspace, int *info, void *stream);
 int tb_lu_factor_f32(float *A, size_t lda, int N, int *ipiv, void *workspace, int *info, void *stream);
+/* Instrumented factorisation (no counterpart in the reference: measurement only).  As
+   tb_lu_factor / tb_lu_factor_f32 (fp32 selects the type of A), and accumulates into
+   *gemm_timing the number, HIP-event duration (ms, on `stream`) and algorithmic FLOP
+   (2 M N K each) of the MFMA GEMM launches of this factorisation. */
+typedef struct { long long launches; double ms; double flops; } tb_kernel_timing_t;
+int tb_lu_factor_timed(void *A, size_t lda, int N, int *ipiv, void *workspace, int *info, int fp32,
+                       tb_kernel_timing_t *gemm_timing, void *stream);
 /* X (nrhs x N, nrhs <= 64) <- A^{-1} X with the factors and the workspace of the same
    factorisation, which holds the diagonal-block inverses and the row permutation
    (FP64 vectors; FP32 factors promoted on load) */
@@ -220,6 +227,16 @@ typedef struct {
    queue (0: 8 per CU).  Ring-source sums of direct BEM fields run wave-uniformly, every
    lane of a wave sharing each source. */
 int tb_trace(const tb_trace_params_t *params, int waves, void *stream);
+/* Stage-synchronous tracer for a few particles (n <= 65535) in fields with ring-source
+   sums: each RK stage spreads every particle's source sum over nchunk workgroups and adds
+   the partials in a fixed order; the RK arithmetic per particle is that of tb_trace.
+   nchunk from tb_trace_split_chunks(n, total ring sources); workspace of
+   tb_trace_split_workspace_bytes(n, nchunk) bytes.  Synchronises `stream`.  Returns -1
+   (tb_last_error) after 2^22 attempted steps without every particle done. */
+int tb_trace_split_chunks(int n_particles, long long n_sources);
+size_t tb_trace_split_workspace_bytes(int n_particles, int nchunk);
+int tb_trace_split(const tb_trace_params_t *params, int nchunk, void *workspace, void *stream);
+/* Field program at M points (E, H nullable). */
 int tb_eval_program(const tb_field_program_t *program, const double *pts, int M, double *E, double *H, void *stream);
 
 #ifdef __cplusplus

@@ -198,7 +198,11 @@ def test_superposition_tracing(small):
     start, vel = np.array([0.001e-3, 0, 2.3e-3]), tracing.velocity_vec(10, [0, 0, -1])
     _, ts = FieldSuperposition([field, field]).get_tracer(b)(start, vel)
     _, td = (2 * field).get_tracer(b)(start, vel)
-    assert np.allclose(ts, td, rtol=1e-12, atol=1e-15)
+    # the two-component program spreads its source sums over other chunks than the doubled
+    # field, so the sums round differently: same steps, positions equal to far below 1e-9 m
+    assert ts.shape == td.shape
+    assert np.allclose(ts[:, :3], td[:, :3], rtol=0, atol=1e-13)
+    assert np.allclose(ts[:, 3:], td[:, 3:], rtol=1e-10, atol=0)
     sup = FieldSuperposition([field, field.rotate(Ry=np.pi)])
     _, tr = sup.get_tracer(b)(start, vel)
     for z in (0.0, -1.0e-3, -2.0e-3):
@@ -256,3 +260,27 @@ def test_superposition_linearity(gpu):
     combo = 2.0 * sup["a"] + 0 * sup["b"]
     assert np.allclose(full.electrostatic_point_charges.charges, combo.electrostatic_point_charges.charges, rtol=1e-10,
                        atol=1e-10 * np.max(np.abs(full.electrostatic_point_charges.charges)))
+
+
+def test_split_tracer_matches_persistent_tracer(small):
+    """The stage-synchronous tracer (source sums spread over workgroups) and the persistent
+    lane-per-particle tracer take the same steps: the two differ only in the order of the
+    field sums, so landings agree to far below the 1e-9 m bar."""
+    from traceon_amd import tracing
+    field, _, _ = small
+    bounds = [[-0.9e-3, 0.9e-3], [-0.9e-3, 0.9e-3], [-2.4e-3, 2.4e-3]]
+    rng = np.random.default_rng(11)
+    n = 300
+    P = np.column_stack([rng.uniform(0, 0.3e-3, n), np.zeros(n), np.full(n, -2.3e-3)])
+    V = np.array([tracing.velocity_vec(e, [rng.normal(0, 0.02), rng.normal(0, 0.02), 1.0]) for e in rng.uniform(40, 80, n)])
+    prog = tracing.DeviceFieldProgram(field)
+    plane = (np.array([0, 0, 2.0e-3]), np.array([0, 0, 1.0]))
+    a = tracing.run_trace(prog, np.hstack([P, V]), QM, bounds, plane=plane, split=True)
+    b = tracing.run_trace(prog, np.hstack([P, V]), QM, bounds, plane=plane, split=False)
+    la, lb = a.landing.cpu().numpy(), b.landing.cpu().numpy()
+    same_steps = (a.n_attempted == b.n_attempted).float().mean().item()
+    print("split vs persistent: max landing diff", np.nanmax(np.abs(la - lb)[:, :3]), "same step counts", same_steps, flush=True)
+    assert np.array_equal(np.isnan(la[:, 0]), np.isnan(lb[:, 0]))
+    assert np.nanmax(np.abs(la - lb)[:, :3]) < 1e-11
+    assert same_steps > 0.99
+    assert (a.status == b.status).all()

@@ -96,6 +96,7 @@ _SIG = {
     "tb_lu_workspace_bytes": (sz, [i32, i32]),
     "tb_lu_factor": (i32, [vp, sz, i32, vp, vp, vp, vp]),
     "tb_lu_factor_f32": (i32, [vp, sz, i32, vp, vp, vp, vp]),
+    "tb_lu_factor_timed": (i32, [vp, sz, i32, vp, vp, vp, i32, vp, vp]),
     "tb_lu_solve": (i32, [vp, sz, i32, vp, vp, i32, vp, vp]),
     "tb_lu_solve_f32": (i32, [vp, sz, i32, vp, vp, i32, vp, vp]),
     "tb_residual": (i32, [vp, sz, i32, vp, vp, vp, i32, vp, vp]),
@@ -113,6 +114,9 @@ _SIG = {
     "tb_jacobian_3d": (i32, [vp, i32, vp, vp, vp]),
     "tb_charge_radial": (i32, [vp, vp, i32, vp, vp]),
     "tb_trace": (i32, [vp, i32, vp]),
+    "tb_trace_split_chunks": (i32, [i32, C.c_longlong]),
+    "tb_trace_split_workspace_bytes": (sz, [i32, i32]),
+    "tb_trace_split": (i32, [vp, i32, vp, vp]),
     "tb_eval_program": (i32, [vp, vp, i32, vp, vp, vp]),
 }
 

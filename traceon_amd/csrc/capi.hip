@@ -400,7 +400,18 @@ size_t trace_particle(double* times, double* states, double qm, field_fun field,
   P.plane_n[2] = 1.0; P.plane_nn = 1.0;
   P.final_state = fin.p; P.landing = nullptr; P.status = status.p; P.n_accepted = nacc.p; P.n_attempted = natt.p;
   P.t_final = tf.p; P.h_final = hf.p; P.traj = traj.p; P.times = tt.p; P.traj_cap = cap; P.traj_len = tlen.p;
-  tb::launch_trace(P, 1, 0);
+  const long long nsrc = (long long)hp.comp.elec.n + hp.comp.mag.n + hp.comp.cur.n;
+  if (nsrc > 0) {
+    // one particle: spread each stage's source sums over the chip
+    const int nchunk = tb::split_chunks(1, nsrc);
+    Dev<unsigned char> ws(tb::split_workspace_bytes(1, nchunk));
+    if (!ws.good) return 0;
+    const hipError_t e = tb::trace_split(P, nchunk, ws.p, 0);
+    if (e == hipErrorLaunchTimeOut) { set_err("trace_particle: more than 2^22 attempted steps (non-finite field?)"); return 0; }
+    if (!ok(e, "trace_particle")) return 0;
+  } else {
+    tb::launch_trace(P, 1, 0);
+  }
   long long n = 0;
   if (!finish("trace_particle") || !tlen.get(&n, 1)) return 0;
   // row 0 is the caller's initial state; rows 1..n-1 the accepted steps
@@ -448,6 +459,13 @@ int tb_lu_factor(double* A, size_t lda, int N, int* ipiv, void* ws, int* info, v
 }
 int tb_lu_factor_f32(float* A, size_t lda, int N, int* ipiv, void* ws, int* info, void* stream) {
   TB_CALL(tb::lu_factor<float>(A, lda, N, ipiv, ws, info, S(stream), 0), "lu_factor_f32");
+}
+static_assert(sizeof(tb_kernel_timing_t) == sizeof(tb::KernelTiming), "timing layout");
+int tb_lu_factor_timed(void* A, size_t lda, int N, int* ipiv, void* ws, int* info, int fp32, tb_kernel_timing_t* gemm_timing,
+                       void* stream) {
+  auto* tm = reinterpret_cast<tb::KernelTiming*>(gemm_timing);
+  if (fp32) TB_CALL(tb::lu_factor<float>((float*)A, lda, N, ipiv, ws, info, S(stream), 0, tm), "lu_factor_f32");
+  TB_CALL(tb::lu_factor<double>((double*)A, lda, N, ipiv, ws, info, S(stream), 0, tm), "lu_factor");
 }
 int tb_lu_solve(const double* LU, size_t lda, int N, const int* ipiv, double* X, int nrhs, const void* ws, void* stream) {
   TB_CALL(tb::lu_solve<double>(LU, lda, N, ipiv, X, nrhs, ws, S(stream)), "lu_solve");
@@ -506,6 +524,7 @@ int tb_charge_radial(const double* lines, const double* q, int N, double* out, v
 }
 int tb_trace(const tb_trace_params_t* params, int waves, void* stream) {
   if (!params) { set_err("tb_trace: null params"); return -1; }
+  if (params->traj && params->traj_cap < 2) { set_err("tb_trace: traj_cap must be >= 2"); return -1; }
   int nw = waves;
   if (nw <= 0) {
     int dev = 0;
@@ -515,6 +534,15 @@ int tb_trace(const tb_trace_params_t* params, int waves, void* stream) {
     nw = prop.multiProcessorCount * 8;
   }
   TB_CALL(tb::launch_trace(*reinterpret_cast<const tb::TraceParams*>(params), nw, S(stream)), "trace");
+}
+int tb_trace_split_chunks(int n_particles, long long n_sources) { return tb::split_chunks(n_particles, n_sources); }
+size_t tb_trace_split_workspace_bytes(int n_particles, int nchunk) { return tb::split_workspace_bytes(n_particles, nchunk); }
+int tb_trace_split(const tb_trace_params_t* params, int nchunk, void* workspace, void* stream) {
+  if (!params || !workspace) { set_err("tb_trace_split: null params or workspace"); return -1; }
+  if (params->traj && params->traj_cap < 2) { set_err("tb_trace_split: traj_cap must be >= 2"); return -1; }
+  const hipError_t e = tb::trace_split(*reinterpret_cast<const tb::TraceParams*>(params), nchunk, workspace, S(stream));
+  if (e == hipErrorLaunchTimeOut) { set_err("trace_split: more than 2^22 attempted steps (non-finite field?)"); return -1; }
+  return ok(e, "trace_split") ? 0 : -1;
 }
 int tb_eval_program(const tb_field_program_t* program, const double* pts, int M, double* E, double* H, void* stream) {
   TB_CALL(tb::launch_eval_program(*reinterpret_cast<const tb::FieldProgram*>(program), pts, M, E, H, S(stream)), "eval_program");

@@ -19,6 +19,8 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_cooperative_groups.h>
 #include <stdint.h>
+#include <utility>
+#include <vector>
 #include "lu_kernels.h"
 
 namespace cg = cooperative_groups;
@@ -234,18 +236,25 @@ __global__ __launch_bounds__(1024) void k_diag_block(T* __restrict__ A, size_t l
         for (int k = 0; k < KR; k++) colb[p][rg + RG * k] = B[k];
       }
       __syncthreads();
+      T cv[KR];
+#pragma unroll
+      for (int k = 0; k < KR; k++) cv[k] = colb[p][rg + RG * k];
       const T rv = rowb[p][c];
       const T rp = T(1) / rowb[p][j];
 #pragma unroll
       for (int k = 0; k < KR; k++) {
         const int r = rg + RG * k;
-        const T l = colb[p][r] * rp;
+        const T l = cv[k] * rp;
         const T upd = fma(-l, rv, B[k]);
-        const bool below = r > j;
-        const double al = (below && c == j) ? (double)tabs(l) : 0.0;
-        m = (al > m || al != al) ? al : m;
-        B[k] = below ? (c == j ? l : (c > j ? upd : B[k])) : B[k];
+        B[k] = r > j ? (c == j ? l : (c > j ? upd : B[k])) : B[k];
       }
+    }
+    // max |l| over the strictly lower part (NaN counts as large)
+#pragma unroll
+    for (int k = 0; k < KR; k++) {
+      const int r = rg + RG * k;
+      const double al = (r > c && r < nb) ? (double)tabs(B[k]) : 0.0;
+      m = (al > m || al != al) ? al : m;
     }
 #pragma unroll
     for (int k = 0; k < KR; k++) {
@@ -292,11 +301,14 @@ __global__ __launch_bounds__(1024) void k_diag_block(T* __restrict__ A, size_t l
     }
     __syncthreads();
     const T yr = rowb[p][c], zr = rowc[p][c];
+    T lv[KR], uv[KR];
+#pragma unroll
+    for (int k = 0; k < KR; k++) { lv[k] = Bs[(rg + RG * k) * NB + jl]; uv[k] = Bs[(rg + RG * k) * NB + ju]; }
 #pragma unroll
     for (int k = 0; k < KR; k++) {
       const int r = rg + RG * k;
-      const T ny = fma(-Bs[r * NB + jl], yr, Yv[k]);
-      const T nz = fma(-Bs[r * NB + ju], zr, Zv[k]);
+      const T ny = fma(-lv[k], yr, Yv[k]);
+      const T nz = fma(-uv[k], zr, Zv[k]);
       Yv[k] = r > jl ? ny : Yv[k];
       Zv[k] = r < ju ? nz : Zv[k];
     }
@@ -784,9 +796,39 @@ template size_t lu_workspace_bytes<double>(int);
 template size_t lu_workspace_bytes<float>(int);
 
 template <typename T>
-hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev, hipStream_t st, int panel_groups) {
+hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev, hipStream_t st, int panel_groups,
+                     KernelTiming* tm) {
   constexpr int NB = LU_NB;
   if (N <= 0) return hipSuccess;
+  struct Events {   // destroys what an early return leaves behind
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> v;
+    ~Events() {
+      for (auto& p : v) { (void)hipEventDestroy(p.first); (void)hipEventDestroy(p.second); }
+    }
+  } events;
+  auto& ev = events.v;
+  auto timed_gemm = [&](int M, int Nn, int K, T alpha, const T* Ap, size_t la, const T* Bp, size_t lb, T beta, T* Cp,
+                        size_t lc) -> hipError_t {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (tm) {
+      if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return hipErrorOutOfMemory;
+      hipEventRecord(e0, st);
+    }
+    hipError_t r = gemm<T>(M, Nn, K, alpha, Ap, la, Bp, lb, beta, Cp, lc, st);
+    if (tm) {
+      hipEventRecord(e1, st);
+      ev.emplace_back(e0, e1);
+      tm->launches++;
+      tm->flops += 2.0 * M * (double)Nn * K;
+    }
+    return r;
+  };
+  auto finish_timing = [&]() {
+    for (auto& p : ev) {
+      float ms = 0.f;
+      if (hipEventSynchronize(p.second) == hipSuccess && hipEventElapsedTime(&ms, p.first, p.second) == hipSuccess) tm->ms += ms;
+    }
+  };
   int dev = 0;
   hipGetDevice(&dev);
   hipDeviceProp_t prop;
@@ -831,7 +873,7 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
     hipLaunchKernelGGL(k_diag_block<T>, dim3(1), dim3(1024), NB * NB * sizeof(T), st, A, lda, k0, nb, 1, ipiv, Y, Z, lmax);
     if (m2 > 0) {
       T* A21 = A + (size_t)(k0 + nb) * lda + k0;
-      e = gemm<T>(m2, nb, nb, T(1), A21, lda, Z, NB, T(0), U12, (size_t)nb, st);
+      e = timed_gemm(m2, nb, nb, T(1), A21, lda, Z, NB, T(0), U12, (size_t)nb);
       if (e != hipSuccess) return e;
       const size_t nblk_copy = std::min<size_t>(cdiv((size_t)m2 * nb, 256), 1024);
       hipLaunchKernelGGL(k_copy_back_max<T>, dim3(nblk_copy), dim3(256), 0, st, U12, (size_t)nb, A21, lda, m2, nb, lmax);
@@ -862,13 +904,13 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
     if (n2 > 0) {
       T* A12 = A + (size_t)k0 * lda + k0 + nb;
       // U12 = inv(L11) * A12
-      e = gemm<T>(nb, n2, nb, T(1), Y, NB, A12, lda, T(0), U12, (size_t)n2, st);
+      e = timed_gemm(nb, n2, nb, T(1), Y, NB, A12, lda, T(0), U12, (size_t)n2);
       if (e != hipSuccess) return e;
       hipLaunchKernelGGL(k_copy2d<T>, dim3(cdiv((size_t)nb * n2, 256)), dim3(256), 0, st, U12, (size_t)n2, A12, lda, nb, n2);
       // A22 -= L21 * U12
       const T* L21 = A + (size_t)(k0 + nb) * lda + k0;
       T* A22 = A + (size_t)(k0 + nb) * lda + k0 + nb;
-      e = gemm<T>(n2, n2, nb, T(-1), L21, lda, U12, (size_t)n2, T(1), A22, lda, st);
+      e = timed_gemm(n2, n2, nb, T(-1), L21, lda, U12, (size_t)n2, T(1), A22, lda);
       if (e != hipSuccess) return e;
     }
   }
@@ -880,10 +922,11 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_perm, dim3(1), dim3(256), (size_t)N * sizeof(int), st, ipiv, N, W.perm);
   }
+  if (tm) finish_timing();
   return hipGetLastError();
 }
-template hipError_t lu_factor<double>(double*, size_t, int, int*, void*, int*, hipStream_t, int);
-template hipError_t lu_factor<float>(float*, size_t, int, int*, void*, int*, hipStream_t, int);
+template hipError_t lu_factor<double>(double*, size_t, int, int*, void*, int*, hipStream_t, int, KernelTiming*);
+template hipError_t lu_factor<float>(float*, size_t, int, int*, void*, int*, hipStream_t, int, KernelTiming*);
 
 template <typename T>
 hipError_t lu_solve(const T* LU, size_t lda, int N, const int* ipiv, double* X, int nrhs, const void* ws, hipStream_t st) {

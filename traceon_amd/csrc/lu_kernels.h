@@ -11,11 +11,20 @@ template <typename T>
 hipError_t gemm(int M, int N, int K, T alpha, const T* A, size_t lda, const T* B, size_t ldb, T beta, T* C, size_t ldc,
                 hipStream_t st);
 template <typename T> size_t lu_workspace_bytes(int N);
+// Launch count, summed HIP-event duration and algorithmic FLOP (2 M N K per product) of
+// the MFMA GEMM launches inside one factorisation.
+struct KernelTiming {
+  long long launches;
+  double ms;
+  double flops;
+};
 // In-place LU of the row-major N x N matrix A (leading dimension lda) with partial pivoting.
 // ipiv (device, N ints) receives 0-based pivot rows; info_dev (device int, zeroed by the caller)
 // receives 1 + the first zero pivot column, or stays 0.
+// gemm_timing (may be null): HIP events around every GEMM launch on stream st.
 template <typename T>
-hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev, hipStream_t st, int panel_groups);
+hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev, hipStream_t st, int panel_groups,
+                     KernelTiming* gemm_timing = nullptr);
 // X (nrhs x N, row per right-hand side, nrhs <= 64) <- A^{-1} X using the factors and the
 // workspace of the same factorisation (block inverses, permutation)
 template <typename T>

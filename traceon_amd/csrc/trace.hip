@@ -286,6 +286,251 @@ __global__ __launch_bounds__(64) void k_trace(TraceParams P) {
   }
 }
 
+// ---------------------------------------------------------------------------------
+// Stage-synchronous tracer for a few particles in fields with ring-source sums.
+//
+// With fewer particles than the chip has lanes, one lane per particle leaves most CUs
+// idle while each lane walks 10^5 sources.  Here every RK stage is one launch that spreads
+// the source sum of each particle over `nchunk` workgroups (thread j of chunk c takes
+// sources c*256 + j, + nchunk*256, ...), followed by a per-particle launch that adds the
+// chunk partials in a fixed order, forms k_s and the next stage point.  A control launch
+// per attempted step does the error norm, accept/reject, step-size update, landing and
+// trajectory bookkeeping with exactly the arithmetic of k_trace.  The host issues
+// SPLIT_BATCH attempts between reads of the active count.
+// ---------------------------------------------------------------------------------
+struct SplitWs {
+  double *y, *ys, *k, *part, *h, *hmax, *t, *kprev, *land, *qmv;
+  long long *nacc, *natt, *tlen;
+  int *active, *found, *count;
+  SplitWs(void* base, int P, int nchunk) {
+    char* b = (char*)base;
+    auto take = [&](size_t bytes) { char* p = b; b += (bytes + 255) & ~(size_t)255; return p; };
+    y = (double*)take(sizeof(double) * 6 * P);
+    ys = (double*)take(sizeof(double) * 6 * P);
+    k = (double*)take(sizeof(double) * 36 * P);
+    part = (double*)take(sizeof(double) * 6 * (size_t)P * nchunk);
+    h = (double*)take(sizeof(double) * P);
+    hmax = (double*)take(sizeof(double) * P);
+    t = (double*)take(sizeof(double) * P);
+    kprev = (double*)take(sizeof(double) * P);
+    land = (double*)take(sizeof(double) * 6 * P);
+    qmv = (double*)take(sizeof(double) * P);
+    nacc = (long long*)take(sizeof(long long) * P);
+    natt = (long long*)take(sizeof(long long) * P);
+    tlen = (long long*)take(sizeof(long long) * P);
+    active = (int*)take(sizeof(int) * P);
+    found = (int*)take(sizeof(int) * P);
+    count = (int*)take(sizeof(int) * 2);
+  }
+  static size_t bytes(int P, int nchunk) {
+    auto r = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    return r(48 * (size_t)P) * 2 + r(288 * (size_t)P) + r(48 * (size_t)P * nchunk) + r(8 * (size_t)P) * 5 + r(48 * (size_t)P) +
+           r(8 * (size_t)P) * 3 + r(4 * (size_t)P) * 2 + r(8);
+  }
+};
+constexpr int SPLIT_THREADS = 256;
+constexpr int SPLIT_BATCH = 8;
+constexpr long long SPLIT_MAX_ATTEMPTS = 1ll << 22;
+
+__device__ __forceinline__ void split_retire(const TraceParams& P, const SplitWs& W, int i, int status) {
+  for (int c = 0; c < 6; c++) P.final_state[6 * (size_t)i + c] = W.y[6 * (size_t)i + c];
+  if (P.landing)
+    for (int c = 0; c < 6; c++) P.landing[6 * (size_t)i + c] = W.found[i] ? W.land[6 * (size_t)i + c] : __builtin_nan("");
+  if (P.status) P.status[i] = status;
+  if (P.n_accepted) P.n_accepted[i] = W.nacc[i];
+  if (P.n_attempted) P.n_attempted[i] = W.natt[i];
+  if (P.t_final) P.t_final[i] = W.t[i];
+  if (P.h_final) P.h_final[i] = W.h[i];
+  if (P.traj_len) P.traj_len[i] = W.tlen[i];
+  W.active[i] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_split_init(TraceParams P, SplitWs W) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  double y[6];
+  for (int c = 0; c < 6; c++) {
+    y[c] = P.state0[6 * (size_t)i + c];
+    W.y[6 * (size_t)i + c] = y[c];
+    W.ys[6 * (size_t)i + c] = y[c];
+    W.land[6 * (size_t)i + c] = 0.0;
+  }
+  W.t[i] = P.t0 ? P.t0[i] : 0.0;
+  W.qmv[i] = P.qm[P.qm_stride ? i : 0];
+  const double hm = STEP_MAX / sqrt(y[3] * y[3] + y[4] * y[4] + y[5] * y[5]);   // tracing.c:68-70
+  W.hmax[i] = hm;
+  W.h[i] = (P.h0 && P.h0[i] > 0.0) ? P.h0[i] : hm;
+  W.kprev[i] = signed_dist(P, y);
+  W.nacc[i] = 0; W.natt[i] = 0; W.tlen[i] = 1; W.found[i] = 0;
+  if (P.traj) {
+    for (int c = 0; c < 6; c++) P.traj[(size_t)i * P.traj_cap * 6 + c] = y[c];
+    if (P.times) P.times[(size_t)i * P.traj_cap] = W.t[i];
+  }
+  W.active[i] = 1;
+  if (!inside_trace_bounds(P.bounds, y)) split_retire(P, W, i, STATUS_LEFT_BOUNDS);   // tracing.c:79: no step taken
+}
+
+// partial field sums: blockIdx.y = particle, blockIdx.x = source chunk
+__global__ __launch_bounds__(SPLIT_THREADS) void k_split_field(TraceParams P, SplitWs W, int nchunk) {
+  const int i = blockIdx.y;
+  if (!W.active[i]) return;
+  __shared__ double red[SPLIT_THREADS / 64][6];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  double p[3] = {W.ys[6 * (size_t)i], W.ys[6 * (size_t)i + 1], W.ys[6 * (size_t)i + 2]};
+  double E[3], H[3];
+  eval_field(P.field, p, E, H, blockIdx.x * SPLIT_THREADS + tid, nchunk * SPLIT_THREADS, blockIdx.x == 0 && tid == 0);
+  double v[6] = {E[0], E[1], E[2], H[0], H[1], H[2]};
+#pragma unroll
+  for (int c = 0; c < 6; c++) {
+    double x = v[c];
+    for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o, 64);
+    if (lane == 0) red[wave][c] = x;
+  }
+  __syncthreads();
+  if (tid < 6) {
+    double x = 0.0;
+    for (int w = 0; w < SPLIT_THREADS / 64; w++) x += red[w][tid];
+    W.part[((size_t)i * nchunk + blockIdx.x) * 6 + tid] = x;
+  }
+}
+
+// k_s from the chunk partials, then the point of stage s + 1
+__global__ __launch_bounds__(256) void k_split_stage(TraceParams P, SplitWs W, int nchunk, int s) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n || !W.active[i]) return;
+  double f[6] = {0, 0, 0, 0, 0, 0};
+  for (int c = 0; c < nchunk; c++)
+    for (int q = 0; q < 6; q++) f[q] += W.part[((size_t)i * nchunk + c) * 6 + q];
+  const double* ys = W.ys + 6 * (size_t)i;
+  const double h = W.h[i], qm = W.qmv[i];
+  const double vxh0 = ys[4] * f[5] - ys[5] * f[4];
+  const double vxh1 = ys[5] * f[3] - ys[3] * f[5];
+  const double vxh2 = ys[3] * f[4] - ys[4] * f[3];
+  double* k = W.k + 36 * (size_t)i;
+  k[6 * s + 0] = h * ys[3]; k[6 * s + 1] = h * ys[4]; k[6 * s + 2] = h * ys[5];
+  k[6 * s + 3] = h * qm * (f[0] + MU0 * vxh0);
+  k[6 * s + 4] = h * qm * (f[1] + MU0 * vxh1);
+  k[6 * s + 5] = h * qm * (f[2] + MU0 * vxh2);
+  if (s < 5) {
+    const double* y = W.y + 6 * (size_t)i;
+    double* yn = W.ys + 6 * (size_t)i;
+    for (int c = 0; c < 6; c++) {
+      double acc = y[c];
+      for (int j = 0; j <= s; j++) acc = fma(RK_B[s + 1][j], k[6 * j + c], acc);
+      yn[c] = acc;
+    }
+  }
+}
+
+// error norm, accept/reject and step control of one attempted step (as in k_trace)
+__global__ __launch_bounds__(256) void k_split_control(TraceParams P, SplitWs W) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n || !W.active[i]) return;
+  const double* k = W.k + 36 * (size_t)i;
+  double* y = W.y + 6 * (size_t)i;
+  double h = W.h[i];
+  double ep = 0.0, ev = 0.0;
+  for (int c = 0; c < 6; c++) {
+    double e = 0.0;
+    for (int s = 0; s < 6; s++) e = fma(RK_CT[s], k[6 * s + c], e);
+    e = fabs(e);
+    if (c < 3) ep = e > ep ? e : ep;
+    else ev = e > ev ? e : ev;
+  }
+  const double err = ep + h * ev;
+  int st = -1;
+  const long long natt = ++W.natt[i];
+  if (err <= P.atol) {
+    double yn[6];
+    for (int c = 0; c < 6; c++)
+      yn[c] = y[c] + (RK_CH[0] * k[c] + RK_CH[1] * k[6 + c] + RK_CH[2] * k[12 + c] + RK_CH[3] * k[18 + c] + RK_CH[4] * k[24 + c] +
+                      RK_CH[5] * k[30 + c]);
+    W.t[i] += h;
+    W.nacc[i]++;
+    if (P.landing) {
+      const double kn = signed_dist(P, yn), kp = W.kprev[i];
+      if ((kn > 0) != (kp > 0)) {
+        const double diff = kp - kn;
+        const double f = -kn / diff, fp = kp / diff;
+        for (int c = 0; c < 6; c++) W.land[6 * (size_t)i + c] = fp * yn[c] + f * y[c];
+        W.found[i] = 1;
+      }
+      W.kprev[i] = kn;
+    }
+    for (int c = 0; c < 6; c++) y[c] = yn[c];
+    const long long tn = W.tlen[i];
+    if (P.traj) {
+      double* row = P.traj + ((size_t)i * P.traj_cap + tn) * 6;
+      for (int c = 0; c < 6; c++) row[c] = y[c];
+      if (P.times) P.times[(size_t)i * P.traj_cap + tn] = W.t[i];
+    }
+    W.tlen[i] = tn + 1;
+    if (!inside_trace_bounds(P.bounds, y)) st = STATUS_LEFT_BOUNDS;
+    else if (P.traj && tn + 1 == P.traj_cap) st = STATUS_BLOCK_FULL;
+  }
+  const double hn = 0.9 * h * pow(P.atol / err, 0.2);   // tracing.c:119
+  W.h[i] = hn < W.hmax[i] ? hn : W.hmax[i];
+  if (st < 0 && natt >= P.max_attempts) st = STATUS_STEP_CAP;
+  if (st >= 0) {
+    split_retire(P, W, i, st);
+  } else {
+    for (int c = 0; c < 6; c++) W.ys[6 * (size_t)i + c] = y[c];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_split_count(const int* __restrict__ active, int n, int* __restrict__ count) {
+  __shared__ int cs;
+  if (threadIdx.x == 0) cs = 0;
+  __syncthreads();
+  int c = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) c += active[i];
+  if (c) atomicAdd(&cs, c);
+  __syncthreads();
+  if (threadIdx.x == 0 && cs) atomicAdd(count, cs);
+}
+
+int split_chunks(int n_particles, long long n_sources) {
+  // enough workgroups to cover the chip (~2048), at least 8 sources per thread
+  long long by_src = n_sources / (SPLIT_THREADS * 8);
+  long long by_occ = (2048 + n_particles - 1) / n_particles;
+  long long c = by_src < by_occ ? by_src : by_occ;
+  if (c < 1) c = 1;
+  if (c > 1024) c = 1024;
+  return (int)c;
+}
+
+size_t split_workspace_bytes(int n_particles, int nchunk) { return SplitWs::bytes(n_particles, nchunk); }
+
+hipError_t trace_split(const TraceParams& P, int nchunk, void* ws, hipStream_t st) {
+  if (P.n <= 0) return hipSuccess;
+  if (nchunk < 1 || P.n > 65535) return hipErrorInvalidValue;
+  SplitWs W(ws, P.n, nchunk);
+  const dim3 gp((P.n + 255) / 256), bp(256), gf(nchunk, P.n);
+  hipLaunchKernelGGL(k_split_init, gp, bp, 0, st, P, W);
+  const int count_blocks = (P.n + 255) / 256 < 64 ? (P.n + 255) / 256 : 64;
+  for (long long attempts = 0;; attempts += SPLIT_BATCH) {
+    // a trace that needs more attempted steps than this has a non-finite field or step
+    if (attempts > SPLIT_MAX_ATTEMPTS) return hipErrorLaunchTimeOut;
+    for (int a = 0; a < SPLIT_BATCH; a++) {
+      for (int s = 0; s < 6; s++) {
+        hipLaunchKernelGGL(k_split_field, gf, dim3(SPLIT_THREADS), 0, st, P, W, nchunk);
+        hipLaunchKernelGGL(k_split_stage, gp, bp, 0, st, P, W, nchunk, s);
+      }
+      hipLaunchKernelGGL(k_split_control, gp, bp, 0, st, P, W);
+    }
+    hipError_t e = hipMemsetAsync(W.count, 0, sizeof(int), st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_split_count, dim3(count_blocks), dim3(256), 0, st, W.active, P.n, W.count);
+    int remaining = 0;
+    e = hipMemcpyAsync(&remaining, W.count, sizeof(int), hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return e;
+    e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    if (remaining == 0) break;
+  }
+  return hipGetLastError();
+}
+
 // single field evaluation at a batch of points through a field program
 __global__ __launch_bounds__(256) void k_eval_program(FieldProgram F, const double* __restrict__ pts, int M, double* __restrict__ E,
                                                       double* __restrict__ H) {

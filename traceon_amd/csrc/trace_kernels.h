@@ -69,6 +69,11 @@ struct TraceParams {
 
 // One lane per particle; `waves` 64-lane workgroups drain the particle queue.
 hipError_t launch_trace(const TraceParams& P, int waves, hipStream_t st);
+// Stage-synchronous tracer (few particles, n <= 65535): every RK stage spreads each
+// particle's source sums over nchunk workgroups.  ws: split_workspace_bytes(n, nchunk).
+int split_chunks(int n_particles, long long n_sources);
+size_t split_workspace_bytes(int n_particles, int nchunk);
+hipError_t trace_split(const TraceParams& P, int nchunk, void* ws, hipStream_t st);
 hipError_t launch_eval_program(const FieldProgram& F, const double* pts, int M, double* E, double* H, hipStream_t st);
 
 }  // namespace tb

@@ -120,8 +120,14 @@ class LU:
     ws: torch.Tensor   # block inverses and permutation used by the solves
 
 
-def lu_factor(A: torch.Tensor, overwrite: bool = True) -> LU:
-    """In-place (by default) LU with partial pivoting of a square row-major matrix."""
+class KernelTiming(C.Structure):
+    """tb_kernel_timing_t: launches, summed HIP-event ms and algorithmic FLOP of a kernel."""
+    _fields_ = [("launches", C.c_longlong), ("ms", C.c_double), ("flops", C.c_double)]
+
+
+def lu_factor(A: torch.Tensor, overwrite: bool = True, gemm_timing: "KernelTiming | None" = None) -> LU:
+    """In-place (by default) LU with partial pivoting of a square row-major matrix.
+    gemm_timing: accumulates the MFMA GEMM launches of this factorisation (measurement)."""
     assert A.dim() == 2 and A.shape[0] == A.shape[1] and A.is_contiguous()
     N = A.shape[0]
     fp32 = A.dtype == torch.float32
@@ -131,8 +137,12 @@ def lu_factor(A: torch.Tensor, overwrite: bool = True) -> LU:
     ipiv = torch.empty(N, dtype=torch.int32, device=dev)
     info = torch.zeros(1, dtype=torch.int32, device=dev)
     ws = torch.empty(_L.tb_lu_workspace_bytes(N, int(fp32)), dtype=torch.uint8, device=dev)
-    f = _L.tb_lu_factor_f32 if fp32 else _L.tb_lu_factor
-    check(f(ptr(A), N, N, ptr(ipiv), ptr(ws), ptr(info), stream_ptr()), "lu_factor")
+    if gemm_timing is not None:
+        check(_L.tb_lu_factor_timed(ptr(A), N, N, ptr(ipiv), ptr(ws), ptr(info), int(fp32), C.byref(gemm_timing),
+                                    stream_ptr()), "lu_factor")
+    else:
+        f = _L.tb_lu_factor_f32 if fp32 else _L.tb_lu_factor
+        check(f(ptr(A), N, N, ptr(ipiv), ptr(ws), ptr(info), stream_ptr()), "lu_factor")
     lu = LU(A, ipiv, int(info.item()), fp32, ws)
     if lu.info:
         # numpy.linalg.solve (solver.py:173) raises LinAlgError on an exactly singular matrix
@@ -170,7 +180,8 @@ class SolveStats:
 
 
 def solve_system(M64: torch.Tensor | None, F: np.ndarray | torch.Tensor, M32: torch.Tensor | None = None,
-                 mode: str = "fp64", max_iter: int = 30, refine: bool = True, lu: LU | None = None):
+                 mode: str = "fp64", max_iter: int = 30, refine: bool = True, lu: LU | None = None,
+                 gemm_timing: KernelTiming | None = None):
     """Charges for right-hand sides F (nrhs, N) of the assembled system M64 x = F.
 
     mode "fp64":  LU of the FP64 matrix (the dgesv of solver.py:173), then, if ``refine``,
@@ -178,15 +189,15 @@ def solve_system(M64: torch.Tensor | None, F: np.ndarray | torch.Tensor, M32: to
     mode "mixed": LU of the FP32 copy M32 (MFMA FP32), FP64 solves, refinement with
                   double-double residuals against M64.
     Refinement stops when the update is at the rounding level of x (2 eps |x|) or stops
-    shrinking.  Returns (X (nrhs,N) FP64 tensor, SolveStats, LU)."""
+    shrinking.  gemm_timing: see lu_factor.  Returns (X (nrhs,N) FP64 tensor, SolveStats, LU)."""
     Bt = F if isinstance(F, torch.Tensor) else to_dev(np.atleast_2d(F))
     Bt = Bt.reshape(-1, Bt.shape[-1]).contiguous()
     if mode == "fp64":
         if lu is None:
-            lu = lu_factor(M64, overwrite=False)
+            lu = lu_factor(M64, overwrite=False, gemm_timing=gemm_timing)
     elif mode == "mixed":
         if lu is None:
-            lu = lu_factor(M32, overwrite=True)
+            lu = lu_factor(M32, overwrite=True, gemm_timing=gemm_timing)
     else:
         raise ValueError(mode)
     X = Bt.clone()

@@ -136,8 +136,9 @@ class DeviceFieldProgram:
         from .field import FieldSuperposition
         comps = field.device_components(1.0)
         self.keep = [k for _, k in comps]
-        # ring-source sums make a field evaluation cost 10^5 pairs: trace those wave-per-particle
-        self.has_sources = any(c.elec.n or c.mag.n or c.cur.n for c, _ in comps)
+        # ring-source sums make a field evaluation cost ~10^5 pairs
+        self.n_sources = int(sum(c.elec.n + c.mag.n + c.cur.n for c, _ in comps))
+        self.has_sources = self.n_sources > 0
         n = len(comps)
         raw = (FieldCompT * n)(*[c for c, _ in comps])
         host = torch.frombuffer(bytearray(bytes(raw)), dtype=torch.uint8)
@@ -176,9 +177,17 @@ class BatchResult:
         self.t_final = t_final
 
 
+# Below this many particles a field with ring sources is traced stage-synchronously
+# (tb_trace_split: every stage's source sums spread over the chip); above it the
+# persistent lane-per-particle kernel (tb_trace) has enough lanes to fill the GPU.
+SPLIT_MAX_PARTICLES = 16384
+
+
 def run_trace(program: DeviceFieldProgram, state0, qm, bounds, atol=1e-8, plane=None, traj_cap=0, t0=None,
-              max_attempts=1 << 62, waves=0):
-    """Persistent GPU tracer over many particles (all arrays numpy or device tensors)."""
+              max_attempts=1 << 62, waves=0, split=None):
+    """GPU tracer over many particles (all arrays numpy or device tensors).
+    split: None picks tb_trace_split for fields with ring sources and at most
+    SPLIT_MAX_PARTICLES particles, else the persistent tb_trace."""
     import torch
     from . import device
     S0 = state0 if isinstance(state0, torch.Tensor) else device.to_dev(np.atleast_2d(state0))
@@ -229,7 +238,15 @@ def run_trace(program: DeviceFieldProgram, state0, qm, bounds, atol=1e-8, plane=
         times = torch.empty((n, traj_cap), dtype=torch.float64, device=dev)
         tlen = torch.empty(n, dtype=torch.int64, device=dev)
         P.traj, P.times, P.traj_cap, P.traj_len = traj.data_ptr(), times.data_ptr(), int(traj_cap), tlen.data_ptr()
-    backend.check(backend.backend_lib.tb_trace(C.byref(P), int(waves), device.stream_ptr()), "trace")
+    if split is None:
+        split = program.has_sources and n <= SPLIT_MAX_PARTICLES
+    L = backend.backend_lib
+    if split:
+        nchunk = L.tb_trace_split_chunks(n, program.n_sources)
+        ws = torch.empty(L.tb_trace_split_workspace_bytes(n, nchunk), dtype=torch.uint8, device=dev)
+        backend.check(L.tb_trace_split(C.byref(P), nchunk, ws.data_ptr(), device.stream_ptr()), "trace_split")
+    else:
+        backend.check(L.tb_trace(C.byref(P), int(waves), device.stream_ptr()), "trace")
     out = BatchResult(final, landing, status, nacc, natt, tfin)
     out.traj, out.times, out.traj_len = traj, times, tlen
     out._keep = (S0, QM, T0, counter)
