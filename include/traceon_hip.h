@@ -35,14 +35,14 @@ extern "C" {
 /* Tier R — data symbols read with ctypes in_dll (traceon/backend/__init__.py:58-66)    */
 /* ------------------------------------------------------------------------------------ */
 extern const size_t TRACING_BLOCK_SIZE;   /* tracing.c:7 */
-extern const int N_QUAD_2D_SYM;           /* utilities_2d.c:2 */
-extern const int N_TRIANGLE_QUAD_SYM;     /* utilities_3d.c:4 */
-extern const int DERIV_2D_MAX_SYM;        /* radial_ring.c:140 */
+extern const int N_QUAD_2D_SYM;           /* utilities_2d.c:3 */
+extern const int N_TRIANGLE_QUAD_SYM;     /* utilities_3d.c:5 */
+extern const int DERIV_2D_MAX_SYM;        /* radial.c:3 */
 extern const int NU_MAX_SYM;              /* three_d.c:10 */
 extern const int M_MAX_SYM;               /* three_d.c:11 */
 
-/* Reference structs carried through void* arguments (radial.c:147-152, three_d.c:18-31,
-   radial.c:312-317).  Pointers inside are host pointers. */
+/* Reference structs carried through void* arguments (radial.c:10-15, three_d.c:18-31,
+   radial.c:175-180).  Pointers inside are host pointers. */
 struct effective_point_charges_2d { double *charges; double *jacobians; double *positions; size_t N; };
 struct effective_point_charges_3d { double *charges; double *jacobians; double *positions; size_t N; };
 struct field_evaluation_args { void *elec_charges; void *mag_charges; void *currents; double *bounds; };
@@ -72,24 +72,24 @@ double flux_density_to_charge_factor(double K);
 double current_potential_axial_radial_ring(double z0, double r, double z);
 void current_field_radial_ring(double x0, double y0, double x, double y, double *result);
 
-/* radial.c:361-383, three_d.c:34-62 (backend/__init__.py:321, 325) */
+/* radial.c:224-246, three_d.c:34-62 (backend/__init__.py:321, 325) */
 void fill_jacobian_buffer_radial(double *jacobian_buffer, double *pos_buffer, const double *line_points, size_t N_lines);
 void fill_jacobian_buffer_3d(double *jacobian_buffer, double *pos_buffer, const double *triangles, size_t N_triangles);
 
-/* radial.c:386-447 (backend/__init__.py:324): rows [start,end] inclusive, ACCUMULATES into matrix
+/* radial.c:249-310 (backend/__init__.py:324): rows [start,end] inclusive, ACCUMULATES into matrix
    (row stride N_matrix).  An unknown excitation type records an error and changes nothing. */
 void fill_matrix_radial(double *matrix, const double *line_points, const uint8_t *excitation_types,
                         const double *excitation_values, const double *jacobian_buffer, const double *pos_buffer,
                         size_t N_lines, size_t N_matrix, int lines_range_start, int lines_range_end);
 
-/* The singular diagonal.  The reference integrates the integrands of radial.c:319-359 with
+/* The singular diagonal.  The reference integrates the integrands of radial.c:182-222 with
    scipy.integrate.quad (backend/__init__.py:688-708); here the whole integral is one call.
    self_field_dot_normal_radial_integral returns the integral itself (the solver subtracts 1,
    solver.py:245). */
 double self_potential_radial_integral(const double *line_points);
 double self_field_dot_normal_radial_integral(const double *line_points, double K);
 
-/* radial.c:155-310, 449-521 (backend/__init__.py:305-307, 309-311, 317, 319-320) */
+/* radial.c:19-173, 449-521 (backend/__init__.py:305-307, 309-311, 317, 319-320) */
 double potential_radial(const double *point, const double *charges, const double *jacobian_buffer, const double *pos_buffer, size_t N);
 void field_radial(const double *point, double *result, const double *charges, const double *jacobian_buffer, const double *pos_buffer,
                   size_t N);

@@ -302,7 +302,7 @@ void fill_jacobian_buffer_3d(double *jac, double *pos, const double *tri, size_t
     }
 }
 
-/* radial.c:261-285: total charge of one element with uniform density */
+/* radial.c:124-148: total charge of one element with uniform density */
 double charge_radial(const double *line, double q) {
     const double *v[4];
     nodes_of(line, v);
@@ -316,7 +316,7 @@ double charge_radial(const double *line, double q) {
 }
 
 /* ------------------------------------------------------------------------- */
-/* Matrix fill, radial.c:386-447.  Rows [row_start,row_end] inclusive,         */
+/* Matrix fill, radial.c:249-310.  Rows [row_start,row_end] inclusive,         */
 /* accumulating into M (row stride N_matrix).  Unknown type -> returns -1      */
 /* (the reference prints and exits; an oracle must not kill the test run).     */
 /* ------------------------------------------------------------------------- */
@@ -357,7 +357,7 @@ int fill_matrix_radial(double *M, const double *lines, const uint8_t *types, con
     return 0;
 }
 
-/* Self-integrands, radial.c:319-359.  Called by scipy.integrate.quad through
+/* Self-integrands, radial.c:182-222.  Called by scipy.integrate.quad through
    LowLevelCallable with signature double f(double, void*) exactly as the
    reference (backend/__init__.py:688-708). */
 double self_potential_radial(double a, void *user) {
@@ -370,7 +370,7 @@ double self_potential_radial(double a, void *user) {
     return j * potential_radial_ring(tgt[0], tgt[1], d[0], d[1], NULL);
 }
 
-struct self_fdn_args { const double *line; double K; };   /* radial.c:335-338 */
+struct self_fdn_args { const double *line; double K; };   /* radial.c:198-201 */
 
 double self_field_dot_normal_radial(double a, void *user) {
     const struct self_fdn_args *args = (const struct self_fdn_args *)user;
@@ -384,7 +384,7 @@ double self_field_dot_normal_radial(double a, void *user) {
 }
 
 /* ------------------------------------------------------------------------- */
-/* Field sums, radial.c:155-310                                                 */
+/* Field sums, radial.c:19-173                                                 */
 /* ------------------------------------------------------------------------- */
 double potential_radial(const double *pt, const double *q, const double *jac, const double *pos, size_t N) {
     double r0 = sqrt(pt[0] * pt[0] + pt[1] * pt[1]), z0 = pt[2];
@@ -412,7 +412,7 @@ void field_radial(const double *pt, double *out, const double *q, const double *
     out[2] = Ez;
 }
 
-/* radial.c:175-190 (current rings are stored as (r, 0, z) with 12 quad points) */
+/* radial.c:39-53 (current rings are stored as (r, 0, z) with 12 quad points) */
 double current_potential_axial(double z0, const double *I, const double *jac, const double *pos, size_t N) {
     double s = 0.0;
     for (size_t i = 0; i < N; i++)
@@ -423,7 +423,7 @@ double current_potential_axial(double z0, const double *I, const double *jac, co
     return s;
 }
 
-/* radial.c:192-220 */
+/* radial.c:56-83 */
 void current_field_radial(const double *pt, double *out, const double *I, const double *jac, const double *pos, size_t N) {
     double r = sqrt(pt[0] * pt[0] + pt[1] * pt[1]);
     double Br = 0.0, Bz = 0.0;
@@ -441,7 +441,7 @@ void current_field_radial(const double *pt, double *out, const double *I, const 
     out[2] = Bz;
 }
 
-/* radial.c:155-173, accumulating into d (Nz,9) */
+/* radial.c:19-36, accumulating into d (Nz,9) */
 void axial_derivatives_radial(double *d, const double *q, const double *jac, const double *pos, size_t N,
                               const double *z, size_t Nz) {
     for (size_t i = 0; i < Nz; i++)
@@ -454,7 +454,7 @@ void axial_derivatives_radial(double *d, const double *q, const double *jac, con
             }
 }
 
-/* radial.c:222-240, accumulating */
+/* radial.c:86-103, accumulating */
 void current_axial_derivatives_radial(double *d, const double *I, const double *jac, const double *pos, size_t N,
                                       const double *z, size_t Nz) {
     for (size_t i = 0; i < Nz; i++)
@@ -468,7 +468,7 @@ void current_axial_derivatives_radial(double *d, const double *I, const double *
 }
 
 /* ------------------------------------------------------------------------- */
-/* Axial (paraxial series) evaluation, radial.c:449-521.                       */
+/* Axial (paraxial series) evaluation, radial.c:313-384.                       */
 /* coeffs (Nz-1, 9, 6), highest power first.                                   */
 /* ------------------------------------------------------------------------- */
 static int axial_interval(const double *zs, size_t Nz, double z, double *dz_out) {
@@ -517,9 +517,9 @@ void field_radial_derivs(const double *pt, double *out, const double *zs, const 
 /* ------------------------------------------------------------------------- */
 /* Tracing, tracing.c.  Field adaptors keep the reference's ctypes structs.    */
 /* ------------------------------------------------------------------------- */
-struct charges_2d { double *q; double *jac; double *pos; size_t N; };            /* radial.c:147-152 */
+struct charges_2d { double *q; double *jac; double *pos; size_t N; };            /* radial.c:10-15 */
 struct charges_3d { double *q; double *jac; double *pos; size_t N; };            /* three_d.c:18-23 */
-struct field_evaluation_args { void *elec; void *mag; void *cur; double *bounds; }; /* radial.c:312-317 */
+struct field_evaluation_args { void *elec; void *mag; void *cur; double *bounds; }; /* radial.c:175-180 */
 struct field_derivs_args { double *z; double *elec; double *mag; size_t Nz; };    /* three_d.c:26-31 */
 
 typedef void (*field_fn)(double *pos, double *vel, void *args, double *E, double *H);

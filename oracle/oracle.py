@@ -46,7 +46,7 @@ field_fun = C.CFUNCTYPE(None, C.POINTER(_dbl), C.POINTER(_dbl), _vp, C.POINTER(_
 
 
 class Charges2D(C.Structure):
-    """struct effective_point_charges_2d (radial.c:147-152)."""
+    """struct effective_point_charges_2d (radial.c:10-15)."""
     _fields_ = [("charges", C.POINTER(_dbl)), ("jacobians", C.POINTER(_dbl)), ("positions", C.POINTER(_dbl)), ("N", _sz)]
 
     def __init__(self, q, jac, pos):
@@ -57,7 +57,7 @@ class Charges2D(C.Structure):
 
 
 class FieldEvaluationArgs(C.Structure):
-    """struct field_evaluation_args (radial.c:312-317)."""
+    """struct field_evaluation_args (radial.c:175-180)."""
     _fields_ = [("elec", _vp), ("mag", _vp), ("cur", _vp), ("bounds", C.POINTER(_dbl))]
 
     def __init__(self, elec, mag, cur, bounds=None):
@@ -85,7 +85,7 @@ class FieldDerivsArgs(C.Structure):
 
 
 class SelfFdnArgs(C.Structure):
-    """struct self_field_dot_normal_radial_args (radial.c:335-338)."""
+    """struct self_field_dot_normal_radial_args (radial.c:198-201)."""
     _fields_ = [("line", C.POINTER(_dbl)), ("K", _dbl)]
 
 

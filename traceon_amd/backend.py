@@ -357,7 +357,7 @@ def plane_intersection(positions, p0, normal):
 
 # ---- tracing through the reference's argument structs ------------------------------------
 class EffectivePointCharges2D(C.Structure):
-    """struct effective_point_charges_2d (radial.c:147-152), backend/__init__.py:107-128."""
+    """struct effective_point_charges_2d (radial.c:10-15), backend/__init__.py:107-128."""
     _fields_ = [("charges", dbl_p), ("jacobians", dbl_p), ("positions", dbl_p), ("N", C.c_size_t)]
 
     def __init__(self, eff, *args, **kwargs):
@@ -375,7 +375,7 @@ EffectivePointCharges3D = EffectivePointCharges2D
 
 
 class FieldEvaluationArgsRadial(C.Structure):
-    """struct field_evaluation_args (radial.c:312-317), backend/__init__.py:190-223."""
+    """struct field_evaluation_args (radial.c:175-180), backend/__init__.py:190-223."""
     _fields_ = [("elec_charges", vp), ("mag_charges", vp), ("current_charges", vp), ("bounds", dbl_p)]
 
     def __init__(self, elec, mag, current, bounds, *args, **kwargs):

@@ -61,7 +61,7 @@ __global__ __launch_bounds__(256) void k_src_from_buffers(const double* __restri
 }
 
 // ------------------------------------------------------------------------------
-// Dense fill, fill_matrix_radial (radial.c:386-447).
+// Dense fill, fill_matrix_radial (radial.c:249-310).
 // Block = 256 consecutive columns x ROWS rows; the rows' data are wave-uniform
 // (scalar loads), the source Gauss points are per-lane coalesced loads.  Each
 // entry is the 16-point sum over the source element, written once (or added,
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(256) void k_fill(double* __restrict__ M, size_t ld,
 
 // ------------------------------------------------------------------------------
 // Singular diagonal: self_potential_radial / self_field_dot_normal_radial
-// (radial.c:319-359) integrated over alpha in [-1,1] as solver.py:239-247 does with
+// (radial.c:182-222) integrated over alpha in [-1,1] as solver.py:239-247 does with
 // QUADPACK.  Here: split at the singular point alpha = 0, geometric grading towards it
 // (ratio SIGMA, NLEV levels, the innermost panel [0, SIGMA^(NLEV-1)] below 1e-13) and
 // the 16-point Gauss rule on every panel.  A log singularity at distance SIGMA/(1-SIGMA)
@@ -178,7 +178,7 @@ __global__ __launch_bounds__(256) void k_self(double* __restrict__ M, size_t ld,
 
 // ------------------------------------------------------------------------------
 // Sources for the field sums: w = q * jac, flattened over (element, Gauss point).
-// (the products radial.c:253, 296-297 form inside their loops)
+// (the products radial.c:116, 159-160 form inside their loops)
 // ------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_sources(const double* __restrict__ q, const double* __restrict__ jac,
                                                  const double* __restrict__ pos, int N, int nq, int pstride, int zcol,
@@ -192,7 +192,7 @@ __global__ __launch_bounds__(256) void k_sources(const double* __restrict__ q, c
 }
 
 // ------------------------------------------------------------------------------
-// Field sums at a batch of points: potential_radial / field_radial (radial.c:242-310).
+// Field sums at a batch of points: potential_radial / field_radial (radial.c:106-173).
 // One lane per point; the loop over sources is wave-uniform, so the source triple
 // comes in through scalar loads and every lane shares it.  FP64 VALU bound.
 // ------------------------------------------------------------------------------
@@ -231,7 +231,7 @@ __global__ __launch_bounds__(256) void k_field_points(const double* __restrict__
   }
 }
 
-// current_field_radial (radial.c:192-220) at a batch of points; sources (w, r, z)
+// current_field_radial (radial.c:56-83) at a batch of points; sources (w, r, z)
 __global__ __launch_bounds__(256) void k_current_points(const double* __restrict__ pts, int M, const double* __restrict__ sw,
                                                         const double* __restrict__ sr, const double* __restrict__ sz, int Ns,
                                                         double* __restrict__ field, int accumulate) {
@@ -253,7 +253,7 @@ __global__ __launch_bounds__(256) void k_current_points(const double* __restrict
   else { f[0] = fx; f[1] = fy; f[2] = Hz; }
 }
 
-// current_potential_axial (radial.c:175-190) at a batch of z values
+// current_potential_axial (radial.c:39-53) at a batch of z values
 __global__ __launch_bounds__(256) void k_current_axial_potential(const double* __restrict__ zq, int M, const double* __restrict__ sw,
                                                                  const double* __restrict__ sr, const double* __restrict__ sz,
                                                                  int Ns, double* __restrict__ out) {
@@ -266,8 +266,8 @@ __global__ __launch_bounds__(256) void k_current_axial_potential(const double* _
 }
 
 // ------------------------------------------------------------------------------
-// Axial derivatives: axial_derivatives_radial (radial.c:155-173) and
-// current_axial_derivatives_radial (radial.c:222-240), accumulating into (Nz, 9).
+// Axial derivatives: axial_derivatives_radial (radial.c:19-36) and
+// current_axial_derivatives_radial (radial.c:86-103), accumulating into (Nz, 9).
 // One block per z sample, threads stride over the sources, block reduction.
 // ------------------------------------------------------------------------------
 template <bool CURRENT>
@@ -386,7 +386,7 @@ __global__ void k_cubic_notaknot(const double* __restrict__ d, const double* __r
   }
 }
 
-// potential_radial_derivs / field_radial_derivs (radial.c:449-521) at a batch of points
+// potential_radial_derivs / field_radial_derivs (radial.c:313-384) at a batch of points
 __global__ __launch_bounds__(256) void k_axial_points(const double* __restrict__ pts, int M, const double* __restrict__ zq, int Nz,
                                                       const double* __restrict__ C, double* __restrict__ pot,
                                                       double* __restrict__ field) {
@@ -412,7 +412,7 @@ __global__ __launch_bounds__(256) void k_jac3d(const double* __restrict__ tri, i
   for (int c = 0; c < 3; c++) pos[3 * (size_t)t + c] = T[c] + TRI_B1[k] * e1[c] + TRI_B2[k] * e2[c];
 }
 
-// charge_radial (radial.c:260-285): total charge of each element at unit-free density q
+// charge_radial (radial.c:124-148): total charge of each element at unit-free density q
 __global__ __launch_bounds__(256) void k_charge(const double* __restrict__ lines, const double* __restrict__ q, int N,
                                                 double* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -296,7 +296,7 @@ TB_HD void current_ring_axial_derivs(double z0, double r, double z, double d[NDE
 }
 
 // ---------------------------------------------------------------------------------
-// Paraxial series evaluation (radial.c:449-521).  coeffs (Nz-1, 9, 6) highest power first.
+// Paraxial series evaluation (radial.c:313-384).  coeffs (Nz-1, 9, 6) highest power first.
 // ---------------------------------------------------------------------------------
 TB_HD bool axial_local(const double* zs, int Nz, const double* coeffs, double z, double d[NDERIV]) {
   double z0 = zs[0], zl = zs[Nz - 1];

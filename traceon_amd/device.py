@@ -63,7 +63,7 @@ class Elements:
         types = np.ascontiguousarray(types, dtype=np.uint8)
         bad = ~np.isin(types, [1, 2, 3, 5, 6])
         if np.any(bad):
-            # radial.c:442-445 exits the process; this raises instead
+            # radial.c:305-308 exits the process; this raises instead
             raise ValueError(f"ExcitationType unknown at element {int(np.argmax(bad))}")
         self.lines = to_dev(lines)
         self.types = to_dev(types, torch.uint8)
