@@ -177,6 +177,61 @@ def tracing_leg(n_particles, n_sample, n_axial=3000, seed=2):
     }
 
 
+def c3_leg(n_particles, n_sample, n_axial=3000, seed=3):
+    """Magnetic + electrostatic lens (C3): coil 500 A-turn, two mu_r = 1000 pole pieces (bore 2 mm,
+    gap 3 mm) and a three-tube einzel lens, N = 12,000; electrostatic and magnetostatic solves,
+    10 keV electrons traced through the combined field with the Lorentz force."""
+    import torch
+    from traceon_amd import configs, solver, tracing
+    from traceon_amd.excitation import Excitation
+    from traceon_amd.field import FieldRadialAxial
+    mm = 1e-3
+    mesh, volts, perms, currents = configs.magnetic_lens(12000)
+    exc = Excitation(mesh)
+    exc.add_voltage(**volts)
+    exc.add_magnetizable(**perms)
+    exc.add_current(**currents)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    field = solver.solve_direct(exc)
+    torch.cuda.synchronize()
+    t_solve = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    axial = FieldRadialAxial(field, -14.0 * mm, 12.0 * mm, N=n_axial)
+    torch.cuda.synchronize()
+    t_axial = time.perf_counter() - t0
+    rng = np.random.default_rng(seed)
+    n = n_particles
+    rad = 0.1 * mm * np.sqrt(rng.random(n))
+    phi = 2 * np.pi * rng.random(n)
+    tx, ty = rng.normal(0, 5e-3, n), rng.normal(0, 5e-3, n)
+    d = np.column_stack([np.tan(tx), np.tan(ty), np.ones(n)])
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    speed = np.sqrt(2 * 10000.0 * tracing.E_CHARGE / tracing.M_E)
+    P = np.column_stack([rad * np.cos(phi), rad * np.sin(phi), np.full(n, -14.0 * mm)])
+    V = d * speed
+    bounds = np.array([[-1.0 * mm, 1.0 * mm], [-1.0 * mm, 1.0 * mm], [-14.0 * mm, 12.0 * mm]])
+    plane = (np.array([0.0, 0.0, 11.0 * mm]), np.array([0.0, 0.0, 1.0]))
+    tr_ax = axial.get_tracer(bounds)
+    tr_ax.trace_batch(P[:4096], V[:4096], plane=plane)
+    torch.cuda.synchronize()
+    with Timer() as t:
+        res = tr_ax.trace_batch(P, V, plane=plane)
+    land = res.landing.cpu().numpy()
+    with Timer() as tb:
+        ref = field.get_tracer(bounds).trace_batch(P[:n_sample], V[:n_sample], plane=plane)
+    land_ref = ref.landing.cpu().numpy()
+    ok = np.isfinite(land_ref[:, 0]) & np.isfinite(land[:n_sample, 0])
+    err = np.linalg.norm(land[:n_sample][ok, :2] - land_ref[ok, :2], axis=1)
+    return {"N_electrostatic": len(field.electrostatic_point_charges), "N_magnetostatic": len(field.magnetostatic_point_charges),
+            "N_coil_rings": len(field.current_point_charges), "solve_s": t_solve, "axial_build_s": t_axial, "n": n,
+            "ms": t.ms, "electrons_per_s": n / (t.ms * 1e-3), "steps": int(res.n_attempted.sum().item()),
+            "landed_fraction": float(np.mean(np.isfinite(land[:, 0]))),
+            "landing_rms_nm": float(np.sqrt(np.mean(err ** 2)) * 1e9) if ok.any() else None,
+            "landing_max_nm": float(np.max(err) * 1e9) if ok.any() else None, "n_sample": int(n_sample),
+            "electrons_per_s_direct_bem_fp64": n_sample / (tb.ms * 1e-3)}
+
+
 def cpu_baseline(lines, types, values, trace_info, threads):
     """The reference's C backend (oracle/_ref) on bounded samples, on `threads` host cores.
     Fill: a contiguous block of rows split over threads (util.split_collect); diagonal:
@@ -251,6 +306,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-trace", action="store_true")
     ap.add_argument("--no-field", action="store_true")
+    ap.add_argument("--no-c3", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md for how it would shard)",
@@ -318,6 +374,11 @@ def main():
         result["electrons_per_s"] = eps
         log(f"bench: traced {trace_info['n']} electrons in {trace_info['ms']:.1f} ms ({eps:.3e}/s), "
             f"landing RMS {trace_info['landing_rms_nm']} nm vs direct BEM")
+    if not args.no_c3:
+        c3 = c3_leg(args.particles, args.sample, args.n_axial)
+        result["c3_magnetic_lens"] = c3
+        log(f"bench: C3 solve {c3['solve_s']:.3f} s, traced {c3['n']} electrons in {c3['ms']:.1f} ms "
+            f"({c3['electrons_per_s']:.3e}/s), landing RMS {c3['landing_rms_nm']} nm vs direct BEM")
     if not args.no_cpu:
         threads = min(16, len(os.sched_getaffinity(0)))
         cb = cpu_baseline(lines, types, values, trace_info, threads)

@@ -109,3 +109,17 @@ def test_product_path_has_no_host_compute():
         assert "oracle" not in src.replace("the oracle", "")
         assert not re.search(r"np\.linalg\.(solve|inv|lstsq)\(", src.replace("np.linalg.inv(T)", ""))
         assert not re.search(r"^\s*(import|from)\s+scipy", src, flags=re.M)
+
+
+def test_clockwise_path_is_flipped_outward():
+    """mesher.py:217-219 (ensure_outward_normals on construction): a path given clockwise in
+    (r, z) is flipped so that sum(mid_r * n_r * length) > 0, and its 1/3-2/3 nodes follow."""
+    from traceon_amd.mesh import path_mesh, circle_mesh
+    m = path_mesh([[0.5, 0, -0.25], [0.5, 0, 0.25], [1.0, 0, 0.25]], "tri", n_elements=30, closed=True)
+    assert m._normals_outward("tri")
+    v = m.points[m.lines.astype(np.int64)]
+    # node 2 (1/3) and node 3 (2/3) sit at 1/3 and 2/3 from start to end after the flip
+    assert np.allclose(v[:, 2], v[:, 0] + (v[:, 1] - v[:, 0]) / 3) and np.allclose(v[:, 3], v[:, 0] + 2 * (v[:, 1] - v[:, 0]) / 3)
+    c = circle_mesh(3.0, 2.0, 1.0, "c", 40)
+    assert c._normals_outward("c")
+    assert np.allclose(np.linalg.norm(c.points[:, [0, 2]] - [3.0, 2.0], axis=1), 1.0)

@@ -284,3 +284,59 @@ def test_split_tracer_matches_persistent_tracer(small):
     assert np.nanmax(np.abs(la - lb)[:, :3]) < 1e-11
     assert same_steps > 0.99
     assert (a.status == b.status).all()
+
+
+def test_cylindrical_permanent_magnet_axis_field(gpu):
+    """test_radial.py:336-357: on-axis H of a uniformly magnetised cylinder (R 1.5, L 3,
+    Br 2 T) against the closed form Br/(2 mu0) [(dz+L)/sqrt(R^2+(dz+L)^2) - dz/sqrt(R^2+dz^2)]."""
+    from traceon_amd import solver
+    from traceon_amd.excitation import Excitation
+    from traceon_amd.mesh import path_mesh
+    R, Br, L = 1.5, 2.0, 3.0
+    mesh = path_mesh([[0, 0, 0], [R, 0, 0], [R, 0, L], [0, 0, L]], "magnet", n_elements=240)
+    e = Excitation(mesh)
+    e.add_permanent_magnet(magnet=[0.0, 0.0, Br])
+    field = solver.solve_direct(e)
+    for dz in [0.1, 1, 5, 10]:
+        correct = Br / (2 * MU0) * ((dz + L) / np.sqrt(R ** 2 + (dz + L) ** 2) - dz / np.sqrt(R ** 2 + dz ** 2))
+        Hz = field.magnetostatic_field_at_point([0.0, 0.0, L + dz])[2]
+        print("magnet dz", dz, Hz, correct, flush=True)
+        assert np.isclose(correct, Hz, rtol=1e-5)
+
+
+def test_triangular_permanent_magnet_vs_fem(gpu):
+    """test_radial.py:301-334: field of a triangular-section magnet against the reference's
+    FEM values (rtol 1e-5, atol 0.004 on mu0 H, as there)."""
+    from traceon_amd import solver
+    from traceon_amd.excitation import Excitation
+    from traceon_amd.mesh import path_mesh
+    mesh = path_mesh([[0.5, 0, -0.25], [0.5, 0, 0.25], [1.0, 0, 0.25]], "triangle", n_elements=300, closed=True)
+    exc = Excitation(mesh)
+    exc.add_permanent_magnet(triangle=np.array([0.0, 0.0, 2.0]))
+    field = solver.MagnetostaticSolverRadial(exc).get_permanent_magnet_field()
+    pts = np.array([[0.8, 0.0], [1.0, 0.0], [0.75, 0.2], [0.75, 0.5]])
+    fem = np.array([[-0.47191, -0.020779], [-0.19345, -0.12875], [0.28152, -1.3549], [0.098792, 0.18957]])
+    for (r, z), c in zip(pts, fem):
+        Hr, _, Hz = field.magnetostatic_field_at_point([r, 0.0, z])
+        assert np.allclose([MU0 * Hr, MU0 * Hz], c, rtol=1e-5, atol=0.004)
+
+
+def test_magnet_with_magnetizable_material_vs_fem(gpu):
+    """test_radial.py:359-395: a magnet ring beside a mu_r = 5 ring (curved elements), field
+    against the reference's FEM values (rtol 1e-4, atol 2e-4 on mu0 H)."""
+    from traceon_amd import solver
+    from traceon_amd.excitation import Excitation
+    from traceon_amd.mesh import circle_mesh
+    mesh = circle_mesh(3, -2, 1, "magnet", 160) + circle_mesh(3, 2, 1, "circle", 160)
+    exc = Excitation(mesh)
+    exc.add_permanent_magnet(magnet=[0, 0, 2.0])
+    exc.add_magnetizable(circle=5)
+    field = solver.solve_direct(exc)
+    pts = np.array([[3, -4], [3, -2], [3, -0.5], [3, 0.5], [3, 2], [3, 4]], dtype=float)
+    fem = np.array([[-0.080149, 0.25363], [-0.003367, -1.0145], [0.085632, 0.45160], [0.046463, 0.19612],
+                    [0.012013, 0.023293], [0.015441, 0.042159]])
+    for (r, z), c in zip(pts, fem):
+        Hr, _, Hz = field.magnetostatic_field_at_point([r, 0.0, z])
+        print("magnet+iron", r, z, MU0 * Hr, MU0 * Hz, c, flush=True)
+        assert np.isclose(c[0], MU0 * Hr, rtol=1e-4, atol=0.0002)
+        assert np.isclose(c[1], MU0 * Hz, rtol=1e-4, atol=0.0002)

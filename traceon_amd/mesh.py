@@ -86,9 +86,12 @@ def _segment_points(p0, p1, n):
     return np.asarray(p0)[None, :] + t[:, None] * (np.asarray(p1) - np.asarray(p0))[None, :]
 
 
-def path_mesh(vertices, name: str, n_elements: int | None = None, mesh_size: float | None = None, closed: bool = False) -> Mesh:
+def path_mesh(vertices, name: str, n_elements: int | None = None, mesh_size: float | None = None, closed: bool = False,
+              ensure_outward_normals: bool = True) -> Mesh:
     """Mesh a polyline (x, 0, z) vertices into cubic line elements.  The element count of each
-    straight segment is proportional to its length (at least one)."""
+    straight segment is proportional to its length (at least one).  As the reference's Mesh
+    constructor (mesher.py:217-219), the element direction is flipped where needed so that
+    the normals point outward."""
     V = np.asarray(vertices, dtype=np.float64)
     if closed:
         V = np.concatenate([V, V[:1]])
@@ -114,7 +117,10 @@ def path_mesh(vertices, name: str, n_elements: int | None = None, mesh_size: flo
     points = np.concatenate([P, third, two_third])
     n = len(P)
     lines = np.stack([starts, ends, n + np.arange(nseg), n + nseg + np.arange(nseg)], axis=1)
-    return Mesh(points, lines, {name: np.arange(nseg)})
+    mesh = Mesh(points, lines, {name: np.arange(nseg)})
+    if ensure_outward_normals:
+        mesh.ensure_outward_normals(name)
+    return mesh
 
 
 def line_mesh(p0, p1, name, n_elements=None, mesh_size=None) -> Mesh:
@@ -137,3 +143,23 @@ def rectangle_surface_mesh(rmin, rmax, zmin, zmax, name, nr, nz) -> Mesh:
     a, b, c, d = idx[:-1, :-1].ravel(), idx[1:, :-1].ravel(), idx[1:, 1:].ravel(), idx[:-1, 1:].ravel()
     tris = np.concatenate([np.stack([a, b, c], 1), np.stack([a, c, d], 1)])
     return Mesh(points, triangles=tris, physical_to_triangles={name: np.arange(len(tris))})
+
+
+def circle_mesh(x0: float, z0: float, radius: float, name: str, n_elements: int) -> Mesh:
+    """Closed circle around (x0, z0) in the xz plane, counter-clockwise from the positive x
+    direction as Path.circle_xz (geometry.py:293-313), so its normals point outwards.
+    Curved cubic elements: all four nodes of every element lie on the circle (the
+    reference meshes such paths with higher_order=True)."""
+    n = int(n_elements)
+    th = 2 * np.pi * np.arange(n) / n
+    dth = 2 * np.pi / n
+
+    def at(t):
+        return np.stack([x0 + radius * np.cos(t), np.zeros_like(t), z0 + radius * np.sin(t)], axis=1)
+
+    points = np.concatenate([at(th), at(th + dth / 3.0), at(th + 2.0 * dth / 3.0)])
+    starts = np.arange(n)
+    lines = np.stack([starts, (starts + 1) % n, n + starts, 2 * n + starts], axis=1)
+    mesh = Mesh(points, lines, {name: np.arange(n)})
+    mesh.ensure_outward_normals(name)
+    return mesh

@@ -50,20 +50,26 @@ class SolverRadial:
         self.excitation_values = values
         self.elements = device.Elements(self.vertices, types, values) if N else None
         if N:
+            import torch
+
+            def row_info():
+                # RowInfo per element: r0, z0, nr, nz, flux factor, (kind, pad)
+                return self.elements.rows.view(dtype=torch.float64).reshape(N, -1).cpu().numpy()
+
+            info = row_info()
+            self._targets = info[:, 0:2].copy()
             # VOLTAGE_FUN values at the element centres (solver.py:57-61, 100-109)
-            centres = self.centres()
-            for n, idx in names.items():
-                t, f = excitation.excitation_types[n]
-                if t == E.ExcitationType.VOLTAGE_FUN:
-                    values[idx] = [f(*centres[i]) for i in idx]
             if np.any(types == int(E.ExcitationType.VOLTAGE_FUN)):
+                centres = self.centres()
+                for n, idx in names.items():
+                    t, f = excitation.excitation_types[n]
+                    if t == E.ExcitationType.VOLTAGE_FUN:
+                        values[idx] = [f(*centres[i]) for i in idx]
                 self.elements = device.Elements(self.vertices, types, values)
+                info = row_info()
             self.jac_buffer = self.elements.jac.cpu().numpy()
             self.pos_buffer = self.elements.pos.cpu().numpy()
-            import torch
-            info = self.elements.rows.view(dtype=torch.float64).reshape(N, -1).cpu().numpy()
-            self.normals = info[:, 2:4].copy()   # RowInfo: r0, z0, nr, nz, fac, (kind, pad)
-            self._targets = info[:, 0:2].copy()
+            self.normals = info[:, 2:4].copy()
         else:
             self.jac_buffer = np.zeros((0, 16))
             self.pos_buffer = np.zeros((0, 16, 2))
@@ -77,14 +83,9 @@ class SolverRadial:
         return self.normals
 
     def centres(self):
-        """Element centres at alpha = 0 as 3D points (solver.py:100-109)."""
-        t = self._targets if hasattr(self, "_targets") and len(self._targets) == len(self.vertices) else None
-        if t is None:
-            out = np.zeros((len(self.vertices), 3))
-            for i, v in enumerate(self.vertices):
-                _, p = backend.position_and_jacobian_radial(0.0, v[0], v[2], v[3], v[1])
-                out[i] = [p[0], 0.0, p[1]]
-            return out
+        """Element centres at alpha = 0 as 3D points (solver.py:100-109), from the device's
+        per-row data."""
+        t = self._targets
         return np.stack([t[:, 0], np.zeros(len(t)), t[:, 1]], axis=1)
 
     def get_center_of_element(self, index):
@@ -170,9 +171,13 @@ class MagnetostaticSolverRadial(SolverRadial):
         mesh = self.excitation.mesh
         if not len(mesh.lines) or not self.excitation.has_permanent_magnet():
             return FieldRadialBEM(magnetostatic_point_charges=EffectivePointCharges.empty_2d())
-        all_vertices = mesh.points[mesh.lines.astype(np.int64)]
-        jac, pos = backend.fill_jacobian_buffer_radial(all_vertices)
-        normals = np.array([backend.higher_order_normal_radial(0.0, v) for v in all_vertices])
+        import torch
+        from . import device
+        all_vertices = np.ascontiguousarray(mesh.points[mesh.lines.astype(np.int64)], dtype=np.float64)
+        # jacobian buffers and normals at alpha = 0 (higher_order_normal_radial) from one device pass
+        els = device.Elements(all_vertices, np.ones(len(all_vertices), dtype=np.uint8), np.zeros(len(all_vertices)))
+        jac, pos = els.jac.cpu().numpy(), els.pos.cpu().numpy()
+        normals = els.rows.view(dtype=torch.float64).reshape(len(all_vertices), -1).cpu().numpy()[:, 2:4]
         charges, jacs, poss = [], [], []
         for name, (t, vec) in self.excitation.excitation_types.items():
             if t != E.ExcitationType.PERMANENT_MAGNET or name not in mesh.physical_to_lines:
