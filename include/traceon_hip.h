@@ -239,6 +239,26 @@ int tb_trace_split(const tb_trace_params_t *params, int nchunk, void *workspace,
 /* Field program at M points (E, H nullable). */
 int tb_eval_program(const tb_field_program_t *program, const double *pts, int M, double *E, double *H, void *stream);
 
+/* ---- Learned field surrogate (no counterpart in the reference) -----------------------
+   A 2-128-128-128-128-5 SiLU MLP; hidden layers in BF16 on MFMA with FP32 accumulation,
+   FP32 master parameters (tb_mlp_nparams() floats: W1 (128,2) b1 | W2 (128,128) b2 | W3 b3 |
+   W4 b4 | W5 (5,128) b5, W stored [out][in]); wbf = BF16 copies of W2..W4 (3*128*128 u16).
+   Workspaces of tb_mlp_workspace_bytes(B) bytes.  All pointers device pointers. */
+int tb_mlp_nparams(void);
+size_t tb_mlp_workspace_bytes(int B);
+/* C (+)= op(A) op(B): A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn], BF16 in, FP32 out;
+   ksplit > 1 splits K over workgroups (C zeroed first unless accumulate). */
+int tb_gemm_bf16(int M, int N, int K, const uint16_t *A, long sam, long sak, const uint16_t *B, long sbk, long sbn, float *C,
+                 long ldc, int accumulate, int ksplit, void *stream);
+int tb_mlp_forward(const float *params, const uint16_t *wbf, const float *X, int B, float *Y, void *workspace, void *stream);
+/* gradient of mean((Y - T)^2) over the batch and the 5 outputs into grad; *loss = sum of squares */
+int tb_mlp_grad(const float *params, const uint16_t *wbf, const float *X, const float *T, int B, float *grad, float *loss,
+                void *workspace, void *stream);
+int tb_mlp_adam(float *params, const float *grad, float *m, float *v, uint16_t *wbf, float lr, float beta1, float beta2, float eps,
+                int step, void *stream);
+int tb_mlp_to_bf16(const float *params, uint16_t *wbf, void *stream);
+int tb_mlp_gather(int B, const int *idx, const float *X, const float *T, float *Xb, float *Tb, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

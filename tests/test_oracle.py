@@ -411,3 +411,27 @@ def test_spline_coefficients_reproduce_samples():
         for i in range(29):
             poly = np.poly1d(C[i, l, 2:])
             assert np.isclose(poly(0), d[l, i]) and np.isclose(poly(h), d[l, i + 1])
+
+
+def test_mlp_oracle_gradient_vs_finite_differences():
+    """The surrogate's numpy restatement (oracle/mlp_oracle.py, float64, no rounding): its
+    backpropagated gradient against central differences of the loss, on entries of every
+    parameter block."""
+    from oracle import mlp_oracle as mo
+    rng = np.random.default_rng(4)
+    P = mo.init_params(1, scale=0.5)
+    X = rng.uniform(-1, 1, (8, 2))
+    T = rng.standard_normal((8, 5))
+    g, _ = mo.grad(P, X, T, rounding=False)
+
+    def loss(Q):
+        Y, _ = mo.forward(Q, X, rounding=False)
+        return np.mean((Y - T) ** 2)
+
+    h = 1e-6
+    for name, (o, n) in mo.OFF.items():
+        for i in rng.choice(n, size=min(n, 6), replace=False):
+            e = np.zeros_like(P)
+            e[o + i] = h
+            fd = (loss(P + e) - loss(P - e)) / (2 * h)
+            assert abs(fd - g[o + i]) <= 1e-6 * max(1.0, abs(fd)), (name, i, fd, g[o + i])

@@ -117,6 +117,14 @@ _SIG = {
     "tb_trace_split_chunks": (i32, [i32, C.c_longlong]),
     "tb_trace_split_workspace_bytes": (sz, [i32, i32]),
     "tb_trace_split": (i32, [vp, i32, vp, vp]),
+    "tb_mlp_nparams": (i32, []),
+    "tb_mlp_workspace_bytes": (sz, [i32]),
+    "tb_gemm_bf16": (i32, [i32, i32, i32, vp, C.c_long, C.c_long, vp, C.c_long, C.c_long, vp, C.c_long, i32, i32, vp]),
+    "tb_mlp_forward": (i32, [vp, vp, vp, i32, vp, vp, vp]),
+    "tb_mlp_grad": (i32, [vp, vp, vp, vp, i32, vp, vp, vp, vp]),
+    "tb_mlp_adam": (i32, [vp, vp, vp, vp, vp, C.c_float, C.c_float, C.c_float, C.c_float, i32, vp]),
+    "tb_mlp_to_bf16": (i32, [vp, vp, vp]),
+    "tb_mlp_gather": (i32, [i32, vp, vp, vp, vp, vp, vp]),
     "tb_eval_program": (i32, [vp, vp, i32, vp, vp, vp]),
 }
 

@@ -21,7 +21,7 @@ ARCH = os.environ.get("TRACEON_HIP_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 CFLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-SOURCES = ["bem.hip", "lu.hip", "trace.hip", "capi.hip"]
+SOURCES = ["bem.hip", "lu.hip", "trace.hip", "mlp.hip", "capi.hip"]
 
 
 def _compile(src: str) -> str:

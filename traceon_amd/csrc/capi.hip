@@ -15,6 +15,7 @@
 #include "bem_kernels.h"
 #include "lu_kernels.h"
 #include "trace_kernels.h"
+#include "mlp_kernels.h"
 
 static_assert(sizeof(tb_sources_t) == sizeof(tb::SourceSet), "source set layout");
 static_assert(sizeof(tb_field_comp_t) == sizeof(tb::FieldComp), "field comp layout");
@@ -543,6 +544,28 @@ int tb_trace_split(const tb_trace_params_t* params, int nchunk, void* workspace,
   const hipError_t e = tb::trace_split(*reinterpret_cast<const tb::TraceParams*>(params), nchunk, workspace, S(stream));
   if (e == hipErrorLaunchTimeOut) { set_err("trace_split: more than 2^22 attempted steps (non-finite field?)"); return -1; }
   return ok(e, "trace_split") ? 0 : -1;
+}
+int tb_mlp_nparams(void) { return tb::MLP_NPARAM; }
+size_t tb_mlp_workspace_bytes(int B) { return tb::mlp_workspace_bytes(B); }
+int tb_gemm_bf16(int M, int N, int K, const uint16_t* A, long sam, long sak, const uint16_t* B, long sbk, long sbn, float* C, long ldc,
+                 int accumulate, int ksplit, void* stream) {
+  TB_CALL(tb::gemm_bf16(M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, accumulate, ksplit, S(stream)), "gemm_bf16");
+}
+int tb_mlp_forward(const float* params, const uint16_t* wbf, const float* X, int B, float* Y, void* workspace, void* stream) {
+  TB_CALL(tb::mlp_forward(params, wbf, X, B, Y, workspace, S(stream)), "mlp_forward");
+}
+int tb_mlp_grad(const float* params, const uint16_t* wbf, const float* X, const float* T, int B, float* grad, float* loss,
+                void* workspace, void* stream) {
+  TB_CALL(tb::mlp_grad(params, wbf, X, T, B, grad, loss, workspace, S(stream)), "mlp_grad");
+}
+int tb_mlp_adam(float* params, const float* grad, float* m, float* v, uint16_t* wbf, float lr, float beta1, float beta2, float eps,
+                int step, void* stream) {
+  if (step < 1) { set_err("tb_mlp_adam: step must be >= 1"); return -1; }
+  TB_CALL(tb::mlp_adam(params, grad, m, v, wbf, lr, beta1, beta2, eps, step, S(stream)), "mlp_adam");
+}
+int tb_mlp_to_bf16(const float* params, uint16_t* wbf, void* stream) { TB_CALL(tb::mlp_to_bf16(params, wbf, S(stream)), "mlp_to_bf16"); }
+int tb_mlp_gather(int B, const int* idx, const float* X, const float* T, float* Xb, float* Tb, void* stream) {
+  TB_CALL(tb::mlp_gather(B, idx, X, T, Xb, Tb, S(stream)), "mlp_gather");
 }
 int tb_eval_program(const tb_field_program_t* program, const double* pts, int M, double* E, double* H, void* stream) {
   TB_CALL(tb::launch_eval_program(*reinterpret_cast<const tb::FieldProgram*>(program), pts, M, E, H, S(stream)), "eval_program");

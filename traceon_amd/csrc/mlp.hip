@@ -1,0 +1,427 @@
+// Learned field surrogate for the tracer: a 2-128-128-128-128-5 MLP with SiLU activations.
+//
+// Hidden layers (128 x 128) run on MFMA in BF16 (v_mfma_f32_16x16x16bf16_1k, FP32
+// accumulation); the first layer (2 -> 128), the last layer (128 -> 5), activations, loss,
+// reductions and the Adam update run in FP32 on the vector units.  Master weights are FP32;
+// BF16 copies of the hidden-layer weights feed the MFMA.  The surrogate has no
+// counterpart in the reference: it learns the difference between the direct BEM field and
+// the paraxial (axial-derivative) series off the axis, where the series is truncated.
+//
+// Conventions: Z = H_prev W^T + b, W stored [out][in] row-major; batches are row-major
+// (B, width).  Training minimises the mean squared error over the batch and the 5 outputs.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "mlp_kernels.h"
+
+namespace tb {
+
+typedef short s4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f2bf(float x) {   // round to nearest even
+  uint32_t u = __float_as_uint(x);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);   // quiet NaN
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+__device__ __forceinline__ float sigm(float z) { return 1.0f / (1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu(float z) { return z * sigm(z); }
+__device__ __forceinline__ float dsilu(float z) {
+  const float s = sigm(z);
+  return s * (1.0f + z * (1.0f - s));
+}
+
+// ---------------------------------------------------------------------------------
+// C (+)= op(A) op(B), A (M,K) and B (K,N) in BF16 through strides:
+//   A(m,k) = A[m*sam + k*sak],  B(k,n) = B[k*sbk + n*sbn],  C[m*ldc + n] FP32.
+// 64 x 64 tile per workgroup of 4 waves (32 x 32 each = 2 x 2 MFMA 16x16x16), K staged in
+// chunks of 32 through LDS.  gridDim.z splits K; split results are added atomically into
+// C, which the caller has zeroed.  Operand layout of v_mfma_f32_16x16x16bf16_1k: lane l
+// holds A[l%16][4(l/16)+i] and B[4(l/16)+i][l%16], i = 0..3; the result D[4(l/16)+v][l%16].
+// ---------------------------------------------------------------------------------
+constexpr int MG_T = 64, MG_K = 32, MG_PAD = 8;
+__global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, const uint16_t* __restrict__ A, long sam, long sak,
+                                                   const uint16_t* __restrict__ B, long sbk, long sbn, float* __restrict__ C,
+                                                   long ldc, int accumulate, int kchunk) {
+  __shared__ __align__(16) uint16_t As[MG_T][MG_K + MG_PAD];   // [m][k]
+  __shared__ __align__(16) uint16_t Bs[MG_T][MG_K + MG_PAD];   // [n][k]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int m0 = blockIdx.x * MG_T, n0 = blockIdx.y * MG_T;
+  const int kbeg = blockIdx.z * kchunk;
+  const int kend = min(K, kbeg + kchunk);
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  f4v acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = f4v{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = kbeg; k0 < kend; k0 += MG_K) {
+    for (int e = tid; e < MG_T * MG_K; e += 256) {
+      int m, k;
+      if (sak == 1) { m = e / MG_K; k = e % MG_K; } else { k = e / MG_T; m = e % MG_T; }
+      const int gm = m0 + m, gk = k0 + k;
+      As[m][k] = (gm < M && gk < kend) ? A[gm * sam + gk * sak] : (uint16_t)0;
+      int n, kk;
+      if (sbk == 1) { n = e / MG_K; kk = e % MG_K; } else { kk = e / MG_T; n = e % MG_T; }
+      const int gn = n0 + n, gk2 = k0 + kk;
+      Bs[n][kk] = (gn < N && gk2 < kend) ? B[gk2 * sbk + gn * sbn] : (uint16_t)0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < MG_K; kk += 16) {
+      s4v a[2], b[2];
+      const int kb = kk + 4 * (lane >> 4);
+#pragma unroll
+      for (int i = 0; i < 2; i++) {
+        a[i] = *reinterpret_cast<const s4v*>(&As[wm + i * 16 + (lane & 15)][kb]);
+        b[i] = *reinterpret_cast<const s4v*>(&Bs[wn + i * 16 + (lane & 15)][kb]);
+      }
+#pragma unroll
+      for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  const bool split = gridDim.z > 1;
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int row = m0 + wm + i * 16 + 4 * (lane >> 4) + v;
+        const int col = n0 + wn + j * 16 + (lane & 15);
+        if (row < M && col < N) {
+          float* c = C + (long)row * ldc + col;
+          const float x = acc[i][j][v];
+          if (split) atomicAdd(c, x);
+          else *c = accumulate ? *c + x : x;
+        }
+      }
+}
+
+// first layer: Z1 = X W1^T + b1, H1 = silu(Z1) (BF16)
+__global__ __launch_bounds__(256) void k_mlp_in(int B, const float* __restrict__ X, const float* __restrict__ P,
+                                                float* __restrict__ Z, uint16_t* __restrict__ H) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)B * MLP_W) return;
+  const int b = (int)(e / MLP_W), j = (int)(e % MLP_W);
+  const float* W1 = P + MLP_OFF_W1;
+  const float z = fmaf(W1[2 * j], X[2 * b], fmaf(W1[2 * j + 1], X[2 * b + 1], P[MLP_OFF_B1 + j]));
+  Z[e] = z;
+  H[e] = f2bf(silu(z));
+}
+
+// Z += bias (per column), H = silu(Z) in BF16
+__global__ __launch_bounds__(256) void k_bias_silu(int B, float* __restrict__ Z, const float* __restrict__ bias, uint16_t* __restrict__ H) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)B * MLP_W) return;
+  const float z = Z[e] + bias[e % MLP_W];
+  Z[e] = z;
+  H[e] = f2bf(silu(z));
+}
+
+// last layer from the FP32 pre-activations of layer 4: Y = silu(Z4) W5^T + b5.
+// With targets T: dY = 2 (Y - T) / (5 B) and the summed squared error into *loss.
+__global__ __launch_bounds__(256) void k_mlp_out(int B, const float* __restrict__ Z4, const float* __restrict__ P,
+                                                 float* __restrict__ Y, const float* __restrict__ T, float* __restrict__ dY,
+                                                 float* __restrict__ loss) {
+  __shared__ float red[256];
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  float se = 0.f;
+  if (b < B) {
+    const float* W5 = P + MLP_OFF_W5;
+    float y[MLP_OUT];
+#pragma unroll
+    for (int o = 0; o < MLP_OUT; o++) y[o] = P[MLP_OFF_B5 + o];
+    const float* z = Z4 + (long)b * MLP_W;
+    for (int j = 0; j < MLP_W; j++) {
+      const float h = silu(z[j]);
+#pragma unroll
+      for (int o = 0; o < MLP_OUT; o++) y[o] = fmaf(W5[o * MLP_W + j], h, y[o]);
+    }
+#pragma unroll
+    for (int o = 0; o < MLP_OUT; o++) {
+      if (Y) Y[(long)b * MLP_OUT + o] = y[o];
+      if (T) {
+        const float d = y[o] - T[(long)b * MLP_OUT + o];
+        se = fmaf(d, d, se);
+        dY[(long)b * MLP_OUT + o] = 2.0f * d / (float)(MLP_OUT * (long)B);
+      }
+    }
+  }
+  if (!T) return;
+  red[threadIdx.x] = se;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(loss, red[0]);
+}
+
+// backward through the last layer and the layer-4 activation:
+// dZ4 = (dY W5) * silu'(Z4), in FP32 and BF16
+__global__ __launch_bounds__(256) void k_out_bwd(int B, const float* __restrict__ dY, const float* __restrict__ Z4,
+                                                 const float* __restrict__ P, float* __restrict__ dZ, uint16_t* __restrict__ dZbf) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)B * MLP_W) return;
+  const int b = (int)(e / MLP_W), j = (int)(e % MLP_W);
+  const float* W5 = P + MLP_OFF_W5;
+  float dh = 0.f;
+#pragma unroll
+  for (int o = 0; o < MLP_OUT; o++) dh = fmaf(dY[(long)b * MLP_OUT + o], W5[o * MLP_W + j], dh);
+  const float dz = dh * dsilu(Z4[e]);
+  dZ[e] = dz;
+  dZbf[e] = f2bf(dz);
+}
+
+// dZ = dH * silu'(Z), FP32 and BF16
+__global__ __launch_bounds__(256) void k_silu_bwd(int B, const float* __restrict__ dH, const float* __restrict__ Z, float* __restrict__ dZ,
+                                                  uint16_t* __restrict__ dZbf) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)B * MLP_W) return;
+  const float dz = dH[e] * dsilu(Z[e]);
+  dZ[e] = dz;
+  dZbf[e] = f2bf(dz);
+}
+
+// small weight gradients (one workgroup per column j of the width-128 layer), sums over B:
+//   mode 0 (layer 5): dW5[o][j] = sum_b dY[b][o] silu(Z4[b][j])  (db5: k_out_bias_grad)
+//   mode 1 (layer 1): dW1[j][c] = sum_b dZ1[b][j] X[b][c],        db1[j] = sum_b dZ1[b][j]
+//   mode 2 (bias):    db[j]     = sum_b dZ[b][j]
+__global__ __launch_bounds__(256) void k_small_grads(int B, int mode, const float* __restrict__ dY, const float* __restrict__ Z4,
+                                                     const float* __restrict__ dZ, const float* __restrict__ X,
+                                                     float* __restrict__ G, int bias_off) {
+  __shared__ float red[256][MLP_OUT + 1];
+  const int j = blockIdx.x, tid = threadIdx.x;
+  float s[MLP_OUT + 1];
+#pragma unroll
+  for (int q = 0; q <= MLP_OUT; q++) s[q] = 0.f;
+  for (int b = tid; b < B; b += 256) {
+    if (mode == 0) {
+      const float h = silu(Z4[(long)b * MLP_W + j]);
+#pragma unroll
+      for (int o = 0; o < MLP_OUT; o++) s[o] = fmaf(dY[(long)b * MLP_OUT + o], h, s[o]);
+    } else {
+      const float d = dZ[(long)b * MLP_W + j];
+      if (mode == 1) {
+        s[0] = fmaf(d, X[2 * b], s[0]);
+        s[1] = fmaf(d, X[2 * b + 1], s[1]);
+      }
+      s[2] += d;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q <= MLP_OUT; q++) red[tid][q] = s[q];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (tid < st)
+#pragma unroll
+      for (int q = 0; q <= MLP_OUT; q++) red[tid][q] += red[tid + st][q];
+    __syncthreads();
+  }
+  if (tid != 0) return;
+  if (mode == 0) {
+#pragma unroll
+    for (int o = 0; o < MLP_OUT; o++) G[MLP_OFF_W5 + o * MLP_W + j] = red[0][o];
+  } else if (mode == 1) {
+    G[MLP_OFF_W1 + 2 * j] = red[0][0];
+    G[MLP_OFF_W1 + 2 * j + 1] = red[0][1];
+    G[MLP_OFF_B1 + j] = red[0][2];
+  } else {
+    G[bias_off + j] = red[0][2];
+  }
+}
+
+// db5[o] = sum_b dY[b][o]: one workgroup
+__global__ __launch_bounds__(256) void k_out_bias_grad(int B, const float* __restrict__ dY, float* __restrict__ G) {
+  __shared__ float red[256][MLP_OUT];
+  const int tid = threadIdx.x;
+  float s[MLP_OUT] = {0.f, 0.f, 0.f, 0.f, 0.f};
+  for (int b = tid; b < B; b += 256)
+#pragma unroll
+    for (int o = 0; o < MLP_OUT; o++) s[o] += dY[(long)b * MLP_OUT + o];
+#pragma unroll
+  for (int o = 0; o < MLP_OUT; o++) red[tid][o] = s[o];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (tid < st)
+#pragma unroll
+      for (int o = 0; o < MLP_OUT; o++) red[tid][o] += red[tid + st][o];
+    __syncthreads();
+  }
+  if (tid == 0)
+#pragma unroll
+    for (int o = 0; o < MLP_OUT; o++) G[MLP_OFF_B5 + o] = red[0][o];
+}
+
+// Adam on the FP32 master parameters; BF16 copies of the three hidden weight matrices.
+__global__ __launch_bounds__(256) void k_adam(int n, float* __restrict__ P, const float* __restrict__ G, float* __restrict__ Mo,
+                                              float* __restrict__ V, float lr, float b1, float b2, float eps, float c1, float c2,
+                                              uint16_t* __restrict__ Wbf) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float g = G[i];
+  const float m = fmaf(b1, Mo[i], (1.f - b1) * g);
+  const float v = fmaf(b2, V[i], (1.f - b2) * g * g);
+  Mo[i] = m;
+  V[i] = v;
+  const float p = P[i] - lr * (m * c1) / (sqrtf(v * c2) + eps);
+  P[i] = p;
+  // BF16 copies: W2 at 0, W3 at W*W, W4 at 2*W*W
+  if (i >= MLP_OFF_W2 && i < MLP_OFF_W2 + MLP_W * MLP_W) Wbf[i - MLP_OFF_W2] = f2bf(p);
+  else if (i >= MLP_OFF_W3 && i < MLP_OFF_W3 + MLP_W * MLP_W) Wbf[MLP_W * MLP_W + i - MLP_OFF_W3] = f2bf(p);
+  else if (i >= MLP_OFF_W4 && i < MLP_OFF_W4 + MLP_W * MLP_W) Wbf[2 * MLP_W * MLP_W + i - MLP_OFF_W4] = f2bf(p);
+}
+
+__global__ __launch_bounds__(256) void k_to_bf16(int n, const float* __restrict__ P, uint16_t* __restrict__ Wbf) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 3 * MLP_W * MLP_W) return;
+  const int l = i / (MLP_W * MLP_W), r = i % (MLP_W * MLP_W);
+  const int off = l == 0 ? MLP_OFF_W2 : (l == 1 ? MLP_OFF_W3 : MLP_OFF_W4);
+  (void)n;
+  Wbf[i] = f2bf(P[off + r]);
+}
+
+// rows idx[b] of (X, T) into the batch buffers
+__global__ __launch_bounds__(256) void k_gather(int B, const int* __restrict__ idx, const float* __restrict__ X, const float* __restrict__ T,
+                                                float* __restrict__ Xb, float* __restrict__ Tb) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  const long i = idx[b];
+  Xb[2 * b] = X[2 * i];
+  Xb[2 * b + 1] = X[2 * i + 1];
+#pragma unroll
+  for (int o = 0; o < MLP_OUT; o++) Tb[(long)b * MLP_OUT + o] = T[i * MLP_OUT + o];
+}
+
+// ---------------------------------------------------------------------------------
+// host drivers
+// ---------------------------------------------------------------------------------
+static inline int cdivi(long a, long b) { return (int)((a + b - 1) / b); }
+
+hipError_t gemm_bf16(int M, int N, int K, const uint16_t* A, long sam, long sak, const uint16_t* B, long sbk, long sbn, float* C,
+                     long ldc, int accumulate, int ksplit, hipStream_t st) {
+  if (M <= 0 || N <= 0) return hipSuccess;
+  if (ksplit < 1) ksplit = 1;
+  int kchunk = cdivi(K, ksplit);
+  kchunk = ((kchunk + MG_K - 1) / MG_K) * MG_K;
+  const int kz = cdivi(K, kchunk);
+  if (kz > 1 && !accumulate) {
+    hipError_t e = hipMemsetAsync(C, 0, sizeof(float) * (size_t)M * ldc, st);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_gemm_bf16, dim3(cdivi(M, MG_T), cdivi(N, MG_T), kz), dim3(256), 0, st, M, N, K, A, sam, sak, B, sbk, sbn, C,
+                     ldc, accumulate, kchunk);
+  return hipGetLastError();
+}
+
+size_t mlp_workspace_bytes(int B) {
+  const size_t f = sizeof(float), h = sizeof(uint16_t), W = MLP_W;
+  return f * (size_t)B * W * 4          // Z1..Z4
+         + h * (size_t)B * W * 3        // H1..H3 (BF16)
+         + f * (size_t)B * W * 2        // dZ, dH
+         + h * (size_t)B * W            // dZ (BF16)
+         + f * (size_t)B * (2 + 2 * MLP_OUT)   // X batch, T batch, dY
+         + f * (size_t)B * MLP_OUT      // Y
+         + 64;
+}
+
+struct MlpWs {
+  float *Z[4], *dZ, *dH, *Xb, *Tb, *dY, *Y, *loss;
+  uint16_t *H[3], *dZbf;
+  MlpWs(void* base, int B) {
+    char* p = (char*)base;
+    const size_t W = MLP_W;
+    for (int l = 0; l < 4; l++) { Z[l] = (float*)p; p += sizeof(float) * B * W; }
+    for (int l = 0; l < 3; l++) { H[l] = (uint16_t*)p; p += sizeof(uint16_t) * B * W; }
+    dZ = (float*)p; p += sizeof(float) * B * W;
+    dH = (float*)p; p += sizeof(float) * B * W;
+    dZbf = (uint16_t*)p; p += sizeof(uint16_t) * B * W;
+    Xb = (float*)p; p += sizeof(float) * B * 2;
+    Tb = (float*)p; p += sizeof(float) * B * MLP_OUT;
+    dY = (float*)p; p += sizeof(float) * B * MLP_OUT;
+    Y = (float*)p; p += sizeof(float) * B * MLP_OUT;
+    loss = (float*)p;
+  }
+};
+
+static hipError_t forward(const float* P, const uint16_t* Wbf, const float* X, int B, MlpWs& w, hipStream_t st) {
+  const int g = cdivi((long)B * MLP_W, 256);
+  hipLaunchKernelGGL(k_mlp_in, dim3(g), dim3(256), 0, st, B, X, P, w.Z[0], w.H[0]);
+  const int boff[3] = {MLP_OFF_B2, MLP_OFF_B3, MLP_OFF_B4};
+  for (int l = 0; l < 3; l++) {
+    // Z_{l+2} = H_{l+1} W_{l+2}^T: A = H (B x W, row-major), B(k, n) = W[n][k]
+    hipError_t e = gemm_bf16(B, MLP_W, MLP_W, w.H[l], MLP_W, 1, Wbf + (size_t)l * MLP_W * MLP_W, 1, MLP_W, w.Z[l + 1], MLP_W, 0, 1, st);
+    if (e != hipSuccess) return e;
+    uint16_t* Hout = l < 2 ? w.H[l + 1] : w.dZbf;   // layer 4's BF16 activations are not used
+    hipLaunchKernelGGL(k_bias_silu, dim3(g), dim3(256), 0, st, B, w.Z[l + 1], P + boff[l], Hout);
+  }
+  return hipGetLastError();
+}
+
+hipError_t mlp_forward(const float* P, const uint16_t* Wbf, const float* X, int B, float* Y, void* ws, hipStream_t st) {
+  if (B <= 0) return hipSuccess;
+  MlpWs w(ws, B);
+  hipError_t e = forward(P, Wbf, X, B, w, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_mlp_out, dim3(cdivi(B, 256)), dim3(256), 0, st, B, w.Z[3], P, Y, (const float*)nullptr, (float*)nullptr,
+                     (float*)nullptr);
+  return hipGetLastError();
+}
+
+hipError_t mlp_grad(const float* P, const uint16_t* Wbf, const float* X, const float* T, int B, float* G, float* loss, void* ws,
+                    hipStream_t st) {
+  if (B <= 0) return hipSuccess;
+  MlpWs w(ws, B);
+  hipError_t e = forward(P, Wbf, X, B, w, st);
+  if (e != hipSuccess) return e;
+  e = hipMemsetAsync(loss, 0, sizeof(float), st);
+  if (e != hipSuccess) return e;
+  const int g = cdivi((long)B * MLP_W, 256);
+  hipLaunchKernelGGL(k_mlp_out, dim3(cdivi(B, 256)), dim3(256), 0, st, B, w.Z[3], P, w.Y, T, w.dY, loss);
+  // layer 5
+  hipLaunchKernelGGL(k_small_grads, dim3(MLP_W), dim3(256), 0, st, B, 0, w.dY, w.Z[3], (const float*)nullptr, (const float*)nullptr, G, 0);
+  hipLaunchKernelGGL(k_out_bias_grad, dim3(1), dim3(256), 0, st, B, w.dY, G);
+  hipLaunchKernelGGL(k_out_bwd, dim3(g), dim3(256), 0, st, B, w.dY, w.Z[3], P, w.dZ, w.dZbf);
+  // layers 4, 3, 2
+  const int woff[3] = {MLP_OFF_W2, MLP_OFF_W3, MLP_OFF_W4};
+  const int boff[3] = {MLP_OFF_B2, MLP_OFF_B3, MLP_OFF_B4};
+  const int ksplit = cdivi(B, 512);
+  for (int l = 2; l >= 0; l--) {
+    // dW = dZ^T H_prev: A(m, k) = dZ[k][m], B(k, n) = H_prev[k][n]; K = batch, split over workgroups
+    e = gemm_bf16(MLP_W, MLP_W, B, w.dZbf, 1, MLP_W, w.H[l], MLP_W, 1, G + woff[l], MLP_W, 0, ksplit, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_small_grads, dim3(MLP_W), dim3(256), 0, st, B, 2, (const float*)nullptr, (const float*)nullptr, w.dZ,
+                       (const float*)nullptr, G, boff[l]);
+    // dH_prev = dZ W: A = dZ (row-major), B(k, n) = W[k][n]
+    e = gemm_bf16(B, MLP_W, MLP_W, w.dZbf, MLP_W, 1, Wbf + (size_t)l * MLP_W * MLP_W, MLP_W, 1, w.dH, MLP_W, 0, 1, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_silu_bwd, dim3(g), dim3(256), 0, st, B, w.dH, w.Z[l], w.dZ, w.dZbf);
+  }
+  // layer 1
+  hipLaunchKernelGGL(k_small_grads, dim3(MLP_W), dim3(256), 0, st, B, 1, (const float*)nullptr, (const float*)nullptr, w.dZ, X, G, 0);
+  return hipGetLastError();
+}
+
+hipError_t mlp_adam(float* P, const float* G, float* M, float* V, uint16_t* Wbf, float lr, float b1, float b2, float eps, int t,
+                    hipStream_t st) {
+  const float c1 = 1.0f / (1.0f - powf(b1, (float)t)), c2 = 1.0f / (1.0f - powf(b2, (float)t));
+  hipLaunchKernelGGL(k_adam, dim3(cdivi(MLP_NPARAM, 256)), dim3(256), 0, st, MLP_NPARAM, P, G, M, V, lr, b1, b2, eps, c1, c2, Wbf);
+  return hipGetLastError();
+}
+
+hipError_t mlp_to_bf16(const float* P, uint16_t* Wbf, hipStream_t st) {
+  hipLaunchKernelGGL(k_to_bf16, dim3(cdivi(3 * MLP_W * MLP_W, 256)), dim3(256), 0, st, MLP_NPARAM, P, Wbf);
+  return hipGetLastError();
+}
+
+hipError_t mlp_gather(int B, const int* idx, const float* X, const float* T, float* Xb, float* Tb, hipStream_t st) {
+  hipLaunchKernelGGL(k_gather, dim3(cdivi(B, 256)), dim3(256), 0, st, B, idx, X, T, Xb, Tb);
+  return hipGetLastError();
+}
+
+}  // namespace tb
