@@ -121,7 +121,7 @@ def field_grid_leg(X, lines, types, values, n_side=1000):
     return t.ms, pairs, P, Fd, pts
 
 
-def tracing_leg(n_particles, n_sample, n_axial=3000, seed=2):
+def tracing_leg(n_particles, n_sample, n_axial=3400, seed=2):
     """Batched tracing through the 4-electrode lens (field from a GPU solve)."""
     import torch
     from traceon_amd import configs, solver, tracing
@@ -134,7 +134,9 @@ def tracing_leg(n_particles, n_sample, n_axial=3000, seed=2):
     field = solver.solve_direct(exc)
     bounds = np.array([[-1.0 * mm, 1.0 * mm], [-1.0 * mm, 1.0 * mm], [-5.0 * mm, 10.0 * mm]])
     t0 = time.perf_counter()
-    axial = FieldRadialAxial(field, -5.0 * mm, 10.0 * mm, N=n_axial)
+    # the interpolation extends 1 mm past the tracing bounds on both sides, so that no RK stage
+    # point falls outside it and the interval-end conditions stay away from the traced region
+    axial = FieldRadialAxial(field, -6.0 * mm, 11.0 * mm, N=n_axial)
     torch.cuda.synchronize()
     t_axial_build = time.perf_counter() - t0
     rng = np.random.default_rng(seed)
@@ -177,7 +179,7 @@ def tracing_leg(n_particles, n_sample, n_axial=3000, seed=2):
     }
 
 
-def c3_leg(n_particles, n_sample, n_axial=3000, seed=3):
+def c3_leg(n_particles, n_sample, n_axial=3400, seed=3):
     """Magnetic + electrostatic lens (C3): coil 500 A-turn, two mu_r = 1000 pole pieces (bore 2 mm,
     gap 3 mm) and a three-tube einzel lens, N = 12,000; electrostatic and magnetostatic solves,
     10 keV electrons traced through the combined field with the Lorentz force."""
@@ -197,7 +199,7 @@ def c3_leg(n_particles, n_sample, n_axial=3000, seed=3):
     torch.cuda.synchronize()
     t_solve = time.perf_counter() - t0
     t0 = time.perf_counter()
-    axial = FieldRadialAxial(field, -14.0 * mm, 12.0 * mm, N=n_axial)
+    axial = FieldRadialAxial(field, -15.0 * mm, 13.0 * mm, N=int(n_axial * 28 / 17))
     torch.cuda.synchronize()
     t_axial = time.perf_counter() - t0
     rng = np.random.default_rng(seed)
@@ -302,7 +304,7 @@ def main():
     ap.add_argument("--N", type=int, default=20000)
     ap.add_argument("--particles", type=int, default=1_000_000)
     ap.add_argument("--sample", type=int, default=1024)
-    ap.add_argument("--n-axial", type=int, default=3000)
+    ap.add_argument("--n-axial", type=int, default=3400)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-trace", action="store_true")
     ap.add_argument("--no-field", action="store_true")

@@ -364,7 +364,7 @@ void field_radial_traceable(double* position, double* velocity, void* args, doub
   HostProgram hp;
   int kind;
   if (!program_from_args(&field_radial_traceable, args, hp, kind)) return;
-  tb::FieldProgram F{hp.dcomp->p, 1, tb::BOUNDS_NONE, {0, 0, 0, 0, 0, 0}};
+  tb::FieldProgram F{hp.dcomp->p, 1, tb::BOUNDS_NONE, 0, 0, {0, 0, 0, 0, 0, 0}};
   Dev<double> dp(position, 3), E(3), H(3);
   tb::launch_eval_program(F, dp.p, 1, E.p, H.p, 0);
   if (finish("field_radial_traceable")) { E.get(elec_out, 3); H.get(mag_out, 3); }
@@ -375,7 +375,7 @@ void field_radial_derivs_traceable(double* position, double* velocity, void* arg
   HostProgram hp;
   int kind;
   if (!program_from_args(&field_radial_derivs_traceable, args, hp, kind)) return;
-  tb::FieldProgram F{hp.dcomp->p, 1, tb::BOUNDS_NONE, {0, 0, 0, 0, 0, 0}};
+  tb::FieldProgram F{hp.dcomp->p, 1, tb::BOUNDS_NONE, 0, 0, {0, 0, 0, 0, 0, 0}};
   Dev<double> dp(position, 3), E(3), H(3);
   tb::launch_eval_program(F, dp.p, 1, E.p, H.p, 0);
   if (finish("field_radial_derivs_traceable")) { E.get(elec_out, 3); H.get(mag_out, 3); }
@@ -393,7 +393,7 @@ size_t trace_particle(double* times, double* states, double qm, field_fun field,
   if (!s0.good || !traj.good || !tt.good || !fin.good || !counter.good) return 0;
   hipMemset(counter.p, 0, sizeof(int));
   tb::TraceParams P{};
-  P.field = tb::FieldProgram{hp.dcomp->p, 1, tb::BOUNDS_NONE, {0, 0, 0, 0, 0, 0}};
+  P.field = tb::FieldProgram{hp.dcomp->p, 1, tb::BOUNDS_NONE, 0, 0, {0, 0, 0, 0, 0, 0}};
   P.state0 = s0.p; P.t0 = t0.p; P.h0 = nullptr; P.qm = dq.p; P.qm_stride = 0; P.n = 1; P.counter = counter.p;
   for (int i = 0; i < 6; i++) P.bounds[i] = bounds[i];
   P.atol = atol;

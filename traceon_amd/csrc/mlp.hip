@@ -18,20 +18,6 @@ namespace tb {
 typedef short s4v __attribute__((ext_vector_type(4)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ uint16_t f2bf(float x) {   // round to nearest even
-  uint32_t u = __float_as_uint(x);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);   // quiet NaN
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
-__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
-__device__ __forceinline__ float sigm(float z) { return 1.0f / (1.0f + __expf(-z)); }
-__device__ __forceinline__ float silu(float z) { return z * sigm(z); }
-__device__ __forceinline__ float dsilu(float z) {
-  const float s = sigm(z);
-  return s * (1.0f + z * (1.0f - s));
-}
-
 // ---------------------------------------------------------------------------------
 // C (+)= op(A) op(B), A (M,K) and B (K,N) in BF16 through strides:
 //   A(m,k) = A[m*sam + k*sak],  B(k,n) = B[k*sbk + n*sbn],  C[m*ldc + n] FP32.

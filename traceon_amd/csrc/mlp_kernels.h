@@ -22,6 +22,27 @@ constexpr int MLP_OFF_W5 = MLP_OFF_B4 + MLP_W;
 constexpr int MLP_OFF_B5 = MLP_OFF_W5 + MLP_OUT * MLP_W;
 constexpr int MLP_NPARAM = MLP_OFF_B5 + MLP_OUT;   // 50565
 
+// FP32 -> BF16, round to nearest even (NaN kept quiet), and back; SiLU and its derivative
+__host__ __device__ __forceinline__ uint16_t f2bf(float x) {
+  uint32_t u;
+  __builtin_memcpy(&u, &x, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__host__ __device__ __forceinline__ float bf2f(uint16_t b) {
+  const uint32_t u = (uint32_t)b << 16;
+  float x;
+  __builtin_memcpy(&x, &u, 4);
+  return x;
+}
+__device__ __forceinline__ float sigm(float z) { return 1.0f / (1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu(float z) { return z * sigm(z); }
+__device__ __forceinline__ float dsilu(float z) {
+  const float s = sigm(z);
+  return s * (1.0f + z * (1.0f - s));
+}
+
 // C (+)= op(A) op(B) in BF16 on MFMA with FP32 accumulation (strided operands, optional split-K)
 hipError_t gemm_bf16(int M, int N, int K, const uint16_t* A, long sam, long sak, const uint16_t* B, long sbk, long sbn, float* C,
                      long ldc, int accumulate, int ksplit, hipStream_t st);

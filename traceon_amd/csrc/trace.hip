@@ -16,6 +16,7 @@
 // backwards, and the last crossing found is kept.
 #include "radial_math.h"
 #include "trace_kernels.h"
+#include "mlp_kernels.h"
 
 namespace tb {
 
@@ -77,12 +78,98 @@ __device__ __forceinline__ bool in_bounds(const double* b, int mode, const doubl
   return b[0] <= q[0] && q[0] <= b[1] && b[2] <= q[1] && q[1] <= b[3] && b[4] <= q[2] && q[2] <= b[5];
 }
 
+// Ring-source part of a component at local point q (sources s0, s0+ds, ...).
+__device__ void bem_component(const FieldComp& c, const double q[3], double e[3], double h[3], int s0, int ds) {
+  if (c.elec.n) bem_field(c.elec, q[0], q[1], q[2], e, s0, ds);
+  if (c.mag.n) bem_field(c.mag, q[0], q[1], q[2], h, s0, ds);
+  if (c.cur.n) {
+    double hc[3];
+    if (c.transform) {
+      // FieldRadialBEM.magnetostatic_field_at_local_point (field.py:672-695) evaluates the
+      // coil field through current_field_at_point on the already local point: that maps
+      // it once more, checks the bounds there and rotates by the basis.
+      double q2[3];
+      to_local(c, q, q2);
+      if (in_bounds(c.bounds, c.bounds_mode, q2)) { coil_field(c.cur, q2[0], q2[1], q2[2], hc, s0, ds); rotate(c, hc); }
+      else { hc[0] = hc[1] = hc[2] = 0.0; }
+    } else {
+      coil_field(c.cur, q[0], q[1], q[2], hc, s0, ds);
+    }
+    h[0] += hc[0]; h[1] += hc[1]; h[2] += hc[2];
+  }
+}
+
+// Learned off-axis correction of an axial component (FieldSurrogate): per lane, FP32
+// arithmetic with FP32 weights read as wave-uniform scalars, BF16 activations in this
+// wave's LDS columns act[layer parity][neuron][lane] (as the training forward stores them);
+// layer 5 is folded into layer 4 so that layer 4's output stays FP32.  Adds
+// y_k * w(r) * s_k(z), w = (r/r_max)^p, to (E_r, E_z, H_r, H_z) for r >= r_lo.
+constexpr int TMLP_B1 = 256, TMLP_W2 = 384, TMLP_W3 = TMLP_W2 + 16512, TMLP_W4 = TMLP_W3 + 16512, TMLP_W5 = TMLP_W4 + 16512;
+__device__ void mlp_correct(const FieldComp& c, const double q[3], double e[3], double h[3], uint16_t* act, int lane) {
+  const double r = sqrt(q[0] * q[0] + q[1] * q[1]);
+  const double r_lo = c.mlp_geom[0], r_max = c.mlp_geom[1], z0 = c.mlp_geom[2], z1 = c.mlp_geom[3];
+  if (r < r_lo) return;
+  const float u = (float)(2.0 * (r - r_lo) / (r_max - r_lo) - 1.0);
+  const float v = (float)(2.0 * (q[2] - z0) / (z1 - z0) - 1.0);
+  const float* W = c.mlp;
+  uint16_t* a[2] = {act, act + 128 * 64};
+  for (int j = 0; j < 128; j++)
+    a[0][j * 64 + lane] = f2bf(silu(fmaf(W[2 * j], u, fmaf(W[2 * j + 1], v, W[TMLP_B1 + j]))));
+  float y[5];
+#pragma unroll
+  for (int o = 0; o < 5; o++) y[o] = W[TMLP_W5 + 640 + o];
+  for (int layer = 0; layer < 3; layer++) {
+    const float* WT = W + TMLP_W2 + layer * 16512;   // (in, out), then the 128 biases
+    const uint16_t* in = a[layer & 1];
+    uint16_t* out = a[(layer + 1) & 1];
+    for (int jc = 0; jc < 128; jc += 16) {
+      float acc[16];
+#pragma unroll
+      for (int t = 0; t < 16; t++) acc[t] = WT[16384 + jc + t];
+      for (int i = 0; i < 128; i++) {
+        const float x = bf2f(in[i * 64 + lane]);
+#pragma unroll
+        for (int t = 0; t < 16; t++) acc[t] = fmaf(WT[i * 128 + jc + t], x, acc[t]);
+      }
+      if (layer < 2) {
+#pragma unroll
+        for (int t = 0; t < 16; t++) out[(jc + t) * 64 + lane] = f2bf(silu(acc[t]));
+      } else {
+#pragma unroll
+        for (int t = 0; t < 16; t++) {
+          const float hs = silu(acc[t]);
+#pragma unroll
+          for (int o = 0; o < 5; o++) y[o] = fmaf(W[TMLP_W5 + o * 128 + jc + t], hs, y[o]);
+        }
+      }
+    }
+  }
+  // s_k(z): linear between bin centres; w(r)
+  const int nz = c.mlp_nz;
+  double tz = (q[2] - z0) / (z1 - z0) * nz - 0.5;
+  tz = tz < 0.0 ? 0.0 : (tz > nz - 1.0 ? nz - 1.0 : tz);
+  int i0 = (int)tz;
+  if (i0 > nz - 2) i0 = nz > 1 ? nz - 2 : 0;
+  const double f = tz - i0;
+  const int i1 = nz > 1 ? i0 + 1 : 0;
+  const double w = pow(r / r_max, c.mlp_geom[4]);
+  double d[5];
+#pragma unroll
+  for (int k = 1; k < 5; k++) d[k] = (double)y[k] * w * (c.mlp_zs[k * nz + i0] * (1.0 - f) + c.mlp_zs[k * nz + i1] * f);
+  const double cx = r > 0.0 ? q[0] / r : 0.0, cy = r > 0.0 ? q[1] / r : 0.0;
+  e[0] += d[1] * cx; e[1] += d[1] * cy; e[2] += d[2];
+  h[0] += d[3] * cx; h[1] += d[3] * cy; h[2] += d[4];
+}
+
 // Field of a program at p.  Ring-source sums run over sources s0, s0+ds, ...; terms without
 // a source sum (axial, uniform) are added only when `lead` is set, so that partial results
-// of the lanes of a wave add up to the field (wave-per-particle mode) and (0, 1, true)
-// gives the whole field in one lane.
+// of several threads add up to the field and (0, 1, true) gives the whole field in one lane.
+// MLP = true (act: this wave's LDS) applies learned corrections of axial components inside
+// their trust region and their ring sources outside it; MLP = false evaluates such a
+// component as its plain axial field (programs with corrections are traced with MLP = true).
+template <bool MLP = false>
 __device__ void eval_field(const FieldProgram& F, const double p[3], double E[3], double H[3], int s0 = 0, int ds = 1,
-                           bool lead = true) {
+                           bool lead = true, uint16_t* act = nullptr, int lane = 0) {
   E[0] = E[1] = E[2] = 0.0;
   H[0] = H[1] = H[2] = 0.0;
   if (!in_bounds(F.bounds, F.bounds_mode, p)) return;
@@ -93,27 +180,20 @@ __device__ void eval_field(const FieldProgram& F, const double p[3], double E[3]
     if (!in_bounds(c.bounds, c.bounds_mode, q)) continue;
     double e[3] = {0, 0, 0}, h[3] = {0, 0, 0};
     if (c.kind == COMP_BEM) {
-      if (c.elec.n) bem_field(c.elec, q[0], q[1], q[2], e, s0, ds);
-      if (c.mag.n) bem_field(c.mag, q[0], q[1], q[2], h, s0, ds);
-      if (c.cur.n) {
-        double hc[3];
-        if (c.transform) {
-          // FieldRadialBEM.magnetostatic_field_at_local_point (field.py:672-695) evaluates the
-          // coil field through current_field_at_point on the already local point: that maps
-          // it once more, checks the bounds there and rotates by the basis.
-          double q2[3];
-          to_local(c, q, q2);
-          if (in_bounds(c.bounds, c.bounds_mode, q2)) { coil_field(c.cur, q2[0], q2[1], q2[2], hc, s0, ds); rotate(c, hc); }
-          else { hc[0] = hc[1] = hc[2] = 0.0; }
-        } else {
-          coil_field(c.cur, q[0], q[1], q[2], hc, s0, ds);
-        }
-        h[0] += hc[0]; h[1] += hc[1]; h[2] += hc[2];
-      }
+      bem_component(c, q, e, h, s0, ds);
     } else if (c.kind == COMP_AXIAL) {
-      if (lead) {
+      bool use_bem = false;
+      if (MLP && c.mlp) {
+        const double r = sqrt(q[0] * q[0] + q[1] * q[1]);
+        const bool trust = r <= c.mlp_geom[1] && q[2] >= c.mlp_geom[2] && q[2] <= c.mlp_geom[3];
+        use_bem = !trust && (c.elec.n || c.mag.n || c.cur.n);
+      }
+      if (MLP && use_bem) {
+        bem_component(c, q, e, h, s0, ds);
+      } else if (lead) {
         axial_field(c.zs, c.nz, c.ce, q[0], q[1], q[2], e);
         axial_field(c.zs, c.nz, c.cm, q[0], q[1], q[2], h);
+        if (MLP && c.mlp) mlp_correct(c, q, e, h, act, lane);
       }
     } else if (lead) {
       for (int i = 0; i < 3; i++) { e[i] = c.uE[i]; h[i] = c.uH[i]; }
@@ -137,8 +217,11 @@ __device__ __forceinline__ bool inside_trace_bounds(const double* b, const doubl
   return b[0] <= y[0] && y[0] <= b[1] && b[2] <= y[1] && y[1] <= b[3] && b[4] <= y[2] && y[2] <= b[5];
 }
 
+template <bool MLP>
 __global__ __launch_bounds__(64) void k_trace(TraceParams P) {
   const int lane = threadIdx.x & 63;
+  __shared__ uint16_t act_s[MLP ? 2 * 128 * 64 : 1];   // learned-correction activations, 32 KB
+  uint16_t* act = MLP ? act_s : nullptr;
   // lane state
   int pid = -1;
   double y[6], h = 0.0, hmax = 0.0, t = 0.0, qm = 0.0;
@@ -222,7 +305,11 @@ __global__ __launch_bounds__(64) void k_trace(TraceParams P) {
         ys[c] = acc;
       }
       double E[3], H[3];
-      eval_field(P.field, ys, E, H);
+      if (active) {
+        eval_field<MLP>(P.field, ys, E, H, 0, 1, true, act, lane);
+      } else {   // a lane without a particle: no field work (its state is stale or out of range)
+        E[0] = E[1] = E[2] = H[0] = H[1] = H[2] = 0.0;
+      }
       const double vxh0 = ys[4] * H[2] - ys[5] * H[1];
       const double vxh1 = ys[5] * H[0] - ys[3] * H[2];
       const double vxh2 = ys[3] * H[1] - ys[4] * H[0];
@@ -532,13 +619,15 @@ hipError_t trace_split(const TraceParams& P, int nchunk, void* ws, hipStream_t s
 }
 
 // single field evaluation at a batch of points through a field program
-__global__ __launch_bounds__(256) void k_eval_program(FieldProgram F, const double* __restrict__ pts, int M, double* __restrict__ E,
-                                                      double* __restrict__ H) {
+template <bool MLP>
+__global__ __launch_bounds__(64) void k_eval_program(FieldProgram F, const double* __restrict__ pts, int M, double* __restrict__ E,
+                                                     double* __restrict__ H) {
+  __shared__ uint16_t act_s[MLP ? 2 * 128 * 64 : 1];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= M) return;
   double p[3] = {pts[3 * (size_t)t], pts[3 * (size_t)t + 1], pts[3 * (size_t)t + 2]};
   double e[3], h[3];
-  eval_field(F, p, e, h);
+  eval_field<MLP>(F, p, e, h, 0, 1, true, MLP ? act_s : nullptr, threadIdx.x & 63);
   for (int i = 0; i < 3; i++) { if (E) E[3 * (size_t)t + i] = e[i]; if (H) H[3 * (size_t)t + i] = h[i]; }
 }
 
@@ -547,13 +636,15 @@ hipError_t launch_trace(const TraceParams& P, int waves, hipStream_t st) {
   int nw = waves > 0 ? waves : 1;
   const int need = (P.n + 63) / 64;
   if (nw > need) nw = need;
-  hipLaunchKernelGGL(k_trace, dim3(nw), dim3(64), 0, st, P);
+  if (P.field.flags & PROGRAM_HAS_MLP) hipLaunchKernelGGL(k_trace<true>, dim3(nw), dim3(64), 0, st, P);
+  else hipLaunchKernelGGL(k_trace<false>, dim3(nw), dim3(64), 0, st, P);
   return hipGetLastError();
 }
 
 hipError_t launch_eval_program(const FieldProgram& F, const double* pts, int M, double* E, double* H, hipStream_t st) {
   if (M <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_eval_program, dim3((M + 255) / 256), dim3(256), 0, st, F, pts, M, E, H);
+  if (F.flags & PROGRAM_HAS_MLP) hipLaunchKernelGGL(k_eval_program<true>, dim3((M + 63) / 64), dim3(64), 0, st, F, pts, M, E, H);
+  else hipLaunchKernelGGL(k_eval_program<false>, dim3((M + 63) / 64), dim3(64), 0, st, F, pts, M, E, H);
   return hipGetLastError();
 }
 

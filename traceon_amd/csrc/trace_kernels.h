@@ -34,11 +34,21 @@ struct FieldComp {
   const double* cm;
   int nz, pad2;
   double uE[3], uH[3];
+  // learned correction of an axial component (null: none).  mlp: FP32 tracer weights
+  // W1 (128,2) b1 | W2^T (in,out) b2 | W3^T b3 | W4^T b4 | W5 (5,128) b5; mlp_zs: s_k(z)
+  // (5, mlp_nz) at the z-bin centres; mlp_geom: r_lo, r_max, z_min, z_max, weight power.
+  // Outside r <= r_max, z in [z_min, z_max] the component uses its ring sources (BEM).
+  const float* mlp;
+  const double* mlp_zs;
+  int mlp_nz, pad3;
+  double mlp_geom[6];
 };
 
+enum { PROGRAM_HAS_MLP = 1 };
 struct FieldProgram {
   const FieldComp* comp;   // device array
   int ncomp, bounds_mode;
+  int flags, pad;          // PROGRAM_HAS_MLP: some component carries a learned correction
   double bounds[6];
 };
 

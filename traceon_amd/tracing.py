@@ -56,11 +56,16 @@ class FieldCompT(C.Structure):
                 ("inv", C.c_double * 12), ("basis", C.c_double * 9), ("bounds", C.c_double * 6),
                 ("elec", SourcesT), ("mag", SourcesT), ("cur", SourcesT),
                 ("zs", C.c_void_p), ("ce", C.c_void_p), ("cm", C.c_void_p), ("nz", C.c_int), ("pad2", C.c_int),
-                ("uE", C.c_double * 3), ("uH", C.c_double * 3)]
+                ("uE", C.c_double * 3), ("uH", C.c_double * 3),
+                ("mlp", C.c_void_p), ("mlp_zs", C.c_void_p), ("mlp_nz", C.c_int), ("pad3", C.c_int), ("mlp_geom", C.c_double * 6)]
 
 
 class FieldProgramT(C.Structure):
-    _fields_ = [("comp", C.c_void_p), ("ncomp", C.c_int), ("bounds_mode", C.c_int), ("bounds", C.c_double * 6)]
+    _fields_ = [("comp", C.c_void_p), ("ncomp", C.c_int), ("bounds_mode", C.c_int), ("flags", C.c_int), ("pad", C.c_int),
+                ("bounds", C.c_double * 6)]
+
+
+PROGRAM_HAS_MLP = 1
 
 
 class TraceParamsT(C.Structure):
@@ -72,7 +77,7 @@ class TraceParamsT(C.Structure):
                 ("times", C.c_void_p), ("traj_cap", C.c_longlong), ("traj_len", C.c_void_p)]
 
 
-COMP_KIND = {"bem": 0, "axial": 1, "uniform": 2}
+COMP_KIND = {"bem": 0, "axial": 1, "uniform": 2, "learned": 1}
 BOUNDS_NONE, BOUNDS_STRICT, BOUNDS_INCLUSIVE = 0, 1, 2
 STATUS = {0: "left bounds", 1: "block full", 2: "step cap"}
 
@@ -117,10 +122,21 @@ def make_component(field, factor: float, kind: str, in_superposition: bool):
         keep.append(_set_sources(c.elec, field.electrostatic_point_charges))
         keep.append(_set_sources(c.mag, field.magnetostatic_point_charges))
         keep.append(_set_sources(c.cur, field.current_point_charges))
-    elif kind == "axial":
+    elif kind in ("axial", "learned"):
         z, ce, cm = field.device_arrays()
         c.zs, c.ce, c.cm, c.nz = z.data_ptr(), ce.data_ptr(), cm.data_ptr(), len(field.z)
         keep += [z, ce, cm]
+        if kind == "learned":
+            # the learned correction, and the ring sources for outside its trust region
+            sur = field.surrogate
+            wts, zs, geom = sur.tracer_arrays()
+            c.mlp, c.mlp_zs, c.mlp_nz = wts.data_ptr(), zs.data_ptr(), zs.shape[1]
+            for i, v in enumerate(geom):
+                c.mlp_geom[i] = v
+            keep += [wts, zs]
+            keep.append(_set_sources(c.elec, sur.field.electrostatic_point_charges))
+            keep.append(_set_sources(c.mag, sur.field.magnetostatic_point_charges))
+            keep.append(_set_sources(c.cur, sur.field.current_point_charges))
     else:
         for i in range(3):
             c.uE[i] = field.E[i]
@@ -139,6 +155,7 @@ class DeviceFieldProgram:
         # ring-source sums make a field evaluation cost ~10^5 pairs
         self.n_sources = int(sum(c.elec.n + c.mag.n + c.cur.n for c, _ in comps))
         self.has_sources = self.n_sources > 0
+        self.has_mlp = any(c.mlp for c, _ in comps)
         n = len(comps)
         raw = (FieldCompT * n)(*[c for c, _ in comps])
         host = torch.frombuffer(bytearray(bytes(raw)), dtype=torch.uint8)
@@ -146,6 +163,7 @@ class DeviceFieldProgram:
         self.prog = FieldProgramT()
         self.prog.comp = self.comp_buf.data_ptr()
         self.prog.ncomp = n
+        self.prog.flags = PROGRAM_HAS_MLP if self.has_mlp else 0
         # the superposition's own bounds (field.py:285-290 on the top-level object)
         if isinstance(field, FieldSuperposition) and field.field_bounds is not None:
             self.prog.bounds_mode = BOUNDS_INCLUSIVE
@@ -239,7 +257,9 @@ def run_trace(program: DeviceFieldProgram, state0, qm, bounds, atol=1e-8, plane=
         tlen = torch.empty(n, dtype=torch.int64, device=dev)
         P.traj, P.times, P.traj_cap, P.traj_len = traj.data_ptr(), times.data_ptr(), int(traj_cap), tlen.data_ptr()
     if split is None:
-        split = program.has_sources and n <= SPLIT_MAX_PARTICLES
+        split = program.has_sources and n <= SPLIT_MAX_PARTICLES and not program.has_mlp
+    if split and program.has_mlp:
+        raise ValueError("fields with a learned correction are traced by the persistent tracer (split=False)")
     L = backend.backend_lib
     if split:
         nchunk = L.tb_trace_split_chunks(n, program.n_sources)
@@ -263,7 +283,7 @@ class Tracer:
         self.bounds = bounds
         self._program = None
         from .field import FieldRadialBEM, FieldRadialAxial
-        if isinstance(field, (FieldRadialBEM, FieldRadialAxial)):
+        if type(field) in (FieldRadialBEM, FieldRadialAxial):
             self.trace_fun, self.low_level_args = field.get_low_level_trace_function()
             self.trace_args = C.cast(C.pointer(self.low_level_args), C.c_void_p)
         else:
