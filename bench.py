@@ -121,7 +121,7 @@ def field_grid_leg(X, lines, types, values, n_side=1000):
     return t.ms, pairs, P, Fd, pts
 
 
-def tracing_leg(n_particles, n_sample, n_axial=3400, seed=2):
+def tracing_leg(n_particles, n_sample, n_axial=7000, seed=2):
     """Batched tracing through the 4-electrode lens (field from a GPU solve)."""
     import torch
     from traceon_amd import configs, solver, tracing
@@ -134,9 +134,10 @@ def tracing_leg(n_particles, n_sample, n_axial=3400, seed=2):
     field = solver.solve_direct(exc)
     bounds = np.array([[-1.0 * mm, 1.0 * mm], [-1.0 * mm, 1.0 * mm], [-5.0 * mm, 10.0 * mm]])
     t0 = time.perf_counter()
-    # the interpolation extends 1 mm past the tracing bounds on both sides, so that no RK stage
-    # point falls outside it and the interval-end conditions stay away from the traced region
-    axial = FieldRadialAxial(field, -6.0 * mm, 11.0 * mm, N=n_axial)
+    # The interpolation covers the tracing bounds plus the longest step (hmax |v| = 1 cm,
+    # tracing.c:68-70) on both sides: RK stage points of a step can lie that far ahead, and
+    # FieldRadialAxial is zero outside its range (radial.c:313-384).  5 um sample spacing.
+    axial = FieldRadialAxial(field, -15.0 * mm, 20.0 * mm, N=n_axial)
     torch.cuda.synchronize()
     t_axial_build = time.perf_counter() - t0
     rng = np.random.default_rng(seed)
@@ -179,7 +180,7 @@ def tracing_leg(n_particles, n_sample, n_axial=3400, seed=2):
     }
 
 
-def c3_leg(n_particles, n_sample, n_axial=3400, seed=3):
+def c3_leg(n_particles, n_sample, n_axial=7000, seed=3):
     """Magnetic + electrostatic lens (C3): coil 500 A-turn, two mu_r = 1000 pole pieces (bore 2 mm,
     gap 3 mm) and a three-tube einzel lens, N = 12,000; electrostatic and magnetostatic solves,
     10 keV electrons traced through the combined field with the Lorentz force."""
@@ -199,7 +200,7 @@ def c3_leg(n_particles, n_sample, n_axial=3400, seed=3):
     torch.cuda.synchronize()
     t_solve = time.perf_counter() - t0
     t0 = time.perf_counter()
-    axial = FieldRadialAxial(field, -15.0 * mm, 13.0 * mm, N=int(n_axial * 28 / 17))
+    axial = FieldRadialAxial(field, -24.0 * mm, 22.0 * mm, N=int(n_axial * 46 / 35))
     torch.cuda.synchronize()
     t_axial = time.perf_counter() - t0
     rng = np.random.default_rng(seed)
@@ -225,13 +226,53 @@ def c3_leg(n_particles, n_sample, n_axial=3400, seed=3):
     land_ref = ref.landing.cpu().numpy()
     ok = np.isfinite(land_ref[:, 0]) & np.isfinite(land[:n_sample, 0])
     err = np.linalg.norm(land[:n_sample][ok, :2] - land_ref[ok, :2], axis=1)
-    return {"N_electrostatic": len(field.electrostatic_point_charges), "N_magnetostatic": len(field.magnetostatic_point_charges),
-            "N_coil_rings": len(field.current_point_charges), "solve_s": t_solve, "axial_build_s": t_axial, "n": n,
-            "ms": t.ms, "electrons_per_s": n / (t.ms * 1e-3), "steps": int(res.n_attempted.sum().item()),
-            "landed_fraction": float(np.mean(np.isfinite(land[:, 0]))),
+    out = {"N_electrostatic": len(field.electrostatic_point_charges), "N_magnetostatic": len(field.magnetostatic_point_charges),
+           "N_coil_rings": len(field.current_point_charges), "solve_s": t_solve, "axial_build_s": t_axial, "n": n,
+           "ms": t.ms, "electrons_per_s": n / (t.ms * 1e-3), "steps": int(res.n_attempted.sum().item()),
+           "landed_fraction": float(np.mean(np.isfinite(land[:, 0]))),
+           "landing_rms_nm": float(np.sqrt(np.mean(err ** 2)) * 1e9) if ok.any() else None,
+           "landing_max_nm": float(np.max(err) * 1e9) if ok.any() else None, "n_sample": int(n_sample),
+           "electrons_per_s_direct_bem_fp64": n_sample / (tb.ms * 1e-3)}
+    ctx = {"field": field, "axial": axial, "P": P, "V": V, "bounds": bounds, "plane": plane, "land_ref": land_ref}
+    return out, ctx
+
+
+def c4_leg(ctx, n_samples=1 << 22, steps=8000):
+    """Learned surrogate (C4) in the C3 geometry: the MLP (2-128x4-5, SiLU, BF16 MFMA, FP32
+    master weights) learns BEM - axial from n_samples BEM-labelled (r, z) points over the lens
+    region; the tracer adds it to the axial field inside r <= 0.6 mm and uses the direct BEM
+    sum outside.  Reported: build times, field error per radial band with and without the
+    correction, and 10 keV tracing of the same electrons as C3 against direct FP64 BEM."""
+    import torch
+    from traceon_amd.surrogate import FieldRadialAxialLearned, FieldSurrogate
+    mm = 1e-3
+    field, axial, P, V = ctx["field"], ctx["axial"], ctx["P"], ctx["V"]
+    bounds, plane, land_ref = ctx["bounds"], ctx["plane"], ctx["land_ref"]
+    sur = FieldSurrogate(field, axial, r_max=0.6 * mm, z_min=bounds[2, 0], z_max=bounds[2, 1])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sur.generate(n_samples)
+    torch.cuda.synchronize()
+    t_gen = time.perf_counter() - t0
+    hist = sur.train(steps=steps, batch=16384, log_every=max(steps // 4, 1))
+    val = sur.validate()
+    learned = FieldRadialAxialLearned(sur)
+    tr = learned.get_tracer(bounds)
+    tr.trace_batch(P[:4096], V[:4096], plane=plane)
+    torch.cuda.synchronize()
+    with Timer() as t:
+        res = tr.trace_batch(P, V, plane=plane)
+    land = res.landing.cpu().numpy()
+    ns = len(land_ref)
+    ok = np.isfinite(land_ref[:, 0]) & np.isfinite(land[:ns, 0])
+    err = np.linalg.norm(land[:ns][ok, :2] - land_ref[ok, :2], axis=1)
+    bands = {k: {"axial_E_rms": v["axial"]["E_rms"], "learned_E_rms": v["learned"]["E_rms"], "axial_H_rms": v["axial"]["H_rms"],
+                 "learned_H_rms": v["learned"]["H_rms"]} for k, v in val.items()}
+    return {"model": "MLP 2-128-128-128-128-5 SiLU, BF16 MFMA hidden layers, FP32 master weights", "samples": n_samples,
+            "train_steps": steps, "batch": 16384, "label_s": t_gen, "train_s": sur.train_seconds, "loss_history": hist,
+            "field_error_by_radius": bands, "n": len(P), "ms": t.ms, "electrons_per_s": len(P) / (t.ms * 1e-3),
             "landing_rms_nm": float(np.sqrt(np.mean(err ** 2)) * 1e9) if ok.any() else None,
-            "landing_max_nm": float(np.max(err) * 1e9) if ok.any() else None, "n_sample": int(n_sample),
-            "electrons_per_s_direct_bem_fp64": n_sample / (tb.ms * 1e-3)}
+            "landing_max_nm": float(np.max(err) * 1e9) if ok.any() else None}
 
 
 def cpu_baseline(lines, types, values, trace_info, threads):
@@ -304,11 +345,12 @@ def main():
     ap.add_argument("--N", type=int, default=20000)
     ap.add_argument("--particles", type=int, default=1_000_000)
     ap.add_argument("--sample", type=int, default=1024)
-    ap.add_argument("--n-axial", type=int, default=3400)
+    ap.add_argument("--n-axial", type=int, default=7000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-trace", action="store_true")
     ap.add_argument("--no-field", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--no-c4", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md for how it would shard)",
@@ -377,10 +419,16 @@ def main():
         log(f"bench: traced {trace_info['n']} electrons in {trace_info['ms']:.1f} ms ({eps:.3e}/s), "
             f"landing RMS {trace_info['landing_rms_nm']} nm vs direct BEM")
     if not args.no_c3:
-        c3 = c3_leg(args.particles, args.sample, args.n_axial)
+        c3, ctx = c3_leg(args.particles, args.sample, args.n_axial)
         result["c3_magnetic_lens"] = c3
         log(f"bench: C3 solve {c3['solve_s']:.3f} s, traced {c3['n']} electrons in {c3['ms']:.1f} ms "
             f"({c3['electrons_per_s']:.3e}/s), landing RMS {c3['landing_rms_nm']} nm vs direct BEM")
+        if not args.no_c4:
+            c4 = c4_leg(ctx)
+            c4["speedup_vs_direct_bem_fp64"] = c4["electrons_per_s"] / c3["electrons_per_s_direct_bem_fp64"]
+            result["c4_surrogate"] = c4
+            log(f"bench: C4 surrogate labels {c4['label_s']:.2f} s, training {c4['train_s']:.2f} s, traced {c4['n']} electrons "
+                f"in {c4['ms']:.1f} ms ({c4['electrons_per_s']:.3e}/s), landing RMS {c4['landing_rms_nm']} nm vs direct BEM")
     if not args.no_cpu:
         threads = min(16, len(os.sched_getaffinity(0)))
         cb = cpu_baseline(lines, types, values, trace_info, threads)

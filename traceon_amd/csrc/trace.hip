@@ -101,39 +101,49 @@ __device__ void bem_component(const FieldComp& c, const double q[3], double e[3]
 
 // Learned off-axis correction of an axial component (FieldSurrogate): per lane, FP32
 // arithmetic with FP32 weights read as wave-uniform scalars, BF16 activations in this
-// wave's LDS columns act[layer parity][neuron][lane] (as the training forward stores them);
+// wave's LDS columns g_mlp_act[layer parity][neuron][lane] (as the training forward stores
+// them; the workgroups that evaluate corrections are one wave);
 // layer 5 is folded into layer 4 so that layer 4's output stays FP32.  Adds
 // y_k * w(r) * s_k(z), w = (r/r_max)^p, to (E_r, E_z, H_r, H_z) for r >= r_lo.
 constexpr int TMLP_B1 = 256, TMLP_W2 = 384, TMLP_W3 = TMLP_W2 + 16512, TMLP_W4 = TMLP_W3 + 16512, TMLP_W5 = TMLP_W4 + 16512;
-__device__ void mlp_correct(const FieldComp& c, const double q[3], double e[3], double h[3], uint16_t* act, int lane) {
+__shared__ uint16_t g_mlp_act[2 * 128 * 64];   // [layer parity][neuron][lane] of the wave in this workgroup
+typedef const __attribute__((address_space(4))) float cfloat_t;   // read-only weights: scalar loads
+
+__device__ __forceinline__ cfloat_t* uniform_weights(const float* p) {
+  // the component is the same for every lane of the wave: make the pointer wave-uniform
+  const uint64_t v = reinterpret_cast<uint64_t>(p);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return reinterpret_cast<cfloat_t*>(((uint64_t)hi << 32) | lo);
+}
+
+__device__ void mlp_correct(const FieldComp& c, const double q[3], double e[3], double h[3], int lane) {
   const double r = sqrt(q[0] * q[0] + q[1] * q[1]);
   const double r_lo = c.mlp_geom[0], r_max = c.mlp_geom[1], z0 = c.mlp_geom[2], z1 = c.mlp_geom[3];
-  if (r < r_lo) return;
+  if (r < r_lo || q[2] < z0 || q[2] > z1) return;
   const float u = (float)(2.0 * (r - r_lo) / (r_max - r_lo) - 1.0);
   const float v = (float)(2.0 * (q[2] - z0) / (z1 - z0) - 1.0);
-  const float* W = c.mlp;
-  uint16_t* a[2] = {act, act + 128 * 64};
+  cfloat_t* W = uniform_weights(c.mlp);
   for (int j = 0; j < 128; j++)
-    a[0][j * 64 + lane] = f2bf(silu(fmaf(W[2 * j], u, fmaf(W[2 * j + 1], v, W[TMLP_B1 + j]))));
+    g_mlp_act[j * 64 + lane] = f2bf(silu(fmaf(W[2 * j], u, fmaf(W[2 * j + 1], v, W[TMLP_B1 + j]))));
   float y[5];
 #pragma unroll
   for (int o = 0; o < 5; o++) y[o] = W[TMLP_W5 + 640 + o];
   for (int layer = 0; layer < 3; layer++) {
-    const float* WT = W + TMLP_W2 + layer * 16512;   // (in, out), then the 128 biases
-    const uint16_t* in = a[layer & 1];
-    uint16_t* out = a[(layer + 1) & 1];
+    cfloat_t* WT = W + TMLP_W2 + layer * 16512;   // (in, out), then the 128 biases
+    const int in0 = (layer & 1) * 128 * 64, out0 = ((layer + 1) & 1) * 128 * 64;
     for (int jc = 0; jc < 128; jc += 16) {
       float acc[16];
 #pragma unroll
       for (int t = 0; t < 16; t++) acc[t] = WT[16384 + jc + t];
       for (int i = 0; i < 128; i++) {
-        const float x = bf2f(in[i * 64 + lane]);
+        const float x = bf2f(g_mlp_act[in0 + i * 64 + lane]);
 #pragma unroll
         for (int t = 0; t < 16; t++) acc[t] = fmaf(WT[i * 128 + jc + t], x, acc[t]);
       }
       if (layer < 2) {
 #pragma unroll
-        for (int t = 0; t < 16; t++) out[(jc + t) * 64 + lane] = f2bf(silu(acc[t]));
+        for (int t = 0; t < 16; t++) g_mlp_act[out0 + (jc + t) * 64 + lane] = f2bf(silu(acc[t]));
       } else {
 #pragma unroll
         for (int t = 0; t < 16; t++) {
@@ -164,12 +174,12 @@ __device__ void mlp_correct(const FieldComp& c, const double q[3], double e[3], 
 // Field of a program at p.  Ring-source sums run over sources s0, s0+ds, ...; terms without
 // a source sum (axial, uniform) are added only when `lead` is set, so that partial results
 // of several threads add up to the field and (0, 1, true) gives the whole field in one lane.
-// MLP = true (act: this wave's LDS) applies learned corrections of axial components inside
+// MLP = true (one-wave workgroups) applies learned corrections of axial components inside
 // their trust region and their ring sources outside it; MLP = false evaluates such a
 // component as its plain axial field (programs with corrections are traced with MLP = true).
 template <bool MLP = false>
 __device__ void eval_field(const FieldProgram& F, const double p[3], double E[3], double H[3], int s0 = 0, int ds = 1,
-                           bool lead = true, uint16_t* act = nullptr, int lane = 0) {
+                           bool lead = true, int lane = 0) {
   E[0] = E[1] = E[2] = 0.0;
   H[0] = H[1] = H[2] = 0.0;
   if (!in_bounds(F.bounds, F.bounds_mode, p)) return;
@@ -184,8 +194,10 @@ __device__ void eval_field(const FieldProgram& F, const double p[3], double E[3]
     } else if (c.kind == COMP_AXIAL) {
       bool use_bem = false;
       if (MLP && c.mlp) {
+        // the axial series is trusted up to r_max inside its interpolation range; the learned
+        // correction adds to it where it was trained (mlp_correct); elsewhere: ring sources
         const double r = sqrt(q[0] * q[0] + q[1] * q[1]);
-        const bool trust = r <= c.mlp_geom[1] && q[2] >= c.mlp_geom[2] && q[2] <= c.mlp_geom[3];
+        const bool trust = r <= c.mlp_geom[1] && q[2] >= c.zs[0] && q[2] <= c.zs[c.nz - 1];
         use_bem = !trust && (c.elec.n || c.mag.n || c.cur.n);
       }
       if (MLP && use_bem) {
@@ -193,7 +205,7 @@ __device__ void eval_field(const FieldProgram& F, const double p[3], double E[3]
       } else if (lead) {
         axial_field(c.zs, c.nz, c.ce, q[0], q[1], q[2], e);
         axial_field(c.zs, c.nz, c.cm, q[0], q[1], q[2], h);
-        if (MLP && c.mlp) mlp_correct(c, q, e, h, act, lane);
+        if (MLP && c.mlp) mlp_correct(c, q, e, h, lane);
       }
     } else if (lead) {
       for (int i = 0; i < 3; i++) { e[i] = c.uE[i]; h[i] = c.uH[i]; }
@@ -220,8 +232,6 @@ __device__ __forceinline__ bool inside_trace_bounds(const double* b, const doubl
 template <bool MLP>
 __global__ __launch_bounds__(64) void k_trace(TraceParams P) {
   const int lane = threadIdx.x & 63;
-  __shared__ uint16_t act_s[MLP ? 2 * 128 * 64 : 1];   // learned-correction activations, 32 KB
-  uint16_t* act = MLP ? act_s : nullptr;
   // lane state
   int pid = -1;
   double y[6], h = 0.0, hmax = 0.0, t = 0.0, qm = 0.0;
@@ -306,7 +316,7 @@ __global__ __launch_bounds__(64) void k_trace(TraceParams P) {
       }
       double E[3], H[3];
       if (active) {
-        eval_field<MLP>(P.field, ys, E, H, 0, 1, true, act, lane);
+        eval_field<MLP>(P.field, ys, E, H, 0, 1, true, lane);
       } else {   // a lane without a particle: no field work (its state is stale or out of range)
         E[0] = E[1] = E[2] = H[0] = H[1] = H[2] = 0.0;
       }
@@ -622,12 +632,11 @@ hipError_t trace_split(const TraceParams& P, int nchunk, void* ws, hipStream_t s
 template <bool MLP>
 __global__ __launch_bounds__(64) void k_eval_program(FieldProgram F, const double* __restrict__ pts, int M, double* __restrict__ E,
                                                      double* __restrict__ H) {
-  __shared__ uint16_t act_s[MLP ? 2 * 128 * 64 : 1];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= M) return;
   double p[3] = {pts[3 * (size_t)t], pts[3 * (size_t)t + 1], pts[3 * (size_t)t + 2]};
   double e[3], h[3];
-  eval_field<MLP>(F, p, e, h, 0, 1, true, MLP ? act_s : nullptr, threadIdx.x & 63);
+  eval_field<MLP>(F, p, e, h, 0, 1, true, threadIdx.x & 63);
   for (int i = 0; i < 3; i++) { if (E) E[3 * (size_t)t + i] = e[i]; if (H) H[3 * (size_t)t + i] = h[i]; }
 }
 

@@ -15,9 +15,10 @@ therefore D_k / (w(r) s_k(z)), with w(r) = (r/r_max)^p and s_k(z) the RMS of D_k
 z bins, so that they are O(1) everywhere and the fit error scales with the local residual.
 Below r_lo the axial series is already accurate, so there is no correction there.
 
-Inside the trust region (r <= r_max, z in the interpolation range) the traced field is
-F_axial + correction. Outside it, or where the correction's error check fails, the tracer
-falls back to the direct BEM sum. The surrogate has no counterpart in the reference.
+In the tracer the field is F_axial + correction inside the trust region, which is r <= r_max
+within the axial interpolation range. The correction applies for r >= r_lo within the z range
+it was trained on. Outside the trust region the tracer falls back to the direct BEM sum. The
+surrogate has no counterpart in the reference.
 """
 from __future__ import annotations
 
@@ -253,13 +254,15 @@ class FieldRadialAxialLearned(FieldRadialAxial):
         self._update_inverse_transformation_matrix()
 
     def _corrected(self, points):
+        """Trust region (as the tracer's): r <= r_max inside the axial range; the correction
+        where r >= r_lo inside the trained z range."""
         P = np.atleast_2d(np.asarray(points, dtype=np.float64))
         s = self.surrogate
         r = np.hypot(P[:, 0], P[:, 1])
-        inside = (r <= s.r_max) & (P[:, 2] >= s.z_min) & (P[:, 2] <= s.z_max)
+        inside = (r <= s.r_max) & (P[:, 2] >= self.z[0]) & (P[:, 2] <= self.z[-1])
         dev = device.device()
         C = s.correction(torch.tensor(r, device=dev), torch.tensor(P[:, 2], device=dev)).cpu().numpy()
-        C[~inside] = 0.0
+        C[~((P[:, 2] >= s.z_min) & (P[:, 2] <= s.z_max) & inside)] = 0.0
         return P, r, inside, C
 
     def electrostatic_field_at_points(self, points):
