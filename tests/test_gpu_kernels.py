@@ -35,16 +35,27 @@ def test_gemm_mfma_layout(dev):
     import torch
     rng = np.random.default_rng(0)
     for dt, tol in ((torch.float64, 1e-13), (torch.float32, 1e-5)):
-        for (M, N, K) in ((37, 53, 19), (128, 128, 128), (200, 300, 128), (129, 257, 64)):
+        # edge tiles, whole tiles, several tiles and K chunks, K not a multiple of the chunk,
+        # K = 0, beta = 0, and operands that are strided views (element-wise path)
+        for (M, N, K, beta, view) in ((37, 53, 19, 1.0, False), (128, 128, 128, 1.0, False), (200, 300, 128, 1.0, False),
+                                      (129, 257, 64, 1.0, False), (256, 384, 96, 1.0, False), (384, 256, 160, 0.0, False),
+                                      (256, 256, 0, 1.0, False), (256, 128, 128, 1.0, True)):
             A = rng.standard_normal((M, K))
             Bm = rng.standard_normal((K, N))
             C0 = rng.standard_normal((M, N))
             a = torch.tensor(A, dtype=dt, device="cuda")
             b = torch.tensor(Bm, dtype=dt, device="cuda")
             c = torch.tensor(C0, dtype=dt, device="cuda")
-            out = dev.gemm(a, b, c, alpha=-1.0, beta=1.0).double().cpu().numpy()
-            ref = C0 - A @ Bm
-            print("gemm", dt, M, N, K, rel(out, ref), flush=True)
+            if view:   # leading dimensions M+1, N+3 and N+1: unaligned rows
+                a = torch.zeros((M, K + 1), dtype=dt, device="cuda")[:, 1:]
+                a.copy_(torch.tensor(A, dtype=dt))
+                b = torch.zeros((K, N + 3), dtype=dt, device="cuda")[:, 3:]
+                b.copy_(torch.tensor(Bm, dtype=dt))
+                c = torch.zeros((M, N + 1), dtype=dt, device="cuda")[:, 1:]
+                c.copy_(torch.tensor(C0, dtype=dt))
+            out = dev.gemm(a, b, c, alpha=-1.0, beta=beta).double().cpu().numpy()
+            ref = beta * C0 - A @ Bm
+            print("gemm", dt, M, N, K, beta, view, rel(out, ref), flush=True)
             assert rel(out, ref) < tol
 
 

@@ -369,12 +369,16 @@ __global__ __launch_bounds__(256) void k_copy_back_max(const T* __restrict__ S, 
 // ---------------------------------------------------------------------------------
 // GEMM on MFMA: C(MxN) = beta*C + alpha*A(MxK)*B(KxN), all row-major.
 // Block 256 threads = 2x2 waves, block tile 128x128, wave tile 64x64 made of 4x4
-// MFMA 16x16 tiles; K staged through LDS in chunks of BK, A stored k-major in LDS so
-// the 16 rows an MFMA reads are consecutive.  Instruction per element type:
+// MFMA 16x16 tiles; K staged through LDS in chunks of BK.  Instruction per element type:
 //   double: v_mfma_f64_16x16x4f64, float: v_mfma_f32_16x16x4f32
 // Operand layout of both (16x16x4, one block): lane l holds A[l%16][l/16] and
 // B[l/16][l%16]; the four accumulator values v = 0..3 of lane l are D[row][l%16] with
 // row = l/16 + 4v for the f64 instruction and row = 4(l/16) + v for the f32 one.
+//
+// LDS: A as [row][k] with a row stride of 36 words, B as [k][col] with 144 words (in
+// units of 4 bytes; 18 / 144 elements for double).  The 16 rows x 4 k an MFMA reads from
+// A and the 4 k x 16 cols it reads from B then fall in distinct banks, and both tiles are
+// written with one 16-byte store per global vector load.
 // ---------------------------------------------------------------------------------
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -394,9 +398,14 @@ template <> struct MfmaAcc<float> {
 constexpr int GBM = 128, GBN = 128;
 
 // Software pipeline: the global loads of K-chunk kt+1 are issued into registers before the
-// MFMA loop of chunk kt and written to the other LDS buffer after it, so their latency hides
-// behind the matrix instructions.  Loads are 16-byte vectors along the contiguous dimension
-// (k for A, n for B).  A is stored k-major in LDS (the 16 rows an MFMA reads are adjacent).
+// MFMA loop of chunk kt and written to the other LDS buffer after it; within a chunk the
+// operand fragments of step s+1 are read from LDS before the MFMAs of step s.
+// Write-back goes through LDS: the accumulators (times alpha) are stored in the MFMA layout,
+// then each thread reads back 16-byte row segments and updates C with vector loads and
+// stores, 8 segments in flight per thread, so the read-modify-write of a tile costs a few
+// memory latencies instead of one per element.  Tiles that lie wholly inside the matrices
+// (with 16-byte aligned rows) take unguarded vector accesses; edge tiles take element
+// accesses with bounds checks.
 template <typename T, int BK>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, const T* __restrict__ A, size_t lda,
                                               const T* __restrict__ B, size_t ldb, T beta, T* __restrict__ C, size_t ldc) {
@@ -404,14 +413,22 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, cons
   constexpr int VEC = 16 / sizeof(T);                     // elements per 16-byte load
   constexpr int A_LD = (GBM * BK) / (256 * VEC);          // vector loads per thread for A
   constexpr int B_LD = (BK * GBN) / (256 * VEC);
-  constexpr int APAD = (sizeof(T) == 8) ? 2 : 4;
-  __shared__ T As[2][BK][GBM + APAD];
-  __shared__ T Bs[2][BK][GBN + APAD];
+  constexpr int SA = BK + 16 / sizeof(T);                 // 36 words (float), 18 doubles
+  constexpr int SB = GBN + 64 / sizeof(T);                // 144 words (float), 136 doubles
+  constexpr int KS = BK / 4;                              // MFMA steps per chunk
+  constexpr int SC = GBN + 4;                             // row stride of the staged C tile
+  constexpr size_t AB_BYTES = sizeof(T) * (2 * GBM * SA + 2 * BK * SB);
+  constexpr int CROWS = (int)((AB_BYTES / (sizeof(T) * SC)) >= GBM ? GBM : 64);   // tile rows staged per pass
+  static_assert(CROWS * SC * sizeof(T) <= AB_BYTES, "staged C tile exceeds the LDS of the operands");
+  __shared__ __attribute__((aligned(16))) char smem[AB_BYTES];
+  T (*As)[GBM][SA] = reinterpret_cast<T (*)[GBM][SA]>(smem);
+  T (*Bs)[BK][SB] = reinterpret_cast<T (*)[BK][SB]>(smem + sizeof(T) * 2 * GBM * SA);
+  T* Cs = reinterpret_cast<T*>(smem);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
   const int bm = blockIdx.y * GBM, bn = blockIdx.x * GBN;
-  const bool vec_ok_a = ((lda % VEC) == 0) && ((((uintptr_t)A) % 16) == 0);
-  const bool vec_ok_b = ((ldb % VEC) == 0) && ((((uintptr_t)B) % 16) == 0);
+  const bool full = ((lda % VEC) == 0) && ((((uintptr_t)A) % 16) == 0) && ((ldb % VEC) == 0) && ((((uintptr_t)B) % 16) == 0) &&
+                    bm + GBM <= M && bn + GBN <= N && (K % BK) == 0;
 
   acc_t acc[4][4];
 #pragma unroll
@@ -419,43 +436,33 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, cons
 #pragma unroll
     for (int b = 0; b < 4; b++) acc[a][b] = acc_t{0, 0, 0, 0};
 
-  T ra[A_LD][VEC], rb[B_LD][VEC];
+  typedef T vec_t __attribute__((ext_vector_type(VEC)));
+  vec_t ra[A_LD], rb[B_LD];
 
   auto fetch = [&](int k0) {
+    if (full) {
 #pragma unroll
-    for (int i = 0; i < A_LD; i++) {
-      const int e = (tid + i * 256) * VEC;   // element index in the GBM x BK tile, k fastest
-      const int r = e / BK, k = e % BK;
-      const int gr = bm + r, gk = k0 + k;
-      if (vec_ok_a && gr < M && gk + VEC <= K) {
-        const T* src = A + (size_t)gr * lda + gk;
-        if constexpr (sizeof(T) == 8) {
-          double2 v = *reinterpret_cast<const double2*>(src);
-          ra[i][0] = v.x; ra[i][1] = v.y;
-        } else {
-          float4 v = *reinterpret_cast<const float4*>(src);
-          ra[i][0] = v.x; ra[i][1] = v.y; ra[i][2] = v.z; ra[i][3] = v.w;
-        }
-      } else {
+      for (int i = 0; i < A_LD; i++) {
+        const int e = (tid + i * 256) * VEC;   // element index in the GBM x BK tile, k fastest
+        ra[i] = *reinterpret_cast<const vec_t*>(A + (size_t)(bm + e / BK) * lda + k0 + e % BK);
+      }
+#pragma unroll
+      for (int i = 0; i < B_LD; i++) {
+        const int e = (tid + i * 256) * VEC;   // element index in the BK x GBN tile, n fastest
+        rb[i] = *reinterpret_cast<const vec_t*>(B + (size_t)(k0 + e / GBN) * ldb + bn + e % GBN);
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < A_LD; i++) {
+        const int e = (tid + i * 256) * VEC;
+        const int gr = bm + e / BK, gk = k0 + e % BK;
 #pragma unroll
         for (int v = 0; v < VEC; v++) ra[i][v] = (gr < M && gk + v < K) ? A[(size_t)gr * lda + gk + v] : T(0);
       }
-    }
 #pragma unroll
-    for (int i = 0; i < B_LD; i++) {
-      const int e = (tid + i * 256) * VEC;   // element index in the BK x GBN tile, n fastest
-      const int k = e / GBN, c = e % GBN;
-      const int gk = k0 + k, gc = bn + c;
-      if (vec_ok_b && gk < K && gc + VEC <= N) {
-        const T* src = B + (size_t)gk * ldb + gc;
-        if constexpr (sizeof(T) == 8) {
-          double2 v = *reinterpret_cast<const double2*>(src);
-          rb[i][0] = v.x; rb[i][1] = v.y;
-        } else {
-          float4 v = *reinterpret_cast<const float4*>(src);
-          rb[i][0] = v.x; rb[i][1] = v.y; rb[i][2] = v.z; rb[i][3] = v.w;
-        }
-      } else {
+      for (int i = 0; i < B_LD; i++) {
+        const int e = (tid + i * 256) * VEC;
+        const int gk = k0 + e / GBN, gc = bn + e % GBN;
 #pragma unroll
         for (int v = 0; v < VEC; v++) rb[i][v] = (gk < K && gc + v < N) ? B[(size_t)gk * ldb + gc + v] : T(0);
       }
@@ -465,58 +472,112 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, cons
 #pragma unroll
     for (int i = 0; i < A_LD; i++) {
       const int e = (tid + i * 256) * VEC;
-      const int r = e / BK, k = e % BK;
-#pragma unroll
-      for (int v = 0; v < VEC; v++) As[buf][k + v][r] = ra[i][v];
+      *reinterpret_cast<vec_t*>(&As[buf][e / BK][e % BK]) = ra[i];
     }
 #pragma unroll
     for (int i = 0; i < B_LD; i++) {
       const int e = (tid + i * 256) * VEC;
-      const int k = e / GBN, c = e % GBN;
-#pragma unroll
-      for (int v = 0; v < VEC; v++) Bs[buf][k][c + v] = rb[i][v];
+      *reinterpret_cast<vec_t*>(&Bs[buf][e / GBN][e % GBN]) = rb[i];
     }
   };
 
   const int nk = (K + BK - 1) / BK;
-  fetch(0);
-  stash(0);
+  if (nk > 0) {
+    fetch(0);
+    stash(0);
+  }
   __syncthreads();
+  const int ci = lane & 15, kq = lane >> 4;
   for (int kt = 0; kt < nk; kt++) {
     const int cur = kt & 1;
     if (kt + 1 < nk) fetch((kt + 1) * BK);
+    T af[2][4], bf[2][4];
 #pragma unroll
-    for (int kk = 0; kk < BK; kk += 4) {
-      T af[4], bf[4];
-      const int kr = kk + (lane >> 4), ci = lane & 15;
+    for (int a = 0; a < 4; a++) af[0][a] = As[cur][wm + a * 16 + ci][kq];
 #pragma unroll
-      for (int a = 0; a < 4; a++) af[a] = As[cur][kr][wm + a * 16 + ci];
+    for (int b = 0; b < 4; b++) bf[0][b] = Bs[cur][kq][wn + b * 16 + ci];
 #pragma unroll
-      for (int b = 0; b < 4; b++) bf[b] = Bs[cur][kr][wn + b * 16 + ci];
+    for (int s = 0; s < KS; s++) {
+      const int p = s & 1;
+      if (s + 1 < KS) {
+#pragma unroll
+        for (int a = 0; a < 4; a++) af[p ^ 1][a] = As[cur][wm + a * 16 + ci][4 * (s + 1) + kq];
+#pragma unroll
+        for (int b = 0; b < 4; b++) bf[p ^ 1][b] = Bs[cur][4 * (s + 1) + kq][wn + b * 16 + ci];
+      }
 #pragma unroll
       for (int a = 0; a < 4; a++)
 #pragma unroll
-        for (int b = 0; b < 4; b++) acc[a][b] = MfmaAcc<T>::mma(af[a], bf[b], acc[a][b]);
+        for (int b = 0; b < 4; b++) acc[a][b] = MfmaAcc<T>::mma(af[p][a], bf[p][b], acc[a][b]);
     }
     if (kt + 1 < nk) stash(cur ^ 1);
     __syncthreads();
   }
-  // write back
+
+  // write back through LDS, CROWS tile rows per pass
+  const bool use_c = beta != T(0);
+  const bool vec_c = full && ((ldc % VEC) == 0) && ((((uintptr_t)C) % 16) == 0);
+  constexpr int SEG = GBN / VEC;                          // 16-byte segments per tile row
+  constexpr int PER = CROWS * SEG / 256;                  // segments per thread per pass
+  constexpr int BATCH = PER < 8 ? PER : 8;
 #pragma unroll
-  for (int a = 0; a < 4; a++)
+  for (int pass = 0; pass < GBM / CROWS; pass++) {
+    const int r0 = pass * CROWS;
+    if (pass > 0) __syncthreads();
 #pragma unroll
-    for (int b = 0; b < 4; b++) {
-      const int col = bn + wn + b * 16 + (lane & 15);
+    for (int a = 0; a < 4; a++) {
+      if (wm + a * 16 < r0 || wm + a * 16 >= r0 + CROWS) continue;
 #pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int row = bm + wm + a * 16 + MfmaAcc<T>::row(lane, v);
-        if (row < M && col < N) {
-          T* dst = C + (size_t)row * ldc + col;
-          T val = alpha * acc[a][b][v];
-          *dst = (beta == T(0)) ? val : fma(beta, *dst, val);
+      for (int b = 0; b < 4; b++)
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+          Cs[(wm + a * 16 + MfmaAcc<T>::row(lane, v) - r0) * SC + wn + b * 16 + ci] = alpha * acc[a][b][v];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i0 = 0; i0 < PER; i0 += BATCH) {
+      vec_t cvv[BATCH];
+      if (vec_c) {
+        if (use_c) {
+#pragma unroll
+          for (int j = 0; j < BATCH; j++) {
+            const int idx = tid + (i0 + j) * 256, row = r0 + idx / SEG, c = (idx % SEG) * VEC;
+            cvv[j] = *reinterpret_cast<const vec_t*>(C + (size_t)(bm + row) * ldc + bn + c);
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < BATCH; j++) {
+          const int idx = tid + (i0 + j) * 256, row = r0 + idx / SEG, c = (idx % SEG) * VEC;
+          vec_t out;
+#pragma unroll
+          for (int v = 0; v < VEC; v++) {
+            const T val = Cs[(row - r0) * SC + c + v];
+            out[v] = use_c ? fma(beta, cvv[j][v], val) : val;
+          }
+          *reinterpret_cast<vec_t*>(C + (size_t)(bm + row) * ldc + bn + c) = out;
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < BATCH; j++) {
+          const int idx = tid + (i0 + j) * 256, row = r0 + idx / SEG, c = (idx % SEG) * VEC;
+#pragma unroll
+          for (int v = 0; v < VEC; v++)
+            cvv[j][v] = (use_c && bm + row < M && bn + c + v < N) ? C[(size_t)(bm + row) * ldc + bn + c + v] : T(0);
+        }
+#pragma unroll
+        for (int j = 0; j < BATCH; j++) {
+          const int idx = tid + (i0 + j) * 256, row = r0 + idx / SEG, c = (idx % SEG) * VEC;
+#pragma unroll
+          for (int v = 0; v < VEC; v++) {
+            if (bm + row < M && bn + c + v < N) {
+              const T val = Cs[(row - r0) * SC + c + v];
+              C[(size_t)(bm + row) * ldc + bn + c + v] = use_c ? fma(beta, cvv[j][v], val) : val;
+            }
+          }
         }
       }
     }
+  }
 }
 
 // copy a rows x cols block (row-major, separate leading dims)
