@@ -628,39 +628,68 @@ constexpr long long SPIN_LIMIT = 1ll << 22;   // polls of a flag before a wait g
 
 __global__ void k_set_int(int* p, int v) { *p = v; }
 
-// Blocked triangular solve with the stored diagonal-block inverses, one workgroup per
-// block row of 128.  Workgroups take block rows in dependency order from a ticket
-// counter (so a waiting workgroup never blocks one that has not started) and walk their
-// dependencies in the order they complete: forward for L, backward for U.  Per
-// dependency block: wave w accumulates rows 32w..32w+31, lanes across the 128 columns.
-// A workgroup polls the flag of the next block it needs and, once set, also counts the
-// blocks behind it that are already done, so one acquire fence covers the whole run.
+// Blocked triangular solve with the stored diagonal-block inverses, one workgroup of 512
+// threads per block row of 128.  Workgroups take block rows in dependency order from a
+// ticket counter (so a waiting workgroup never blocks one that has not started) and walk
+// their dependencies in solve order: forward for L, backward for U, in chunks of 1 KB of
+// row (256 columns in FP32, 128 in FP64).  Per chunk, wave w loads its rows 16w..16w+15
+// with one 16-byte load per lane and row (branch-free: edge lanes load a clamped address
+// and discard it), then waits for the solution values of the chunk's columns.
+//
+// Completion is carried by the values themselves: the output vector is first filled with
+// TRSV_PENDING (a signalling-NaN bit pattern that no arithmetic produces), solutions are
+// written with relaxed agent-scope atomic stores, and readers poll them with relaxed
+// agent-scope atomic loads until no lane of the wave still sees the marker.  Each value is
+// its own flag, so no release/acquire fence (an L2 writeback / invalidate on this
+// multi-die part) sits on the critical path.  A wait past SPIN_LIMIT polls records the
+// error and writes NaN for the block, which propagates to the residual.
 // The block's inverse is staged in LDS while the dependencies are summed; the row sums
-// are reduced through LDS; x_b = inv(D_b) (b_b - sum) is one LDS matvec.
-constexpr int TRSV_PAD = LU_NB + 1;   // LDS row stride of the staged inverse (conflict-free)
+// are reduced through LDS; x_b = inv(D_b) (b_b - sum) is an LDS matvec with four threads
+// per row.
+constexpr int TRSV_PAD = LU_NB + 1;   // LDS row stride of the staged inverse
 constexpr int TRSV_RED = 16;          // partial sums per row left after two shuffle steps
+constexpr int TRSV_THREADS = 512;
+constexpr unsigned long long TRSV_PENDING = 0x7FF4A5A5C3C3E1E1ull;
 template <typename T>
 constexpr size_t trsv_lds_bytes() {
   return sizeof(T) * LU_NB * TRSV_PAD + sizeof(double) * LU_NB * (TRSV_RED + 1) + sizeof(double) * LU_NB;
 }
 
+// out[0..n) = TRSV_PENDING, *counter = 0 (a kernel, not a memset node, so that the solve
+// can be captured in a graph)
+__global__ __launch_bounds__(256) void k_trsv_prep(double* __restrict__ out, int n, int* __restrict__ counter) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = __longlong_as_double((long long)TRSV_PENDING);
+  if (i == 0) *counter = 0;
+}
+
+__device__ __forceinline__ bool trsv_pending(double v) {
+  return (unsigned long long)__double_as_longlong(v) == TRSV_PENDING;
+}
+
 template <typename T, bool LOWER>
-__global__ __launch_bounds__(256) void k_trsv_blocked(const T* __restrict__ LU, size_t lda, int N, const int* __restrict__ perm,
-                                                      const int* __restrict__ pivoted, const double* Bv,   // Bv may be X
-                                                      double* X, int rhs, const T* __restrict__ Dinv,
-                                                      int* __restrict__ flags, int* __restrict__ counter, int* __restrict__ err) {
+__global__ __launch_bounds__(TRSV_THREADS) void k_trsv_blocked(const T* __restrict__ LU, size_t lda, int N,
+                                                               const int* __restrict__ perm, const int* __restrict__ pivoted,
+                                                               const double* __restrict__ Bv, double* X,
+                                                               const T* __restrict__ Dinv, int* __restrict__ counter,
+                                                               int* __restrict__ err) {
   constexpr int NB = LU_NB;
-  constexpr int RPW = NB / 4;   // rows per wave
+  constexpr int NW = TRSV_THREADS / 64;   // waves
+  constexpr int RPW = NB / NW;            // rows per wave (16)
+  constexpr int VEC = 16 / (int)sizeof(T);
+  constexpr int CH = 64 * VEC;            // columns per chunk
+  typedef T vec_t __attribute__((ext_vector_type(VEC)));
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* Ds = reinterpret_cast<T*>(smem_raw);                                        // NB x TRSV_PAD
   double* part = reinterpret_cast<double*>(smem_raw + sizeof(T) * NB * TRSV_PAD);   // NB x (TRSV_RED + 1)
   double* vs = part + NB * (TRSV_RED + 1);                                         // NB
-  __shared__ int s_b, s_ready[2];
+  __shared__ int s_b, s_fail;
   const int nblk = (N + NB - 1) / NB;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (tid == 0) {
     const int t = atomicAdd(counter, 1);
     s_b = LOWER ? t : nblk - 1 - t;
+    s_fail = 0;
   }
   __syncthreads();
   const int b = s_b;
@@ -669,46 +698,69 @@ __global__ __launch_bounds__(256) void k_trsv_blocked(const T* __restrict__ LU, 
   const int nr = min(NB, N - r0);
   // stage inv(D_b) (independent of every other block)
   const T* Db = Dinv + (size_t)b * NB * NB;
-  for (int e = tid; e < NB * NB; e += 256) Ds[(e / NB) * TRSV_PAD + (e % NB)] = Db[e];
+  for (int e = tid; e < NB * NB; e += TRSV_THREADS) Ds[(e / NB) * TRSV_PAD + (e % NB)] = Db[e];
   double acc[RPW];
 #pragma unroll
   for (int i = 0; i < RPW; i++) acc[i] = 0.0;
-  const double* xr = X + (size_t)rhs * N;
-  const int ndep = LOWER ? b : nblk - 1 - b;
-  int ready = 0, round = 0;
-  for (int k = 0; k < ndep; k++) {
-    const int jb = LOWER ? k : nblk - 1 - k;
-    if (k >= ready) {
-      if (tid == 0) {
-        long long spins = 0;
-        while (__hip_atomic_load(flags + jb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-          if (++spins > SPIN_LIMIT) { atomicOr(err, 1); break; }
-          __builtin_amdgcn_s_sleep(1);
-        }
-        int rdy = k + 1;
-        while (rdy < ndep && __hip_atomic_load(flags + (LOWER ? rdy : nblk - 1 - rdy), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
-          rdy++;
-        s_ready[round & 1] = rdy;
-      }
-      __syncthreads();
-      ready = s_ready[round & 1];
-      round++;
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    }
-    const int c0 = jb * NB;
-    const int nc = min(NB, N - c0);
-    const double x0 = lane < nc ? xr[c0 + lane] : 0.0;
-    const double x1 = lane + 64 < nc ? xr[c0 + lane + 64] : 0.0;
+  // this wave's rows (clamped into the matrix; rows past N are discarded at the end)
+  const T* rowp[RPW];
 #pragma unroll
-    for (int i = 0; i < RPW; i++) {
-      const int row = r0 + wave * RPW + i;
-      if (row < N) {
-        const T* Lr = LU + (size_t)row * lda + c0;
-        double p = lane < nc ? (double)Lr[lane] * x0 : 0.0;
-        if (lane + 64 < nc) p = fma((double)Lr[lane + 64], x1, p);
-        acc[i] += p;
-      }
+  for (int i = 0; i < RPW; i++) rowp[i] = LU + (size_t)min(r0 + wave * RPW + i, N - 1) * lda;
+  const bool vec_ok = (lda % VEC) == 0 && (((uintptr_t)LU) % 16) == 0;
+  // dependency columns [lo, hi), chunks in solve order
+  const int lo = LOWER ? 0 : r0 + nr, hi = LOWER ? r0 : N;
+  const int nch = (hi - lo + CH - 1) / CH;
+  const int top = LOWER ? 0 : nblk * NB;   // U chunks are aligned to the block grid from its end
+  bool failed = false;
+  for (int k = 0; k < nch; k++) {
+    const int cs = LOWER ? top + k * CH : top - (k + 1) * CH;   // nominal chunk start
+    const int c = cs + lane * VEC;                               // this lane's first column
+    vec_t l[RPW];
+    if (vec_ok && cs >= lo && cs + CH <= hi) {
+#pragma unroll
+      for (int i = 0; i < RPW; i++) l[i] = *reinterpret_cast<const vec_t*>(rowp[i] + c);
+    } else {
+#pragma unroll
+      for (int i = 0; i < RPW; i++)
+#pragma unroll
+        for (int v = 0; v < VEC; v++) {
+          const bool in = c + v >= lo && c + v < hi;
+          const T t = rowp[i][in ? c + v : lo];
+          l[i][v] = in ? t : T(0);
+        }
     }
+    double x[VEC];
+#pragma unroll
+    for (int v = 0; v < VEC; v++) x[v] = 0.0;
+    long long spins = 0;
+    while (!failed) {   // after one wait has timed out, the rest are skipped
+      bool pend = false;
+#pragma unroll
+      for (int v = 0; v < VEC; v++) {
+        if (c + v >= lo && c + v < hi) {
+          x[v] = __hip_atomic_load(X + c + v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          pend = pend || trsv_pending(x[v]);
+        }
+      }
+      if (!__any(pend)) break;
+      if (++spins > SPIN_LIMIT) {
+        failed = true;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    if (failed) {
+#pragma unroll
+      for (int v = 0; v < VEC; v++) x[v] = 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < RPW; i++)
+#pragma unroll
+      for (int v = 0; v < VEC; v++) acc[i] = fma((double)l[i][v], x[v], acc[i]);
+  }
+  if (failed && lane == 0) {
+    atomicOr(err, 1);
+    s_fail = 1;
   }
   // row sums: two shuffle steps leave 16 partials per row, summed from LDS in fixed order
 #pragma unroll
@@ -722,26 +774,36 @@ __global__ __launch_bounds__(256) void k_trsv_blocked(const T* __restrict__ LU, 
   if (tid < NB) {
     double ssum = 0.0;
 #pragma unroll
-    for (int l = 0; l < TRSV_RED; l++) ssum += part[tid * (TRSV_RED + 1) + l];
+    for (int q = 0; q < TRSV_RED; q++) ssum += part[tid * (TRSV_RED + 1) + q];
     const int row = r0 + tid;
     double bv = 0.0;
     if (row < N) {
       const bool use_perm = LOWER && perm && pivoted && *pivoted;
-      bv = Bv[(size_t)rhs * N + (use_perm ? perm[row] : row)];
+      bv = Bv[use_perm ? perm[row] : row];
     }
     vs[tid] = row < N ? bv - ssum : 0.0;
   }
   __syncthreads();
-  if (tid < nr) {
-    const T* Dr = Ds + tid * TRSV_PAD;
+  // x = inv(D_b) vs: row 16w + l%16, quarter q = l/16 takes columns [16q, 16q+16) and
+  // [64+16q, 80+16q) (the 64 lanes of a step then read 64 distinct banks)
+  {
+    const int row = wave * RPW + (lane & 15), q = lane >> 4;
+    const T* Dr = Ds + row * TRSV_PAD;
     double xv = 0.0;
-#pragma unroll 8
-    for (int c = 0; c < NB; c++) xv = fma((double)Dr[c], vs[c], xv);
-    X[(size_t)rhs * N + r0 + tid] = xv;
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+      for (int cc = 0; cc < 16; cc++) {
+        const int col = h * 64 + q * 16 + cc;
+        xv = fma((double)Dr[col], vs[col], xv);
+      }
+    xv += __shfl_xor(xv, 16, 64);
+    xv += __shfl_xor(xv, 32, 64);
+    if (q == 0 && row < nr) {
+      if (s_fail) xv = __builtin_nan("");
+      __hip_atomic_store(X + r0 + row, xv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
-  __threadfence();
-  __syncthreads();
-  if (tid == 0) __hip_atomic_store(flags + b, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // residual r = b - A x with the FP64 matrix, accumulated in double-double: each product
@@ -825,8 +887,9 @@ struct LuWorkspace {
   T* Y; T* Z;                                     // scratch inverses of the current panel
   T* U12; T* Pbak;                                // trailing-update staging, panel backup
   T* Linv; T* Uinv;                               // inverses of every diagonal block (nblk x NB x NB)
-  int* perm; int* flags; int* err; int* pivoted;  // permutation, solve flags, error, any-interchange flag
-  double* rhs;                                    // permuted right-hand sides (nrhs x N, nrhs <= 64)
+  int* perm; int* flags; int* err; int* pivoted;  // permutation, solve tickets, error, any-interchange flag
+  double* rhs;                                    // copy of the right-hand sides (nrhs x N, nrhs <= 64)
+  double* ytmp;                                   // L^{-1} P b of the solve in progress (N)
   unsigned long long* lmax;
   size_t bytes;
   LuWorkspace(void* base, int N) {
@@ -840,12 +903,14 @@ struct LuWorkspace {
     size_t a_li = take((size_t)nblk * NB * NB * sizeof(T)), a_ui = take((size_t)nblk * NB * NB * sizeof(T));
     size_t a_pm = take((size_t)N * sizeof(int)), a_fl = take((2 * (size_t)nblk + 2) * sizeof(int));
     size_t a_er = take(sizeof(int)), a_pv = take(sizeof(int)), a_rhs = take(64 * (size_t)N * sizeof(double)), a_lm = take(8);
+    size_t a_yt = take((size_t)N * sizeof(double));
     bytes = o;
     if (!base) return;
     cand_val = (double*)(p + a_cv); cand_idx = (int*)(p + a_ci); cand_row = (T*)(p + a_cr);
     Y = (T*)(p + a_y); Z = (T*)(p + a_z); U12 = (T*)(p + a_u); Pbak = (T*)(p + a_pb);
     Linv = (T*)(p + a_li); Uinv = (T*)(p + a_ui); perm = (int*)(p + a_pm); flags = (int*)(p + a_fl);
     err = (int*)(p + a_er); pivoted = (int*)(p + a_pv); rhs = (double*)(p + a_rhs); lmax = (unsigned long long*)(p + a_lm);
+    ytmp = (double*)(p + a_yt);
   }
 };
 
@@ -1002,12 +1067,17 @@ hipError_t lu_solve(const T* LU, size_t lda, int N, const int* ipiv, double* X, 
   if (e != hipSuccess) return e;
   e = hipFuncSetAttribute((const void*)k_trsv_blocked<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
+  const unsigned pgrid = cdiv(N, 256);
   for (int r = 0; r < nrhs; r++) {
-    hipMemsetAsync(W.flags, 0, sizeof(int) * (2 * (size_t)nblk + 2), st);
-    hipLaunchKernelGGL((k_trsv_blocked<T, true>), dim3(nblk), dim3(256), lds, st, LU, lda, N, W.perm, W.pivoted, W.rhs, X, r, W.Linv,
-                       W.flags, W.flags + 2 * nblk, W.err);
-    hipLaunchKernelGGL((k_trsv_blocked<T, false>), dim3(nblk), dim3(256), lds, st, LU, lda, N, (const int*)nullptr,
-                       (const int*)nullptr, X, X, r, W.Uinv, W.flags + nblk, W.flags + 2 * nblk + 1, W.err);
+    // y = L^{-1} P b into the workspace, then x = U^{-1} y into X (each output prefilled
+    // with the pending marker its readers poll)
+    double* xr = X + (size_t)r * N;
+    hipLaunchKernelGGL(k_trsv_prep, dim3(pgrid), dim3(256), 0, st, W.ytmp, N, W.flags);
+    hipLaunchKernelGGL((k_trsv_blocked<T, true>), dim3(nblk), dim3(TRSV_THREADS), lds, st, LU, lda, N, W.perm, W.pivoted,
+                       W.rhs + (size_t)r * N, W.ytmp, W.Linv, W.flags, W.err);
+    hipLaunchKernelGGL(k_trsv_prep, dim3(pgrid), dim3(256), 0, st, xr, N, W.flags + 1);
+    hipLaunchKernelGGL((k_trsv_blocked<T, false>), dim3(nblk), dim3(TRSV_THREADS), lds, st, LU, lda, N, (const int*)nullptr,
+                       (const int*)nullptr, W.ytmp, xr, W.Uinv, W.flags + 1, W.err);
   }
   return hipGetLastError();
 }
