@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_cooperative_groups.h>
 #include <stdint.h>
+#include <mutex>
 #include <utility>
 #include <vector>
 #include "lu_kernels.h"
@@ -28,6 +29,11 @@ namespace cg = cooperative_groups;
 namespace tb {
 
 template <typename T> __device__ __forceinline__ T tabs(T x) { return x < T(0) ? -x : x; }
+
+// Gate of the speculative factorisation: the kernels of panel p run only while no panel
+// q <= p has failed the no-pivot test (*gate holds the first failed panel, or nblk).
+// A null gate always runs.
+__device__ __forceinline__ bool gated_off(const int* gate, int p) { return gate != nullptr && *gate <= p; }
 
 // ---------------------------------------------------------------------------------
 // Panel factorisation
@@ -203,8 +209,9 @@ __global__ __launch_bounds__(256) void k_laswp(T* __restrict__ A, size_t lda, in
 template <typename T>
 __global__ __launch_bounds__(1024) void k_diag_block(T* __restrict__ A, size_t lda, int k0, int nb, int factor,
                                                     int* __restrict__ ipiv, T* __restrict__ Y, T* __restrict__ Z,
-                                                    unsigned long long* __restrict__ lmax) {
+                                                    unsigned long long* __restrict__ lmax, const int* gate, int gp) {
   constexpr int NB = LU_NB, RG = 1024 / NB, KR = NB / RG;
+  if (gated_off(gate, gp)) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* Bs = reinterpret_cast<T*>(smem_raw);   // the factored block, NB x NB, for the inverse sweep
   __shared__ T rowb[2][NB], colb[2][NB], rowc[2][NB];
@@ -341,8 +348,9 @@ __global__ __launch_bounds__(1024) void k_diag_check(const T* __restrict__ A, si
 // copy a rows x cols block back into the matrix and track max |value| (bits, nonnegative)
 template <typename T>
 __global__ __launch_bounds__(256) void k_copy_back_max(const T* __restrict__ S, size_t lds, T* __restrict__ D, size_t ldd, int rows,
-                                                       int cols, unsigned long long* __restrict__ vmax) {
+                                                       int cols, unsigned long long* __restrict__ vmax, const int* gate, int gp) {
   __shared__ double wm[4];
+  if (gated_off(gate, gp)) return;
   const size_t total = (size_t)rows * cols;
   double m = 0.0;
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
@@ -408,8 +416,10 @@ constexpr int GBM = 128, GBN = 128;
 // accesses with bounds checks.
 template <typename T, int BK>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, const T* __restrict__ A, size_t lda,
-                                              const T* __restrict__ B, size_t ldb, T beta, T* __restrict__ C, size_t ldc) {
+                                              const T* __restrict__ B, size_t ldb, T beta, T* __restrict__ C, size_t ldc,
+                                              const int* gate, int gp) {
   typedef typename MfmaAcc<T>::type acc_t;
+  if (gated_off(gate, gp)) return;
   constexpr int VEC = 16 / sizeof(T);                     // elements per 16-byte load
   constexpr int A_LD = (GBM * BK) / (256 * VEC);          // vector loads per thread for A
   constexpr int B_LD = (BK * GBN) / (256 * VEC);
@@ -582,7 +592,9 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, cons
 
 // copy a rows x cols block (row-major, separate leading dims)
 template <typename T>
-__global__ __launch_bounds__(256) void k_copy2d(const T* __restrict__ S, size_t lds, T* __restrict__ D, size_t ldd, int rows, int cols) {
+__global__ __launch_bounds__(256) void k_copy2d(const T* __restrict__ S, size_t lds, T* __restrict__ D, size_t ldd, int rows, int cols,
+                                                const int* gate, int gp) {
+  if (gated_off(gate, gp)) return;
   size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (size_t)rows * cols) return;
   int r = (int)(e / cols), c = (int)(e % cols);
@@ -627,6 +639,17 @@ __global__ __launch_bounds__(256) void k_perm(const int* __restrict__ ipiv, int 
 constexpr long long SPIN_LIMIT = 1ll << 22;   // polls of a flag before a wait gives up (err set)
 
 __global__ void k_set_int(int* p, int v) { *p = v; }
+
+__global__ __launch_bounds__(256) void k_zero_u64(unsigned long long* __restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+// panel p of the speculative pass: record it as failed when some |l| > 1 (or NaN)
+__global__ void k_panel_check(const unsigned long long* __restrict__ lmax, int* __restrict__ first_fail, int p) {
+  if (*first_fail <= p) return;
+  if (*lmax > 0x3FF0000000000000ull) atomicMin(first_fail, p);
+}
 
 // Blocked triangular solve with the stored diagonal-block inverses, one workgroup of 512
 // threads per block row of 128.  Workgroups take block rows in dependency order from a
@@ -867,16 +890,17 @@ static inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) 
 
 template <typename T>
 hipError_t gemm(int M, int N, int K, T alpha, const T* A, size_t lda, const T* B, size_t ldb, T beta, T* C, size_t ldc,
-                hipStream_t st) {
+                hipStream_t st, const int* gate, int gp) {
   if (M <= 0 || N <= 0) return hipSuccess;
   dim3 grid(cdiv(N, GBN), cdiv(M, GBM));
   constexpr int BK = sizeof(T) == 8 ? 16 : 32;
-  hipLaunchKernelGGL((k_gemm<T, BK>), grid, dim3(256), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+  hipLaunchKernelGGL((k_gemm<T, BK>), grid, dim3(256), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, gp);
   return hipGetLastError();
 }
 template hipError_t gemm<double>(int, int, int, double, const double*, size_t, const double*, size_t, double, double*, size_t,
-                                 hipStream_t);
-template hipError_t gemm<float>(int, int, int, float, const float*, size_t, const float*, size_t, float, float*, size_t, hipStream_t);
+                                 hipStream_t, const int*, int);
+template hipError_t gemm<float>(int, int, int, float, const float*, size_t, const float*, size_t, float, float*, size_t, hipStream_t,
+                                const int*, int);
 
 // Workspace of one factorisation (kept with the factors: the solves read the block inverses,
 // the permutation and the pivot flag from it).
@@ -890,6 +914,9 @@ struct LuWorkspace {
   int* perm; int* flags; int* err; int* pivoted;  // permutation, solve tickets, error, any-interchange flag
   double* rhs;                                    // copy of the right-hand sides (nrhs x N, nrhs <= 64)
   double* ytmp;                                   // L^{-1} P b of the solve in progress (N)
+  T* Ltmp;                                        // L21 = A21 inv(U11) before it is checked (N x NB)
+  unsigned long long* lmaxv;                      // max |l| per panel of the speculative pass (nblk)
+  int* first_fail;                                // first panel of that pass that failed the no-pivot test
   unsigned long long* lmax;
   size_t bytes;
   LuWorkspace(void* base, int N) {
@@ -904,6 +931,7 @@ struct LuWorkspace {
     size_t a_pm = take((size_t)N * sizeof(int)), a_fl = take((2 * (size_t)nblk + 2) * sizeof(int));
     size_t a_er = take(sizeof(int)), a_pv = take(sizeof(int)), a_rhs = take(64 * (size_t)N * sizeof(double)), a_lm = take(8);
     size_t a_yt = take((size_t)N * sizeof(double));
+    size_t a_lt = take((size_t)N * NB * sizeof(T)), a_lv = take((size_t)nblk * 8), a_ff = take(sizeof(int));
     bytes = o;
     if (!base) return;
     cand_val = (double*)(p + a_cv); cand_idx = (int*)(p + a_ci); cand_row = (T*)(p + a_cr);
@@ -911,6 +939,7 @@ struct LuWorkspace {
     Linv = (T*)(p + a_li); Uinv = (T*)(p + a_ui); perm = (int*)(p + a_pm); flags = (int*)(p + a_fl);
     err = (int*)(p + a_er); pivoted = (int*)(p + a_pv); rhs = (double*)(p + a_rhs); lmax = (unsigned long long*)(p + a_lm);
     ytmp = (double*)(p + a_yt);
+    Ltmp = (T*)(p + a_lt); lmaxv = (unsigned long long*)(p + a_lv); first_fail = (int*)(p + a_ff);
   }
 };
 
@@ -921,6 +950,47 @@ size_t lu_workspace_bytes(int N) {
 template size_t lu_workspace_bytes<double>(int);
 template size_t lu_workspace_bytes<float>(int);
 
+// Second stream and events of the factorisation's lookahead, one set per device, kept for
+// the life of the process (default priority).  Calls on one device are serialised.
+struct SideStream {
+  hipStream_t s = nullptr;
+  hipEvent_t ev_start = nullptr, ev_p = nullptr, ev_u = nullptr, ev_end = nullptr;
+  std::mutex mu;
+};
+static SideStream g_side[64];
+static std::mutex g_side_mu;
+
+static hipError_t side_stream(int dev, SideStream** out) {
+  if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_side_mu);
+  SideStream& x = g_side[dev];
+  if (!x.s) {
+    hipError_t e = hipStreamCreateWithFlags(&x.s, hipStreamNonBlocking);
+    if (e != hipSuccess) { x.s = nullptr; return e; }
+    for (hipEvent_t* ev : {&x.ev_start, &x.ev_p, &x.ev_u, &x.ev_end}) {
+      e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
+      if (e != hipSuccess) return e;
+    }
+  }
+  *out = &x;
+  return hipSuccess;
+}
+
+// Factorisation driver.  Most BEM panels need no row interchange, so the factorisation
+// first runs speculatively without pivoting and without host round trips: per panel p
+//   P(p)  (side stream): keep a copy of the panel column, factor the diagonal block and
+//         invert its triangles (k_diag_block), L21 = A21 inv(U11), record max |l|, and
+//         mark p as failed when some |l| > 1;
+//   U(p)  (caller's stream): U12 = inv(L11) A12;
+//   Tc(p) (side stream): the trailing update of the next panel's column only;
+//   Tr(p) (caller's stream): the trailing update of the remaining columns.
+// P(p+1) depends only on Tc(p), so the diagonal-block work of the next panel runs beside
+// the large Tr(p) product (one-panel lookahead).  Every kernel of panel p is gated on "no
+// panel <= p failed" (first_fail), so after a failure at panel f the matrix holds panels
+// < f complete with their trailing updates, and panel f's column is restored from its copy
+// and factored with partial pivoting (cooperative k_panel); the speculative pass then
+// resumes at f + 1.  A panel that passes the test (every |l| <= 1) is exactly the partially
+// pivoted result: the diagonal entries are the column maxima, so ipiv is the identity.
 template <typename T>
 hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev, hipStream_t st, int panel_groups,
                      KernelTiming* tm) {
@@ -933,17 +1003,18 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
     }
   } events;
   auto& ev = events.v;
-  auto timed_gemm = [&](int M, int Nn, int K, T alpha, const T* Ap, size_t la, const T* Bp, size_t lb, T beta, T* Cp,
-                        size_t lc) -> hipError_t {
+  auto timed_gemm = [&](hipStream_t s, const int* gate, int gp, int M, int Nn, int K, T alpha, const T* Ap, size_t la, const T* Bp,
+                        size_t lb, T beta, T* Cp, size_t lc) -> hipError_t {
+    if (M <= 0 || Nn <= 0) return hipSuccess;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (tm) {
       if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return hipErrorOutOfMemory;
-      hipEventRecord(e0, st);
-    }
-    hipError_t r = gemm<T>(M, Nn, K, alpha, Ap, la, Bp, lb, beta, Cp, lc, st);
-    if (tm) {
-      hipEventRecord(e1, st);
       ev.emplace_back(e0, e1);
+      if (hipEventRecord(e0, s) != hipSuccess) return hipErrorLaunchFailure;
+    }
+    hipError_t r = gemm<T>(M, Nn, K, alpha, Ap, la, Bp, lb, beta, Cp, lc, s, gate, gp);
+    if (tm) {
+      if (hipEventRecord(e1, s) != hipSuccess) return hipErrorLaunchFailure;
       tm->launches++;
       tm->flops += 2.0 * M * (double)Nn * K;
     }
@@ -956,9 +1027,10 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
     }
   };
   int dev = 0;
-  hipGetDevice(&dev);
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
   hipDeviceProp_t prop;
-  hipError_t e = hipGetDeviceProperties(&prop, dev);
+  e = hipGetDeviceProperties(&prop, dev);
   if (e != hipSuccess) return e;
   int G = panel_groups > 0 ? panel_groups : prop.multiProcessorCount;
   if (G > 1024) G = 1024;
@@ -972,73 +1044,119 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
   e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kpanel, 256, Rmax * NB * sizeof(T));
   if (e != hipSuccess) return e;
   if (occ * prop.multiProcessorCount < G) return hipErrorCooperativeLaunchTooLarge;
-
   e = hipFuncSetAttribute((const void*)k_diag_block<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(NB * NB * sizeof(T)));
   if (e != hipSuccess) return e;
+  SideStream* ss = nullptr;
+  e = side_stream(dev, &ss);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lk(ss->mu);
+  const hipStream_t sa = ss->s;
+
   LuWorkspace<T> W(ws, N);
-  double* wsd = W.cand_val;
-  int* wsi = W.cand_idx;
-  T* wsr = W.cand_row;
+  const int nblk = (N + NB - 1) / NB;
   T* U12 = W.U12;
   T* Pbak = W.Pbak;
-  unsigned long long* lmax = W.lmax;
   bool any_pivot = false;
-  hipMemsetAsync(W.err, 0, sizeof(int), st);
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, W.err, 0);
+  auto Yp = [&](int q) { return W.Linv + (size_t)q * NB * NB; };
+  auto Zp = [&](int q) { return W.Uinv + (size_t)q * NB * NB; };
 
-  const unsigned long long ONE_BITS = 0x3FF0000000000000ull;   // bit pattern of 1.0
-  for (int k0 = 0; k0 < N; k0 += NB) {
-    const int nb = min(NB, N - k0);
-    const int m2 = N - k0 - nb;   // rows below the diagonal block
-    bool pivoted = true;
-    // ---- fast path: unpivoted panel, accepted when every |l| <= 1 ----
-    hipLaunchKernelGGL(k_copy2d<T>, dim3(cdiv((size_t)(N - k0) * nb, 256)), dim3(256), 0, st, A + (size_t)k0 * lda + k0, lda, Pbak,
-                       (size_t)nb, N - k0, nb);
-    hipMemsetAsync(lmax, 0, sizeof(unsigned long long), st);
-    T* Z = W.Uinv + (size_t)(k0 / NB) * NB * NB;
-    T* Y = W.Linv + (size_t)(k0 / NB) * NB * NB;
-    hipLaunchKernelGGL(k_diag_block<T>, dim3(1), dim3(1024), NB * NB * sizeof(T), st, A, lda, k0, nb, 1, ipiv, Y, Z, lmax);
-    if (m2 > 0) {
-      T* A21 = A + (size_t)(k0 + nb) * lda + k0;
-      e = timed_gemm(m2, nb, nb, T(1), A21, lda, Z, NB, T(0), U12, (size_t)nb);
-      if (e != hipSuccess) return e;
-      const size_t nblk_copy = std::min<size_t>(cdiv((size_t)m2 * nb, 256), 1024);
-      hipLaunchKernelGGL(k_copy_back_max<T>, dim3(nblk_copy), dim3(256), 0, st, U12, (size_t)nb, A21, lda, m2, nb, lmax);
+  // ---- speculative pass from panel s0; *kf = first failed panel, or nblk ----
+  auto speculate = [&](int s0, int* kf) -> hipError_t {
+    const int* g = W.first_fail;
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, W.first_fail, nblk);
+    hipLaunchKernelGGL(k_zero_u64, dim3(cdiv(nblk, 256)), dim3(256), 0, st, W.lmaxv, nblk);
+    hipError_t r = hipEventRecord(ss->ev_start, st);
+    if (r != hipSuccess) return r;
+    r = hipStreamWaitEvent(sa, ss->ev_start, 0);
+    if (r != hipSuccess) return r;
+    auto panel = [&](int q) -> hipError_t {   // P(q) on the side stream
+      const int k0 = q * NB, nb = min(NB, N - k0), m2 = N - k0 - nb;
+      hipLaunchKernelGGL(k_copy2d<T>, dim3(cdiv((size_t)(N - k0) * nb, 256)), dim3(256), 0, sa, A + (size_t)k0 * lda + k0, lda, Pbak,
+                         (size_t)nb, N - k0, nb, g, q);
+      hipLaunchKernelGGL(k_diag_block<T>, dim3(1), dim3(1024), NB * NB * sizeof(T), sa, A, lda, k0, nb, 1, ipiv, Yp(q), Zp(q),
+                         W.lmaxv + q, g, q);
+      if (m2 > 0) {
+        T* A21 = A + (size_t)(k0 + nb) * lda + k0;
+        hipError_t r2 = timed_gemm(sa, g, q, m2, nb, nb, T(1), A21, lda, Zp(q), NB, T(0), W.Ltmp, (size_t)nb);
+        if (r2 != hipSuccess) return r2;
+        const size_t nblk_copy = std::min<size_t>(cdiv((size_t)m2 * nb, 256), 1024);
+        hipLaunchKernelGGL(k_copy_back_max<T>, dim3(nblk_copy), dim3(256), 0, sa, W.Ltmp, (size_t)nb, A21, lda, m2, nb, W.lmaxv + q,
+                           g, q);
+      }
+      hipLaunchKernelGGL(k_panel_check, dim3(1), dim3(1), 0, sa, W.lmaxv + q, W.first_fail, q);
+      return hipEventRecord(ss->ev_p, sa);
+    };
+    r = panel(s0);
+    if (r != hipSuccess) return r;
+    for (int q = s0; q < nblk; q++) {
+      const int k0 = q * NB, nb = min(NB, N - k0), n2 = N - k0 - nb;
+      if (n2 <= 0) break;
+      const int k1 = k0 + nb, nb1 = min(NB, n2);
+      T* A12 = A + (size_t)k0 * lda + k1;
+      const T* L21 = A + (size_t)k1 * lda + k0;
+      // U(q) on the caller's stream, after P(q)
+      if ((r = hipStreamWaitEvent(st, ss->ev_p, 0)) != hipSuccess) return r;
+      if ((r = timed_gemm(st, g, q, nb, n2, nb, T(1), Yp(q), NB, A12, lda, T(0), U12, (size_t)n2)) != hipSuccess) return r;
+      hipLaunchKernelGGL(k_copy2d<T>, dim3(cdiv((size_t)nb * n2, 256)), dim3(256), 0, st, U12, (size_t)n2, A12, lda, nb, n2, g, q);
+      if ((r = hipEventRecord(ss->ev_u, st)) != hipSuccess) return r;
+      // Tc(q) and P(q+1) on the side stream
+      if ((r = hipStreamWaitEvent(sa, ss->ev_u, 0)) != hipSuccess) return r;
+      if ((r = timed_gemm(sa, g, q, n2, nb1, nb, T(-1), L21, lda, U12, (size_t)n2, T(1), A + (size_t)k1 * lda + k1, lda)) != hipSuccess)
+        return r;
+      if ((r = panel(q + 1)) != hipSuccess) return r;
+      // Tr(q) on the caller's stream, beside P(q+1)
+      if ((r = timed_gemm(st, g, q, n2, n2 - nb1, nb, T(-1), L21, lda, U12 + nb1, (size_t)n2, T(1), A + (size_t)k1 * lda + k1 + nb1,
+                          lda)) != hipSuccess)
+        return r;
     }
-    unsigned long long lm = 0;
-    e = hipMemcpyAsync(&lm, lmax, sizeof(lm), hipMemcpyDeviceToHost, st);
-    if (e != hipSuccess) return e;
-    e = hipStreamSynchronize(st);
-    if (e != hipSuccess) return e;
-    if (lm <= ONE_BITS) {
-      pivoted = false;
-    } else {
-      // ---- slow path: restore the panel and factor it with partial pivoting ----
-      hipLaunchKernelGGL(k_copy2d<T>, dim3(cdiv((size_t)(N - k0) * nb, 256)), dim3(256), 0, st, Pbak, (size_t)nb,
-                         A + (size_t)k0 * lda + k0, lda, N - k0, nb);
-      T* Ap = A; size_t ldap = lda; int Np = N, k0p = k0;
-      int* ipivp = ipiv; double* wsdp = wsd; int* wsip = wsi; T* wsrp = wsr; int* infop = info_dev;
-      void* args[] = {&Ap, &ldap, &Np, &k0p, &ipivp, &wsdp, &wsip, &wsrp, &infop};
-      const int m = N - k0;
-      size_t R = ((size_t)m + G - 1) / G;
-      e = hipLaunchCooperativeKernel((const void*)kpanel, dim3(G), dim3(256), args, (unsigned)(R * NB * sizeof(T)), st);
-      if (e != hipSuccess) return e;
-      any_pivot = true;
-      hipLaunchKernelGGL(k_diag_block<T>, dim3(1), dim3(1024), NB * NB * sizeof(T), st, A, lda, k0, nb, 0, ipiv, Y, Z, lmax);
-    }
-    if (pivoted && N - nb > 0) hipLaunchKernelGGL(k_laswp<T>, dim3(cdiv(N - nb, 256)), dim3(256), 0, st, A, lda, N, k0, nb, ipiv);
+    if ((r = hipEventRecord(ss->ev_end, sa)) != hipSuccess) return r;
+    if ((r = hipStreamWaitEvent(st, ss->ev_end, 0)) != hipSuccess) return r;
+    int h = nblk;
+    if ((r = hipMemcpyAsync(&h, W.first_fail, sizeof(int), hipMemcpyDeviceToHost, st)) != hipSuccess) return r;
+    if ((r = hipStreamSynchronize(st)) != hipSuccess) return r;
+    *kf = h;
+    return hipGetLastError();
+  };
+
+  // ---- one panel with partial pivoting (its column already restored) ----
+  auto pivoted_panel = [&](int q) -> hipError_t {
+    const int k0 = q * NB, nb = min(NB, N - k0);
+    T* Ap = A; size_t ldap = lda; int Np = N, k0p = k0;
+    int* ipivp = ipiv; double* wsdp = W.cand_val; int* wsip = W.cand_idx; T* wsrp = W.cand_row; int* infop = info_dev;
+    void* args[] = {&Ap, &ldap, &Np, &k0p, &ipivp, &wsdp, &wsip, &wsrp, &infop};
+    const int m = N - k0;
+    size_t R = ((size_t)m + G - 1) / G;
+    hipError_t r = hipLaunchCooperativeKernel((const void*)kpanel, dim3(G), dim3(256), args, (unsigned)(R * NB * sizeof(T)), st);
+    if (r != hipSuccess) return r;
+    any_pivot = true;
+    hipLaunchKernelGGL(k_diag_block<T>, dim3(1), dim3(1024), NB * NB * sizeof(T), st, A, lda, k0, nb, 0, ipiv, Yp(q), Zp(q), W.lmax,
+                       (const int*)nullptr, 0);
+    if (N - nb > 0) hipLaunchKernelGGL(k_laswp<T>, dim3(cdiv(N - nb, 256)), dim3(256), 0, st, A, lda, N, k0, nb, ipiv);
     const int n2 = N - k0 - nb;
     if (n2 > 0) {
       T* A12 = A + (size_t)k0 * lda + k0 + nb;
-      // U12 = inv(L11) * A12
-      e = timed_gemm(nb, n2, nb, T(1), Y, NB, A12, lda, T(0), U12, (size_t)n2);
-      if (e != hipSuccess) return e;
-      hipLaunchKernelGGL(k_copy2d<T>, dim3(cdiv((size_t)nb * n2, 256)), dim3(256), 0, st, U12, (size_t)n2, A12, lda, nb, n2);
-      // A22 -= L21 * U12
+      if ((r = timed_gemm(st, nullptr, 0, nb, n2, nb, T(1), Yp(q), NB, A12, lda, T(0), U12, (size_t)n2)) != hipSuccess) return r;
+      hipLaunchKernelGGL(k_copy2d<T>, dim3(cdiv((size_t)nb * n2, 256)), dim3(256), 0, st, U12, (size_t)n2, A12, lda, nb, n2,
+                         (const int*)nullptr, 0);
       const T* L21 = A + (size_t)(k0 + nb) * lda + k0;
       T* A22 = A + (size_t)(k0 + nb) * lda + k0 + nb;
-      e = timed_gemm(n2, n2, nb, T(-1), L21, lda, U12, (size_t)n2, T(1), A22, lda);
-      if (e != hipSuccess) return e;
+      if ((r = timed_gemm(st, nullptr, 0, n2, n2, nb, T(-1), L21, lda, U12, (size_t)n2, T(1), A22, lda)) != hipSuccess) return r;
     }
+    return hipGetLastError();
+  };
+
+  for (int q = 0; q < nblk;) {
+    int kf = nblk;
+    e = speculate(q, &kf);
+    if (e != hipSuccess) return e;
+    if (kf < q || kf >= nblk) break;
+    const int k0 = kf * NB, nb = min(NB, N - k0);
+    hipLaunchKernelGGL(k_copy2d<T>, dim3(cdiv((size_t)(N - k0) * nb, 256)), dim3(256), 0, st, Pbak, (size_t)nb,
+                       A + (size_t)k0 * lda + k0, lda, N - k0, nb, (const int*)nullptr, 0);
+    e = pivoted_panel(kf);
+    if (e != hipSuccess) return e;
+    q = kf + 1;
   }
   hipLaunchKernelGGL(k_diag_check<T>, dim3(1), dim3(1024), 0, st, A, lda, N, info_dev);
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, W.pivoted, any_pivot ? 1 : 0);

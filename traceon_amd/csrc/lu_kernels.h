@@ -7,9 +7,10 @@ namespace tb {
 
 constexpr int LU_NB = 128;   // panel width of the blocked factorisation
 
+// gate (device int, may be null): the launch does nothing when *gate <= gp (see lu.hip)
 template <typename T>
 hipError_t gemm(int M, int N, int K, T alpha, const T* A, size_t lda, const T* B, size_t ldb, T beta, T* C, size_t ldc,
-                hipStream_t st);
+                hipStream_t st, const int* gate = nullptr, int gp = 0);
 template <typename T> size_t lu_workspace_bytes(int N);
 // Launch count, summed HIP-event duration and algorithmic FLOP (2 M N K per product) of
 // the MFMA GEMM launches inside one factorisation.
