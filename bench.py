@@ -31,6 +31,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "hbm": 8.0e12}
+# HBM bytes per launch of the LU's FP32 GEMM in this workload (N = 20,000): rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE averaged over the 623 launches of one factorisation, each scaled by
+# what the counter reads for a 256 MiB device-to-device copy (--copy-calibration 256): fetch
+# x2.0, write x1.0.  Counters cannot be read inside this process, so the figure is the one
+# measured for this kernel and launch schedule; other N report null.
+GEMM_F32_TRAFFIC_N20000 = 388.0e6
 # algorithmic FLOP per unit (reference formulas, +-*/sqrt = 1, FMA = 2, log = 20)
 C_POT, C_FIELD, C_FLUX = 71, 215, 219
 
@@ -351,6 +357,8 @@ def main():
     ap.add_argument("--no-field", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--no-c4", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device-to-device three times (a known-size kernel for HBM counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md for how it would shard)",
@@ -358,6 +366,14 @@ def main():
         sys.exit(2)
     import torch
     assert torch.cuda.is_available(), "bench.py needs the GPU"
+    if args.copy_calibration > 0:
+        src = torch.ones(args.copy_calibration << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"bench: copy calibration, 3 x {args.copy_calibration} MiB device-to-device")
+        del src, dst
     lines, types, values = mirror_problem(args.N)
     N = len(lines)
     log(f"bench: mirror N={N}, warmup {args.warmup}, steps {args.steps}")
@@ -394,7 +410,9 @@ def main():
     gemm_flop_avg = gt.flops / max(gt.launches, 1)
     roof = {"kernel": "k_gemm<float,32> (LU products, v_mfma_f32_16x16x4f32)", "bound": "mfma",
             "achieved": gemm_flop_avg / (gemm_ms_avg * 1e-3) / 1e12, "peak": PEAK["fp32"] / 1e12, "unit": "TFLOP/s",
-            "traffic": None, "launches": gt.launches, "flop_per_launch": gemm_flop_avg, "launch_ms": gemm_ms_avg,
+            "traffic": GEMM_F32_TRAFFIC_N20000 if N == 20000 else None, "traffic_unit": "bytes per launch",
+            "algorithmic_bytes_per_launch": gt.bytes / max(gt.launches, 1),
+            "launches": gt.launches, "flop_per_launch": gemm_flop_avg, "launch_ms": gemm_ms_avg,
             "share_of_step": gt.ms / max(sum(t["fill_ms"] + t["solve_ms"] for t in tfs), 1e-9)}
     roof["frac"] = roof["achieved"] / roof["peak"]
     result["roofline"] = roof

@@ -147,9 +147,11 @@ int tb_lu_factor(double *A, size_t lda, int N, int *ipiv, void *workspace, int *
 int tb_lu_factor_f32(float *A, size_t lda, int N, int *ipiv, void *workspace, int *info, void *stream);
 /* Instrumented factorisation (no counterpart in the reference: measurement only).  As
    tb_lu_factor / tb_lu_factor_f32 (fp32 selects the type of A), and accumulates into
-   *gemm_timing the number, HIP-event duration (ms, on `stream`) and algorithmic FLOP
-   (2 M N K each) of the MFMA GEMM launches of this factorisation. */
-typedef struct { long long launches; double ms; double flops; } tb_kernel_timing_t;
+   *gemm_timing the number, HIP-event duration (ms, on the stream each is launched on:
+   `stream` or the factorisation's second stream, so launches that overlap count in full),
+   algorithmic FLOP (2 M N K each) and algorithmic HBM bytes (A and B read once, C read
+   and written) of the MFMA GEMM launches of this factorisation. */
+typedef struct { long long launches; double ms; double flops; double bytes; } tb_kernel_timing_t;
 int tb_lu_factor_timed(void *A, size_t lda, int N, int *ipiv, void *workspace, int *info, int fp32,
                        tb_kernel_timing_t *gemm_timing, void *stream);
 /* X (nrhs x N, nrhs <= 64) <- A^{-1} X with the factors and the workspace of the same

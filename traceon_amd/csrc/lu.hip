@@ -1017,6 +1017,7 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
       if (hipEventRecord(e1, s) != hipSuccess) return hipErrorLaunchFailure;
       tm->launches++;
       tm->flops += 2.0 * M * (double)Nn * K;
+      tm->bytes += sizeof(T) * ((double)M * K + (double)K * Nn + (beta != T(0) ? 2.0 : 1.0) * M * (double)Nn);
     }
     return r;
   };

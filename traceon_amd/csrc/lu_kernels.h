@@ -12,12 +12,14 @@ template <typename T>
 hipError_t gemm(int M, int N, int K, T alpha, const T* A, size_t lda, const T* B, size_t ldb, T beta, T* C, size_t ldc,
                 hipStream_t st, const int* gate = nullptr, int gp = 0);
 template <typename T> size_t lu_workspace_bytes(int N);
-// Launch count, summed HIP-event duration and algorithmic FLOP (2 M N K per product) of
-// the MFMA GEMM launches inside one factorisation.
+// Launch count, summed HIP-event duration, algorithmic FLOP (2 M N K per product) and
+// algorithmic HBM bytes (A and B read once, C read and written) of the MFMA GEMM launches
+// inside one factorisation.
 struct KernelTiming {
   long long launches;
   double ms;
   double flops;
+  double bytes;
 };
 // In-place LU of the row-major N x N matrix A (leading dimension lda) with partial pivoting.
 // ipiv (device, N ints) receives 0-based pivot rows; info_dev (device int, zeroed by the caller)

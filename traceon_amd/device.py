@@ -121,8 +121,8 @@ class LU:
 
 
 class KernelTiming(C.Structure):
-    """tb_kernel_timing_t: launches, summed HIP-event ms and algorithmic FLOP of a kernel."""
-    _fields_ = [("launches", C.c_longlong), ("ms", C.c_double), ("flops", C.c_double)]
+    """tb_kernel_timing_t: launches, summed HIP-event ms, algorithmic FLOP and HBM bytes of a kernel."""
+    _fields_ = [("launches", C.c_longlong), ("ms", C.c_double), ("flops", C.c_double), ("bytes", C.c_double)]
 
 
 def lu_factor(A: torch.Tensor, overwrite: bool = True, gemm_timing: "KernelTiming | None" = None) -> LU:
