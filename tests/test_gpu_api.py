@@ -340,3 +340,42 @@ def test_magnet_with_magnetizable_material_vs_fem(gpu):
         print("magnet+iron", r, z, MU0 * Hr, MU0 * Hz, c, flush=True)
         assert np.isclose(c[0], MU0 * Hr, rtol=1e-4, atol=0.0002)
         assert np.isclose(c[1], MU0 * Hz, rtol=1e-4, atol=0.0002)
+
+
+def test_mixed_precision_falls_back_to_fp64(gpu):
+    """cond ~ 1e9 > 1/eps32: refinement on the FP32 factors cannot converge, so the solve
+    falls back to an FP64 LU and still reaches the FP64 answer; a well-conditioned system
+    stays on the FP32 factors."""
+    import torch
+    from traceon_amd import device
+    rng = np.random.default_rng(11)
+    n = 300
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    for cond, expect in ((1e9, "mixed->fp64"), (1e2, "mixed")):
+        A = (Q * np.logspace(0, -np.log10(cond), n)) @ Q.T
+        b = rng.standard_normal((1, n))
+        M64 = torch.tensor(A, device="cuda")
+        X, st, _ = device.solve_system(M64, b, M32=M64.float(), mode="mixed")
+        x = X.cpu().numpy()[0]
+        ref = np.linalg.solve(A, b[0])
+        err = rel(x, ref)
+        print("cond", cond, st, "rel vs LAPACK", err, flush=True)
+        assert st.mode == expect
+        assert err < cond * 1e-15
+
+
+def test_step_cap_retires_with_status(small):
+    """A particle that reaches the attempted-step cap is retired with status 'step cap' and
+    the cap's step count, in the persistent and the stage-synchronous tracer."""
+    from traceon_amd import tracing
+    field, _, _ = small
+    prog = tracing.DeviceFieldProgram(field)
+    v = tracing.velocity_vec(100.0, [0, 0, 1])
+    state = np.array([[0.05e-3, 0.0, -1e-3, *v], [0.02e-3, 0.01e-3, -1e-3, *v]])
+    bounds = np.array([[-1.0, 1.0], [-1.0, 1.0], [-1.0, 1.0]])
+    for split in (False, True):
+        r = tracing.run_trace(prog, state, QM, bounds, max_attempts=7, split=split)
+        st = r.status.cpu().numpy()
+        natt = r.n_attempted.cpu().numpy()
+        print("split", split, "status", st, "attempts", natt, flush=True)
+        assert np.all(st == 2) and np.all(natt == 7)

@@ -132,7 +132,10 @@ def test_surrogate_reduces_off_axis_field_error(lens):
 def test_tracer_learned_field_matches_python_path(lens):
     """The tracer's in-kernel correction (FP32 per lane, BF16-valued weights) against the
     MFMA forward pass of FieldRadialAxialLearned, inside the trust region, and the BEM
-    fallback outside it (r > r_max) against FieldRadialBEM."""
+    fallback outside it (r > r_max) against FieldRadialBEM.  Both round the hidden
+    activations to BF16 (2^-8 relative) but sum in different orders, so an activation can
+    round the other way: the bar is 1e-2 of the largest correction for the worst point and
+    1e-3 for the RMS (a dropped or misplaced term would be O(1))."""
     from traceon_amd.tracing import DeviceFieldProgram
     field, axial, sur, learned = lens
     rng = np.random.default_rng(4)
@@ -147,11 +150,14 @@ def test_tracer_learned_field_matches_python_path(lens):
     corr = np.max(np.abs(Ep - Ea)[r <= sur.r_max])
     out = r > sur.r_max
     Eb = field.electrostatic_field_at_points(pts[out])
-    e_in = np.max(np.abs(Ek - Ep)[~out])
+    d = np.abs(Ek - Ep)[~out]
+    e_in, e_rms = np.max(d), np.sqrt(np.mean(d ** 2))
     e_out = rel(Ek[out], Eb)
-    print("learned field: max correction %.3e, kernel-python %.3e, fallback rel %.3e" % (corr, e_in, e_out), flush=True)
+    print("learned field: max correction %.3e, kernel-python max %.3e rms %.3e, fallback rel %.3e" % (corr, e_in, e_rms, e_out),
+          flush=True)
     assert corr > 0
-    assert e_in <= 1e-3 * corr
+    assert e_in <= 1e-2 * corr
+    assert e_rms <= 1e-3 * corr
     assert e_out < 1e-12
 
 
@@ -182,3 +188,28 @@ def test_learned_tracing_closer_to_bem_than_axial(lens):
     assert np.array_equal(np.isfinite(land["learned"][:, 0]), ok)
     assert err["learned"] < 10e-9
     assert err["learned"] < err["axial"] / 3
+
+
+def test_zero_trust_region_is_direct_bem(lens):
+    """With the trust radius shrunk to zero every evaluation off the axis falls back to the
+    direct BEM sum, so the learned tracer reproduces the direct FP64 BEM tracer."""
+    from traceon_amd import tracing
+    from traceon_amd.surrogate import FieldRadialAxialLearned
+    field, axial, sur, _ = lens
+    learned0 = FieldRadialAxialLearned(sur, trust_radius=0.0)
+    rng = np.random.default_rng(8)
+    n = 64
+    r = rng.uniform(0.1 * MM, 0.5 * MM, n)
+    phi = rng.uniform(0, 2 * np.pi, n)
+    P = np.column_stack([r * np.cos(phi), r * np.sin(phi), np.full(n, -4.9 * MM)])
+    V = np.tile(tracing.velocity_vec(1000.0, [0, 0, 1]), (n, 1))
+    bounds = np.array([[-0.95 * MM, 0.95 * MM], [-0.95 * MM, 0.95 * MM], [-5 * MM, 5 * MM]])
+    plane = (np.array([0.0, 0.0, 4.5 * MM]), np.array([0.0, 0.0, 1.0]))
+    a = learned0.get_tracer(bounds).trace_batch(P, V, plane=plane)
+    b = field.get_tracer(bounds).trace_batch(P, V, plane=plane)
+    la, lb = a.landing.cpu().numpy(), b.landing.cpu().numpy()
+    ok = np.isfinite(lb[:, 0])
+    print("zero trust: steps equal", np.array_equal(a.n_attempted.cpu().numpy(), b.n_attempted.cpu().numpy()),
+          "max landing diff", np.max(np.abs(la[ok] - lb[ok])), flush=True)
+    assert np.array_equal(np.isfinite(la[:, 0]), ok)
+    assert np.max(np.abs(la[ok, :3] - lb[ok, :3])) < 1e-12

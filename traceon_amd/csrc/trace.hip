@@ -197,7 +197,7 @@ __device__ void eval_field(const FieldProgram& F, const double p[3], double E[3]
         // the axial series is trusted up to r_max inside its interpolation range; the learned
         // correction adds to it where it was trained (mlp_correct); elsewhere: ring sources
         const double r = sqrt(q[0] * q[0] + q[1] * q[1]);
-        const bool trust = r <= c.mlp_geom[1] && q[2] >= c.zs[0] && q[2] <= c.zs[c.nz - 1];
+        const bool trust = r <= c.mlp_geom[5] && q[2] >= c.zs[0] && q[2] <= c.zs[c.nz - 1];
         use_bem = !trust && (c.elec.n || c.mag.n || c.cur.n);
       }
       if (MLP && use_bem) {

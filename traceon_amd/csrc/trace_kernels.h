@@ -36,7 +36,8 @@ struct FieldComp {
   double uE[3], uH[3];
   // learned correction of an axial component (null: none).  mlp: FP32 tracer weights
   // W1 (128,2) b1 | W2^T (in,out) b2 | W3^T b3 | W4^T b4 | W5 (5,128) b5; mlp_zs: s_k(z)
-  // (5, mlp_nz) at the z-bin centres; mlp_geom: r_lo, r_max, z_min, z_max, weight power.
+  // (5, mlp_nz) at the z-bin centres; mlp_geom: r_lo, r_max, z_min, z_max, weight power,
+  // trust radius (the BEM sources serve r > trust radius).
   // Outside r <= r_max, z in [z_min, z_max] the component uses its ring sources (BEM).
   const float* mlp;
   const double* mlp_zs;

@@ -15,6 +15,7 @@ modules assemble:
 from __future__ import annotations
 
 import ctypes as C
+import math
 from dataclasses import dataclass
 
 import numpy as np
@@ -181,13 +182,16 @@ class SolveStats:
 
 def solve_system(M64: torch.Tensor | None, F: np.ndarray | torch.Tensor, M32: torch.Tensor | None = None,
                  mode: str = "fp64", max_iter: int = 30, refine: bool = True, lu: LU | None = None,
-                 gemm_timing: KernelTiming | None = None):
+                 gemm_timing: KernelTiming | None = None, fallback: bool = True):
     """Charges for right-hand sides F (nrhs, N) of the assembled system M64 x = F.
 
     mode "fp64":  LU of the FP64 matrix (the dgesv of solver.py:173), then, if ``refine``,
                   iterative refinement with double-double residuals against M64.
     mode "mixed": LU of the FP32 copy M32 (MFMA FP32), FP64 solves, refinement with
-                  double-double residuals against M64.
+                  double-double residuals against M64.  Refinement converges when
+                  cond(M) * eps32 < 1; when it does not reach the rounding level of x (it
+                  stalls or diverges) and ``fallback`` is set, the system is solved again
+                  with an FP64 LU, and the stats report mode "mixed->fp64".
     Refinement stops when the update is at the rounding level of x (2 eps |x|) or stops
     shrinking.  gemm_timing: see lu_factor.  Returns (X (nrhs,N) FP64 tensor, SolveStats, LU)."""
     Bt = F if isinstance(F, torch.Tensor) else to_dev(np.atleast_2d(F))
@@ -203,18 +207,26 @@ def solve_system(M64: torch.Tensor | None, F: np.ndarray | torch.Tensor, M32: to
     X = Bt.clone()
     lu_solve(lu, X)
     it = 0
+    converged = True
     if refine or mode == "mixed":
         last = float("inf")
         eps = 2.0 ** -52
+        converged = False
         for it in range(1, max_iter + 1):
             R, _ = residual(M64, Bt, X)
             lu_solve(lu, R)
             dmax = float(R.abs().max().item())
             xmax = float(X.abs().max().item())
+            if not math.isfinite(dmax) or (dmax >= 0.5 * last and dmax > 64.0 * eps * xmax):
+                break                      # diverging or stalled above the rounding level
             check(_L.tb_axpy(ptr(X), ptr(R), X.numel(), stream_ptr()), "axpy")
             if dmax <= 2.0 * eps * xmax or dmax >= 0.5 * last:
+                converged = True
                 break
             last = dmax
+    if mode == "mixed" and not converged and fallback and M64 is not None and lu.info == 0:
+        X, st, lu = solve_system(M64, Bt, mode="fp64", max_iter=max_iter, refine=True, fallback=False)
+        return X, SolveStats("mixed->fp64", it + st.iterations, st.residual, lu.info), lu
     R, rmax = residual(M64, Bt, X)
     return X, SolveStats(mode, it, float(rmax.max().item()), lu.info), lu
 

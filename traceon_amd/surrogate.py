@@ -246,20 +246,24 @@ class FieldRadialAxialLearned(FieldRadialAxial):
     the surrogate's trust region.  Field evaluation and tracing go through the GPU; the
     reference-named single-point trace functions do not apply (trace_batch / Tracer)."""
 
-    def __init__(self, surrogate: FieldSurrogate):
+    def __init__(self, surrogate: FieldSurrogate, trust_radius: float | None = None):
+        """trust_radius: the axial field (+ correction) serves r <= trust_radius inside the axial
+        range; the direct BEM sum serves everything else.  Default: the surrogate's r_max.
+        A trust radius of 0 sends every off-axis evaluation to the BEM sum."""
         ax = surrogate.axial
         FieldRadialAxial.__bases__[0].__init__(self, ax.field, ax.z, ax.electrostatic_coeffs, ax.magnetostatic_coeffs)
         self.surrogate = surrogate
+        self.trust_radius = float(surrogate.r_max if trust_radius is None else trust_radius)
         self._origin, self._basis = ax._origin, ax._basis
         self._update_inverse_transformation_matrix()
 
     def _corrected(self, points):
-        """Trust region (as the tracer's): r <= r_max inside the axial range; the correction
-        where r >= r_lo inside the trained z range."""
+        """Trust region (as the tracer's): r <= trust_radius inside the axial range; the
+        correction where r_lo <= r <= r_max inside the trained z range."""
         P = np.atleast_2d(np.asarray(points, dtype=np.float64))
         s = self.surrogate
         r = np.hypot(P[:, 0], P[:, 1])
-        inside = (r <= s.r_max) & (P[:, 2] >= self.z[0]) & (P[:, 2] <= self.z[-1])
+        inside = (r <= self.trust_radius) & (P[:, 2] >= self.z[0]) & (P[:, 2] <= self.z[-1])
         dev = device.device()
         C = s.correction(torch.tensor(r, device=dev), torch.tensor(P[:, 2], device=dev)).cpu().numpy()
         C[~((P[:, 2] >= s.z_min) & (P[:, 2] <= s.z_max) & inside)] = 0.0

@@ -131,8 +131,9 @@ def make_component(field, factor: float, kind: str, in_superposition: bool):
             sur = field.surrogate
             wts, zs, geom = sur.tracer_arrays()
             c.mlp, c.mlp_zs, c.mlp_nz = wts.data_ptr(), zs.data_ptr(), zs.shape[1]
-            for i, v in enumerate(geom):
+            for i, v in enumerate(geom[:5]):
                 c.mlp_geom[i] = v
+            c.mlp_geom[5] = field.trust_radius
             keep += [wts, zs]
             keep.append(_set_sources(c.elec, sur.field.electrostatic_point_charges))
             keep.append(_set_sources(c.mag, sur.field.magnetostatic_point_charges))
