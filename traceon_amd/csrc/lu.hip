@@ -198,32 +198,58 @@ __global__ __launch_bounds__(256) void k_laswp(T* __restrict__ A, size_t lda, in
 // ---------------------------------------------------------------------------------
 
 // Diagonal block of a panel in one workgroup of 1024 threads holding the NB x NB block in
-// registers (thread t: column t % NB, rows t / NB + 8k, k = 0..15); outside nb x nb the
-// block is padded with the identity.
+// registers.  Wave w holds 16 consecutive rows (row block rb) and 64 consecutive columns
+// (column half h): thread = (rb, column c), rows 16 rb + k, k = 0..15.  The waves are
+// numbered so that each SIMD holds two row blocks four apart and both column halves, which
+// keeps the work of a step spread over the SIMDs while finished blocks drop out.  Outside
+// nb x nb the block is padded with the identity.
 //   factor = 1: unpivoted LU of the block (fast path).  Multipliers are formed with the
 //     reciprocal of the pivot, as LAPACK's dgetf2 scales its column; max |l| -> lmax.
 //   Then, always: Y = inv(L11) (unit lower) and Z = inv(U11), row-major NB x NB with zero
 //   padding.  L is inverted forwards and U backwards in the same sweep.
 // Every step publishes one row and one column through LDS, double-buffered by step
-// parity, so a step costs one barrier.
+// parity, so a step costs one barrier.  A wave whose rows or columns a step cannot change
+// skips the step's arithmetic with one wave-uniform branch (the work of a step is then
+// proportional to the trailing block).  The column a thread needs (its 16 rows) is stored
+// contiguously: four (FP32) or eight (FP64) 16-byte broadcast reads.  The factored block
+// is staged column by column in the same order for the inverse sweep.  The kernel runs
+// beside the trailing GEMM of the previous panel, so it is kept near 72 VGPRs (FP32): 16
+// waves of it and one GEMM workgroup then fit a CU's register file together.
+template <typename T>
+constexpr int diag_col_stride() { return LU_NB + 16 / (int)sizeof(T); }
+template <typename T>
+constexpr size_t diag_lds_bytes() { return sizeof(T) * LU_NB * diag_col_stride<T>(); }
+
 template <typename T>
 __global__ __launch_bounds__(1024) void k_diag_block(T* __restrict__ A, size_t lda, int k0, int nb, int factor,
                                                     int* __restrict__ ipiv, T* __restrict__ Y, T* __restrict__ Z,
-                                                    unsigned long long* __restrict__ lmax, const int* gate, int gp) {
-  constexpr int NB = LU_NB, RG = 1024 / NB, KR = NB / RG;
+                                                    unsigned long long* __restrict__ lmax, const int* gate, int gp,
+                                                    T* __restrict__ backup) {
+  constexpr int NB = LU_NB, NRB = 8, KR = NB / NRB;   // 8 row blocks of 16 rows
+  constexpr int VEC = 16 / (int)sizeof(T), NV = KR / VEC;
+  constexpr int CS = diag_col_stride<T>();   // column stride of the staged block (padded: conflict-free staging)
+  typedef T vec_t __attribute__((ext_vector_type(VEC)));
   if (gated_off(gate, gp)) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* Bs = reinterpret_cast<T*>(smem_raw);   // the factored block, NB x NB, for the inverse sweep
-  __shared__ T rowb[2][NB], colb[2][NB], rowc[2][NB];
+  T* Bs = reinterpret_cast<T*>(smem_raw);   // staged block: element (16 rb + k, col) at col * CS + 16 rb + k
+  __shared__ __align__(16) T colv[2][NB];
+  __shared__ T rowb[2][NB], rowc[2][NB], udinv[NB];
   __shared__ double wmax[1024 / 64];
-  const int tid = threadIdx.x, c = tid & (NB - 1), rg = tid / NB;
-  // The inner loops are written with selects only: the rows a thread holds differ by k,
-  // and a branch per k would serialise one LDS latency per row.
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int h = (w >> 2) & 1;                       // column half
+  const int rb = (w & 3) | ((w >> 3) << 2);         // row block
+  const int c = h * 64 + lane, r0 = rb * KR;        // this thread's column, first row
+  const int c0 = h * 64;                            // the wave's first column (uniform)
   T B[KR];
 #pragma unroll
   for (int k = 0; k < KR; k++) {
-    const int r = rg + RG * k;
+    const int r = r0 + k;
     B[k] = (r < nb && c < nb) ? A[(size_t)(k0 + r) * lda + k0 + c] : (r == c ? T(1) : T(0));
+  }
+  if (backup) {   // the block as it was, for a restore when the panel fails the no-pivot test
+#pragma unroll
+    for (int k = 0; k < KR; k++) backup[(size_t)(r0 + k) * NB + c] = B[k];
   }
   // v[kk] for a runtime kk, as a chain of selects (keeps v in registers)
 #define TB_PICK(dst, v, kk)                                          \
@@ -235,37 +261,47 @@ __global__ __launch_bounds__(1024) void k_diag_block(T* __restrict__ A, size_t l
   if (factor) {
     for (int j = 0; j < nb; j++) {
       const int p = j & 1;
-      T own;
-      TB_PICK(own, B, j / RG);
-      if (rg == j % RG) rowb[p][c] = own;
-      if (c == j) {
+      if (rb == j / KR) {   // the pivot row
+        T own;
+        TB_PICK(own, B, j % KR);
+        rowb[p][c] = own;
+      }
+      if (c == j) {         // the pivot column, this row block's 16 rows
 #pragma unroll
-        for (int k = 0; k < KR; k++) colb[p][rg + RG * k] = B[k];
+        for (int v = 0; v < NV; v++) {
+          vec_t x;
+#pragma unroll
+          for (int e = 0; e < VEC; e++) x[e] = B[v * VEC + e];
+          *reinterpret_cast<vec_t*>(&colv[p][r0 + v * VEC]) = x;
+        }
       }
       __syncthreads();
-      T cv[KR];
+      // rows up to j and columns below j are final
+      if (r0 + KR - 1 > j && c0 + 63 >= j) {
+        vec_t cvv[NV];
 #pragma unroll
-      for (int k = 0; k < KR; k++) cv[k] = colb[p][rg + RG * k];
-      const T rv = rowb[p][c];
-      const T rp = T(1) / rowb[p][j];
+        for (int v = 0; v < NV; v++) cvv[v] = *reinterpret_cast<const vec_t*>(&colv[p][r0 + v * VEC]);
+        const T rv = rowb[p][c];
+        const T rp = T(1) / rowb[p][j];
 #pragma unroll
-      for (int k = 0; k < KR; k++) {
-        const int r = rg + RG * k;
-        const T l = cv[k] * rp;
-        const T upd = fma(-l, rv, B[k]);
-        B[k] = r > j ? (c == j ? l : (c > j ? upd : B[k])) : B[k];
+        for (int k = 0; k < KR; k++) {
+          const int r = r0 + k;
+          const T l = cvv[k / VEC][k % VEC] * rp;
+          const T upd = fma(-l, rv, B[k]);
+          B[k] = r > j ? (c == j ? l : (c > j ? upd : B[k])) : B[k];
+        }
       }
     }
     // max |l| over the strictly lower part (NaN counts as large)
 #pragma unroll
     for (int k = 0; k < KR; k++) {
-      const int r = rg + RG * k;
+      const int r = r0 + k;
       const double al = (r > c && r < nb) ? (double)tabs(B[k]) : 0.0;
       m = (al > m || al != al) ? al : m;
     }
 #pragma unroll
     for (int k = 0; k < KR; k++) {
-      const int r = rg + RG * k;
+      const int r = r0 + k;
       if (r < nb && c < nb) A[(size_t)(k0 + r) * lda + k0 + c] = B[k];
     }
     if (tid < nb) ipiv[k0 + tid] = k0 + tid;
@@ -273,56 +309,79 @@ __global__ __launch_bounds__(1024) void k_diag_block(T* __restrict__ A, size_t l
       const double x = __shfl_xor(m, o, 64);
       m = (x > m || x != x) ? x : m;
     }
-    if ((tid & 63) == 0) wmax[tid >> 6] = m;
+    if (lane == 0) wmax[w] = m;
+  }
+  // stage the factored block column by column; reciprocals of U's diagonal
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    vec_t x;
+#pragma unroll
+    for (int e = 0; e < VEC; e++) x[e] = B[v * VEC + e];
+    *reinterpret_cast<vec_t*>(&Bs[c * CS + r0 + v * VEC]) = x;
   }
 #pragma unroll
-  for (int k = 0; k < KR; k++) Bs[(rg + RG * k) * NB + c] = B[k];
+  for (int k = 0; k < KR; k++)
+    if (r0 + k == c) udinv[c] = T(1) / B[k];
   __syncthreads();
   if (factor && tid == 0) {
     double mm = 0.0;
-    for (int w = 0; w < 1024 / 64; w++) mm = (wmax[w] > mm || wmax[w] != wmax[w]) ? wmax[w] : mm;
+    for (int q = 0; q < 1024 / 64; q++) mm = (wmax[q] > mm || wmax[q] != wmax[q]) ? wmax[q] : mm;
     if (mm != mm) mm = 1e300;
     atomicMax(lmax, (unsigned long long)__double_as_longlong(mm));
   }
-  // Y = inv(L) forwards and Z = inv(U) backwards, one row of each final per step
+  // Y = inv(L) forwards and Z = inv(U) backwards, one row of each final per step.  Row jl
+  // of Y is zero right of jl and changes rows below it; row ju of Z is zero left of ju and
+  // changes rows above it.
   T Yv[KR], Zv[KR];
 #pragma unroll
   for (int k = 0; k < KR; k++) {
-    const int r = rg + RG * k;
+    const int r = r0 + k;
     Yv[k] = (r == c) ? T(1) : T(0);
     Zv[k] = (r == c) ? T(1) : T(0);
   }
   for (int st = 0; st < NB; st++) {
     const int p = st & 1;
     const int jl = st, ju = NB - 1 - st;
-    T yown;
-    TB_PICK(yown, Yv, jl / RG);
-    if (rg == jl % RG) rowb[p][c] = yown;
-    if (rg == ju % RG) {
+    if (rb == jl / KR) {
+      T yown;
+      TB_PICK(yown, Yv, jl % KR);
+      rowb[p][c] = yown;
+    }
+    if (rb == ju / KR) {
       T zown;
-      TB_PICK(zown, Zv, ju / RG);
-      zown = zown / Bs[ju * NB + ju];
+      TB_PICK(zown, Zv, ju % KR);
+      zown = zown * udinv[ju];
 #pragma unroll
-      for (int k = 0; k < KR; k++) Zv[k] = (k == ju / RG) ? zown : Zv[k];
+      for (int k = 0; k < KR; k++) Zv[k] = (k == ju % KR) ? zown : Zv[k];
       rowc[p][c] = zown;
     }
     __syncthreads();
-    const T yr = rowb[p][c], zr = rowc[p][c];
-    T lv[KR], uv[KR];
+    if (r0 + KR - 1 > jl && c0 <= jl) {
+      const T yr = rowb[p][c];
+      vec_t lvv[NV];
 #pragma unroll
-    for (int k = 0; k < KR; k++) { lv[k] = Bs[(rg + RG * k) * NB + jl]; uv[k] = Bs[(rg + RG * k) * NB + ju]; }
+      for (int v = 0; v < NV; v++) lvv[v] = *reinterpret_cast<const vec_t*>(&Bs[jl * CS + r0 + v * VEC]);
 #pragma unroll
-    for (int k = 0; k < KR; k++) {
-      const int r = rg + RG * k;
-      const T ny = fma(-lv[k], yr, Yv[k]);
-      const T nz = fma(-uv[k], zr, Zv[k]);
-      Yv[k] = r > jl ? ny : Yv[k];
-      Zv[k] = r < ju ? nz : Zv[k];
+      for (int k = 0; k < KR; k++) {
+        const T ny = fma(-lvv[k / VEC][k % VEC], yr, Yv[k]);
+        Yv[k] = r0 + k > jl ? ny : Yv[k];
+      }
+    }
+    if (r0 < ju && c0 + 63 >= ju) {
+      const T zr = rowc[p][c];
+      vec_t uvv[NV];
+#pragma unroll
+      for (int v = 0; v < NV; v++) uvv[v] = *reinterpret_cast<const vec_t*>(&Bs[ju * CS + r0 + v * VEC]);
+#pragma unroll
+      for (int k = 0; k < KR; k++) {
+        const T nz = fma(-uvv[k / VEC][k % VEC], zr, Zv[k]);
+        Zv[k] = r0 + k < ju ? nz : Zv[k];
+      }
     }
   }
 #pragma unroll
   for (int k = 0; k < KR; k++) {
-    const int r = rg + RG * k;
+    const int r = r0 + k;
     const bool in = r < nb && c < nb;
     Y[(size_t)r * NB + c] = in ? Yv[k] : T(0);
     Z[(size_t)r * NB + c] = in ? Zv[k] : T(0);
@@ -414,10 +473,12 @@ constexpr int GBM = 128, GBN = 128;
 // memory latencies instead of one per element.  Tiles that lie wholly inside the matrices
 // (with 16-byte aligned rows) take unguarded vector accesses; edge tiles take element
 // accesses with bounds checks.
-template <typename T, int BK>
+// TRACK_MAX: also the largest |C| written (NaN counts as large), folded into *vmax as the
+// bit pattern of a nonnegative double (one atomic per workgroup).
+template <typename T, int BK, bool TRACK_MAX>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, const T* __restrict__ A, size_t lda,
                                               const T* __restrict__ B, size_t ldb, T beta, T* __restrict__ C, size_t ldc,
-                                              const int* gate, int gp) {
+                                              const int* gate, int gp, unsigned long long* __restrict__ vmax) {
   typedef typename MfmaAcc<T>::type acc_t;
   if (gated_off(gate, gp)) return;
   constexpr int VEC = 16 / sizeof(T);                     // elements per 16-byte load
@@ -530,6 +591,13 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, cons
   constexpr int SEG = GBN / VEC;                          // 16-byte segments per tile row
   constexpr int PER = CROWS * SEG / 256;                  // segments per thread per pass
   constexpr int BATCH = PER < 8 ? PER : 8;
+  double vm = 0.0;
+  auto track = [&](T x) {
+    if constexpr (TRACK_MAX) {
+      const double a = (double)tabs(x);
+      vm = (a > vm || a != a) ? a : vm;
+    }
+  };
 #pragma unroll
   for (int pass = 0; pass < GBM / CROWS; pass++) {
     const int r0 = pass * CROWS;
@@ -563,6 +631,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, cons
           for (int v = 0; v < VEC; v++) {
             const T val = Cs[(row - r0) * SC + c + v];
             out[v] = use_c ? fma(beta, cvv[j][v], val) : val;
+            track(out[v]);
           }
           *reinterpret_cast<vec_t*>(C + (size_t)(bm + row) * ldc + bn + c) = out;
         }
@@ -581,11 +650,29 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, cons
           for (int v = 0; v < VEC; v++) {
             if (bm + row < M && bn + c + v < N) {
               const T val = Cs[(row - r0) * SC + c + v];
-              C[(size_t)(bm + row) * ldc + bn + c + v] = use_c ? fma(beta, cvv[j][v], val) : val;
+              const T o = use_c ? fma(beta, cvv[j][v], val) : val;
+              track(o);
+              C[(size_t)(bm + row) * ldc + bn + c + v] = o;
             }
           }
         }
       }
+    }
+  }
+  if constexpr (TRACK_MAX) {
+    for (int o = 32; o >= 1; o >>= 1) {
+      const double x = __shfl_xor(vm, o, 64);
+      vm = (x > vm || x != x) ? x : vm;
+    }
+    __syncthreads();   // Cs is reused for the four wave maxima
+    double* wmx = reinterpret_cast<double*>(smem);
+    if (lane == 0) wmx[wave] = vm;
+    __syncthreads();
+    if (tid == 0) {
+      double mm = 0.0;
+      for (int w = 0; w < 4; w++) mm = (wmx[w] > mm || wmx[w] != wmx[w]) ? wmx[w] : mm;
+      if (mm != mm) mm = 1e300;
+      if (mm > 0.0) atomicMax(vmax, (unsigned long long)__double_as_longlong(mm));
     }
   }
 }
@@ -890,17 +977,21 @@ static inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) 
 
 template <typename T>
 hipError_t gemm(int M, int N, int K, T alpha, const T* A, size_t lda, const T* B, size_t ldb, T beta, T* C, size_t ldc,
-                hipStream_t st, const int* gate, int gp) {
+                hipStream_t st, const int* gate, int gp, unsigned long long* vmax) {
   if (M <= 0 || N <= 0) return hipSuccess;
   dim3 grid(cdiv(N, GBN), cdiv(M, GBM));
   constexpr int BK = sizeof(T) == 8 ? 16 : 32;
-  hipLaunchKernelGGL((k_gemm<T, BK>), grid, dim3(256), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, gp);
+  if (vmax)
+    hipLaunchKernelGGL((k_gemm<T, BK, true>), grid, dim3(256), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, gp, vmax);
+  else
+    hipLaunchKernelGGL((k_gemm<T, BK, false>), grid, dim3(256), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, gp,
+                       (unsigned long long*)nullptr);
   return hipGetLastError();
 }
 template hipError_t gemm<double>(int, int, int, double, const double*, size_t, const double*, size_t, double, double*, size_t,
-                                 hipStream_t, const int*, int);
+                                 hipStream_t, const int*, int, unsigned long long*);
 template hipError_t gemm<float>(int, int, int, float, const float*, size_t, const float*, size_t, float, float*, size_t, hipStream_t,
-                                const int*, int);
+                                const int*, int, unsigned long long*);
 
 // Workspace of one factorisation (kept with the factors: the solves read the block inverses,
 // the permutation and the pivot flag from it).
@@ -909,12 +1000,13 @@ struct LuWorkspace {
   static constexpr int G = 1024, NB = LU_NB;
   double* cand_val; int* cand_idx; T* cand_row;   // pivot candidates of the cooperative panel
   T* Y; T* Z;                                     // scratch inverses of the current panel
-  T* U12; T* Pbak;                                // trailing-update staging, panel backup
+  T* U12; T* U12b;                                // U12 of the current panel (two, alternating by panel)
   T* Linv; T* Uinv;                               // inverses of every diagonal block (nblk x NB x NB)
   int* perm; int* flags; int* err; int* pivoted;  // permutation, solve tickets, error, any-interchange flag
   double* rhs;                                    // copy of the right-hand sides (nrhs x N, nrhs <= 64)
   double* ytmp;                                   // L^{-1} P b of the solve in progress (N)
-  T* Ltmp;                                        // L21 = A21 inv(U11) before it is checked (N x NB)
+  T* Ltmp;                                        // L21 = A21 inv(U11) of the speculative pass (2 x N x NB, by panel parity)
+  T* Dbak;                                        // the current diagonal block before its factorisation (NB x NB)
   unsigned long long* lmaxv;                      // max |l| per panel of the speculative pass (nblk)
   int* first_fail;                                // first panel of that pass that failed the no-pivot test
   unsigned long long* lmax;
@@ -931,15 +1023,16 @@ struct LuWorkspace {
     size_t a_pm = take((size_t)N * sizeof(int)), a_fl = take((2 * (size_t)nblk + 2) * sizeof(int));
     size_t a_er = take(sizeof(int)), a_pv = take(sizeof(int)), a_rhs = take(64 * (size_t)N * sizeof(double)), a_lm = take(8);
     size_t a_yt = take((size_t)N * sizeof(double));
-    size_t a_lt = take((size_t)N * NB * sizeof(T)), a_lv = take((size_t)nblk * 8), a_ff = take(sizeof(int));
+    size_t a_lt = take(2 * (size_t)N * NB * sizeof(T)), a_lv = take((size_t)nblk * 8), a_ff = take(sizeof(int));
+    size_t a_db = take((size_t)NB * NB * sizeof(T));
     bytes = o;
     if (!base) return;
     cand_val = (double*)(p + a_cv); cand_idx = (int*)(p + a_ci); cand_row = (T*)(p + a_cr);
-    Y = (T*)(p + a_y); Z = (T*)(p + a_z); U12 = (T*)(p + a_u); Pbak = (T*)(p + a_pb);
+    Y = (T*)(p + a_y); Z = (T*)(p + a_z); U12 = (T*)(p + a_u); U12b = (T*)(p + a_pb);
     Linv = (T*)(p + a_li); Uinv = (T*)(p + a_ui); perm = (int*)(p + a_pm); flags = (int*)(p + a_fl);
     err = (int*)(p + a_er); pivoted = (int*)(p + a_pv); rhs = (double*)(p + a_rhs); lmax = (unsigned long long*)(p + a_lm);
     ytmp = (double*)(p + a_yt);
-    Ltmp = (T*)(p + a_lt); lmaxv = (unsigned long long*)(p + a_lv); first_fail = (int*)(p + a_ff);
+    Ltmp = (T*)(p + a_lt); lmaxv = (unsigned long long*)(p + a_lv); first_fail = (int*)(p + a_ff); Dbak = (T*)(p + a_db);
   }
 };
 
@@ -954,7 +1047,7 @@ template size_t lu_workspace_bytes<float>(int);
 // the life of the process (default priority).  Calls on one device are serialised.
 struct SideStream {
   hipStream_t s = nullptr;
-  hipEvent_t ev_start = nullptr, ev_p = nullptr, ev_u = nullptr, ev_end = nullptr;
+  hipEvent_t ev_start = nullptr, ev_r = nullptr, ev_u = nullptr, ev_end = nullptr;
   std::mutex mu;
 };
 static SideStream g_side[64];
@@ -967,7 +1060,7 @@ static hipError_t side_stream(int dev, SideStream** out) {
   if (!x.s) {
     hipError_t e = hipStreamCreateWithFlags(&x.s, hipStreamNonBlocking);
     if (e != hipSuccess) { x.s = nullptr; return e; }
-    for (hipEvent_t* ev : {&x.ev_start, &x.ev_p, &x.ev_u, &x.ev_end}) {
+    for (hipEvent_t* ev : {&x.ev_start, &x.ev_r, &x.ev_u, &x.ev_end}) {
       e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
       if (e != hipSuccess) return e;
     }
@@ -977,20 +1070,25 @@ static hipError_t side_stream(int dev, SideStream** out) {
 }
 
 // Factorisation driver.  Most BEM panels need no row interchange, so the factorisation
-// first runs speculatively without pivoting and without host round trips: per panel p
-//   P(p)  (side stream): keep a copy of the panel column, factor the diagonal block and
-//         invert its triangles (k_diag_block), L21 = A21 inv(U11), record max |l|, and
-//         mark p as failed when some |l| > 1;
-//   U(p)  (caller's stream): U12 = inv(L11) A12;
-//   Tc(p) (side stream): the trailing update of the next panel's column only;
-//   Tr(p) (caller's stream): the trailing update of the remaining columns.
-// P(p+1) depends only on Tc(p), so the diagonal-block work of the next panel runs beside
-// the large Tr(p) product (one-panel lookahead).  Every kernel of panel p is gated on "no
-// panel <= p failed" (first_fail), so after a failure at panel f the matrix holds panels
-// < f complete with their trailing updates, and panel f's column is restored from its copy
-// and factored with partial pivoting (cooperative k_panel); the speculative pass then
-// resumes at f + 1.  A panel that passes the test (every |l| <= 1) is exactly the partially
-// pivoted result: the diagonal entries are the column maxima, so ipiv is the identity.
+// first runs speculatively without pivoting and without host round trips.  Per panel p
+// (side stream A unless stated):
+//   P(p):  factor the diagonal block and invert its triangles (k_diag_block, which keeps a
+//          copy of the block), L21 = A21 inv(U11) into a panel buffer with max |l| folded
+//          in by the GEMM, and mark p failed if some |l| > 1;
+//   U(p):  U12 = inv(L11) A12 into a buffer, after the bulk stream's Tr(p-1);
+//   Tc(p): the trailing update of the next panel's column only;
+//   Tr(p)  (caller's stream B): the trailing update of the remaining columns, then the
+//          copies of L21 and U12 into the matrix.
+// The chain P(p) -> U(p) -> Tc(p) -> P(p+1) stays on one stream, and P(p+1) and Tc(p) run
+// beside the large Tr(p) (one-panel lookahead); L21 and U12 alternate between two buffers
+// by panel parity, so Tr(p) still reads panel p's while panel p+1 writes its own.  Every
+// kernel of panel p is gated on "no panel <= p failed" (a device-side first_fail).  After a
+// pass that failed at panel f, the matrix holds panels < f complete with their trailing
+// updates and panel f's column untouched except its diagonal block, which is restored from
+// the copy; f is then factored with partial pivoting (cooperative k_panel) and the
+// speculative pass resumes at f + 1.  A panel that passes the test (every |l| <= 1) is
+// exactly the partially pivoted result: its diagonal entries are the column maxima, so its
+// ipiv is the identity.
 template <typename T>
 hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev, hipStream_t st, int panel_groups,
                      KernelTiming* tm) {
@@ -1004,7 +1102,7 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
   } events;
   auto& ev = events.v;
   auto timed_gemm = [&](hipStream_t s, const int* gate, int gp, int M, int Nn, int K, T alpha, const T* Ap, size_t la, const T* Bp,
-                        size_t lb, T beta, T* Cp, size_t lc) -> hipError_t {
+                        size_t lb, T beta, T* Cp, size_t lc, unsigned long long* vmax = nullptr) -> hipError_t {
     if (M <= 0 || Nn <= 0) return hipSuccess;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (tm) {
@@ -1012,7 +1110,7 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
       ev.emplace_back(e0, e1);
       if (hipEventRecord(e0, s) != hipSuccess) return hipErrorLaunchFailure;
     }
-    hipError_t r = gemm<T>(M, Nn, K, alpha, Ap, la, Bp, lb, beta, Cp, lc, s, gate, gp);
+    hipError_t r = gemm<T>(M, Nn, K, alpha, Ap, la, Bp, lb, beta, Cp, lc, s, gate, gp, vmax);
     if (tm) {
       if (hipEventRecord(e1, s) != hipSuccess) return hipErrorLaunchFailure;
       tm->launches++;
@@ -1045,7 +1143,7 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
   e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kpanel, 256, Rmax * NB * sizeof(T));
   if (e != hipSuccess) return e;
   if (occ * prop.multiProcessorCount < G) return hipErrorCooperativeLaunchTooLarge;
-  e = hipFuncSetAttribute((const void*)k_diag_block<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(NB * NB * sizeof(T)));
+  e = hipFuncSetAttribute((const void*)k_diag_block<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)diag_lds_bytes<T>());
   if (e != hipSuccess) return e;
   SideStream* ss = nullptr;
   e = side_stream(dev, &ss);
@@ -1056,7 +1154,6 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
   LuWorkspace<T> W(ws, N);
   const int nblk = (N + NB - 1) / NB;
   T* U12 = W.U12;
-  T* Pbak = W.Pbak;
   bool any_pivot = false;
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, W.err, 0);
   auto Yp = [&](int q) { return W.Linv + (size_t)q * NB * NB; };
@@ -1069,47 +1166,42 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
     hipLaunchKernelGGL(k_zero_u64, dim3(cdiv(nblk, 256)), dim3(256), 0, st, W.lmaxv, nblk);
     hipError_t r = hipEventRecord(ss->ev_start, st);
     if (r != hipSuccess) return r;
-    r = hipStreamWaitEvent(sa, ss->ev_start, 0);
-    if (r != hipSuccess) return r;
-    auto panel = [&](int q) -> hipError_t {   // P(q) on the side stream
-      const int k0 = q * NB, nb = min(NB, N - k0), m2 = N - k0 - nb;
-      hipLaunchKernelGGL(k_copy2d<T>, dim3(cdiv((size_t)(N - k0) * nb, 256)), dim3(256), 0, sa, A + (size_t)k0 * lda + k0, lda, Pbak,
-                         (size_t)nb, N - k0, nb, g, q);
-      hipLaunchKernelGGL(k_diag_block<T>, dim3(1), dim3(1024), NB * NB * sizeof(T), sa, A, lda, k0, nb, 1, ipiv, Yp(q), Zp(q),
-                         W.lmaxv + q, g, q);
+    if ((r = hipStreamWaitEvent(sa, ss->ev_start, 0)) != hipSuccess) return r;
+    auto Lbuf = [&](int q) { return W.Ltmp + (size_t)(q & 1) * N * NB; };
+    auto Ubuf = [&](int q) { return (q & 1) ? W.U12b : U12; };
+    for (int q = s0; q < nblk; q++) {
+      const int k0 = q * NB, nb = min(NB, N - k0), m2 = N - k0 - nb, n2 = m2;
+      const int k1 = k0 + nb;
+      // P(q)
+      hipLaunchKernelGGL(k_diag_block<T>, dim3(1), dim3(1024), diag_lds_bytes<T>(), sa, A, lda, k0, nb, 1, ipiv, Yp(q), Zp(q),
+                         W.lmaxv + q, g, q, W.Dbak);
       if (m2 > 0) {
-        T* A21 = A + (size_t)(k0 + nb) * lda + k0;
-        hipError_t r2 = timed_gemm(sa, g, q, m2, nb, nb, T(1), A21, lda, Zp(q), NB, T(0), W.Ltmp, (size_t)nb);
-        if (r2 != hipSuccess) return r2;
-        const size_t nblk_copy = std::min<size_t>(cdiv((size_t)m2 * nb, 256), 1024);
-        hipLaunchKernelGGL(k_copy_back_max<T>, dim3(nblk_copy), dim3(256), 0, sa, W.Ltmp, (size_t)nb, A21, lda, m2, nb, W.lmaxv + q,
-                           g, q);
+        const T* A21 = A + (size_t)k1 * lda + k0;
+        if ((r = timed_gemm(sa, g, q, m2, nb, nb, T(1), A21, lda, Zp(q), NB, T(0), Lbuf(q), (size_t)nb, W.lmaxv + q)) != hipSuccess)
+          return r;
       }
       hipLaunchKernelGGL(k_panel_check, dim3(1), dim3(1), 0, sa, W.lmaxv + q, W.first_fail, q);
-      return hipEventRecord(ss->ev_p, sa);
-    };
-    r = panel(s0);
-    if (r != hipSuccess) return r;
-    for (int q = s0; q < nblk; q++) {
-      const int k0 = q * NB, nb = min(NB, N - k0), n2 = N - k0 - nb;
       if (n2 <= 0) break;
-      const int k1 = k0 + nb, nb1 = min(NB, n2);
-      T* A12 = A + (size_t)k0 * lda + k1;
-      const T* L21 = A + (size_t)k1 * lda + k0;
-      // U(q) on the caller's stream, after P(q)
-      if ((r = hipStreamWaitEvent(st, ss->ev_p, 0)) != hipSuccess) return r;
-      if ((r = timed_gemm(st, g, q, nb, n2, nb, T(1), Yp(q), NB, A12, lda, T(0), U12, (size_t)n2)) != hipSuccess) return r;
-      hipLaunchKernelGGL(k_copy2d<T>, dim3(cdiv((size_t)nb * n2, 256)), dim3(256), 0, st, U12, (size_t)n2, A12, lda, nb, n2, g, q);
-      if ((r = hipEventRecord(ss->ev_u, st)) != hipSuccess) return r;
-      // Tc(q) and P(q+1) on the side stream
-      if ((r = hipStreamWaitEvent(sa, ss->ev_u, 0)) != hipSuccess) return r;
-      if ((r = timed_gemm(sa, g, q, n2, nb1, nb, T(-1), L21, lda, U12, (size_t)n2, T(1), A + (size_t)k1 * lda + k1, lda)) != hipSuccess)
-        return r;
-      if ((r = panel(q + 1)) != hipSuccess) return r;
-      // Tr(q) on the caller's stream, beside P(q+1)
-      if ((r = timed_gemm(st, g, q, n2, n2 - nb1, nb, T(-1), L21, lda, U12 + nb1, (size_t)n2, T(1), A + (size_t)k1 * lda + k1 + nb1,
+      const int nb1 = min(NB, n2);
+      const T* A12 = A + (size_t)k0 * lda + k1;
+      // U(q) after Tr(q-1) of the bulk stream
+      if (q > s0 && (r = hipStreamWaitEvent(sa, ss->ev_r, 0)) != hipSuccess) return r;
+      if ((r = timed_gemm(sa, g, q, nb, n2, nb, T(1), Yp(q), NB, A12, lda, T(0), Ubuf(q), (size_t)n2)) != hipSuccess) return r;
+      if ((r = hipEventRecord(ss->ev_u, sa)) != hipSuccess) return r;
+      // Tc(q): the next panel's column
+      if ((r = timed_gemm(sa, g, q, n2, nb1, nb, T(-1), Lbuf(q), (size_t)nb, Ubuf(q), (size_t)n2, T(1), A + (size_t)k1 * lda + k1,
                           lda)) != hipSuccess)
         return r;
+      // Tr(q) on the caller's stream, then the factors of panel q into the matrix
+      if ((r = hipStreamWaitEvent(st, ss->ev_u, 0)) != hipSuccess) return r;
+      if ((r = timed_gemm(st, g, q, n2, n2 - nb1, nb, T(-1), Lbuf(q), (size_t)nb, Ubuf(q) + nb1, (size_t)n2, T(1),
+                          A + (size_t)k1 * lda + k1 + nb1, lda)) != hipSuccess)
+        return r;
+      hipLaunchKernelGGL(k_copy2d<T>, dim3(cdiv((size_t)m2 * nb, 256)), dim3(256), 0, st, Lbuf(q), (size_t)nb,
+                         A + (size_t)k1 * lda + k0, lda, m2, nb, g, q);
+      hipLaunchKernelGGL(k_copy2d<T>, dim3(cdiv((size_t)nb * n2, 256)), dim3(256), 0, st, Ubuf(q), (size_t)n2,
+                         A + (size_t)k0 * lda + k1, lda, nb, n2, g, q);
+      if ((r = hipEventRecord(ss->ev_r, st)) != hipSuccess) return r;
     }
     if ((r = hipEventRecord(ss->ev_end, sa)) != hipSuccess) return r;
     if ((r = hipStreamWaitEvent(st, ss->ev_end, 0)) != hipSuccess) return r;
@@ -1131,8 +1223,8 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
     hipError_t r = hipLaunchCooperativeKernel((const void*)kpanel, dim3(G), dim3(256), args, (unsigned)(R * NB * sizeof(T)), st);
     if (r != hipSuccess) return r;
     any_pivot = true;
-    hipLaunchKernelGGL(k_diag_block<T>, dim3(1), dim3(1024), NB * NB * sizeof(T), st, A, lda, k0, nb, 0, ipiv, Yp(q), Zp(q), W.lmax,
-                       (const int*)nullptr, 0);
+    hipLaunchKernelGGL(k_diag_block<T>, dim3(1), dim3(1024), diag_lds_bytes<T>(), st, A, lda, k0, nb, 0, ipiv, Yp(q), Zp(q), W.lmax,
+                       (const int*)nullptr, 0, (T*)nullptr);
     if (N - nb > 0) hipLaunchKernelGGL(k_laswp<T>, dim3(cdiv(N - nb, 256)), dim3(256), 0, st, A, lda, N, k0, nb, ipiv);
     const int n2 = N - k0 - nb;
     if (n2 > 0) {
@@ -1153,8 +1245,8 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
     if (e != hipSuccess) return e;
     if (kf < q || kf >= nblk) break;
     const int k0 = kf * NB, nb = min(NB, N - k0);
-    hipLaunchKernelGGL(k_copy2d<T>, dim3(cdiv((size_t)(N - k0) * nb, 256)), dim3(256), 0, st, Pbak, (size_t)nb,
-                       A + (size_t)k0 * lda + k0, lda, N - k0, nb, (const int*)nullptr, 0);
+    hipLaunchKernelGGL(k_copy2d<T>, dim3(cdiv((size_t)nb * nb, 256)), dim3(256), 0, st, W.Dbak, (size_t)NB,
+                       A + (size_t)k0 * lda + k0, lda, nb, nb, (const int*)nullptr, 0);
     e = pivoted_panel(kf);
     if (e != hipSuccess) return e;
     q = kf + 1;
