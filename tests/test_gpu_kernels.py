@@ -36,10 +36,13 @@ def test_gemm_mfma_layout(dev):
     rng = np.random.default_rng(0)
     for dt, tol in ((torch.float64, 1e-13), (torch.float32, 1e-5)):
         # edge tiles, whole tiles, several tiles and K chunks, K not a multiple of the chunk,
-        # K = 0, beta = 0, and operands that are strided views (element-wise path)
+        # K = 0, beta = 0, and operands that are strided views (element-wise path); products
+        # of fewer than 512 128-tiles run on 64 x 64 tiles
         for (M, N, K, beta, view) in ((37, 53, 19, 1.0, False), (128, 128, 128, 1.0, False), (200, 300, 128, 1.0, False),
                                       (129, 257, 64, 1.0, False), (256, 384, 96, 1.0, False), (384, 256, 160, 0.0, False),
-                                      (256, 256, 0, 1.0, False), (256, 128, 128, 1.0, True)):
+                                      (256, 256, 0, 1.0, False), (256, 128, 128, 1.0, True),
+                                      # 512 or more 128 x 128 tiles: the large-tile kernel
+                                      (2048, 4096, 128, 1.0, False), (2100, 3970, 160, 0.0, False)):
             A = rng.standard_normal((M, K))
             Bm = rng.standard_normal((K, N))
             C0 = rng.standard_normal((M, N))

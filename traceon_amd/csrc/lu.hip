@@ -435,15 +435,16 @@ __global__ __launch_bounds__(256) void k_copy_back_max(const T* __restrict__ S, 
 
 // ---------------------------------------------------------------------------------
 // GEMM on MFMA: C(MxN) = beta*C + alpha*A(MxK)*B(KxN), all row-major.
-// Block 256 threads = 2x2 waves, block tile 128x128, wave tile 64x64 made of 4x4
-// MFMA 16x16 tiles; K staged through LDS in chunks of BK.  Instruction per element type:
+// Block 256 threads = 2x2 waves, block tile GB x GB (128; 64 for products too small to
+// fill the chip with 128-tiles), wave tile GB/2 x GB/2 made of MFMA 16x16 tiles; K staged
+// through LDS in chunks of BK.  Instruction per element type:
 //   double: v_mfma_f64_16x16x4f64, float: v_mfma_f32_16x16x4f32
 // Operand layout of both (16x16x4, one block): lane l holds A[l%16][l/16] and
 // B[l/16][l%16]; the four accumulator values v = 0..3 of lane l are D[row][l%16] with
 // row = l/16 + 4v for the f64 instruction and row = 4(l/16) + v for the f32 one.
 //
-// LDS: A as [row][k] with a row stride of 36 words, B as [k][col] with 144 words (in
-// units of 4 bytes; 18 / 144 elements for double).  The 16 rows x 4 k an MFMA reads from
+// LDS: A as [row][k] with a row stride of 36 words, B as [k][col] with GB + 16 words (in
+// units of 4 bytes; 18 and GB + 8 elements for double).  The 16 rows x 4 k an MFMA reads from
 // A and the 4 k x 16 cols it reads from B then fall in distinct banks, and both tiles are
 // written with one 16-byte store per global vector load.
 // ---------------------------------------------------------------------------------
@@ -462,7 +463,6 @@ template <> struct MfmaAcc<float> {
   static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
-constexpr int GBM = 128, GBN = 128;
 
 // Software pipeline: the global loads of K-chunk kt+1 are issued into registers before the
 // MFMA loop of chunk kt and written to the other LDS buffer after it; within a chunk the
@@ -475,37 +475,39 @@ constexpr int GBM = 128, GBN = 128;
 // accesses with bounds checks.
 // TRACK_MAX: also the largest |C| written (NaN counts as large), folded into *vmax as the
 // bit pattern of a nonnegative double (one atomic per workgroup).
-template <typename T, int BK, bool TRACK_MAX>
+template <typename T, int BK, bool TRACK_MAX, int GB>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, const T* __restrict__ A, size_t lda,
                                               const T* __restrict__ B, size_t ldb, T beta, T* __restrict__ C, size_t ldc,
                                               const int* gate, int gp, unsigned long long* __restrict__ vmax) {
   typedef typename MfmaAcc<T>::type acc_t;
   if (gated_off(gate, gp)) return;
+  constexpr int GBM = GB, GBN = GB;                       // block tile
+  constexpr int MT = GB / 32;                             // 16 x 16 MFMA tiles per wave and dimension
   constexpr int VEC = 16 / sizeof(T);                     // elements per 16-byte load
   constexpr int A_LD = (GBM * BK) / (256 * VEC);          // vector loads per thread for A
   constexpr int B_LD = (BK * GBN) / (256 * VEC);
   constexpr int SA = BK + 16 / sizeof(T);                 // 36 words (float), 18 doubles
-  constexpr int SB = GBN + 64 / sizeof(T);                // 144 words (float), 136 doubles
+  constexpr int SB = GBN + 64 / sizeof(T);                // 144 / 80 words (float), 136 / 72 doubles
   constexpr int KS = BK / 4;                              // MFMA steps per chunk
   constexpr int SC = GBN + 4;                             // row stride of the staged C tile
   constexpr size_t AB_BYTES = sizeof(T) * (2 * GBM * SA + 2 * BK * SB);
-  constexpr int CROWS = (int)((AB_BYTES / (sizeof(T) * SC)) >= GBM ? GBM : 64);   // tile rows staged per pass
+  constexpr int CROWS = (int)((AB_BYTES / (sizeof(T) * SC)) >= GBM ? GBM : GBM / 2);   // tile rows staged per pass
   static_assert(CROWS * SC * sizeof(T) <= AB_BYTES, "staged C tile exceeds the LDS of the operands");
   __shared__ __attribute__((aligned(16))) char smem[AB_BYTES];
   T (*As)[GBM][SA] = reinterpret_cast<T (*)[GBM][SA]>(smem);
   T (*Bs)[BK][SB] = reinterpret_cast<T (*)[BK][SB]>(smem + sizeof(T) * 2 * GBM * SA);
   T* Cs = reinterpret_cast<T*>(smem);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+  const int wm = (wave >> 1) * (GB / 2), wn = (wave & 1) * (GB / 2);
   const int bm = blockIdx.y * GBM, bn = blockIdx.x * GBN;
   const bool full = ((lda % VEC) == 0) && ((((uintptr_t)A) % 16) == 0) && ((ldb % VEC) == 0) && ((((uintptr_t)B) % 16) == 0) &&
                     bm + GBM <= M && bn + GBN <= N && (K % BK) == 0;
 
-  acc_t acc[4][4];
+  acc_t acc[MT][MT];
 #pragma unroll
-  for (int a = 0; a < 4; a++)
+  for (int a = 0; a < MT; a++)
 #pragma unroll
-    for (int b = 0; b < 4; b++) acc[a][b] = acc_t{0, 0, 0, 0};
+    for (int b = 0; b < MT; b++) acc[a][b] = acc_t{0, 0, 0, 0};
 
   typedef T vec_t __attribute__((ext_vector_type(VEC)));
   vec_t ra[A_LD], rb[B_LD];
@@ -562,24 +564,24 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, cons
   for (int kt = 0; kt < nk; kt++) {
     const int cur = kt & 1;
     if (kt + 1 < nk) fetch((kt + 1) * BK);
-    T af[2][4], bf[2][4];
+    T af[2][MT], bf[2][MT];
 #pragma unroll
-    for (int a = 0; a < 4; a++) af[0][a] = As[cur][wm + a * 16 + ci][kq];
+    for (int a = 0; a < MT; a++) af[0][a] = As[cur][wm + a * 16 + ci][kq];
 #pragma unroll
-    for (int b = 0; b < 4; b++) bf[0][b] = Bs[cur][kq][wn + b * 16 + ci];
+    for (int b = 0; b < MT; b++) bf[0][b] = Bs[cur][kq][wn + b * 16 + ci];
 #pragma unroll
     for (int s = 0; s < KS; s++) {
       const int p = s & 1;
       if (s + 1 < KS) {
 #pragma unroll
-        for (int a = 0; a < 4; a++) af[p ^ 1][a] = As[cur][wm + a * 16 + ci][4 * (s + 1) + kq];
+        for (int a = 0; a < MT; a++) af[p ^ 1][a] = As[cur][wm + a * 16 + ci][4 * (s + 1) + kq];
 #pragma unroll
-        for (int b = 0; b < 4; b++) bf[p ^ 1][b] = Bs[cur][4 * (s + 1) + kq][wn + b * 16 + ci];
+        for (int b = 0; b < MT; b++) bf[p ^ 1][b] = Bs[cur][4 * (s + 1) + kq][wn + b * 16 + ci];
       }
 #pragma unroll
-      for (int a = 0; a < 4; a++)
+      for (int a = 0; a < MT; a++)
 #pragma unroll
-        for (int b = 0; b < 4; b++) acc[a][b] = MfmaAcc<T>::mma(af[p][a], bf[p][b], acc[a][b]);
+        for (int b = 0; b < MT; b++) acc[a][b] = MfmaAcc<T>::mma(af[p][a], bf[p][b], acc[a][b]);
     }
     if (kt + 1 < nk) stash(cur ^ 1);
     __syncthreads();
@@ -603,10 +605,10 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, cons
     const int r0 = pass * CROWS;
     if (pass > 0) __syncthreads();
 #pragma unroll
-    for (int a = 0; a < 4; a++) {
+    for (int a = 0; a < MT; a++) {
       if (wm + a * 16 < r0 || wm + a * 16 >= r0 + CROWS) continue;
 #pragma unroll
-      for (int b = 0; b < 4; b++)
+      for (int b = 0; b < MT; b++)
 #pragma unroll
         for (int v = 0; v < 4; v++)
           Cs[(wm + a * 16 + MfmaAcc<T>::row(lane, v) - r0) * SC + wn + b * 16 + ci] = alpha * acc[a][b][v];
@@ -979,13 +981,24 @@ template <typename T>
 hipError_t gemm(int M, int N, int K, T alpha, const T* A, size_t lda, const T* B, size_t ldb, T beta, T* C, size_t ldc,
                 hipStream_t st, const int* gate, int gp, unsigned long long* vmax) {
   if (M <= 0 || N <= 0) return hipSuccess;
-  dim3 grid(cdiv(N, GBN), cdiv(M, GBM));
   constexpr int BK = sizeof(T) == 8 ? 16 : 32;
-  if (vmax)
-    hipLaunchKernelGGL((k_gemm<T, BK, true>), grid, dim3(256), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, gp, vmax);
-  else
-    hipLaunchKernelGGL((k_gemm<T, BK, false>), grid, dim3(256), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, gp,
-                       (unsigned long long*)nullptr);
+  // fewer than two 128 x 128 tiles per CU (the skinny products of a panel): 64 x 64 tiles,
+  // four times as many workgroups
+  const bool small = (size_t)cdiv(M, 128) * cdiv(N, 128) < 512;
+  unsigned long long* vm = vmax;
+  if (small) {
+    dim3 grid(cdiv(N, 64), cdiv(M, 64));
+    if (vm)
+      hipLaunchKernelGGL((k_gemm<T, BK, true, 64>), grid, dim3(256), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, gp, vm);
+    else
+      hipLaunchKernelGGL((k_gemm<T, BK, false, 64>), grid, dim3(256), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, gp, vm);
+  } else {
+    dim3 grid(cdiv(N, 128), cdiv(M, 128));
+    if (vm)
+      hipLaunchKernelGGL((k_gemm<T, BK, true, 128>), grid, dim3(256), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, gp, vm);
+    else
+      hipLaunchKernelGGL((k_gemm<T, BK, false, 128>), grid, dim3(256), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, gp, vm);
+  }
   return hipGetLastError();
 }
 template hipError_t gemm<double>(int, int, int, double, const double*, size_t, const double*, size_t, double, double*, size_t,
