@@ -473,8 +473,9 @@ template <> struct MfmaAcc<float> {
 // memory latencies instead of one per element.  Tiles that lie wholly inside the matrices
 // (with 16-byte aligned rows) take unguarded vector accesses; edge tiles take element
 // accesses with bounds checks.
-// TRACK_MAX: also the largest |C| written (NaN counts as large), folded into *vmax as the
-// bit pattern of a nonnegative double (one atomic per workgroup).
+// TRACK_MAX: also the largest |C| written by each workgroup (NaN counts as large), stored
+// as the bit pattern of a nonnegative double in vmax[workgroup] (no atomics: the slots are
+// reduced by the consumer).
 template <typename T, int BK, bool TRACK_MAX, int GB>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, const T* __restrict__ A, size_t lda,
                                               const T* __restrict__ B, size_t ldb, T beta, T* __restrict__ C, size_t ldc,
@@ -674,7 +675,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, cons
       double mm = 0.0;
       for (int w = 0; w < 4; w++) mm = (wmx[w] > mm || wmx[w] != wmx[w]) ? wmx[w] : mm;
       if (mm != mm) mm = 1e300;
-      if (mm > 0.0) atomicMax(vmax, (unsigned long long)__double_as_longlong(mm));
+      vmax[blockIdx.y * gridDim.x + blockIdx.x] = (unsigned long long)__double_as_longlong(mm);
     }
   }
 }
@@ -734,10 +735,26 @@ __global__ __launch_bounds__(256) void k_zero_u64(unsigned long long* __restrict
   if (i < n) p[i] = 0ull;
 }
 
-// panel p of the speculative pass: record it as failed when some |l| > 1 (or NaN)
-__global__ void k_panel_check(const unsigned long long* __restrict__ lmax, int* __restrict__ first_fail, int p) {
+// panel p of the speculative pass: max |l| of the diagonal block (*lmax) and of the
+// workgroups of the L21 product (slots[0 .. nslot)); record p as failed when some |l| > 1
+// (or NaN).  Bit patterns of nonnegative doubles order as integers.
+__global__ __launch_bounds__(256) void k_panel_check(unsigned long long* __restrict__ lmax,
+                                                     const unsigned long long* __restrict__ slots, int nslot,
+                                                     int* __restrict__ first_fail, int p) {
+  __shared__ unsigned long long red[256];
   if (*first_fail <= p) return;
-  if (*lmax > 0x3FF0000000000000ull) atomicMin(first_fail, p);
+  unsigned long long m = threadIdx.x == 0 ? *lmax : 0ull;
+  for (int i = threadIdx.x; i < nslot; i += 256) m = slots[i] > m ? slots[i] : m;
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o && red[threadIdx.x + o] > red[threadIdx.x]) red[threadIdx.x] = red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    *lmax = red[0];
+    if (red[0] > 0x3FF0000000000000ull) atomicMin(first_fail, p);
+  }
 }
 
 // Blocked triangular solve with the stored diagonal-block inverses, one workgroup of 512
@@ -977,14 +994,20 @@ __global__ void k_axpy(double* __restrict__ X, const double* __restrict__ D, siz
 // ---------------------------------------------------------------------------------
 static inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
+// fewer than two 128 x 128 tiles per CU (the skinny products of a panel): 64 x 64 tiles,
+// four times as many workgroups
+static inline bool gemm_small(int M, int N) { return (size_t)cdiv(M, 128) * cdiv(N, 128) < 512; }
+size_t gemm_tiles(int M, int N) {
+  const int t = gemm_small(M, N) ? 64 : 128;
+  return (size_t)cdiv(M, t) * cdiv(N, t);
+}
+
 template <typename T>
 hipError_t gemm(int M, int N, int K, T alpha, const T* A, size_t lda, const T* B, size_t ldb, T beta, T* C, size_t ldc,
                 hipStream_t st, const int* gate, int gp, unsigned long long* vmax) {
   if (M <= 0 || N <= 0) return hipSuccess;
   constexpr int BK = sizeof(T) == 8 ? 16 : 32;
-  // fewer than two 128 x 128 tiles per CU (the skinny products of a panel): 64 x 64 tiles,
-  // four times as many workgroups
-  const bool small = (size_t)cdiv(M, 128) * cdiv(N, 128) < 512;
+  const bool small = gemm_small(M, N);
   unsigned long long* vm = vmax;
   if (small) {
     dim3 grid(cdiv(N, 64), cdiv(M, 64));
@@ -1022,6 +1045,7 @@ struct LuWorkspace {
   T* Dbak;                                        // the current diagonal block before its factorisation (NB x NB)
   unsigned long long* lmaxv;                      // max |l| per panel of the speculative pass (nblk)
   int* first_fail;                                // first panel of that pass that failed the no-pivot test
+  unsigned long long* lslots;                     // max |l| per workgroup of a panel's L21 product
   unsigned long long* lmax;
   size_t bytes;
   LuWorkspace(void* base, int N) {
@@ -1038,6 +1062,7 @@ struct LuWorkspace {
     size_t a_yt = take((size_t)N * sizeof(double));
     size_t a_lt = take(2 * (size_t)N * NB * sizeof(T)), a_lv = take((size_t)nblk * 8), a_ff = take(sizeof(int));
     size_t a_db = take((size_t)NB * NB * sizeof(T));
+    size_t a_ls = take(((size_t)N / 32 + 64) * 8);
     bytes = o;
     if (!base) return;
     cand_val = (double*)(p + a_cv); cand_idx = (int*)(p + a_ci); cand_row = (T*)(p + a_cr);
@@ -1046,6 +1071,7 @@ struct LuWorkspace {
     err = (int*)(p + a_er); pivoted = (int*)(p + a_pv); rhs = (double*)(p + a_rhs); lmax = (unsigned long long*)(p + a_lm);
     ytmp = (double*)(p + a_yt);
     Ltmp = (T*)(p + a_lt); lmaxv = (unsigned long long*)(p + a_lv); first_fail = (int*)(p + a_ff); Dbak = (T*)(p + a_db);
+    lslots = (unsigned long long*)(p + a_ls);
   }
 };
 
@@ -1188,12 +1214,14 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
       // P(q)
       hipLaunchKernelGGL(k_diag_block<T>, dim3(1), dim3(1024), diag_lds_bytes<T>(), sa, A, lda, k0, nb, 1, ipiv, Yp(q), Zp(q),
                          W.lmaxv + q, g, q, W.Dbak);
+      int nslot = 0;
       if (m2 > 0) {
         const T* A21 = A + (size_t)k1 * lda + k0;
-        if ((r = timed_gemm(sa, g, q, m2, nb, nb, T(1), A21, lda, Zp(q), NB, T(0), Lbuf(q), (size_t)nb, W.lmaxv + q)) != hipSuccess)
+        nslot = (int)gemm_tiles(m2, nb);
+        if ((r = timed_gemm(sa, g, q, m2, nb, nb, T(1), A21, lda, Zp(q), NB, T(0), Lbuf(q), (size_t)nb, W.lslots)) != hipSuccess)
           return r;
       }
-      hipLaunchKernelGGL(k_panel_check, dim3(1), dim3(1), 0, sa, W.lmaxv + q, W.first_fail, q);
+      hipLaunchKernelGGL(k_panel_check, dim3(1), dim3(256), 0, sa, W.lmaxv + q, W.lslots, nslot, W.first_fail, q);
       if (n2 <= 0) break;
       const int nb1 = min(NB, n2);
       const T* A12 = A + (size_t)k0 * lda + k1;

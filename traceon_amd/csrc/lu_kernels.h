@@ -8,10 +8,11 @@ namespace tb {
 constexpr int LU_NB = 128;   // panel width of the blocked factorisation
 
 // gate (device int, may be null): the launch does nothing when *gate <= gp (see lu.hip);
-// vmax (may be null): max |C| written is folded into *vmax (bit pattern of a double)
+// vmax (may be null): max |C| of each workgroup into vmax[0 .. gemm_tiles(M, N)) (bit patterns of doubles)
 template <typename T>
 hipError_t gemm(int M, int N, int K, T alpha, const T* A, size_t lda, const T* B, size_t ldb, T beta, T* C, size_t ldc,
                 hipStream_t st, const int* gate = nullptr, int gp = 0, unsigned long long* vmax = nullptr);
+size_t gemm_tiles(int M, int N);   // workgroups of one gemm launch (the slots vmax needs)
 template <typename T> size_t lu_workspace_bytes(int N);
 // Launch count, summed HIP-event duration, algorithmic FLOP (2 M N K per product) and
 // algorithmic HBM bytes (A and B read once, C read and written) of the MFMA GEMM launches
