@@ -939,7 +939,7 @@ __global__ __launch_bounds__(TRSV_THREADS) void k_trsv_blocked(const T* __restri
 // is split exactly with an FMA (a*x = p + e) and summed with TwoSum, so the residual of
 // an already accurate x is itself accurate and refinement converges to the correctly
 // rounded solution of the assembled system rather than stalling at cond * eps.
-// One wave per row; rmax[r] = bit pattern of max |r| (nonnegative doubles order as integers).
+// rmax[r] = bit pattern of max |r| (nonnegative doubles order as integers).
 __device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
   s = a + b;
   double bb = s - a;
@@ -956,31 +956,56 @@ __device__ __forceinline__ void dd_add(double& hi, double& lo, double a, double 
 __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ A, size_t lda, int N, const double* __restrict__ B,
                                                   const double* __restrict__ X, double* __restrict__ Rv, int nrhs,
                                                   unsigned long long* __restrict__ rmax) {
-  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (row >= N) return;
-  const double* a = A + (size_t)row * lda;
+  // one wave per row, four independent double-double partial sums per lane (four loads in
+  // flight); one atomic per workgroup and right-hand side for the maximum
+  __shared__ double wm[4];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int row = blockIdx.x * 4 + wave;
+  const double* a = A + (size_t)(row < N ? row : 0) * lda;
   for (int r = 0; r < nrhs; r++) {
     const double* x = X + (size_t)r * N;
-    double hi = 0.0, lo = 0.0;
-    for (int s = lane; s < N; s += 64) {
-      const double p = a[s] * x[s];
-      const double e = fma(a[s], x[s], -p);
-      dd_add(hi, lo, p, e);
+    double av = 0.0;
+    if (row < N) {
+      double hi[4] = {0.0, 0.0, 0.0, 0.0}, lo[4] = {0.0, 0.0, 0.0, 0.0};
+      int s = lane;
+      for (; s + 192 < N; s += 256) {
+        double pa[4], px[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) { pa[u] = a[s + 64 * u]; px[u] = x[s + 64 * u]; }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const double pr = pa[u] * px[u];
+          dd_add(hi[u], lo[u], pr, fma(pa[u], px[u], -pr));
+        }
+      }
+      for (; s < N; s += 64) {
+        const double pr = a[s] * x[s];
+        dd_add(hi[0], lo[0], pr, fma(a[s], x[s], -pr));
+      }
+#pragma unroll
+      for (int u = 1; u < 4; u++) dd_add(hi[0], lo[0], hi[u], lo[u]);
+      double h = hi[0], l = lo[0];
+      for (int o = 32; o >= 1; o >>= 1) {
+        const double h2 = __shfl_xor(h, o, 64), l2 = __shfl_xor(l, o, 64);
+        dd_add(h, l, h2, l2);
+      }
+      if (lane == 0) {
+        // b - (hi + lo)
+        double sv, e;
+        two_sum(B[(size_t)r * N + row], -h, sv, e);
+        const double v = sv + (e - l);
+        Rv[(size_t)r * N + row] = v;
+        av = v < 0 ? -v : v;
+      }
     }
-    for (int o = 32; o >= 1; o >>= 1) {
-      const double h2 = __shfl_xor(hi, o, 64), l2 = __shfl_xor(lo, o, 64);
-      dd_add(hi, lo, h2, l2);
+    if (lane == 0) wm[wave] = av;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double m = wm[0];
+      for (int w = 1; w < 4; w++) m = (wm[w] > m || wm[w] != wm[w]) ? wm[w] : m;
+      atomicMax(rmax + r, (unsigned long long)__double_as_longlong(m));
     }
-    if (lane == 0) {
-      // b - (hi + lo)
-      double s, e;
-      two_sum(B[(size_t)r * N + row], -hi, s, e);
-      const double v = s + (e - lo);
-      Rv[(size_t)r * N + row] = v;
-      const double av = v < 0 ? -v : v;
-      atomicMax(rmax + r, (unsigned long long)__double_as_longlong(av));
-    }
+    __syncthreads();
   }
 }
 
