@@ -205,13 +205,17 @@ typedef struct {
   double bounds[6];
   tb_sources_t elec, mag, cur;
   const double *zs; const double *ce; const double *cm;
-  int nz, pad2;
+  int nz, axial_parts;   /* bit 0: electrostatic coefficients present, bit 1: magnetostatic;
+                            0 = both.  An absent series is not evaluated (its field is zero). */
   double uE[3], uH[3];
   /* learned correction of an axial component (null: none): FP32 weights W1 b1 W2^T b2 W3^T b3
      W4^T b4 W5 b5, z-scale table (5, mlp_nz), geometry {r_lo, r_max, z_min, z_max, power, 0};
      outside the trust region the component's ring sources give the field */
   const float *mlp; const double *mlp_zs; int mlp_nz, pad3; double mlp_geom[6];
 } tb_field_comp_t;
+/* flags: 1 = some component carries a learned correction; 2 = no component has ring sources
+   (axial and uniform fields only: the tracer runs its variant without source sums, and a BEM
+   component in such a program evaluates to NaN) */
 typedef struct { const tb_field_comp_t *comp; int ncomp, bounds_mode; int flags, pad; double bounds[6]; } tb_field_program_t;
 typedef struct {
   tb_field_program_t field;

@@ -393,7 +393,8 @@ size_t trace_particle(double* times, double* states, double qm, field_fun field,
   if (!s0.good || !traj.good || !tt.good || !fin.good || !counter.good) return 0;
   hipMemset(counter.p, 0, sizeof(int));
   tb::TraceParams P{};
-  P.field = tb::FieldProgram{hp.dcomp->p, 1, tb::BOUNDS_NONE, 0, 0, {0, 0, 0, 0, 0, 0}};
+  const long long nsrc = (long long)hp.comp.elec.n + hp.comp.mag.n + hp.comp.cur.n;
+  P.field = tb::FieldProgram{hp.dcomp->p, 1, tb::BOUNDS_NONE, nsrc > 0 ? 0 : (int)tb::PROGRAM_NO_SOURCES, 0, {0, 0, 0, 0, 0, 0}};
   P.state0 = s0.p; P.t0 = t0.p; P.h0 = nullptr; P.qm = dq.p; P.qm_stride = 0; P.n = 1; P.counter = counter.p;
   for (int i = 0; i < 6; i++) P.bounds[i] = bounds[i];
   P.atol = atol;
@@ -401,7 +402,6 @@ size_t trace_particle(double* times, double* states, double qm, field_fun field,
   P.plane_n[2] = 1.0; P.plane_nn = 1.0;
   P.final_state = fin.p; P.landing = nullptr; P.status = status.p; P.n_accepted = nacc.p; P.n_attempted = natt.p;
   P.t_final = tf.p; P.h_final = hf.p; P.traj = traj.p; P.times = tt.p; P.traj_cap = cap; P.traj_len = tlen.p;
-  const long long nsrc = (long long)hp.comp.elec.n + hp.comp.mag.n + hp.comp.cur.n;
   if (nsrc > 0) {
     // one particle: spread each stage's source sums over the chip
     const int nchunk = tb::split_chunks(1, nsrc);

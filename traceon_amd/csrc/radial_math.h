@@ -298,6 +298,8 @@ TB_HD void current_ring_axial_derivs(double z0, double r, double z, double d[NDE
 // ---------------------------------------------------------------------------------
 // Paraxial series evaluation (radial.c:313-384).  coeffs (Nz-1, 9, 6) highest power first.
 // ---------------------------------------------------------------------------------
+// d[l] for l = L0 .. NDERIV-1 (the field needs no d[0])
+template <int L0 = 0>
 TB_HD bool axial_local(const double* zs, int Nz, const double* coeffs, double z, double d[NDERIV]) {
   double z0 = zs[0], zl = zs[Nz - 1];
   if (!(z0 <= z && z <= zl)) return false;
@@ -308,7 +310,7 @@ TB_HD bool axial_local(const double* zs, int Nz, const double* coeffs, double z,
   double x = z - zs[idx];
   const double* C = coeffs + (size_t)idx * (NDERIV * 6);
 #pragma unroll
-  for (int l = 0; l < NDERIV; l++) {
+  for (int l = L0; l < NDERIV; l++) {
     const double* c = C + 6 * l;
     d[l] = fma(fma(fma(fma(fma(c[0], x, c[1]), x, c[2]), x, c[3]), x, c[4]), x, c[5]);
   }
@@ -324,7 +326,7 @@ TB_HD double axial_potential(const double* zs, int Nz, const double* coeffs, dou
 
 TB_HD void axial_field(const double* zs, int Nz, const double* coeffs, double x, double y, double z, double out[3]) {
   double d[NDERIV];
-  if (!axial_local(zs, Nz, coeffs, z, d)) { out[0] = out[1] = out[2] = 0.0; return; }
+  if (!axial_local<1>(zs, Nz, coeffs, z, d)) { out[0] = out[1] = out[2] = 0.0; return; }
   double r2 = x * x + y * y;
   double fr = 0.5 * (d[2] - r2 / 8.0 * d[4] + r2 * r2 / 192.0 * d[6] - r2 * r2 * r2 / 9216.0 * d[8]);
   double fz = -d[1] + r2 / 4.0 * d[3] - r2 * r2 / 64.0 * d[5] + r2 * r2 * r2 / 2304.0 * d[7];

@@ -177,7 +177,9 @@ __device__ void mlp_correct(const FieldComp& c, const double q[3], double e[3], 
 // MLP = true (one-wave workgroups) applies learned corrections of axial components inside
 // their trust region and their ring sources outside it; MLP = false evaluates such a
 // component as its plain axial field (programs with corrections are traced with MLP = true).
-template <bool MLP = false>
+// SOURCES = false compiles no source sums (programs flagged PROGRAM_NO_SOURCES); a BEM
+// component met there gives NaN.
+template <bool MLP = false, bool SOURCES = true>
 __device__ void eval_field(const FieldProgram& F, const double p[3], double E[3], double H[3], int s0 = 0, int ds = 1,
                            bool lead = true, int lane = 0) {
   E[0] = E[1] = E[2] = 0.0;
@@ -190,7 +192,8 @@ __device__ void eval_field(const FieldProgram& F, const double p[3], double E[3]
     if (!in_bounds(c.bounds, c.bounds_mode, q)) continue;
     double e[3] = {0, 0, 0}, h[3] = {0, 0, 0};
     if (c.kind == COMP_BEM) {
-      bem_component(c, q, e, h, s0, ds);
+      if constexpr (SOURCES) bem_component(c, q, e, h, s0, ds);
+      else e[0] = e[1] = e[2] = h[0] = h[1] = h[2] = __builtin_nan("");
     } else if (c.kind == COMP_AXIAL) {
       bool use_bem = false;
       if (MLP && c.mlp) {
@@ -203,8 +206,9 @@ __device__ void eval_field(const FieldProgram& F, const double p[3], double E[3]
       if (MLP && use_bem) {
         bem_component(c, q, e, h, s0, ds);
       } else if (lead) {
-        axial_field(c.zs, c.nz, c.ce, q[0], q[1], q[2], e);
-        axial_field(c.zs, c.nz, c.cm, q[0], q[1], q[2], h);
+        const int parts = c.axial_parts ? c.axial_parts : 3;
+        if (parts & 1) axial_field(c.zs, c.nz, c.ce, q[0], q[1], q[2], e);
+        if (parts & 2) axial_field(c.zs, c.nz, c.cm, q[0], q[1], q[2], h);
         if (MLP && c.mlp) mlp_correct(c, q, e, h, lane);
       }
     } else if (lead) {
@@ -229,8 +233,8 @@ __device__ __forceinline__ bool inside_trace_bounds(const double* b, const doubl
   return b[0] <= y[0] && y[0] <= b[1] && b[2] <= y[1] && y[1] <= b[3] && b[4] <= y[2] && y[2] <= b[5];
 }
 
-template <bool MLP>
-__global__ __launch_bounds__(64) void k_trace(TraceParams P) {
+template <bool MLP, bool SOURCES>
+__device__ __forceinline__ void trace_body(const TraceParams& P) {
   const int lane = threadIdx.x & 63;
   // lane state
   int pid = -1;
@@ -316,7 +320,7 @@ __global__ __launch_bounds__(64) void k_trace(TraceParams P) {
       }
       double E[3], H[3];
       if (active) {
-        eval_field<MLP>(P.field, ys, E, H, 0, 1, true, lane);
+        eval_field<MLP, SOURCES>(P.field, ys, E, H, 0, 1, true, lane);
       } else {   // a lane without a particle: no field work (its state is stale or out of range)
         E[0] = E[1] = E[2] = H[0] = H[1] = H[2] = 0.0;
       }
@@ -381,6 +385,14 @@ __global__ __launch_bounds__(64) void k_trace(TraceParams P) {
     if (need) pid = -1;
     next_particle(need);
   }
+}
+
+template <bool MLP>
+__global__ __launch_bounds__(64) void k_trace(TraceParams P) { trace_body<MLP, true>(P); }
+// axial and uniform fields only: no source-sum code, so that two waves fit a SIMD (forcing
+// three costs more in register spills than it gains)
+__global__ __launch_bounds__(64) void k_trace_axial(TraceParams P) {
+  trace_body<false, false>(P);
 }
 
 // ---------------------------------------------------------------------------------
@@ -646,6 +658,7 @@ hipError_t launch_trace(const TraceParams& P, int waves, hipStream_t st) {
   const int need = (P.n + 63) / 64;
   if (nw > need) nw = need;
   if (P.field.flags & PROGRAM_HAS_MLP) hipLaunchKernelGGL(k_trace<true>, dim3(nw), dim3(64), 0, st, P);
+  else if (P.field.flags & PROGRAM_NO_SOURCES) hipLaunchKernelGGL(k_trace_axial, dim3(nw), dim3(64), 0, st, P);
   else hipLaunchKernelGGL(k_trace<false>, dim3(nw), dim3(64), 0, st, P);
   return hipGetLastError();
 }

@@ -32,7 +32,7 @@ struct FieldComp {
   const double* zs;
   const double* ce;
   const double* cm;
-  int nz, pad2;
+  int nz, axial_parts;     // bit 0: electrostatic series present, bit 1: magnetostatic; 0 = both
   double uE[3], uH[3];
   // learned correction of an axial component (null: none).  mlp: FP32 tracer weights
   // W1 (128,2) b1 | W2^T (in,out) b2 | W3^T b3 | W4^T b4 | W5 (5,128) b5; mlp_zs: s_k(z)
@@ -45,11 +45,15 @@ struct FieldComp {
   double mlp_geom[6];
 };
 
-enum { PROGRAM_HAS_MLP = 1 };
+// PROGRAM_HAS_MLP: some component carries a learned correction.  PROGRAM_NO_SOURCES: no
+// component has ring sources (axial and uniform fields only): the tracer then runs a
+// variant without the source-sum code (far fewer registers); a BEM component in such a
+// program evaluates to NaN rather than being skipped.
+enum { PROGRAM_HAS_MLP = 1, PROGRAM_NO_SOURCES = 2 };
 struct FieldProgram {
   const FieldComp* comp;   // device array
   int ncomp, bounds_mode;
-  int flags, pad;          // PROGRAM_HAS_MLP: some component carries a learned correction
+  int flags, pad;          // PROGRAM_HAS_MLP, PROGRAM_NO_SOURCES
   double bounds[6];
 };
 

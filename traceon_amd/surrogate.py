@@ -68,6 +68,7 @@ class FieldSurrogate:
         self.p = float(weight_power)
         self.nz = int(z_bins)
         self.rng = np.random.default_rng(seed)
+        self.seed = int(seed)
         L = _L()
         self.nparam = L.tb_mlp_nparams()
         dev = device.device()
@@ -78,6 +79,7 @@ class FieldSurrogate:
         self.Wbf = torch.empty(3 * 128 * 128, dtype=torch.int16, device=dev)
         backend.check(L.tb_mlp_to_bf16(device.ptr(self.P), device.ptr(self.Wbf), device.stream_ptr()), "mlp_to_bf16")
         self.zscale = np.ones((NOUT, self.nz))   # s_k(z) at the bin centres
+        self.active = np.ones(NOUT, dtype=bool)  # components with a nonzero residual
         self.step = 0
         self.X = self.T = None
         self.history = []
@@ -147,6 +149,9 @@ class FieldSurrogate:
         pad = np.pad(rms, ((1, 1), (0, 0)), mode="edge")
         rms = np.maximum(np.maximum(pad[:-2], pad[1:-1]), pad[2:])
         top = rms.max(axis=0)
+        # a component with no residual at all (e.g. H of a purely electrostatic field) is not
+        # learned: unit scale for its (zero) targets, and its correction is zero
+        self.active = top > 0
         rms = np.where(top[None, :] > 0, np.maximum(rms, 1e-6 * top[None, :]), 1.0)
         self.zscale = rms.T.copy()
         Y = Y / self._zscale(z)
@@ -168,9 +173,11 @@ class FieldSurrogate:
         loss = torch.zeros(1, dtype=torch.float32, device=dev)
         st = device.stream_ptr()
         p = device.ptr
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(self.seed + self.step)   # mini-batch order follows the surrogate's seed
         t0 = time.time()
         for it in range(steps):
-            idx = torch.randint(0, n, (batch,), dtype=torch.int32, device=dev)
+            idx = torch.randint(0, n, (batch,), dtype=torch.int32, device=dev, generator=gen)
             backend.check(L.tb_mlp_gather(batch, p(idx), p(self.X), p(self.T), p(Xb), p(Tb), st), "mlp_gather")
             backend.check(L.tb_mlp_grad(p(self.P), p(self.Wbf), p(Xb), p(Tb), batch, p(self.G), p(loss), p(ws), st), "mlp_grad")
             self.step += 1
@@ -193,7 +200,8 @@ class FieldSurrogate:
         ws = torch.empty(L.tb_mlp_workspace_bytes(M), dtype=torch.uint8, device=r.device)
         backend.check(L.tb_mlp_forward(device.ptr(self.P), device.ptr(self.Wbf), device.ptr(X), M, device.ptr(Y), device.ptr(ws),
                                        device.stream_ptr()), "mlp_forward")
-        return Y.double() * self._zscale(z) * self._weight(r)[:, None]
+        act = torch.tensor(self.active, dtype=torch.float64, device=r.device)
+        return Y.double() * self._zscale(z) * self._weight(r)[:, None] * act[None, :]
 
     def tracer_arrays(self):
         """Device arrays of the tracer's learned-correction component (csrc/trace_kernels.h):
@@ -208,7 +216,7 @@ class FieldSurrogate:
             w = (w << 16).view(torch.float32).view(128, 128)
             P[o:o + 128 * 128] = w.t().contiguous().view(-1)
             o += 128 * 128 + 128
-        zs = torch.tensor(self.zscale, dtype=torch.float64, device=P.device).contiguous()
+        zs = torch.tensor(self.zscale * self.active[:, None], dtype=torch.float64, device=P.device).contiguous()
         geom = [self.r_lo, self.r_max, self.z_min, self.z_max, self.p, 0.0]
         return P.contiguous(), zs, geom
 
@@ -224,8 +232,9 @@ class FieldSurrogate:
         Fc = Fa + self.correction(r, z)
         out = {}
         Fb_, Fa_, Fc_, r_ = Fb.cpu().numpy(), Fa.cpu().numpy(), Fc.cpu().numpy(), r.cpu().numpy() / self.r_max
-        sE = max(np.max(np.abs(Fb_[:, 1:3])), 1e-300)
-        sH = max(np.max(np.abs(Fb_[:, 3:5])), 1e-300)
+        # errors relative to the largest field of the samples (absolute where that field is zero)
+        sE = np.max(np.abs(Fb_[:, 1:3])) or 1.0
+        sH = np.max(np.abs(Fb_[:, 3:5])) or 1.0
         lo = 0.0
         for hi in bins:
             m = (r_ >= lo) & (r_ < hi)

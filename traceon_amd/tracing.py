@@ -55,7 +55,7 @@ class FieldCompT(C.Structure):
     _fields_ = [("kind", C.c_int), ("transform", C.c_int), ("bounds_mode", C.c_int), ("pad", C.c_int), ("factor", C.c_double),
                 ("inv", C.c_double * 12), ("basis", C.c_double * 9), ("bounds", C.c_double * 6),
                 ("elec", SourcesT), ("mag", SourcesT), ("cur", SourcesT),
-                ("zs", C.c_void_p), ("ce", C.c_void_p), ("cm", C.c_void_p), ("nz", C.c_int), ("pad2", C.c_int),
+                ("zs", C.c_void_p), ("ce", C.c_void_p), ("cm", C.c_void_p), ("nz", C.c_int), ("axial_parts", C.c_int),
                 ("uE", C.c_double * 3), ("uH", C.c_double * 3),
                 ("mlp", C.c_void_p), ("mlp_zs", C.c_void_p), ("mlp_nz", C.c_int), ("pad3", C.c_int), ("mlp_geom", C.c_double * 6)]
 
@@ -65,7 +65,7 @@ class FieldProgramT(C.Structure):
                 ("bounds", C.c_double * 6)]
 
 
-PROGRAM_HAS_MLP = 1
+PROGRAM_HAS_MLP, PROGRAM_NO_SOURCES = 1, 2
 
 
 class TraceParamsT(C.Structure):
@@ -125,6 +125,10 @@ def make_component(field, factor: float, kind: str, in_superposition: bool):
     elif kind in ("axial", "learned"):
         z, ce, cm = field.device_arrays()
         c.zs, c.ce, c.cm, c.nz = z.data_ptr(), ce.data_ptr(), cm.data_ptr(), len(field.z)
+        # series that are identically zero (no charges of that kind) are not evaluated
+        c.axial_parts = (1 if np.any(field.electrostatic_coeffs) else 0) | (2 if np.any(field.magnetostatic_coeffs) else 0)
+        if c.axial_parts == 0:
+            c.axial_parts = 3
         keep += [z, ce, cm]
         if kind == "learned":
             # the learned correction, and the ring sources for outside its trust region
@@ -164,7 +168,8 @@ class DeviceFieldProgram:
         self.prog = FieldProgramT()
         self.prog.comp = self.comp_buf.data_ptr()
         self.prog.ncomp = n
-        self.prog.flags = PROGRAM_HAS_MLP if self.has_mlp else 0
+        # a program without ring sources runs the tracer variant without source-sum code
+        self.prog.flags = (PROGRAM_HAS_MLP if self.has_mlp else 0) | (0 if self.has_sources else PROGRAM_NO_SOURCES)
         # the superposition's own bounds (field.py:285-290 on the top-level object)
         if isinstance(field, FieldSuperposition) and field.field_bounds is not None:
             self.prog.bounds_mode = BOUNDS_INCLUSIVE
