@@ -212,6 +212,9 @@ typedef struct {
      W4^T b4 W5 b5, z-scale table (5, mlp_nz), geometry {r_lo, r_max, z_min, z_max, power, 0};
      outside the trust region the component's ring sources give the field */
   const float *mlp; const double *mlp_zs; int mlp_nz, pad3; double mlp_geom[6];
+  /* BF16 hidden weights W2 W3 W4 as trained, (out, in) each (3 x 128 x 128): with them a
+     component traced alone evaluates its correction for a whole wave on MFMA; null: per lane */
+  const uint16_t *mlp_wbf;
 } tb_field_comp_t;
 /* flags: 1 = some component carries a learned correction; 2 = no component has ring sources
    (axial and uniform fields only: the tracer runs its variant without source sums, and a BEM

@@ -106,7 +106,13 @@ __device__ void bem_component(const FieldComp& c, const double q[3], double e[3]
 // layer 5 is folded into layer 4 so that layer 4's output stays FP32.  Adds
 // y_k * w(r) * s_k(z), w = (r/r_max)^p, to (E_r, E_z, H_r, H_z) for r >= r_lo.
 constexpr int TMLP_B1 = 256, TMLP_W2 = 384, TMLP_W3 = TMLP_W2 + 16512, TMLP_W4 = TMLP_W3 + 16512, TMLP_W5 = TMLP_W4 + 16512;
-__shared__ uint16_t g_mlp_act[2 * 128 * 64];   // [layer parity][neuron][lane] of the wave in this workgroup
+// LDS of the learned correction (the workgroups that evaluate corrections are one wave):
+// mlp_correct: g_mlp_act[layer parity][neuron][lane], BF16; mlp_wave: activations A0 and
+// A1 (64 points x 136 BF16, the row stride that keeps the MFMA operand reads conflict-free)
+// and, in place of A1, the FP32 output of the last hidden layer (64 x 129 floats).
+constexpr int MW_S = 136, MW_SF = 129;
+__shared__ __attribute__((aligned(16))) unsigned char g_mlp_lds[64 * MW_S * 2 + 64 * MW_SF * 4];
+#define g_mlp_act (reinterpret_cast<uint16_t*>(g_mlp_lds))
 typedef const __attribute__((address_space(4))) float cfloat_t;   // read-only weights: scalar loads
 
 __device__ __forceinline__ cfloat_t* uniform_weights(const float* p) {
@@ -171,6 +177,99 @@ __device__ void mlp_correct(const FieldComp& c, const double q[3], double e[3], 
   h[0] += d[3] * cx; h[1] += d[3] * cy; h[2] += d[4];
 }
 
+typedef short mw_s4 __attribute__((ext_vector_type(4)));
+typedef float mw_f4 __attribute__((ext_vector_type(4)));
+
+// The learned correction for all 64 lanes of a wave at once (each lane's own point q),
+// the hidden layers on MFMA (v_mfma_f32_16x16x16bf16_1k, the training's layout and BF16
+// rounding points): the input layer per lane in FP32 with BF16 activations, three 64 x 128
+// x 128 products with FP32 accumulation, bias and SiLU, BF16 activations in between, the last
+// SiLU kept in FP32 and the output layer per lane.  Must be called by every lane of the wave
+// (uniform control flow); lanes with need = false compute on a dummy point and add nothing.
+// Operand layout of the instruction: lane l holds A[l%16][4(l/16)+i] (points x inputs),
+// B[4(l/16)+i][l%16] (inputs x outputs: W[out][in] read row-wise) and D[4(l/16)+v][l%16].
+__device__ void mlp_wave(const FieldComp& c, const double q[3], bool need, int lane, double e[3], double h[3]) {
+  uint16_t* A0 = reinterpret_cast<uint16_t*>(g_mlp_lds);
+  uint16_t* A1 = reinterpret_cast<uint16_t*>(g_mlp_lds + 64 * MW_S * 2);
+  float* Fo = reinterpret_cast<float*>(g_mlp_lds + 64 * MW_S * 2);
+  const double r = sqrt(q[0] * q[0] + q[1] * q[1]);
+  const double r_lo = c.mlp_geom[0], r_max = c.mlp_geom[1], z0 = c.mlp_geom[2], z1 = c.mlp_geom[3];
+  const float u = need ? (float)(2.0 * (r - r_lo) / (r_max - r_lo) - 1.0) : 0.f;
+  const float v = need ? (float)(2.0 * (q[2] - z0) / (z1 - z0) - 1.0) : 0.f;
+  cfloat_t* W = uniform_weights(c.mlp);
+  // input layer: this lane's row of A0
+#pragma unroll 4
+  for (int j = 0; j < 128; j += 4) {
+    uint64_t pk = 0;
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      const float z = fmaf(W[2 * (j + t)], u, fmaf(W[2 * (j + t) + 1], v, W[TMLP_B1 + j + t]));
+      pk |= (uint64_t)f2bf(silu(z)) << (16 * t);
+    }
+    *reinterpret_cast<uint64_t*>(&A0[lane * MW_S + j]) = pk;
+  }
+  __syncthreads();
+  const int lr = lane & 15, lq = lane >> 4;
+  for (int layer = 0; layer < 3; layer++) {
+    const uint16_t* Ain = layer == 1 ? A1 : A0;
+    uint16_t* Aout = layer == 0 ? A1 : A0;
+    const uint16_t* Wl = c.mlp_wbf + (size_t)layer * 16384;
+    const float* bias = c.mlp + TMLP_W2 + layer * 16512 + 16384;
+    for (int nb = 0; nb < 8; nb++) {
+      mw_f4 acc[4];
+#pragma unroll
+      for (int mb = 0; mb < 4; mb++) acc[mb] = mw_f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+      for (int kb = 0; kb < 8; kb++) {
+        const mw_s4 bfr = *reinterpret_cast<const mw_s4*>(Wl + (16 * nb + lr) * 128 + 16 * kb + 4 * lq);
+#pragma unroll
+        for (int mb = 0; mb < 4; mb++) {
+          const mw_s4 afr = *reinterpret_cast<const mw_s4*>(Ain + (16 * mb + lr) * MW_S + 16 * kb + 4 * lq);
+          acc[mb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(afr, bfr, acc[mb], 0, 0, 0);
+        }
+      }
+      const int n = 16 * nb + lr;
+      const float bn = bias[n];
+#pragma unroll
+      for (int mb = 0; mb < 4; mb++)
+#pragma unroll
+        for (int vv = 0; vv < 4; vv++) {
+          const int pt = 16 * mb + 4 * lq + vv;
+          const float a = silu(acc[mb][vv] + bn);
+          if (layer < 2) Aout[pt * MW_S + n] = f2bf(a);
+          else Fo[pt * MW_SF + n] = a;
+        }
+    }
+    __syncthreads();
+  }
+  // output layer, per lane on its own point
+  float y[5];
+#pragma unroll
+  for (int o = 0; o < 5; o++) y[o] = W[TMLP_W5 + 640 + o];
+  for (int n = 0; n < 128; n++) {
+    const float hv = Fo[lane * MW_SF + n];
+#pragma unroll
+    for (int o = 0; o < 5; o++) y[o] = fmaf(W[TMLP_W5 + o * 128 + n], hv, y[o]);
+  }
+  __syncthreads();   // the LDS is reused by the next evaluation
+  if (!need) return;
+  // s_k(z): linear between bin centres; w(r)
+  const int nz = c.mlp_nz;
+  double tz = (q[2] - z0) / (z1 - z0) * nz - 0.5;
+  tz = tz < 0.0 ? 0.0 : (tz > nz - 1.0 ? nz - 1.0 : tz);
+  int i0 = (int)tz;
+  if (i0 > nz - 2) i0 = nz > 1 ? nz - 2 : 0;
+  const double f = tz - i0;
+  const int i1 = nz > 1 ? i0 + 1 : 0;
+  const double w = pow(r / r_max, c.mlp_geom[4]);
+  double d[5];
+#pragma unroll
+  for (int k = 1; k < 5; k++) d[k] = (double)y[k] * w * (c.mlp_zs[k * nz + i0] * (1.0 - f) + c.mlp_zs[k * nz + i1] * f);
+  const double cx = r > 0.0 ? q[0] / r : 0.0, cy = r > 0.0 ? q[1] / r : 0.0;
+  e[0] += d[1] * cx; e[1] += d[1] * cy; e[2] += d[2];
+  h[0] += d[3] * cx; h[1] += d[3] * cy; h[2] += d[4];
+}
+
 // Field of a program at p.  Ring-source sums run over sources s0, s0+ds, ...; terms without
 // a source sum (axial, uniform) are added only when `lead` is set, so that partial results
 // of several threads add up to the field and (0, 1, true) gives the whole field in one lane.
@@ -179,11 +278,14 @@ __device__ void mlp_correct(const FieldComp& c, const double q[3], double e[3], 
 // component as its plain axial field (programs with corrections are traced with MLP = true).
 // SOURCES = false compiles no source sums (programs flagged PROGRAM_NO_SOURCES); a BEM
 // component met there gives NaN.
-template <bool MLP = false, bool SOURCES = true>
+// WAVE (with MLP): the correction is left to mlp_wave; *mlp_need reports whether this
+// point gets it (one learned component, untransformed, as the tracer checks).
+template <bool MLP = false, bool SOURCES = true, bool WAVE = false>
 __device__ void eval_field(const FieldProgram& F, const double p[3], double E[3], double H[3], int s0 = 0, int ds = 1,
-                           bool lead = true, int lane = 0) {
+                           bool lead = true, int lane = 0, bool* mlp_need = nullptr) {
   E[0] = E[1] = E[2] = 0.0;
   H[0] = H[1] = H[2] = 0.0;
+  if (WAVE) *mlp_need = false;
   if (!in_bounds(F.bounds, F.bounds_mode, p)) return;
   for (int ci = 0; ci < F.ncomp; ci++) {
     const FieldComp& c = F.comp[ci];
@@ -209,7 +311,14 @@ __device__ void eval_field(const FieldProgram& F, const double p[3], double E[3]
         const int parts = c.axial_parts ? c.axial_parts : 3;
         if (parts & 1) axial_field(c.zs, c.nz, c.ce, q[0], q[1], q[2], e);
         if (parts & 2) axial_field(c.zs, c.nz, c.cm, q[0], q[1], q[2], h);
-        if (MLP && c.mlp) mlp_correct(c, q, e, h, lane);
+        if (MLP && c.mlp) {
+          if constexpr (WAVE) {
+            const double r = sqrt(q[0] * q[0] + q[1] * q[1]);
+            *mlp_need = r >= c.mlp_geom[0] && q[2] >= c.mlp_geom[2] && q[2] <= c.mlp_geom[3];
+          } else {
+            mlp_correct(c, q, e, h, lane);
+          }
+        }
       }
     } else if (lead) {
       for (int i = 0; i < 3; i++) { e[i] = c.uE[i]; h[i] = c.uH[i]; }
@@ -236,6 +345,8 @@ __device__ __forceinline__ bool inside_trace_bounds(const double* b, const doubl
 template <bool MLP, bool SOURCES>
 __device__ __forceinline__ void trace_body(const TraceParams& P) {
   const int lane = threadIdx.x & 63;
+  // one learned, untransformed component with BF16 weights: corrections on MFMA per wave
+  const bool wave_mlp = MLP && P.field.ncomp == 1 && P.field.comp[0].mlp && P.field.comp[0].mlp_wbf && !P.field.comp[0].transform;
   // lane state
   int pid = -1;
   double y[6], h = 0.0, hmax = 0.0, t = 0.0, qm = 0.0;
@@ -319,10 +430,18 @@ __device__ __forceinline__ void trace_body(const TraceParams& P) {
         ys[c] = acc;
       }
       double E[3], H[3];
+      bool need = false;
       if (active) {
-        eval_field<MLP, SOURCES>(P.field, ys, E, H, 0, 1, true, lane);
+        if (MLP && wave_mlp) eval_field<MLP, SOURCES, true>(P.field, ys, E, H, 0, 1, true, lane, &need);
+        else eval_field<MLP, SOURCES>(P.field, ys, E, H, 0, 1, true, lane);
       } else {   // a lane without a particle: no field work (its state is stale or out of range)
         E[0] = E[1] = E[2] = H[0] = H[1] = H[2] = 0.0;
+      }
+      if (MLP && wave_mlp && __any(need)) {   // the learned correction of the whole wave on MFMA
+        const FieldComp& c0 = P.field.comp[0];
+        double ce[3] = {0.0, 0.0, 0.0}, ch[3] = {0.0, 0.0, 0.0};
+        mlp_wave(c0, ys, need, lane, ce, ch);
+        for (int i = 0; i < 3; i++) { E[i] = fma(c0.factor, ce[i], E[i]); H[i] = fma(c0.factor, ch[i], H[i]); }
       }
       const double vxh0 = ys[4] * H[2] - ys[5] * H[1];
       const double vxh1 = ys[5] * H[0] - ys[3] * H[2];
@@ -645,10 +764,28 @@ template <bool MLP>
 __global__ __launch_bounds__(64) void k_eval_program(FieldProgram F, const double* __restrict__ pts, int M, double* __restrict__ E,
                                                      double* __restrict__ H) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= M) return;
-  double p[3] = {pts[3 * (size_t)t], pts[3 * (size_t)t + 1], pts[3 * (size_t)t + 2]};
+  const int lane = threadIdx.x & 63;
+  // as the tracer: one learned, untransformed component with BF16 weights takes the MFMA path,
+  // for which every lane of the wave stays (lanes past M on a dummy point)
+  const bool wave_mlp = MLP && F.ncomp == 1 && F.comp[0].mlp && F.comp[0].mlp_wbf && !F.comp[0].transform;
+  if (!wave_mlp && t >= M) return;
+  const int tt = t < M ? t : M - 1;
+  double p[3] = {pts[3 * (size_t)tt], pts[3 * (size_t)tt + 1], pts[3 * (size_t)tt + 2]};
   double e[3], h[3];
-  eval_field<MLP>(F, p, e, h, 0, 1, true, threadIdx.x & 63);
+  if (MLP && wave_mlp) {
+    bool need = false;
+    eval_field<MLP, true, true>(F, p, e, h, 0, 1, true, lane, &need);
+    need = need && t < M;
+    if (__any(need)) {
+      const FieldComp& c0 = F.comp[0];
+      double ce[3] = {0.0, 0.0, 0.0}, ch[3] = {0.0, 0.0, 0.0};
+      mlp_wave(c0, p, need, lane, ce, ch);
+      for (int i = 0; i < 3; i++) { e[i] = fma(c0.factor, ce[i], e[i]); h[i] = fma(c0.factor, ch[i], h[i]); }
+    }
+    if (t >= M) return;
+  } else {
+    eval_field<MLP>(F, p, e, h, 0, 1, true, lane);
+  }
   for (int i = 0; i < 3; i++) { if (E) E[3 * (size_t)t + i] = e[i]; if (H) H[3 * (size_t)t + i] = h[i]; }
 }
 

@@ -43,6 +43,9 @@ struct FieldComp {
   const double* mlp_zs;
   int mlp_nz, pad3;
   double mlp_geom[6];
+  // BF16 hidden weights W2 W3 W4 as trained, (out, in) each (3 x 128 x 128); null: the
+  // correction runs per lane in FP32 (mlp_correct) instead of on MFMA (mlp_wave)
+  const uint16_t* mlp_wbf;
 };
 
 // PROGRAM_HAS_MLP: some component carries a learned correction.  PROGRAM_NO_SOURCES: no

@@ -57,7 +57,8 @@ class FieldCompT(C.Structure):
                 ("elec", SourcesT), ("mag", SourcesT), ("cur", SourcesT),
                 ("zs", C.c_void_p), ("ce", C.c_void_p), ("cm", C.c_void_p), ("nz", C.c_int), ("axial_parts", C.c_int),
                 ("uE", C.c_double * 3), ("uH", C.c_double * 3),
-                ("mlp", C.c_void_p), ("mlp_zs", C.c_void_p), ("mlp_nz", C.c_int), ("pad3", C.c_int), ("mlp_geom", C.c_double * 6)]
+                ("mlp", C.c_void_p), ("mlp_zs", C.c_void_p), ("mlp_nz", C.c_int), ("pad3", C.c_int), ("mlp_geom", C.c_double * 6),
+                ("mlp_wbf", C.c_void_p)]
 
 
 class FieldProgramT(C.Structure):
@@ -138,7 +139,8 @@ def make_component(field, factor: float, kind: str, in_superposition: bool):
             for i, v in enumerate(geom[:5]):
                 c.mlp_geom[i] = v
             c.mlp_geom[5] = field.trust_radius
-            keep += [wts, zs]
+            c.mlp_wbf = sur.Wbf.data_ptr()
+            keep += [wts, zs, sur.Wbf]
             keep.append(_set_sources(c.elec, sur.field.electrostatic_point_charges))
             keep.append(_set_sources(c.mag, sur.field.magnetostatic_point_charges))
             keep.append(_set_sources(c.cur, sur.field.current_point_charges))
