@@ -36,7 +36,9 @@ __host__ __device__ __forceinline__ float bf2f(uint16_t b) {
   __builtin_memcpy(&x, &u, 4);
   return x;
 }
-__device__ __forceinline__ float sigm(float z) { return 1.0f / (1.0f + __expf(-z)); }
+// logistic function with the hardware exponential and reciprocal (about 1 ulp each; an IEEE
+// division here costs more than the rest of a hidden-layer element in the tracer)
+__device__ __forceinline__ float sigm(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 __device__ __forceinline__ float silu(float z) { return z * sigm(z); }
 __device__ __forceinline__ float dsilu(float z) {
   const float s = sigm(z);
