@@ -163,6 +163,35 @@ def test_lu_random(dev, N):
     assert np.array_equal(lu.ipiv.cpu().numpy(), piv)
 
 
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_lu_pivot_in_a_middle_panel(dev, dtype):
+    """Diagonally dominant except two rows of panel 3 (of 8) swapped: the speculative
+    no-pivot pass accepts panels 0-2, fails at panel 3 (a multiplier > 1), restores that
+    panel's diagonal block, factors it with partial pivoting and resumes.  Pivots equal
+    LAPACK's (one interchange, in panel 3) and the solution is LAPACK's."""
+    import torch
+    import scipy.linalg
+    rng = np.random.default_rng(17)
+    N = 1000
+    A = rng.uniform(-1, 1, (N, N)) + 1.5 * N ** 0.5 * np.eye(N) * 10
+    k0 = 3 * 128
+    A[[k0 + 10, k0 + 20]] = A[[k0 + 20, k0 + 10]]
+    b = rng.standard_normal((1, N))
+    _, piv = scipy.linalg.lu_factor(A)
+    moved = np.nonzero(piv != np.arange(N))[0]
+    assert list(moved) == [k0 + 10]          # the matrix needs exactly that interchange
+    t = torch.float64 if dtype == "f64" else torch.float32
+    lu = dev.lu_factor(torch.tensor(A, device="cuda").to(t), overwrite=True)
+    assert lu.info == 0
+    assert np.array_equal(lu.ipiv.cpu().numpy(), piv)
+    X = torch.tensor(b, device="cuda")
+    dev.lu_solve(lu, X)
+    ref = np.linalg.solve(A, b[0])
+    err = rel(X.cpu().numpy()[0], ref)
+    print("pivot in panel 3", dtype, err, flush=True)
+    assert err < (1e-12 if dtype == "f64" else 1e-5)
+
+
 def test_field_points(dev, geo, oracle):
     import torch
     lines, types, values = geo
