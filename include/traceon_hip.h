@@ -209,8 +209,9 @@ typedef struct {
                             0 = both.  An absent series is not evaluated (its field is zero). */
   double uE[3], uH[3];
   /* learned correction of an axial component (null: none): FP32 weights W1 b1 W2^T b2 W3^T b3
-     W4^T b4 W5 b5, z-scale table (5, mlp_nz), geometry {r_lo, r_max, z_min, z_max, power, 0};
-     outside the trust region the component's ring sources give the field */
+     W4^T b4 W5 b5, z-scale table (5, mlp_nz), geometry {r_lo, r_max, z_min, z_max, power,
+     trust radius}; outside the trust region (r above the trust radius, or z outside the axial
+     range) the component's ring sources give the field */
   const float *mlp; const double *mlp_zs; int mlp_nz, pad3; double mlp_geom[6];
   /* BF16 hidden weights W2 W3 W4 as trained, (out, in) each (3 x 128 x 128): with them a
      component traced alone evaluates its correction for a whole wave on MFMA; null: per lane */
