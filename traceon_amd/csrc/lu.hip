@@ -197,196 +197,348 @@ __global__ __launch_bounds__(256) void k_laswp(T* __restrict__ A, size_t lda, in
 // flux type (dielectric / magnetizable) usually do.
 // ---------------------------------------------------------------------------------
 
-// Diagonal block of a panel in one workgroup of 1024 threads holding the NB x NB block in
-// registers.  Wave w holds 16 consecutive rows (row block rb) and 64 consecutive columns
-// (column half h): thread = (rb, column c), rows 16 rb + k, k = 0..15.  The waves are
-// numbered so that each SIMD holds two row blocks four apart and both column halves, which
-// keeps the work of a step spread over the SIMDs while finished blocks drop out.  Outside
-// nb x nb the block is padded with the identity.
-//   factor = 1: unpivoted LU of the block (fast path).  Multipliers are formed with the
-//     reciprocal of the pivot, as LAPACK's dgetf2 scales its column; max |l| -> lmax.
-//   Then, always: Y = inv(L11) (unit lower) and Z = inv(U11), row-major NB x NB with zero
-//   padding.  L is inverted forwards and U backwards in the same sweep.
-// Every step publishes one row and one column through LDS, double-buffered by step
-// parity, so a step costs one barrier.  A wave whose rows or columns a step cannot change
-// skips the step's arithmetic with one wave-uniform branch (the work of a step is then
-// proportional to the trailing block).  The column a thread needs (its 16 rows) is stored
-// contiguously: four (FP32) or eight (FP64) 16-byte broadcast reads.  The factored block
-// is staged column by column in the same order for the inverse sweep.  The kernel runs
-// beside the trailing GEMM of the previous panel, so it is kept near 72 VGPRs (FP32): 16
-// waves of it and one GEMM workgroup then fit a CU's register file together.
+// Diagonal block of a panel in one workgroup of 256 threads (4 waves) with the NB x NB block
+// in LDS, row-major with a padded row stride.  Outside nb x nb the block is padded with the
+// identity.
+//   factor = 1: unpivoted LU of the block, right-looking in 16-column sub-panels s:
+//     1. the sub-panel (rows 16s..NB-1 of its 16 columns) is factored in registers, one row
+//        per lane: every wave holds the sub-panel's 16 diagonal rows in lanes 0-15 and up to
+//        48 of the rows below them in lanes 16-63, and eliminates them redundantly, the
+//        pivot row reaching its lanes by readlane, so the 16 steps need no barrier.
+//        Multipliers are formed with the reciprocal of the pivot, as LAPACK's dgetf2 scales
+//        its column; max |l| -> lmax;
+//     2. the block row right of it is solved with the sub-panel's unit-lower 16 x 16
+//        diagonal block, one thread per column (column-oriented, so the 15 dependent steps
+//        each carry up to 15 independent FMAs);
+//     3. the trailing block is updated with the rank-16 product in 4 x 4 register tiles.
+//   Then, always: Y = inv(L11) (unit lower) and Z = inv(U11), in place (the strictly lower
+//   part of the block becomes Y's, the upper part Z's).  The 16 x 16 diagonal blocks of both
+//   are inverted by 16-lane groups, one column per lane; then three doubling levels
+//   b = 16, 32, 64 form the off-diagonal blocks of the 2b x 2b inverses,
+//     Y21 = -Y22 (L21 Y11),   Z12 = -Z11 (U12 Z22),
+//   each product in 4 x 4 register tiles (16-byte LDS reads over runs of 4 k; only the 4 x 4
+//   block on a triangle's diagonal is masked) and written back over the block it replaces.
+//   Y and Z are written row-major NB x NB with zero padding.
+// Work: 2/3 NB^3 FMAs for the LU and as many for the inverses, about 40 barriers in all.
 template <typename T>
-constexpr int diag_col_stride() { return LU_NB + 16 / (int)sizeof(T); }
+constexpr int diag_row_stride() { return LU_NB + 16 / (int)sizeof(T); }
 template <typename T>
-constexpr size_t diag_lds_bytes() { return sizeof(T) * LU_NB * diag_col_stride<T>(); }
+constexpr size_t diag_lds_bytes() { return sizeof(T) * LU_NB * diag_row_stride<T>(); }
+
+__device__ __forceinline__ float lane_bcast(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
+__device__ __forceinline__ double lane_bcast(double v, int l) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffull), l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+// reciprocal of a pivot: FP32 by the hardware reciprocal (1 ulp; the FP32 factors only
+// precondition the FP64 refinement), FP64 correctly rounded
+__device__ __forceinline__ float pivot_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double pivot_rcp(double x) { return 1.0 / x; }
+
+// four consecutive elements of an LDS row (16-byte aligned start)
+template <typename T>
+__device__ __forceinline__ void lds_load4(const T* p, T (&v)[4]) {
+  if constexpr (sizeof(T) == 4) {
+    typedef float f4v __attribute__((ext_vector_type(4)));
+    const f4v x = *reinterpret_cast<const f4v*>(p);
+    v[0] = x[0]; v[1] = x[1]; v[2] = x[2]; v[3] = x[3];
+  } else {
+    typedef double d2v __attribute__((ext_vector_type(2)));
+    const d2v x = *reinterpret_cast<const d2v*>(p), y = *reinterpret_cast<const d2v*>(p + 2);
+    v[0] = x[0]; v[1] = x[1]; v[2] = y[0]; v[3] = y[1];
+  }
+}
+// acc[i][j] += sum over k in [k0, k1) of S(r + i, ka + k) * S(kb + k, c + j), k1 - k0 a multiple of 4
+template <typename T, int DS>
+__device__ __forceinline__ void tile_mac(T (&acc)[4][4], const T* S, int r, int ka, int kb, int c, int k0, int k1) {
+#pragma unroll 2
+  for (int k = k0; k < k1; k += 4) {
+    T a[4][4], q[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) lds_load4(&S[(r + i) * DS + ka + k], a[i]);
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++) lds_load4(&S[(kb + k + kk) * DS + c], q[kk]);
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++)
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = fma(a[i][kk], q[kk][j], acc[i][j]);
+  }
+}
 
 template <typename T>
-__global__ __launch_bounds__(1024) void k_diag_block(T* __restrict__ A, size_t lda, int k0, int nb, int factor,
-                                                    int* __restrict__ ipiv, T* __restrict__ Y, T* __restrict__ Z,
-                                                    unsigned long long* __restrict__ lmax, const int* gate, int gp,
-                                                    T* __restrict__ backup) {
-  constexpr int NB = LU_NB, NRB = 8, KR = NB / NRB;   // 8 row blocks of 16 rows
-  constexpr int VEC = 16 / (int)sizeof(T), NV = KR / VEC;
-  constexpr int CS = diag_col_stride<T>();   // column stride of the staged block (padded: conflict-free staging)
+__global__ __launch_bounds__(256) void k_diag_block(T* __restrict__ A, size_t lda, int k0, int nb, int factor,
+                                                   int* __restrict__ ipiv, T* __restrict__ Y, T* __restrict__ Z,
+                                                   unsigned long long* __restrict__ lmax, const int* gate, int gp,
+                                                   T* __restrict__ backup) {
+  constexpr int NB = LU_NB, DS = diag_row_stride<T>(), PER = NB * NB / 256, CH = 16, VEC = 16 / (int)sizeof(T), PERV = PER / VEC;
   typedef T vec_t __attribute__((ext_vector_type(VEC)));
   if (gated_off(gate, gp)) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* Bs = reinterpret_cast<T*>(smem_raw);   // staged block: element (16 rb + k, col) at col * CS + 16 rb + k
-  __shared__ __align__(16) T colv[2][NB];
-  __shared__ T rowb[2][NB], rowc[2][NB], udinv[NB];
-  __shared__ double wmax[1024 / 64];
-  const int tid = threadIdx.x, lane = tid & 63;
-  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int h = (w >> 2) & 1;                       // column half
-  const int rb = (w & 3) | ((w >> 3) << 2);         // row block
-  const int c = h * 64 + lane, r0 = rb * KR;        // this thread's column, first row
-  const int c0 = h * 64;                            // the wave's first column (uniform)
-  T B[KR];
+  T* S = reinterpret_cast<T*>(smem_raw);   // element (r, c) at r * DS + c
+  __shared__ T udinv[NB];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+
+  // the block: 16-byte row segments when A allows them (all in flight at once), else elements
+  const bool vec = (lda % VEC) == 0 && (k0 % VEC) == 0 && (((uintptr_t)A) % 16) == 0 && nb == NB;
+  if (vec) {
+    vec_t x[PERV];
 #pragma unroll
-  for (int k = 0; k < KR; k++) {
-    const int r = r0 + k;
-    B[k] = (r < nb && c < nb) ? A[(size_t)(k0 + r) * lda + k0 + c] : (r == c ? T(1) : T(0));
-  }
-  if (backup) {   // the block as it was, for a restore when the panel fails the no-pivot test
+    for (int i = 0; i < PERV; i++) {
+      const int e = tid + 256 * i, r = e / (NB / VEC), c = (e % (NB / VEC)) * VEC;
+      x[i] = *reinterpret_cast<const vec_t*>(A + (size_t)(k0 + r) * lda + k0 + c);
+    }
 #pragma unroll
-    for (int k = 0; k < KR; k++) backup[(size_t)(r0 + k) * NB + c] = B[k];
-  }
-  // v[kk] for a runtime kk, as a chain of selects (keeps v in registers)
-#define TB_PICK(dst, v, kk)                                          \
-  do {                                                               \
-    dst = v[0];                                                      \
-    _Pragma("unroll") for (int q_ = 1; q_ < KR; q_++) dst = (q_ == (kk)) ? v[q_] : dst; \
-  } while (0)
-  double m = 0.0;
-  if (factor) {
-    for (int j = 0; j < nb; j++) {
-      const int p = j & 1;
-      if (rb == j / KR) {   // the pivot row
-        T own;
-        TB_PICK(own, B, j % KR);
-        rowb[p][c] = own;
+    for (int i = 0; i < PERV; i++) {
+      const int e = tid + 256 * i, r = e / (NB / VEC), c = (e % (NB / VEC)) * VEC;
+      *reinterpret_cast<vec_t*>(&S[r * DS + c]) = x[i];
+      if (backup) *reinterpret_cast<vec_t*>(&backup[r * NB + c]) = x[i];   // for a restore when the panel fails
+    }
+  } else {
+#pragma unroll 1
+    for (int i0 = 0; i0 < PER; i0 += CH) {
+      T x[CH];
+#pragma unroll
+      for (int i = 0; i < CH; i++) {
+        const int e = tid + 256 * (i0 + i), r = e / NB, c = e % NB;
+        x[i] = (r < nb && c < nb) ? A[(size_t)(k0 + r) * lda + k0 + c] : (r == c ? T(1) : T(0));
       }
-      if (c == j) {         // the pivot column, this row block's 16 rows
 #pragma unroll
-        for (int v = 0; v < NV; v++) {
-          vec_t x;
+      for (int i = 0; i < CH; i++) {
+        const int e = tid + 256 * (i0 + i), r = e / NB, c = e % NB;
+        S[r * DS + c] = x[i];
+        if (backup) backup[e] = x[i];
+      }
+    }
+  }
+  __syncthreads();
+
+  if (factor) {
+    T m = T(0);   // max |l|
+    for (int p0 = 0; p0 < NB; p0 += 16) {
+      {
+        const int row = lane < 16 ? p0 + lane : p0 + 16 + 48 * wave + (lane - 16);
+        const bool own = row < NB && (lane >= 16 || wave == 0);   // stored back by this lane
+        T x[16];
 #pragma unroll
-          for (int e = 0; e < VEC; e++) x[e] = B[v * VEC + e];
-          *reinterpret_cast<vec_t*>(&colv[p][r0 + v * VEC]) = x;
+        for (int j = 0; j < 16; j++) x[j] = row < NB ? S[row * DS + p0 + j] : T(0);
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+          const T rp = pivot_rcp(lane_bcast(x[j], j));   // row p0 + j sits in lane j
+          T u[16];
+#pragma unroll
+          for (int k = j + 1; k < 16; k++) u[k] = lane_bcast(x[k], j);
+          if (lane > j) {
+            const T l = x[j] * rp;
+            x[j] = l;
+#pragma unroll
+            for (int k = j + 1; k < 16; k++) x[k] = fma(-l, u[k], x[k]);
+            const T al = row < nb ? tabs(l) : T(0);
+            m = (al > m || al != al) ? al : m;
+          }
+        }
+        __syncthreads();   // every wave has read the sub-panel before any lane writes it
+        if (own) {
+#pragma unroll
+          for (int j = 0; j < 16; j++) S[row * DS + p0 + j] = x[j];
         }
       }
       __syncthreads();
-      // rows up to j and columns below j are final
-      if (r0 + KR - 1 > j && c0 + 63 >= j) {
-        vec_t cvv[NV];
+      const int nc = NB - p0 - 16;   // columns right of the sub-panel (and rows below it)
+      if (tid < nc) {                // U12 = inv(L_ss) A12, one column per thread
+        const int c = p0 + 16 + tid;
+        T x[16];
 #pragma unroll
-        for (int v = 0; v < NV; v++) cvv[v] = *reinterpret_cast<const vec_t*>(&colv[p][r0 + v * VEC]);
-        const T rv = rowb[p][c];
-        const T rp = T(1) / rowb[p][j];
+        for (int i = 0; i < 16; i++) x[i] = S[(p0 + i) * DS + c];
 #pragma unroll
-        for (int k = 0; k < KR; k++) {
-          const int r = r0 + k;
-          const T l = cvv[k / VEC][k % VEC] * rp;
-          const T upd = fma(-l, rv, B[k]);
-          B[k] = r > j ? (c == j ? l : (c > j ? upd : B[k])) : B[k];
+        for (int k = 0; k < 15; k++) {
+#pragma unroll
+          for (int i = k + 1; i < 16; i++) x[i] = fma(-S[(p0 + i) * DS + p0 + k], x[k], x[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < 16; i++) S[(p0 + i) * DS + c] = x[i];
+      }
+      __syncthreads();
+      const int mt = nc / 4;         // trailing block -= L21 U12, 4 x 4 tiles
+      for (int t = tid; t < mt * mt; t += 256) {
+        const int r = p0 + 16 + 4 * (t / mt), c = p0 + 16 + 4 * (t % mt);
+        T acc[4][4] = {};
+        tile_mac<T, DS>(acc, S, r, p0, p0, c, 0, 16);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+          for (int j = 0; j < 4; j++) S[(r + i) * DS + c + j] -= acc[i][j];
+      }
+      __syncthreads();
+    }
+    double md = (double)m;
+    for (int o = 32; o >= 1; o >>= 1) {
+      const double x = __shfl_xor(md, o, 64);
+      md = (x > md || x != x) ? x : md;
+    }
+    if (lane == 0) {
+      if (md != md) md = 1e300;
+      if (md > 0.0) atomicMax(lmax, (unsigned long long)__double_as_longlong(md));
+    }
+    if (vec) {
+#pragma unroll
+      for (int i = 0; i < PERV; i++) {
+        const int e = tid + 256 * i, r = e / (NB / VEC), c = (e % (NB / VEC)) * VEC;
+        *reinterpret_cast<vec_t*>(A + (size_t)(k0 + r) * lda + k0 + c) = *reinterpret_cast<const vec_t*>(&S[r * DS + c]);
+      }
+    } else {
+#pragma unroll 1
+      for (int i0 = 0; i0 < PER; i0 += CH) {
+        T x[CH];
+#pragma unroll
+        for (int i = 0; i < CH; i++) {
+          const int e = tid + 256 * (i0 + i);
+          x[i] = S[(e / NB) * DS + e % NB];
+        }
+#pragma unroll
+        for (int i = 0; i < CH; i++) {
+          const int e = tid + 256 * (i0 + i), r = e / NB, c = e % NB;
+          if (r < nb && c < nb) A[(size_t)(k0 + r) * lda + k0 + c] = x[i];
         }
       }
     }
-    // max |l| over the strictly lower part (NaN counts as large)
-#pragma unroll
-    for (int k = 0; k < KR; k++) {
-      const int r = r0 + k;
-      const double al = (r > c && r < nb) ? (double)tabs(B[k]) : 0.0;
-      m = (al > m || al != al) ? al : m;
-    }
-#pragma unroll
-    for (int k = 0; k < KR; k++) {
-      const int r = r0 + k;
-      if (r < nb && c < nb) A[(size_t)(k0 + r) * lda + k0 + c] = B[k];
-    }
     if (tid < nb) ipiv[k0 + tid] = k0 + tid;
-    for (int o = 32; o >= 1; o >>= 1) {
-      const double x = __shfl_xor(m, o, 64);
-      m = (x > m || x != x) ? x : m;
-    }
-    if (lane == 0) wmax[w] = m;
+    __syncthreads();   // the block is read out before the inverses overwrite it
   }
-  // stage the factored block column by column; reciprocals of U's diagonal
-#pragma unroll
-  for (int v = 0; v < NV; v++) {
-    vec_t x;
-#pragma unroll
-    for (int e = 0; e < VEC; e++) x[e] = B[v * VEC + e];
-    *reinterpret_cast<vec_t*>(&Bs[c * CS + r0 + v * VEC]) = x;
-  }
-#pragma unroll
-  for (int k = 0; k < KR; k++)
-    if (r0 + k == c) udinv[c] = T(1) / B[k];
+
+  if (tid < NB) udinv[tid] = T(1) / S[tid * DS + tid];
   __syncthreads();
-  if (factor && tid == 0) {
-    double mm = 0.0;
-    for (int q = 0; q < 1024 / 64; q++) mm = (wmax[q] > mm || wmax[q] != wmax[q]) ? wmax[q] : mm;
-    if (mm != mm) mm = 1e300;
-    atomicMax(lmax, (unsigned long long)__double_as_longlong(mm));
-  }
-  // Y = inv(L) forwards and Z = inv(U) backwards, one row of each final per step.  Row jl
-  // of Y is zero right of jl and changes rows below it; row ju of Z is zero left of ju and
-  // changes rows above it.
-  T Yv[KR], Zv[KR];
+  {  // 16 x 16 diagonal blocks: lanes 0-31 invert L of blocks 2w, 2w+1, lanes 32-63 U of the same
+    const int g = lane >> 4, cl = lane & 15, o = 16 * (2 * wave + (g & 1));
+    T x[16];
 #pragma unroll
-  for (int k = 0; k < KR; k++) {
-    const int r = r0 + k;
-    Yv[k] = (r == c) ? T(1) : T(0);
-    Zv[k] = (r == c) ? T(1) : T(0);
-  }
-  for (int st = 0; st < NB; st++) {
-    const int p = st & 1;
-    const int jl = st, ju = NB - 1 - st;
-    if (rb == jl / KR) {
-      T yown;
-      TB_PICK(yown, Yv, jl % KR);
-      rowb[p][c] = yown;
-    }
-    if (rb == ju / KR) {
-      T zown;
-      TB_PICK(zown, Zv, ju % KR);
-      zown = zown * udinv[ju];
+    for (int i = 0; i < 16; i++) x[i] = (i == cl) ? T(1) : T(0);
+    if (g < 2) {
 #pragma unroll
-      for (int k = 0; k < KR; k++) Zv[k] = (k == ju % KR) ? zown : Zv[k];
-      rowc[p][c] = zown;
-    }
-    __syncthreads();
-    if (r0 + KR - 1 > jl && c0 <= jl) {
-      const T yr = rowb[p][c];
-      vec_t lvv[NV];
+      for (int k = 0; k < 15; k++) {
 #pragma unroll
-      for (int v = 0; v < NV; v++) lvv[v] = *reinterpret_cast<const vec_t*>(&Bs[jl * CS + r0 + v * VEC]);
-#pragma unroll
-      for (int k = 0; k < KR; k++) {
-        const T ny = fma(-lvv[k / VEC][k % VEC], yr, Yv[k]);
-        Yv[k] = r0 + k > jl ? ny : Yv[k];
+        for (int i = k + 1; i < 16; i++) x[i] = fma(-S[(o + i) * DS + o + k], x[k], x[i]);
       }
-    }
-    if (r0 < ju && c0 + 63 >= ju) {
-      const T zr = rowc[p][c];
-      vec_t uvv[NV];
 #pragma unroll
-      for (int v = 0; v < NV; v++) uvv[v] = *reinterpret_cast<const vec_t*>(&Bs[ju * CS + r0 + v * VEC]);
+      for (int i = 0; i < 16; i++)
+        if (i > cl) S[(o + i) * DS + o + cl] = x[i];
+    } else {
 #pragma unroll
-      for (int k = 0; k < KR; k++) {
-        const T nz = fma(-uvv[k / VEC][k % VEC], zr, Zv[k]);
-        Zv[k] = r0 + k < ju ? nz : Zv[k];
+      for (int k = 15; k >= 0; k--) {
+        x[k] *= udinv[o + k];
+#pragma unroll
+        for (int i = 0; i < k; i++) x[i] = fma(-S[(o + i) * DS + o + k], x[k], x[i]);
       }
+#pragma unroll
+      for (int i = 0; i < 16; i++)
+        if (i <= cl) S[(o + i) * DS + o + cl] = x[i];
     }
   }
+  __syncthreads();
+  // doubling: for each pair of b x b diagonal blocks at o = 2b p,
+  //   L: T = L21 Y11 -> L21's place, then Y21 = -Y22 T;  U: T = U12 Z22 -> U12's place, then Z12 = -Z11 T
+  // (Y is unit lower: stored strictly below the diagonal; Z upper: stored on and above it).
+  // A triangular factor is masked only in the 4 x 4 block on its diagonal.
+  for (int b = 16; b < NB; b *= 2) {
+    const int tb = b / 4, ntile = tb * tb, npair = NB / (2 * b), total = 2 * npair * ntile;
+#pragma unroll 1
+    for (int step = 0; step < 2; step++) {
+      T acc[2][4][4] = {};
+      int rr[2], cc[2];
 #pragma unroll
-  for (int k = 0; k < KR; k++) {
-    const int r = r0 + k;
-    const bool in = r < nb && c < nb;
-    Y[(size_t)r * NB + c] = in ? Yv[k] : T(0);
-    Z[(size_t)r * NB + c] = in ? Zv[k] : T(0);
+      for (int n = 0; n < 2; n++) {
+        const int it = tid + 256 * n;
+        rr[n] = -1;
+        cc[n] = 0;
+        if (it >= total) continue;
+        const int side = it / (npair * ntile), rest = it % (npair * ntile);
+        const int o = 2 * b * (rest / ntile), tile = rest % ntile;
+        // a wave's lanes share the bound of the k range (tj in step 0, ti in step 1) as far as
+        // possible: the other index runs fastest
+        const int ti = 4 * (step == 0 ? tile % tb : tile / tb), tj = 4 * (step == 0 ? tile / tb : tile % tb);
+        T (&ac)[4][4] = acc[n];
+        if (side == 0) {   // lower: rows o+b+ti, columns o+tj
+          const int r = o + b + ti, c = o + tj;
+          rr[n] = r;
+          cc[n] = c;
+          if (step == 0) {   // L21 Y11: Y11(o+k, c+j) vanishes for k < tj + j
+            tile_mac<T, DS>(ac, S, r, o, o, c, tj + 4, b);
+#pragma unroll
+            for (int kk = 0; kk < 4; kk++)
+#pragma unroll
+              for (int j = 0; j < 4; j++) {
+                const T y = kk > j ? S[(c + kk) * DS + c + j] : (kk == j ? T(1) : T(0));
+#pragma unroll
+                for (int i = 0; i < 4; i++) ac[i][j] = fma(S[(r + i) * DS + c + kk], y, ac[i][j]);
+              }
+          } else {           // Y22 T (negated below): Y22(r+i, o+b+k) vanishes for k > ti + i
+            tile_mac<T, DS>(ac, S, r, o + b, o + b, c, 0, ti);
+#pragma unroll
+            for (int kk = 0; kk < 4; kk++)
+#pragma unroll
+              for (int i = 0; i < 4; i++) {
+                const T y = i > kk ? S[(r + i) * DS + r + kk] : (i == kk ? T(1) : T(0));
+#pragma unroll
+                for (int j = 0; j < 4; j++) ac[i][j] = fma(y, S[(r + kk) * DS + c + j], ac[i][j]);
+              }
+          }
+        } else {           // upper: rows o+ti, columns o+b+tj
+          const int r = o + ti, c = o + b + tj;
+          rr[n] = r;
+          cc[n] = c;
+          if (step == 0) {   // U12 Z22: Z22(o+b+k, c+j) vanishes for k > tj + j
+            tile_mac<T, DS>(ac, S, r, o + b, o + b, c, 0, tj);
+#pragma unroll
+            for (int kk = 0; kk < 4; kk++)
+#pragma unroll
+              for (int j = 0; j < 4; j++) {
+                const T z = kk <= j ? S[(c + kk) * DS + c + j] : T(0);
+#pragma unroll
+                for (int i = 0; i < 4; i++) ac[i][j] = fma(S[(r + i) * DS + c + kk], z, ac[i][j]);
+              }
+          } else {           // Z11 T (negated below): Z11(r+i, o+k) vanishes for k < ti + i
+            tile_mac<T, DS>(ac, S, r, o, o, c, ti + 4, b);
+#pragma unroll
+            for (int kk = 0; kk < 4; kk++)
+#pragma unroll
+              for (int i = 0; i < 4; i++) {
+                const T z = i <= kk ? S[(r + i) * DS + r + kk] : T(0);
+#pragma unroll
+                for (int j = 0; j < 4; j++) ac[i][j] = fma(z, S[(r + kk) * DS + c + j], ac[i][j]);
+              }
+          }
+        }
+      }
+      __syncthreads();   // every operand read before the results replace them
+#pragma unroll
+      for (int n = 0; n < 2; n++) {
+        if (rr[n] < 0) continue;
+        const T sg = step == 0 ? T(1) : T(-1);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+          for (int j = 0; j < 4; j++) S[(rr[n] + i) * DS + cc[n] + j] = sg * acc[n][i][j];
+      }
+      __syncthreads();
+    }
   }
-#undef TB_PICK
+#pragma unroll 4
+  for (int i = 0; i < PERV; i++) {
+    const int e = tid + 256 * i, r = e / (NB / VEC), c = (e % (NB / VEC)) * VEC;
+    const vec_t x = *reinterpret_cast<const vec_t*>(&S[r * DS + c]);
+    vec_t y, z;
+#pragma unroll
+    for (int v = 0; v < VEC; v++) {
+      const bool in = r < nb && c + v < nb;
+      y[v] = in ? (r > c + v ? x[v] : (r == c + v ? T(1) : T(0))) : T(0);
+      z[v] = in ? (r <= c + v ? x[v] : T(0)) : T(0);
+    }
+    *reinterpret_cast<vec_t*>(&Y[r * NB + c]) = y;
+    *reinterpret_cast<vec_t*>(&Z[r * NB + c]) = z;
+  }
 }
 
 // After the factorisation: info = 1 + first column whose U diagonal is zero or NaN
@@ -464,66 +616,72 @@ template <> struct MfmaAcc<float> {
 };
 
 
-// Software pipeline: the global loads of K-chunk kt+1 are issued into registers before the
-// MFMA loop of chunk kt and written to the other LDS buffer after it; within a chunk the
-// operand fragments of step s+1 are read from LDS before the MFMAs of step s.
-// Write-back goes through LDS: the accumulators (times alpha) are stored in the MFMA layout,
-// then each thread reads back 16-byte row segments and updates C with vector loads and
-// stores, 8 segments in flight per thread, so the read-modify-write of a tile costs a few
-// memory latencies instead of one per element.  Tiles that lie wholly inside the matrices
-// (with 16-byte aligned rows) take unguarded vector accesses; edge tiles take element
-// accesses with bounds checks.
+// Schedule of one tile:
+//  1. per K chunk: the next chunk's global loads are issued into registers, the MFMAs of the
+//     chunk run from LDS (the operand fragments of step s+1 are read before the MFMAs of step
+//     s), then the registers are written to LDS;
+//  2. write-back through LDS: the accumulators (times alpha) are stored in the MFMA layout,
+//     then each thread updates 16-byte row segments of C, 8 in flight, CROWS tile rows per
+//     pass, so the read-modify-write of a tile costs a few memory latencies instead of one
+//     per element.  The products are summed from zero and C is added once at the end (adding
+//     them into C one MFMA at a time rounds each partial sum at the magnitude of C, and an
+//     FP32 LU built that way needs more refinement steps).
+// NBUF = 1 (FP32, 128-tiles): one LDS buffer and two barriers per chunk, 36 KiB of LDS, so
+// three workgroups fit a CU and one's write-back overlaps the others' MFMAs; otherwise two
+// buffers and one barrier per chunk.
+// Edge tiles: when the operands are 16-byte aligned, K is a multiple of BK, N of the vector
+// width and both M and N are at least GB ("fast"), a tile that would cross the edge of C is
+// moved back inside it (origin min(bm, M - GB), min(bn, N - GB)) and stores only the rows and
+// columns at or beyond its own origin, which the tile before it does not store: every tile
+// then takes unguarded vector loads.  Other shapes take element loads with bounds checks.
 // TRACK_MAX: also the largest |C| written by each workgroup (NaN counts as large), stored
 // as the bit pattern of a nonnegative double in vmax[workgroup] (no atomics: the slots are
 // reduced by the consumer).
-template <typename T, int BK, bool TRACK_MAX, int GB>
-__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, const T* __restrict__ A, size_t lda,
-                                              const T* __restrict__ B, size_t ldb, T beta, T* __restrict__ C, size_t ldc,
-                                              const int* gate, int gp, unsigned long long* __restrict__ vmax) {
+template <typename T, int BK, bool TRACK_MAX, int GB, bool FAST>
+__device__ __forceinline__ void gemm_tile(int M, int N, int K, T alpha, const T* __restrict__ A, size_t lda,
+                                          const T* __restrict__ B, size_t ldb, T beta, T* __restrict__ C, size_t ldc,
+                                          const int* gate, int gp, unsigned long long* __restrict__ vmax) {
   typedef typename MfmaAcc<T>::type acc_t;
   if (gated_off(gate, gp)) return;
-  constexpr int GBM = GB, GBN = GB;                       // block tile
   constexpr int MT = GB / 32;                             // 16 x 16 MFMA tiles per wave and dimension
   constexpr int VEC = 16 / sizeof(T);                     // elements per 16-byte load
-  constexpr int A_LD = (GBM * BK) / (256 * VEC);          // vector loads per thread for A
-  constexpr int B_LD = (BK * GBN) / (256 * VEC);
+  constexpr int A_LD = (GB * BK) / (256 * VEC);           // vector loads per thread for A
+  constexpr int B_LD = (BK * GB) / (256 * VEC);
   constexpr int SA = BK + 16 / sizeof(T);                 // 36 words (float), 18 doubles
-  constexpr int SB = GBN + 64 / sizeof(T);                // 144 / 80 words (float), 136 / 72 doubles
+  constexpr int SB = GB + 64 / sizeof(T);                 // 144 / 80 words (float), 136 / 72 doubles
   constexpr int KS = BK / 4;                              // MFMA steps per chunk
-  constexpr int SC = GBN + 4;                             // row stride of the staged C tile
-  constexpr size_t AB_BYTES = sizeof(T) * (2 * GBM * SA + 2 * BK * SB);
-  constexpr int CROWS = (int)((AB_BYTES / (sizeof(T) * SC)) >= GBM ? GBM : GBM / 2);   // tile rows staged per pass
+  constexpr int SC = GB + 4;                              // row stride of the staged C tile
+  constexpr int NBUF = (sizeof(T) == 4 && GB == 128) ? 1 : 2;
+  constexpr size_t AB_BYTES = sizeof(T) * NBUF * (GB * SA + BK * SB);
+  constexpr int CROWS = AB_BYTES >= sizeof(T) * SC * GB ? GB : AB_BYTES >= sizeof(T) * SC * (GB / 2) ? GB / 2 : GB / 4;
+  constexpr int SEG = GB / VEC;                           // 16-byte segments per tile row
+  constexpr int PER = CROWS * SEG / 256;                  // segments per thread per pass
   static_assert(CROWS * SC * sizeof(T) <= AB_BYTES, "staged C tile exceeds the LDS of the operands");
+  static_assert(PER * 256 == CROWS * SEG, "staged rows must split evenly over the threads");
   __shared__ __attribute__((aligned(16))) char smem[AB_BYTES];
-  T (*As)[GBM][SA] = reinterpret_cast<T (*)[GBM][SA]>(smem);
-  T (*Bs)[BK][SB] = reinterpret_cast<T (*)[BK][SB]>(smem + sizeof(T) * 2 * GBM * SA);
+  T (*As)[GB][SA] = reinterpret_cast<T (*)[GB][SA]>(smem);
+  T (*Bs)[BK][SB] = reinterpret_cast<T (*)[BK][SB]>(smem + sizeof(T) * NBUF * GB * SA);
   T* Cs = reinterpret_cast<T*>(smem);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave >> 1) * (GB / 2), wn = (wave & 1) * (GB / 2);
-  const int bm = blockIdx.y * GBM, bn = blockIdx.x * GBN;
-  const bool full = ((lda % VEC) == 0) && ((((uintptr_t)A) % 16) == 0) && ((ldb % VEC) == 0) && ((((uintptr_t)B) % 16) == 0) &&
-                    bm + GBM <= M && bn + GBN <= N && (K % BK) == 0;
-
-  acc_t acc[MT][MT];
-#pragma unroll
-  for (int a = 0; a < MT; a++)
-#pragma unroll
-    for (int b = 0; b < MT; b++) acc[a][b] = acc_t{0, 0, 0, 0};
+  const int ci = lane & 15, kq = lane >> 4;
+  const int bm = FAST ? min((int)blockIdx.y * GB, M - GB) : (int)blockIdx.y * GB;   // tile origin
+  const int bn = FAST ? min((int)blockIdx.x * GB, N - GB) : (int)blockIdx.x * GB;
+  const int sm = (int)blockIdx.y * GB - bm, sn = (int)blockIdx.x * GB - bn;         // first row / column this tile stores
 
   typedef T vec_t __attribute__((ext_vector_type(VEC)));
   vec_t ra[A_LD], rb[B_LD];
-
   auto fetch = [&](int k0) {
-    if (full) {
+    if constexpr (FAST) {
 #pragma unroll
       for (int i = 0; i < A_LD; i++) {
-        const int e = (tid + i * 256) * VEC;   // element index in the GBM x BK tile, k fastest
+        const int e = (tid + i * 256) * VEC;   // element index in the GB x BK tile, k fastest
         ra[i] = *reinterpret_cast<const vec_t*>(A + (size_t)(bm + e / BK) * lda + k0 + e % BK);
       }
 #pragma unroll
       for (int i = 0; i < B_LD; i++) {
-        const int e = (tid + i * 256) * VEC;   // element index in the BK x GBN tile, n fastest
-        rb[i] = *reinterpret_cast<const vec_t*>(B + (size_t)(k0 + e / GBN) * ldb + bn + e % GBN);
+        const int e = (tid + i * 256) * VEC;   // element index in the BK x GB tile, n fastest
+        rb[i] = *reinterpret_cast<const vec_t*>(B + (size_t)(k0 + e / GB) * ldb + bn + e % GB);
       }
     } else {
 #pragma unroll
@@ -536,7 +694,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, cons
 #pragma unroll
       for (int i = 0; i < B_LD; i++) {
         const int e = (tid + i * 256) * VEC;
-        const int gk = k0 + e / GBN, gc = bn + e % GBN;
+        const int gk = k0 + e / GB, gc = bn + e % GB;
 #pragma unroll
         for (int v = 0; v < VEC; v++) rb[i][v] = (gk < K && gc + v < N) ? B[(size_t)gk * ldb + gc + v] : T(0);
       }
@@ -551,19 +709,21 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, cons
 #pragma unroll
     for (int i = 0; i < B_LD; i++) {
       const int e = (tid + i * 256) * VEC;
-      *reinterpret_cast<vec_t*>(&Bs[buf][e / GBN][e % GBN]) = rb[i];
+      *reinterpret_cast<vec_t*>(&Bs[buf][e / GB][e % GB]) = rb[i];
     }
   };
 
   const int nk = (K + BK - 1) / BK;
-  if (nk > 0) {
-    fetch(0);
-    stash(0);
-  }
+  if (nk > 0) fetch(0);
+  acc_t acc[MT][MT];
+#pragma unroll
+  for (int a = 0; a < MT; a++)
+#pragma unroll
+    for (int b = 0; b < MT; b++) acc[a][b] = acc_t{0, 0, 0, 0};
+  if (nk > 0) stash(0);
   __syncthreads();
-  const int ci = lane & 15, kq = lane >> 4;
   for (int kt = 0; kt < nk; kt++) {
-    const int cur = kt & 1;
+    const int cur = NBUF == 1 ? 0 : (kt & 1);
     if (kt + 1 < nk) fetch((kt + 1) * BK);
     T af[2][MT], bf[2][MT];
 #pragma unroll
@@ -584,15 +744,15 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, cons
 #pragma unroll
         for (int b = 0; b < MT; b++) acc[a][b] = MfmaAcc<T>::mma(af[p][a], bf[p][b], acc[a][b]);
     }
-    if (kt + 1 < nk) stash(cur ^ 1);
+    if (kt + 1 < nk) {
+      if constexpr (NBUF == 1) __syncthreads();   // every wave is done reading the buffer
+      stash(NBUF == 1 ? 0 : cur ^ 1);
+    }
     __syncthreads();
   }
 
   // write back through LDS, CROWS tile rows per pass
   const bool use_c = beta != T(0);
-  const bool vec_c = full && ((ldc % VEC) == 0) && ((((uintptr_t)C) % 16) == 0);
-  constexpr int SEG = GBN / VEC;                          // 16-byte segments per tile row
-  constexpr int PER = CROWS * SEG / 256;                  // segments per thread per pass
   constexpr int BATCH = PER < 8 ? PER : 8;
   double vm = 0.0;
   auto track = [&](T x) {
@@ -602,7 +762,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, cons
     }
   };
 #pragma unroll
-  for (int pass = 0; pass < GBM / CROWS; pass++) {
+  for (int pass = 0; pass < GB / CROWS; pass++) {
     const int r0 = pass * CROWS;
     if (pass > 0) __syncthreads();
 #pragma unroll
@@ -611,51 +771,42 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, cons
 #pragma unroll
       for (int b = 0; b < MT; b++)
 #pragma unroll
-        for (int v = 0; v < 4; v++)
-          Cs[(wm + a * 16 + MfmaAcc<T>::row(lane, v) - r0) * SC + wn + b * 16 + ci] = alpha * acc[a][b][v];
+        for (int v = 0; v < 4; v++) Cs[(wm + a * 16 + MfmaAcc<T>::row(lane, v) - r0) * SC + wn + b * 16 + ci] = alpha * acc[a][b][v];
     }
     __syncthreads();
 #pragma unroll
     for (int i0 = 0; i0 < PER; i0 += BATCH) {
       vec_t cvv[BATCH];
-      if (vec_c) {
-        if (use_c) {
 #pragma unroll
-          for (int j = 0; j < BATCH; j++) {
-            const int idx = tid + (i0 + j) * 256, row = r0 + idx / SEG, c = (idx % SEG) * VEC;
-            cvv[j] = *reinterpret_cast<const vec_t*>(C + (size_t)(bm + row) * ldc + bn + c);
-          }
-        }
-#pragma unroll
-        for (int j = 0; j < BATCH; j++) {
-          const int idx = tid + (i0 + j) * 256, row = r0 + idx / SEG, c = (idx % SEG) * VEC;
-          vec_t out;
-#pragma unroll
-          for (int v = 0; v < VEC; v++) {
-            const T val = Cs[(row - r0) * SC + c + v];
-            out[v] = use_c ? fma(beta, cvv[j][v], val) : val;
-            track(out[v]);
-          }
-          *reinterpret_cast<vec_t*>(C + (size_t)(bm + row) * ldc + bn + c) = out;
-        }
-      } else {
-#pragma unroll
-        for (int j = 0; j < BATCH; j++) {
-          const int idx = tid + (i0 + j) * 256, row = r0 + idx / SEG, c = (idx % SEG) * VEC;
+      for (int j = 0; j < BATCH; j++) {
+        const int idx = tid + (i0 + j) * 256, row = r0 + idx / SEG, c = (idx % SEG) * VEC;
+        if constexpr (FAST) {
+          cvv[j] = use_c ? *reinterpret_cast<const vec_t*>(C + (size_t)(bm + row) * ldc + bn + c) : vec_t{};
+        } else {
 #pragma unroll
           for (int v = 0; v < VEC; v++)
             cvv[j][v] = (use_c && bm + row < M && bn + c + v < N) ? C[(size_t)(bm + row) * ldc + bn + c + v] : T(0);
         }
+      }
 #pragma unroll
-        for (int j = 0; j < BATCH; j++) {
-          const int idx = tid + (i0 + j) * 256, row = r0 + idx / SEG, c = (idx % SEG) * VEC;
+      for (int j = 0; j < BATCH; j++) {
+        const int idx = tid + (i0 + j) * 256, row = r0 + idx / SEG, c = (idx % SEG) * VEC;
+        const vec_t val = *reinterpret_cast<const vec_t*>(&Cs[(row - r0) * SC + c]);
+        vec_t out;
+#pragma unroll
+        for (int v = 0; v < VEC; v++) out[v] = use_c ? fma(beta, cvv[j][v], val[v]) : val[v];
+        if constexpr (FAST) {
+          if (row >= sm && c >= sn) {
+#pragma unroll
+            for (int v = 0; v < VEC; v++) track(out[v]);
+            *reinterpret_cast<vec_t*>(C + (size_t)(bm + row) * ldc + bn + c) = out;
+          }
+        } else {
 #pragma unroll
           for (int v = 0; v < VEC; v++) {
             if (bm + row < M && bn + c + v < N) {
-              const T val = Cs[(row - r0) * SC + c + v];
-              const T o = use_c ? fma(beta, cvv[j][v], val) : val;
-              track(o);
-              C[(size_t)(bm + row) * ldc + bn + c + v] = o;
+              track(out[v]);
+              C[(size_t)(bm + row) * ldc + bn + c + v] = out[v];
             }
           }
         }
@@ -678,6 +829,23 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, cons
       vmax[blockIdx.y * gridDim.x + blockIdx.x] = (unsigned long long)__double_as_longlong(mm);
     }
   }
+}
+
+template <typename T, int BK, bool TRACK_MAX, int GB, bool FAST>
+__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, const T* __restrict__ A, size_t lda,
+                                              const T* __restrict__ B, size_t ldb, T beta, T* __restrict__ C, size_t ldc,
+                                              const int* gate, int gp, unsigned long long* __restrict__ vmax) {
+  gemm_tile<T, BK, TRACK_MAX, GB, FAST>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, gp, vmax);
+}
+// FP32 128-tiles: with this register budget the compiler keeps the accumulators in
+// architectural VGPRs and allocates 144 registers per lane in all, so three workgroups fit a
+// SIMD's 512 and, when one of them ends, the diagonal-block kernel of the next panel (4 waves
+// of 56 registers per SIMD) fits beside the other two
+template <bool TRACK_MAX>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_num_vgpr(72))) void k_gemm_f32(
+    int M, int N, int K, float alpha, const float* __restrict__ A, size_t lda, const float* __restrict__ B, size_t ldb, float beta,
+    float* __restrict__ C, size_t ldc, const int* gate, int gp, unsigned long long* __restrict__ vmax) {
+  gemm_tile<float, 32, TRACK_MAX, 128, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, gp, vmax);
 }
 
 // copy a rows x cols block (row-major, separate leading dims)
@@ -1027,26 +1195,39 @@ size_t gemm_tiles(int M, int N) {
   return (size_t)cdiv(M, t) * cdiv(N, t);
 }
 
+template <typename T, int GB, bool TM, bool FAST>
+static void launch_gemm(dim3 grid, int M, int N, int K, T alpha, const T* A, size_t lda, const T* B, size_t ldb, T beta, T* C,
+                        size_t ldc, hipStream_t st, const int* gate, int gp, unsigned long long* vm) {
+  constexpr int BK = sizeof(T) == 8 ? 16 : 32;
+  if constexpr (sizeof(T) == 4 && GB == 128 && FAST)
+    hipLaunchKernelGGL((k_gemm_f32<TM>), grid, dim3(256), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, gp, vm);
+  else
+    hipLaunchKernelGGL((k_gemm<T, BK, TM, GB, FAST>), grid, dim3(256), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, gp, vm);
+}
+
+template <typename T, int GB>
+static void launch_gemm(int M, int N, int K, T alpha, const T* A, size_t lda, const T* B, size_t ldb, T beta, T* C, size_t ldc,
+                        hipStream_t st, const int* gate, int gp, unsigned long long* vm) {
+  constexpr int BK = sizeof(T) == 8 ? 16 : 32;
+  constexpr size_t VEC = 16 / sizeof(T);
+  const bool fast = lda % VEC == 0 && (uintptr_t)A % 16 == 0 && ldb % VEC == 0 && (uintptr_t)B % 16 == 0 && ldc % VEC == 0 &&
+                    (uintptr_t)C % 16 == 0 && K % BK == 0 && N % VEC == 0 && M >= GB && N >= GB;
+  const dim3 grid(cdiv(N, GB), cdiv(M, GB));
+  if (vm) {
+    if (fast) launch_gemm<T, GB, true, true>(grid, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, st, gate, gp, vm);
+    else launch_gemm<T, GB, true, false>(grid, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, st, gate, gp, vm);
+  } else {
+    if (fast) launch_gemm<T, GB, false, true>(grid, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, st, gate, gp, vm);
+    else launch_gemm<T, GB, false, false>(grid, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, st, gate, gp, vm);
+  }
+}
+
 template <typename T>
 hipError_t gemm(int M, int N, int K, T alpha, const T* A, size_t lda, const T* B, size_t ldb, T beta, T* C, size_t ldc,
                 hipStream_t st, const int* gate, int gp, unsigned long long* vmax) {
   if (M <= 0 || N <= 0) return hipSuccess;
-  constexpr int BK = sizeof(T) == 8 ? 16 : 32;
-  const bool small = gemm_small(M, N);
-  unsigned long long* vm = vmax;
-  if (small) {
-    dim3 grid(cdiv(N, 64), cdiv(M, 64));
-    if (vm)
-      hipLaunchKernelGGL((k_gemm<T, BK, true, 64>), grid, dim3(256), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, gp, vm);
-    else
-      hipLaunchKernelGGL((k_gemm<T, BK, false, 64>), grid, dim3(256), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, gp, vm);
-  } else {
-    dim3 grid(cdiv(N, 128), cdiv(M, 128));
-    if (vm)
-      hipLaunchKernelGGL((k_gemm<T, BK, true, 128>), grid, dim3(256), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, gp, vm);
-    else
-      hipLaunchKernelGGL((k_gemm<T, BK, false, 128>), grid, dim3(256), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, gp, vm);
-  }
+  if (gemm_small(M, N)) launch_gemm<T, 64>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, st, gate, gp, vmax);
+  else launch_gemm<T, 128>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, st, gate, gp, vmax);
   return hipGetLastError();
 }
 template hipError_t gemm<double>(int, int, int, double, const double*, size_t, const double*, size_t, double, double*, size_t,
@@ -1237,7 +1418,7 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
       const int k0 = q * NB, nb = min(NB, N - k0), m2 = N - k0 - nb, n2 = m2;
       const int k1 = k0 + nb;
       // P(q)
-      hipLaunchKernelGGL(k_diag_block<T>, dim3(1), dim3(1024), diag_lds_bytes<T>(), sa, A, lda, k0, nb, 1, ipiv, Yp(q), Zp(q),
+      hipLaunchKernelGGL(k_diag_block<T>, dim3(1), dim3(256), diag_lds_bytes<T>(), sa, A, lda, k0, nb, 1, ipiv, Yp(q), Zp(q),
                          W.lmaxv + q, g, q, W.Dbak);
       int nslot = 0;
       if (m2 > 0) {
@@ -1289,7 +1470,7 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
     hipError_t r = hipLaunchCooperativeKernel((const void*)kpanel, dim3(G), dim3(256), args, (unsigned)(R * NB * sizeof(T)), st);
     if (r != hipSuccess) return r;
     any_pivot = true;
-    hipLaunchKernelGGL(k_diag_block<T>, dim3(1), dim3(1024), diag_lds_bytes<T>(), st, A, lda, k0, nb, 0, ipiv, Yp(q), Zp(q), W.lmax,
+    hipLaunchKernelGGL(k_diag_block<T>, dim3(1), dim3(256), diag_lds_bytes<T>(), st, A, lda, k0, nb, 0, ipiv, Yp(q), Zp(q), W.lmax,
                        (const int*)nullptr, 0, (T*)nullptr);
     if (N - nb > 0) hipLaunchKernelGGL(k_laswp<T>, dim3(cdiv(N - nb, 256)), dim3(256), 0, st, A, lda, N, k0, nb, ipiv);
     const int n2 = N - k0 - nb;
