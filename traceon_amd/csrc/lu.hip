@@ -839,13 +839,181 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, cons
 }
 // FP32 128-tiles: with this register budget the compiler keeps the accumulators in
 // architectural VGPRs and allocates 144 registers per lane in all, so three workgroups fit a
-// SIMD's 512 and, when one of them ends, the diagonal-block kernel of the next panel (4 waves
-// of 56 registers per SIMD) fits beside the other two
+// SIMD's 512 and, when one of them ends, the diagonal-block kernel of the next panel (one
+// wave of 144 registers per SIMD) fits beside the other two
 template <bool TRACK_MAX>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_num_vgpr(72))) void k_gemm_f32(
     int M, int N, int K, float alpha, const float* __restrict__ A, size_t lda, const float* __restrict__ B, size_t ldb, float beta,
     float* __restrict__ C, size_t ldc, const int* gate, int gp, unsigned long long* __restrict__ vmax) {
   gemm_tile<float, 32, TRACK_MAX, 128, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, gp, vmax);
+}
+
+// ---------------------------------------------------------------------------------
+// FP32 GEMM on BF16 MFMA (the large products of the FP32 factorisation):
+// C = beta*C + alpha*A*B with every FP32 operand split into three BF16 parts,
+// x = x0 + x1 + x2 (x0 = bf16(x), x1 = bf16(x - x0), x2 = bf16(x - x0 - x1), round to
+// nearest even, 24 significant bits in all), and a*b formed from the six products whose
+// parts' orders sum to at most 2 (the three dropped ones are below 2^-24 |a b|).  BF16
+// products are exact in FP32 and v_mfma_f32_16x16x32_bf16 accumulates in FP32, so the result
+// has the accuracy of an FP32 GEMM (measured: 6e-7 relative against an FP64 product of
+// 1024 x 256 x 1536, the FP32 MFMA kernel 8e-7) at 6 x 16 cycles per 16 x 16 x 32 block
+// instead of 8 x 32 for v_mfma_f32_16x16x4f32.
+// Operand layout of v_mfma_f32_16x16x32_bf16: lane l holds A[l%16][8(l/16)+i] and
+// B[8(l/16)+i][l%16], i = 0..7; the accumulators are those of v_mfma_f32_16x16x4f32.
+// Tile 128 x 128, 256 threads (2 x 2 waves of 64 x 64), K chunks of 32.  The split happens
+// when a chunk moves from registers to LDS: A as [part][row][k], B as [part][col][k], 64-byte
+// rows whose four 16-byte segments are XOR-swizzled by (row / 4) % 4, so the 16 rows a
+// fragment read touches fall in distinct banks.  48 KiB of LDS; two workgroups per CU.
+// Fast shapes only (see gemm_tile): A rows and C rows 16-byte aligned, K % 32 == 0,
+// N % 4 == 0, M, N >= 128; B is read element-wise (its rows need no alignment).
+// ---------------------------------------------------------------------------------
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void split_bf16x3(float x, __bf16& p0, __bf16& p1, __bf16& p2) {
+  p0 = (__bf16)x;
+  const float r1 = x - (float)p0;
+  p1 = (__bf16)r1;
+  p2 = (__bf16)(r1 - (float)p1);
+}
+// byte offset of (row, k) in one 128 x 32 BF16 part
+__device__ __forceinline__ int x6_off(int row, int k) { return row * 64 + 16 * ((k >> 3) ^ ((row >> 2) & 3)) + 2 * (k & 7); }
+
+__global__ __launch_bounds__(256) void k_gemm_x6(int M, int N, int K, float alpha, const float* __restrict__ A, size_t lda,
+                                                 const float* __restrict__ B, size_t ldb, float beta, float* __restrict__ C,
+                                                 size_t ldc, const int* gate, int gp) {
+  typedef float f4v __attribute__((ext_vector_type(4)));
+  constexpr int GB = 128, BK = 32, MT = 4, PART = GB * BK * 2;
+  constexpr int SC = GB + 4, CROWS = 64, SEG = GB / 4, PER = CROWS * SEG / 256;
+  if (gated_off(gate, gp)) return;
+  __shared__ __attribute__((aligned(16))) char smem[6 * PART];   // A parts 0-2, B parts 0-2; later the staged C rows
+  char* As = smem;
+  char* Bs = smem + 3 * PART;
+  float* Cs = reinterpret_cast<float*>(smem);
+  static_assert(CROWS * SC * 4 <= 6 * PART, "staged C rows exceed the LDS of the operands");
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+  const int li = lane & 15, lq = lane >> 4;
+  const int bm = min((int)blockIdx.y * GB, M - GB), bn = min((int)blockIdx.x * GB, N - GB);   // tile origin (inside C)
+  const int sm = (int)blockIdx.y * GB - bm, sn = (int)blockIdx.x * GB - bn;                  // first row / column stored
+  f4v acc[MT][MT];
+#pragma unroll
+  for (int a = 0; a < MT; a++)
+#pragma unroll
+    for (int b = 0; b < MT; b++) acc[a][b] = f4v{0, 0, 0, 0};
+  // per chunk, thread: A elements (tid + 256 i) * 4 .. + 3 of the 128 x 32 tile (k fastest);
+  // B column bn + (tid % 128), k = 4 (tid / 128 + 2 i) + 0..3
+  f4v ra[4];
+  float rb[16];
+  const int bc = tid & 127, bkg = tid >> 7;
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int e = (tid + i * 256) * 4;
+      ra[i] = *reinterpret_cast<const f4v*>(A + (size_t)(bm + e / BK) * lda + k0 + e % BK);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) rb[4 * i + j] = B[(size_t)(k0 + 4 * (bkg + 2 * i) + j) * ldb + bn + bc];
+  };
+  auto stash = [&]() {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int e = (tid + i * 256) * 4;
+      bf16x4_t p0, p1, p2;
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        __bf16 x0, x1, x2;
+        split_bf16x3(ra[i][v], x0, x1, x2);
+        p0[v] = x0; p1[v] = x1; p2[v] = x2;
+      }
+      const int off = x6_off(e / BK, e % BK);
+      *reinterpret_cast<bf16x4_t*>(As + off) = p0;
+      *reinterpret_cast<bf16x4_t*>(As + PART + off) = p1;
+      *reinterpret_cast<bf16x4_t*>(As + 2 * PART + off) = p2;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      bf16x4_t p0, p1, p2;
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        __bf16 x0, x1, x2;
+        split_bf16x3(rb[4 * i + v], x0, x1, x2);
+        p0[v] = x0; p1[v] = x1; p2[v] = x2;
+      }
+      const int off = x6_off(bc, 4 * (bkg + 2 * i));
+      *reinterpret_cast<bf16x4_t*>(Bs + off) = p0;
+      *reinterpret_cast<bf16x4_t*>(Bs + PART + off) = p1;
+      *reinterpret_cast<bf16x4_t*>(Bs + 2 * PART + off) = p2;
+    }
+  };
+  const int nk = K / BK;
+  if (nk > 0) {
+    fetch(0);
+    stash();
+  }
+  __syncthreads();
+  for (int kt = 0; kt < nk; kt++) {
+    if (kt + 1 < nk) fetch((kt + 1) * BK);
+    bf16x8_t af[MT][3];
+#pragma unroll
+    for (int a = 0; a < MT; a++)
+#pragma unroll
+      for (int p = 0; p < 3; p++) af[a][p] = *reinterpret_cast<const bf16x8_t*>(As + p * PART + x6_off(wm + a * 16 + li, 8 * lq));
+#pragma unroll
+    for (int b = 0; b < MT; b++) {
+      bf16x8_t bq[3];
+#pragma unroll
+      for (int p = 0; p < 3; p++) bq[p] = *reinterpret_cast<const bf16x8_t*>(Bs + p * PART + x6_off(wn + b * 16 + li, 8 * lq));
+#pragma unroll
+      for (int a = 0; a < MT; a++) {   // smallest products first
+        f4v c = acc[a][b];
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a][2], bq[0], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a][1], bq[1], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a][0], bq[2], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a][1], bq[0], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a][0], bq[1], c, 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a][0], bq[0], c, 0, 0, 0);
+      }
+    }
+    if (kt + 1 < nk) {
+      __syncthreads();   // every wave is done reading the chunk
+      stash();
+    }
+    __syncthreads();
+  }
+  // write back through LDS, 64 tile rows per pass, 8 segments of C in flight per thread
+  const bool use_c = beta != 0.0f;
+#pragma unroll
+  for (int pass = 0; pass < GB / CROWS; pass++) {
+    const int r0 = pass * CROWS;
+    if (pass > 0) __syncthreads();
+#pragma unroll
+    for (int a = 0; a < MT; a++) {
+      if (wm + a * 16 < r0 || wm + a * 16 >= r0 + CROWS) continue;
+#pragma unroll
+      for (int b = 0; b < MT; b++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) Cs[(wm + a * 16 + 4 * lq + v - r0) * SC + wn + b * 16 + li] = alpha * acc[a][b][v];
+    }
+    __syncthreads();
+    f4v cv[PER];
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+      const int idx = tid + j * 256, row = r0 + idx / SEG, c = (idx % SEG) * 4;
+      cv[j] = use_c ? *reinterpret_cast<const f4v*>(C + (size_t)(bm + row) * ldc + bn + c) : f4v{0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+      const int idx = tid + j * 256, row = r0 + idx / SEG, c = (idx % SEG) * 4;
+      const f4v val = *reinterpret_cast<const f4v*>(&Cs[(row - r0) * SC + c]);
+      f4v out;
+#pragma unroll
+      for (int v = 0; v < 4; v++) out[v] = use_c ? fmaf(beta, cv[j][v], val[v]) : val[v];
+      if (row >= sm && c >= sn) *reinterpret_cast<f4v*>(C + (size_t)(bm + row) * ldc + bn + c) = out;
+    }
+  }
 }
 
 // copy a rows x cols block (row-major, separate leading dims)
@@ -1199,7 +1367,9 @@ template <typename T, int GB, bool TM, bool FAST>
 static void launch_gemm(dim3 grid, int M, int N, int K, T alpha, const T* A, size_t lda, const T* B, size_t ldb, T beta, T* C,
                         size_t ldc, hipStream_t st, const int* gate, int gp, unsigned long long* vm) {
   constexpr int BK = sizeof(T) == 8 ? 16 : 32;
-  if constexpr (sizeof(T) == 4 && GB == 128 && FAST)
+  if constexpr (sizeof(T) == 4 && GB == 128 && FAST && !TM)
+    hipLaunchKernelGGL(k_gemm_x6, grid, dim3(256), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, gp);
+  else if constexpr (sizeof(T) == 4 && GB == 128 && FAST)
     hipLaunchKernelGGL((k_gemm_f32<TM>), grid, dim3(256), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, gp, vm);
   else
     hipLaunchKernelGGL((k_gemm<T, BK, TM, GB, FAST>), grid, dim3(256), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, gp, vm);
@@ -1350,13 +1520,14 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
                         size_t lb, T beta, T* Cp, size_t lc, unsigned long long* vmax = nullptr) -> hipError_t {
     if (M <= 0 || Nn <= 0) return hipSuccess;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (tm) {
+    const bool timed = tm != nullptr && !gemm_small(M, Nn);   // the large (trailing) products only
+    if (timed) {
       if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return hipErrorOutOfMemory;
       ev.emplace_back(e0, e1);
       if (hipEventRecord(e0, s) != hipSuccess) return hipErrorLaunchFailure;
     }
     hipError_t r = gemm<T>(M, Nn, K, alpha, Ap, la, Bp, lb, beta, Cp, lc, s, gate, gp, vmax);
-    if (tm) {
+    if (timed) {
       if (hipEventRecord(e1, s) != hipSuccess) return hipErrorLaunchFailure;
       tm->launches++;
       tm->flops += 2.0 * M * (double)Nn * K;
