@@ -187,7 +187,7 @@ def solve_system(M64: torch.Tensor | None, F: np.ndarray | torch.Tensor, M32: to
 
     mode "fp64":  LU of the FP64 matrix (the dgesv of solver.py:173), then, if ``refine``,
                   iterative refinement with double-double residuals against M64.
-    mode "mixed": LU of the FP32 copy M32 (MFMA FP32), FP64 solves, refinement with
+    mode "mixed": LU of the FP32 copy M32 (MFMA; FP32 accuracy), FP64 solves, refinement with
                   double-double residuals against M64.  Refinement converges when
                   cond(M) * eps32 < 1; when it does not reach the rounding level of x (it
                   stalls or diverges) and ``fallback`` is set, the system is solved again
