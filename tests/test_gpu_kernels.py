@@ -163,6 +163,32 @@ def test_lu_random(dev, N):
     assert np.array_equal(lu.ipiv.cpu().numpy(), piv)
 
 
+def test_lu_f32_split_trailing_updates(dev):
+    """N = 4000 in FP32: the first panels' trailing updates are large enough for the
+    128-tile product that splits its FP32 operands into BF16 parts (six BF16 products per
+    FP32 product).  Diagonally dominant with two rows of panel 2 swapped, so the pass also
+    stops at panel 2, factors it with pivoting and resumes.  Pivots equal LAPACK's; the
+    solution agrees with LAPACK's to FP32 rounding (the matrix is well conditioned:
+    cond ~ 1.1)."""
+    import torch
+    import scipy.linalg
+    rng = np.random.default_rng(23)
+    N = 4000
+    A = rng.uniform(-1, 1, (N, N)) + 10 * N ** 0.5 * np.eye(N)
+    k0 = 2 * 128
+    A[[k0 + 5, k0 + 40]] = A[[k0 + 40, k0 + 5]]
+    b = rng.standard_normal((1, N))
+    _, piv = scipy.linalg.lu_factor(A)
+    lu = dev.lu_factor(torch.tensor(A, device="cuda").to(torch.float32), overwrite=True)
+    assert lu.info == 0
+    assert np.array_equal(lu.ipiv.cpu().numpy(), piv)
+    X = torch.tensor(b, device="cuda")
+    dev.lu_solve(lu, X)
+    err = rel(X.cpu().numpy()[0], np.linalg.solve(A, b[0]))
+    print("split trailing updates f32", N, err, flush=True)
+    assert err < 1e-6
+
+
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 def test_lu_pivot_in_a_middle_panel(dev, dtype):
     """Diagonally dominant except two rows of panel 3 (of 8) swapped: the speculative
