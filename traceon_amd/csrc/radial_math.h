@@ -117,12 +117,48 @@ TB_HD double rsqrt_(double x) {
 #endif
 }
 
-TB_HD double ellipkm1_(double p) { double L = neg_log(p); return fma(L, horner_kb(p), horner_ka(p)); }
-TB_HD double ellipem1_(double p) { double L = neg_log(p); return fma(L, horner_eb(p), horner_ea(p)); }
+// The same fits as sums of c_i p^i with the powers of p formed once and shared by the fits
+// of a pair (the reference evaluates its fits with pow).  On gfx950 an FP64 coefficient
+// cannot be an instruction operand, so it sits in a register; in Horner's form it is the
+// FMA's accumulator and each step first copies it, while in a power sum the running sum is
+// the accumulator and the coefficient a multiplicand.  Terms are added from the highest
+// power (the smallest) to the constant.
+struct PPow { double p1, p2, p3, p4, p5, p6, p7; };
+TB_HD PPow ppow(double p) {
+  PPow w;
+  w.p1 = p; w.p2 = p * p; w.p3 = w.p2 * p; w.p4 = w.p2 * w.p2; w.p5 = w.p4 * p; w.p6 = w.p3 * w.p3; w.p7 = w.p6 * p;
+  return w;
+}
+TB_HD double psum8(const PPow& w, double c7, double c6, double c5, double c4, double c3, double c2, double c1, double c0) {
+  double s = c7 * w.p7;
+  s = fma(c6, w.p6, s); s = fma(c5, w.p5, s); s = fma(c4, w.p4, s); s = fma(c3, w.p3, s);
+  s = fma(c2, w.p2, s); s = fma(c1, w.p1, s);
+  return s + c0;
+}
+TB_HD double pka(const PPow& w) {
+  return psum8(w, 1.40704915496101e-3, 1.09423810688623e-2, 1.68695685967517e-2, 1.25565693543211e-2, 1.52618320622534e-2,
+               3.08909633861795e-2, 9.65736020516771e-2, 1.3862943611198906);
+}
+TB_HD double pkb(const PPow& w) {
+  return psum8(w, 3.42805719229748e-4, 5.81807961871996e-3, 2.09857677336790e-2, 3.57218443007327e-2, 4.87379510945218e-2,
+               7.03114105853296e-2, 1.24999998585309e-1, 0.5);
+}
+TB_HD double pea(const PPow& w) {
+  return psum8(w, 1.55618744745296e-3, 1.21819481486695e-2, 1.92284389022977e-2, 1.56847700239786e-2, 2.21862206993846e-2,
+               5.68115681053803e-2, 4.43147193467733e-1, 1.0);
+}
+TB_HD double peb(const PPow& w) {   // p (b6 p^6 + ... + b0)
+  return psum8(w, 0.0, 3.78886487349367e-4, 6.45682247315060e-3, 2.35091602564984e-2, 4.09074821593164e-2,
+               5.84950297066166e-2, 9.37488062098189e-2, 2.49999998448655e-1) * w.p1;
+}
+
+TB_HD double ellipkm1_(double p) { double L = neg_log(p); const PPow w = ppow(p); return fma(L, pkb(w), pka(w)); }
+TB_HD double ellipem1_(double p) { double L = neg_log(p); const PPow w = ppow(p); return fma(L, peb(w), pea(w)); }
 TB_HD void ellip_km1_em1(double p, double& K, double& E) {
   double L = neg_log(p);
-  K = fma(L, horner_kb(p), horner_ka(p));
-  E = fma(L, horner_eb(p), horner_ea(p));
+  const PPow w = ppow(p);
+  K = fma(L, pkb(w), pka(w));
+  E = fma(L, peb(w), pea(w));
 }
 // elliptic.c:37-41, 71-75 (parameter forms used by the current-ring kernel)
 TB_HD double ellipk_(double k) { return k > -1.0 ? ellipkm1_(1.0 - k) : ellipkm1_(1.0 / (1.0 - k)) / sqrt(k); }
