@@ -1120,6 +1120,57 @@ constexpr size_t trsv_lds_bytes() {
   return sizeof(T) * LU_NB * TRSV_PAD + sizeof(double) * LU_NB * (TRSV_RED + 1) + sizeof(double) * LU_NB;
 }
 
+// Coupling blocks of the blocked solves, after the factorisation: for every block row b with
+// a neighbour in the solve order, M_b = inv(L_bb) L_{b,b-1} (forward, L) or inv(U_bb) U_{b,b+1}
+// (backward, U), NB x NB row-major, zero outside the blocks' extents.  Block b's solution is
+// then x_b = inv(D_b) (b_b - sum over the other blocks) - M_b x_neighbour, so the solve of block
+// b can finish everything but one matrix-vector product before its neighbour is known.
+// One workgroup per block; 8 x 8 output tiles per thread, K staged through LDS in chunks of 32;
+// FP64 accumulation, stored as T.
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(256) void k_trsv_coupling(const T* __restrict__ LU, size_t lda, int N, const T* __restrict__ Dinv,
+                                                       T* __restrict__ Mout) {
+  constexpr int NB = LU_NB, KC = 32;
+  const int nblk = (N + NB - 1) / NB, b = blockIdx.x;
+  const int nbr = LOWER ? b - 1 : b + 1;   // the neighbour block
+  if (nbr < 0 || nbr >= nblk) return;
+  const int r0 = b * NB, nr = min(NB, N - r0);
+  const int c0 = nbr * NB, nc = min(NB, N - c0);
+  __shared__ T Ds[NB][KC + 1];   // inv(D_b)[:, k0:k0+KC]
+  __shared__ T Bs[KC][NB + 1];   // coupling block rows k0:k0+KC
+  const int tid = threadIdx.x, ti = tid / 16, tj = tid % 16;   // rows 8 ti.., columns 8 tj..
+  double acc[8][8] = {};
+  const T* D = Dinv + (size_t)b * NB * NB;
+  for (int k0 = 0; k0 < NB; k0 += KC) {
+    for (int e = tid; e < NB * KC; e += 256) {
+      const int i = e / KC, k = e % KC;
+      Ds[i][k] = D[(size_t)i * NB + k0 + k];
+      const int kk = e / NB, j = e % NB;
+      Bs[kk][j] = (k0 + kk < nr && j < nc) ? LU[(size_t)(r0 + k0 + kk) * lda + c0 + j] : T(0);
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int k = 0; k < KC; k++) {
+      double dv[8], bv[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) { dv[u] = (double)Ds[8 * ti + u][k]; bv[u] = (double)Bs[k][8 * tj + u]; }
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+#pragma unroll
+        for (int v = 0; v < 8; v++) acc[u][v] = fma(dv[u], bv[v], acc[u][v]);
+    }
+    __syncthreads();
+  }
+  T* M = Mout + (size_t)b * NB * NB;
+#pragma unroll
+  for (int u = 0; u < 8; u++)
+#pragma unroll
+    for (int v = 0; v < 8; v++) {
+      const int i = 8 * ti + u, j = 8 * tj + v;
+      M[(size_t)i * NB + j] = (i < nr && j < nc) ? (T)acc[u][v] : T(0);
+    }
+}
+
 // out[0..n) = TRSV_PENDING, *counter = 0 (a kernel, not a memset node, so that the solve
 // can be captured in a graph)
 __global__ __launch_bounds__(256) void k_trsv_prep(double* __restrict__ out, int n, int* __restrict__ counter) {
@@ -1136,8 +1187,8 @@ template <typename T, bool LOWER>
 __global__ __launch_bounds__(TRSV_THREADS) void k_trsv_blocked(const T* __restrict__ LU, size_t lda, int N,
                                                                const int* __restrict__ perm, const int* __restrict__ pivoted,
                                                                const double* __restrict__ Bv, double* X,
-                                                               const T* __restrict__ Dinv, int* __restrict__ counter,
-                                                               int* __restrict__ err) {
+                                                               const T* __restrict__ Dinv, const T* __restrict__ Mc,
+                                                               int* __restrict__ counter, int* __restrict__ err) {
   constexpr int NB = LU_NB;
   constexpr int NW = TRSV_THREADS / 64;   // waves
   constexpr int RPW = NB / NW;            // rows per wave (16)
@@ -1149,6 +1200,7 @@ __global__ __launch_bounds__(TRSV_THREADS) void k_trsv_blocked(const T* __restri
   double* part = reinterpret_cast<double*>(smem_raw + sizeof(T) * NB * TRSV_PAD);   // NB x (TRSV_RED + 1)
   double* vs = part + NB * (TRSV_RED + 1);                                         // NB
   __shared__ int s_b, s_fail;
+  __shared__ double s_xn[NW][NB];   // the neighbour block's solution, one copy per wave
   const int nblk = (N + NB - 1) / NB;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (tid == 0) {
@@ -1172,8 +1224,21 @@ __global__ __launch_bounds__(TRSV_THREADS) void k_trsv_blocked(const T* __restri
 #pragma unroll
   for (int i = 0; i < RPW; i++) rowp[i] = LU + (size_t)min(r0 + wave * RPW + i, N - 1) * lda;
   const bool vec_ok = (lda % VEC) == 0 && (((uintptr_t)LU) % 16) == 0;
-  // dependency columns [lo, hi), chunks in solve order
-  const int lo = LOWER ? 0 : r0 + nr, hi = LOWER ? r0 : N;
+  // the neighbour block in solve order (b - 1 for L, b + 1 for U) enters last, through the
+  // coupling block M_b: this thread's share of M_b (row 16w + l%16, columns 32q..32q+31) is
+  // loaded now
+  const int nbr = LOWER ? b - 1 : b + 1;
+  const bool has_nbr = nbr >= 0 && nbr < nblk;
+  const int c0n = nbr * NB, ncn = has_nbr ? min(NB, N - c0n) : 0;
+  const int mrow = wave * RPW + (lane & 15), mq = lane >> 4;
+  vec_t mv[32 / VEC];
+  if (has_nbr) {
+    const T* mp = Mc + (size_t)b * NB * NB + (size_t)mrow * NB + 32 * mq;
+#pragma unroll
+    for (int i = 0; i < 32 / VEC; i++) mv[i] = *reinterpret_cast<const vec_t*>(mp + i * VEC);
+  }
+  // the other dependency columns [lo, hi), chunks in solve order
+  const int lo = LOWER ? 0 : (has_nbr ? c0n + ncn : r0 + nr), hi = LOWER ? (has_nbr ? c0n : r0) : N;
   const int nch = (hi - lo + CH - 1) / CH;
   const int top = LOWER ? 0 : nblk * NB;   // U chunks are aligned to the block grid from its end
   bool failed = false;
@@ -1249,10 +1314,10 @@ __global__ __launch_bounds__(TRSV_THREADS) void k_trsv_blocked(const T* __restri
     vs[tid] = row < N ? bv - ssum : 0.0;
   }
   __syncthreads();
-  // x = inv(D_b) vs: row 16w + l%16, quarter q = l/16 takes columns [16q, 16q+16) and
+  // y = inv(D_b) vs: row 16w + l%16, quarter q = l/16 takes columns [16q, 16q+16) and
   // [64+16q, 80+16q) (the 64 lanes of a step then read 64 distinct banks)
   {
-    const int row = wave * RPW + (lane & 15), q = lane >> 4;
+    const int row = mrow, q = mq;
     const T* Dr = Ds + row * TRSV_PAD;
     double xv = 0.0;
 #pragma unroll
@@ -1264,8 +1329,48 @@ __global__ __launch_bounds__(TRSV_THREADS) void k_trsv_blocked(const T* __restri
       }
     xv += __shfl_xor(xv, 16, 64);
     xv += __shfl_xor(xv, 32, 64);
+    bool wfail = s_fail != 0;
+    if (has_nbr) {
+      // x_b = y - M_b x_nbr: each lane polls two of the neighbour's values, the wave shares
+      // them through its own LDS row (no workgroup barrier on this path)
+      const int ca = c0n + 2 * lane;
+      double xa = 0.0, xb = 0.0;
+      long long spins = 0;
+      while (!wfail) {
+        bool pend = false;
+        if (ca < c0n + ncn) {
+          xa = __hip_atomic_load(X + ca, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          pend = trsv_pending(xa);
+        }
+        if (ca + 1 < c0n + ncn) {
+          xb = __hip_atomic_load(X + ca + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          pend = pend || trsv_pending(xb);
+        }
+        if (!__any(pend)) break;
+        if (++spins > SPIN_LIMIT) {
+          wfail = true;
+          if (lane == 0) atomicOr(err, 1);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+      double* xs = s_xn[wave];
+      xs[2 * lane] = ca < c0n + ncn ? xa : 0.0;
+      xs[2 * lane + 1] = ca + 1 < c0n + ncn ? xb : 0.0;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      double z = 0.0;
+#pragma unroll
+      for (int i = 0; i < 32 / VEC; i++)
+#pragma unroll
+        for (int v = 0; v < VEC; v++) z = fma((double)mv[i][v], xs[32 * q + i * VEC + v], z);
+      z += __shfl_xor(z, 16, 64);
+      z += __shfl_xor(z, 32, 64);
+      xv -= z;
+    }
     if (q == 0 && row < nr) {
-      if (s_fail) xv = __builtin_nan("");
+      if (wfail) xv = __builtin_nan("");
       __hip_atomic_store(X + r0 + row, xv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
@@ -1414,6 +1519,7 @@ struct LuWorkspace {
   T* Y; T* Z;                                     // scratch inverses of the current panel
   T* U12; T* U12b;                                // U12 of the current panel (two, alternating by panel)
   T* Linv; T* Uinv;                               // inverses of every diagonal block (nblk x NB x NB)
+  T* Lm; T* Um;                                   // coupling blocks of the solves: inv(L_bb) L_b,b-1, inv(U_bb) U_b,b+1 (nblk x NB x NB)
   int* perm; int* flags; int* err; int* pivoted;  // permutation, solve tickets, error, any-interchange flag
   double* rhs;                                    // copy of the right-hand sides (nrhs x N, nrhs <= 64)
   double* ytmp;                                   // L^{-1} P b of the solve in progress (N)
@@ -1433,6 +1539,7 @@ struct LuWorkspace {
     size_t a_y = take((size_t)NB * NB * sizeof(T)), a_z = take((size_t)NB * NB * sizeof(T));
     size_t a_u = take((size_t)NB * N * sizeof(T)), a_pb = take((size_t)NB * N * sizeof(T));
     size_t a_li = take((size_t)nblk * NB * NB * sizeof(T)), a_ui = take((size_t)nblk * NB * NB * sizeof(T));
+    size_t a_lc = take((size_t)nblk * NB * NB * sizeof(T)), a_uc = take((size_t)nblk * NB * NB * sizeof(T));
     size_t a_pm = take((size_t)N * sizeof(int)), a_fl = take((2 * (size_t)nblk + 2) * sizeof(int));
     size_t a_er = take(sizeof(int)), a_pv = take(sizeof(int)), a_rhs = take(64 * (size_t)N * sizeof(double)), a_lm = take(8);
     size_t a_yt = take((size_t)N * sizeof(double));
@@ -1444,6 +1551,7 @@ struct LuWorkspace {
     cand_val = (double*)(p + a_cv); cand_idx = (int*)(p + a_ci); cand_row = (T*)(p + a_cr);
     Y = (T*)(p + a_y); Z = (T*)(p + a_z); U12 = (T*)(p + a_u); U12b = (T*)(p + a_pb);
     Linv = (T*)(p + a_li); Uinv = (T*)(p + a_ui); perm = (int*)(p + a_pm); flags = (int*)(p + a_fl);
+    Lm = (T*)(p + a_lc); Um = (T*)(p + a_uc);
     err = (int*)(p + a_er); pivoted = (int*)(p + a_pv); rhs = (double*)(p + a_rhs); lmax = (unsigned long long*)(p + a_lm);
     ytmp = (double*)(p + a_yt);
     Ltmp = (T*)(p + a_lt); lmaxv = (unsigned long long*)(p + a_lv); first_fail = (int*)(p + a_ff); Dbak = (T*)(p + a_db);
@@ -1670,6 +1778,8 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
     q = kf + 1;
   }
   hipLaunchKernelGGL(k_diag_check<T>, dim3(1), dim3(1024), 0, st, A, lda, N, info_dev);
+  hipLaunchKernelGGL((k_trsv_coupling<T, true>), dim3(nblk), dim3(256), 0, st, A, lda, N, W.Linv, W.Lm);
+  hipLaunchKernelGGL((k_trsv_coupling<T, false>), dim3(nblk), dim3(256), 0, st, A, lda, N, W.Uinv, W.Um);
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, W.pivoted, any_pivot ? 1 : 0);
   if (any_pivot) {
     if ((size_t)N * sizeof(int) > 150 * 1024) return hipErrorInvalidValue;
@@ -1703,10 +1813,10 @@ hipError_t lu_solve(const T* LU, size_t lda, int N, const int* ipiv, double* X, 
     double* xr = X + (size_t)r * N;
     hipLaunchKernelGGL(k_trsv_prep, dim3(pgrid), dim3(256), 0, st, W.ytmp, N, W.flags);
     hipLaunchKernelGGL((k_trsv_blocked<T, true>), dim3(nblk), dim3(TRSV_THREADS), lds, st, LU, lda, N, W.perm, W.pivoted,
-                       W.rhs + (size_t)r * N, W.ytmp, W.Linv, W.flags, W.err);
+                       W.rhs + (size_t)r * N, W.ytmp, W.Linv, W.Lm, W.flags, W.err);
     hipLaunchKernelGGL(k_trsv_prep, dim3(pgrid), dim3(256), 0, st, xr, N, W.flags + 1);
     hipLaunchKernelGGL((k_trsv_blocked<T, false>), dim3(nblk), dim3(TRSV_THREADS), lds, st, LU, lda, N, (const int*)nullptr,
-                       (const int*)nullptr, W.ytmp, xr, W.Uinv, W.flags + 1, W.err);
+                       (const int*)nullptr, W.ytmp, xr, W.Uinv, W.Um, W.flags + 1, W.err);
   }
   return hipGetLastError();
 }
