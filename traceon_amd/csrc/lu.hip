@@ -837,6 +837,35 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, cons
                                               const int* gate, int gp, unsigned long long* __restrict__ vmax) {
   gemm_tile<T, BK, TRACK_MAX, GB, FAST>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, gp, vmax);
 }
+// U12 = inv(L11) A12 on 64-tiles, with the panel's no-pivot test in front (what
+// k_panel_check does, without a launch of its own): every workgroup reduces the max |l|
+// of the diagonal block (*lmax) and of the L21 product's workgroups (slots); above 1 (or
+// NaN) the panel fails: workgroup (0, 0) records it in first_fail and no workgroup writes.
+// All workgroups reach the same verdict from the same values.
+template <typename T, bool FAST>
+__global__ __launch_bounds__(256) void k_gemm_checked(int M, int N, int K, T alpha, const T* __restrict__ A, size_t lda,
+                                                      const T* __restrict__ B, size_t ldb, T beta, T* __restrict__ C, size_t ldc,
+                                                      int* first_fail, int gp, const unsigned long long* __restrict__ lmax,
+                                                      const unsigned long long* __restrict__ slots, int nslot) {
+  __shared__ unsigned long long red[256];
+  if (gated_off(first_fail, gp)) return;
+  unsigned long long m = threadIdx.x == 0 ? *lmax : 0ull;
+  for (int i = threadIdx.x; i < nslot; i += 256) m = slots[i] > m ? slots[i] : m;
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o && red[threadIdx.x + o] > red[threadIdx.x]) red[threadIdx.x] = red[threadIdx.x + o];
+    __syncthreads();
+  }
+  const bool fail = red[0] > 0x3FF0000000000000ull;
+  __syncthreads();   // red is done with before the tile reuses nothing of it; keeps the verdict uniform
+  if (fail) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && blockIdx.y == 0) atomicMin(first_fail, gp);
+    return;
+  }
+  gemm_tile<T, sizeof(T) == 8 ? 16 : 32, false, 64, FAST>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, 0, nullptr);
+}
+
 // FP32 128-tiles: with this register budget the compiler keeps the accumulators in
 // architectural VGPRs and allocates 144 registers per lane in all, so three workgroups fit a
 // SIMD's 512 and, when one of them ends, the diagonal-block kernel of the next panel (one
@@ -1025,6 +1054,24 @@ __global__ __launch_bounds__(256) void k_copy2d(const T* __restrict__ S, size_t 
   if (e >= (size_t)rows * cols) return;
   int r = (int)(e / cols), c = (int)(e % cols);
   D[(size_t)r * ldd + c] = S[(size_t)r * lds + c];
+}
+
+// the factors of a panel into the matrix: L21 (rows x nb, ld nb) and U12 (nb x cols, ld cols)
+// in one launch
+template <typename T>
+__global__ __launch_bounds__(256) void k_copy_panel(const T* __restrict__ L, T* __restrict__ A21, const T* __restrict__ U,
+                                                    T* __restrict__ A12, size_t lda, int rows, int cols, int nb, const int* gate,
+                                                    int gp) {
+  if (gated_off(gate, gp)) return;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x, nl = (size_t)rows * nb;
+  if (e < nl) {
+    const int r = (int)(e / nb), c = (int)(e % nb);
+    A21[(size_t)r * lda + c] = L[e];
+  } else if (e < nl + (size_t)nb * cols) {
+    const size_t f = e - nl;
+    const int r = (int)(f / cols), c = (int)(f % cols);
+    A12[(size_t)r * lda + c] = U[f];
+  }
 }
 
 __global__ __launch_bounds__(256) void k_d2f(const double* __restrict__ S, size_t lds, float* __restrict__ D, size_t ldd, int rows,
@@ -1507,6 +1554,27 @@ hipError_t gemm(int M, int N, int K, T alpha, const T* A, size_t lda, const T* B
 }
 template hipError_t gemm<double>(int, int, int, double, const double*, size_t, const double*, size_t, double, double*, size_t,
                                  hipStream_t, const int*, int, unsigned long long*);
+
+// U12 product with the panel check in front (k_gemm_checked); products of fewer than 512
+// 128-tiles only (64-tiles)
+template <typename T>
+static hipError_t gemm_checked(int M, int N, int K, T alpha, const T* A, size_t lda, const T* B, size_t ldb, T beta, T* C, size_t ldc,
+                               hipStream_t st, int* first_fail, int gp, const unsigned long long* lmax,
+                               const unsigned long long* slots, int nslot) {
+  if (M <= 0 || N <= 0) return hipSuccess;
+  constexpr int BK = sizeof(T) == 8 ? 16 : 32;
+  constexpr size_t VEC = 16 / sizeof(T);
+  const bool fast = lda % VEC == 0 && (uintptr_t)A % 16 == 0 && ldb % VEC == 0 && (uintptr_t)B % 16 == 0 && ldc % VEC == 0 &&
+                    (uintptr_t)C % 16 == 0 && K % BK == 0 && N % VEC == 0 && M >= 64 && N >= 64;
+  const dim3 grid(cdiv(N, 64), cdiv(M, 64));
+  if (fast)
+    hipLaunchKernelGGL((k_gemm_checked<T, true>), grid, dim3(256), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, first_fail,
+                       gp, lmax, slots, nslot);
+  else
+    hipLaunchKernelGGL((k_gemm_checked<T, false>), grid, dim3(256), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, first_fail,
+                       gp, lmax, slots, nslot);
+  return hipGetLastError();
+}
 template hipError_t gemm<float>(int, int, int, float, const float*, size_t, const float*, size_t, float, float*, size_t, hipStream_t,
                                 const int*, int, unsigned long long*);
 
@@ -1706,13 +1774,17 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
         if ((r = timed_gemm(sa, g, q, m2, nb, nb, T(1), A21, lda, Zp(q), NB, T(0), Lbuf(q), (size_t)nb, W.lslots)) != hipSuccess)
           return r;
       }
-      hipLaunchKernelGGL(k_panel_check, dim3(1), dim3(256), 0, sa, W.lmaxv + q, W.lslots, nslot, W.first_fail, q);
-      if (n2 <= 0) break;
+      if (n2 <= 0) {   // the last panel: its test alone (otherwise it runs in front of U12)
+        hipLaunchKernelGGL(k_panel_check, dim3(1), dim3(256), 0, sa, W.lmaxv + q, W.lslots, nslot, W.first_fail, q);
+        break;
+      }
       const int nb1 = min(NB, n2);
       const T* A12 = A + (size_t)k0 * lda + k1;
       // U(q) after Tr(q-1) of the bulk stream
       if (q > s0 && (r = hipStreamWaitEvent(sa, ss->ev_r, 0)) != hipSuccess) return r;
-      if ((r = timed_gemm(sa, g, q, nb, n2, nb, T(1), Yp(q), NB, A12, lda, T(0), Ubuf(q), (size_t)n2)) != hipSuccess) return r;
+      if ((r = gemm_checked<T>(nb, n2, nb, T(1), Yp(q), NB, A12, lda, T(0), Ubuf(q), (size_t)n2, sa, W.first_fail, q, W.lmaxv + q,
+                               W.lslots, nslot)) != hipSuccess)
+        return r;
       if ((r = hipEventRecord(ss->ev_u, sa)) != hipSuccess) return r;
       // Tc(q): the next panel's column
       if ((r = timed_gemm(sa, g, q, n2, nb1, nb, T(-1), Lbuf(q), (size_t)nb, Ubuf(q), (size_t)n2, T(1), A + (size_t)k1 * lda + k1,
@@ -1723,10 +1795,8 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
       if ((r = timed_gemm(st, g, q, n2, n2 - nb1, nb, T(-1), Lbuf(q), (size_t)nb, Ubuf(q) + nb1, (size_t)n2, T(1),
                           A + (size_t)k1 * lda + k1 + nb1, lda)) != hipSuccess)
         return r;
-      hipLaunchKernelGGL(k_copy2d<T>, dim3(cdiv((size_t)m2 * nb, 256)), dim3(256), 0, st, Lbuf(q), (size_t)nb,
-                         A + (size_t)k1 * lda + k0, lda, m2, nb, g, q);
-      hipLaunchKernelGGL(k_copy2d<T>, dim3(cdiv((size_t)nb * n2, 256)), dim3(256), 0, st, Ubuf(q), (size_t)n2,
-                         A + (size_t)k0 * lda + k1, lda, nb, n2, g, q);
+      hipLaunchKernelGGL(k_copy_panel<T>, dim3(cdiv((size_t)m2 * nb + (size_t)nb * n2, 256)), dim3(256), 0, st, Lbuf(q),
+                         A + (size_t)k1 * lda + k0, Ubuf(q), A + (size_t)k0 * lda + k1, lda, m2, n2, nb, g, q);
       if ((r = hipEventRecord(ss->ev_r, st)) != hipSuccess) return r;
     }
     if ((r = hipEventRecord(ss->ev_end, sa)) != hipSuccess) return r;
