@@ -97,7 +97,19 @@ TB_HD double neg_log(double p) {
   int e;
   double m = frexp(p, &e);            // m in [0.5, 1)
   if (m < 0.70710678118654752440) { m *= 2.0; e -= 1; }
+#if defined(__HIP_DEVICE_COMPILE__)
+  // (m - 1) / (m + 1) with the denominator in [1.7, 2.5]: the hardware reciprocal refined by
+  // two Newton steps and the quotient corrected once (no scaling or special cases in this
+  // range, which the general division sequence spends four instructions on)
+  const double dn = m + 1.0, nm = m - 1.0;
+  double y = __builtin_amdgcn_rcp(dn);
+  y = fma(fma(-dn, y, 1.0), y, y);
+  y = fma(fma(-dn, y, 1.0), y, y);
+  double s = nm * y;
+  s = fma(fma(-dn, s, nm), y, s);
+#else
   const double s = (m - 1.0) / (m + 1.0);
+#endif
   const double s2 = s * s;
   double q = 1.0 / 19.0;
   q = fma(q, s2, 1.0 / 17.0);
@@ -110,9 +122,14 @@ TB_HD double neg_log(double p) {
   return -(fma(de, 6.93147180369123816490e-01, fma(de, 1.90821492927058770002e-10, lnm)));
 }
 
+// 1/sqrt(x) for positive finite x (the callers pass sums of squares of a ring geometry): the
+// hardware estimate and one third-order correction, y (1 + e/2 + 3e^2/8) with
+// e = 1 - x y^2, without the library's special-value selects
 TB_HD double rsqrt_(double x) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  return rsqrt(x);
+  const double y = __builtin_amdgcn_rsq(x);
+  const double e = fma(-x * y, y, 1.0);
+  return fma(y * e, fma(e, 0.375, 0.5), y);
 #else
   return 1.0 / sqrt(x);
 #endif
