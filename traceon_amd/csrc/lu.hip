@@ -277,6 +277,9 @@ __global__ __launch_bounds__(256) void k_diag_block(T* __restrict__ A, size_t ld
   constexpr int NB = LU_NB, DS = diag_row_stride<T>(), PER = NB * NB / 256, CH = 16, VEC = 16 / (int)sizeof(T), PERV = PER / VEC;
   typedef T vec_t __attribute__((ext_vector_type(VEC)));
   if (gated_off(gate, gp)) return;
+  // this kernel is on the factorisation's critical path and shares its CU with trailing-update
+  // workgroups: its waves take issue priority over theirs
+  __builtin_amdgcn_s_setprio(3);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* S = reinterpret_cast<T*>(smem_raw);   // element (r, c) at r * DS + c
   __shared__ T udinv[NB];
@@ -643,6 +646,9 @@ __device__ __forceinline__ void gemm_tile(int M, int N, int K, T alpha, const T*
                                           const int* gate, int gp, unsigned long long* __restrict__ vmax) {
   typedef typename MfmaAcc<T>::type acc_t;
   if (gated_off(gate, gp)) return;
+  // the 64-tile products are the per-panel products of the factorisation's critical path,
+  // run beside trailing-update workgroups: their waves take issue priority
+  if constexpr (GB == 64) __builtin_amdgcn_s_setprio(3);
   constexpr int MT = GB / 32;                             // 16 x 16 MFMA tiles per wave and dimension
   constexpr int VEC = 16 / sizeof(T);                     // elements per 16-byte load
   constexpr int A_LD = (GB * BK) / (256 * VEC);           // vector loads per thread for A
