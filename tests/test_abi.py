@@ -123,3 +123,17 @@ def test_clockwise_path_is_flipped_outward():
     c = circle_mesh(3.0, 2.0, 1.0, "c", 40)
     assert c._normals_outward("c")
     assert np.allclose(np.linalg.norm(c.points[:, [0, 2]] - [3.0, 2.0], axis=1), 1.0)
+
+
+def test_library_binds_to_torchs_hip_runtime():
+    """Importing the bindings before torch still leaves one HIP and one HSA runtime in the
+    process (torch's), the one whose streams and allocations the library is handed."""
+    import subprocess
+    import sys
+    code = ("import traceon_amd.backend\nimport torch\n"
+            "libs = {l.split()[-1] for l in open('/proc/self/maps') if 'libamdhip64' in l or 'libhsa-runtime64' in l}\n"
+            "print(len(libs)); print(sorted(libs))\n")
+    out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    n, libs = out.stdout.splitlines()[:2]
+    assert n == "2", libs

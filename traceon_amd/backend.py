@@ -22,6 +22,12 @@ LIB_PATH = os.path.join(HERE, "libtraceon_hip.so")
 if not os.path.isfile(LIB_PATH):
     raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m traceon_amd.build` (hipcc, gfx950)")
 
+# PyTorch ships its own HIP and HSA runtimes.  The library's calls take torch's streams and
+# allocations, so it must bind to those same runtimes: loaded after torch, its HIP dependency
+# resolves to torch's copy; loaded first, the process would hold two runtimes and the second
+# one to open the device finds none.
+import torch  # noqa: E402,F401
+
 backend_lib = C.CDLL(LIB_PATH)
 
 TRACING_BLOCK_SIZE = C.c_size_t.in_dll(backend_lib, "TRACING_BLOCK_SIZE").value
