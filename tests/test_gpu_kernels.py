@@ -145,8 +145,21 @@ def test_solve_vs_numpy(dev, geo, oracle):
     assert e1 <= 1e-14 and em <= 1e-14        # refined: the correctly rounded charges
 
 
+# the factorisation takes panels in pairs (one bulk update at K = 256 for two panels) while
+# the trailing matrix has at least TRACEON_LU_PAIR_ROWS rows (4096 by default); "0" runs the
+# same matrices with pairs throughout
+PAIR_MODES = [None, "0"]
+
+
+@pytest.fixture(params=PAIR_MODES, ids=["pairs-large", "pairs-all"])
+def pair_rows(request, monkeypatch):
+    if request.param is not None:
+        monkeypatch.setenv("TRACEON_LU_PAIR_ROWS", request.param)
+    return request.param
+
+
 @pytest.mark.parametrize("N", [300, 1000])
-def test_lu_random(dev, N):
+def test_lu_random(dev, N, pair_rows):
     import torch
     rng = np.random.default_rng(N)
     A = rng.standard_normal((N, N))
@@ -163,11 +176,12 @@ def test_lu_random(dev, N):
     assert np.array_equal(lu.ipiv.cpu().numpy(), piv)
 
 
-def test_lu_f32_split_trailing_updates(dev):
+def test_lu_f32_split_trailing_updates(dev, pair_rows):
     """N = 4000 in FP32: the first panels' trailing updates are large enough for the
     128-tile product that splits its FP32 operands into BF16 parts (six BF16 products per
     FP32 product).  Diagonally dominant with two rows of panel 2 swapped, so the pass also
-    stops at panel 2, factors it with pivoting and resumes.  Pivots equal LAPACK's; the
+    stops at panel 2 (with pairs: the first panel of the pair 2, 3), factors it with
+    pivoting and resumes.  Pivots equal LAPACK's; the
     solution agrees with LAPACK's to FP32 rounding (the matrix is well conditioned:
     cond ~ 1.1)."""
     import torch
@@ -190,11 +204,13 @@ def test_lu_f32_split_trailing_updates(dev):
 
 
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
-def test_lu_pivot_in_a_middle_panel(dev, dtype):
+def test_lu_pivot_in_a_middle_panel(dev, dtype, pair_rows):
     """Diagonally dominant except two rows of panel 3 (of 8) swapped: the speculative
     no-pivot pass accepts panels 0-2, fails at panel 3 (a multiplier > 1), restores that
-    panel's diagonal block, factors it with partial pivoting and resumes.  Pivots equal
-    LAPACK's (one interchange, in panel 3) and the solution is LAPACK's."""
+    panel's diagonal block, factors it with partial pivoting and resumes.  With pairs,
+    panel 3 is the second panel of the pair 2, 3, so panel 2's update of the matrix beyond
+    the pair is applied after the pass.  Pivots equal LAPACK's (one interchange, in panel
+    3) and the solution is LAPACK's."""
     import torch
     import scipy.linalg
     rng = np.random.default_rng(17)

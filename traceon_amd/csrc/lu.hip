@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_cooperative_groups.h>
 #include <stdint.h>
+#include <cstdlib>
 #include <mutex>
 #include <utility>
 #include <vector>
@@ -757,9 +758,12 @@ __device__ __forceinline__ void gemm_tile(int M, int N, int K, T alpha, const T*
     __syncthreads();
   }
 
-  // write back through LDS, CROWS tile rows per pass
+  // write back through LDS, CROWS tile rows per pass.  The C of a large product (128-tiles, the
+  // trailing updates) is read once and written once: it streams past the caches (non-temporal),
+  // which leaves L2 to the A and B panels that the other tiles read again
   const bool use_c = beta != T(0);
   constexpr int BATCH = PER < 8 ? PER : 8;
+  constexpr bool STREAM_C = GB == 128 && !TRACK_MAX;
   double vm = 0.0;
   auto track = [&](T x) {
     if constexpr (TRACK_MAX) {
@@ -787,7 +791,11 @@ __device__ __forceinline__ void gemm_tile(int M, int N, int K, T alpha, const T*
       for (int j = 0; j < BATCH; j++) {
         const int idx = tid + (i0 + j) * 256, row = r0 + idx / SEG, c = (idx % SEG) * VEC;
         if constexpr (FAST) {
-          cvv[j] = use_c ? *reinterpret_cast<const vec_t*>(C + (size_t)(bm + row) * ldc + bn + c) : vec_t{};
+          const vec_t* cp = reinterpret_cast<const vec_t*>(C + (size_t)(bm + row) * ldc + bn + c);
+          if constexpr (STREAM_C)
+            cvv[j] = use_c ? __builtin_nontemporal_load(cp) : vec_t{};
+          else
+            cvv[j] = use_c ? *cp : vec_t{};
         } else {
 #pragma unroll
           for (int v = 0; v < VEC; v++)
@@ -805,7 +813,11 @@ __device__ __forceinline__ void gemm_tile(int M, int N, int K, T alpha, const T*
           if (row >= sm && c >= sn) {
 #pragma unroll
             for (int v = 0; v < VEC; v++) track(out[v]);
-            *reinterpret_cast<vec_t*>(C + (size_t)(bm + row) * ldc + bn + c) = out;
+            vec_t* cp = reinterpret_cast<vec_t*>(C + (size_t)(bm + row) * ldc + bn + c);
+            if constexpr (STREAM_C)
+              __builtin_nontemporal_store(out, cp);
+            else
+              *cp = out;
           }
         } else {
 #pragma unroll
@@ -989,8 +1001,7 @@ __global__ __launch_bounds__(256) void k_gemm_x6(int M, int N, int K, float alph
     stash();
   }
   __syncthreads();
-  for (int kt = 0; kt < nk; kt++) {
-    if (kt + 1 < nk) fetch((kt + 1) * BK);
+  auto mma_chunk = [&]() {
     bf16x8_t af[MT][3];
 #pragma unroll
     for (int a = 0; a < MT; a++)
@@ -1012,13 +1023,19 @@ __global__ __launch_bounds__(256) void k_gemm_x6(int M, int N, int K, float alph
         acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a][0], bq[0], c, 0, 0, 0);
       }
     }
+  };
+  for (int kt = 0; kt < nk; kt++) {
+    if (kt + 1 < nk) fetch((kt + 1) * BK);
+    mma_chunk();
     if (kt + 1 < nk) {
       __syncthreads();   // every wave is done reading the chunk
       stash();
     }
     __syncthreads();
   }
-  // write back through LDS, 64 tile rows per pass, 8 segments of C in flight per thread
+  // write back through LDS, 64 tile rows per pass, 8 segments of C in flight per thread.  C is
+  // read once and written once: it streams past the caches (non-temporal), which leaves L2 to
+  // the A and B panels that the other tiles of the launch read again
   const bool use_c = beta != 0.0f;
 #pragma unroll
   for (int pass = 0; pass < GB / CROWS; pass++) {
@@ -1037,7 +1054,7 @@ __global__ __launch_bounds__(256) void k_gemm_x6(int M, int N, int K, float alph
 #pragma unroll
     for (int j = 0; j < PER; j++) {
       const int idx = tid + j * 256, row = r0 + idx / SEG, c = (idx % SEG) * 4;
-      cv[j] = use_c ? *reinterpret_cast<const f4v*>(C + (size_t)(bm + row) * ldc + bn + c) : f4v{0, 0, 0, 0};
+      cv[j] = use_c ? __builtin_nontemporal_load(reinterpret_cast<const f4v*>(C + (size_t)(bm + row) * ldc + bn + c)) : f4v{0, 0, 0, 0};
     }
 #pragma unroll
     for (int j = 0; j < PER; j++) {
@@ -1046,7 +1063,7 @@ __global__ __launch_bounds__(256) void k_gemm_x6(int M, int N, int K, float alph
       f4v out;
 #pragma unroll
       for (int v = 0; v < 4; v++) out[v] = use_c ? fmaf(beta, cv[j][v], val[v]) : val[v];
-      if (row >= sm && c >= sn) *reinterpret_cast<f4v*>(C + (size_t)(bm + row) * ldc + bn + c) = out;
+      if (row >= sm && c >= sn) __builtin_nontemporal_store(out, reinterpret_cast<f4v*>(C + (size_t)(bm + row) * ldc + bn + c));
     }
   }
 }
@@ -1065,18 +1082,18 @@ __global__ __launch_bounds__(256) void k_copy2d(const T* __restrict__ S, size_t 
 // the factors of a panel into the matrix: L21 (rows x nb, ld nb) and U12 (nb x cols, ld cols)
 // in one launch
 template <typename T>
-__global__ __launch_bounds__(256) void k_copy_panel(const T* __restrict__ L, T* __restrict__ A21, const T* __restrict__ U,
-                                                    T* __restrict__ A12, size_t lda, int rows, int cols, int nb, const int* gate,
-                                                    int gp) {
+__global__ __launch_bounds__(256) void k_copy_panel(const T* __restrict__ L, size_t ldl, T* __restrict__ A21, const T* __restrict__ U,
+                                                    size_t ldu, T* __restrict__ A12, size_t lda, int rows, int cols, int nb,
+                                                    const int* gate, int gp) {
   if (gated_off(gate, gp)) return;
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x, nl = (size_t)rows * nb;
   if (e < nl) {
     const int r = (int)(e / nb), c = (int)(e % nb);
-    A21[(size_t)r * lda + c] = L[e];
+    A21[(size_t)r * lda + c] = L[(size_t)r * ldl + c];
   } else if (e < nl + (size_t)nb * cols) {
     const size_t f = e - nl;
     const int r = (int)(f / cols), c = (int)(f % cols);
-    A12[(size_t)r * lda + c] = U[f];
+    A12[(size_t)r * lda + c] = U[(size_t)r * ldu + c];
   }
 }
 
@@ -1598,6 +1615,9 @@ struct LuWorkspace {
   double* rhs;                                    // copy of the right-hand sides (nrhs x N, nrhs <= 64)
   double* ytmp;                                   // L^{-1} P b of the solve in progress (N)
   T* Ltmp;                                        // L21 = A21 inv(U11) of the speculative pass (2 x N x NB, by panel parity)
+  T* Lp; T* Up;                                   // L21 and U12 of a panel pair: [L_q | L_q+1] (N x 2NB, rows absolute) and
+                                                  // [U_q ; U_q+1] (2NB x ldu, columns absolute), two of each by pair parity
+  size_t ldu;
   T* Dbak;                                        // the current diagonal block before its factorisation (NB x NB)
   unsigned long long* lmaxv;                      // max |l| per panel of the speculative pass (nblk)
   int* first_fail;                                // first panel of that pass that failed the no-pivot test
@@ -1619,6 +1639,8 @@ struct LuWorkspace {
     size_t a_yt = take((size_t)N * sizeof(double));
     size_t a_lt = take(2 * (size_t)N * NB * sizeof(T)), a_lv = take((size_t)nblk * 8), a_ff = take(sizeof(int));
     size_t a_db = take((size_t)NB * NB * sizeof(T));
+    ldu = ((size_t)N + 31) / 32 * 32;
+    size_t a_lp = take(2 * (size_t)N * 2 * NB * sizeof(T)), a_up = take(2 * (size_t)2 * NB * ldu * sizeof(T));
     size_t a_ls = take(((size_t)N / 32 + 64) * 8);
     bytes = o;
     if (!base) return;
@@ -1629,6 +1651,7 @@ struct LuWorkspace {
     err = (int*)(p + a_er); pivoted = (int*)(p + a_pv); rhs = (double*)(p + a_rhs); lmax = (unsigned long long*)(p + a_lm);
     ytmp = (double*)(p + a_yt);
     Ltmp = (T*)(p + a_lt); lmaxv = (unsigned long long*)(p + a_lv); first_fail = (int*)(p + a_ff); Dbak = (T*)(p + a_db);
+    Lp = (T*)(p + a_lp); Up = (T*)(p + a_up);
     lslots = (unsigned long long*)(p + a_ls);
   }
 };
@@ -1666,6 +1689,14 @@ static hipError_t side_stream(int dev, SideStream** out) {
   return hipSuccess;
 }
 
+// Panel pairs (one bulk update at K = 2 NB for two panels) are used while the trailing matrix
+// has at least this many rows; TRACEON_LU_PAIR_ROWS overrides it (0: pairs throughout, a
+// value above N: one panel at a time).
+static int lu_pair_rows() {
+  const char* e = getenv("TRACEON_LU_PAIR_ROWS");
+  return e && *e ? atoi(e) : 4096;
+}
+
 // Factorisation driver.  Most BEM panels need no row interchange, so the factorisation
 // first runs speculatively without pivoting and without host round trips.  Per panel p
 // (side stream A unless stated):
@@ -1686,6 +1717,12 @@ static hipError_t side_stream(int dev, SideStream** out) {
 // speculative pass resumes at f + 1.  A panel that passes the test (every |l| <= 1) is
 // exactly the partially pivoted result: its diagonal entries are the column maxima, so its
 // ipiv is the identity.
+// While the trailing matrix is large the pass takes panels in pairs (q, q+1): the side stream
+// factors both (P, L21, U12 of q; q's update of q+1's column and row; P, L21, U12 of q+1) and
+// then updates the next pair's columns and rows with both panels at once (K = 2 NB); the
+// bulk stream applies the pair's update to the rest of the matrix in one product, so that
+// trailing C is read and written once per two panels.  A failure at q+1 leaves q's update
+// beyond the pair unapplied; the driver applies it before the pivoted panel.
 template <typename T>
 hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev, hipStream_t st, int panel_groups,
                      KernelTiming* tm) {
@@ -1757,9 +1794,17 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
   auto Yp = [&](int q) { return W.Linv + (size_t)q * NB * NB; };
   auto Zp = [&](int q) { return W.Uinv + (size_t)q * NB * NB; };
 
-  // ---- speculative pass from panel s0; *kf = first failed panel, or nblk ----
-  auto speculate = [&](int s0, int* kf) -> hipError_t {
+  // ---- speculative pass from panel s0; *kf = first failed panel, or nblk; *kpair = the first
+  //      panel of the pair whose second panel is *kf, or -1 (panel *kpair's update of the
+  //      matrix beyond the pair is then still to apply) ----
+  constexpr int NB2 = 2 * NB;
+  const size_t ldu = W.ldu;
+  const int pair_min = lu_pair_rows();
+  auto Lpr = [&](int pi) { return W.Lp + (size_t)(pi & 1) * N * NB2; };     // [L_q | L_q+1], row r at r * NB2
+  auto Upr = [&](int pi) { return W.Up + (size_t)(pi & 1) * NB2 * ldu; };   // [U_q ; U_q+1], column c at c
+  auto speculate = [&](int s0, int* kf, int* kpair, int* kpar) -> hipError_t {
     const int* g = W.first_fail;
+    *kpair = -1;
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, W.first_fail, nblk);
     hipLaunchKernelGGL(k_zero_u64, dim3(cdiv(nblk, 256)), dim3(256), 0, st, W.lmaxv, nblk);
     hipError_t r = hipEventRecord(ss->ev_start, st);
@@ -1767,26 +1812,77 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
     if ((r = hipStreamWaitEvent(sa, ss->ev_start, 0)) != hipSuccess) return r;
     auto Lbuf = [&](int q) { return W.Ltmp + (size_t)(q & 1) * N * NB; };
     auto Ubuf = [&](int q) { return (q & 1) ? W.U12b : U12; };
-    for (int q = s0; q < nblk; q++) {
-      const int k0 = q * NB, nb = min(NB, N - k0), m2 = N - k0 - nb, n2 = m2;
-      const int k1 = k0 + nb;
-      // P(q)
+    // P(q) and L21(q) = A21 inv(U11) into C (ldc) with max |l| per workgroup; returns the slot count
+    auto diag_l21 = [&](int q, T* C, size_t ldc, int* nslot) -> hipError_t {
+      const int k0 = q * NB, nb = min(NB, N - k0), k1 = k0 + nb, m2 = N - k1;
       hipLaunchKernelGGL(k_diag_block<T>, dim3(1), dim3(256), diag_lds_bytes<T>(), sa, A, lda, k0, nb, 1, ipiv, Yp(q), Zp(q),
                          W.lmaxv + q, g, q, W.Dbak);
+      *nslot = 0;
+      if (m2 <= 0) return hipSuccess;
+      *nslot = (int)gemm_tiles(m2, nb);
+      return timed_gemm(sa, g, q, m2, nb, nb, T(1), A + (size_t)k1 * lda + k0, lda, Zp(q), NB, T(0), C, ldc, W.lslots);
+    };
+    int q = s0;
+    // Pair mode while the trailing matrix is large: panels q and q+1 are factored on the side
+    // stream (P, L21, U12 of q; q's update of q+1's column and row; P, L21, U12 of q+1), then
+    // the next two panels' columns and rows receive both panels' update at K = 2 NB, and the
+    // bulk stream applies it to the rest of the matrix in one product, reading and writing
+    // that C once for two panels.  Everything of the pair except the bulk update and the
+    // strips runs beside the previous pair's bulk update.
+    for (int pi = 0; q + 2 < nblk && N - q * NB >= pair_min; pi++, q += 2) {
+      const int b0 = q * NB, b1 = b0 + NB, b2 = b1 + NB, b4 = min(b2 + NB2, N);
+      T* Lq = Lpr(pi);
+      T* Uq = Upr(pi);
       int nslot = 0;
-      if (m2 > 0) {
-        const T* A21 = A + (size_t)k1 * lda + k0;
-        nslot = (int)gemm_tiles(m2, nb);
-        if ((r = timed_gemm(sa, g, q, m2, nb, nb, T(1), A21, lda, Zp(q), NB, T(0), Lbuf(q), (size_t)nb, W.lslots)) != hipSuccess)
-          return r;
-      }
+      if ((r = diag_l21(q, Lq + (size_t)b1 * NB2, NB2, &nslot)) != hipSuccess) return r;
+      if ((r = gemm_checked<T>(NB, N - b1, NB, T(1), Yp(q), NB, A + (size_t)b0 * lda + b1, lda, T(0), Uq + b1, ldu, sa, W.first_fail,
+                               q, W.lmaxv + q, W.lslots, nslot)) != hipSuccess)
+        return r;
+      // q's update of panel q+1's column and row
+      if ((r = timed_gemm(sa, g, q, N - b1, NB, NB, T(-1), Lq + (size_t)b1 * NB2, NB2, Uq + b1, ldu, T(1), A + (size_t)b1 * lda + b1,
+                          lda)) != hipSuccess)
+        return r;
+      if ((r = timed_gemm(sa, g, q, NB, N - b2, NB, T(-1), Lq + (size_t)b1 * NB2, NB2, Uq + b2, ldu, T(1), A + (size_t)b1 * lda + b2,
+                          lda)) != hipSuccess)
+        return r;
+      if ((r = diag_l21(q + 1, Lq + (size_t)b2 * NB2 + NB, NB2, &nslot)) != hipSuccess) return r;
+      if ((r = gemm_checked<T>(NB, N - b2, NB, T(1), Yp(q + 1), NB, A + (size_t)b1 * lda + b2, lda, T(0), Uq + NB * ldu + b2, ldu, sa,
+                               W.first_fail, q + 1, W.lmaxv + q + 1, W.lslots, nslot)) != hipSuccess)
+        return r;
+      // the next two panels' columns and rows, after the previous pair's bulk update and copies
+      if (pi > 0 && (r = hipStreamWaitEvent(sa, ss->ev_r, 0)) != hipSuccess) return r;
+      if ((r = timed_gemm(sa, g, q + 1, N - b2, b4 - b2, NB2, T(-1), Lq + (size_t)b2 * NB2, NB2, Uq + b2, ldu, T(1),
+                          A + (size_t)b2 * lda + b2, lda)) != hipSuccess)
+        return r;
+      if ((r = timed_gemm(sa, g, q + 1, b4 - b2, N - b4, NB2, T(-1), Lq + (size_t)b2 * NB2, NB2, Uq + b4, ldu, T(1),
+                          A + (size_t)b2 * lda + b4, lda)) != hipSuccess)
+        return r;
+      if ((r = hipEventRecord(ss->ev_u, sa)) != hipSuccess) return r;
+      // the bulk update on the caller's stream, then both panels' factors into the matrix
+      if ((r = hipStreamWaitEvent(st, ss->ev_u, 0)) != hipSuccess) return r;
+      if ((r = timed_gemm(st, g, q + 1, N - b4, N - b4, NB2, T(-1), Lq + (size_t)b4 * NB2, NB2, Uq + b4, ldu, T(1),
+                          A + (size_t)b4 * lda + b4, lda)) != hipSuccess)
+        return r;
+      hipLaunchKernelGGL(k_copy_panel<T>, dim3(cdiv((size_t)2 * (N - b1) * NB, 256)), dim3(256), 0, st, Lq + (size_t)b1 * NB2,
+                         (size_t)NB2, A + (size_t)b1 * lda + b0, Uq + b1, ldu, A + (size_t)b0 * lda + b1, lda, N - b1, N - b1, NB, g, q);
+      hipLaunchKernelGGL(k_copy_panel<T>, dim3(cdiv((size_t)2 * (N - b2) * NB, 256)), dim3(256), 0, st, Lq + (size_t)b2 * NB2 + NB,
+                         (size_t)NB2, A + (size_t)b2 * lda + b1, Uq + NB * ldu + b2, ldu, A + (size_t)b1 * lda + b2, lda, N - b2, N - b2,
+                         NB, g, q + 1);
+      if ((r = hipEventRecord(ss->ev_r, st)) != hipSuccess) return r;
+    }
+    const int s1 = q;   // first panel of the single-panel mode
+    for (; q < nblk; q++) {
+      const int k0 = q * NB, nb = min(NB, N - k0), m2 = N - k0 - nb, n2 = m2;
+      const int k1 = k0 + nb;
+      int nslot = 0;
+      if ((r = diag_l21(q, Lbuf(q), (size_t)nb, &nslot)) != hipSuccess) return r;
       if (n2 <= 0) {   // the last panel: its test alone (otherwise it runs in front of U12)
         hipLaunchKernelGGL(k_panel_check, dim3(1), dim3(256), 0, sa, W.lmaxv + q, W.lslots, nslot, W.first_fail, q);
         break;
       }
       const int nb1 = min(NB, n2);
       const T* A12 = A + (size_t)k0 * lda + k1;
-      // U(q) after Tr(q-1) of the bulk stream
+      // U(q) after Tr(q-1) of the bulk stream (or the last pair's bulk update)
       if (q > s0 && (r = hipStreamWaitEvent(sa, ss->ev_r, 0)) != hipSuccess) return r;
       if ((r = gemm_checked<T>(nb, n2, nb, T(1), Yp(q), NB, A12, lda, T(0), Ubuf(q), (size_t)n2, sa, W.first_fail, q, W.lmaxv + q,
                                W.lslots, nslot)) != hipSuccess)
@@ -1801,8 +1897,8 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
       if ((r = timed_gemm(st, g, q, n2, n2 - nb1, nb, T(-1), Lbuf(q), (size_t)nb, Ubuf(q) + nb1, (size_t)n2, T(1),
                           A + (size_t)k1 * lda + k1 + nb1, lda)) != hipSuccess)
         return r;
-      hipLaunchKernelGGL(k_copy_panel<T>, dim3(cdiv((size_t)m2 * nb + (size_t)nb * n2, 256)), dim3(256), 0, st, Lbuf(q),
-                         A + (size_t)k1 * lda + k0, Ubuf(q), A + (size_t)k0 * lda + k1, lda, m2, n2, nb, g, q);
+      hipLaunchKernelGGL(k_copy_panel<T>, dim3(cdiv((size_t)m2 * nb + (size_t)nb * n2, 256)), dim3(256), 0, st, Lbuf(q), (size_t)nb,
+                         A + (size_t)k1 * lda + k0, Ubuf(q), (size_t)n2, A + (size_t)k0 * lda + k1, lda, m2, n2, nb, g, q);
       if ((r = hipEventRecord(ss->ev_r, st)) != hipSuccess) return r;
     }
     if ((r = hipEventRecord(ss->ev_end, sa)) != hipSuccess) return r;
@@ -1811,6 +1907,11 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
     if ((r = hipMemcpyAsync(&h, W.first_fail, sizeof(int), hipMemcpyDeviceToHost, st)) != hipSuccess) return r;
     if ((r = hipStreamSynchronize(st)) != hipSuccess) return r;
     *kf = h;
+    // a failure at the second panel of a pair leaves the first panel's update beyond the pair
+    if (h > s0 && h < s1 && ((h - s0) & 1)) {
+      *kpair = h - 1;
+      *kpar = ((h - 1 - s0) / 2) & 1;
+    }
     return hipGetLastError();
   };
 
@@ -1842,10 +1943,16 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
   };
 
   for (int q = 0; q < nblk;) {
-    int kf = nblk;
-    e = speculate(q, &kf);
+    int kf = nblk, kpair = -1, kpar = 0;
+    e = speculate(q, &kf, &kpair, &kpar);
     if (e != hipSuccess) return e;
     if (kf < q || kf >= nblk) break;
+    if (kpair >= 0) {   // panel kpair's update of the matrix beyond its pair
+      const int b2 = (kpair + 2) * NB;
+      if ((e = timed_gemm(st, nullptr, 0, N - b2, N - b2, NB, T(-1), Lpr(kpar) + (size_t)b2 * NB2, NB2, Upr(kpar) + b2, ldu, T(1),
+                          A + (size_t)b2 * lda + b2, lda)) != hipSuccess)
+        return e;
+    }
     const int k0 = kf * NB, nb = min(NB, N - k0);
     hipLaunchKernelGGL(k_copy2d<T>, dim3(cdiv((size_t)nb * nb, 256)), dim3(256), 0, st, W.Dbak, (size_t)NB,
                        A + (size_t)k0 * lda + k0, lda, nb, nb, (const int*)nullptr, 0);
