@@ -145,16 +145,18 @@ def test_solve_vs_numpy(dev, geo, oracle):
     assert e1 <= 1e-14 and em <= 1e-14        # refined: the correctly rounded charges
 
 
-# the factorisation takes panels in pairs (one bulk update at K = 256 for two panels) while
-# the trailing matrix has at least TRACEON_LU_PAIR_ROWS rows (4096 by default); "0" runs the
-# same matrices with pairs throughout
-PAIR_MODES = [None, "0"]
+# the factorisation takes panels in groups of 4 and 2 (one bulk update at K = 512 or 256 for
+# the group) while the trailing matrix has at least TRACEON_LU_QUAD_ROWS / TRACEON_LU_PAIR_ROWS
+# rows; the LU tests also run the same matrices with pairs throughout and with groups of four
+# throughout
+GROUP_MODES = {"default": {}, "pairs-all": {"TRACEON_LU_PAIR_ROWS": "0", "TRACEON_LU_QUAD_ROWS": "1000000000"},
+               "quads-all": {"TRACEON_LU_QUAD_ROWS": "0"}}
 
 
-@pytest.fixture(params=PAIR_MODES, ids=["pairs-large", "pairs-all"])
+@pytest.fixture(params=list(GROUP_MODES))
 def pair_rows(request, monkeypatch):
-    if request.param is not None:
-        monkeypatch.setenv("TRACEON_LU_PAIR_ROWS", request.param)
+    for k, v in GROUP_MODES[request.param].items():
+        monkeypatch.setenv(k, v)
     return request.param
 
 
@@ -180,7 +182,7 @@ def test_lu_f32_split_trailing_updates(dev, pair_rows):
     """N = 4000 in FP32: the first panels' trailing updates are large enough for the
     128-tile product that splits its FP32 operands into BF16 parts (six BF16 products per
     FP32 product).  Diagonally dominant with two rows of panel 2 swapped, so the pass also
-    stops at panel 2 (with pairs: the first panel of the pair 2, 3), factors it with
+    stops at panel 2 (the first panel of the pair 2, 3, or the third of the group 0-3), factors it with
     pivoting and resumes.  Pivots equal LAPACK's; the
     solution agrees with LAPACK's to FP32 rounding (the matrix is well conditioned:
     cond ~ 1.1)."""
@@ -207,9 +209,9 @@ def test_lu_f32_split_trailing_updates(dev, pair_rows):
 def test_lu_pivot_in_a_middle_panel(dev, dtype, pair_rows):
     """Diagonally dominant except two rows of panel 3 (of 8) swapped: the speculative
     no-pivot pass accepts panels 0-2, fails at panel 3 (a multiplier > 1), restores that
-    panel's diagonal block, factors it with partial pivoting and resumes.  With pairs,
-    panel 3 is the second panel of the pair 2, 3, so panel 2's update of the matrix beyond
-    the pair is applied after the pass.  Pivots equal LAPACK's (one interchange, in panel
+    panel's diagonal block, factors it with partial pivoting and resumes.  In a group,
+    panel 3 is the second panel of the pair 2, 3 or the fourth of the group 0-3, so the
+    group's earlier panels' update of the matrix beyond the group is applied after the pass.  Pivots equal LAPACK's (one interchange, in panel
     3) and the solution is LAPACK's."""
     import torch
     import scipy.linalg

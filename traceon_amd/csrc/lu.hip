@@ -1615,8 +1615,8 @@ struct LuWorkspace {
   double* rhs;                                    // copy of the right-hand sides (nrhs x N, nrhs <= 64)
   double* ytmp;                                   // L^{-1} P b of the solve in progress (N)
   T* Ltmp;                                        // L21 = A21 inv(U11) of the speculative pass (2 x N x NB, by panel parity)
-  T* Lp; T* Up;                                   // L21 and U12 of a panel pair: [L_q | L_q+1] (N x 2NB, rows absolute) and
-                                                  // [U_q ; U_q+1] (2NB x ldu, columns absolute), two of each by pair parity
+  T* Lp; T* Up;                                   // L21 and U12 of a panel group of up to 4: [L_q | ..] (N x 4NB, rows
+                                                  // absolute) and [U_q ; ..] (4NB x ldu, columns absolute), two of each
   size_t ldu;
   T* Dbak;                                        // the current diagonal block before its factorisation (NB x NB)
   unsigned long long* lmaxv;                      // max |l| per panel of the speculative pass (nblk)
@@ -1640,7 +1640,7 @@ struct LuWorkspace {
     size_t a_lt = take(2 * (size_t)N * NB * sizeof(T)), a_lv = take((size_t)nblk * 8), a_ff = take(sizeof(int));
     size_t a_db = take((size_t)NB * NB * sizeof(T));
     ldu = ((size_t)N + 31) / 32 * 32;
-    size_t a_lp = take(2 * (size_t)N * 2 * NB * sizeof(T)), a_up = take(2 * (size_t)2 * NB * ldu * sizeof(T));
+    size_t a_lp = take(2 * (size_t)N * 4 * NB * sizeof(T)), a_up = take(2 * (size_t)4 * NB * ldu * sizeof(T));
     size_t a_ls = take(((size_t)N / 32 + 64) * 8);
     bytes = o;
     if (!base) return;
@@ -1689,12 +1689,15 @@ static hipError_t side_stream(int dev, SideStream** out) {
   return hipSuccess;
 }
 
-// Panel pairs (one bulk update at K = 2 NB for two panels) are used while the trailing matrix
-// has at least this many rows; TRACEON_LU_PAIR_ROWS overrides it (0: pairs throughout, a
-// value above N: one panel at a time).
-static int lu_pair_rows() {
-  const char* e = getenv("TRACEON_LU_PAIR_ROWS");
-  return e && *e ? atoi(e) : 4096;
+// Panel groups (one bulk update at K = G NB for G panels): groups of 4 while the trailing
+// matrix has at least lu_group_rows(4) rows, of 2 while it has lu_group_rows(2), single panels
+// after.  TRACEON_LU_QUAD_ROWS and TRACEON_LU_PAIR_ROWS override them (0: throughout, a value
+// above N: never).
+static constexpr int LU_GROUP_MAX = 4;
+static_assert(LU_GROUP_MAX == 4, "the workspace's group buffers hold four panels");
+static int lu_group_rows(int G) {
+  const char* e = getenv(G == 4 ? "TRACEON_LU_QUAD_ROWS" : "TRACEON_LU_PAIR_ROWS");
+  return e && *e ? atoi(e) : (G == 4 ? 8192 : 4096);
 }
 
 // Factorisation driver.  Most BEM panels need no row interchange, so the factorisation
@@ -1717,12 +1720,13 @@ static int lu_pair_rows() {
 // speculative pass resumes at f + 1.  A panel that passes the test (every |l| <= 1) is
 // exactly the partially pivoted result: its diagonal entries are the column maxima, so its
 // ipiv is the identity.
-// While the trailing matrix is large the pass takes panels in pairs (q, q+1): the side stream
-// factors both (P, L21, U12 of q; q's update of q+1's column and row; P, L21, U12 of q+1) and
-// then updates the next pair's columns and rows with both panels at once (K = 2 NB); the
-// bulk stream applies the pair's update to the rest of the matrix in one product, so that
-// trailing C is read and written once per two panels.  A failure at q+1 leaves q's update
-// beyond the pair unapplied; the driver applies it before the pivoted panel.
+// While the trailing matrix is large the pass takes panels in groups of G = 4 or 2: the side
+// stream factors them one after another (P, L21, U12 of each, then its update of the group's
+// later panels' columns and rows) and then updates the next group's columns and rows with the
+// whole group at once (K = G NB); the bulk stream applies the group's update to the rest of the
+// matrix in one product, so that trailing C is read and written once per G panels.  A failure
+// at panel j > 0 of a group leaves panels 0 .. j-1's update beyond the group unapplied; the
+// driver applies it before the pivoted panel.
 template <typename T>
 hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev, hipStream_t st, int panel_groups,
                      KernelTiming* tm) {
@@ -1794,17 +1798,20 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
   auto Yp = [&](int q) { return W.Linv + (size_t)q * NB * NB; };
   auto Zp = [&](int q) { return W.Uinv + (size_t)q * NB * NB; };
 
-  // ---- speculative pass from panel s0; *kf = first failed panel, or nblk; *kpair = the first
-  //      panel of the pair whose second panel is *kf, or -1 (panel *kpair's update of the
-  //      matrix beyond the pair is then still to apply) ----
-  constexpr int NB2 = 2 * NB;
+  // ---- speculative pass from panel s0; *kf = first failed panel, or nblk; *fix = the
+  //      panel group that holds *kf at a position j > 0 (its panels 0 .. j-1's update of the
+  //      matrix beyond the group is then still to apply), j = 0 if none ----
+  constexpr int LDL = LU_GROUP_MAX * NB;
   const size_t ldu = W.ldu;
-  const int pair_min = lu_pair_rows();
-  auto Lpr = [&](int pi) { return W.Lp + (size_t)(pi & 1) * N * NB2; };     // [L_q | L_q+1], row r at r * NB2
-  auto Upr = [&](int pi) { return W.Up + (size_t)(pi & 1) * NB2 * ldu; };   // [U_q ; U_q+1], column c at c
-  auto speculate = [&](int s0, int* kf, int* kpair, int* kpar) -> hipError_t {
+  const int pair_min = lu_group_rows(2), quad_min = lu_group_rows(4);
+  auto Lgr = [&](int gi) { return W.Lp + (size_t)(gi & 1) * N * LDL; };   // [L_q | .. | L_q+G-1], row r at r * LDL
+  auto Ugr = [&](int gi) { return W.Up + (size_t)(gi & 1) * LDL * ldu; };  // [U_q ; .. ; U_q+G-1], column c at c
+  struct Group { int q, G, gi; };
+  struct Fix { int q, j, gi, G; };
+  auto speculate = [&](int s0, int* kf, Fix* fix) -> hipError_t {
     const int* g = W.first_fail;
-    *kpair = -1;
+    *fix = {0, 0, 0, 0};
+    std::vector<Group> groups;
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, W.first_fail, nblk);
     hipLaunchKernelGGL(k_zero_u64, dim3(cdiv(nblk, 256)), dim3(256), 0, st, W.lmaxv, nblk);
     hipError_t r = hipEventRecord(ss->ev_start, st);
@@ -1823,54 +1830,64 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
       return timed_gemm(sa, g, q, m2, nb, nb, T(1), A + (size_t)k1 * lda + k0, lda, Zp(q), NB, T(0), C, ldc, W.lslots);
     };
     int q = s0;
-    // Pair mode while the trailing matrix is large: panels q and q+1 are factored on the side
-    // stream (P, L21, U12 of q; q's update of q+1's column and row; P, L21, U12 of q+1), then
-    // the next two panels' columns and rows receive both panels' update at K = 2 NB, and the
-    // bulk stream applies it to the rest of the matrix in one product, reading and writing
-    // that C once for two panels.  Everything of the pair except the bulk update and the
-    // strips runs beside the previous pair's bulk update.
-    for (int pi = 0; q + 2 < nblk && N - q * NB >= pair_min; pi++, q += 2) {
-      const int b0 = q * NB, b1 = b0 + NB, b2 = b1 + NB, b4 = min(b2 + NB2, N);
-      T* Lq = Lpr(pi);
-      T* Uq = Upr(pi);
-      int nslot = 0;
-      if ((r = diag_l21(q, Lq + (size_t)b1 * NB2, NB2, &nslot)) != hipSuccess) return r;
-      if ((r = gemm_checked<T>(NB, N - b1, NB, T(1), Yp(q), NB, A + (size_t)b0 * lda + b1, lda, T(0), Uq + b1, ldu, sa, W.first_fail,
-                               q, W.lmaxv + q, W.lslots, nslot)) != hipSuccess)
+    // Group mode while the trailing matrix is large: the G panels q .. q+G-1 (G = 4 or 2) are
+    // factored on the side stream, each followed by its update of the group's later panels'
+    // columns and rows; then the next G panels' columns and rows receive the whole group's
+    // update at K = G NB, and the bulk stream applies it to the rest of the matrix in one
+    // product, reading and writing that C once for G panels.  Everything of the group except
+    // the bulk update and the strips runs beside the previous group's bulk update.  Group
+    // sizes never grow along the pass, so a group's strips always cover the next group.
+    for (int gi = 0;; gi++) {
+      int G = N - q * NB >= quad_min ? 4 : N - q * NB >= pair_min ? 2 : 1;
+      while (G > 1 && q + G >= nblk) G /= 2;
+      if (G == 1) break;
+      const int b0 = q * NB, bG = b0 + G * NB, b2G = min(bG + G * NB, N);
+      T* Lq = Lgr(gi);
+      T* Uq = Ugr(gi);
+      for (int j = 0; j < G; j++) {
+        const int bj = b0 + j * NB, bj1 = bj + NB;
+        int nslot = 0;
+        if ((r = diag_l21(q + j, Lq + (size_t)bj1 * LDL + j * NB, LDL, &nslot)) != hipSuccess) return r;
+        if ((r = gemm_checked<T>(NB, N - bj1, NB, T(1), Yp(q + j), NB, A + (size_t)bj * lda + bj1, lda, T(0),
+                                 Uq + (size_t)j * NB * ldu + bj1, ldu, sa, W.first_fail, q + j, W.lmaxv + q + j, W.lslots, nslot)) !=
+            hipSuccess)
+          return r;
+        if (j + 1 == G) break;
+        // panel q+j's update of the group's later panels: their columns, then their rows
+        const T* Lj = Lq + (size_t)bj1 * LDL + j * NB;
+        const T* Uj = Uq + (size_t)j * NB * ldu;
+        if ((r = timed_gemm(sa, g, q + j, N - bj1, bG - bj1, NB, T(-1), Lj, LDL, Uj + bj1, ldu, T(1), A + (size_t)bj1 * lda + bj1,
+                            lda)) != hipSuccess)
+          return r;
+        if ((r = timed_gemm(sa, g, q + j, bG - bj1, N - bG, NB, T(-1), Lj, LDL, Uj + bG, ldu, T(1), A + (size_t)bj1 * lda + bG,
+                            lda)) != hipSuccess)
+          return r;
+      }
+      groups.push_back({q, G, gi});
+      // the next G panels' columns and rows, after the previous group's bulk update and copies
+      const int gl = q + G - 1, KG = G * NB;
+      if (gi > 0 && (r = hipStreamWaitEvent(sa, ss->ev_r, 0)) != hipSuccess) return r;
+      if ((r = timed_gemm(sa, g, gl, N - bG, b2G - bG, KG, T(-1), Lq + (size_t)bG * LDL, LDL, Uq + bG, ldu, T(1),
+                          A + (size_t)bG * lda + bG, lda)) != hipSuccess)
         return r;
-      // q's update of panel q+1's column and row
-      if ((r = timed_gemm(sa, g, q, N - b1, NB, NB, T(-1), Lq + (size_t)b1 * NB2, NB2, Uq + b1, ldu, T(1), A + (size_t)b1 * lda + b1,
-                          lda)) != hipSuccess)
-        return r;
-      if ((r = timed_gemm(sa, g, q, NB, N - b2, NB, T(-1), Lq + (size_t)b1 * NB2, NB2, Uq + b2, ldu, T(1), A + (size_t)b1 * lda + b2,
-                          lda)) != hipSuccess)
-        return r;
-      if ((r = diag_l21(q + 1, Lq + (size_t)b2 * NB2 + NB, NB2, &nslot)) != hipSuccess) return r;
-      if ((r = gemm_checked<T>(NB, N - b2, NB, T(1), Yp(q + 1), NB, A + (size_t)b1 * lda + b2, lda, T(0), Uq + NB * ldu + b2, ldu, sa,
-                               W.first_fail, q + 1, W.lmaxv + q + 1, W.lslots, nslot)) != hipSuccess)
-        return r;
-      // the next two panels' columns and rows, after the previous pair's bulk update and copies
-      if (pi > 0 && (r = hipStreamWaitEvent(sa, ss->ev_r, 0)) != hipSuccess) return r;
-      if ((r = timed_gemm(sa, g, q + 1, N - b2, b4 - b2, NB2, T(-1), Lq + (size_t)b2 * NB2, NB2, Uq + b2, ldu, T(1),
-                          A + (size_t)b2 * lda + b2, lda)) != hipSuccess)
-        return r;
-      if ((r = timed_gemm(sa, g, q + 1, b4 - b2, N - b4, NB2, T(-1), Lq + (size_t)b2 * NB2, NB2, Uq + b4, ldu, T(1),
-                          A + (size_t)b2 * lda + b4, lda)) != hipSuccess)
+      if ((r = timed_gemm(sa, g, gl, b2G - bG, N - b2G, KG, T(-1), Lq + (size_t)bG * LDL, LDL, Uq + b2G, ldu, T(1),
+                          A + (size_t)bG * lda + b2G, lda)) != hipSuccess)
         return r;
       if ((r = hipEventRecord(ss->ev_u, sa)) != hipSuccess) return r;
-      // the bulk update on the caller's stream, then both panels' factors into the matrix
+      // the bulk update on the caller's stream, then the group's factors into the matrix
       if ((r = hipStreamWaitEvent(st, ss->ev_u, 0)) != hipSuccess) return r;
-      if ((r = timed_gemm(st, g, q + 1, N - b4, N - b4, NB2, T(-1), Lq + (size_t)b4 * NB2, NB2, Uq + b4, ldu, T(1),
-                          A + (size_t)b4 * lda + b4, lda)) != hipSuccess)
+      if ((r = timed_gemm(st, g, gl, N - b2G, N - b2G, KG, T(-1), Lq + (size_t)b2G * LDL, LDL, Uq + b2G, ldu, T(1),
+                          A + (size_t)b2G * lda + b2G, lda)) != hipSuccess)
         return r;
-      hipLaunchKernelGGL(k_copy_panel<T>, dim3(cdiv((size_t)2 * (N - b1) * NB, 256)), dim3(256), 0, st, Lq + (size_t)b1 * NB2,
-                         (size_t)NB2, A + (size_t)b1 * lda + b0, Uq + b1, ldu, A + (size_t)b0 * lda + b1, lda, N - b1, N - b1, NB, g, q);
-      hipLaunchKernelGGL(k_copy_panel<T>, dim3(cdiv((size_t)2 * (N - b2) * NB, 256)), dim3(256), 0, st, Lq + (size_t)b2 * NB2 + NB,
-                         (size_t)NB2, A + (size_t)b2 * lda + b1, Uq + NB * ldu + b2, ldu, A + (size_t)b1 * lda + b2, lda, N - b2, N - b2,
-                         NB, g, q + 1);
+      for (int j = 0; j < G; j++) {
+        const int bj = b0 + j * NB, bj1 = bj + NB;
+        hipLaunchKernelGGL(k_copy_panel<T>, dim3(cdiv((size_t)2 * (N - bj1) * NB, 256)), dim3(256), 0, st,
+                           Lq + (size_t)bj1 * LDL + j * NB, (size_t)LDL, A + (size_t)bj1 * lda + bj, Uq + (size_t)j * NB * ldu + bj1, ldu,
+                           A + (size_t)bj * lda + bj1, lda, N - bj1, N - bj1, NB, g, q + j);
+      }
       if ((r = hipEventRecord(ss->ev_r, st)) != hipSuccess) return r;
+      q += G;
     }
-    const int s1 = q;   // first panel of the single-panel mode
     for (; q < nblk; q++) {
       const int k0 = q * NB, nb = min(NB, N - k0), m2 = N - k0 - nb, n2 = m2;
       const int k1 = k0 + nb;
@@ -1907,11 +1924,10 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
     if ((r = hipMemcpyAsync(&h, W.first_fail, sizeof(int), hipMemcpyDeviceToHost, st)) != hipSuccess) return r;
     if ((r = hipStreamSynchronize(st)) != hipSuccess) return r;
     *kf = h;
-    // a failure at the second panel of a pair leaves the first panel's update beyond the pair
-    if (h > s0 && h < s1 && ((h - s0) & 1)) {
-      *kpair = h - 1;
-      *kpar = ((h - 1 - s0) / 2) & 1;
-    }
+    // a failure at panel j > 0 of a group leaves panels 0 .. j-1's update beyond the group
+    for (const Group& gr : groups)
+      if (h > gr.q && h < gr.q + gr.G) *fix = {gr.q, h - gr.q, gr.gi, gr.G};
+
     return hipGetLastError();
   };
 
@@ -1943,14 +1959,15 @@ hipError_t lu_factor(T* A, size_t lda, int N, int* ipiv, void* ws, int* info_dev
   };
 
   for (int q = 0; q < nblk;) {
-    int kf = nblk, kpair = -1, kpar = 0;
-    e = speculate(q, &kf, &kpair, &kpar);
+    int kf = nblk;
+    Fix fix;
+    e = speculate(q, &kf, &fix);
     if (e != hipSuccess) return e;
     if (kf < q || kf >= nblk) break;
-    if (kpair >= 0) {   // panel kpair's update of the matrix beyond its pair
-      const int b2 = (kpair + 2) * NB;
-      if ((e = timed_gemm(st, nullptr, 0, N - b2, N - b2, NB, T(-1), Lpr(kpar) + (size_t)b2 * NB2, NB2, Upr(kpar) + b2, ldu, T(1),
-                          A + (size_t)b2 * lda + b2, lda)) != hipSuccess)
+    if (fix.j > 0) {   // the update of the group's panels before kf, beyond the group
+      const int bG = (fix.q + fix.G) * NB;
+      if ((e = timed_gemm(st, nullptr, 0, N - bG, N - bG, fix.j * NB, T(-1), Lgr(fix.gi) + (size_t)bG * LDL, LDL, Ugr(fix.gi) + bG, ldu,
+                          T(1), A + (size_t)bG * lda + bG, lda)) != hipSuccess)
         return e;
     }
     const int k0 = kf * NB, nb = min(NB, N - k0);
