@@ -30,14 +30,14 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "hbm": 8.0e12}
-# HBM bytes per launch of the LU's trailing-update GEMM (k_gemm_x6) in this workload
-# (N = 20,000): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE averaged over its 133 launches in one
+PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16": 2500e12, "hbm": 8.0e12}
+# HBM bytes per launch of the LU's bulk-update GEMM (k_gemm_x6) in this workload
+# (N = 20,000): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE averaged over its 59 launches in one
 # factorisation, each scaled by what the counter reads for a 256 MiB device-to-device copy
 # (--copy-calibration 256): fetch x2.0, write x1.0.  Counters cannot be read inside this
 # process, so the figure is the one measured for this kernel and launch schedule; other N
 # report null.
-GEMM_X6_TRAFFIC_N20000 = 1688.0e6
+GEMM_X6_TRAFFIC_N20000 = 2120.7e6
 # algorithmic FLOP per unit (reference formulas, +-*/sqrt = 1, FMA = 2, log = 20)
 C_POT, C_FIELD, C_FLUX = 71, 215, 219
 
@@ -402,28 +402,35 @@ def main():
         "metric": "electrons traced/s @<=10nm landing error; BEM fill+solve time at N=20k",
         "value": float(np.median(times)), "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": float(np.median(times) * 1e3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64 (fill, residuals) + f32 (LU: split-BF16 MFMA trailing updates)", "data": "synthetic geometry (seeded), no external data",
+        "dtype": "f64 (fill, residuals) + f32 (LU: split-BF16 MFMA bulk updates)", "data": "synthetic geometry (seeded), no external data",
         "config": {"workload": f"electrostatic mirror, 5 electrodes, N={N} line elements: fill + FP32-LU + FP64 refinement",
                    "N": N, "parallelism": "single GPU"},
         "fill_solve_fp64_s": t64["total_s"], "fill_ms": float(np.median([t["fill_ms"] for t in tfs])),
         "solve_ms": float(np.median([t["solve_ms"] for t in tfs])), "refine_iterations": stats.iterations,
         "charge_rel_diff_mixed_vs_fp64": dq,
     }
-    # dominant kernel of the step: the trailing-update GEMM of the FP32 factorisation, timed
-    # with HIP events around each of its launches on the stream it runs on.  Its FP32 operands
-    # are split into three BF16 parts and multiplied on BF16 MFMA (six products per block), so
-    # per HBM byte it issues 6 x 2K/8 = 192 BF16 FLOP (K = 128), below the BF16 ridge point of
-    # 2.5 PFLOP/s / 8 TB/s = 312: the bound is HBM (C read and written once per update).
+    # dominant kernel of the step: the bulk-update GEMM of the FP32 factorisation (panel groups
+    # of 4 and 2, K = 512 and 256), timed with HIP events around each of its launches on the
+    # stream it runs on.  Its FP32 operands are split into three BF16 parts and multiplied on
+    # BF16 MFMA, six products per FP32 product: the algorithm's unit, one FP32 multiply-add of
+    # the update, is 12 BF16 FLOP.  Per HBM byte (A and B once, C read and written) that is
+    # 6 x 2K / (8 + 8K/n) BF16 FLOP, about 690 on average here, above the BF16 ridge point of
+    # 2.5 PFLOP/s / 8 TB/s = 312: the bound is the MFMA rate (below the ridge, HBM).
     gemm_s_avg = gt.ms / max(gt.launches, 1) * 1e-3
     gemm_flop_avg = gt.flops / max(gt.launches, 1)
     gemm_bytes_avg = gt.bytes / max(gt.launches, 1)
-    roof = {"kernel": "k_gemm_x6 (LU trailing updates: FP32 operands split into three BF16 parts, "
+    bf16_flop_avg = 6 * gemm_flop_avg
+    mfma_bound = bf16_flop_avg / max(gemm_bytes_avg, 1.0) > PEAK["bf16"] / PEAK["hbm"]
+    roof = {"kernel": "k_gemm_x6 (LU bulk updates: FP32 operands split into three BF16 parts, "
                       "six v_mfma_f32_16x16x32_bf16 per 16x16x32 block)",
-            "bound": "hbm", "achieved": gemm_bytes_avg / gemm_s_avg / 1e9, "peak": PEAK["hbm"] / 1e9, "unit": "GB/s",
+            "bound": "mfma" if mfma_bound else "hbm",
+            "achieved": bf16_flop_avg / gemm_s_avg / 1e12 if mfma_bound else gemm_bytes_avg / gemm_s_avg / 1e9,
+            "peak": PEAK["bf16"] / 1e12 if mfma_bound else PEAK["hbm"] / 1e9, "unit": "TFLOP/s" if mfma_bound else "GB/s",
             "traffic": GEMM_X6_TRAFFIC_N20000 if N == 20000 else None, "traffic_unit": "bytes per launch",
-            "algorithmic_bytes_per_launch": gemm_bytes_avg, "launches": gt.launches, "flop_per_launch": gemm_flop_avg,
-            "launch_ms": gemm_s_avg * 1e3, "fp32_tflops": gemm_flop_avg / gemm_s_avg / 1e12,
-            "bf16_mfma_tflops": 6 * gemm_flop_avg / gemm_s_avg / 1e12, "bf16_mfma_peak_tflops": 2500.0,
+            "algorithmic_bytes_per_launch": gemm_bytes_avg, "launches": gt.launches,
+            "flop_per_launch_fp32": gemm_flop_avg, "flop_per_launch_bf16": bf16_flop_avg,
+            "launch_ms": gemm_s_avg * 1e3, "hbm_algorithmic_gbps": gemm_bytes_avg / gemm_s_avg / 1e9,
+            "fp32_equivalent_tflops": gemm_flop_avg / gemm_s_avg / 1e12, "fp32_mfma_peak_tflops": PEAK["fp32"] / 1e12,
             "share_of_step": gt.ms / max(tg["fill_ms"] + tg["solve_ms"], 1e-9)}
     roof["frac"] = roof["achieved"] / roof["peak"]
     result["roofline"] = roof

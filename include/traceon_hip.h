@@ -151,7 +151,8 @@ int tb_lu_factor_f32(float *A, size_t lda, int N, int *ipiv, void *workspace, in
    `stream` or the factorisation's second stream, so launches that overlap count in full),
    algorithmic FLOP (2 M N K each) and algorithmic HBM bytes (A and B read once, C read
    and written) of the large GEMM launches of this factorisation: those on 128 x 128 tiles
-   (the trailing updates; FP32: split-BF16 MFMA), not the per-panel products. */
+   (the bulk updates of the panel groups and their strips; FP32: split-BF16 MFMA), not
+   the per-panel products. */
 typedef struct { long long launches; double ms; double flops; double bytes; } tb_kernel_timing_t;
 int tb_lu_factor_timed(void *A, size_t lda, int N, int *ipiv, void *workspace, int *info, int fp32,
                        tb_kernel_timing_t *gemm_timing, void *stream);
