@@ -362,8 +362,8 @@ def main():
                     help="first copy MIB MiB device-to-device three times (a known-size kernel for HBM counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md for how it would shard)",
-              file=sys.stderr)
+        print("bench.py runs on one GPU only: the fill, the factorisation and the tracers here use one device; "
+              "there is no multi-GPU path", file=sys.stderr)
         sys.exit(2)
     import torch
     assert torch.cuda.is_available(), "bench.py needs the GPU"

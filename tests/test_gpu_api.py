@@ -4,7 +4,7 @@ points, against the oracle on the same inputs.
 
 Tolerances: potentials/fields of a solved system 1e-9 relative (the charges themselves
 carry the system's cond*eps, cond ~1e5 here); trajectories take the same number of steps
-and land within 1e-9 m."""
+and land within 1e-9 m (the landing bar of the path)."""
 import ctypes
 import numpy as np
 import pytest

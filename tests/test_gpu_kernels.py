@@ -1,7 +1,7 @@
 """GPU parity of the HIP kernels against the oracle (C restatement of the reference).
 
 Tolerances: the path is FP64 end to end; entries, potentials and fields agree with the
-oracle to rounding (1e-12 relative on the matrix, 1e-10 on sums as the north-star bar),
+oracle to rounding (1e-12 relative on the matrix, 1e-10 on sums),
 the singular diagonal to 1e-13 against a tight QUADPACK run (the reference itself asks
 QUADPACK for only 1e-9, backend/__init__.py:692)."""
 import numpy as np

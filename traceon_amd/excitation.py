@@ -3,7 +3,7 @@
 Same enum values (excitation.py:28-87, definitions.c:34-40), same ``add_*`` methods,
 same superposition split (excitation.py:318-346) and active-element selection
 (excitation.py:348-404), so solver code and user scripts read as with the reference.
-Only the radial symmetry exists in this build.
+Only the radial symmetry is implemented.
 """
 from __future__ import annotations
 
@@ -52,7 +52,7 @@ _SUPERPOSED = (ExcitationType.VOLTAGE_FIXED, ExcitationType.VOLTAGE_FUN, Excitat
 class Excitation:
     def __init__(self, mesh, symmetry: Symmetry = Symmetry.RADIAL) -> None:
         if symmetry != Symmetry.RADIAL:
-            raise NotImplementedError("only radial symmetry is part of this build")
+            raise NotImplementedError("only radial symmetry is implemented")
         self.mesh = mesh
         self.electrodes = mesh.get_electrodes()
         self.excitation_types: dict[str, tuple[ExcitationType, object]] = {}

@@ -6,7 +6,7 @@ the reference's semantics; each runs on the GPU:
 
 * element buffers, normals and the dense fill + singular diagonal in HIP kernels on a
   device-resident matrix (no host copy of the N x N matrix);
-* the solve by this build's LU with partial pivoting.  ``precision="fp64"`` (default)
+* the solve by the library's LU with partial pivoting.  ``precision="fp64"`` (default)
   factors the FP64 matrix; ``precision="mixed"`` factors an FP32 copy on MFMA.  Both refine
   the charges with double-double residuals against the FP64 matrix, so they converge to
   the correctly rounded solution of the assembled system.
