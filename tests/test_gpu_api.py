@@ -78,6 +78,46 @@ def test_mixed_precision_solve(gpu):
     assert rel(b, a) < 1e-13
 
 
+def test_solve_with_pivots_inside_panel_groups(gpu):
+    """The mirror at N = 9,000 (no row interchange needed) at the default group thresholds,
+    which take the first panels in groups of four.  Two row pairs of the assembled system are swapped, in panel 2
+    (the third panel of the first group) and in panel 6, so the speculative no-pivot pass fails
+    inside a group: the group recovery, the pivoted panel and the resumed pass run in FP64 and
+    in FP32.  Swapping rows of a system leaves its solution unchanged: both paths reproduce the
+    charges of the unswapped system, pivot exactly at those rows, and the mixed path converges
+    in a few refinement steps without the FP64 fallback."""
+    import torch
+    from traceon_amd import configs, device
+    mesh, volts = configs.mirror(9000)
+    lines, types, values = configs.lines_types_values(mesh, volts)
+    F = np.where(np.isin(types, [1, 2, 5]), values, 0.0)
+    E = device.Elements(lines, types, values)
+    M, _ = device.assemble(E, fp64=True, fp32=False)
+    Xref, sref, lu = device.solve_system(M, F, mode="fp64")
+    assert np.array_equal(lu.ipiv.cpu().numpy(), np.arange(len(F)))   # the unswapped system needs no interchange
+    Xref = Xref.cpu().numpy()[0]
+    del M, lu
+    swaps = [(2 * 128 + 5, 2 * 128 + 40), (6 * 128 + 3, 6 * 128 + 90)]
+    M, M32 = device.assemble(E, fp64=True, fp32=True)
+    Fs = F.copy()
+    for a, b in swaps:
+        for T in (M, M32):
+            T[[a, b]] = T[[b, a]]
+        Fs[[a, b]] = Fs[[b, a]]
+    X64, s64, lu = device.solve_system(M, Fs, mode="fp64")
+    moved = np.nonzero(lu.ipiv.cpu().numpy() != np.arange(len(F)))[0]
+    print("fp64", s64, "interchanges at", moved, flush=True)
+    assert list(moved) == [a for a, _ in swaps]
+    del lu
+    X32, s32, _ = device.solve_system(M, Fs, M32=M32, mode="mixed")
+    print("mixed", s32, flush=True)
+    assert s64.info == 0 and s32.info == 0
+    assert s32.mode == "mixed" and s32.iterations <= 6
+    e64, e32 = rel(X64.cpu().numpy()[0], Xref), rel(X32.cpu().numpy()[0], Xref)
+    print("charges vs the unswapped system: fp64", e64, "mixed", e32, flush=True)
+    assert e64 < 1e-13 and e32 < 1e-13
+
+
 def test_reference_named_fill_accumulates(gpu, oracle):
     from traceon_amd import backend, configs
     mesh, volts, diel = configs.small_geometry(120)
