@@ -928,7 +928,7 @@ __device__ __forceinline__ int x6_off(int row, int k) { return row * 64 + 16 * (
 
 __global__ __launch_bounds__(256) void k_gemm_x6(int M, int N, int K, float alpha, const float* __restrict__ A, size_t lda,
                                                  const float* __restrict__ B, size_t ldb, float beta, float* __restrict__ C,
-                                                 size_t ldc, const int* gate, int gp) {
+                                                 size_t ldc, const int* gate, int gp, int band) {
   typedef float f4v __attribute__((ext_vector_type(4)));
   constexpr int GB = 128, BK = 32, MT = 4, PART = GB * BK * 2;
   constexpr int SC = GB + 4, CROWS = 64, SEG = GB / 4, PER = CROWS * SEG / 256;
@@ -941,8 +941,19 @@ __global__ __launch_bounds__(256) void k_gemm_x6(int M, int N, int K, float alph
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
   const int li = lane & 15, lq = lane >> 4;
-  const int bm = min((int)blockIdx.y * GB, M - GB), bn = min((int)blockIdx.x * GB, N - GB);   // tile origin (inside C)
-  const int sm = (int)blockIdx.y * GB - bm, sn = (int)blockIdx.x * GB - bn;                  // first row / column stored
+  int ty = blockIdx.y, tx = blockIdx.x;
+  if (band > 0) {
+    // XCD-aware order: workgroup w runs on XCD w % 8 as its (w / 8)-th; each XCD takes a
+    // contiguous range of tiles, walked down bands of `band` tile rows column by column, so
+    // that the tiles an XCD holds at once share their A rows and B columns in its L2
+    const int gx = gridDim.x, T = gx * gridDim.y, w = ty * gx + tx, x = w & 7, per = T >> 3, rem = T & 7;
+    const int t = x * per + min(x, rem) + (w >> 3);
+    const int g = t / (band * gx), rows = min(band, (int)gridDim.y - g * band), i = t - g * band * gx;
+    ty = g * band + i % rows;
+    tx = i / rows;
+  }
+  const int bm = min(ty * GB, M - GB), bn = min(tx * GB, N - GB);   // tile origin (inside C)
+  const int sm = ty * GB - bm, sn = tx * GB - bn;                  // first row / column stored
   f4v acc[MT][MT];
 #pragma unroll
   for (int a = 0; a < MT; a++)
@@ -1538,12 +1549,25 @@ size_t gemm_tiles(int M, int N) {
   return (size_t)cdiv(M, t) * cdiv(N, t);
 }
 
+// tile rows per band of k_gemm_x6's XCD-aware tile order (0: the plain row-major order).  At
+// K = 128 the plain order is faster (C, read and written once, keeps its row locality); from
+// K = 256 on the A and B panels (256-512 KiB per tile) make up most of the fetches, and the
+// banded order serves them from each XCD's L2.  TRACEON_X6_BAND overrides it for K >= 256.
+static int x6_band(int K) {
+  static const int env = [] {
+    const char* e = getenv("TRACEON_X6_BAND");
+    return e && *e ? atoi(e) : -1;
+  }();
+  if (K < 256) return 0;
+  return env >= 0 ? env : 8;
+}
+
 template <typename T, int GB, bool TM, bool FAST>
 static void launch_gemm(dim3 grid, int M, int N, int K, T alpha, const T* A, size_t lda, const T* B, size_t ldb, T beta, T* C,
                         size_t ldc, hipStream_t st, const int* gate, int gp, unsigned long long* vm) {
   constexpr int BK = sizeof(T) == 8 ? 16 : 32;
   if constexpr (sizeof(T) == 4 && GB == 128 && FAST && !TM)
-    hipLaunchKernelGGL(k_gemm_x6, grid, dim3(256), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, gp);
+    hipLaunchKernelGGL(k_gemm_x6, grid, dim3(256), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, gp, x6_band(K));
   else if constexpr (sizeof(T) == 4 && GB == 128 && FAST)
     hipLaunchKernelGGL((k_gemm_f32<TM>), grid, dim3(256), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, gp, vm);
   else
