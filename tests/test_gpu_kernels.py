@@ -178,6 +178,27 @@ def test_lu_random(dev, N, pair_rows):
     assert np.array_equal(lu.ipiv.cpu().numpy(), piv)
 
 
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_lu_no_pivot_odd_size(dev, dtype, pair_rows):
+    """Diagonally dominant at N = 1001: no panel needs an interchange, so every group product
+    runs on data; with N % 4 != 0 the rows are not 16-byte aligned and the products take the
+    element-wise kernels.  ipiv is the identity, as LAPACK's; the solution is LAPACK's."""
+    import torch
+    rng = np.random.default_rng(1001)
+    N = 1001
+    A = rng.uniform(-1, 1, (N, N)) + 10 * N ** 0.5 * np.eye(N)
+    b = rng.standard_normal((1, N))
+    t = torch.float64 if dtype == "f64" else torch.float32
+    lu = dev.lu_factor(torch.tensor(A, device="cuda").to(t), overwrite=True)
+    assert lu.info == 0
+    assert np.array_equal(lu.ipiv.cpu().numpy(), np.arange(N))
+    X = torch.tensor(b, device="cuda")
+    dev.lu_solve(lu, X)
+    err = rel(X.cpu().numpy()[0], np.linalg.solve(A, b[0]))
+    print("no-pivot odd size", dtype, pair_rows, err, flush=True)
+    assert err < (1e-13 if dtype == "f64" else 1e-6)
+
+
 def test_lu_f32_split_trailing_updates(dev, pair_rows):
     """N = 4000 in FP32: the first panels' trailing updates are large enough for the
     128-tile product that splits its FP32 operands into BF16 parts (six BF16 products per
