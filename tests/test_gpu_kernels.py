@@ -199,6 +199,41 @@ def test_lu_no_pivot_odd_size(dev, dtype, pair_rows):
     assert err < (1e-13 if dtype == "f64" else 1e-6)
 
 
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_lu_padded_leading_dimension(dev, dtype, monkeypatch):
+    """The C ABI with lda > N (N = 1000, lda = 1007, groups of four throughout, two rows of
+    panel 2 swapped so the pass fails inside a group): the factors and the solve read and
+    write only the N x N part; the padding columns stay as they were; ipiv and the solution
+    are LAPACK's."""
+    import torch
+    import scipy.linalg
+    monkeypatch.setenv("TRACEON_LU_QUAD_ROWS", "0")
+    rng = np.random.default_rng(7)
+    N, lda = 1000, 1007
+    A = rng.uniform(-1, 1, (N, N)) + 10 * N ** 0.5 * np.eye(N)
+    A[[2 * 128 + 3, 2 * 128 + 70]] = A[[2 * 128 + 70, 2 * 128 + 3]]
+    b = rng.standard_normal(N)
+    _, piv = scipy.linalg.lu_factor(A)
+    t = torch.float64 if dtype == "f64" else torch.float32
+    big = torch.full((N, lda), 7.0, dtype=t, device="cuda")
+    big[:, :N] = torch.tensor(A, device="cuda").to(t)
+    ipiv = torch.empty(N, dtype=torch.int32, device="cuda")
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    L = dev._L
+    ws = torch.empty(L.tb_lu_workspace_bytes(N, int(dtype == "f32")), dtype=torch.uint8, device="cuda")
+    f, g = (L.tb_lu_factor, L.tb_lu_solve) if dtype == "f64" else (L.tb_lu_factor_f32, L.tb_lu_solve_f32)
+    dev.check(f(dev.ptr(big), lda, N, dev.ptr(ipiv), dev.ptr(ws), dev.ptr(info), dev.stream_ptr()), "lu_factor")
+    X = torch.tensor(b[None, :], device="cuda")
+    dev.check(g(dev.ptr(big), lda, N, dev.ptr(ipiv), dev.ptr(X), 1, dev.ptr(ws), dev.stream_ptr()), "lu_solve")
+    torch.cuda.synchronize()
+    assert int(info.item()) == 0
+    assert np.array_equal(ipiv.cpu().numpy(), piv)
+    assert bool((big[:, N:] == 7.0).all())
+    err = rel(X.cpu().numpy()[0], np.linalg.solve(A, b))
+    print("padded lda", dtype, err, flush=True)
+    assert err < (1e-13 if dtype == "f64" else 1e-6)
+
+
 def test_lu_f32_split_trailing_updates(dev, pair_rows):
     """N = 4000 in FP32: the first panels' trailing updates are large enough for the
     128-tile product that splits its FP32 operands into BF16 parts (six BF16 products per
