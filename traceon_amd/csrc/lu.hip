@@ -907,10 +907,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_num_vgpr(72))) void k_ge
 // instead of 8 x 32 for v_mfma_f32_16x16x4f32.
 // Operand layout of v_mfma_f32_16x16x32_bf16: lane l holds A[l%16][8(l/16)+i] and
 // B[8(l/16)+i][l%16], i = 0..7; the accumulators are those of v_mfma_f32_16x16x4f32.
-// Tile 128 x 128, 256 threads (2 x 2 waves of 64 x 64), K chunks of 32.  The split happens
+// Tile 128 x 128, 512 threads (4 x 2 waves of 32 x 64), K chunks of 32.  The split happens
 // when a chunk moves from registers to LDS: A as [part][row][k], B as [part][col][k], 64-byte
 // rows whose four 16-byte segments are XOR-swizzled by (row / 4) % 4, so the 16 rows a
-// fragment read touches fall in distinct banks.  48 KiB of LDS; two workgroups per CU.
+// fragment read touches fall in distinct banks.  48 KiB of LDS and 116 VGPRs: two workgroups
+// (16 waves) per CU, so one wave's split and stores overlap another's MFMAs.
 // Fast shapes only (see gemm_tile): A rows and C rows 16-byte aligned, K % 32 == 0,
 // N % 4 == 0, M, N >= 128; B is read element-wise (its rows need no alignment).
 // ---------------------------------------------------------------------------------
@@ -926,12 +927,12 @@ __device__ __forceinline__ void split_bf16x3(float x, __bf16& p0, __bf16& p1, __
 // byte offset of (row, k) in one 128 x 32 BF16 part
 __device__ __forceinline__ int x6_off(int row, int k) { return row * 64 + 16 * ((k >> 3) ^ ((row >> 2) & 3)) + 2 * (k & 7); }
 
-__global__ __launch_bounds__(256) void k_gemm_x6(int M, int N, int K, float alpha, const float* __restrict__ A, size_t lda,
-                                                 const float* __restrict__ B, size_t ldb, float beta, float* __restrict__ C,
-                                                 size_t ldc, const int* gate, int gp, int band) {
+__global__ __launch_bounds__(512, 2) void k_gemm_x6(int M, int N, int K, float alpha, const float* __restrict__ A, size_t lda,
+                                                    const float* __restrict__ B, size_t ldb, float beta, float* __restrict__ C,
+                                                    size_t ldc, const int* gate, int gp, int band) {
   typedef float f4v __attribute__((ext_vector_type(4)));
-  constexpr int GB = 128, BK = 32, MT = 4, PART = GB * BK * 2;
-  constexpr int SC = GB + 4, CROWS = 64, SEG = GB / 4, PER = CROWS * SEG / 256;
+  constexpr int GB = 128, BK = 32, MA = 2, MB = 4, PART = GB * BK * 2, NT = 512;
+  constexpr int SC = GB + 4, CROWS = 64, SEG = GB / 4, PER = CROWS * SEG / NT;
   if (gated_off(gate, gp)) return;
   __shared__ __attribute__((aligned(16))) char smem[6 * PART];   // A parts 0-2, B parts 0-2; later the staged C rows
   char* As = smem;
@@ -939,7 +940,7 @@ __global__ __launch_bounds__(256) void k_gemm_x6(int M, int N, int K, float alph
   float* Cs = reinterpret_cast<float*>(smem);
   static_assert(CROWS * SC * 4 <= 6 * PART, "staged C rows exceed the LDS of the operands");
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 64;   // 8 waves in a 4 x 2 grid of 32 x 64 sub-tiles
   const int li = lane & 15, lq = lane >> 4;
   int ty = blockIdx.y, tx = blockIdx.x;
   if (band > 0) {
@@ -954,31 +955,31 @@ __global__ __launch_bounds__(256) void k_gemm_x6(int M, int N, int K, float alph
   }
   const int bm = min(ty * GB, M - GB), bn = min(tx * GB, N - GB);   // tile origin (inside C)
   const int sm = ty * GB - bm, sn = tx * GB - bn;                  // first row / column stored
-  f4v acc[MT][MT];
+  f4v acc[MA][MB];
 #pragma unroll
-  for (int a = 0; a < MT; a++)
+  for (int a = 0; a < MA; a++)
 #pragma unroll
-    for (int b = 0; b < MT; b++) acc[a][b] = f4v{0, 0, 0, 0};
-  // per chunk, thread: A elements (tid + 256 i) * 4 .. + 3 of the 128 x 32 tile (k fastest);
-  // B column bn + (tid % 128), k = 4 (tid / 128 + 2 i) + 0..3
-  f4v ra[4];
-  float rb[16];
+    for (int b = 0; b < MB; b++) acc[a][b] = f4v{0, 0, 0, 0};
+  // per chunk, thread: A elements (tid + 512 i) * 4 .. + 3 of the 128 x 32 tile (k fastest);
+  // B column bn + (tid % 128), k = 4 (tid / 128 + 4 i) + 0..3
+  f4v ra[2];
+  float rb[8];
   const int bc = tid & 127, bkg = tid >> 7;
   auto fetch = [&](int k0) {
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int e = (tid + i * 256) * 4;
+    for (int i = 0; i < 2; i++) {
+      const int e = (tid + i * NT) * 4;
       ra[i] = *reinterpret_cast<const f4v*>(A + (size_t)(bm + e / BK) * lda + k0 + e % BK);
     }
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < 2; i++)
 #pragma unroll
-      for (int j = 0; j < 4; j++) rb[4 * i + j] = B[(size_t)(k0 + 4 * (bkg + 2 * i) + j) * ldb + bn + bc];
+      for (int j = 0; j < 4; j++) rb[4 * i + j] = B[(size_t)(k0 + 4 * (bkg + 4 * i) + j) * ldb + bn + bc];
   };
   auto stash = [&]() {
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int e = (tid + i * 256) * 4;
+    for (int i = 0; i < 2; i++) {
+      const int e = (tid + i * NT) * 4;
       bf16x4_t p0, p1, p2;
 #pragma unroll
       for (int v = 0; v < 4; v++) {
@@ -992,7 +993,7 @@ __global__ __launch_bounds__(256) void k_gemm_x6(int M, int N, int K, float alph
       *reinterpret_cast<bf16x4_t*>(As + 2 * PART + off) = p2;
     }
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
+    for (int i = 0; i < 2; i++) {
       bf16x4_t p0, p1, p2;
 #pragma unroll
       for (int v = 0; v < 4; v++) {
@@ -1000,7 +1001,7 @@ __global__ __launch_bounds__(256) void k_gemm_x6(int M, int N, int K, float alph
         split_bf16x3(rb[4 * i + v], x0, x1, x2);
         p0[v] = x0; p1[v] = x1; p2[v] = x2;
       }
-      const int off = x6_off(bc, 4 * (bkg + 2 * i));
+      const int off = x6_off(bc, 4 * (bkg + 4 * i));
       *reinterpret_cast<bf16x4_t*>(Bs + off) = p0;
       *reinterpret_cast<bf16x4_t*>(Bs + PART + off) = p1;
       *reinterpret_cast<bf16x4_t*>(Bs + 2 * PART + off) = p2;
@@ -1012,19 +1013,20 @@ __global__ __launch_bounds__(256) void k_gemm_x6(int M, int N, int K, float alph
     stash();
   }
   __syncthreads();
-  auto mma_chunk = [&]() {
-    bf16x8_t af[MT][3];
+  for (int kt = 0; kt < nk; kt++) {
+    if (kt + 1 < nk) fetch((kt + 1) * BK);
+    bf16x8_t af[MA][3];
 #pragma unroll
-    for (int a = 0; a < MT; a++)
+    for (int a = 0; a < MA; a++)
 #pragma unroll
       for (int p = 0; p < 3; p++) af[a][p] = *reinterpret_cast<const bf16x8_t*>(As + p * PART + x6_off(wm + a * 16 + li, 8 * lq));
 #pragma unroll
-    for (int b = 0; b < MT; b++) {
+    for (int b = 0; b < MB; b++) {
       bf16x8_t bq[3];
 #pragma unroll
       for (int p = 0; p < 3; p++) bq[p] = *reinterpret_cast<const bf16x8_t*>(Bs + p * PART + x6_off(wn + b * 16 + li, 8 * lq));
 #pragma unroll
-      for (int a = 0; a < MT; a++) {   // smallest products first
+      for (int a = 0; a < MA; a++) {   // smallest products first
         f4v c = acc[a][b];
         c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a][2], bq[0], c, 0, 0, 0);
         c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a][1], bq[1], c, 0, 0, 0);
@@ -1034,17 +1036,13 @@ __global__ __launch_bounds__(256) void k_gemm_x6(int M, int N, int K, float alph
         acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a][0], bq[0], c, 0, 0, 0);
       }
     }
-  };
-  for (int kt = 0; kt < nk; kt++) {
-    if (kt + 1 < nk) fetch((kt + 1) * BK);
-    mma_chunk();
     if (kt + 1 < nk) {
       __syncthreads();   // every wave is done reading the chunk
       stash();
     }
     __syncthreads();
   }
-  // write back through LDS, 64 tile rows per pass, 8 segments of C in flight per thread.  C is
+  // write back through LDS, 64 tile rows per pass, 4 segments of C in flight per thread.  C is
   // read once and written once: it streams past the caches (non-temporal), which leaves L2 to
   // the A and B panels that the other tiles of the launch read again
   const bool use_c = beta != 0.0f;
@@ -1053,10 +1051,10 @@ __global__ __launch_bounds__(256) void k_gemm_x6(int M, int N, int K, float alph
     const int r0 = pass * CROWS;
     if (pass > 0) __syncthreads();
 #pragma unroll
-    for (int a = 0; a < MT; a++) {
+    for (int a = 0; a < MA; a++) {
       if (wm + a * 16 < r0 || wm + a * 16 >= r0 + CROWS) continue;
 #pragma unroll
-      for (int b = 0; b < MT; b++)
+      for (int b = 0; b < MB; b++)
 #pragma unroll
         for (int v = 0; v < 4; v++) Cs[(wm + a * 16 + 4 * lq + v - r0) * SC + wn + b * 16 + li] = alpha * acc[a][b][v];
     }
@@ -1064,12 +1062,12 @@ __global__ __launch_bounds__(256) void k_gemm_x6(int M, int N, int K, float alph
     f4v cv[PER];
 #pragma unroll
     for (int j = 0; j < PER; j++) {
-      const int idx = tid + j * 256, row = r0 + idx / SEG, c = (idx % SEG) * 4;
+      const int idx = tid + j * NT, row = r0 + idx / SEG, c = (idx % SEG) * 4;
       cv[j] = use_c ? __builtin_nontemporal_load(reinterpret_cast<const f4v*>(C + (size_t)(bm + row) * ldc + bn + c)) : f4v{0, 0, 0, 0};
     }
 #pragma unroll
     for (int j = 0; j < PER; j++) {
-      const int idx = tid + j * 256, row = r0 + idx / SEG, c = (idx % SEG) * 4;
+      const int idx = tid + j * NT, row = r0 + idx / SEG, c = (idx % SEG) * 4;
       const f4v val = *reinterpret_cast<const f4v*>(&Cs[(row - r0) * SC + c]);
       f4v out;
 #pragma unroll
@@ -1567,7 +1565,7 @@ static void launch_gemm(dim3 grid, int M, int N, int K, T alpha, const T* A, siz
                         size_t ldc, hipStream_t st, const int* gate, int gp, unsigned long long* vm) {
   constexpr int BK = sizeof(T) == 8 ? 16 : 32;
   if constexpr (sizeof(T) == 4 && GB == 128 && FAST && !TM)
-    hipLaunchKernelGGL(k_gemm_x6, grid, dim3(256), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, gp, x6_band(K));
+    hipLaunchKernelGGL(k_gemm_x6, grid, dim3(512), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, gp, x6_band(K));
   else if constexpr (sizeof(T) == 4 && GB == 128 && FAST)
     hipLaunchKernelGGL((k_gemm_f32<TM>), grid, dim3(256), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, gp, vm);
   else
