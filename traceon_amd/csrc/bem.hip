@@ -516,7 +516,7 @@ hipError_t launch_src_from_buffers(const double* jac, const double* pos, int N, 
   return hipGetLastError();
 }
 
-constexpr int FILL_ROWS = 8;
+constexpr int FILL_ROWS = 4;
 hipError_t launch_fill(double* M, size_t ld, float* M32, size_t ld32, const RowInfo* rows, const double* src, int N, int row0,
                        int row1, int accumulate, hipStream_t st) {
   if (N <= 0 || row1 < row0) return hipSuccess;
