@@ -82,12 +82,37 @@ void fill_matrix_radial(double *matrix, const double *line_points, const uint8_t
                         const double *excitation_values, const double *jacobian_buffer, const double *pos_buffer,
                         size_t N_lines, size_t N_matrix, int lines_range_start, int lines_range_end);
 
-/* The singular diagonal.  The reference integrates the integrands of radial.c:182-222 with
-   scipy.integrate.quad (backend/__init__.py:688-708); here the whole integral is one call.
-   self_field_dot_normal_radial_integral returns the integral itself (the solver subtracts 1,
-   solver.py:245). */
+/* The singular diagonal's integrands (radial.c:182-222; backend/__init__.py:322-323), with
+   the reference's signatures, for scipy.integrate.quad through LowLevelCallable
+   (backend/__init__.py:688-708): user data is the (4,3) element, or the struct below.  Each
+   call evaluates one alpha on the GPU. */
+struct self_field_dot_normal_radial_args { double *line_points; double K; };   /* radial.c:198-201 */
+double self_potential_radial(double alpha, void *line_points);
+double self_field_dot_normal_radial(double alpha, void *args);
+/* The same integrals in one call each (no reference counterpart): a graded Gauss rule over
+   [-1, 1] on the GPU.  self_field_dot_normal_radial_integral returns the integral itself (the
+   solver subtracts 1, solver.py:245). */
 double self_potential_radial_integral(const double *line_points);
 double self_field_dot_normal_radial_integral(const double *line_points, double K);
+
+/* The flat-triangle helpers of the reference's FFI (utilities_3d.c:95-134, three_d.c:310-328;
+   backend/__init__.py:297-298, 327), evaluated on the GPU.  triangle_areas is used by the
+   mesher for coil triangles (mesher.py:249). */
+void normal_3d(const double *triangle, double *normal);
+void position_and_jacobian_3d(double alpha, double beta, const double *triangle, double *pos_out, double *jac);
+void triangle_areas(const double *triangles, double *out, size_t N);
+
+/* Entries of the reference's FFI table that lie off the radially symmetric path
+   (backend/__init__.py:289, 312-315): exported so the reference's loader binds every name of
+   its table; each records an error readable with tb_last_error() and returns NaN or leaves its
+   outputs untouched.  kronrod_adaptive (kronrod.c:43) integrates a host callback, which cannot
+   run on the GPU; the other four belong to 3D BEM. */
+double kronrod_adaptive(double (*f)(double, void *), double a, double b, void *args, double abs_tol, double rel_tol);
+void axial_coefficients_3d(const void *charges, const void *jacobian_buffer, const void *pos_buffer, void *trig_cos_buffer,
+                           void *trig_sin_buffer, size_t N_charges, const double *z, void *output_coeffs, size_t N_z);
+void fill_jacobian_buffer_current_three_d(const double *lines, double *jacobians, double *positions, double *directions, size_t N_lines);
+double potential_3d_derivs(const double *point, const double *z, const double *coeffs, size_t N_z);
+void field_3d_derivs(const double *point, double *field, const double *z, const double *coeffs, size_t N_z);
 
 /* radial.c:19-173, 449-521 (backend/__init__.py:305-307, 309-311, 317, 319-320) */
 double potential_radial(const double *point, const double *charges, const double *jacobian_buffer, const double *pos_buffer, size_t N);
@@ -165,6 +190,12 @@ int tb_lu_solve_f32(const float *LU, size_t lda, int N, const int *ipiv, double 
 int tb_residual(const double *A, size_t lda, int N, const double *B, const double *X, double *R, int nrhs,
                 unsigned long long *rmax, void *stream);
 int tb_axpy(double *X, const double *D, size_t n, void *stream);
+/* status (device int) <- 1 if a dependency wait of the last tb_lu_solve / tb_lu_solve_f32 on
+   this workspace passed its spin limit (its solution is then NaN), else 0; fp32 as the
+   factorisation.  The refinement loop reads it with the norms below (one host sync per step). */
+int tb_lu_solve_status(const void *workspace, int N, int fp32, int *status, void *stream);
+/* *out (device) = bit pattern of max |x[i]| over n doubles (NaN: the largest pattern) */
+int tb_absmax(const double *x, size_t n, unsigned long long *out, void *stream);
 int tb_convert_f64_f32(const double *S, size_t lds, float *D, size_t ldd, int rows, int cols, void *stream);
 /* C = beta C + alpha A B (row-major), the MFMA GEMM used by the factorisation */
 int tb_gemm_f64(int M, int N, int K, double alpha, const double *A, size_t lda, const double *B, size_t ldb, double beta, double *C,
@@ -236,6 +267,10 @@ typedef struct {
   long long *n_accepted; long long *n_attempted;
   double *t_final; double *h_final;
   double *traj; double *times; long long traj_cap; long long *traj_len;
+  /* second landing plane (landing2 non-null; NaN rows where never crossed): with the entrance
+     plane it records where reflected particles leave, beside the exit plane's landing */
+  double plane2_p[3], plane2_n[3], plane2_nn;
+  double *landing2;
 } tb_trace_params_t;
 
 /* Persistent RK45 tracer over params->n particles (all pointers device pointers).

@@ -339,6 +339,78 @@ def quintic_spline_coefficients(z, derivs):
     return c
 
 
+# -- whole-workload drivers (the reference's Python steps around the C calls) --------------
+def _threads(threads=None):
+    return threads or max(1, len(os.sched_getaffinity(0)))
+
+
+def run_threaded(fn, items, threads=None):
+    """fn(item) over items on a thread pool (the C calls release the GIL), results in order."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(_threads(threads)) as ex:
+        return list(ex.map(fn, items))
+
+
+def axial_derivatives_threaded(lib, z, q, jac, pos, current=False, threads=None):
+    """field.py:983: the on-axis derivatives of order 0..8 split over threads by z."""
+    z = np.ascontiguousarray(z, dtype=np.float64)
+    chunks = [c for c in np.array_split(np.arange(len(z)), 4 * _threads(threads)) if len(c)]
+    f = lib.current_axial_derivatives_radial if current else lib.axial_derivatives_radial
+    parts = run_threaded(lambda c: f(z[c], q, jac, pos), chunks, threads)
+    return np.concatenate(parts, axis=0)
+
+
+def centres_and_normals(lib, lines):
+    """Element centres (alpha = 0) and normals there, solver.py:100-109, 142-147."""
+    c = np.array([lib.position_and_jacobian_radial(0.0, ln)[1] for ln in lines])
+    n = np.array([lib.higher_order_normal_radial(0.0, ln) for ln in lines])
+    return c, n
+
+
+def coil_currents(lib, triangles, total_current):
+    """solver.py:317-348: ring sources of a coil's triangles with density I / area."""
+    cj, cp = lib.fill_jacobian_buffer_3d(triangles)
+    return np.full(len(cj), total_current / np.sum(cj)), cj, cp
+
+
+def magnetizable_rhs(lib, lines, K, I, cj, cp):
+    """solver.py:118-150 for magnetizable rows: -factor(mu_r) (H_pre(centre) . n), H_pre the
+    coil field (current_field_radial)."""
+    c, n = centres_and_normals(lib, lines)
+    H = np.array(run_threaded(lambda p: lib.current_field_radial(np.array([p[0], 0.0, p[1]]), I, cj, cp), list(c)))
+    fac = 2.0 * (K - 1.0) / (1.0 + K)
+    return -fac * (H[:, 0] * n[:, 0] + H[:, 2] * n[:, 1])
+
+
+def refine_long_double(M, F, x, iters=3, block=1024):
+    """Iterative refinement of x against M with residuals in long double (80-bit): returns the
+    solution of the assembled system to about the rounding of FP64, and the factor's pivots."""
+    import scipy.linalg as sla
+    lu = sla.lu_factor(M)
+    xl = np.asarray(x, dtype=np.longdouble).copy()
+    Fl = np.asarray(F, dtype=np.longdouble)
+    for _ in range(iters):
+        r = np.empty(len(F), dtype=np.longdouble)
+        for r0 in range(0, len(F), block):
+            Ml = M[r0:r0 + block].astype(np.longdouble)
+            r[r0:r0 + block] = Fl[r0:r0 + block] - Ml @ xl
+        xl += sla.lu_solve(lu, r.astype(np.float64))
+    return xl.astype(np.float64)
+
+
+def landing(lib, states, exit_plane, back_plane=None):
+    """The landing state: where the trajectory last crosses exit_plane, else back_plane
+    (plane_intersection, tracing.c:184-223); NaN row if neither is crossed."""
+    for pl in (exit_plane, back_plane):
+        if pl is None:
+            continue
+        try:
+            return lib.plane_intersection(states, pl[0], pl[1]), pl
+        except ValueError:
+            pass
+    return np.full(6, np.nan), None
+
+
 _cache = {}
 
 

@@ -25,17 +25,39 @@ def declared_data():
     return re.findall(r"extern const \w+ (\w+);", open(HEADER).read())
 
 
+TABLE = os.path.join(ROOT, "tests", "golden", "backend_functions.json")
+
+
+def reference_table():
+    """The reference's FFI table (traceon/backend/__init__.py:287-328), written by
+    tests/golden/make_backend_table.py."""
+    import json
+    return json.load(open(TABLE))
+
+
 def test_header_lists_the_reference_ffi():
     names = declared_functions()
-    for ref_name in ["ellipkm1", "ellipk", "ellipem1", "ellipe", "normal_2d", "higher_order_normal_radial",
-                     "position_and_jacobian_radial", "delta_position_and_jacobian_radial", "trace_particle",
-                     "potential_radial_ring", "dr1_potential_radial_ring", "dz1_potential_radial_ring", "axial_derivatives_radial",
-                     "potential_radial", "potential_radial_derivs", "flux_density_to_charge_factor", "charge_radial", "field_radial",
-                     "field_radial_derivs", "current_potential_axial_radial_ring", "current_potential_axial",
-                     "current_field_radial_ring", "current_field_radial", "current_axial_derivatives_radial",
-                     "fill_jacobian_buffer_radial", "fill_matrix_radial", "fill_jacobian_buffer_3d", "plane_intersection",
-                     "field_radial_traceable", "field_radial_derivs_traceable"]:
-        assert ref_name in names, ref_name
+    table = reference_table()
+    # the table, plus the two traceable field functions the reference passes by name (field.py:816, 1066)
+    want = [e["name"] for e in table["functions"]] + ["field_radial_traceable", "field_radial_derivs_traceable"]
+    assert len(table["functions"]) == 38
+    missing = [n for n in want if n not in names]
+    assert not missing, missing
+    assert sorted(table["data"]) == sorted(declared_data())
+
+
+def test_reference_loader_binds_every_table_entry():
+    """The reference's loader loop (traceon/backend/__init__.py:341-354) run over this library:
+    getattr of every table name, then restype and an argtypes list of the table's length.  It
+    raises AttributeError at the first name a library lacks."""
+    lib = C.CDLL(os.path.join(ROOT, "traceon_amd", "libtraceon_hip.so"))
+    kinds = {"dbl": C.c_double, "sz": C.c_size_t, "bool": C.c_bool, "void": None}
+    for e in reference_table()["functions"]:
+        f = getattr(lib, e["name"])
+        f.restype = kinds[e["result"]]
+        f.argtypes = [C.c_void_p] * e["nargs"]
+    for d in reference_table()["data"]:
+        (C.c_size_t if d == "TRACING_BLOCK_SIZE" else C.c_int).in_dll(lib, d)
 
 
 def test_library_exports_every_declared_symbol():
@@ -137,3 +159,30 @@ def test_library_binds_to_torchs_hip_runtime():
     assert out.returncode == 0, out.stderr
     n, libs = out.stdout.splitlines()[:2]
     assert n == "2", libs
+
+
+def test_off_path_entries_record_an_error():
+    """The table's entries that lie off the radial path (kronrod_adaptive, the 3D series) are
+    bound by the loader but refuse work: NaN and an error message, no exit (host only)."""
+    lib = C.CDLL(os.path.join(ROOT, "traceon_amd", "libtraceon_hip.so"))
+    lib.tb_last_error.restype = C.c_char_p
+    cb = C.CFUNCTYPE(C.c_double, C.c_double, C.c_void_p)(lambda x, a: x)
+    lib.kronrod_adaptive.restype = C.c_double
+    lib.kronrod_adaptive.argtypes = [cb.__class__, C.c_double, C.c_double, C.c_void_p, C.c_double, C.c_double]
+    lib.tb_clear_error()
+    assert np.isnan(lib.kronrod_adaptive(cb, 0.0, 1.0, None, 1e-9, 1e-9))
+    assert b"kronrod_adaptive" in lib.tb_last_error()
+    lib.potential_3d_derivs.restype = C.c_double
+    lib.potential_3d_derivs.argtypes = [C.c_void_p] * 3 + [C.c_size_t]
+    lib.tb_clear_error()
+    assert np.isnan(lib.potential_3d_derivs(None, None, None, 0))
+    assert b"3D" in lib.tb_last_error()
+
+
+def test_solve_status_maps_to_an_error():
+    """A triangular solve whose dependency wait passed its spin limit reports status 1; the
+    solver turns it into an exception instead of returning NaN charges (host logic only)."""
+    from traceon_amd import device
+    device.check_solve_status(0)
+    with pytest.raises(RuntimeError, match="spin limit"):
+        device.check_solve_status(1)

@@ -1,0 +1,122 @@
+"""The golden fixtures of the benchmark workloads (tests/golden/config_c*.npz, written from the
+reference's own C backend by tests/golden/make_config_fixtures.py) against the workload
+definitions and against the oracle's independent C restatement (CPU only):
+
+- each fixture's geometry hash and beam equal what traceon_amd/configs.py builds now;
+- sampled entries (a matrix row, diagonal elements, fields at points, one trajectory per
+  workload) recomputed by the restatement agree with the fixture to rounding, so the fixtures
+  hold the reference's results and not an artefact of one build."""
+import ast
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+MM = 1e-3
+QM = -1.602176634e-19 / 9.1093837015e-31
+
+
+def load(name):
+    path = os.path.join(HERE, "golden", name)
+    if not os.path.isfile(path):
+        pytest.skip(f"{name} absent")
+    f = np.load(path, allow_pickle=False)
+    return f, ast.literal_eval(str(f["meta"]))
+
+
+def h(lines):
+    return hashlib.sha256(np.ascontiguousarray(lines, dtype=np.float64).tobytes()).hexdigest()
+
+
+def rel(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / np.max(np.abs(b)))
+
+
+def test_c0_fixture_vs_restatement(oracle):
+    from traceon_amd import configs
+    f, meta = load("config_c0.npz")
+    mesh, volts = configs.einzel_lens(3000)
+    lines, types, values = configs.lines_types_values(mesh, volts)
+    assert h(lines) == meta["lines_sha256"]
+    P, V, bounds, plane = configs.einzel_beam(100)
+    assert np.array_equal(P, f["P"]) and np.array_equal(V, f["V"])
+    jac, pos = oracle.fill_jacobian_buffer_radial(lines)
+    i = int(f["rows_index"][1])
+    M = np.zeros((len(lines), len(lines)))
+    oracle.fill_matrix_rows(M, lines, types, values, jac, pos, i, i)
+    got, ref = M[i].copy(), f["rows"][1].copy()
+    got[i] = ref[i] = 0.0
+    assert np.abs(got - ref).sum() <= 1e-12 * np.abs(ref).sum()   # -ffast-math in the reference build
+    for k in (0, 1234, 2999):
+        assert abs(oracle.self_potential_quad(lines[k]) - f["diag"][k]) <= 1e-12 * abs(f["diag"][k])
+    q = f["q_lapack"]
+    for k in (0, 57, 199):
+        p = f["pts"][k]
+        assert abs(oracle.potential_radial(p, q, jac, pos) - f["pot"][k]) <= 1e-12 * np.max(np.abs(f["pot"]))
+    # one trajectory, restated tracer: same steps, landing to rounding
+    t, s = oracle.trace_bem(P[3], V[3], QM, (q, jac, pos), bounds=bounds)
+    land = oracle.plane_intersection(s, *plane)
+    assert len(s) - 1 == f["trace_steps"][3]
+    assert np.max(np.abs(land[:3] - f["trace_landing"][3, :3])) < 1e-12
+
+
+def test_c1_fixture_vs_restatement(oracle):
+    from traceon_amd import configs
+    f, meta = load("config_c1.npz")
+    mesh, volts = configs.mirror(20000, seed=1)
+    lines, types, values = configs.lines_types_values(mesh, volts)
+    assert h(lines) == meta["lines_sha256"]
+    jac, pos = oracle.fill_jacobian_buffer_radial(lines)
+    for k in (0, 17):
+        p = f["pts"][k]
+        assert abs(oracle.potential_radial(p, f["q_lapack"], jac, pos) - f["pot"][k]) <= 1e-12 * np.max(np.abs(f["pot"]))
+    # the refined charges solve the reference's system at least as well as LAPACK's
+    assert rel(f["q_refined"], f["q_lapack"]) < 1e-8
+
+
+def test_c2_c3_c4_fixtures_vs_workloads(oracle):
+    from traceon_amd import configs
+    from traceon_amd.excitation import Excitation
+    f2, m2 = load("config_c2.npz")
+    mesh, volts = configs.reflecting_lens(8000)
+    lines, types, values = configs.lines_types_values(mesh, volts)
+    assert h(lines) == m2["lines_sha256"]
+    P, V, bounds, exit_plane, back_plane = configs.reflecting_beam(1_000_000)
+    assert np.array_equal(P[:len(f2["P"])], f2["P"])
+    # about a fifth of the beam reflects (the workload's definition)
+    frac = np.mean(f2["axial_plane"] == 1)
+    assert 0.12 < frac < 0.32, frac
+    # an axial trajectory recomputed by the restatement from the fixture's own derivatives
+    from oracle.oracle import quintic_spline_coefficients as co
+    z = f2["z"]
+    jac, pos = oracle.fill_jacobian_buffer_radial(lines)
+    d = oracle.axial_derivatives_radial(z, f2["q"], jac, pos)
+    assert rel(d[::8], f2["derivs"]) < 1e-12
+    c = co(z, d.T)
+    k = int(np.argmax(f2["axial_plane"] == 1))      # a reflected particle
+    t, s = oracle.trace_axial(P[k], V[k], QM, z, c, np.zeros_like(c), bounds)
+    assert len(s) - 1 == f2["axial_steps"][k]
+    land = oracle.plane_intersection(s, *back_plane)
+    assert np.max(np.abs(land[:3] - f2["axial_landing"][k, :3])) < 1e-12
+    f3, m3 = load("config_c3.npz")
+    mesh, volts, perms, currents = configs.magnetic_lens(12000)
+    exc = Excitation(mesh)
+    exc.add_voltage(**volts)
+    exc.add_magnetizable(**perms)
+    exc.add_current(**currents)
+    assert h(exc.get_electrostatic_active_elements()[0]) == m3["lines_sha256_e"]
+    assert h(exc.get_magnetostatic_active_elements()[0]) == m3["lines_sha256_m"]
+    P3 = configs.magnetic_beam(1_000_000)[0]
+    assert np.array_equal(P3[:len(f3["P"])], f3["P"])
+    assert np.all(f3["axial_plane"] == 0) and np.all(f3["bem_plane"] == 0)
+    f4, m4 = load("config_c4.npz")
+    P4 = configs.magnetic_beam(64, seed=4, r_range=(0.3 * MM, 0.5 * MM))[0]
+    assert np.array_equal(P4, f4["P"])
+    r = np.hypot(P4[:, 0], P4[:, 1])
+    assert r.min() >= 0.3 * MM and r.max() <= 0.5 * MM
+    # off the axis the paraxial series alone misses the reference's BEM landings by nanometres
+    ok = f4["bem_plane"] == 0
+    err = np.linalg.norm(f4["axial_landing"][ok, :2] - f4["bem_landing"][ok, :2], axis=1)
+    assert np.sqrt(np.mean(err ** 2)) > 1e-9

@@ -404,6 +404,37 @@ def test_mixed_precision_falls_back_to_fp64(gpu):
         assert err < cond * 1e-15
 
 
+def test_fp32_singular_matrix_falls_back_to_fp64(gpu):
+    """A column scaled by 1e-46 is all zeros once rounded to FP32 (below its smallest
+    subnormal), so the FP32 LU meets an exactly zero first pivot and reports the matrix
+    singular.  In FP64 the column scaling only rescales x0 (partial pivoting is invariant under
+    it), so the system is as well conditioned as the unscaled one, and the mixed solve answers
+    it through the FP64 LU instead of raising."""
+    import torch
+    from traceon_amd import device
+    n = 200
+    rng = np.random.default_rng(2)
+    Bm = np.eye(n) * 4.0 + 0.1 * rng.standard_normal((n, n))
+    s = np.ones(n)
+    s[0] = 1e-46
+    A = Bm * s[None, :]
+    M64 = torch.tensor(A, device="cuda")
+    M32 = M64.float()
+    assert float(M32[:, 0].abs().max().item()) == 0.0
+    b = rng.standard_normal((1, n))
+    with pytest.raises(np.linalg.LinAlgError):
+        device.lu_factor(M32.clone())
+    X, st, _ = device.solve_system(M64, b, M32=M32.clone(), mode="mixed")
+    x = X.cpu().numpy()[0]
+    ref = np.linalg.solve(Bm, b[0]) / s        # the unscaled system's solution, rescaled
+    err0, err1 = abs(x[0] - ref[0]) / abs(ref[0]), rel(x[1:], ref[1:])
+    print(st, "rel vs LAPACK: x0", err0, "rest", err1, flush=True)
+    assert st.mode == "mixed->fp64"
+    assert err0 < 1e-12 and err1 < 1e-12
+    with pytest.raises(np.linalg.LinAlgError):
+        device.solve_system(M64, b, M32=M32.clone(), mode="mixed", fallback=False)
+
+
 def test_step_cap_retires_with_status(small):
     """A particle that reaches the attempted-step cap is retired with status 'step cap' and
     the cap's step count, in the persistent and the stage-synchronous tracer."""

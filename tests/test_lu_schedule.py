@@ -36,24 +36,30 @@ def view(buf, off, ld, rows, cols):
 
 
 def region(buf, off, ld, rows, cols):
-    if rows <= 0 or cols <= 0:
-        return (buf.name, frozenset())
-    return (buf.name, frozenset(off + r * ld + c for r in range(rows) for c in range(cols)))
+    """A rows x cols block of a buffer, materialised as a set of cells only by the race check."""
+    return (buf.name, (off, ld, max(rows, 0), max(cols, 0)))
+
+
+def cells(reg):
+    off, ld, rows, cols = reg
+    return frozenset(off + r * ld + c for r in range(rows) for c in range(cols))
 
 
 class Pass:
-    def __init__(self, A0, lda, pair_min, quad_min, fail_at=()):
+    def __init__(self, A0, lda, pair_min, quad_min, fail_at=(), nb=NB):
+        self.NB, self.GMAX = nb, GMAX
+        self.dry = False   # bounds only: no arithmetic (set by Driver)
         N = A0.shape[0]
-        self.N, self.lda, self.nblk = N, lda, (N + NB - 1) // NB
+        self.N, self.lda, self.nblk = N, lda, (N + self.NB - 1) // self.NB
         self.ldu = (N + 31) // 32 * 32
         self.A = Buf("A", N * lda)
         view(self.A, 0, lda, N, N)[...] = A0
-        self.Lp = Buf("Lp", 2 * N * GMAX * NB)
-        self.Up = Buf("Up", 2 * GMAX * NB * self.ldu)
-        self.Ltmp = Buf("Ltmp", 2 * N * NB)
-        self.U12 = Buf("U12", NB * N)
-        self.U12b = Buf("U12b", NB * N)
-        self.Dbak = np.zeros((NB, NB))
+        self.Lp = Buf("Lp", 2 * N * self.GMAX * self.NB)
+        self.Up = Buf("Up", 2 * self.GMAX * self.NB * self.ldu)
+        self.Ltmp = Buf("Ltmp", 2 * N * self.NB)
+        self.U12 = Buf("U12", self.NB * N)
+        self.U12b = Buf("U12b", self.NB * N)
+        self.Dbak = np.zeros((self.NB, self.NB))
         self.Y, self.Z, self.lmax = {}, {}, 0.0
         self.fail_at = set(fail_at)
         self.pair_min, self.quad_min = pair_min, quad_min
@@ -72,6 +78,11 @@ class Pass:
     def wait(self, s, ev):
         self.q[s].append(("wait", (ev, self.ver.get(ev, 0))))
 
+    def set_identity_pivots(self, k0, nb):
+        """k_diag_block with factor = 1 writes ipiv[k0 + i] = k0 + i for its panel."""
+        if hasattr(self, "ipiv"):
+            self.ipiv[k0:k0 + nb] = np.arange(k0, k0 + nb)
+
     def gated(self, gp):
         return gp is not None and self.first_fail <= gp
 
@@ -82,7 +93,10 @@ class Pass:
         def fn():
             if self.gated(gp):
                 return
-            a, b, c = view(Ab, ao, la, M, K).copy(), view(Bb, bo, lb, K, Nn).copy(), view(Cb, co, lc, M, Nn)
+            a, b, c = view(Ab, ao, la, M, K), view(Bb, bo, lb, K, Nn), view(Cb, co, lc, M, Nn)
+            if self.dry:
+                return
+            a, b = a.copy(), b.copy()
             assert not np.isnan(a).any() and not np.isnan(b).any(), ("unwritten operand", Ab.name, Bb.name)
             c[...] = alpha * (a @ b) if beta == 0 else beta * c + alpha * (a @ b)
         rc = region(Cb, co, lc, M, Nn)
@@ -90,14 +104,17 @@ class Pass:
 
     def diag_l21(self, q, Cb, co, ldc):
         N, lda = self.N, self.lda
-        k0 = q * NB
-        nb = min(NB, N - k0)
+        k0 = q * self.NB
+        nb = min(self.NB, N - k0)
         k1, m2 = k0 + nb, N - k0 - nb
 
         def diag():
             if self.gated(q):
                 return
             D = view(self.A, k0 * lda + k0, lda, nb, nb)
+            self.set_identity_pivots(k0, nb)
+            if self.dry:
+                return
             self.Dbak[:nb, :nb] = D
             L, U = np.eye(nb), D.copy()
             for j in range(nb):
@@ -114,7 +131,10 @@ class Pass:
                 if self.gated(q):
                     return
                 c = view(Cb, co, ldc, m2, nb)
-                c[...] = view(self.A, k1 * lda + k0, lda, m2, nb) @ self.Z[q]
+                a = view(self.A, k1 * lda + k0, lda, m2, nb)
+                if self.dry:
+                    return
+                c[...] = a @ self.Z[q]
                 self.lmax = max(self.lmax, np.abs(c).max())
             self.enq("sa", l21, rd=(region(self.A, k1 * lda + k0, lda, m2, nb),), wr=(region(Cb, co, ldc, m2, nb),))
 
@@ -129,7 +149,9 @@ class Pass:
 
         def fn():
             if not self.gated(q):
-                view(Cb, co, ldc, M, Nn)[...] = self.Y[q][:M, :M] @ view(self.A, Bo, self.lda, M, Nn)
+                c, b = view(Cb, co, ldc, M, Nn), view(self.A, Bo, self.lda, M, Nn)
+                if not self.dry:
+                    c[...] = self.Y[q][:M, :M] @ b
         self.enq("sa", fn, rd=(region(self.A, Bo, self.lda, M, Nn),), wr=(region(Cb, co, ldc, M, Nn),))
 
     def copy_panel(self, gp, Lb, lo, ldl, a21, Ub, uo, ldu, a12, rows, cols, nb):
@@ -146,32 +168,32 @@ class Pass:
     def speculate(self, s0=0):
         N, lda, nblk, ldu = self.N, self.lda, self.nblk, self.ldu
         self.first_fail = nblk
-        LDL = GMAX * NB
+        LDL = self.GMAX * self.NB
         Lgr = lambda gi: (gi & 1) * N * LDL
         Ugr = lambda gi: (gi & 1) * LDL * ldu
         q, gi = s0, 0
         while True:
-            G = 4 if N - q * NB >= self.quad_min else 2 if N - q * NB >= self.pair_min else 1
+            G = 4 if N - q * self.NB >= self.quad_min else 2 if N - q * self.NB >= self.pair_min else 1
             while G > 1 and q + G >= nblk:
                 G //= 2
             if G == 1:
                 break
-            b0, bG = q * NB, q * NB + G * NB
-            b2G = min(bG + G * NB, N)
+            b0, bG = q * self.NB, q * self.NB + G * self.NB
+            b2G = min(bG + G * self.NB, N)
             Lq, Uq = Lgr(gi), Ugr(gi)
             for j in range(G):
-                bj, bj1 = b0 + j * NB, b0 + j * NB + NB
-                self.diag_l21(q + j, self.Lp, Lq + bj1 * LDL + j * NB, LDL)
-                self.u12(q + j, NB, N - bj1, bj * lda + bj1, self.Up, Uq + j * NB * ldu + bj1, ldu)
+                bj, bj1 = b0 + j * self.NB, b0 + j * self.NB + self.NB
+                self.diag_l21(q + j, self.Lp, Lq + bj1 * LDL + j * self.NB, LDL)
+                self.u12(q + j, self.NB, N - bj1, bj * lda + bj1, self.Up, Uq + j * self.NB * ldu + bj1, ldu)
                 if j + 1 == G:
                     break
-                Lj, Uj = Lq + bj1 * LDL + j * NB, Uq + j * NB * ldu
-                self.gemm("sa", q + j, N - bj1, bG - bj1, NB, -1, self.Lp, Lj, LDL, self.Up, Uj + bj1, ldu, 1, self.A,
+                Lj, Uj = Lq + bj1 * LDL + j * self.NB, Uq + j * self.NB * ldu
+                self.gemm("sa", q + j, N - bj1, bG - bj1, self.NB, -1, self.Lp, Lj, LDL, self.Up, Uj + bj1, ldu, 1, self.A,
                           bj1 * lda + bj1, lda)
-                self.gemm("sa", q + j, bG - bj1, N - bG, NB, -1, self.Lp, Lj, LDL, self.Up, Uj + bG, ldu, 1, self.A,
+                self.gemm("sa", q + j, bG - bj1, N - bG, self.NB, -1, self.Lp, Lj, LDL, self.Up, Uj + bG, ldu, 1, self.A,
                           bj1 * lda + bG, lda)
             self.groups.append((q, G, gi))
-            gl, KG = q + G - 1, G * NB
+            gl, KG = q + G - 1, G * self.NB
             if gi > 0:
                 self.wait("sa", "ev_r")
             self.gemm("sa", gl, N - bG, b2G - bG, KG, -1, self.Lp, Lq + bG * LDL, LDL, self.Up, Uq + bG, ldu, 1, self.A,
@@ -183,24 +205,24 @@ class Pass:
             self.gemm("st", gl, N - b2G, N - b2G, KG, -1, self.Lp, Lq + b2G * LDL, LDL, self.Up, Uq + b2G, ldu, 1, self.A,
                       b2G * lda + b2G, lda)
             for j in range(G):
-                bj, bj1 = b0 + j * NB, b0 + j * NB + NB
-                self.copy_panel(q + j, self.Lp, Lq + bj1 * LDL + j * NB, LDL, bj1 * lda + bj, self.Up, Uq + j * NB * ldu + bj1,
-                                ldu, bj * lda + bj1, N - bj1, N - bj1, NB)
+                bj, bj1 = b0 + j * self.NB, b0 + j * self.NB + self.NB
+                self.copy_panel(q + j, self.Lp, Lq + bj1 * LDL + j * self.NB, LDL, bj1 * lda + bj, self.Up, Uq + j * self.NB * ldu + bj1,
+                                ldu, bj * lda + bj1, N - bj1, N - bj1, self.NB)
             self.record("st", "ev_r")
             q += G
             gi += 1
-        Lbuf = lambda q: (q & 1) * N * NB
+        Lbuf = lambda q: (q & 1) * N * self.NB
         Ubuf = lambda q: self.U12b if q & 1 else self.U12
         while q < nblk:
-            k0 = q * NB
-            nb = min(NB, N - k0)
+            k0 = q * self.NB
+            nb = min(self.NB, N - k0)
             m2 = n2 = N - k0 - nb
             k1 = k0 + nb
             self.diag_l21(q, self.Ltmp, Lbuf(q), nb)
             if n2 <= 0:
                 self.check(q)
                 break
-            nb1 = min(NB, n2)
+            nb1 = min(self.NB, n2)
             if q > s0:
                 self.wait("sa", "ev_r")
             self.u12(q, nb, n2, k0 * lda + k1, Ubuf(q), 0, n2)
@@ -245,15 +267,21 @@ class Pass:
                 arr.append(k)
             knows[s] = arr
         bad = 0
+        cache = {}
+
+        def memo(reg):
+            if reg not in cache:
+                cache[reg] = cells(reg)
+            return cache[reg]
         ops = {s: [(i, it) for i, it in enumerate(self.q[s]) if it[0] == "op"] for s in ("sa", "st")}
         for i, a in ops["sa"]:
             for j, b in ops["st"]:
                 if i < knows["st"][j] or j < knows["sa"][i]:
                     continue
                 for na, ra in a[3]:
-                    bad += sum(1 for nb_, rb in tuple(b[2]) + tuple(b[3]) if na == nb_ and ra & rb)
+                    bad += sum(1 for nb_, rb in tuple(b[2]) + tuple(b[3]) if na == nb_ and memo(ra) & memo(rb))
                 for nb_, rb in b[3]:
-                    bad += sum(1 for na, ra in a[2] if na == nb_ and ra & rb)
+                    bad += sum(1 for na, ra in a[2] if na == nb_ and memo(ra) & memo(rb))
         return bad
 
     def driver_after_failure(self):
@@ -262,15 +290,15 @@ class Pass:
         N, lda, kf = self.N, self.lda, self.first_fail
         for gq, G, gi in self.groups:
             if gq < kf < gq + G:
-                j, bG, LDL = kf - gq, (gq + G) * NB, GMAX * NB
+                j, bG, LDL = kf - gq, (gq + G) * self.NB, self.GMAX * self.NB
                 M = N - bG
                 if M > 0:
-                    a = view(self.Lp, (gi & 1) * N * LDL + bG * LDL, LDL, M, j * NB)
-                    b = view(self.Up, (gi & 1) * LDL * self.ldu + bG, self.ldu, j * NB, M)
+                    a = view(self.Lp, (gi & 1) * N * LDL + bG * LDL, LDL, M, j * self.NB)
+                    b = view(self.Up, (gi & 1) * LDL * self.ldu + bG, self.ldu, j * self.NB, M)
                     c = view(self.A, bG * lda + bG, lda, M, M)
                     c[...] = c - a @ b
-        k0 = kf * NB
-        nb = min(NB, N - k0)
+        k0 = kf * self.NB
+        nb = min(self.NB, N - k0)
         view(self.A, k0 * lda + k0, lda, nb, nb)[...] = self.Dbak[:nb, :nb]
 
 
@@ -325,3 +353,160 @@ def test_model_detects_a_missing_wait():
     P.speculate()
     P.q["sa"] = [it for it in P.q["sa"] if not (it[0] == "wait" and it[1][0] == "ev_r")]
     assert P.unordered_conflicts() > 0
+
+
+# ---------------------------------------------------------------------------------------
+# The driver around the passes (lu_factor's loop in lu.hip): speculative pass from panel q;
+# after a failure at kf, the recovery product of a failure inside a group, the restore of
+# kf's diagonal block from its copy, the pivoted panel (k_panel, cooperative over G
+# workgroups), k_diag_block with factor = 0, k_laswp, U12 = inv(L11) A12 into the U12 buffer,
+# its copy into the matrix and the A22 update; then the next pass from kf + 1.
+# ---------------------------------------------------------------------------------------
+WS_G = 1024   # LuWorkspace::G: pivot-candidate slots of the cooperative panel
+
+
+class Driver(Pass):
+    def __init__(self, A0, lda, pair_min, quad_min, nb=NB, G=5, dry=False, forced=()):
+        super().__init__(A0, lda, pair_min, quad_min, fail_at=forced, nb=nb)
+        self.dry = dry
+        self.G = G
+        N, nblk = self.N, self.nblk
+        self.ipiv = np.full(N, -1, dtype=np.int64)
+        self.cand_row = Buf("cand_row", 2 * WS_G * nb + 2 * nb)
+        self.Linv = Buf("Linv", nblk * nb * nb)
+        self.Uinv = Buf("Uinv", nblk * nb * nb)
+        self.Dbak_ws = Buf("Dbak", nb * nb)
+        self.launches = 0
+
+    def pivoted_panel(self, q):
+        N, lda, NB, G = self.N, self.lda, self.NB, self.G
+        k0 = q * NB
+        nb = min(NB, N - k0)
+        m = N - k0
+        # k_panel: workgroup g holds rows [k0 + g R, +R) of the panel in LDS (R x NB, the dynamic
+        # LDS set for Rmax = ceil(N / G) rows) and reads / writes those rows' panel columns
+        R, Rmax = -(-m // G), -(-N // G)
+        assert R <= Rmax and G <= WS_G
+        for g in range(G):
+            rbeg = k0 + g * R
+            rend = min(rbeg + R, N)
+            if rend > rbeg:
+                view(self.A, rbeg * lda + k0, lda, rend - rbeg, nb)
+        view(self.cand_row, 0, NB, 2 * G + 2, NB)
+        Pn = view(self.A, k0 * lda + k0, lda, m, nb)
+        for j in range(nb):   # LAPACK's choice: largest |a|, first index on ties
+            d = k0 + j
+            if self.dry:
+                p = N - 1 - (d * 7919) % (N - d)   # any row in [d, N): bounds only
+            else:
+                p = d + int(np.argmax(np.abs(Pn[j:, j])))
+                Pn[[j, p - k0]] = Pn[[p - k0, j]]
+                Pn[j + 1:, j] /= Pn[j, j]
+                Pn[j + 1:, j + 1:] -= np.outer(Pn[j + 1:, j], Pn[j, j + 1:])
+            assert d <= p < N
+            self.ipiv[d] = p
+        # k_diag_block, factor = 0: the block (all of it through 16-byte segments when
+        # nb == NB, else element loads inside nb x nb), inverses into the solve's slots
+        blk = NB if nb == NB else nb
+        view(self.A, k0 * lda + k0, lda, blk, blk)
+        Y, Z = view(self.Linv, q * NB * NB, NB, NB, NB), view(self.Uinv, q * NB * NB, NB, NB, NB)
+        if not self.dry:
+            D = view(self.A, k0 * lda + k0, lda, nb, nb)
+            L = np.tril(D, -1) + np.eye(nb)
+            self.Y[q], self.Z[q] = np.linalg.inv(L), np.linalg.inv(np.triu(D))
+            Y[...] = 0.0
+            Y[:nb, :nb] = self.Y[q]
+        # k_laswp: one thread per column outside [k0, k0 + nb), swaps in panel order
+        cols = [c for c in (np.arange(0, k0), np.arange(k0 + nb, N)) if len(c)]
+        for j in range(nb):
+            d, p = k0 + j, int(self.ipiv[k0 + j])
+            assert 0 <= p < N
+            for c in cols:
+                view(self.A, d * lda + int(c[0]), lda, 1, len(c))
+                view(self.A, p * lda + int(c[0]), lda, 1, len(c))
+            if p != d and not self.dry:
+                full = view(self.A, 0, lda, N, N)
+                for c in cols:
+                    full[[d, p], c[0]:c[-1] + 1] = full[[p, d], c[0]:c[-1] + 1]
+        n2 = N - k0 - nb
+        if n2 > 0:
+            a12 = k0 * lda + k0 + nb
+            self.direct_gemm(nb, n2, nb, 1.0, self.Linv, q * NB * NB, NB, self.A, a12, lda, 0.0, self.U12, 0, n2)
+            u, a = view(self.U12, 0, n2, nb, n2), view(self.A, a12, lda, nb, n2)
+            if not self.dry:
+                a[...] = u
+            self.direct_gemm(n2, n2, nb, -1.0, self.A, (k0 + nb) * lda + k0, lda, self.U12, 0, n2, 1.0, self.A,
+                             (k0 + nb) * lda + k0 + nb, lda)
+
+    def direct_gemm(self, M, Nn, K, alpha, Ab, ao, la, Bb, bo, lb, beta, Cb, co, lc):
+        a, b, c = view(Ab, ao, la, M, K), view(Bb, bo, lb, K, Nn), view(Cb, co, lc, M, Nn)
+        if not self.dry:
+            c[...] = alpha * (a @ b) if beta == 0 else beta * c + alpha * (a @ b)
+
+    def factor(self, first="sa"):
+        q = 0
+        nblk = self.nblk
+        while q < nblk:
+            self.q, self.groups = {"sa": [], "st": []}, []
+            self.speculate(q)
+            self.run(first)
+            kf = self.first_fail
+            if kf < q or kf >= nblk:
+                break
+            if not self.dry:
+                self.driver_after_failure()
+            else:   # the same launches, bounds only
+                for gq, G, gi in self.groups:
+                    if gq < kf < gq + G:
+                        bG, LDL = (gq + G) * self.NB, self.GMAX * self.NB
+                        Mr = self.N - bG
+                        view(self.Lp, (gi & 1) * self.N * LDL + bG * LDL, LDL, Mr, (kf - gq) * self.NB)
+                        view(self.Up, (gi & 1) * LDL * self.ldu + bG, self.ldu, (kf - gq) * self.NB, Mr)
+                        view(self.A, bG * self.lda + bG, self.lda, Mr, Mr)
+                k0 = kf * self.NB
+                nbk = min(self.NB, self.N - k0)
+                view(self.A, k0 * self.lda + k0, self.lda, nbk, nbk)
+            self.pivoted_panel(kf)
+            q = kf + 1
+        assert np.all((self.ipiv >= np.arange(self.N)) & (self.ipiv < self.N))
+
+
+@pytest.mark.parametrize("N", [NB * 9 + 3, NB * 16, NB * 21 + 1, NB * 13 + 4])
+@pytest.mark.parametrize("mode", list(MODES))
+def test_pivoted_factorisation_matches_lapack(N, mode):
+    """The whole driver on matrices that need row interchanges (random, no diagonal dominance):
+    natural failures of the no-pivot test, the pivoted panels and the resumed passes give
+    LAPACK's getrf factors and pivots, under both stream orders and with a padded leading
+    dimension; also with extra forced failures at panels that would have passed."""
+    import scipy.linalg as sla
+    pair_min, quad_min = MODES[mode]
+    rng = np.random.default_rng(N + 7)
+    A0 = rng.normal(size=(N, N))
+    A0[:, :NB * 2] += 3 * np.eye(N)[:, :NB * 2]     # the first panels pass, later ones pivot
+    lu, piv = sla.lu_factor(A0)
+    nblk = (N + NB - 1) // NB
+    for lda in (N, N + 5):
+        for forced in ((), (1, nblk - 1)):
+            for first in ("sa", "st"):
+                D = Driver(A0, lda, pair_min, quad_min, forced=forced)
+                D.factor(first)
+                got = view(D.A, 0, lda, N, N)
+                assert np.array_equal(D.ipiv, piv), (N, mode, lda, forced, first)
+                assert np.abs(got - lu).max() <= 1e-10 * np.abs(lu).max(), (N, mode, lda, forced, first)
+
+
+@pytest.mark.parametrize("N", [5004, 5000, 4097, 4109, 8204, 8193, 2060])
+def test_pivoted_driver_bounds_at_the_gpu_panel_width(N):
+    """Bounds of every launch of the driver at the GPU's panel width (NB = 128) and 256
+    cooperative workgroups, for sizes whose last panel is 1-15 columns wide (N mod 128 < 16)
+    among others, with the no-pivot test failing at every panel, at every third and only at the
+    last: the pivoted panel's rows, the diagonal-block loads, the row interchanges (pivot rows
+    anywhere below the diagonal), the U12 and A22 products and the group recovery product all
+    stay inside their buffers."""
+    nb = 128
+    nblk = (N + nb - 1) // nb
+    A0 = np.zeros((N, N))
+    for forced in (tuple(range(nblk)), tuple(range(0, nblk, 3)), (nblk - 1,)):
+        for lda in (N, N + 3):
+            D = Driver(A0, lda, 4096, 8192, nb=nb, G=256, dry=True, forced=forced)
+            D.factor("sa")

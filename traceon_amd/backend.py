@@ -83,6 +83,11 @@ _SIG = {
     "current_field_radial": (None, [v3, v3, arr(ndim=1), arr(ndim=2), arr(ndim=3), sz]),
     "current_axial_derivatives_radial": (None, [arr(ndim=2), arr(ndim=1), arr(ndim=2), arr(ndim=3), sz, arr(ndim=1), sz]),
     "fill_jacobian_buffer_radial": (None, [arr(ndim=2), arr(ndim=3), arr(ndim=3), sz]),
+    "self_potential_radial": (dbl, [dbl, vp]),
+    "self_field_dot_normal_radial": (dbl, [dbl, vp]),
+    "normal_3d": (None, [arr(shape=(3, 3)), v3]),
+    "position_and_jacobian_3d": (None, [dbl, dbl, arr(ndim=2), v3, dbl_p]),
+    "triangle_areas": (None, [arr(ndim=3), arr(ndim=1), sz]),
     "self_potential_radial_integral": (dbl, [arr(shape=(4, 3))]),
     "self_field_dot_normal_radial_integral": (dbl, [arr(shape=(4, 3)), dbl]),
     "fill_matrix_radial": (None, [arr(ndim=2), arr(ndim=3), arr(dtype=np.uint8, ndim=1), arr(ndim=1), arr(ndim=2), arr(ndim=3),
@@ -107,6 +112,8 @@ _SIG = {
     "tb_lu_solve_f32": (i32, [vp, sz, i32, vp, vp, i32, vp, vp]),
     "tb_residual": (i32, [vp, sz, i32, vp, vp, vp, i32, vp, vp]),
     "tb_axpy": (i32, [vp, vp, sz, vp]),
+    "tb_lu_solve_status": (i32, [vp, i32, i32, vp, vp]),
+    "tb_absmax": (i32, [vp, sz, vp, vp]),
     "tb_convert_f64_f32": (i32, [vp, sz, vp, sz, i32, i32, vp]),
     "tb_gemm_f64": (i32, [i32, i32, i32, dbl, vp, sz, vp, sz, dbl, vp, sz, vp]),
     "tb_gemm_f32": (i32, [i32, i32, i32, C.c_float, vp, sz, vp, sz, C.c_float, vp, sz, vp]),
@@ -326,6 +333,35 @@ def fill_jacobian_buffer_3d(vertices):
     if N:
         _call("fill_jacobian_buffer_3d", jac_buffer, pos_buffer, vertices, N)
     return jac_buffer, pos_buffer
+
+
+def normal_3d(tri) -> np.ndarray:
+    """Unit normal of a flat triangle (backend/__init__.py:377-380)."""
+    tri = np.ascontiguousarray(tri, dtype=np.float64)
+    assert tri.shape == (3, 3)
+    normal = np.zeros(3)
+    _call("normal_3d", tri, normal)
+    return normal
+
+
+def position_and_jacobian_3d(alpha: float, beta: float, triangle):
+    """(jac, pos) of a flat triangle's affine map (backend/__init__.py:424-432)."""
+    triangle = np.ascontiguousarray(triangle, dtype=np.float64)
+    assert triangle.shape == (3, 3)
+    pos = np.zeros(3)
+    jac = dbl(0.0)
+    _call("position_and_jacobian_3d", float(alpha), float(beta), triangle, pos, C.pointer(jac))
+    return jac.value, pos
+
+
+def triangle_areas(triangles) -> np.ndarray:
+    """Areas of flat triangles (backend/__init__.py:776-780)."""
+    triangles = np.ascontiguousarray(triangles, dtype=np.float64)
+    assert triangles.shape == (len(triangles), 3, 3)
+    out = np.zeros(len(triangles))
+    if len(triangles):
+        _call("triangle_areas", triangles, out, len(triangles))
+    return out
 
 
 def self_potential_radial(vertices) -> float:

@@ -126,3 +126,79 @@ def lines_types_values(mesh: Mesh, volts: dict, flux: dict | None = None, flux_t
         active[idx] = True
     lines = mesh.points[mesh.lines.astype(np.int64)]
     return lines[active], types[active], values[active]
+
+
+# ---------------------------------------------------------------------------------------
+# Beams of the tracing workloads: (positions (n,3), velocities (n,3) m/s), tracer bounds
+# (3,2) and landing planes (point, normal).  Electrons; seeded, so every consumer draws the
+# same particles.
+# ---------------------------------------------------------------------------------------
+E_CHARGE = 1.602176634e-19
+M_ELECTRON = 9.1093837015e-31
+
+
+def _speed(eV):
+    return np.sqrt(2.0 * np.asarray(eV, dtype=np.float64) * E_CHARGE / M_ELECTRON)
+
+
+def _directions(rng, n, sigma):
+    tx, ty = rng.normal(0.0, sigma, n), rng.normal(0.0, sigma, n)
+    d = np.column_stack([np.tan(tx), np.tan(ty), np.ones(n)])
+    return d / np.linalg.norm(d, axis=1, keepdims=True)
+
+
+def einzel_beam(n: int = 100, seed: int = 0):
+    """Plain einzel workload: electrons at 1 keV from z = -5 mm with radial offset uniform in
+    [0, 0.1] mm (along x) and velocity along +z, traced to z = +10 mm; landing on z = +10 mm."""
+    rng = np.random.default_rng(seed)
+    r0 = rng.uniform(0.0, 0.1 * MM, n)
+    P = np.column_stack([r0, np.zeros(n), np.full(n, -5.0 * MM)])
+    V = np.zeros((n, 3))
+    V[:, 2] = _speed(1000.0)
+    bounds = np.array([[-1.0 * MM, 1.0 * MM], [-1.0 * MM, 1.0 * MM], [-5.0 * MM, 10.0 * MM]])
+    plane = (np.array([0.0, 0.0, 10.0 * MM]), np.array([0.0, 0.0, 1.0]))
+    return P, V, bounds, plane
+
+
+# The batched-tracing lens: the einzel lens plus a -1.1 kV tube of 0.08 mm at z = 4 mm, whose
+# saddle potential reflects about a fifth (22%) of the lognormal 1 keV beam below.
+REFLECTOR = (4.0 * MM, 0.08 * MM, -1100.0)
+
+
+def reflecting_lens(n_elements: int = 8000):
+    return einzel_lens(n_elements, voltages=(0.0, 1800.0, 0.0), fourth=REFLECTOR)
+
+
+def reflecting_beam(n: int, seed: int = 2):
+    """Batched-tracing workload: kinetic energy 1000 eV * exp(0.5 N(0,1)), start points uniform
+    in a disc of 0.2 mm diameter at z = -5 mm, angles N(0, 5 mrad).  Transmitted electrons land
+    on z = +9.5 mm, reflected ones on their way back through z = -4.5 mm (the last crossing of
+    that plane, as plane_intersection scans backwards, tracing.c:184-223)."""
+    rng = np.random.default_rng(seed)
+    energy = 1000.0 * np.exp(0.5 * rng.standard_normal(n))
+    rad = 0.1 * MM * np.sqrt(rng.random(n))
+    phi = 2 * np.pi * rng.random(n)
+    d = _directions(rng, n, 5e-3)
+    P = np.column_stack([rad * np.cos(phi), rad * np.sin(phi), np.full(n, -5.0 * MM)])
+    V = d * _speed(energy)[:, None]
+    bounds = np.array([[-1.0 * MM, 1.0 * MM], [-1.0 * MM, 1.0 * MM], [-5.0 * MM, 10.0 * MM]])
+    exit_plane = (np.array([0.0, 0.0, 9.5 * MM]), np.array([0.0, 0.0, 1.0]))
+    back_plane = (np.array([0.0, 0.0, -4.5 * MM]), np.array([0.0, 0.0, 1.0]))
+    return P, V, bounds, exit_plane, back_plane
+
+
+def magnetic_beam(n: int, seed: int = 3, r_range=(0.0, 0.1 * MM)):
+    """Magnetic + electrostatic workload: 10 keV electrons from z = -14 mm, start radius with
+    r^2 uniform over r_range (a disc for r_range = (0, 0.1 mm)), angles N(0, 5 mrad); landing on
+    z = +11 mm.  r_range = (0.3 mm, 0.5 mm) is the off-axis beam of the learned-surrogate
+    workload, where the paraxial series loses accuracy."""
+    rng = np.random.default_rng(seed)
+    lo, hi = r_range
+    rad = np.sqrt(lo * lo + (hi * hi - lo * lo) * rng.random(n))
+    phi = 2 * np.pi * rng.random(n)
+    d = _directions(rng, n, 5e-3)
+    P = np.column_stack([rad * np.cos(phi), rad * np.sin(phi), np.full(n, -14.0 * MM)])
+    V = d * _speed(10000.0)
+    bounds = np.array([[-1.0 * MM, 1.0 * MM], [-1.0 * MM, 1.0 * MM], [-14.0 * MM, 12.0 * MM]])
+    plane = (np.array([0.0, 0.0, 11.0 * MM]), np.array([0.0, 0.0, 1.0]))
+    return P, V, bounds, plane

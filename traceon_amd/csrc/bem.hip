@@ -468,6 +468,54 @@ __global__ void k_line_eval(int op, const double* __restrict__ in, int n, double
   else { line4_normal(L, a[0], o[0], o[1]); o[2] = 0.0; }
 }
 
+// The integrands of the singular diagonal at a batch of parameter values:
+// self_potential_radial (kind 0) and self_field_dot_normal_radial (kind 1), radial.c:182-222.
+// The target is the element's own point at alpha = 0 (its normal there for kind 1), the
+// source the offset of the point at alpha.  in per item (alpha, K, line[12]); out: one value.
+__global__ void k_self_integrand(int kind, const double* __restrict__ in, int n, double* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double* a = in + 14 * (size_t)t;
+  const Line4 L = load_line4(a + 2);
+  double r0, z0, j0, dr, dz, j;
+  line4_point(L, 0.0, r0, z0, j0);
+  line4_delta(L, a[0], dr, dz, j);
+  if (kind == 0) {
+    out[t] = j * ring_potential(r0, dr, dz);
+  } else {
+    double nr, nz, gr, gz;
+    line4_normal(L, 0.0, nr, nz);
+    ring_grad(r0, dr, dz, gr, gz);
+    out[t] = j * -flux_factor(a[1]) * (nr * gr + nz * gz);   // factor * (n . E), E = -grad
+  }
+}
+
+// Flat-triangle helpers of the 3D utilities (utilities_3d.c:95-134, three_d.c:310-328):
+// op 0 normal_3d (unit normal of (v1 - v0) x (v2 - v0)), op 1 position_and_jacobian_3d
+// (v0 + alpha (v1 - v0) + beta (v2 - v0), jacobian = twice the area), op 2 the area.
+// in per item (alpha, beta, tri[9]); out per item 4 values.
+__global__ void k_tri_eval(int op, const double* __restrict__ in, int n, double* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double* a = in + 11 * (size_t)t;
+  const double* T = a + 2;
+  const double e1[3] = {T[3] - T[0], T[4] - T[1], T[5] - T[2]};
+  const double e2[3] = {T[6] - T[0], T[7] - T[1], T[8] - T[2]};
+  const double c[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+  const double len = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+  double* o = out + 4 * (size_t)t;
+  if (op == 0) {
+    for (int k = 0; k < 3; k++) o[k] = c[k] / len;
+    o[3] = 0.0;
+  } else if (op == 1) {
+    for (int k = 0; k < 3; k++) o[k] = T[k] + a[0] * e1[k] + a[1] * e2[k];
+    o[3] = len;   // 2 * area
+  } else {
+    o[0] = 0.5 * len;
+    o[1] = o[2] = o[3] = 0.0;
+  }
+}
+
 // plane_intersection (tracing.c:184-223): the last index i whose signed distance
 // differs in sign from i+1 (zero counts as negative), found in parallel with an atomic max.
 __global__ void k_plane_scan(const double* __restrict__ p0, const double* __restrict__ nrm, const double* __restrict__ S, int Np,
@@ -615,6 +663,18 @@ hipError_t launch_scalar(int op, const double* in, int n, double* out, hipStream
 hipError_t launch_line_eval(int op, const double* in, int n, double* out, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_line_eval, dim3(cdiv(n, 256)), dim3(256), 0, st, op, in, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_self_integrand(int kind, const double* in, int n, double* out, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_self_integrand, dim3(cdiv(n, 256)), dim3(256), 0, st, kind, in, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_tri_eval(int op, const double* in, int n, double* out, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tri_eval, dim3(cdiv(n, 256)), dim3(256), 0, st, op, in, n, out);
   return hipGetLastError();
 }
 

@@ -37,6 +37,8 @@ hipError_t launch_jac3d(const double* tri, int N, double* jac, double* pos, hipS
 hipError_t launch_charge(const double* lines, const double* q, int N, double* out, hipStream_t st);
 hipError_t launch_scalar(int op, const double* in, int n, double* out, hipStream_t st);
 hipError_t launch_line_eval(int op, const double* in, int n, double* out, hipStream_t st);
+hipError_t launch_self_integrand(int kind, const double* in, int n, double* out, hipStream_t st);
+hipError_t launch_tri_eval(int op, const double* in, int n, double* out, hipStream_t st);
 hipError_t launch_plane(const double* p0, const double* nrm, const double* S, int Np, int* found, double* out, hipStream_t st);
 
 }  // namespace tb

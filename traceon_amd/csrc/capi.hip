@@ -73,12 +73,44 @@ bool finish(const char* what) {
   return ok(hipDeviceSynchronize(), what);
 }
 
+// Device staging of the one-value entry points (scalar kernels, element geometry, the
+// diagonal's integrands, which QUADPACK calls about a thousand times per element): one
+// 64-double buffer per host thread and device, allocated at its first use and kept until the
+// process ends, so such a call costs a copy in, a launch and a copy out, and no allocation.
+constexpr size_t STAGE = 64;
+struct Staging {
+  int dev = -1;
+  double* p = nullptr;
+};
+thread_local Staging g_stage;
+
+double* staging() {
+  int dev = 0;
+  if (!ok(hipGetDevice(&dev), "hipGetDevice")) return nullptr;
+  if (g_stage.p && g_stage.dev == dev) return g_stage.p;
+  double* p = nullptr;
+  if (!ok(hipMalloc(&p, STAGE * sizeof(double)), "hipMalloc (staging)")) return nullptr;
+  g_stage.dev = dev;
+  g_stage.p = p;   // a buffer of another device stays allocated: the thread may return to it
+  return p;
+}
+
+// nin inputs in, `launch(din, dout)` on the default stream, nout outputs back (nin + nout <= STAGE)
+template <typename F>
+bool staged_call(const char* what, const double* in, size_t nin, double* out, size_t nout, F launch) {
+  double* d = staging();
+  if (!d) return false;
+  if (!ok(hipMemcpy(d, in, nin * sizeof(double), hipMemcpyHostToDevice), what)) return false;
+  launch(d, d + nin);
+  if (!finish(what)) return false;
+  return ok(hipMemcpy(out, d + nin, nout * sizeof(double), hipMemcpyDeviceToHost), what);
+}
+
 double scalar_op(int op, double a0, double a1 = 0, double a2 = 0, double a3 = 0, double* second = nullptr) {
-  double in[4] = {a0, a1, a2, a3}, out[4] = {0, 0, 0, 0};
-  Dev<double> din(in, 4), dout(4);
-  if (!din.good || !dout.good) return __builtin_nan("");
-  tb::launch_scalar(op, din.p, 1, dout.p, 0);
-  if (!finish("scalar") || !dout.get(out, 4)) return __builtin_nan("");
+  const double in[4] = {a0, a1, a2, a3};
+  double out[4] = {0, 0, 0, 0};
+  if (!staged_call("scalar", in, 4, out, 4, [&](double* di, double* dout) { tb::launch_scalar(op, di, 1, dout, 0); }))
+    return __builtin_nan("");
   if (second) *second = out[1];
   return out[0];
 }
@@ -90,11 +122,31 @@ void line_op(int op, double alpha, const double* v1, const double* v2, const dou
   // rows 0, 2, 3, 1 of the (4,3) element are the nodes at alpha = -1, -1/3, 1/3, 1
   const double* rowsrc[4] = {v1, v4, v2, v3};
   for (int r = 0; r < 4; r++) { in[1 + 3 * r] = rowsrc[r][0]; in[2 + 3 * r] = 0.0; in[3 + 3 * r] = rowsrc[r][2]; }
-  Dev<double> din(in, 13), dout(3);
-  if (!din.good || !dout.good) return;
-  tb::launch_line_eval(op, din.p, 1, dout.p, 0);
-  if (finish("line_eval")) dout.get(out3, 3);
+  staged_call("line_eval", in, 13, out3, 3, [&](double* di, double* dout) { tb::launch_line_eval(op, di, 1, dout, 0); });
 }
+
+// (alpha, beta, triangle (3,3)) -> 4 values of k_tri_eval
+bool tri_op(int op, double alpha, double beta, const double* tri, double out4[4]) {
+  double in[11];
+  in[0] = alpha;
+  in[1] = beta;
+  std::memcpy(in + 2, tri, 9 * sizeof(double));
+  return staged_call("triangle", in, 11, out4, 4, [&](double* di, double* dout) { tb::launch_tri_eval(op, di, 1, dout, 0); });
+}
+
+// one value of a diagonal integrand (kind 0 potential, 1 flux with K)
+double self_integrand(int kind, double alpha, double K, const double* line) {
+  double in[14], out = __builtin_nan("");
+  in[0] = alpha;
+  in[1] = K;
+  std::memcpy(in + 2, line, 12 * sizeof(double));
+  staged_call("self integrand", in, 14, &out, 1, [&](double* di, double* dout) { tb::launch_self_integrand(kind, di, 1, dout, 0); });
+  return out;
+}
+
+// entry points of the reference's FFI that lie off the radial path: they record why and
+// return NaN / leave their outputs untouched
+void off_path(const char* name, const char* why) { set_err(std::string(name) + ": " + why); }
 
 bool valid_type(uint8_t t) { return t == 1 || t == 2 || t == 3 || t == 5 || t == 6; }
 
@@ -224,6 +276,61 @@ static double self_integral(const double* line, int type, double K) {
 }
 double self_potential_radial_integral(const double* line) { return self_integral(line, 1, 0.0); }
 double self_field_dot_normal_radial_integral(const double* line, double K) { return self_integral(line, 3, K); }
+
+// The integrands themselves, with the reference's signatures (radial.c:182-222), for
+// scipy.integrate.quad through LowLevelCallable (backend/__init__.py:688-708): each call
+// evaluates one alpha on the GPU.
+double self_potential_radial(double alpha, void* line_points) {
+  if (!line_points) { set_err("self_potential_radial: null element"); return __builtin_nan(""); }
+  return self_integrand(0, alpha, 0.0, (const double*)line_points);
+}
+double self_field_dot_normal_radial(double alpha, void* args) {
+  const auto* a = (const self_field_dot_normal_radial_args*)args;
+  if (!a || !a->line_points) { set_err("self_field_dot_normal_radial: null arguments"); return __builtin_nan(""); }
+  return self_integrand(1, alpha, a->K, a->line_points);
+}
+
+// ---- the reference FFI's 3D and generic entries (backend/__init__.py:289, 297-298, 312-315, 327) ----
+double kronrod_adaptive(double (*f)(double, void*), double a, double b, void* args, double abs_tol, double rel_tol) {
+  (void)f; (void)a; (void)b; (void)args; (void)abs_tol; (void)rel_tol;
+  off_path("kronrod_adaptive", "integrates a host callback, which cannot run on the GPU; not on the radial path");
+  return __builtin_nan("");
+}
+void normal_3d(const double* tri, double* normal) {
+  double o[4];
+  if (tri_op(0, 0.0, 0.0, tri, o)) for (int k = 0; k < 3; k++) normal[k] = o[k];
+}
+void position_and_jacobian_3d(double alpha, double beta, const double* tri, double* pos_out, double* jac) {
+  double o[4];
+  if (tri_op(1, alpha, beta, tri, o)) {
+    for (int k = 0; k < 3; k++) pos_out[k] = o[k];
+    *jac = o[3];
+  }
+}
+void triangle_areas(const double* triangles, double* out, size_t N) {
+  if (!N) return;
+  std::vector<double> in(11 * N, 0.0);
+  for (size_t i = 0; i < N; i++) std::memcpy(&in[11 * i + 2], triangles + 9 * i, 9 * sizeof(double));
+  Dev<double> din(in.data(), in.size()), dout(4 * N);
+  if (!din.good || !dout.good) return;
+  tb::launch_tri_eval(2, din.p, (int)N, dout.p, 0);
+  std::vector<double> o(4 * N);
+  if (!finish("triangle_areas") || !dout.get(o.data(), o.size())) return;
+  for (size_t i = 0; i < N; i++) out[i] = o[4 * i];
+}
+void axial_coefficients_3d(const void*, const void*, const void*, void*, void*, size_t, const double*, void*, size_t) {
+  off_path("axial_coefficients_3d", "3D BEM is outside the radially symmetric path of this library");
+}
+void fill_jacobian_buffer_current_three_d(const double*, double*, double*, double*, size_t) {
+  off_path("fill_jacobian_buffer_current_three_d", "3D current lines are outside the radially symmetric path of this library");
+}
+double potential_3d_derivs(const double*, const double*, const double*, size_t) {
+  off_path("potential_3d_derivs", "3D BEM is outside the radially symmetric path of this library");
+  return __builtin_nan("");
+}
+void field_3d_derivs(const double*, double*, const double*, const double*, size_t) {
+  off_path("field_3d_derivs", "3D BEM is outside the radially symmetric path of this library");
+}
 
 double potential_radial(const double* point, const double* q, const double* jac, const double* pos, size_t N) {
   DevSources s(q, jac, pos, N, 16, 2, 1);
@@ -479,6 +586,11 @@ int tb_residual(const double* A, size_t lda, int N, const double* B, const doubl
   TB_CALL(tb::residual(A, lda, N, B, X, R, nrhs, rmax, S(stream)), "residual");
 }
 int tb_axpy(double* X, const double* D, size_t n, void* stream) { TB_CALL(tb::axpy(X, D, n, S(stream)), "axpy"); }
+int tb_lu_solve_status(const void* ws, int N, int fp32, int* status, void* stream) {
+  if (fp32) TB_CALL(tb::lu_solve_status<float>(ws, N, status, S(stream)), "lu_solve_status");
+  TB_CALL(tb::lu_solve_status<double>(ws, N, status, S(stream)), "lu_solve_status");
+}
+int tb_absmax(const double* x, size_t n, unsigned long long* out, void* stream) { TB_CALL(tb::absmax(x, n, out, S(stream)), "absmax"); }
 int tb_convert_f64_f32(const double* Sx, size_t lds, float* D, size_t ldd, int rows, int cols, void* stream) {
   TB_CALL(tb::convert_d2f(Sx, lds, D, ldd, rows, cols, S(stream)), "convert");
 }

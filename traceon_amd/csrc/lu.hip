@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_cooperative_groups.h>
 #include <stdint.h>
+#include <algorithm>
 #include <cstdlib>
 #include <mutex>
 #include <utility>
@@ -1150,6 +1151,24 @@ __global__ __launch_bounds__(256) void k_zero_u64(unsigned long long* __restrict
   if (i < n) p[i] = 0ull;
 }
 
+// max |x| over n values as the bit pattern of a nonnegative double (NaN as the largest
+// pattern, so a caller sees a non-finite maximum); *out zeroed by the launcher's first kernel
+__global__ __launch_bounds__(256) void k_absmax(const double* __restrict__ x, size_t n, unsigned long long* __restrict__ out) {
+  unsigned long long m = 0ull;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const double a = fabs(x[i]);
+    const unsigned long long b = a != a ? 0x7FFFFFFFFFFFFFFFull : (unsigned long long)__double_as_longlong(a);
+    m = b > m ? b : m;
+  }
+  for (int o = 32; o >= 1; o >>= 1) {
+    const unsigned long long y = __shfl_xor(m, o, 64);
+    m = y > m ? y : m;
+  }
+  if ((threadIdx.x & 63) == 0 && m) atomicMax(out, m);
+}
+
+__global__ void k_copy_int(const int* __restrict__ src, int* __restrict__ dst) { *dst = *src; }
+
 // panel p of the speculative pass: max |l| of the diagonal block (*lmax) and of the
 // workgroups of the L21 product (slots[0 .. nslot)); record p as failed when some |l| > 1
 // (or NaN).  Bit patterns of nonnegative doubles order as integers.
@@ -2021,6 +2040,9 @@ hipError_t lu_solve(const T* LU, size_t lda, int N, const int* ipiv, double* X, 
   if (nrhs > 64) return hipErrorInvalidValue;
   const LuWorkspace<T> W(const_cast<void*>(ws), N);
   const int nblk = (N + LU_NB - 1) / LU_NB;
+  // a wait of the triangular solves that passed its spin limit records itself in W.err: cleared
+  // per solve call, read back with lu_solve_status
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, W.err, 0);
   // the right-hand sides are read through the permutation into a copy
   hipMemcpyAsync(W.rhs, X, sizeof(double) * (size_t)nrhs * N, hipMemcpyDeviceToDevice, st);
   constexpr size_t lds = trsv_lds_bytes<T>();
@@ -2049,6 +2071,24 @@ hipError_t residual(const double* A, size_t lda, int N, const double* B, const d
                     unsigned long long* rmax, hipStream_t st) {
   hipMemsetAsync(rmax, 0, sizeof(unsigned long long) * nrhs, st);
   hipLaunchKernelGGL(k_residual, dim3(cdiv(N, 4)), dim3(256), 0, st, A, lda, N, B, X, R, nrhs, rmax);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t lu_solve_status(const void* ws, int N, int* status, hipStream_t st) {
+  const LuWorkspace<T> W(const_cast<void*>(ws), N);
+  hipLaunchKernelGGL(k_copy_int, dim3(1), dim3(1), 0, st, (const int*)W.err, status);
+  return hipGetLastError();
+}
+template hipError_t lu_solve_status<double>(const void*, int, int*, hipStream_t);
+template hipError_t lu_solve_status<float>(const void*, int, int*, hipStream_t);
+
+hipError_t absmax(const double* x, size_t n, unsigned long long* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, out, 1);
+  if (n) {
+    const unsigned g = (unsigned)std::min<size_t>(cdiv(n, 256), 1024);
+    hipLaunchKernelGGL(k_absmax, dim3(g), dim3(256), 0, st, x, n, out);
+  }
   return hipGetLastError();
 }
 

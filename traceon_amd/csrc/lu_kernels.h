@@ -37,6 +37,12 @@ hipError_t lu_solve(const T* LU, size_t lda, int N, const int* ipiv, double* X, 
 hipError_t residual(const double* A, size_t lda, int N, const double* B, const double* X, double* R, int nrhs,
                     unsigned long long* rmax, hipStream_t st);
 hipError_t axpy(double* X, const double* D, size_t n, hipStream_t st);
+// W.err of the last lu_solve with this workspace (1: a wait of its triangular solves passed the
+// spin limit and the solution is NaN) into the device int *status
+template <typename T>
+hipError_t lu_solve_status(const void* ws, int N, int* status, hipStream_t st);
+// *out = bit pattern of max |x[i]| (a nonnegative double; NaN gives the largest pattern)
+hipError_t absmax(const double* x, size_t n, unsigned long long* out, hipStream_t st);
 hipError_t convert_d2f(const double* S, size_t lds, float* D, size_t ldd, int rows, int cols, hipStream_t st);
 
 }  // namespace tb

@@ -13,7 +13,9 @@
 // The landing state on a plane is tracked on the fly: after every accepted step the
 // signed distances of the previous and new point are compared exactly as
 // plane_intersection (tracing.c:184-223) does when it scans the stored trajectory
-// backwards, and the last crossing found is kept.
+// backwards; each crossing writes its interpolated state to the particle's landing row
+// (NaN from the particle's start), so the row ends with the last crossing.  An optional
+// second plane is tracked the same way.
 #include "radial_math.h"
 #include "trace_kernels.h"
 #include "mlp_kernels.h"
@@ -331,10 +333,23 @@ __device__ void eval_field(const FieldProgram& F, const double p[3], double E[3]
 // ---------------------------------------------------------------------------------
 // the persistent tracer
 // ---------------------------------------------------------------------------------
+__device__ __forceinline__ double plane_dist(const double* p0, const double* n, double nn, const double y[6]) {
+  return (n[0] * (p0[0] - y[0]) + n[1] * (p0[1] - y[1]) + n[2] * (p0[2] - y[2])) / nn;
+}
 __device__ __forceinline__ double signed_dist(const TraceParams& P, const double y[6]) {
-  const double* n = P.plane_n;
-  const double* p0 = P.plane_p;
-  return (n[0] * (p0[0] - y[0]) + n[1] * (p0[1] - y[1]) + n[2] * (p0[2] - y[2])) / P.plane_nn;
+  return plane_dist(P.plane_p, P.plane_n, P.plane_nn, y);
+}
+__device__ __forceinline__ double signed_dist2(const TraceParams& P, const double y[6]) {
+  return plane_dist(P.plane2_p, P.plane2_n, P.plane2_nn, y);
+}
+// a crossing between y (distance kp) and yn (distance kn): the interpolated state into row
+__device__ __forceinline__ void land_if_crossed(double kp, double kn, const double y[6], const double yn[6], double* row) {
+  if ((kn > 0) != (kp > 0)) {
+    const double diff = kp - kn;   // kappa(i) - kappa(i+1) with i the previous point
+    const double f = -kn / diff, fp = kp / diff;
+#pragma unroll
+    for (int c = 0; c < 6; c++) row[c] = fp * yn[c] + f * y[c];
+  }
 }
 
 __device__ __forceinline__ bool inside_trace_bounds(const double* b, const double y[6]) {
@@ -351,9 +366,7 @@ __device__ __forceinline__ void trace_body(const TraceParams& P) {
   int pid = -1;
   double y[6], h = 0.0, hmax = 0.0, t = 0.0, qm = 0.0;
   long long nacc = 0, natt = 0;
-  double kprev = 0.0;
-  int found = 0;
-  double land[6];
+  double kprev = 0.0, kprev2 = 0.0;
   long long traj_n = 0;
 
   auto next_particle = [&](bool need) {
@@ -377,8 +390,14 @@ __device__ __forceinline__ void trace_body(const TraceParams& P) {
         const double speed = sqrt(y[3] * y[3] + y[4] * y[4] + y[5] * y[5]);
         hmax = STEP_MAX / speed;   // TRACING_STEP_MAX, tracing.c:5, 68-70
         h = (P.h0 && P.h0[id] > 0.0) ? P.h0[id] : hmax;
-        nacc = 0; natt = 0; found = 0;
+        nacc = 0; natt = 0;
         kprev = signed_dist(P, y);
+        if (P.landing)
+          for (int c = 0; c < 6; c++) P.landing[6 * (size_t)id + c] = __builtin_nan("");
+        if (P.landing2) {
+          kprev2 = signed_dist2(P, y);
+          for (int c = 0; c < 6; c++) P.landing2[6 * (size_t)id + c] = __builtin_nan("");
+        }
         traj_n = 1;
         if (P.traj) {
           double* row = P.traj + (size_t)id * P.traj_cap * 6;
@@ -394,9 +413,6 @@ __device__ __forceinline__ void trace_body(const TraceParams& P) {
   auto retire = [&](int status) {
     const size_t id = (size_t)pid;
     for (int c = 0; c < 6; c++) P.final_state[6 * id + c] = y[c];
-    if (P.landing) {
-      for (int c = 0; c < 6; c++) P.landing[6 * id + c] = found ? land[c] : __builtin_nan("");
-    }
     if (P.status) P.status[id] = status;
     if (P.n_accepted) P.n_accepted[id] = nacc;
     if (P.n_attempted) P.n_attempted[id] = natt;
@@ -405,7 +421,7 @@ __device__ __forceinline__ void trace_body(const TraceParams& P) {
     if (P.traj_len) P.traj_len[id] = traj_n;
   };
 
-  for (int c = 0; c < 6; c++) { y[c] = 0.0; land[c] = 0.0; }
+  for (int c = 0; c < 6; c++) y[c] = 0.0;
   next_particle(true);
   while (true) {
     // a fresh particle that starts outside the bounds takes no step (tracing.c:79 loop not entered)
@@ -475,14 +491,13 @@ __device__ __forceinline__ void trace_body(const TraceParams& P) {
         nacc++;
         if (P.landing) {
           const double kn = signed_dist(P, yn);
-          if ((kn > 0) != (kprev > 0)) {
-            const double diff = kprev - kn;   // kappa(i) - kappa(i+1) with i the previous point
-            const double f = -kn / diff, fp = kprev / diff;
-#pragma unroll
-            for (int c = 0; c < 6; c++) land[c] = fp * yn[c] + f * y[c];
-            found = 1;
-          }
+          land_if_crossed(kprev, kn, y, yn, P.landing + 6 * (size_t)pid);
           kprev = kn;
+        }
+        if (P.landing2) {
+          const double kn = signed_dist2(P, yn);
+          land_if_crossed(kprev2, kn, y, yn, P.landing2 + 6 * (size_t)pid);
+          kprev2 = kn;
         }
 #pragma unroll
         for (int c = 0; c < 6; c++) y[c] = yn[c];
@@ -527,12 +542,17 @@ __global__ __launch_bounds__(64) void k_trace_axial(TraceParams P) {
 // SPLIT_BATCH attempts between reads of the active count.
 // ---------------------------------------------------------------------------------
 struct SplitWs {
-  double *y, *ys, *k, *part, *h, *hmax, *t, *kprev, *land, *qmv;
+  double *y, *ys, *k, *part, *h, *hmax, *t, *kprev, *kprev2, *qmv;
   long long *nacc, *natt, *tlen;
-  int *active, *found, *count;
+  int *active, *count;
+  size_t used = 0;   // bytes of the layout (a null base only measures it)
   SplitWs(void* base, int P, int nchunk) {
     char* b = (char*)base;
-    auto take = [&](size_t bytes) { char* p = b; b += (bytes + 255) & ~(size_t)255; return p; };
+    auto take = [&](size_t bytes) {
+      char* p = b ? b + used : nullptr;
+      used += (bytes + 255) & ~(size_t)255;
+      return p;
+    };
     y = (double*)take(sizeof(double) * 6 * P);
     ys = (double*)take(sizeof(double) * 6 * P);
     k = (double*)take(sizeof(double) * 36 * P);
@@ -541,20 +561,15 @@ struct SplitWs {
     hmax = (double*)take(sizeof(double) * P);
     t = (double*)take(sizeof(double) * P);
     kprev = (double*)take(sizeof(double) * P);
-    land = (double*)take(sizeof(double) * 6 * P);
+    kprev2 = (double*)take(sizeof(double) * P);
     qmv = (double*)take(sizeof(double) * P);
     nacc = (long long*)take(sizeof(long long) * P);
     natt = (long long*)take(sizeof(long long) * P);
     tlen = (long long*)take(sizeof(long long) * P);
     active = (int*)take(sizeof(int) * P);
-    found = (int*)take(sizeof(int) * P);
     count = (int*)take(sizeof(int) * 2);
   }
-  static size_t bytes(int P, int nchunk) {
-    auto r = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    return r(48 * (size_t)P) * 2 + r(288 * (size_t)P) + r(48 * (size_t)P * nchunk) + r(8 * (size_t)P) * 5 + r(48 * (size_t)P) +
-           r(8 * (size_t)P) * 3 + r(4 * (size_t)P) * 2 + r(8);
-  }
+  static size_t bytes(int P, int nchunk) { return SplitWs(nullptr, P, nchunk).used; }
 };
 constexpr int SPLIT_THREADS = 256;
 constexpr int SPLIT_BATCH = 8;
@@ -562,8 +577,6 @@ constexpr long long SPLIT_MAX_ATTEMPTS = 1ll << 22;
 
 __device__ __forceinline__ void split_retire(const TraceParams& P, const SplitWs& W, int i, int status) {
   for (int c = 0; c < 6; c++) P.final_state[6 * (size_t)i + c] = W.y[6 * (size_t)i + c];
-  if (P.landing)
-    for (int c = 0; c < 6; c++) P.landing[6 * (size_t)i + c] = W.found[i] ? W.land[6 * (size_t)i + c] : __builtin_nan("");
   if (P.status) P.status[i] = status;
   if (P.n_accepted) P.n_accepted[i] = W.nacc[i];
   if (P.n_attempted) P.n_attempted[i] = W.natt[i];
@@ -581,7 +594,8 @@ __global__ __launch_bounds__(256) void k_split_init(TraceParams P, SplitWs W) {
     y[c] = P.state0[6 * (size_t)i + c];
     W.y[6 * (size_t)i + c] = y[c];
     W.ys[6 * (size_t)i + c] = y[c];
-    W.land[6 * (size_t)i + c] = 0.0;
+    if (P.landing) P.landing[6 * (size_t)i + c] = __builtin_nan("");
+    if (P.landing2) P.landing2[6 * (size_t)i + c] = __builtin_nan("");
   }
   W.t[i] = P.t0 ? P.t0[i] : 0.0;
   W.qmv[i] = P.qm[P.qm_stride ? i : 0];
@@ -589,7 +603,8 @@ __global__ __launch_bounds__(256) void k_split_init(TraceParams P, SplitWs W) {
   W.hmax[i] = hm;
   W.h[i] = (P.h0 && P.h0[i] > 0.0) ? P.h0[i] : hm;
   W.kprev[i] = signed_dist(P, y);
-  W.nacc[i] = 0; W.natt[i] = 0; W.tlen[i] = 1; W.found[i] = 0;
+  W.kprev2[i] = P.landing2 ? signed_dist2(P, y) : 0.0;
+  W.nacc[i] = 0; W.natt[i] = 0; W.tlen[i] = 1;
   if (P.traj) {
     for (int c = 0; c < 6; c++) P.traj[(size_t)i * P.traj_cap * 6 + c] = y[c];
     if (P.times) P.times[(size_t)i * P.traj_cap] = W.t[i];
@@ -676,14 +691,14 @@ __global__ __launch_bounds__(256) void k_split_control(TraceParams P, SplitWs W)
     W.t[i] += h;
     W.nacc[i]++;
     if (P.landing) {
-      const double kn = signed_dist(P, yn), kp = W.kprev[i];
-      if ((kn > 0) != (kp > 0)) {
-        const double diff = kp - kn;
-        const double f = -kn / diff, fp = kp / diff;
-        for (int c = 0; c < 6; c++) W.land[6 * (size_t)i + c] = fp * yn[c] + f * y[c];
-        W.found[i] = 1;
-      }
+      const double kn = signed_dist(P, yn);
+      land_if_crossed(W.kprev[i], kn, y, yn, P.landing + 6 * (size_t)i);
       W.kprev[i] = kn;
+    }
+    if (P.landing2) {
+      const double kn = signed_dist2(P, yn);
+      land_if_crossed(W.kprev2[i], kn, y, yn, P.landing2 + 6 * (size_t)i);
+      W.kprev2[i] = kn;
     }
     for (int c = 0; c < 6; c++) y[c] = yn[c];
     const long long tn = W.tlen[i];

@@ -83,6 +83,10 @@ struct TraceParams {
   double* times;          // (n, traj_cap) or null
   long long traj_cap;
   long long* traj_len;
+  // a second landing plane (landing2 non-null): e.g. the entrance plane that reflected
+  // particles cross on their way back
+  double plane2_p[3], plane2_n[3], plane2_nn;
+  double* landing2;       // (n, 6) or null
 };
 
 // One lane per particle; `waves` 64-lane workgroups drain the particle queue.

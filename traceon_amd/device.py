@@ -151,16 +151,38 @@ def lu_factor(A: torch.Tensor, overwrite: bool = True, gemm_timing: "KernelTimin
     return lu
 
 
-def lu_solve(lu: LU, X: torch.Tensor) -> torch.Tensor:
-    """In-place X (nrhs, N) FP64 <- A^{-1} X."""
+def check_solve_status(err: int) -> None:
+    """Raise if a triangular solve reported a dependency wait past its spin limit (its solution
+    is then NaN): a failure of the device, not of the matrix."""
+    if err:
+        raise RuntimeError("lu_solve: a triangular-solve dependency wait passed its spin limit; the solution is invalid")
+
+
+def lu_solve(lu: LU, X: torch.Tensor, status: torch.Tensor | None = None) -> torch.Tensor:
+    """In-place X (nrhs, N) FP64 <- A^{-1} X.  Without `status` the call waits and raises on a
+    failed solve; with it (a device int32), the solve status is written there (nonzero on
+    failure) for the caller to read with other results."""
     assert X.dtype == F64 and X.is_contiguous()
     N = lu.A.shape[0]
     X2 = X.view(-1, N)
     f = _L.tb_lu_solve_f32 if lu.fp32 else _L.tb_lu_solve
+    st = status if status is not None else torch.zeros(1, dtype=torch.int32, device=X.device)
     for r0 in range(0, X2.shape[0], 64):
         blk = X2[r0:r0 + 64]
         check(f(ptr(lu.A), N, N, ptr(lu.ipiv), ptr(blk), blk.shape[0], ptr(lu.ws), stream_ptr()), "lu_solve")
+        # each call clears the flag: fold every block's status into st
+        tmp = st if r0 == 0 else torch.zeros(1, dtype=torch.int32, device=X.device)
+        check(_L.tb_lu_solve_status(ptr(lu.ws), N, int(lu.fp32), ptr(tmp), stream_ptr()), "lu_solve_status")
+        if r0:
+            st.bitwise_or_(tmp)
+    if status is None:
+        check_solve_status(int(st.item()))
     return X
+
+
+def absmax_into(x: torch.Tensor, out: torch.Tensor) -> None:
+    """out (one int64 slot) <- bit pattern of max |x| (library kernel; NaN gives a NaN pattern)."""
+    check(_L.tb_absmax(ptr(x), x.numel(), ptr(out), stream_ptr()), "absmax")
 
 
 def residual(A: torch.Tensor, B_: torch.Tensor, X: torch.Tensor):
@@ -203,24 +225,37 @@ def solve_system(M64: torch.Tensor | None, F: np.ndarray | torch.Tensor, M32: to
             lu = lu_factor(M64, overwrite=False, gemm_timing=gemm_timing)
     elif mode == "mixed":
         if lu is None:
-            lu = lu_factor(M32, overwrite=True, gemm_timing=gemm_timing)
+            try:
+                lu = lu_factor(M32, overwrite=True, gemm_timing=gemm_timing)
+            except np.linalg.LinAlgError:
+                # singular in FP32 only (a zero or NaN FP32 pivot): the FP64 LU decides
+                if not (fallback and M64 is not None):
+                    raise
+                X, st, lu = solve_system(M64, Bt, mode="fp64", max_iter=max_iter, refine=True, fallback=False)
+                return X, SolveStats("mixed->fp64", st.iterations, st.residual, lu.info), lu
     else:
         raise ValueError(mode)
     X = Bt.clone()
-    lu_solve(lu, X)
     it = 0
     converged = True
     rlast = None
     if refine or mode == "mixed":
+        lu_solve(lu, X)
         last = float("inf")
         eps = 2.0 ** -52
         converged = False
+        # per step, one readback: max |update|, max |x| (bit patterns) and the solve status
+        stat = torch.zeros(3, dtype=torch.int64, device=X.device)
         for it in range(1, max_iter + 1):
             R, rmax = residual(M64, Bt, X)
             rlast = rmax
-            lu_solve(lu, R)
-            dmax = float(R.abs().max().item())
-            xmax = float(X.abs().max().item())
+            stat.zero_()
+            lu_solve(lu, R, status=stat[2:].view(torch.int32)[:1])
+            absmax_into(R, stat[0:1])
+            absmax_into(X, stat[1:2])
+            h = stat.cpu()
+            check_solve_status(int(h[2]))
+            dmax, xmax = (float(v) for v in h[:2].view(torch.float64))
             if not math.isfinite(dmax) or (dmax >= 0.5 * last and dmax > 64.0 * eps * xmax):
                 break                      # diverging or stalled above the rounding level
             check(_L.tb_axpy(ptr(X), ptr(R), X.numel(), stream_ptr()), "axpy")
@@ -228,6 +263,8 @@ def solve_system(M64: torch.Tensor | None, F: np.ndarray | torch.Tensor, M32: to
                 converged = True
                 break
             last = dmax
+    else:
+        lu_solve(lu, X)
     if mode == "mixed" and not converged and fallback and M64 is not None and lu.info == 0:
         X, st, lu = solve_system(M64, Bt, mode="fp64", max_iter=max_iter, refine=True, fallback=False)
         return X, SolveStats("mixed->fp64", it + st.iterations, st.residual, lu.info), lu

@@ -75,7 +75,8 @@ class TraceParamsT(C.Structure):
                 ("max_attempts", C.c_longlong), ("plane_p", C.c_double * 3), ("plane_n", C.c_double * 3), ("plane_nn", C.c_double),
                 ("final_state", C.c_void_p), ("landing", C.c_void_p), ("status", C.c_void_p), ("n_accepted", C.c_void_p),
                 ("n_attempted", C.c_void_p), ("t_final", C.c_void_p), ("h_final", C.c_void_p), ("traj", C.c_void_p),
-                ("times", C.c_void_p), ("traj_cap", C.c_longlong), ("traj_len", C.c_void_p)]
+                ("times", C.c_void_p), ("traj_cap", C.c_longlong), ("traj_len", C.c_void_p),
+                ("plane2_p", C.c_double * 3), ("plane2_n", C.c_double * 3), ("plane2_nn", C.c_double), ("landing2", C.c_void_p)]
 
 
 COMP_KIND = {"bem": 0, "axial": 1, "uniform": 2, "learned": 1}
@@ -194,9 +195,10 @@ class DeviceFieldProgram:
 
 
 class BatchResult:
-    def __init__(self, final, landing, status, n_accepted, n_attempted, t_final):
+    def __init__(self, final, landing, status, n_accepted, n_attempted, t_final, landing2=None):
         self.final = final
         self.landing = landing
+        self.landing2 = landing2
         self.status = status
         self.n_accepted = n_accepted
         self.n_attempted = n_attempted
@@ -210,8 +212,9 @@ SPLIT_MAX_PARTICLES = 16384
 
 
 def run_trace(program: DeviceFieldProgram, state0, qm, bounds, atol=1e-8, plane=None, traj_cap=0, t0=None,
-              max_attempts=1 << 62, waves=0, split=None):
+              max_attempts=1 << 62, waves=0, split=None, plane2=None):
     """GPU tracer over many particles (all arrays numpy or device tensors).
+    plane, plane2: (p0, normal); the state at each one's last crossing (NaN if none).
     split: None picks tb_trace_split for fields with ring sources and at most
     SPLIT_MAX_PARTICLES particles, else the persistent tb_trace."""
     import torch
@@ -245,8 +248,16 @@ def run_trace(program: DeviceFieldProgram, state0, qm, bounds, atol=1e-8, plane=
     else:
         P.plane_n[2] = 1.0
         P.plane_nn = 1.0
+    if plane2 is not None:
+        p0, nrm = plane2
+        for i in range(3):
+            P.plane2_p[i] = p0[i]
+            P.plane2_n[i] = nrm[i]
+        P.plane2_nn = float(np.linalg.norm(nrm))
     final = torch.empty((n, 6), dtype=torch.float64, device=dev)
     landing = torch.empty((n, 6), dtype=torch.float64, device=dev) if plane is not None else None
+    landing2 = torch.empty((n, 6), dtype=torch.float64, device=dev) if plane2 is not None else None
+    P.landing2 = landing2.data_ptr() if landing2 is not None else None
     status = torch.empty(n, dtype=torch.int32, device=dev)
     nacc = torch.empty(n, dtype=torch.int64, device=dev)
     natt = torch.empty(n, dtype=torch.int64, device=dev)
@@ -275,7 +286,7 @@ def run_trace(program: DeviceFieldProgram, state0, qm, bounds, atol=1e-8, plane=
         backend.check(L.tb_trace_split(C.byref(P), nchunk, ws.data_ptr(), device.stream_ptr()), "trace_split")
     else:
         backend.check(L.tb_trace(C.byref(P), int(waves), device.stream_ptr()), "trace")
-    out = BatchResult(final, landing, status, nacc, natt, tfin)
+    out = BatchResult(final, landing, status, nacc, natt, tfin, landing2)
     out.traj, out.times, out.traj_len = traj, times, tlen
     out._keep = (S0, QM, T0, counter)
     return out
@@ -347,10 +358,11 @@ class Tracer:
                 for p, v, m, c in zip(positions, velocities, masses, charges)]
 
     def trace_batch(self, positions, velocities, mass=M_E, charge=-E_CHARGE, atol: float = 1e-8, plane=None,
-                    max_attempts=1 << 62, waves=0) -> BatchResult:
+                    max_attempts=1 << 62, waves=0, plane2=None) -> BatchResult:
         """Trace many particles in one persistent launch; no trajectories are stored.
         plane = (p0, normal) records the state where each trajectory last crosses it
-        (what xy_plane_intersection returns on the full trajectory), NaN if never."""
+        (what xy_plane_intersection returns on the full trajectory), NaN if never; plane2 a
+        second such plane (result.landing2), e.g. the entrance plane for reflected particles."""
         positions = np.atleast_2d(np.asarray(positions, dtype=np.float64))
         velocities = np.atleast_2d(np.asarray(velocities, dtype=np.float64))
         positions, velocities = np.broadcast_arrays(positions, velocities)
@@ -358,7 +370,8 @@ class Tracer:
         qm = np.asarray(charge, dtype=np.float64) / np.asarray(mass, dtype=np.float64)
         if qm.size > 1:
             qm = np.broadcast_to(qm, (len(state0),))
-        return run_trace(self.program(), state0, qm, self.bounds, atol, plane=plane, max_attempts=max_attempts, waves=waves)
+        return run_trace(self.program(), state0, qm, self.bounds, atol, plane=plane, max_attempts=max_attempts, waves=waves,
+                         plane2=plane2)
 
 
 def plane_intersection(positions, p0, normal):
