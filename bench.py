@@ -9,13 +9,17 @@ BEM fill+solve of the metric.  Reported beside it in the same JSON line:
 * the field (potential + E) on a 1000 x 1000 (r, z) grid (10^6 points) from the charges;
 * batched tracing: the 4-electrode lens (N = 8,000) with a -1.1 kV reflecting electrode,
   10^6 electrons (lognormal energy, median 1 keV, sigma 0.5; disc of 0.2 mm diameter;
-  5 mrad angles) through the FieldRadialAxial interpolation, landing RMS error against
-  the direct FP64 BEM tracer on a seeded sample;
+  5 mrad angles) through the FieldRadialAxial interpolation, transmitted and reflected
+  landings in one launch; landing errors against the FP64 oracle's direct BEM traces (golden
+  fixtures from the reference's C backend) and the GPU's direct FP64 BEM tracer, fate
+  mismatches counted as failures; the tracer's roofline;
+* the magnetic + electrostatic lens (N = 12,000, 10 keV) and its learned surrogate, also on an
+  off-axis beam against the oracle;
 * roofline of the dominant kernel (HIP events on the launch stream);
 * cpu_baseline: the reference's own C backend (oracle/_ref, compiled from its sources)
   on bounded samples of the same work, on this host's cores.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-trace] [--no-field]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-trace] [--no-field] [--trace-only]
 """
 from __future__ import annotations
 
@@ -128,18 +132,80 @@ def field_grid_leg(X, lines, types, values, n_side=1000):
     return t.ms, pairs, P, Fd, pts
 
 
-def tracing_leg(n_particles, n_sample, n_axial=7000, seed=2):
-    """Batched tracing through the 4-electrode lens (field from a GPU solve)."""
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+C_RK, C_AXEVAL = 530, 125   # RKF45 step overhead; one paraxial series evaluation (E or H)
+
+
+def golden(name):
+    """A golden fixture (FP64 oracle results written by tests/golden/make_config_fixtures.py), or None."""
+    path = os.path.join(GOLDEN, name)
+    return np.load(path, allow_pickle=False) if os.path.isfile(path) else None
+
+
+def traces(tracer, P, V, exit_plane, back_plane=None, timer=False):
+    """One batched launch: landing (exit plane, else back plane), plane index (0 exit, 1 back,
+    -1 none), accepted and attempted steps; optionally timed with HIP events."""
     import torch
-    from traceon_amd import configs, solver, tracing
+    torch.cuda.synchronize()
+    if timer:
+        with Timer() as t:
+            r = tracer.trace_batch(P, V, plane=exit_plane, plane2=back_plane)
+    else:
+        r, t = tracer.trace_batch(P, V, plane=exit_plane, plane2=back_plane), None
+    land = r.landing.cpu().numpy()
+    plane = np.where(np.isfinite(land[:, 0]), 0, -1)
+    if back_plane is not None:
+        l2 = r.landing2.cpu().numpy()
+        back = (plane < 0) & np.isfinite(l2[:, 0])
+        land[back] = l2[back]
+        plane[back] = 1
+    out = {"landing": land, "plane": plane, "n_accepted": r.n_accepted.cpu().numpy(), "n_attempted": r.n_attempted.cpu().numpy()}
+    if t is not None:
+        out["ms"] = t.ms
+    return out
+
+
+def landing_error(a, b):
+    """Landing statistics of trace set a against reference b (same particles): RMS and max over
+    the particles with the same fate; a particle whose fate differs counts as a failure of the
+    10 nm bar (fate_mismatches)."""
+    n = min(len(a["plane"]), len(b["plane"]))
+    same = a["plane"][:n] == b["plane"][:n]
+    ok = same & (b["plane"][:n] >= 0)
+    e = np.linalg.norm(a["landing"][:n][ok, :3] - b["landing"][:n][ok, :3], axis=1)
+    rms = float(np.sqrt(np.mean(e ** 2)) * 1e9) if ok.any() else None
+    return {"n": int(n), "fate_mismatches": int((~same).sum()), "landing_rms_nm": rms,
+            "landing_max_nm": float(e.max() * 1e9) if ok.any() else None,
+            "within_10nm": bool(ok.any() and (~same).sum() == 0 and rms is not None and rms <= 10.0)}
+
+
+def oracle_traces(f, prefix):
+    return {"landing": f[prefix + "landing"], "plane": f[prefix + "plane"]}
+
+
+def roofline_trace(steps_attempted, ms, series_per_eval):
+    """Tracer roofline: algorithmic FLOP per attempted RKF45 step = C_rk + 6 evaluations of
+    series_per_eval paraxial series (C_axeval each), against the FP64 vector peak."""
+    flop = float(steps_attempted) * (C_RK + 6 * series_per_eval * C_AXEVAL)
+    achieved = flop / (ms * 1e-3) / 1e12
+    return {"kernel": "k_trace_axial (persistent RKF45, lane per particle)", "bound": "fp64-valu",
+            "flop_per_attempted_step": C_RK + 6 * series_per_eval * C_AXEVAL, "attempted_steps": int(steps_attempted),
+            "achieved": achieved, "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s", "frac": achieved / (PEAK["fp64"] / 1e12)}
+
+
+def tracing_leg(n_particles, n_sample, n_axial=7000):
+    """Batched tracing (C2): the reflecting 4-electrode lens (N = 8000) and 10^6 electrons
+    (lognormal 1 keV, 0.2 mm disc, 5 mrad) through FieldRadialAxial; transmitted electrons land
+    on z = 9.5 mm, reflected ones back through z = -4.5 mm, both recorded in the same launch."""
+    import torch
+    from traceon_amd import configs, solver
     from traceon_amd.excitation import Excitation
     from traceon_amd.field import FieldRadialAxial
     mm = 1e-3
-    mesh, volts = configs.einzel_lens(8000, voltages=(0.0, 1800.0, 0.0), fourth=(4.0 * mm, 1.0 * mm, -1100.0))
+    mesh, volts = configs.reflecting_lens(8000)
     exc = Excitation(mesh)
     exc.add_voltage(**volts)
     field = solver.solve_direct(exc)
-    bounds = np.array([[-1.0 * mm, 1.0 * mm], [-1.0 * mm, 1.0 * mm], [-5.0 * mm, 10.0 * mm]])
     t0 = time.perf_counter()
     # The interpolation covers the tracing bounds plus the longest step (hmax |v| = 1 cm,
     # tracing.c:68-70) on both sides: RK stage points of a step can lie that far ahead, and
@@ -147,60 +213,54 @@ def tracing_leg(n_particles, n_sample, n_axial=7000, seed=2):
     axial = FieldRadialAxial(field, -15.0 * mm, 20.0 * mm, N=n_axial)
     torch.cuda.synchronize()
     t_axial_build = time.perf_counter() - t0
-    rng = np.random.default_rng(seed)
-    n = n_particles
-    energy = 1000.0 * np.exp(0.5 * rng.standard_normal(n))
-    rad = 0.1 * mm * np.sqrt(rng.random(n))       # uniform in a disc of 0.2 mm diameter
-    phi = 2 * np.pi * rng.random(n)
-    tx, ty = rng.normal(0, 5e-3, n), rng.normal(0, 5e-3, n)
-    d = np.column_stack([np.tan(tx), np.tan(ty), np.ones(n)])
-    d /= np.linalg.norm(d, axis=1, keepdims=True)
-    speed = np.sqrt(2 * energy * tracing.E_CHARGE / tracing.M_E)
-    P = np.column_stack([rad * np.cos(phi), rad * np.sin(phi), np.full(n, -5.0 * mm)])
-    V = d * speed[:, None]
-    plane = (np.array([0.0, 0.0, 9.5 * mm]), np.array([0.0, 0.0, 1.0]))
+    P, V, bounds, exit_plane, back_plane = configs.reflecting_beam(n_particles)
     tr_ax = axial.get_tracer(bounds)
-    warm = tr_ax.trace_batch(P[:4096], V[:4096], plane=plane)
-    torch.cuda.synchronize()
-    with Timer() as t:
-        res = tr_ax.trace_batch(P, V, plane=plane)
-    steps = int(res.n_attempted.sum().item())
-    land = res.landing.cpu().numpy()
-    reflected = float(np.mean(res.final[:, 2].cpu().numpy() < -4.999 * mm))
-    nacc = res.n_accepted.cpu().numpy()
-    # error reference: direct FP64 BEM field, same particles (sample)
+    traces(tr_ax, P[:4096], V[:4096], exit_plane, back_plane)   # warm-up
+    res = traces(tr_ax, P, V, exit_plane, back_plane, timer=True)
+    nacc = res["n_accepted"]
+    out = {"n": n_particles, "ms": res["ms"], "electrons_per_s": n_particles / (res["ms"] * 1e-3),
+           "steps_attempted": int(res["n_attempted"].sum()), "steps_accepted": int(nacc.sum()),
+           "reflected_fraction": float(np.mean(res["plane"] == 1)), "lost_fraction": float(np.mean(res["plane"] < 0)),
+           "len_ratio_max_min": float(nacc.max() / max(nacc.min(), 1)),
+           "len_ratio_p999_p001": float(np.percentile(nacc, 99.9) / max(np.percentile(nacc, 0.1), 1)),
+           "n_axial": len(axial.z), "axial_build_s": t_axial_build, "N_elements": len(field.electrostatic_point_charges)}
+    out["roofline"] = roofline_trace(out["steps_attempted"], res["ms"], 1)
+    # accuracy: against the FP64 oracle's direct BEM traces (reference C backend, converged
+    # diagonal) of the beam's first particles, and against the GPU's direct FP64 BEM tracer
+    # (itself within 1e-9 m of the oracle) on a larger sample
+    f = golden("config_c2.npz")
+    if f is not None:
+        out["vs_oracle_bem"] = landing_error(res, oracle_traces(f, "bem_"))
+        out["vs_oracle_axial"] = landing_error(res, oracle_traces(f, "axial_"))
     tr_bem = field.get_tracer(bounds)
-    with Timer() as tb:
-        ref = tr_bem.trace_batch(P[:n_sample], V[:n_sample], plane=plane)
-    land_ref = ref.landing.cpu().numpy()
-    ok = np.isfinite(land_ref[:, 0]) & np.isfinite(land[:n_sample, 0])
-    err = np.linalg.norm(land[:n_sample][ok, :2] - land_ref[ok, :2], axis=1)
-    steps_bem = int(ref.n_attempted.sum().item())
-    return {
-        "n": n, "ms": t.ms, "steps": steps, "reflected_fraction": reflected,
-        "len_ratio": float(np.percentile(nacc, 99.9) / max(np.percentile(nacc, 0.1), 1)),
-        "landing_rms_nm": float(np.sqrt(np.mean(err ** 2)) * 1e9) if ok.any() else None,
-        "landing_max_nm": float(np.max(err) * 1e9) if ok.any() else None,
-        "n_sample": int(n_sample), "bem_ms": tb.ms, "bem_steps": steps_bem, "n_axial": len(axial.z),
-        "axial_build_s": t_axial_build, "field": field, "axial": axial, "bounds": bounds, "P": P, "V": V, "plane": plane,
-        "N_elements": len(field.electrostatic_point_charges),
-    }
+    ref = traces(tr_bem, P[:n_sample], V[:n_sample], exit_plane, back_plane, timer=True)
+    out["vs_gpu_bem"] = landing_error(res, ref)
+    out["electrons_per_s_direct_bem_fp64"] = n_sample / (ref["ms"] * 1e-3)
+    out["n_sample"] = int(n_sample)
+    ctx = {"field": field, "axial": axial, "P": P, "V": V, "bounds": bounds, "plane": exit_plane}
+    return out, ctx
 
 
-def c3_leg(n_particles, n_sample, n_axial=7000, seed=3):
-    """Magnetic + electrostatic lens (C3): coil 500 A-turn, two mu_r = 1000 pole pieces (bore 2 mm,
-    gap 3 mm) and a three-tube einzel lens, N = 12,000; electrostatic and magnetostatic solves,
-    10 keV electrons traced through the combined field with the Lorentz force."""
-    import torch
-    from traceon_amd import configs, solver, tracing
+def magnetic_problem():
+    from traceon_amd import configs
     from traceon_amd.excitation import Excitation
-    from traceon_amd.field import FieldRadialAxial
-    mm = 1e-3
     mesh, volts, perms, currents = configs.magnetic_lens(12000)
     exc = Excitation(mesh)
     exc.add_voltage(**volts)
     exc.add_magnetizable(**perms)
     exc.add_current(**currents)
+    return exc
+
+
+def c3_leg(n_particles, n_sample, n_axial=7000):
+    """Magnetic + electrostatic lens (C3): coil 500 A-turn, two mu_r = 1000 pole pieces (bore 2 mm,
+    gap 3 mm) and a three-tube einzel lens, N = 12,000; electrostatic and magnetostatic solves,
+    10 keV electrons traced through both axial series with the Lorentz force."""
+    import torch
+    from traceon_amd import configs, solver
+    from traceon_amd.field import FieldRadialAxial
+    mm = 1e-3
+    exc = magnetic_problem()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     field = solver.solve_direct(exc)
@@ -210,37 +270,24 @@ def c3_leg(n_particles, n_sample, n_axial=7000, seed=3):
     axial = FieldRadialAxial(field, -24.0 * mm, 22.0 * mm, N=int(n_axial * 46 / 35))
     torch.cuda.synchronize()
     t_axial = time.perf_counter() - t0
-    rng = np.random.default_rng(seed)
-    n = n_particles
-    rad = 0.1 * mm * np.sqrt(rng.random(n))
-    phi = 2 * np.pi * rng.random(n)
-    tx, ty = rng.normal(0, 5e-3, n), rng.normal(0, 5e-3, n)
-    d = np.column_stack([np.tan(tx), np.tan(ty), np.ones(n)])
-    d /= np.linalg.norm(d, axis=1, keepdims=True)
-    speed = np.sqrt(2 * 10000.0 * tracing.E_CHARGE / tracing.M_E)
-    P = np.column_stack([rad * np.cos(phi), rad * np.sin(phi), np.full(n, -14.0 * mm)])
-    V = d * speed
-    bounds = np.array([[-1.0 * mm, 1.0 * mm], [-1.0 * mm, 1.0 * mm], [-14.0 * mm, 12.0 * mm]])
-    plane = (np.array([0.0, 0.0, 11.0 * mm]), np.array([0.0, 0.0, 1.0]))
+    P, V, bounds, plane = configs.magnetic_beam(n_particles)
     tr_ax = axial.get_tracer(bounds)
-    tr_ax.trace_batch(P[:4096], V[:4096], plane=plane)
-    torch.cuda.synchronize()
-    with Timer() as t:
-        res = tr_ax.trace_batch(P, V, plane=plane)
-    land = res.landing.cpu().numpy()
-    with Timer() as tb:
-        ref = field.get_tracer(bounds).trace_batch(P[:n_sample], V[:n_sample], plane=plane)
-    land_ref = ref.landing.cpu().numpy()
-    ok = np.isfinite(land_ref[:, 0]) & np.isfinite(land[:n_sample, 0])
-    err = np.linalg.norm(land[:n_sample][ok, :2] - land_ref[ok, :2], axis=1)
+    traces(tr_ax, P[:4096], V[:4096], plane)
+    res = traces(tr_ax, P, V, plane, timer=True)
     out = {"N_electrostatic": len(field.electrostatic_point_charges), "N_magnetostatic": len(field.magnetostatic_point_charges),
-           "N_coil_rings": len(field.current_point_charges), "solve_s": t_solve, "axial_build_s": t_axial, "n": n,
-           "ms": t.ms, "electrons_per_s": n / (t.ms * 1e-3), "steps": int(res.n_attempted.sum().item()),
-           "landed_fraction": float(np.mean(np.isfinite(land[:, 0]))),
-           "landing_rms_nm": float(np.sqrt(np.mean(err ** 2)) * 1e9) if ok.any() else None,
-           "landing_max_nm": float(np.max(err) * 1e9) if ok.any() else None, "n_sample": int(n_sample),
-           "electrons_per_s_direct_bem_fp64": n_sample / (tb.ms * 1e-3)}
-    ctx = {"field": field, "axial": axial, "P": P, "V": V, "bounds": bounds, "plane": plane, "land_ref": land_ref}
+           "N_coil_rings": len(field.current_point_charges), "solve_s": t_solve, "axial_build_s": t_axial, "n": n_particles,
+           "ms": res["ms"], "electrons_per_s": n_particles / (res["ms"] * 1e-3), "steps_attempted": int(res["n_attempted"].sum()),
+           "landed_fraction": float(np.mean(res["plane"] == 0))}
+    out["roofline"] = roofline_trace(out["steps_attempted"], res["ms"], 2)
+    f = golden("config_c3.npz")
+    if f is not None:
+        out["vs_oracle_bem"] = landing_error(res, oracle_traces(f, "bem_"))
+        out["vs_oracle_axial"] = landing_error(res, oracle_traces(f, "axial_"))
+    ref = traces(field.get_tracer(bounds), P[:n_sample], V[:n_sample], plane, timer=True)
+    out["vs_gpu_bem"] = landing_error(res, ref)
+    out["electrons_per_s_direct_bem_fp64"] = n_sample / (ref["ms"] * 1e-3)
+    out["n_sample"] = int(n_sample)
+    ctx = {"field": field, "axial": axial, "P": P, "V": V, "bounds": bounds, "plane": plane, "ref": ref, "res": res}
     return out, ctx
 
 
@@ -249,12 +296,15 @@ def c4_leg(ctx, n_samples=1 << 22, steps=8000):
     master weights) learns BEM - axial from n_samples BEM-labelled (r, z) points over the lens
     region; the tracer adds it to the axial field inside r <= 0.6 mm and uses the direct BEM
     sum outside.  Reported: build times, field error per radial band with and without the
-    correction, and 10 keV tracing of the same electrons as C3 against direct FP64 BEM."""
+    correction, tracing of the C3 beam (10^6 electrons) against direct FP64 BEM, and the
+    off-axis beam (start radius 0.3-0.5 mm) against the FP64 oracle's direct BEM traces, beside
+    the axial series alone."""
     import torch
+    from traceon_amd import configs
     from traceon_amd.surrogate import FieldRadialAxialLearned, FieldSurrogate
     mm = 1e-3
     field, axial, P, V = ctx["field"], ctx["axial"], ctx["P"], ctx["V"]
-    bounds, plane, land_ref = ctx["bounds"], ctx["plane"], ctx["land_ref"]
+    bounds, plane = ctx["bounds"], ctx["plane"]
     sur = FieldSurrogate(field, axial, r_max=0.6 * mm, z_min=bounds[2, 0], z_max=bounds[2, 1])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -265,28 +315,32 @@ def c4_leg(ctx, n_samples=1 << 22, steps=8000):
     val = sur.validate()
     learned = FieldRadialAxialLearned(sur)
     tr = learned.get_tracer(bounds)
-    tr.trace_batch(P[:4096], V[:4096], plane=plane)
-    torch.cuda.synchronize()
-    with Timer() as t:
-        res = tr.trace_batch(P, V, plane=plane)
-    land = res.landing.cpu().numpy()
-    ns = len(land_ref)
-    ok = np.isfinite(land_ref[:, 0]) & np.isfinite(land[:ns, 0])
-    err = np.linalg.norm(land[:ns][ok, :2] - land_ref[ok, :2], axis=1)
+    traces(tr, P[:4096], V[:4096], plane)
+    res = traces(tr, P, V, plane, timer=True)
     bands = {k: {"axial_E_rms": v["axial"]["E_rms"], "learned_E_rms": v["learned"]["E_rms"], "axial_H_rms": v["axial"]["H_rms"],
                  "learned_H_rms": v["learned"]["H_rms"]} for k, v in val.items()}
-    return {"model": "MLP 2-128-128-128-128-5 SiLU, BF16 MFMA hidden layers, FP32 master weights", "samples": n_samples,
-            "train_steps": steps, "batch": 16384, "label_s": t_gen, "train_s": sur.train_seconds, "loss_history": hist,
-            "field_error_by_radius": bands, "n": len(P), "ms": t.ms, "electrons_per_s": len(P) / (t.ms * 1e-3),
-            "landing_rms_nm": float(np.sqrt(np.mean(err ** 2)) * 1e9) if ok.any() else None,
-            "landing_max_nm": float(np.max(err) * 1e9) if ok.any() else None}
+    out = {"model": "MLP 2-128-128-128-128-5 SiLU, BF16 MFMA hidden layers, FP32 master weights", "samples": n_samples,
+           "train_steps": steps, "batch": 16384, "label_s": t_gen, "train_s": sur.train_seconds, "loss_history": hist,
+           "field_error_by_radius": bands, "n": len(P), "ms": res["ms"], "electrons_per_s": len(P) / (res["ms"] * 1e-3),
+           "vs_gpu_bem": landing_error(res, ctx["ref"])}
+    f = golden("config_c4.npz")
+    if f is not None:
+        Po, Vo, bo, po = configs.magnetic_beam(64, seed=4, r_range=(0.3 * mm, 0.5 * mm))
+        off = {"n": 64, "start_radius_mm": [0.3, 0.5]}
+        oracle = oracle_traces(f, "bem_")
+        for name, fld in (("learned", learned), ("axial", axial), ("direct_bem_gpu", field)):
+            off[name] = landing_error(traces(fld.get_tracer(bo), Po, Vo, po), oracle)
+        out["off_axis_beam_vs_oracle_bem"] = off
+    return out
 
 
-def cpu_baseline(lines, types, values, trace_info, threads):
-    """The reference's C backend (oracle/_ref) on bounded samples, on `threads` host cores.
-    Fill: a contiguous block of rows split over threads (util.split_collect); diagonal:
-    scipy QUADPACK on a sample of elements; solve: numpy.linalg.solve at a smaller N
-    scaled by N^3.  Result: estimated seconds for the full N = 20,000 fill+solve."""
+def cpu_baseline(lines, types, values, M64, F, trace_info, threads):
+    """The reference's C backend (oracle/_ref) on bounded samples, on `threads` host cores:
+    fill of a contiguous block of rows split over threads (util.split_collect, extrapolated to
+    all rows); scipy QUADPACK diagonal on a sample of elements (extrapolated); numpy.linalg.solve
+    (LAPACK dgesv, the reference's call) on the full N x N matrix of the workload, copied from the
+    device (equal to the reference's fill to 1e-13), timed whole.  Result: seconds for the N =
+    20,000 fill+solve."""
     from oracle.oracle import reference, restatement
     lib = reference()
     kind = "reference"
@@ -314,15 +368,15 @@ def cpu_baseline(lines, types, values, trace_info, threads):
     for i in np.linspace(0, N - 1, nd).astype(int):
         lib.self_potential_quad(lines[i])
     t_diag = (time.perf_counter() - t0) * N / nd
-    Ns = 6000
-    A = np.random.default_rng(0).standard_normal((Ns, Ns)) + Ns * np.eye(Ns)
-    b = np.ones(Ns)
+    A = M64.cpu().numpy()
     t0 = time.perf_counter()
-    np.linalg.solve(A, b)
-    t_solve = (time.perf_counter() - t0) * (N / Ns) ** 3
+    np.linalg.solve(A, F)
+    t_solve = time.perf_counter() - t0
+    del A
     out = {"fill_solve_s": t_fill + t_diag + t_solve, "fill_s": t_fill, "diag_s": t_diag, "solve_s": t_solve, "kind": kind,
-           "sample": f"fill {rows} of {N} rows over {threads} threads, QUADPACK diagonal {nd} of {N} elements (1 thread, as "
-                     f"solver.py:239-247), numpy.linalg.solve at N={Ns} scaled by (N/{Ns})^3"}
+           "sample": f"fill {rows} of {N} rows over {threads} threads (extrapolated), QUADPACK diagonal {nd} of {N} elements "
+                     f"(1 thread, as solver.py:239-247; extrapolated), numpy.linalg.solve of the full {N} x {N} matrix "
+                     f"(OpenBLAS, timed whole)"}
     if trace_info is not None:
         # tracing: the reference C tracer through its axial field function, one particle per call
         ax = trace_info["axial"]
@@ -358,6 +412,8 @@ def main():
     ap.add_argument("--no-field", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--no-c4", action="store_true")
+    ap.add_argument("--trace-only", action="store_true",
+                    help="run only the C2 and C3 tracing legs (for kernel profiles); prints their JSON")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device three times (a known-size kernel for HBM counter scaling)")
     args = ap.parse_args()
@@ -375,6 +431,11 @@ def main():
         torch.cuda.synchronize()
         log(f"bench: copy calibration, 3 x {args.copy_calibration} MiB device-to-device")
         del src, dst
+    if args.trace_only:
+        c2, _ = tracing_leg(args.particles, args.sample, args.n_axial)
+        c3, _ = c3_leg(args.particles, args.sample, args.n_axial)
+        print(json.dumps({"tracing": c2, "c3_magnetic_lens": c3}))
+        return
     lines, types, values = mirror_problem(args.N)
     N = len(lines)
     log(f"bench: mirror N={N}, warmup {args.warmup}, steps {args.steps}")
@@ -445,29 +506,32 @@ def main():
         log(f"bench: field grid {pts.shape[0]} points in {ms:.1f} ms")
     trace_info = None
     if not args.no_trace:
-        trace_info = tracing_leg(args.particles, args.sample, args.n_axial)
-        eps = trace_info["n"] / (trace_info["ms"] * 1e-3)
-        result["tracing"] = {k: trace_info[k] for k in ("n", "ms", "steps", "reflected_fraction", "len_ratio", "landing_rms_nm",
-                                                        "landing_max_nm", "n_sample", "n_axial", "N_elements")}
-        result["tracing"]["electrons_per_s"] = eps
-        result["tracing"]["electrons_per_s_direct_bem_fp64"] = trace_info["n_sample"] / (trace_info["bem_ms"] * 1e-3)
-        result["electrons_per_s"] = eps
-        log(f"bench: traced {trace_info['n']} electrons in {trace_info['ms']:.1f} ms ({eps:.3e}/s), "
-            f"landing RMS {trace_info['landing_rms_nm']} nm vs direct BEM")
+        c2, trace_info = tracing_leg(args.particles, args.sample, args.n_axial)
+        result["tracing"] = c2
+        result["electrons_per_s"] = c2["electrons_per_s"]
+        log(f"bench: C2 traced {c2['n']} electrons in {c2['ms']:.1f} ms ({c2['electrons_per_s']:.3e}/s), reflected "
+            f"{c2['reflected_fraction']:.3f}, lengths max/min {c2['len_ratio_max_min']:.1f}; vs oracle BEM "
+            f"{c2.get('vs_oracle_bem')}; vs GPU BEM {c2['vs_gpu_bem']}; roofline {c2['roofline']['frac']:.3f}")
     if not args.no_c3:
         c3, ctx = c3_leg(args.particles, args.sample, args.n_axial)
         result["c3_magnetic_lens"] = c3
         log(f"bench: C3 solve {c3['solve_s']:.3f} s, traced {c3['n']} electrons in {c3['ms']:.1f} ms "
-            f"({c3['electrons_per_s']:.3e}/s), landing RMS {c3['landing_rms_nm']} nm vs direct BEM")
+            f"({c3['electrons_per_s']:.3e}/s); vs oracle BEM {c3.get('vs_oracle_bem')}; vs GPU BEM {c3['vs_gpu_bem']}")
         if not args.no_c4:
             c4 = c4_leg(ctx)
             c4["speedup_vs_direct_bem_fp64"] = c4["electrons_per_s"] / c3["electrons_per_s_direct_bem_fp64"]
             result["c4_surrogate"] = c4
             log(f"bench: C4 surrogate labels {c4['label_s']:.2f} s, training {c4['train_s']:.2f} s, traced {c4['n']} electrons "
-                f"in {c4['ms']:.1f} ms ({c4['electrons_per_s']:.3e}/s), landing RMS {c4['landing_rms_nm']} nm vs direct BEM")
+                f"in {c4['ms']:.1f} ms ({c4['electrons_per_s']:.3e}/s); vs GPU BEM {c4['vs_gpu_bem']}; off-axis beam "
+                f"{c4.get('off_axis_beam_vs_oracle_bem')}")
     if not args.no_cpu:
+        from traceon_amd import device
         threads = min(16, len(os.sched_getaffinity(0)))
-        cb = cpu_baseline(lines, types, values, trace_info, threads)
+        E = device.Elements(lines, types, values)
+        M64, _ = device.assemble(E)
+        F = np.where(np.isin(types, [1, 2, 5]), values, 0.0)
+        cb = cpu_baseline(lines, types, values, M64, F, trace_info, threads)
+        del M64, E
         result["cpu_baseline"] = {"value": cb["fill_solve_s"], "unit": "s", "cores": threads, "kind": cb["kind"],
                                   "sample": cb["sample"], "fill_s": cb["fill_s"], "diag_s": cb["diag_s"], "solve_s": cb["solve_s"]}
         if "electrons_per_s_axial" in cb:

@@ -1,8 +1,9 @@
 """Writes the golden fixtures of the five benchmark workloads (config_c0.npz ... config_c4.npz)
 from the reference's own C backend (oracle/_ref, compiled from its sources by oracle/Makefile;
 the oracle's restatement where that build is absent), with the reference's Python steps
-restated in oracle/oracle.py: QUADPACK diagonal, numpy.linalg.solve, scipy axial
-coefficients, the trace block loop and plane_intersection.  The GPU tests in
+restated in oracle/oracle.py: QUADPACK diagonal (converged, see TIGHT, and at the reference's
+tolerance), numpy.linalg.solve, scipy axial coefficients, the trace block loop and
+plane_intersection.  The GPU tests in
 tests/test_gpu_configs.py compare the HIP path against these files.
 
     python tests/golden/make_config_fixtures.py c0 c1 c2 c3 c4     (all: no argument)
@@ -44,15 +45,36 @@ def log(*a):
     print(*a, flush=True)
 
 
-def solve(L, lines, types, values, F=None):
+# QUADPACK at the reference's tolerance (epsabs = epsrel = 1e-9, backend/__init__.py:692, 708)
+# accepts, on some elements of fine meshes, results 1e-5 from the integral: the integrals are
+# ~6e-6 there and the absolute tolerance admits that.  The oracle's charges, fields and traces
+# therefore come from the same integrands integrated to convergence; the reference-tolerance
+# diagonal and its solution are kept beside them to check that the GPU solves the reference's
+# own assembled system.
+TIGHT = dict(epsabs=1e-30, epsrel=1e-13, limit=1000)
+
+
+def solve(L, lines, types, values, F=None, reference_tolerance=False):
+    """Fill, QUADPACK diagonal (converged; with reference_tolerance also the reference's),
+    numpy.linalg.solve.  Returns M (converged diagonal), d, F, q and, if asked, the
+    reference-tolerance diagonal and charges."""
     t = time.time()
     M = L.fill_matrix(lines, types, values)
-    d = L.diagonal(lines, types, values)
+    d = L.diagonal(lines, types, values, **TIGHT)
     M[np.diag_indices(len(lines))] = d
     F = np.where(np.isin(types, [1, 2, 5]), values, 0.0) if F is None else F
     q = np.linalg.solve(M, F)
+    extra = {}
+    if reference_tolerance:
+        d_ref = L.diagonal(lines, types, values)
+        Mr = M.copy()
+        Mr[np.diag_indices(len(lines))] = d_ref
+        q_ref = np.linalg.solve(Mr, F)
+        extra = {"diag_ref": d_ref, "q_ref": q_ref, "q_ref_refined": O.refine_long_double(Mr, F, q_ref, iters=2)}
+        del Mr
+        log(f"  reference-tolerance diagonal: worst {np.max(np.abs(d_ref - d) / np.abs(d)):.2e} from the converged one")
     log(f"  solve N={len(lines)}: {time.time() - t:.1f} s")
-    return M, d, F, q
+    return M, d, F, q, extra
 
 
 def mv_vectors(seed, N):
@@ -107,7 +129,7 @@ def c0():
     L, kind = lib()
     mesh, volts = configs.einzel_lens(3000)
     lines, types, values = configs.lines_types_values(mesh, volts)
-    M, d, F, q = solve(L, lines, types, values)
+    M, d, F, q, ref = solve(L, lines, types, values, reference_tolerance=True)
     qr = O.refine_long_double(M, F, q)
     jac, pos = L.fill_jacobian_buffer_radial(lines)
     mc = matrix_checks(M, d, 100, [0, 777, 1500, 2999])
@@ -116,10 +138,11 @@ def c0():
     P, V, bounds, plane = configs.einzel_beam(100)
     tr = trace_set(L, P, V, lambda p, v: L.trace_bem(p, v, QM, (q, jac, pos), bounds=bounds), plane)
     save("config_c0.npz", {"kind": kind, "lines_sha256": line_hash(lines), "N": len(lines),
-                           "contents": "off-diagonal M v for two seeded vectors, four rows, QUADPACK diagonal, LAPACK charges, "
-                                       "charges refined in long double, potential and field at 200 points, 100 direct-BEM "
-                                       "traces (landing on z = 10 mm, accepted steps)"},
-         diag=d, q_lapack=q, q_refined=qr, pts=pts, pot=pot, field=fld, P=P, V=V, bounds=bounds, **mc,
+                           "contents": "off-diagonal M v for two seeded vectors, four rows, QUADPACK diagonal converged "
+                                       "(diag) and at the reference's tolerance (diag_ref), LAPACK charges of both (q_lapack, "
+                                       "q_ref) and both refined in long double, potential and field at 200 points, 100 "
+                                       "direct-BEM traces (landing on z = 10 mm, accepted steps)"},
+         diag=d, q_lapack=q, q_refined=qr, **ref, pts=pts, pot=pot, field=fld, P=P, V=V, bounds=bounds, **mc,
          **{f"trace_{k}": v for k, v in tr.items()})
 
 
@@ -129,7 +152,7 @@ def c1():
     L, kind = lib()
     mesh, volts = configs.mirror(20000, seed=1)
     lines, types, values = configs.lines_types_values(mesh, volts)
-    M, d, F, q = solve(L, lines, types, values)
+    M, d, F, q, ref = solve(L, lines, types, values, reference_tolerance=True)
     t = time.time()
     qr = O.refine_long_double(M, F, q, iters=2)
     log(f"  refinement {time.time() - t:.1f} s")
@@ -146,9 +169,9 @@ def c1():
     pot, fld = field_points(L, pts, q, jac, pos)
     save("config_c1.npz", {"kind": kind, "lines_sha256": line_hash(lines), "N": len(lines),
                            "contents": "off-diagonal M v for two seeded vectors, rows 0, 9999, 19999 (every 10th column), "
-                                       "QUADPACK diagonal, LAPACK charges, charges refined in long double, potential and "
-                                       "field at 256 grid points"},
-         diag=d, q_lapack=q, q_refined=qr, pts=pts, pot=pot, field=fld, **mc)
+                                       "QUADPACK diagonal converged (diag) and at the reference's tolerance (diag_ref), LAPACK "
+                                       "charges of both and both refined in long double, potential and field at 256 grid points"},
+         diag=d, q_lapack=q, q_refined=qr, **ref, pts=pts, pot=pot, field=fld, **mc)
 
 
 def c2():
@@ -157,7 +180,7 @@ def c2():
     L, kind = lib()
     mesh, volts = configs.reflecting_lens(8000)
     lines, types, values = configs.lines_types_values(mesh, volts)
-    M, d, F, q = solve(L, lines, types, values)
+    M, d, F, q, _ = solve(L, lines, types, values)
     del M
     jac, pos = L.fill_jacobian_buffer_radial(lines)
     z = np.linspace(-15.0 * MM, 20.0 * MM, 7000)
@@ -206,9 +229,9 @@ def c3_c4(which):
     beam (start radius 0.3-0.5 mm) that the learned surrogate serves."""
     L, kind = lib()
     (le, te, ve), (lm, tm, vm), (I, cj, cp) = magnetic_problem(L)
-    _, _, _, qe = solve(L, le, te, ve)
+    _, _, _, qe, _ = solve(L, le, te, ve)
     F = O.magnetizable_rhs(L, lm, vm, I, cj, cp)
-    _, _, _, qm = solve(L, lm, tm, vm, F=F)
+    _, _, _, qm, _ = solve(L, lm, tm, vm, F=F)
     je, pe = L.fill_jacobian_buffer_radial(le)
     jm, pm = L.fill_jacobian_buffer_radial(lm)
     z = np.linspace(-24.0 * MM, 22.0 * MM, int(7000 * 46 / 35))

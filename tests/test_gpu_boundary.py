@@ -4,7 +4,8 @@ LowLevelCallable (backend/__init__.py:688-708), the flat-triangle helpers, the c
 on-axis potential and axial derivatives (radial.c:38-53, 85-103), and the axial-series
 potential (radial.c:312-344) with the magnetostatic series behind the C3 tracer.
 
-Bars: integrand values 1e-13 relative to the oracle's restatement of radial.c:182-222;
+Bars: integrand values 2e-12 relative to the oracle's restatement of radial.c:182-222 (for the
+flux integrand max(2e-12, 1e-13 / |alpha|): near its singular point two terms ~ 1/alpha cancel);
 QUADPACK over them 1e-12 (the same adaptive rule on integrands equal to rounding); the
 graded Gauss diagonal within QUADPACK's requested 1e-9; axial derivatives 1e-11 per order;
 series potentials and fields 1e-10 (the bar of the path)."""
@@ -72,7 +73,11 @@ def test_diagonal_integrands_through_quadpack(gpu, oracle):
             oud = line.ctypes.data_as(C.c_void_p)
         for a in (-0.93, -0.4, -1e-3, 1e-7, 0.25, 0.999):
             g, o = fn(a, ud), ofn(a, oud)
-            assert abs(g - o) <= 1e-13 * abs(o), (i, a, g, o)
+            # next to the singular point the flux integrand (n . E of the element's ring seen
+            # from a point alpha element lengths away) is the difference of two terms ~ 1/alpha
+            # that cancel to O(1): both implementations lose digits as 1/alpha there
+            bar = max(2e-12, 1e-13 / abs(a)) if flux else 2e-12
+            assert abs(g - o) <= bar * abs(o), (i, a, g, o)
         B.backend_lib.tb_clear_error()
         q = quad(LowLevelCallable(fn, user_data=ud), -1, 1, points=(0,), epsabs=1e-9, epsrel=1e-9, limit=250)[0]
         assert B.last_error() == ""

@@ -6,10 +6,13 @@ and plane_intersection restated in oracle/oracle.py).
 Bars (the path's: charges, potentials and fields to 1e-10 relative, landings to 1e-9 m):
 - off-diagonal matrix entries through M v for two seeded vectors and whole rows: 1e-12 of the
   row's absolute sum (the reference is compiled with -ffast-math);
-- diagonal: within QUADPACK's requested 1e-9 relative (backend/__init__.py:692);
-- charges: with the fixture's QUADPACK diagonal put into the GPU matrix, the GPU solve must
-  reach the reference system's solution (refined in long double) to 1e-11; with the GPU's own
-  diagonal the charges are reported against LAPACK's and bounded by the diagonal's tolerance;
+- diagonal: 1e-11 of QUADPACK run to convergence on the reference's integrands, and within
+  the absolute 1e-9 that the reference's own QUADPACK call requests (backend/__init__.py:692):
+  on fine meshes that call stops up to 1e-5 (relative) from the integral on a few elements, so
+  the oracle's charges, fields and traces come from the converged diagonal;
+- charges: 1e-10 of the oracle's solution refined in long double; with the reference-tolerance
+  diagonal put into the GPU matrix, 1e-10 of that system's refined solution (the GPU solves
+  the reference's own assembled system);
 - potentials and fields at points: 1e-10 of the largest value;
 - traces: the same fate (exit plane or reflection), and landings within 1e-9 m with the same
   number of accepted steps for at least 99% of the particles; the fraction that diverges is
@@ -73,9 +76,12 @@ def matrix_parity(M, f, cols=slice(None)):
         ref_rows[k, c == i] = 0.0
     e_rows = float(np.max(np.abs(got - ref_rows).sum(axis=1) / np.abs(ref_rows).sum(axis=1)))
     e_diag = float(np.max(np.abs(d - f["diag"]) / np.abs(f["diag"])))
-    print(f"matrix: M v (off-diagonal) {e_mv:.2e}, rows {e_rows:.2e}, diagonal vs QUADPACK {e_diag:.2e}", flush=True)
+    e_ref = float(np.max(np.abs(d - f["diag_ref"])))
+    e_ref_rel = float(np.max(np.abs(d - f["diag_ref"]) / np.abs(f["diag_ref"])))
+    print(f"matrix: M v (off-diagonal) {e_mv:.2e}, rows {e_rows:.2e}, diagonal vs converged QUADPACK {e_diag:.2e}, vs the "
+          f"reference-tolerance QUADPACK {e_ref:.2e} absolute ({e_ref_rel:.2e} relative)", flush=True)
     assert e_mv < 1e-12 and e_rows < 1e-12
-    assert e_diag < 1e-9
+    assert e_diag < 1e-11 and e_ref <= 1e-9
     return d
 
 
@@ -135,14 +141,13 @@ def test_c0_matrix_charges_fields(c0):
     F = np.where(np.isin(types, [1, 2, 5]), values, 0.0)
     X, st, _ = device.solve_system(M, F)
     q = X.cpu().numpy()[0]
-    e_lapack = rel(q, f["q_lapack"])
-    M.diagonal().copy_(torch.tensor(f["diag"], device=M.device))
+    e_q, e_lapack = rel(q, f["q_refined"]), rel(q, f["q_lapack"])
+    M.diagonal().copy_(torch.tensor(f["diag_ref"], device=M.device))
     Xs, _, _ = device.solve_system(M, F)
-    e_sub = rel(Xs.cpu().numpy()[0], f["q_refined"])
-    print(f"C0 charges: vs LAPACK {e_lapack:.2e} (own diagonal), with the QUADPACK diagonal vs the refined solution "
-          f"{e_sub:.2e}; {st}", flush=True)
-    assert e_sub < 1e-11
-    assert e_lapack < 1e-8
+    e_sub = rel(Xs.cpu().numpy()[0], f["q_ref_refined"])
+    print(f"C0 charges: {e_q:.2e} from the oracle's refined solution ({e_lapack:.2e} from its LAPACK solution); with the "
+          f"reference-tolerance diagonal {e_sub:.2e} from that system's refined solution; {st}", flush=True)
+    assert e_q < 1e-10 and e_sub < 1e-10
     S = device.Sources(X[0], E.jac, E.pos)
     P, Fd = S.field_points(device.to_dev(f["pts"]))
     e_p, e_f = rel(P.cpu().numpy(), f["pot"]), rel(Fd.cpu().numpy(), f["field"])
@@ -180,17 +185,17 @@ def test_c1_mirror_full_size(gpu):
     X, st, _ = device.solve_system(M64, F, M32=M32, mode="mixed")
     del M32
     q = X.cpu().numpy()[0]
-    e_lapack = rel(q, f["q_lapack"])
+    e_q, e_lapack = rel(q, f["q_refined"]), rel(q, f["q_lapack"])
     res_rel = st.residual / np.max(np.abs(F))
-    # the reference system itself: the fixture's QUADPACK diagonal in both matrices
-    M64.diagonal().copy_(torch.tensor(f["diag"], device=M64.device))
+    # the reference's own system: its reference-tolerance QUADPACK diagonal in both matrices
+    M64.diagonal().copy_(torch.tensor(f["diag_ref"], device=M64.device))
     Xs, st2, _ = device.solve_system(M64, F, M32=M64.float(), mode="mixed")
-    e_sub = rel(Xs.cpu().numpy()[0], f["q_refined"])
-    print(f"C1 mixed solve {st}: residual {res_rel:.2e} of max|F|; charges vs LAPACK {e_lapack:.2e} (own diagonal); "
-          f"with the QUADPACK diagonal vs the refined solution {e_sub:.2e} ({st2.mode}, {st2.iterations} steps)", flush=True)
+    e_sub = rel(Xs.cpu().numpy()[0], f["q_ref_refined"])
+    print(f"C1 mixed solve {st}: residual {res_rel:.2e} of max|F|; charges {e_q:.2e} from the oracle's refined solution "
+          f"({e_lapack:.2e} from its LAPACK solution); with the reference-tolerance diagonal {e_sub:.2e} from that system's "
+          f"refined solution ({st2.mode}, {st2.iterations} steps)", flush=True)
     assert st.mode == "mixed" and res_rel < 1e-12
-    assert e_sub < 1e-11
-    assert e_lapack < 1e-8
+    assert e_q < 1e-10 and e_sub < 1e-10
     del M64
     S = device.Sources(X[0], E.jac, E.pos)
     P, Fd = S.field_points(device.to_dev(f["pts"]))
@@ -215,7 +220,7 @@ def test_c2_reflecting_lens_landings(gpu):
     field = solver.solve_direct(exc)
     q = field.electrostatic_point_charges.charges
     print(f"C2 charges vs LAPACK {rel(q, f['q']):.2e}", flush=True)
-    assert rel(q, f["q"]) < 1e-8
+    assert rel(q, f["q"]) < 1e-9
     axial = FieldRadialAxial(field, f["z"][0], f["z"][-1], N=len(f["z"]))
     from traceon_amd import device
     d = field.electrostatic_point_charges.device_sources().axial_derivatives(device.to_dev(f["z"][::8])).cpu().numpy()
@@ -260,7 +265,7 @@ def test_c3_magnetic_lens_charges_series_landings(c3):
     qe = field.electrostatic_point_charges.charges
     qm = field.magnetostatic_point_charges.charges[-len(f["qm"]):]
     print(f"C3 charges vs LAPACK: electrostatic {rel(qe, f['qe']):.2e}, magnetostatic {rel(qm, f['qm']):.2e}", flush=True)
-    assert rel(qe, f["qe"]) < 1e-8 and rel(qm, f["qm"]) < 1e-8
+    assert rel(qe, f["qe"]) < 1e-9 and rel(qm, f["qm"]) < 1e-9
     de = field.get_electrostatic_axial_potential_derivatives(f["z"][::8])
     dm = field.get_magnetostatic_axial_potential_derivatives(f["z"][::8])
     e_e = max(rel(de[:, l], f["derivs_e"][:, l]) for l in range(9))

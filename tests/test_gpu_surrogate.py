@@ -215,3 +215,45 @@ def test_zero_trust_region_is_direct_bem(lens):
           "max landing diff", np.max(np.abs(la[ok] - lb[ok])), flush=True)
     assert np.array_equal(np.isfinite(la[:, 0]), ok)
     assert np.max(np.abs(la[ok, :3] - lb[ok, :3])) < 1e-12
+
+
+def test_error_check_routes_points_to_bem(lens):
+    """The embedded error check: with a zero tolerance every point that would get a correction
+    (r >= r_lo inside the trained z range) is evaluated by the direct BEM sum, in the tracer's
+    kernel as in the Python path; with an infinite one no point is; and the share of points
+    sent to the BEM sum grows as the tolerance shrinks."""
+    from traceon_amd.surrogate import FieldRadialAxialLearned
+    from traceon_amd.tracing import DeviceFieldProgram
+    import torch
+    field, axial, sur, _ = lens
+    rng = np.random.default_rng(12)
+    n = 3000
+    r = rng.uniform(0.0, 0.6 * MM, n)
+    phi = rng.uniform(0, 2 * np.pi, n)
+    z = rng.uniform(-5.5 * MM, 5.5 * MM, n)
+    pts = np.column_stack([r * np.cos(phi), r * np.sin(phi), z])
+    checked = (r >= sur.r_lo) & (z >= sur.z_min) & (z <= sur.z_max)
+    Eb = field.electrostatic_field_at_points(pts)
+    Ea = axial.electrostatic_field_at_points(pts)
+    for tol in (0.0, float("inf")):
+        learned = FieldRadialAxialLearned(sur, error_tolerance=tol)
+        Ek, _ = DeviceFieldProgram(learned).evaluate(pts)
+        Ep = learned.electrostatic_field_at_points(pts)
+        if tol == 0.0:
+            assert rel(Ek[checked], Eb[checked]) < 1e-12 and rel(Ep[checked], Eb[checked]) < 1e-12
+            assert rel(Ek[~checked], Ea[~checked]) < 1e-12
+        else:
+            corr = np.max(np.abs(Ep - Ea))
+            assert np.max(np.abs(Ek - Ep)) <= 1e-2 * corr
+            assert not np.allclose(Ek[checked], Eb[checked], rtol=1e-12, atol=0)
+    pe = sur.predicted_error(torch.tensor(r, device="cuda"), torch.tensor(z, device="cuda")).cpu().numpy()
+    shares = [float(np.mean(pe[checked] > t)) for t in (1e-4, 1e-5, 1e-6, 1e-7)]
+    print("share of corrected points sent to BEM at tolerances 1e-4..1e-7:", shares, flush=True)
+    assert all(a <= b for a, b in zip(shares, shares[1:]))
+    # the predicted error bounds the actual one: measured against BEM at the checked points
+    E_l = FieldRadialAxialLearned(sur, error_tolerance=float("inf")).electrostatic_field_at_points(pts)
+    sE = np.max(np.abs(Eb))
+    actual = np.max(np.abs(E_l - Eb), axis=1) / sE
+    frac_bounded = float(np.mean(actual[checked] <= pe[checked] + 1e-15))
+    print("fraction of corrected points whose error is within the prediction:", frac_bounded, flush=True)
+    assert frac_bounded > 0.9

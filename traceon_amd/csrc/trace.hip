@@ -179,6 +179,29 @@ __device__ void mlp_correct(const FieldComp& c, const double q[3], double e[3], 
   h[0] += d[3] * cx; h[1] += d[3] * cy; h[2] += d[4];
 }
 
+// The surrogate's embedded error check (FieldSurrogate.calibrate): its predicted error at
+// (r, z) relative to the field's scale, w(r) max_k sigma_k(z), over the tolerance (rows 5..9 of
+// mlp_zs hold sigma_k / tolerance); above 1 the point is evaluated by the ring sources.
+__device__ __forceinline__ bool mlp_error_exceeds(const FieldComp& c, const double q[3], double r) {
+  const double r_lo = c.mlp_geom[0], r_max = c.mlp_geom[1], z0 = c.mlp_geom[2], z1 = c.mlp_geom[3];
+  if (r < r_lo || q[2] < z0 || q[2] > z1) return false;
+  const int nz = c.mlp_nz;
+  double tz = (q[2] - z0) / (z1 - z0) * nz - 0.5;
+  tz = tz < 0.0 ? 0.0 : (tz > nz - 1.0 ? nz - 1.0 : tz);
+  int i0 = (int)tz;
+  if (i0 > nz - 2) i0 = nz > 1 ? nz - 2 : 0;
+  const double f = tz - i0;
+  const int i1 = nz > 1 ? i0 + 1 : 0;
+  const double* Se = c.mlp_zs + 5 * nz;
+  double m = 0.0;
+#pragma unroll
+  for (int k = 1; k < 5; k++) {
+    const double v = Se[k * nz + i0] * (1.0 - f) + Se[k * nz + i1] * f;
+    m = v > m ? v : m;
+  }
+  return pow(r / r_max, c.mlp_geom[4]) * m > 1.0;
+}
+
 typedef short mw_s4 __attribute__((ext_vector_type(4)));
 typedef float mw_f4 __attribute__((ext_vector_type(4)));
 
@@ -302,9 +325,10 @@ __device__ void eval_field(const FieldProgram& F, const double p[3], double E[3]
       bool use_bem = false;
       if (MLP && c.mlp) {
         // the axial series is trusted up to r_max inside its interpolation range; the learned
-        // correction adds to it where it was trained (mlp_correct); elsewhere: ring sources
+        // correction adds to it where it was trained (mlp_correct) unless its predicted error
+        // exceeds the tolerance; elsewhere: ring sources
         const double r = sqrt(q[0] * q[0] + q[1] * q[1]);
-        const bool trust = r <= c.mlp_geom[5] && q[2] >= c.zs[0] && q[2] <= c.zs[c.nz - 1];
+        const bool trust = r <= c.mlp_geom[5] && q[2] >= c.zs[0] && q[2] <= c.zs[c.nz - 1] && !mlp_error_exceeds(c, q, r);
         use_bem = !trust && (c.elec.n || c.mag.n || c.cur.n);
       }
       if (MLP && use_bem) {

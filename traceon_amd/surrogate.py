@@ -17,8 +17,12 @@ Below r_lo the axial series is already accurate, so there is no correction there
 
 In the tracer the field is F_axial + correction inside the trust region, which is r <= r_max
 within the axial interpolation range. The correction applies for r >= r_lo within the z range
-it was trained on. Outside the trust region the tracer falls back to the direct BEM sum. The
-surrogate has no counterpart in the reference.
+it was trained on. Outside the trust region the tracer falls back to the direct BEM sum, and
+so it does wherever the surrogate's embedded error estimate exceeds a tolerance: after
+training, ``calibrate`` measures the fit error of the scaled targets per z bin on fresh
+BEM-labelled samples, so that w(r) sigma_k(z) predicts the error of the correction at (r, z)
+relative to the field's scale; the tracer compares that prediction with the tolerance before it
+evaluates the network. The surrogate has no counterpart in the reference.
 """
 from __future__ import annotations
 
@@ -79,6 +83,7 @@ class FieldSurrogate:
         self.Wbf = torch.empty(3 * 128 * 128, dtype=torch.int16, device=dev)
         backend.check(L.tb_mlp_to_bf16(device.ptr(self.P), device.ptr(self.Wbf), device.stream_ptr()), "mlp_to_bf16")
         self.zscale = np.ones((NOUT, self.nz))   # s_k(z) at the bin centres
+        self.zerr = None                         # sigma_k(z): predicted error per unit weight, relative to the field scale
         self.active = np.ones(NOUT, dtype=bool)  # components with a nonzero residual
         self.step = 0
         self.X = self.T = None
@@ -116,14 +121,18 @@ class FieldSurrogate:
         return torch.stack([2.0 * (r - self.r_lo) / (self.r_max - self.r_lo) - 1.0,
                             2.0 * (z - self.z_min) / (self.z_max - self.z_min) - 1.0], dim=1)
 
-    def _zscale(self, z):
-        """s_k(z), piecewise linear between the z-bin centres, (M, 5)."""
+    def _ztable(self, table, z):
+        """A (5, nz) bin-centre table, piecewise linear between the z-bin centres, at (M,) points: (M, 5)."""
         t = (z - self.z_min) / (self.z_max - self.z_min) * self.nz - 0.5
         t = torch.clamp(t, 0.0, self.nz - 1.0)
         i0 = torch.clamp(torch.floor(t).long(), 0, self.nz - 2) if self.nz > 1 else torch.zeros_like(t, dtype=torch.long)
         f = (t - i0.double())[:, None]
-        zs = torch.tensor(self.zscale.T, device=z.device)   # (nz, 5)
+        zs = torch.tensor(table.T, device=z.device)   # (nz, 5)
         return zs[i0] * (1.0 - f) + zs[torch.clamp(i0 + 1, max=self.nz - 1)] * f
+
+    def _zscale(self, z):
+        """s_k(z), piecewise linear between the z-bin centres, (M, 5)."""
+        return self._ztable(self.zscale, z)
 
     def generate(self, n_samples: int, chunk: int = 1 << 19):
         """n_samples (r, z) uniform in [r_lo, r_max] x [z_min, z_max], labelled by BEM - axial."""
@@ -190,6 +199,48 @@ class FieldSurrogate:
         self.train_seconds = time.time() - t0
         return self.history
 
+    # ---- the embedded error estimate ----------------------------------------------------------
+    def calibrate(self, n_samples: int = 1 << 20, chunk: int = 1 << 19, safety: float = 3.0, seed: int = 12345):
+        """Fit error of the trained network per z bin on fresh samples (r uniform in
+        [r_lo, r_max], z uniform in the trained range), labelled by BEM - axial like the training
+        set: sigma_k(z) = safety * RMS(prediction - label) / w(r), per component, spread to the
+        neighbouring bins (max over 3) and divided by the largest |E| (components 1, 2) or |H|
+        (3, 4) of the samples.  w(r) sigma_k(z) then bounds the correction's error at (r, z)
+        relative to the field's scale."""
+        dev = device.device()
+        rng = np.random.default_rng(seed)
+        ss = torch.zeros((self.nz, NOUT), dtype=torch.float64, device=dev)
+        cnt = torch.zeros(self.nz, dtype=torch.float64, device=dev)
+        sE = sH = 0.0
+        for s0 in range(0, n_samples, chunk):
+            m = min(chunk, n_samples - s0)
+            r = torch.tensor(rng.uniform(self.r_lo, self.r_max, m), dtype=torch.float64, device=dev)
+            z = torch.tensor(rng.uniform(self.z_min, self.z_max, m), dtype=torch.float64, device=dev)
+            pts = torch.stack([r, torch.zeros_like(r), z], dim=1).contiguous()
+            Fb, Fa = self._exact_and_axial(pts)
+            sE = max(sE, float(Fb[:, 1:3].abs().max().item()))
+            sH = max(sH, float(Fb[:, 3:5].abs().max().item()))
+            err = (self.correction(r, z) - (Fb - Fa)) / self._weight(r)[:, None]
+            b = torch.clamp(((z - self.z_min) / (self.z_max - self.z_min) * self.nz).long(), 0, self.nz - 1)
+            ss.index_add_(0, b, err * err)
+            cnt += torch.bincount(b, minlength=self.nz).double()
+        rms = torch.sqrt(ss / cnt.clamp(min=1.0)[:, None]).cpu().numpy()
+        pad = np.pad(rms, ((1, 1), (0, 0)), mode="edge")
+        rms = np.maximum(np.maximum(pad[:-2], pad[1:-1]), pad[2:])
+        scale = np.array([1.0, sE or 1.0, sE or 1.0, sH or 1.0, sH or 1.0])
+        self.zerr = (safety * rms / scale[None, :]).T.copy() * self.active[:, None]
+        self.zerr[0] = 0.0   # the potential is not part of the traced force
+        return self
+
+    def predicted_error(self, r: torch.Tensor, z: torch.Tensor) -> torch.Tensor:
+        """max_k w(r) sigma_k(z) at (r, z) (device tensors): the predicted error of the correction
+        relative to the field's scale; 0 where no correction applies."""
+        if self.zerr is None:
+            self.calibrate()
+        e = self._ztable(self.zerr, z).max(dim=1).values * self._weight(r)
+        inside = (r >= self.r_lo) & (z >= self.z_min) & (z <= self.z_max)
+        return torch.where(inside, e, torch.zeros_like(e))
+
     # ---- evaluation ----------------------------------------------------------------------
     def correction(self, r: torch.Tensor, z: torch.Tensor) -> torch.Tensor:
         """The learned residual (M, 5) in physical units at (r, z) (device tensors)."""
@@ -203,11 +254,12 @@ class FieldSurrogate:
         act = torch.tensor(self.active, dtype=torch.float64, device=r.device)
         return Y.double() * self._zscale(z) * self._weight(r)[:, None] * act[None, :]
 
-    def tracer_arrays(self):
+    def tracer_arrays(self, error_tolerance: float = float("inf")):
         """Device arrays of the tracer's learned-correction component (csrc/trace_kernels.h):
         FP32 weights with W2..W4 taken at their BF16 values (the function the network was
-        trained as) and transposed to (in, out), the s_k(z) table (5, nz) and the geometry
-        (r_lo, r_max, z_min, z_max, p, 0)."""
+        trained as) and transposed to (in, out), the tables (10, nz): s_k(z), then
+        sigma_k(z) / error_tolerance (a point whose w(r) max_k of these exceeds 1 goes to the
+        BEM sum), and the geometry (r_lo, r_max, z_min, z_max, p, 0)."""
         P = self.P.detach().clone()
         o = 384
         for l in range(3):
@@ -216,7 +268,16 @@ class FieldSurrogate:
             w = (w << 16).view(torch.float32).view(128, 128)
             P[o:o + 128 * 128] = w.t().contiguous().view(-1)
             o += 128 * 128 + 128
-        zs = torch.tensor(self.zscale * self.active[:, None], dtype=torch.float64, device=P.device).contiguous()
+        if self.zerr is None:
+            self.calibrate()
+        with np.errstate(divide="ignore", invalid="ignore"):
+            if error_tolerance <= 0:
+                zerr = np.where(self.active[:, None], 1e300, 0.0) * np.ones_like(self.zerr)
+            else:
+                zerr = self.zerr / error_tolerance
+        zerr[0] = 0.0
+        tables = np.concatenate([self.zscale * self.active[:, None], zerr], axis=0)
+        zs = torch.tensor(tables, dtype=torch.float64, device=P.device).contiguous()
         geom = [self.r_lo, self.r_max, self.z_min, self.z_max, self.p, 0.0]
         return P.contiguous(), zs, geom
 
@@ -255,14 +316,18 @@ class FieldRadialAxialLearned(FieldRadialAxial):
     the surrogate's trust region.  Field evaluation and tracing go through the GPU; the
     reference-named single-point trace functions do not apply (trace_batch / Tracer)."""
 
-    def __init__(self, surrogate: FieldSurrogate, trust_radius: float | None = None):
+    def __init__(self, surrogate: FieldSurrogate, trust_radius: float | None = None, error_tolerance: float = 1e-5):
         """trust_radius: the axial field (+ correction) serves r <= trust_radius inside the axial
         range; the direct BEM sum serves everything else.  Default: the surrogate's r_max.
-        A trust radius of 0 sends every off-axis evaluation to the BEM sum."""
+        A trust radius of 0 sends every off-axis evaluation to the BEM sum.
+        error_tolerance: inside the trust region, points where the surrogate's predicted error
+        (FieldSurrogate.predicted_error, relative to the field's scale) exceeds it also go to the
+        BEM sum; inf disables the check, 0 sends every corrected point to the BEM sum."""
         ax = surrogate.axial
         FieldRadialAxial.__bases__[0].__init__(self, ax.field, ax.z, ax.electrostatic_coeffs, ax.magnetostatic_coeffs)
         self.surrogate = surrogate
         self.trust_radius = float(surrogate.r_max if trust_radius is None else trust_radius)
+        self.error_tolerance = float(error_tolerance)
         self._origin, self._basis = ax._origin, ax._basis
         self._update_inverse_transformation_matrix()
 
@@ -274,7 +339,11 @@ class FieldRadialAxialLearned(FieldRadialAxial):
         r = np.hypot(P[:, 0], P[:, 1])
         inside = (r <= self.trust_radius) & (P[:, 2] >= self.z[0]) & (P[:, 2] <= self.z[-1])
         dev = device.device()
-        C = s.correction(torch.tensor(r, device=dev), torch.tensor(P[:, 2], device=dev)).cpu().numpy()
+        rt, zt = torch.tensor(r, device=dev), torch.tensor(P[:, 2], device=dev)
+        pe = s.predicted_error(rt, zt).cpu().numpy()
+        checked = (r >= s.r_lo) & (P[:, 2] >= s.z_min) & (P[:, 2] <= s.z_max)
+        inside &= ~(checked & ((pe > self.error_tolerance) if self.error_tolerance > 0 else True))
+        C = s.correction(rt, zt).cpu().numpy()
         C[~((P[:, 2] >= s.z_min) & (P[:, 2] <= s.z_max) & inside)] = 0.0
         return P, r, inside, C
 

@@ -135,7 +135,7 @@ def make_component(field, factor: float, kind: str, in_superposition: bool):
         if kind == "learned":
             # the learned correction, and the ring sources for outside its trust region
             sur = field.surrogate
-            wts, zs, geom = sur.tracer_arrays()
+            wts, zs, geom = sur.tracer_arrays(field.error_tolerance)
             c.mlp, c.mlp_zs, c.mlp_nz = wts.data_ptr(), zs.data_ptr(), zs.shape[1]
             for i, v in enumerate(geom[:5]):
                 c.mlp_geom[i] = v
