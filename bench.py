@@ -183,6 +183,10 @@ def oracle_traces(f, prefix):
     return {"landing": f[prefix + "landing"], "plane": f[prefix + "plane"]}
 
 
+def shape_str(s):
+    return f"{'ordered' if s.ordered else 'unordered'},batch={s.batch},occ={s.occupancy}"
+
+
 def roofline_trace(steps_attempted, ms, series_per_eval):
     """Tracer roofline: algorithmic FLOP per attempted RKF45 step = C_rk + 6 evaluations of
     series_per_eval paraxial series (C_axeval each), against the FP64 vector peak."""
@@ -198,7 +202,7 @@ def tracing_leg(n_particles, n_sample, n_axial=7000):
     (lognormal 1 keV, 0.2 mm disc, 5 mrad) through FieldRadialAxial; transmitted electrons land
     on z = 9.5 mm, reflected ones back through z = -4.5 mm, both recorded in the same launch."""
     import torch
-    from traceon_amd import configs, solver
+    from traceon_amd import autotune, configs, device, solver
     from traceon_amd.excitation import Excitation
     from traceon_amd.field import FieldRadialAxial
     mm = 1e-3
@@ -215,15 +219,18 @@ def tracing_leg(n_particles, n_sample, n_axial=7000):
     t_axial_build = time.perf_counter() - t0
     P, V, bounds, exit_plane, back_plane = configs.reflecting_beam(n_particles)
     tr_ax = axial.get_tracer(bounds)
-    traces(tr_ax, P[:4096], V[:4096], exit_plane, back_plane)   # warm-up
-    res = traces(tr_ax, P, V, exit_plane, back_plane, timer=True)
+    Pd, Vd = device.to_dev(P), device.to_dev(V)    # the beam is resident in HBM before timing
+    traces(tr_ax, Pd[:4096], Vd[:4096], exit_plane, back_plane)   # warm-up
+    shape, tuning = autotune.tune(tr_ax, Pd, Vd, plane=exit_plane, plane2=back_plane)
+    res = traces(tr_ax, Pd, Vd, exit_plane, back_plane, timer=True)
     nacc = res["n_accepted"]
     out = {"n": n_particles, "ms": res["ms"], "electrons_per_s": n_particles / (res["ms"] * 1e-3),
            "steps_attempted": int(res["n_attempted"].sum()), "steps_accepted": int(nacc.sum()),
            "reflected_fraction": float(np.mean(res["plane"] == 1)), "lost_fraction": float(np.mean(res["plane"] < 0)),
            "len_ratio_max_min": float(nacc.max() / max(nacc.min(), 1)),
            "len_ratio_p999_p001": float(np.percentile(nacc, 99.9) / max(np.percentile(nacc, 0.1), 1)),
-           "n_axial": len(axial.z), "axial_build_s": t_axial_build, "N_elements": len(field.electrostatic_point_charges)}
+           "n_axial": len(axial.z), "axial_build_s": t_axial_build, "N_elements": len(field.electrostatic_point_charges),
+           "launch_shape": shape_str(shape), "tuning_ms": {shape_str(k): round(v, 3) for k, v in tuning.items()}}
     out["roofline"] = roofline_trace(out["steps_attempted"], res["ms"], 1)
     # accuracy: against the FP64 oracle's direct BEM traces (reference C backend, converged
     # diagonal) of the beam's first particles, and against the GPU's direct FP64 BEM tracer
@@ -257,7 +264,7 @@ def c3_leg(n_particles, n_sample, n_axial=7000):
     gap 3 mm) and a three-tube einzel lens, N = 12,000; electrostatic and magnetostatic solves,
     10 keV electrons traced through both axial series with the Lorentz force."""
     import torch
-    from traceon_amd import configs, solver
+    from traceon_amd import autotune, configs, device, solver
     from traceon_amd.field import FieldRadialAxial
     mm = 1e-3
     exc = magnetic_problem()
@@ -272,12 +279,15 @@ def c3_leg(n_particles, n_sample, n_axial=7000):
     t_axial = time.perf_counter() - t0
     P, V, bounds, plane = configs.magnetic_beam(n_particles)
     tr_ax = axial.get_tracer(bounds)
-    traces(tr_ax, P[:4096], V[:4096], plane)
-    res = traces(tr_ax, P, V, plane, timer=True)
+    Pd, Vd = device.to_dev(P), device.to_dev(V)
+    traces(tr_ax, Pd[:4096], Vd[:4096], plane)
+    shape, tuning = autotune.tune(tr_ax, Pd, Vd, plane=plane)
+    res = traces(tr_ax, Pd, Vd, plane, timer=True)
     out = {"N_electrostatic": len(field.electrostatic_point_charges), "N_magnetostatic": len(field.magnetostatic_point_charges),
            "N_coil_rings": len(field.current_point_charges), "solve_s": t_solve, "axial_build_s": t_axial, "n": n_particles,
            "ms": res["ms"], "electrons_per_s": n_particles / (res["ms"] * 1e-3), "steps_attempted": int(res["n_attempted"].sum()),
-           "landed_fraction": float(np.mean(res["plane"] == 0))}
+           "landed_fraction": float(np.mean(res["plane"] == 0)),
+           "launch_shape": shape_str(shape), "tuning_ms": {shape_str(k): round(v, 3) for k, v in tuning.items()}}
     out["roofline"] = roofline_trace(out["steps_attempted"], res["ms"], 2)
     f = golden("config_c3.npz")
     if f is not None:
@@ -287,7 +297,7 @@ def c3_leg(n_particles, n_sample, n_axial=7000):
     out["vs_gpu_bem"] = landing_error(res, ref)
     out["electrons_per_s_direct_bem_fp64"] = n_sample / (ref["ms"] * 1e-3)
     out["n_sample"] = int(n_sample)
-    ctx = {"field": field, "axial": axial, "P": P, "V": V, "bounds": bounds, "plane": plane, "ref": ref, "res": res}
+    ctx = {"field": field, "axial": axial, "P": Pd, "V": Vd, "bounds": bounds, "plane": plane, "ref": ref, "res": res}
     return out, ctx
 
 
@@ -300,7 +310,7 @@ def c4_leg(ctx, n_samples=1 << 22, steps=8000):
     off-axis beam (start radius 0.3-0.5 mm) against the FP64 oracle's direct BEM traces, beside
     the axial series alone."""
     import torch
-    from traceon_amd import configs
+    from traceon_amd import autotune, configs
     from traceon_amd.surrogate import FieldRadialAxialLearned, FieldSurrogate
     mm = 1e-3
     field, axial, P, V = ctx["field"], ctx["axial"], ctx["P"], ctx["V"]
@@ -316,13 +326,15 @@ def c4_leg(ctx, n_samples=1 << 22, steps=8000):
     learned = FieldRadialAxialLearned(sur)
     tr = learned.get_tracer(bounds)
     traces(tr, P[:4096], V[:4096], plane)
+    shape, tuning = autotune.tune(tr, P, V, plane=plane)
     res = traces(tr, P, V, plane, timer=True)
     bands = {k: {"axial_E_rms": v["axial"]["E_rms"], "learned_E_rms": v["learned"]["E_rms"], "axial_H_rms": v["axial"]["H_rms"],
                  "learned_H_rms": v["learned"]["H_rms"]} for k, v in val.items()}
     out = {"model": "MLP 2-128-128-128-128-5 SiLU, BF16 MFMA hidden layers, FP32 master weights", "samples": n_samples,
            "train_steps": steps, "batch": 16384, "label_s": t_gen, "train_s": sur.train_seconds, "loss_history": hist,
            "field_error_by_radius": bands, "n": len(P), "ms": res["ms"], "electrons_per_s": len(P) / (res["ms"] * 1e-3),
-           "vs_gpu_bem": landing_error(res, ctx["ref"])}
+           "vs_gpu_bem": landing_error(res, ctx["ref"]), "launch_shape": shape_str(shape),
+           "tuning_ms": {shape_str(k): round(v, 3) for k, v in tuning.items()}}
     f = golden("config_c4.npz")
     if f is not None:
         Po, Vo, bo, po = configs.magnetic_beam(64, seed=4, r_range=(0.3 * mm, 0.5 * mm))

@@ -271,13 +271,25 @@ typedef struct {
      plane it records where reflected particles leave, beside the exit plane's landing */
   double plane2_p[3], plane2_n[3], plane2_nn;
   double *landing2;
+  /* launch shape of tb_trace (tb_trace_split ignores it): order (n ints, null: 0..n-1) is the
+     order in which the queue hands out particles (tb_trace_order makes one in which the
+     lanes of a wave travel together); batch = 1: a wave takes 64 particles at once when all
+     its lanes are free; occupancy: waves per SIMD of the axial-only tracer (1-4, 0: 2).
+     None of them changes any particle's result. */
+  const int *order;
+  int batch, occupancy;
 } tb_trace_params_t;
 
 /* Persistent RK45 tracer over params->n particles (all pointers device pointers).
    One lane per particle; waves: number of 64-lane workgroups that drain the particle
-   queue (0: 8 per CU).  Ring-source sums of direct BEM fields run wave-uniformly, every
-   lane of a wave sharing each source. */
+   queue (0: 4 x occupancy per CU for axial-only fields, 8 per CU otherwise).  Ring-source sums of direct BEM fields
+   run wave-uniformly, every lane of a wave sharing each source. */
 int tb_trace(const tb_trace_params_t *params, int waves, void *stream);
+/* Queue order for tb_trace: a counting sort of the n particles of state0 (n x 6, device) by
+   start z (major) and start v_z (minor) into order (n ints, device); workspace of
+   tb_trace_order_workspace_bytes(n) bytes. */
+size_t tb_trace_order_workspace_bytes(int n);
+int tb_trace_order(const double *state0, int n, int *order, void *workspace, void *stream);
 /* Stage-synchronous tracer for a few particles (n <= 65535) in fields with ring-source
    sums: each RK stage spreads every particle's source sum over nchunk workgroups and adds
    the partials in a fixed order; the RK arithmetic per particle is that of tb_trace.

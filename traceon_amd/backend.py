@@ -127,6 +127,8 @@ _SIG = {
     "tb_jacobian_3d": (i32, [vp, i32, vp, vp, vp]),
     "tb_charge_radial": (i32, [vp, vp, i32, vp, vp]),
     "tb_trace": (i32, [vp, i32, vp]),
+    "tb_trace_order_workspace_bytes": (sz, [i32]),
+    "tb_trace_order": (i32, [vp, i32, vp, vp, vp]),
     "tb_trace_split_chunks": (i32, [i32, C.c_longlong]),
     "tb_trace_split_workspace_bytes": (sz, [i32, i32]),
     "tb_trace_split": (i32, [vp, i32, vp, vp]),

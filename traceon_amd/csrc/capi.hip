@@ -644,9 +644,15 @@ int tb_trace(const tb_trace_params_t* params, int waves, void* stream) {
     hipGetDevice(&dev);
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, dev) != hipSuccess) { set_err("device properties"); return -1; }
-    nw = prop.multiProcessorCount * 8;
+    const int occ = params->occupancy >= 1 && params->occupancy <= 4 ? params->occupancy : 2;
+    nw = prop.multiProcessorCount * ((params->field.flags & tb::PROGRAM_NO_SOURCES) ? 4 * occ : 8);
   }
   TB_CALL(tb::launch_trace(*reinterpret_cast<const tb::TraceParams*>(params), nw, S(stream)), "trace");
+}
+size_t tb_trace_order_workspace_bytes(int n) { return tb::trace_order_workspace_bytes(n); }
+int tb_trace_order(const double* state0, int n, int* order, void* workspace, void* stream) {
+  if (n > 0 && (!state0 || !order || !workspace)) { set_err("tb_trace_order: null pointer"); return -1; }
+  TB_CALL(tb::trace_order(state0, n, order, workspace, S(stream)), "trace_order");
 }
 int tb_trace_split_chunks(int n_particles, long long n_sources) { return tb::split_chunks(n_particles, n_sources); }
 size_t tb_trace_split_workspace_bytes(int n_particles, int nchunk) { return tb::split_workspace_bytes(n_particles, nchunk); }

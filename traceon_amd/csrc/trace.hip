@@ -405,8 +405,9 @@ __device__ __forceinline__ void trace_body(const TraceParams& P) {
     }
     if (need) {
       const int rank = __popcll(mask & ((1ull << lane) - 1ull));
-      const int id = base + rank;
-      if (id < P.n) {
+      const int k = base + rank;
+      const int id = k < P.n ? (P.order ? P.order[k] : k) : -1;
+      if ((unsigned)id < (unsigned)P.n) {   // an order entry outside 0..n-1 is skipped
         pid = id;
         for (int c = 0; c < 6; c++) y[c] = P.state0[6 * (size_t)id + c];
         t = P.t0 ? P.t0[id] : 0.0;
@@ -454,7 +455,7 @@ __device__ __forceinline__ void trace_body(const TraceParams& P) {
       if (out0) retire(STATUS_LEFT_BOUNDS);
       if (__ballot(out0) == 0ull) break;
       if (out0) pid = -1;
-      next_particle(out0);
+      next_particle(P.batch ? __ballot(pid >= 0) == 0ull : out0);
     }
     if (__ballot(pid >= 0) == 0ull) break;
     const bool active = pid >= 0;
@@ -541,15 +542,18 @@ __device__ __forceinline__ void trace_body(const TraceParams& P) {
     }
     const bool need = st >= 0;
     if (need) pid = -1;
-    next_particle(need);
+    next_particle(P.batch ? __ballot(pid >= 0) == 0ull : need);
   }
 }
 
 template <bool MLP>
 __global__ __launch_bounds__(64) void k_trace(TraceParams P) { trace_body<MLP, true>(P); }
-// axial and uniform fields only: no source-sum code, so that two waves fit a SIMD (forcing
-// three costs more in register spills than it gains)
-__global__ __launch_bounds__(64) void k_trace_axial(TraceParams P) {
+// Axial and uniform fields only: no source-sum code.  W waves per SIMD: at W = 1 the
+// compiler keeps every coefficient load of a field evaluation in flight (256 VGPRs); W = 2
+// fits in 218 VGPRs without spills; W = 3 and 4 spill (about 200 and 320 bytes per lane)
+// and trade scratch traffic for latency hiding.  The launch picks W (occupancy).
+template <int W>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) void k_trace_axial(TraceParams P) {
   trace_body<false, false>(P);
 }
 
@@ -834,8 +838,123 @@ hipError_t launch_trace(const TraceParams& P, int waves, hipStream_t st) {
   const int need = (P.n + 63) / 64;
   if (nw > need) nw = need;
   if (P.field.flags & PROGRAM_HAS_MLP) hipLaunchKernelGGL(k_trace<true>, dim3(nw), dim3(64), 0, st, P);
-  else if (P.field.flags & PROGRAM_NO_SOURCES) hipLaunchKernelGGL(k_trace_axial, dim3(nw), dim3(64), 0, st, P);
-  else hipLaunchKernelGGL(k_trace<false>, dim3(nw), dim3(64), 0, st, P);
+  else if (P.field.flags & PROGRAM_NO_SOURCES) {
+    switch (P.occupancy) {
+      case 1: hipLaunchKernelGGL(k_trace_axial<1>, dim3(nw), dim3(64), 0, st, P); break;
+      case 3: hipLaunchKernelGGL(k_trace_axial<3>, dim3(nw), dim3(64), 0, st, P); break;
+      case 4: hipLaunchKernelGGL(k_trace_axial<4>, dim3(nw), dim3(64), 0, st, P); break;
+      default: hipLaunchKernelGGL(k_trace_axial<2>, dim3(nw), dim3(64), 0, st, P); break;
+    }
+  } else hipLaunchKernelGGL(k_trace<false>, dim3(nw), dim3(64), 0, st, P);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------
+// Queue order: counting sort by a 16-bit key (start z major, start v_z minor).
+// ws layout: key range (4 x u64: ordered-int min/max of z and v_z) | histogram/offsets
+// (ORDER_BINS ints) | per-particle keys (n ints).
+// ---------------------------------------------------------------------------------
+constexpr int ORDER_BINS = 1 << 16, ORDER_ZBINS = 16, ORDER_SCAN_THREADS = 1024;
+
+// doubles as unsigned integers with the same order (NaN excluded by the caller)
+__device__ __forceinline__ unsigned long long ordered_bits(double x) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double from_ordered_bits(unsigned long long u) {
+  return __longlong_as_double((long long)((u >> 63) ? (u & 0x7FFFFFFFFFFFFFFFull) : ~u));
+}
+
+__global__ __launch_bounds__(256) void k_order_init(unsigned long long* range, int* hist) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 2) { range[2 * i] = ~0ull; range[2 * i + 1] = 0ull; }   // (min, max) of z, then of v_z
+  for (int b = i; b < ORDER_BINS; b += gridDim.x * blockDim.x) hist[b] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_order_range(const double* s, int n, unsigned long long* range) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long zmin = ~0ull, zmax = 0ull, vmin = ~0ull, vmax = 0ull;
+  for (int k = i; k < n; k += gridDim.x * blockDim.x) {
+    const double z = s[6 * (size_t)k + 2], v = s[6 * (size_t)k + 5];
+    if (z == z) { const unsigned long long u = ordered_bits(z); zmin = u < zmin ? u : zmin; zmax = u > zmax ? u : zmax; }
+    if (v == v) { const unsigned long long u = ordered_bits(v); vmin = u < vmin ? u : vmin; vmax = u > vmax ? u : vmax; }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    unsigned long long t;
+    t = __shfl_xor(zmin, o, 64); zmin = t < zmin ? t : zmin;
+    t = __shfl_xor(zmax, o, 64); zmax = t > zmax ? t : zmax;
+    t = __shfl_xor(vmin, o, 64); vmin = t < vmin ? t : vmin;
+    t = __shfl_xor(vmax, o, 64); vmax = t > vmax ? t : vmax;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMin(&range[0], zmin); atomicMax(&range[1], zmax);
+    atomicMin(&range[2], vmin); atomicMax(&range[3], vmax);
+  }
+}
+
+__device__ __forceinline__ int order_bin(double x, double lo, double hi, int nb) {
+  if (!(x == x) || !(hi > lo)) return 0;
+  const double f = (x - lo) / (hi - lo) * nb;
+  const int b = (int)f;
+  return b < 0 ? 0 : (b > nb - 1 ? nb - 1 : b);
+}
+
+__global__ __launch_bounds__(256) void k_order_hist(const double* s, int n, const unsigned long long* range, int* hist, int* key) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double zlo = 0.0, zhi = 0.0, vlo = 0.0, vhi = 0.0;
+  if (range[0] <= range[1]) { zlo = from_ordered_bits(range[0]); zhi = from_ordered_bits(range[1]); }
+  if (range[2] <= range[3]) { vlo = from_ordered_bits(range[2]); vhi = from_ordered_bits(range[3]); }
+  const int nzb = zhi > zlo ? ORDER_ZBINS : 1, nvb = ORDER_BINS / nzb;
+  const int k = order_bin(s[6 * (size_t)i + 2], zlo, zhi, nzb) * nvb + order_bin(s[6 * (size_t)i + 5], vlo, vhi, nvb);
+  key[i] = k;
+  atomicAdd(&hist[k], 1);
+}
+
+// exclusive prefix sum of the histogram in place, one workgroup of ORDER_SCAN_THREADS
+__global__ __launch_bounds__(ORDER_SCAN_THREADS) void k_order_scan(int* hist) {
+  constexpr int PER = ORDER_BINS / ORDER_SCAN_THREADS;
+  __shared__ int part[ORDER_SCAN_THREADS];
+  const int t = threadIdx.x;
+  int sum = 0;
+  for (int j = 0; j < PER; j++) sum += hist[t * PER + j];
+  part[t] = sum;
+  __syncthreads();
+  for (int o = 1; o < ORDER_SCAN_THREADS; o <<= 1) {   // Hillis-Steele inclusive scan
+    const int v = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = part[t] - sum;
+  for (int j = 0; j < PER; j++) {
+    const int c = hist[t * PER + j];
+    hist[t * PER + j] = run;
+    run += c;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_order_scatter(const int* key, int n, int* offsets, int* order) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  order[atomicAdd(&offsets[key[i]], 1)] = i;
+}
+
+size_t trace_order_workspace_bytes(int n) {
+  return 4 * sizeof(unsigned long long) + sizeof(int) * (size_t)ORDER_BINS + sizeof(int) * (size_t)(n > 0 ? n : 0);
+}
+
+hipError_t trace_order(const double* state0, int n, int* order, void* ws, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  unsigned long long* range = (unsigned long long*)ws;
+  int* hist = (int*)(range + 4);
+  int* key = hist + ORDER_BINS;
+  const int nb = (n + 255) / 256;
+  hipLaunchKernelGGL(k_order_init, dim3(ORDER_BINS / 256), dim3(256), 0, st, range, hist);
+  hipLaunchKernelGGL(k_order_range, dim3(nb < 1024 ? nb : 1024), dim3(256), 0, st, state0, n, range);
+  hipLaunchKernelGGL(k_order_hist, dim3(nb), dim3(256), 0, st, state0, n, range, hist, key);
+  hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(ORDER_SCAN_THREADS), 0, st, hist);
+  hipLaunchKernelGGL(k_order_scatter, dim3(nb), dim3(256), 0, st, key, n, hist, order);
   return hipGetLastError();
 }
 

@@ -87,10 +87,24 @@ struct TraceParams {
   // particles cross on their way back
   double plane2_p[3], plane2_n[3], plane2_nn;
   double* landing2;       // (n, 6) or null
+  // Launch shape of the persistent tracer (the split tracer ignores these).  order: the
+  // k-th particle taken from the queue is order[k] (null: k); a queue ordered by axial
+  // start and speed lets the lanes of a wave travel together, so that their loads of the
+  // axial coefficients fall in few cache lines.  batch = 1: a wave takes 64 particles at
+  // once, when all its lanes are free, instead of refilling lanes one by one.
+  // occupancy: waves per SIMD the axial-only tracer is compiled for (1-4; 0 picks 2).
+  const int* order;
+  int batch, occupancy;
 };
 
 // One lane per particle; `waves` 64-lane workgroups drain the particle queue.
 hipError_t launch_trace(const TraceParams& P, int waves, hipStream_t st);
+// Queue order for the persistent tracer: a counting sort of the particles by a 16-bit key,
+// the start z (major, 16 bins when the starts differ in z) and the start v_z (minor).
+// Particles with equal keys keep no particular order (their results do not depend on it).
+// ws: trace_order_workspace_bytes(n).
+size_t trace_order_workspace_bytes(int n);
+hipError_t trace_order(const double* state0, int n, int* order, void* ws, hipStream_t st);
 // Stage-synchronous tracer (few particles, n <= 65535): every RK stage spreads each
 // particle's source sums over nchunk workgroups.  ws: split_workspace_bytes(n, nchunk).
 int split_chunks(int n_particles, long long n_sources);

@@ -11,6 +11,7 @@ and returns final states, landing states on a plane and step counts.
 from __future__ import annotations
 
 import ctypes as C
+from dataclasses import dataclass
 from math import sqrt, sin, cos
 
 import numpy as np
@@ -76,7 +77,8 @@ class TraceParamsT(C.Structure):
                 ("final_state", C.c_void_p), ("landing", C.c_void_p), ("status", C.c_void_p), ("n_accepted", C.c_void_p),
                 ("n_attempted", C.c_void_p), ("t_final", C.c_void_p), ("h_final", C.c_void_p), ("traj", C.c_void_p),
                 ("times", C.c_void_p), ("traj_cap", C.c_longlong), ("traj_len", C.c_void_p),
-                ("plane2_p", C.c_double * 3), ("plane2_n", C.c_double * 3), ("plane2_nn", C.c_double), ("landing2", C.c_void_p)]
+                ("plane2_p", C.c_double * 3), ("plane2_n", C.c_double * 3), ("plane2_nn", C.c_double), ("landing2", C.c_void_p),
+                ("order", C.c_void_p), ("batch", C.c_int), ("occupancy", C.c_int)]
 
 
 COMP_KIND = {"bem": 0, "axial": 1, "uniform": 2, "learned": 1}
@@ -211,12 +213,43 @@ class BatchResult:
 SPLIT_MAX_PARTICLES = 16384
 
 
+@dataclass(frozen=True)
+class LaunchShape:
+    """How the persistent tracer drains its particle queue; no choice changes a result.
+    ordered: hand particles out sorted by start z and start v_z (tb_trace_order), so that
+    the lanes of a wave travel together; batch: a wave takes 64 particles at once when all
+    its lanes are free; occupancy: waves per SIMD of the axial-only kernel (1-4);
+    waves: workgroups of the launch (0: as many as the CUs hold at that occupancy)."""
+    ordered: bool = True
+    batch: int = 0
+    occupancy: int = 2
+    waves: int = 0
+
+
+DEFAULT_SHAPE = LaunchShape()
+# below this many particles the queue is not reordered (fewer particles than lanes)
+ORDER_MIN_PARTICLES = 4096
+
+
+def queue_order(S0):
+    """tb_trace_order over a device state tensor (n, 6): the int32 queue order."""
+    import torch
+    from . import device
+    n = S0.shape[0]
+    L = backend.backend_lib
+    order = torch.empty(n, dtype=torch.int32, device=S0.device)
+    ws = torch.empty(L.tb_trace_order_workspace_bytes(n), dtype=torch.uint8, device=S0.device)
+    backend.check(L.tb_trace_order(S0.data_ptr(), n, order.data_ptr(), ws.data_ptr(), device.stream_ptr()), "trace_order")
+    return order
+
+
 def run_trace(program: DeviceFieldProgram, state0, qm, bounds, atol=1e-8, plane=None, traj_cap=0, t0=None,
-              max_attempts=1 << 62, waves=0, split=None, plane2=None):
+              max_attempts=1 << 62, waves=0, split=None, plane2=None, shape: LaunchShape | None = None):
     """GPU tracer over many particles (all arrays numpy or device tensors).
     plane, plane2: (p0, normal); the state at each one's last crossing (NaN if none).
     split: None picks tb_trace_split for fields with ring sources and at most
-    SPLIT_MAX_PARTICLES particles, else the persistent tb_trace."""
+    SPLIT_MAX_PARTICLES particles, else the persistent tb_trace.  shape: launch shape of
+    the persistent tracer (None: DEFAULT_SHAPE); waves > 0 overrides shape.waves."""
     import torch
     from . import device
     S0 = state0 if isinstance(state0, torch.Tensor) else device.to_dev(np.atleast_2d(state0))
@@ -280,15 +313,22 @@ def run_trace(program: DeviceFieldProgram, state0, qm, bounds, atol=1e-8, plane=
     if split and program.has_mlp:
         raise ValueError("fields with a learned correction are traced by the persistent tracer (split=False)")
     L = backend.backend_lib
+    order = None
     if split:
         nchunk = L.tb_trace_split_chunks(n, program.n_sources)
         ws = torch.empty(L.tb_trace_split_workspace_bytes(n, nchunk), dtype=torch.uint8, device=dev)
         backend.check(L.tb_trace_split(C.byref(P), nchunk, ws.data_ptr(), device.stream_ptr()), "trace_split")
     else:
-        backend.check(L.tb_trace(C.byref(P), int(waves), device.stream_ptr()), "trace")
+        shape = shape or DEFAULT_SHAPE
+        if shape.ordered and n >= ORDER_MIN_PARTICLES:
+            order = queue_order(S0)
+            P.order = order.data_ptr()
+        P.batch = int(shape.batch)
+        P.occupancy = int(shape.occupancy)
+        backend.check(L.tb_trace(C.byref(P), int(waves or shape.waves), device.stream_ptr()), "trace")
     out = BatchResult(final, landing, status, nacc, natt, tfin, landing2)
     out.traj, out.times, out.traj_len = traj, times, tlen
-    out._keep = (S0, QM, T0, counter)
+    out._keep = (S0, QM, T0, counter, order)
     return out
 
 
@@ -358,20 +398,37 @@ class Tracer:
                 for p, v, m, c in zip(positions, velocities, masses, charges)]
 
     def trace_batch(self, positions, velocities, mass=M_E, charge=-E_CHARGE, atol: float = 1e-8, plane=None,
-                    max_attempts=1 << 62, waves=0, plane2=None) -> BatchResult:
+                    max_attempts=1 << 62, waves=0, plane2=None, shape="auto") -> BatchResult:
         """Trace many particles in one persistent launch; no trajectories are stored.
         plane = (p0, normal) records the state where each trajectory last crosses it
         (what xy_plane_intersection returns on the full trajectory), NaN if never; plane2 a
-        second such plane (result.landing2), e.g. the entrance plane for reflected particles."""
-        positions = np.atleast_2d(np.asarray(positions, dtype=np.float64))
-        velocities = np.atleast_2d(np.asarray(velocities, dtype=np.float64))
-        positions, velocities = np.broadcast_arrays(positions, velocities)
-        state0 = np.concatenate([positions, velocities], axis=1)
+        second such plane (result.landing2), e.g. the entrance plane for reflected particles.
+        shape: a LaunchShape, or "auto" for the one autotune.tune picked for this field and
+        particle count (DEFAULT_SHAPE before any tuning)."""
+        state0 = batch_state(positions, velocities)
         qm = np.asarray(charge, dtype=np.float64) / np.asarray(mass, dtype=np.float64)
         if qm.size > 1:
             qm = np.broadcast_to(qm, (len(state0),))
+        if isinstance(shape, str):
+            from . import autotune
+            shape = autotune.cached_shape(self.program(), len(state0), atol)
         return run_trace(self.program(), state0, qm, self.bounds, atol, plane=plane, max_attempts=max_attempts, waves=waves,
-                         plane2=plane2)
+                         plane2=plane2, shape=shape)
+
+
+def batch_state(positions, velocities):
+    """Start states (n, 6) from positions and velocities: numpy arrays (broadcast against each
+    other), or device tensors (n, 3) that stay on the device."""
+    import torch
+    if isinstance(positions, torch.Tensor) or isinstance(velocities, torch.Tensor):
+        p = torch.as_tensor(positions, dtype=torch.float64, device="cuda")
+        v = torch.as_tensor(velocities, dtype=torch.float64, device="cuda")
+        p, v = torch.broadcast_tensors(torch.atleast_2d(p), torch.atleast_2d(v))
+        return torch.cat([p, v], dim=1).contiguous()
+    positions = np.atleast_2d(np.asarray(positions, dtype=np.float64))
+    velocities = np.atleast_2d(np.asarray(velocities, dtype=np.float64))
+    positions, velocities = np.broadcast_arrays(positions, velocities)
+    return np.concatenate([positions, velocities], axis=1)
 
 
 def plane_intersection(positions, p0, normal):
