@@ -290,6 +290,18 @@ int tb_trace(const tb_trace_params_t *params, int waves, void *stream);
    tb_trace_order_workspace_bytes(n) bytes. */
 size_t tb_trace_order_workspace_bytes(int n);
 int tb_trace_order(const double *state0, int n, int *order, void *workspace, void *stream);
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code
+   under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): tb_trace
+   with its work counter, and for ordered != 0 the queue order of tb_trace_order and its
+   sort workspace, in a static device array of tb_trace_noalloc_scratch_bytes() bytes (about
+   16.3 MiB, held by each GPU the library runs on until the process ends) instead of caller
+   memory; params->counter is ignored, and params->order too when ordered != 0.  At most
+   tb_trace_noalloc_max_particles() = 2^21 particles (else -1 with an error).  Allocates
+   nothing; the caller still passes every input and output.  Calls that overlap on one GPU
+   share the array and race. */
+int tb_trace_noalloc(const tb_trace_params_t *params, int waves, int ordered, void *stream);
+int tb_trace_noalloc_max_particles(void);
+size_t tb_trace_noalloc_scratch_bytes(void);
 /* Stage-synchronous tracer for a few particles (n <= 65535) in fields with ring-source
    sums: each RK stage spreads every particle's source sum over nchunk workgroups and adds
    the partials in a fixed order; the RK arithmetic per particle is that of tb_trace.

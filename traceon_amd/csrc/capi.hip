@@ -635,20 +635,39 @@ int tb_jacobian_3d(const double* tri, int N, double* jac, double* pos, void* str
 int tb_charge_radial(const double* lines, const double* q, int N, double* out, void* stream) {
   TB_CALL(tb::launch_charge(lines, q, N, out, S(stream)), "charge_radial");
 }
+// workgroups of a persistent tracer launch: `waves`, or by default as many as the CUs hold
+// (4 per CU per wave of occupancy for the axial-only kernel, 8 per CU otherwise); -1 on error
+static int trace_waves(const tb_trace_params_t* params, int waves) {
+  if (waves > 0) return waves;
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+    set_err("device attributes");
+    return -1;
+  }
+  const int occ = params->occupancy >= 1 && params->occupancy <= 4 ? params->occupancy : 2;
+  return cus * ((params->field.flags & tb::PROGRAM_NO_SOURCES) ? 4 * occ : 8);
+}
 int tb_trace(const tb_trace_params_t* params, int waves, void* stream) {
   if (!params) { set_err("tb_trace: null params"); return -1; }
   if (params->traj && params->traj_cap < 2) { set_err("tb_trace: traj_cap must be >= 2"); return -1; }
-  int nw = waves;
-  if (nw <= 0) {
-    int dev = 0;
-    hipGetDevice(&dev);
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) { set_err("device properties"); return -1; }
-    const int occ = params->occupancy >= 1 && params->occupancy <= 4 ? params->occupancy : 2;
-    nw = prop.multiProcessorCount * ((params->field.flags & tb::PROGRAM_NO_SOURCES) ? 4 * occ : 8);
-  }
+  const int nw = trace_waves(params, waves);
+  if (nw < 0) return -1;
   TB_CALL(tb::launch_trace(*reinterpret_cast<const tb::TraceParams*>(params), nw, S(stream)), "trace");
 }
+int tb_trace_noalloc(const tb_trace_params_t* params, int waves, int ordered, void* stream) {
+  if (!params) { set_err("tb_trace_noalloc: null params"); return -1; }
+  if (params->traj && params->traj_cap < 2) { set_err("tb_trace_noalloc: traj_cap must be >= 2"); return -1; }
+  if (params->n > tb::TRACE_STATIC_MAX_PARTICLES) {
+    set_err("tb_trace_noalloc: " + std::to_string(params->n) + " particles exceed the static maximum of " +
+            std::to_string(tb::TRACE_STATIC_MAX_PARTICLES));
+    return -1;
+  }
+  const int nw = trace_waves(params, waves);
+  if (nw < 0) return -1;
+  TB_CALL(tb::trace_noalloc(*reinterpret_cast<const tb::TraceParams*>(params), nw, ordered, S(stream)), "trace_noalloc");
+}
+int tb_trace_noalloc_max_particles(void) { return tb::TRACE_STATIC_MAX_PARTICLES; }
+size_t tb_trace_noalloc_scratch_bytes(void) { return tb::trace_static_scratch_bytes(); }
 size_t tb_trace_order_workspace_bytes(int n) { return tb::trace_order_workspace_bytes(n); }
 int tb_trace_order(const double* state0, int n, int* order, void* workspace, void* stream) {
   if (n > 0 && (!state0 || !order || !workspace)) { set_err("tb_trace_order: null pointer"); return -1; }

@@ -16,6 +16,7 @@
 // backwards; each crossing writes its interpolated state to the particle's landing row
 // (NaN from the particle's start), so the row ends with the last crossing.  An optional
 // second plane is tracked the same way.
+#include <mutex>
 #include "radial_math.h"
 #include "trace_kernels.h"
 #include "mlp_kernels.h"
@@ -963,6 +964,63 @@ hipError_t launch_eval_program(const FieldProgram& F, const double* pts, int M, 
   if (F.flags & PROGRAM_HAS_MLP) hipLaunchKernelGGL(k_eval_program<true>, dim3((M + 63) / 64), dim3(64), 0, st, F, pts, M, E, H);
   else hipLaunchKernelGGL(k_eval_program<false>, dim3((M + 63) / 64), dim3(64), 0, st, F, pts, M, E, H);
   return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------
+// Tracing without allocation: the work counter, the queue order and the order's sort
+// workspace live in one static device array (each GPU holds its own copy until the process
+// ends).  Host code reaches it only through the address hipGetSymbolAddress returns on that
+// device; calls that overlap on one GPU share the array and race.
+// ---------------------------------------------------------------------------------
+constexpr size_t STATIC_ORDER_OFFSET = 256;
+constexpr size_t STATIC_WS_OFFSET = STATIC_ORDER_OFFSET + (((size_t)TRACE_STATIC_MAX_PARTICLES * sizeof(int) + 255) & ~(size_t)255);
+constexpr size_t STATIC_SCRATCH_BYTES = STATIC_WS_OFFSET + ((4 * sizeof(unsigned long long) + sizeof(int) * (size_t)ORDER_BINS +
+                                                              sizeof(int) * (size_t)TRACE_STATIC_MAX_PARTICLES + 255) & ~(size_t)255);
+static __device__ __align__(256) unsigned char scratch[STATIC_SCRATCH_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[MAX_DEVICES] = {};
+
+// the device address of this GPU's scratch array, looked up once per device
+static hipError_t scratch_address(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
+
+__global__ void k_zero_counter(int* c) {
+  if (threadIdx.x == 0) *c = 0;
+}
+
+size_t trace_static_scratch_bytes() { return STATIC_SCRATCH_BYTES; }
+
+hipError_t trace_noalloc(const TraceParams& P, int waves, int ordered, hipStream_t st) {
+  if (P.n > TRACE_STATIC_MAX_PARTICLES) return hipErrorInvalidValue;
+  void* base = nullptr;
+  hipError_t e = scratch_address(&base);
+  if (e != hipSuccess) return e;
+  if (P.n <= 0) return hipSuccess;
+  TraceParams Q = P;
+  Q.counter = reinterpret_cast<int*>(base);
+  hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, st, Q.counter);
+  if (ordered) {
+    int* order = reinterpret_cast<int*>(static_cast<char*>(base) + STATIC_ORDER_OFFSET);
+    e = trace_order(P.state0, P.n, order, static_cast<char*>(base) + STATIC_WS_OFFSET, st);
+    if (e != hipSuccess) return e;
+    Q.order = order;
+  }
+  return launch_trace(Q, waves, st);
 }
 
 }  // namespace tb

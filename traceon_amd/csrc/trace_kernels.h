@@ -105,6 +105,13 @@ hipError_t launch_trace(const TraceParams& P, int waves, hipStream_t st);
 // ws: trace_order_workspace_bytes(n).
 size_t trace_order_workspace_bytes(int n);
 hipError_t trace_order(const double* state0, int n, int* order, void* ws, hipStream_t st);
+// launch_trace with the work counter (and, ordered != 0, the queue order and its sort
+// workspace) in a static device array instead of caller memory: no allocation.  At most
+// TRACE_STATIC_MAX_PARTICLES particles (else hipErrorInvalidValue).  Overlapping calls on
+// one GPU share the array and race.
+constexpr int TRACE_STATIC_MAX_PARTICLES = 1 << 21;
+size_t trace_static_scratch_bytes();
+hipError_t trace_noalloc(const TraceParams& P, int waves, int ordered, hipStream_t st);
 // Stage-synchronous tracer (few particles, n <= 65535): every RK stage spreads each
 // particle's source sums over nchunk workgroups.  ws: split_workspace_bytes(n, nchunk).
 int split_chunks(int n_particles, long long n_sources);

@@ -252,6 +252,36 @@ def run_trace(program: DeviceFieldProgram, state0, qm, bounds, atol=1e-8, plane=
     the persistent tracer (None: DEFAULT_SHAPE); waves > 0 overrides shape.waves."""
     import torch
     from . import device
+    P, out = trace_params(program, state0, qm, bounds, atol, plane, traj_cap, t0, max_attempts, plane2)
+    S0, n, dev = out._keep[0], P.n, out.final.device
+    L = backend.backend_lib
+    order = None
+    if split is None:
+        split = program.has_sources and n <= SPLIT_MAX_PARTICLES and not program.has_mlp
+    if split and program.has_mlp:
+        raise ValueError("fields with a learned correction are traced by the persistent tracer (split=False)")
+    if split:
+        nchunk = L.tb_trace_split_chunks(n, program.n_sources)
+        ws = torch.empty(L.tb_trace_split_workspace_bytes(n, nchunk), dtype=torch.uint8, device=dev)
+        backend.check(L.tb_trace_split(C.byref(P), nchunk, ws.data_ptr(), device.stream_ptr()), "trace_split")
+    else:
+        shape = shape or DEFAULT_SHAPE
+        if shape.ordered and n >= ORDER_MIN_PARTICLES:
+            order = queue_order(S0)
+            P.order = order.data_ptr()
+        P.batch = int(shape.batch)
+        P.occupancy = int(shape.occupancy)
+        backend.check(L.tb_trace(C.byref(P), int(waves or shape.waves), device.stream_ptr()), "trace")
+    out._keep = out._keep + (order,)
+    return out
+
+
+def trace_params(program: DeviceFieldProgram, state0, qm, bounds, atol=1e-8, plane=None, traj_cap=0, t0=None,
+                 max_attempts=1 << 62, plane2=None):
+    """The tb_trace_params_t of a trace (a zeroed work counter, no queue order, launch shape
+    0) and the BatchResult whose device tensors it points to (result._keep holds the inputs)."""
+    import torch
+    from . import device
     S0 = state0 if isinstance(state0, torch.Tensor) else device.to_dev(np.atleast_2d(state0))
     n = S0.shape[0]
     qm_arr = np.atleast_1d(np.asarray(qm, dtype=np.float64))
@@ -308,28 +338,10 @@ def run_trace(program: DeviceFieldProgram, state0, qm, bounds, atol=1e-8, plane=
         times = torch.empty((n, traj_cap), dtype=torch.float64, device=dev)
         tlen = torch.empty(n, dtype=torch.int64, device=dev)
         P.traj, P.times, P.traj_cap, P.traj_len = traj.data_ptr(), times.data_ptr(), int(traj_cap), tlen.data_ptr()
-    if split is None:
-        split = program.has_sources and n <= SPLIT_MAX_PARTICLES and not program.has_mlp
-    if split and program.has_mlp:
-        raise ValueError("fields with a learned correction are traced by the persistent tracer (split=False)")
-    L = backend.backend_lib
-    order = None
-    if split:
-        nchunk = L.tb_trace_split_chunks(n, program.n_sources)
-        ws = torch.empty(L.tb_trace_split_workspace_bytes(n, nchunk), dtype=torch.uint8, device=dev)
-        backend.check(L.tb_trace_split(C.byref(P), nchunk, ws.data_ptr(), device.stream_ptr()), "trace_split")
-    else:
-        shape = shape or DEFAULT_SHAPE
-        if shape.ordered and n >= ORDER_MIN_PARTICLES:
-            order = queue_order(S0)
-            P.order = order.data_ptr()
-        P.batch = int(shape.batch)
-        P.occupancy = int(shape.occupancy)
-        backend.check(L.tb_trace(C.byref(P), int(waves or shape.waves), device.stream_ptr()), "trace")
     out = BatchResult(final, landing, status, nacc, natt, tfin, landing2)
     out.traj, out.times, out.traj_len = traj, times, tlen
-    out._keep = (S0, QM, T0, counter, order)
-    return out
+    out._keep = (S0, QM, T0, counter)
+    return P, out
 
 
 class Tracer:
