@@ -13,6 +13,7 @@ the reference's semantics; each runs on the GPU:
 """
 from __future__ import annotations
 
+import copy
 import time
 
 import numpy as np
@@ -219,8 +220,18 @@ class MagnetostaticSolverRadial(SolverRadial):
                               current_point_charges=self.preexisting_field.current_point_charges)
 
 
+def _excitation_to_higher_order(excitation):
+    """solver.py:356-362: a mesh of two-point lines is raised to cubic elements before solving."""
+    mesh = excitation.mesh
+    if mesh.is_2d() and len(mesh.lines) and not mesh.is_higher_order():
+        excitation = copy.copy(excitation)
+        excitation.mesh = mesh.to_higher_order()
+    return excitation
+
+
 def solve_direct_superposition(excitation, precision: str = "fp64"):
     """solver.py:364-400: one factorisation, one right-hand side per electrode."""
+    excitation = _excitation_to_higher_order(excitation)
     elec, mag = excitation._split_for_superposition()
     out = {}
     if elec:
@@ -236,6 +247,7 @@ def solve_direct_superposition(excitation, precision: str = "fp64"):
 
 def solve_direct(excitation, precision: str = "fp64") -> FieldRadialBEM:
     """solver.py:402-431."""
+    excitation = _excitation_to_higher_order(excitation)
     mag, elec = excitation.is_magnetostatic(), excitation.is_electrostatic()
     if mag and elec:
         e = ElectrostaticSolverRadial(excitation, precision).solve_matrix()[0]
