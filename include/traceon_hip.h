@@ -302,6 +302,14 @@ int tb_trace_order(const double *state0, int n, int *order, void *workspace, voi
 int tb_trace_noalloc(const tb_trace_params_t *params, int waves, int ordered, void *stream);
 int tb_trace_noalloc_max_particles(void);
 size_t tb_trace_noalloc_scratch_bytes(void);
+/* Beam focus, replaces focus_position (focus.py:9-43) on device-resident final states: the
+   least-squares point (out[0..2] = x, y, z) of the n >= 2 rays (x, y, z) + t (vx/vz, vy/vz, 1)
+   whose states (x, y, z, vx, vy, vz) lie `stride` doubles apart (6 for a (n, 6) array), in the
+   closed form of the reference's numpy.linalg.lstsq system (centred sums, fixed summation
+   order).  Rays that are all parallel have no focus: the result is then not finite.
+   Workspace of tb_focus_workspace_bytes() bytes. */
+size_t tb_focus_workspace_bytes(void);
+int tb_focus_position(const double *states, int n, long long stride, double *out, void *workspace, void *stream);
 /* Stage-synchronous tracer for a few particles (n <= 65535) in fields with ring-source
    sums: each RK stage spreads every particle's source sum over nchunk workgroups and adds
    the partials in a fixed order; the RK arithmetic per particle is that of tb_trace.

@@ -426,3 +426,19 @@ def reference() -> OracleLib | None:
     if "r" not in _cache:
         _cache["r"] = OracleLib(REF_SO) if os.path.isfile(REF_SO) else None
     return _cache["r"]
+
+
+def focus_position(positions):
+    """focus.py:9-43 restated: numpy.linalg.lstsq of the 2N x 3 system in (z, x, y) built from
+    the last state of every trajectory."""
+    last = np.array([p[-1] for p in positions], dtype=np.float64)
+    ax, ay = last[:, 3] / last[:, 5], last[:, 4] / last[:, 5]
+    x, y, z = last[:, 0], last[:, 1], last[:, 2]
+    n = len(last)
+    A = np.zeros((2 * n, 3))
+    A[:n, 0], A[n:, 0] = -ax, -ay
+    A[:n, 1] = 1.0
+    A[n:, 2] = 1.0
+    rhs = np.concatenate([x - z * ax, y - z * ay])
+    zf, xf, yf = np.linalg.lstsq(A, rhs, rcond=None)[0]
+    return np.array([xf, yf, zf])

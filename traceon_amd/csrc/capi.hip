@@ -666,6 +666,11 @@ int tb_trace_noalloc(const tb_trace_params_t* params, int waves, int ordered, vo
   if (nw < 0) return -1;
   TB_CALL(tb::trace_noalloc(*reinterpret_cast<const tb::TraceParams*>(params), nw, ordered, S(stream)), "trace_noalloc");
 }
+size_t tb_focus_workspace_bytes(void) { return tb::focus_workspace_bytes(); }
+int tb_focus_position(const double* states, int n, long long stride, double* out, void* workspace, void* stream) {
+  if (n < 2 || stride < 6 || !states || !out || !workspace) { set_err("tb_focus_position: needs n >= 2 rays, stride >= 6"); return -1; }
+  TB_CALL(tb::focus_position(states, n, stride, out, workspace, S(stream)), "focus_position");
+}
 int tb_trace_noalloc_max_particles(void) { return tb::TRACE_STATIC_MAX_PARTICLES; }
 size_t tb_trace_noalloc_scratch_bytes(void) { return tb::trace_static_scratch_bytes(); }
 size_t tb_trace_order_workspace_bytes(int n) { return tb::trace_order_workspace_bytes(n); }

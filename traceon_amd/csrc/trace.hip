@@ -382,6 +382,9 @@ __device__ __forceinline__ bool inside_trace_bounds(const double* b, const doubl
   return b[0] <= y[0] && y[0] <= b[1] && b[2] <= y[1] && y[1] <= b[3] && b[4] <= y[2] && y[2] <= b[5];
 }
 
+template <int N>
+struct StageIndex { static constexpr int value = N; };
+
 template <bool MLP, bool SOURCES>
 __device__ __forceinline__ void trace_body(const TraceParams& P) {
   const int lane = threadIdx.x & 63;
@@ -461,8 +464,9 @@ __device__ __forceinline__ void trace_body(const TraceParams& P) {
     if (__ballot(pid >= 0) == 0ull) break;
     const bool active = pid >= 0;
     double k[6][6];
-#pragma unroll
-    for (int s = 0; s < 6; s++) {
+    // the six stages, each inlined with its index a constant (k stays in registers)
+    auto stage = [&](auto stage_index) __attribute__((always_inline)) {
+      constexpr int s = decltype(stage_index)::value;
       double ys[6];
 #pragma unroll
       for (int c = 0; c < 6; c++) {
@@ -492,7 +496,9 @@ __device__ __forceinline__ void trace_body(const TraceParams& P) {
       k[s][3] = h * qm * (E[0] + MU0 * vxh0);
       k[s][4] = h * qm * (E[1] + MU0 * vxh1);
       k[s][5] = h * qm * (E[2] + MU0 * vxh2);
-    }
+    };
+    stage(StageIndex<0>{}); stage(StageIndex<1>{}); stage(StageIndex<2>{});
+    stage(StageIndex<3>{}); stage(StageIndex<4>{}); stage(StageIndex<5>{});
     double ep = 0.0, ev = 0.0;
 #pragma unroll
     for (int c = 0; c < 6; c++) {
@@ -963,6 +969,89 @@ hipError_t launch_eval_program(const FieldProgram& F, const double* pts, int M, 
   if (M <= 0) return hipSuccess;
   if (F.flags & PROGRAM_HAS_MLP) hipLaunchKernelGGL(k_eval_program<true>, dim3((M + 63) / 64), dim3(64), 0, st, F, pts, M, E, H);
   else hipLaunchKernelGGL(k_eval_program<false>, dim3((M + 63) / 64), dim3(64), 0, st, F, pts, M, E, H);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------
+// Beam focus (focus.py:9-43): the least-squares point closest to the final rays
+// x_i + a_i (Z - z_i), y_i + b_i (Z - z_i) with a = vx / vz, b = vy / vz.  numpy.lstsq of the
+// reference's 2N x 3 system has the closed form
+//   Z = -(S_ac + S_bd) / (S_aa + S_bb),  X = mean(c) + mean(a) Z,  Y = mean(d) + mean(b) Z,
+// with c = x - z a, d = y - z b and S the centred sums of products; the centred form keeps the
+// conditioning of the rays themselves.  Two passes over the states (means, then centred
+// sums), each into FOCUS_BLOCKS partials added in a fixed order: the result does not depend
+// on scheduling.  ws: FOCUS_BLOCKS * 4 + 8 doubles.
+// ---------------------------------------------------------------------------------
+constexpr int FOCUS_BLOCKS = 256, FOCUS_THREADS = 256;
+static_assert(FOCUS_BLOCKS == FOCUS_THREADS, "k_focus_reduce takes one partial per thread");
+
+__device__ __forceinline__ void focus_terms(const double* s, double& a, double& b, double& c, double& d) {
+  a = s[3] / s[5];
+  b = s[4] / s[5];
+  c = s[0] - s[2] * a;
+  d = s[1] - s[2] * b;
+}
+
+// four sums of a workgroup in a fixed tree order into out[0..3]
+__device__ __forceinline__ void block_sum4(double v[4], double* out) {
+  __shared__ double red[4][FOCUS_THREADS];
+  const int t = threadIdx.x;
+  for (int k = 0; k < 4; k++) red[k][t] = v[k];
+  __syncthreads();
+  for (int o = FOCUS_THREADS / 2; o > 0; o >>= 1) {
+    if (t < o)
+      for (int k = 0; k < 4; k++) red[k][t] += red[k][t + o];
+    __syncthreads();
+  }
+  if (t == 0)
+    for (int k = 0; k < 4; k++) out[k] = red[k][0];
+}
+
+// pass 0: sums of a, b, c, d; pass 1: S_aa + S_bb, S_ac + S_bd (means in m[0..3])
+__global__ __launch_bounds__(FOCUS_THREADS) void k_focus_partial(const double* __restrict__ st, int n, long long stride, int pass,
+                                                                 const double* __restrict__ m, double* __restrict__ part) {
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int i = blockIdx.x * FOCUS_THREADS + threadIdx.x; i < n; i += FOCUS_BLOCKS * FOCUS_THREADS) {
+    double a, b, c, d;
+    focus_terms(st + (size_t)i * stride, a, b, c, d);
+    if (pass == 0) {
+      v[0] += a; v[1] += b; v[2] += c; v[3] += d;
+    } else {
+      const double da = a - m[0], db = b - m[1], dc = c - m[2], dd = d - m[3];
+      v[0] += da * da + db * db;
+      v[1] += da * dc + db * dd;
+    }
+  }
+  block_sum4(v, part + 4 * blockIdx.x);
+}
+
+// adds the partials in block order; pass 0 leaves the means in m, pass 1 writes (X, Y, Z)
+__global__ __launch_bounds__(FOCUS_THREADS) void k_focus_reduce(int n, int pass, const double* __restrict__ part, double* m,
+                                                                double* __restrict__ out) {
+  double v[4] = {part[4 * threadIdx.x], part[4 * threadIdx.x + 1], part[4 * threadIdx.x + 2], part[4 * threadIdx.x + 3]};
+  __shared__ double tot[4];
+  block_sum4(v, tot);
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  if (pass == 0) {
+    for (int k = 0; k < 4; k++) m[k] = tot[k] / n;
+  } else {
+    const double Z = -tot[1] / tot[0];
+    out[0] = m[2] + m[0] * Z;
+    out[1] = m[3] + m[1] * Z;
+    out[2] = Z;
+  }
+}
+
+size_t focus_workspace_bytes() { return sizeof(double) * (FOCUS_BLOCKS * 4 + 8); }
+
+hipError_t focus_position(const double* states, int n, long long stride, double* out, void* ws, hipStream_t st) {
+  double* part = static_cast<double*>(ws);
+  double* m = part + FOCUS_BLOCKS * 4;
+  for (int pass = 0; pass < 2; pass++) {
+    hipLaunchKernelGGL(k_focus_partial, dim3(FOCUS_BLOCKS), dim3(FOCUS_THREADS), 0, st, states, n, stride, pass, m, part);
+    hipLaunchKernelGGL(k_focus_reduce, dim3(1), dim3(FOCUS_THREADS), 0, st, n, pass, part, m, out);
+  }
   return hipGetLastError();
 }
 

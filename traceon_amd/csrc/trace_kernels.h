@@ -112,6 +112,10 @@ hipError_t trace_order(const double* state0, int n, int* order, void* ws, hipStr
 constexpr int TRACE_STATIC_MAX_PARTICLES = 1 << 21;
 size_t trace_static_scratch_bytes();
 hipError_t trace_noalloc(const TraceParams& P, int waves, int ordered, hipStream_t st);
+// Least-squares focus of n rays given by states (x, y, z, vx, vy, vz) `stride` doubles apart
+// (focus.py:9-43): out = (X, Y, Z).  ws: focus_workspace_bytes().
+size_t focus_workspace_bytes();
+hipError_t focus_position(const double* states, int n, long long stride, double* out, void* ws, hipStream_t st);
 // Stage-synchronous tracer (few particles, n <= 65535): every RK stage spreads each
 // particle's source sums over nchunk workgroups.  ws: split_workspace_bytes(n, nchunk).
 int split_chunks(int n_particles, long long n_sources);
