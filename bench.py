@@ -147,11 +147,12 @@ def traces(tracer, P, V, exit_plane, back_plane=None, timer=False):
     -1 none), accepted and attempted steps; optionally timed with HIP events."""
     import torch
     torch.cuda.synchronize()
+    shape = TRACE_SHAPE or "auto"
     if timer:
         with Timer() as t:
-            r = tracer.trace_batch(P, V, plane=exit_plane, plane2=back_plane)
+            r = tracer.trace_batch(P, V, plane=exit_plane, plane2=back_plane, shape=shape)
     else:
-        r, t = tracer.trace_batch(P, V, plane=exit_plane, plane2=back_plane), None
+        r, t = tracer.trace_batch(P, V, plane=exit_plane, plane2=back_plane, shape=shape), None
     land = r.landing.cpu().numpy()
     plane = np.where(np.isfinite(land[:, 0]), 0, -1)
     if back_plane is not None:
@@ -187,6 +188,24 @@ def shape_str(s):
     return f"{'ordered' if s.ordered else 'unordered'},batch={s.batch},occ={s.occupancy}"
 
 
+def parse_shape(text):
+    from traceon_amd.tracing import LaunchShape
+    parts = text.split(",")
+    kv = dict(p.split("=") for p in parts[1:])
+    return LaunchShape(ordered=parts[0] == "ordered", batch=int(kv.get("batch", 0)), occupancy=int(kv.get("occ", 2)))
+
+
+# a fixed launch shape for the tracing legs (--trace-shape), else each leg autotunes
+TRACE_SHAPE = None
+
+
+def tuned(tracer, P, V, **planes):
+    from traceon_amd import autotune
+    if TRACE_SHAPE is not None:
+        return TRACE_SHAPE, {}
+    return autotune.tune(tracer, P, V, **planes)
+
+
 def roofline_trace(steps_attempted, ms, series_per_eval):
     """Tracer roofline: algorithmic FLOP per attempted RKF45 step = C_rk + 6 evaluations of
     series_per_eval paraxial series (C_axeval each), against the FP64 vector peak."""
@@ -202,7 +221,7 @@ def tracing_leg(n_particles, n_sample, n_axial=7000):
     (lognormal 1 keV, 0.2 mm disc, 5 mrad) through FieldRadialAxial; transmitted electrons land
     on z = 9.5 mm, reflected ones back through z = -4.5 mm, both recorded in the same launch."""
     import torch
-    from traceon_amd import autotune, configs, device, solver
+    from traceon_amd import configs, device, solver
     from traceon_amd.excitation import Excitation
     from traceon_amd.field import FieldRadialAxial
     mm = 1e-3
@@ -221,7 +240,7 @@ def tracing_leg(n_particles, n_sample, n_axial=7000):
     tr_ax = axial.get_tracer(bounds)
     Pd, Vd = device.to_dev(P), device.to_dev(V)    # the beam is resident in HBM before timing
     traces(tr_ax, Pd[:4096], Vd[:4096], exit_plane, back_plane)   # warm-up
-    shape, tuning = autotune.tune(tr_ax, Pd, Vd, plane=exit_plane, plane2=back_plane)
+    shape, tuning = tuned(tr_ax, Pd, Vd, plane=exit_plane, plane2=back_plane)
     res = traces(tr_ax, Pd, Vd, exit_plane, back_plane, timer=True)
     nacc = res["n_accepted"]
     out = {"n": n_particles, "ms": res["ms"], "electrons_per_s": n_particles / (res["ms"] * 1e-3),
@@ -264,7 +283,7 @@ def c3_leg(n_particles, n_sample, n_axial=7000):
     gap 3 mm) and a three-tube einzel lens, N = 12,000; electrostatic and magnetostatic solves,
     10 keV electrons traced through both axial series with the Lorentz force."""
     import torch
-    from traceon_amd import autotune, configs, device, solver
+    from traceon_amd import configs, device, solver
     from traceon_amd.field import FieldRadialAxial
     mm = 1e-3
     exc = magnetic_problem()
@@ -281,7 +300,7 @@ def c3_leg(n_particles, n_sample, n_axial=7000):
     tr_ax = axial.get_tracer(bounds)
     Pd, Vd = device.to_dev(P), device.to_dev(V)
     traces(tr_ax, Pd[:4096], Vd[:4096], plane)
-    shape, tuning = autotune.tune(tr_ax, Pd, Vd, plane=plane)
+    shape, tuning = tuned(tr_ax, Pd, Vd, plane=plane)
     res = traces(tr_ax, Pd, Vd, plane, timer=True)
     out = {"N_electrostatic": len(field.electrostatic_point_charges), "N_magnetostatic": len(field.magnetostatic_point_charges),
            "N_coil_rings": len(field.current_point_charges), "solve_s": t_solve, "axial_build_s": t_axial, "n": n_particles,
@@ -310,7 +329,7 @@ def c4_leg(ctx, n_samples=1 << 22, steps=8000):
     off-axis beam (start radius 0.3-0.5 mm) against the FP64 oracle's direct BEM traces, beside
     the axial series alone."""
     import torch
-    from traceon_amd import autotune, configs
+    from traceon_amd import configs
     from traceon_amd.surrogate import FieldRadialAxialLearned, FieldSurrogate
     mm = 1e-3
     field, axial, P, V = ctx["field"], ctx["axial"], ctx["P"], ctx["V"]
@@ -326,7 +345,7 @@ def c4_leg(ctx, n_samples=1 << 22, steps=8000):
     learned = FieldRadialAxialLearned(sur)
     tr = learned.get_tracer(bounds)
     traces(tr, P[:4096], V[:4096], plane)
-    shape, tuning = autotune.tune(tr, P, V, plane=plane)
+    shape, tuning = tuned(tr, P, V, plane=plane)
     res = traces(tr, P, V, plane, timer=True)
     bands = {k: {"axial_E_rms": v["axial"]["E_rms"], "learned_E_rms": v["learned"]["E_rms"], "axial_H_rms": v["axial"]["H_rms"],
                  "learned_H_rms": v["learned"]["H_rms"]} for k, v in val.items()}
@@ -426,6 +445,8 @@ def main():
     ap.add_argument("--no-c4", action="store_true")
     ap.add_argument("--trace-only", action="store_true",
                     help="run only the C2 and C3 tracing legs (for kernel profiles); prints their JSON")
+    ap.add_argument("--trace-shape", default=None, metavar="SHAPE",
+                    help="fixed tracer launch shape instead of autotuning, e.g. ordered,batch=1,occ=2 (for kernel profiles)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device three times (a known-size kernel for HBM counter scaling)")
     args = ap.parse_args()
@@ -443,6 +464,9 @@ def main():
         torch.cuda.synchronize()
         log(f"bench: copy calibration, 3 x {args.copy_calibration} MiB device-to-device")
         del src, dst
+    global TRACE_SHAPE
+    if args.trace_shape:
+        TRACE_SHAPE = parse_shape(args.trace_shape)
     if args.trace_only:
         c2, _ = tracing_leg(args.particles, args.sample, args.n_axial)
         c3, _ = c3_leg(args.particles, args.sample, args.n_axial)

@@ -512,3 +512,30 @@ def test_launch_shapes_do_not_change_results(small):
                 continue
             for a, b in zip(got, ref):
                 assert np.array_equal(a, b, equal_nan=True)
+
+
+def test_superposition_components_vs_oracle(gpu, oracle):
+    """solve_direct_superposition (solver.py:364-400): every component field's charges and
+    fields against numpy.linalg.solve of the oracle's matrix with that component's right-hand
+    side (its electrode at its voltage, the others at 0 V), bars as test_solve_direct_vs_oracle."""
+    from traceon_amd import configs, solver
+    from traceon_amd.excitation import Excitation
+    mesh, volts, diel = configs.small_geometry(240)
+    exc = Excitation(mesh)
+    exc.add_voltage(a=5.0, b=-2.0)
+    exc.add_dielectric(d=3.0)
+    sup = solver.solve_direct_superposition(exc)
+    assert set(sup) == {"a", "b"}
+    lines, types, values = configs.lines_types_values(mesh, {"a": 5.0, "b": -2.0}, diel)
+    Mo = oracle.matrix(lines, types, values)
+    jac, pos = oracle.fill_jacobian_buffer_radial(lines)
+    pts = np.array([[r, 0.0, z] for r in (0.0, 0.3e-3) for z in np.linspace(-2.0e-3, 2.0e-3, 9)])
+    for name, V in (("a", 5.0), ("b", -2.0)):
+        F = np.zeros(len(lines))
+        F[mesh.physical_to_lines[name]] = V
+        qo = np.linalg.solve(Mo, F)
+        q = sup[name].electrostatic_point_charges.charges
+        Fd = sup[name].electrostatic_field_at_points(pts)
+        Fo = np.array([oracle.field_radial(p, qo, jac, pos) for p in pts])
+        print(name, "charges rel", rel(q, qo), "field rel", rel(Fd, Fo), flush=True)
+        assert rel(q, qo) < 1e-8 and rel(Fd, Fo) < 1e-9
