@@ -11,6 +11,7 @@
 // (B, width).  Training minimises the mean squared error over the batch and the 5 outputs.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <algorithm>
 #include "mlp_kernels.h"
 
 namespace tb {
@@ -22,14 +23,55 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 // C (+)= op(A) op(B), A (M,K) and B (K,N) in BF16 through strides:
 //   A(m,k) = A[m*sam + k*sak],  B(k,n) = B[k*sbk + n*sbn],  C[m*ldc + n] FP32.
 // 64 x 64 tile per workgroup of 4 waves (32 x 32 each = 2 x 2 MFMA 16x16x16), K staged in
-// chunks of 32 through LDS.  gridDim.z splits K; split results are added atomically into
-// C, which the caller has zeroed.  Operand layout of v_mfma_f32_16x16x16bf16_1k: lane l
-// holds A[l%16][4(l/16)+i] and B[4(l/16)+i][l%16], i = 0..3; the result D[4(l/16)+v][l%16].
+// chunks of 32 through LDS.  An operand that is contiguous along k or along its row/column
+// index is staged with one 16-byte load per thread (8 elements) where the group of 8 lies
+// inside the matrix and is 16-byte aligned, else element by element.  gridDim.z splits K;
+// split results are added atomically into C, which the caller has zeroed.  Operand layout of
+// v_mfma_f32_16x16x16bf16_1k: lane l holds A[l%16][4(l/16)+i] and B[4(l/16)+i][l%16],
+// i = 0..3; the result D[4(l/16)+v][l%16].
+// Epilogues (one K split only): EPI_STORE C = acc (or C += acc); EPI_BIAS_SILU
+// C = acc + aux[col] (aux: the layer's bias), outbf = bf16(silu(C)); EPI_DSILU
+// C = acc * silu'(aux[row][col]) (aux: the layer's pre-activations), outbf = bf16(C).
 // ---------------------------------------------------------------------------------
 constexpr int MG_T = 64, MG_K = 32, MG_PAD = 8;
+enum { EPI_STORE = 0, EPI_BIAS_SILU = 1, EPI_DSILU = 2 };
+
+// one 64 x 32 operand tile into S[64][MG_K + MG_PAD] (S[r][k] = X(r0 + r, k0 + k)), X(r, k) =
+// X[r * sr + k * sk], rows below R and k below kend, zero elsewhere
+__device__ __forceinline__ void stage_tile(uint16_t (*S)[MG_K + MG_PAD], const uint16_t* __restrict__ X, long sr, long sk, int r0,
+                                           int R, int k0, int kend, int tid) {
+  const bool vec_ok = ((reinterpret_cast<uintptr_t>(X) & 15) == 0);
+  if (sk == 1 && vec_ok && (sr & 7) == 0) {          // rows contiguous in k: thread = (row, 8 k)
+    const int r = tid >> 2, kq = (tid & 3) * 8, gr = r0 + r, gk = k0 + kq;
+    if (gr < R && gk + 8 <= kend && (gk & 7) == 0) {
+      *reinterpret_cast<uint4*>(&S[r][kq]) = *reinterpret_cast<const uint4*>(X + gr * sr + gk);
+    } else {
+#pragma unroll
+      for (int i = 0; i < 8; i++) S[r][kq + i] = (gr < R && gk + i < kend) ? X[gr * sr + gk + i] : (uint16_t)0;
+    }
+  } else if (sr == 1 && vec_ok && (sk & 7) == 0) {   // columns contiguous in r: thread = (k, 8 rows)
+    const int k = tid >> 3, rq = (tid & 7) * 8, gr = r0 + rq, gk = k0 + k;
+    if (gk < kend && gr + 8 <= R && (gr & 7) == 0) {
+      const uint4 v = *reinterpret_cast<const uint4*>(X + gk * sk + gr);
+      const uint16_t* e = reinterpret_cast<const uint16_t*>(&v);
+#pragma unroll
+      for (int i = 0; i < 8; i++) S[rq + i][k] = e[i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < 8; i++) S[rq + i][k] = (gk < kend && gr + i < R) ? X[gk * sk + gr + i] : (uint16_t)0;
+    }
+  } else {
+    for (int e = tid; e < MG_T * MG_K; e += 256) {
+      const int r = e / MG_K, k = e % MG_K, gr = r0 + r, gk = k0 + k;
+      S[r][k] = (gr < R && gk < kend) ? X[gr * sr + gk * sk] : (uint16_t)0;
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, const uint16_t* __restrict__ A, long sam, long sak,
                                                    const uint16_t* __restrict__ B, long sbk, long sbn, float* __restrict__ C,
-                                                   long ldc, int accumulate, int kchunk) {
+                                                   long ldc, int accumulate, int kchunk, int epi, const float* __restrict__ aux,
+                                                   uint16_t* __restrict__ outbf) {
   __shared__ __align__(16) uint16_t As[MG_T][MG_K + MG_PAD];   // [m][k]
   __shared__ __align__(16) uint16_t Bs[MG_T][MG_K + MG_PAD];   // [n][k]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -43,16 +85,8 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, const ui
 #pragma unroll
     for (int j = 0; j < 2; j++) acc[i][j] = f4v{0.f, 0.f, 0.f, 0.f};
   for (int k0 = kbeg; k0 < kend; k0 += MG_K) {
-    for (int e = tid; e < MG_T * MG_K; e += 256) {
-      int m, k;
-      if (sak == 1) { m = e / MG_K; k = e % MG_K; } else { k = e / MG_T; m = e % MG_T; }
-      const int gm = m0 + m, gk = k0 + k;
-      As[m][k] = (gm < M && gk < kend) ? A[gm * sam + gk * sak] : (uint16_t)0;
-      int n, kk;
-      if (sbk == 1) { n = e / MG_K; kk = e % MG_K; } else { kk = e / MG_T; n = e % MG_T; }
-      const int gn = n0 + n, gk2 = k0 + kk;
-      Bs[n][kk] = (gn < N && gk2 < kend) ? B[gk2 * sbk + gn * sbn] : (uint16_t)0;
-    }
+    stage_tile(As, A, sam, sak, m0, M, k0, kend, tid);
+    stage_tile(Bs, B, sbn, sbk, n0, N, k0, kend, tid);     // B^T rows: n, contiguous along k when sbk == 1
     __syncthreads();
 #pragma unroll
     for (int kk = 0; kk < MG_K; kk += 16) {
@@ -80,10 +114,21 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, const ui
         const int row = m0 + wm + i * 16 + 4 * (lane >> 4) + v;
         const int col = n0 + wn + j * 16 + (lane & 15);
         if (row < M && col < N) {
-          float* c = C + (long)row * ldc + col;
+          const long o = (long)row * ldc + col;
           const float x = acc[i][j][v];
-          if (split) atomicAdd(c, x);
-          else *c = accumulate ? *c + x : x;
+          if (split) {
+            atomicAdd(C + o, x);
+          } else if (epi == EPI_BIAS_SILU) {
+            const float z = x + aux[col];
+            C[o] = z;
+            outbf[o] = f2bf(silu(z));
+          } else if (epi == EPI_DSILU) {
+            const float dz = x * dsilu(aux[o]);
+            C[o] = dz;
+            outbf[o] = f2bf(dz);
+          } else {
+            C[o] = accumulate ? C[o] + x : x;
+          }
         }
       }
 }
@@ -96,15 +141,6 @@ __global__ __launch_bounds__(256) void k_mlp_in(int B, const float* __restrict__
   const int b = (int)(e / MLP_W), j = (int)(e % MLP_W);
   const float* W1 = P + MLP_OFF_W1;
   const float z = fmaf(W1[2 * j], X[2 * b], fmaf(W1[2 * j + 1], X[2 * b + 1], P[MLP_OFF_B1 + j]));
-  Z[e] = z;
-  H[e] = f2bf(silu(z));
-}
-
-// Z += bias (per column), H = silu(Z) in BF16
-__global__ __launch_bounds__(256) void k_bias_silu(int B, float* __restrict__ Z, const float* __restrict__ bias, uint16_t* __restrict__ H) {
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (long)B * MLP_W) return;
-  const float z = Z[e] + bias[e % MLP_W];
   Z[e] = z;
   H[e] = f2bf(silu(z));
 }
@@ -164,29 +200,24 @@ __global__ __launch_bounds__(256) void k_out_bwd(int B, const float* __restrict_
   dZbf[e] = f2bf(dz);
 }
 
-// dZ = dH * silu'(Z), FP32 and BF16
-__global__ __launch_bounds__(256) void k_silu_bwd(int B, const float* __restrict__ dH, const float* __restrict__ Z, float* __restrict__ dZ,
-                                                  uint16_t* __restrict__ dZbf) {
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (long)B * MLP_W) return;
-  const float dz = dH[e] * dsilu(Z[e]);
-  dZ[e] = dz;
-  dZbf[e] = f2bf(dz);
-}
-
-// small weight gradients (one workgroup per column j of the width-128 layer), sums over B:
+// small weight gradients, sums over the batch per column j of the width-128 layer:
 //   mode 0 (layer 5): dW5[o][j] = sum_b dY[b][o] silu(Z4[b][j])  (db5: k_out_bias_grad)
 //   mode 1 (layer 1): dW1[j][c] = sum_b dZ1[b][j] X[b][c],        db1[j] = sum_b dZ1[b][j]
 //   mode 2 (bias):    db[j]     = sum_b dZ[b][j]
-__global__ __launch_bounds__(256) void k_small_grads(int B, int mode, const float* __restrict__ dY, const float* __restrict__ Z4,
-                                                     const float* __restrict__ dZ, const float* __restrict__ X,
-                                                     float* __restrict__ G, int bias_off) {
-  __shared__ float red[256][MLP_OUT + 1];
-  const int j = blockIdx.x, tid = threadIdx.x;
-  float s[MLP_OUT + 1];
+// k_col_partials: workgroup g sums rows [g R, (g + 1) R) (R = COL_ROWS), thread t column
+// t % 128 over the rows of parity t / 128 (coalesced rows), into part[g][j][0..5];
+// k_col_reduce adds the partials of every column in workgroup order (deterministic).
+constexpr int COL_ROWS = 128, COL_Q = MLP_OUT + 1;
+__global__ __launch_bounds__(256) void k_col_partials(int B, int mode, const float* __restrict__ dY, const float* __restrict__ Z4,
+                                                      const float* __restrict__ dZ, const float* __restrict__ X,
+                                                      float* __restrict__ part) {
+  __shared__ float red[128][COL_Q];
+  const int tid = threadIdx.x, j = tid & (MLP_W - 1), par = tid >> 7;
+  const int b0 = blockIdx.x * COL_ROWS, b1 = min(B, b0 + COL_ROWS);
+  float s[COL_Q];
 #pragma unroll
-  for (int q = 0; q <= MLP_OUT; q++) s[q] = 0.f;
-  for (int b = tid; b < B; b += 256) {
+  for (int q = 0; q < COL_Q; q++) s[q] = 0.f;
+  for (int b = b0 + par; b < b1; b += 2) {
     if (mode == 0) {
       const float h = silu(Z4[(long)b * MLP_W + j]);
 #pragma unroll
@@ -200,25 +231,33 @@ __global__ __launch_bounds__(256) void k_small_grads(int B, int mode, const floa
       s[2] += d;
     }
   }
+  if (par == 1)
 #pragma unroll
-  for (int q = 0; q <= MLP_OUT; q++) red[tid][q] = s[q];
+    for (int q = 0; q < COL_Q; q++) red[j][q] = s[q];
   __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if (tid < st)
+  if (par == 0)
 #pragma unroll
-      for (int q = 0; q <= MLP_OUT; q++) red[tid][q] += red[tid + st][q];
-    __syncthreads();
-  }
-  if (tid != 0) return;
+    for (int q = 0; q < COL_Q; q++) part[((long)blockIdx.x * MLP_W + j) * COL_Q + q] = s[q] + red[j][q];
+}
+
+__global__ __launch_bounds__(128) void k_col_reduce(int nblk, int mode, const float* __restrict__ part, float* __restrict__ G,
+                                                    int bias_off) {
+  const int j = threadIdx.x;
+  float s[COL_Q];
+#pragma unroll
+  for (int q = 0; q < COL_Q; q++) s[q] = 0.f;
+  for (int g = 0; g < nblk; g++)
+#pragma unroll
+    for (int q = 0; q < COL_Q; q++) s[q] += part[((long)g * MLP_W + j) * COL_Q + q];
   if (mode == 0) {
 #pragma unroll
-    for (int o = 0; o < MLP_OUT; o++) G[MLP_OFF_W5 + o * MLP_W + j] = red[0][o];
+    for (int o = 0; o < MLP_OUT; o++) G[MLP_OFF_W5 + o * MLP_W + j] = s[o];
   } else if (mode == 1) {
-    G[MLP_OFF_W1 + 2 * j] = red[0][0];
-    G[MLP_OFF_W1 + 2 * j + 1] = red[0][1];
-    G[MLP_OFF_B1 + j] = red[0][2];
+    G[MLP_OFF_W1 + 2 * j] = s[0];
+    G[MLP_OFF_W1 + 2 * j + 1] = s[1];
+    G[MLP_OFF_B1 + j] = s[2];
   } else {
-    G[bias_off + j] = red[0][2];
+    G[bias_off + j] = s[2];
   }
 }
 
@@ -289,62 +328,85 @@ __global__ __launch_bounds__(256) void k_gather(int B, const int* __restrict__ i
 // ---------------------------------------------------------------------------------
 static inline int cdivi(long a, long b) { return (int)((a + b - 1) / b); }
 
-hipError_t gemm_bf16(int M, int N, int K, const uint16_t* A, long sam, long sak, const uint16_t* B, long sbk, long sbn, float* C,
-                     long ldc, int accumulate, int ksplit, hipStream_t st) {
+// epilogue epi with aux / outbf (EPI_STORE: accumulate selects C += ...; a K split adds
+// atomically into C, which must then hold zeros or the sum to add to)
+static hipError_t gemm_bf16_epi(int M, int N, int K, const uint16_t* A, long sam, long sak, const uint16_t* B, long sbk, long sbn,
+                                float* C, long ldc, int accumulate, int ksplit, int epi, const float* aux, uint16_t* outbf,
+                                hipStream_t st) {
   if (M <= 0 || N <= 0) return hipSuccess;
-  if (ksplit < 1) ksplit = 1;
+  if (ksplit < 1 || epi != EPI_STORE) ksplit = 1;
   int kchunk = cdivi(K, ksplit);
   kchunk = ((kchunk + MG_K - 1) / MG_K) * MG_K;
   const int kz = cdivi(K, kchunk);
-  if (kz > 1 && !accumulate) {
-    hipError_t e = hipMemsetAsync(C, 0, sizeof(float) * (size_t)M * ldc, st);
-    if (e != hipSuccess) return e;
-  }
   hipLaunchKernelGGL(k_gemm_bf16, dim3(cdivi(M, MG_T), cdivi(N, MG_T), kz), dim3(256), 0, st, M, N, K, A, sam, sak, B, sbk, sbn, C,
-                     ldc, accumulate, kchunk);
+                     ldc, accumulate, kchunk, epi, aux, outbf);
   return hipGetLastError();
 }
+
+__global__ __launch_bounds__(256) void k_zero_f32(float* __restrict__ x, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) x[i] = 0.f;
+}
+
+hipError_t gemm_bf16(int M, int N, int K, const uint16_t* A, long sam, long sak, const uint16_t* B, long sbk, long sbn, float* C,
+                     long ldc, int accumulate, int ksplit, hipStream_t st) {
+  if (M <= 0 || N <= 0) return hipSuccess;
+  if (ksplit > 1 && !accumulate && cdivi(K, ((cdivi(K, ksplit) + MG_K - 1) / MG_K) * MG_K) > 1) {
+    const long n = (long)M * ldc;
+    hipLaunchKernelGGL(k_zero_f32, dim3((int)std::min<long>(cdivi(n, 256), 1024)), dim3(256), 0, st, C, n);
+  }
+  return gemm_bf16_epi(M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, accumulate, ksplit, EPI_STORE, nullptr, nullptr, st);
+}
+
+static inline int col_blocks(int B) { return cdivi(B, COL_ROWS); }
 
 size_t mlp_workspace_bytes(int B) {
   const size_t f = sizeof(float), h = sizeof(uint16_t), W = MLP_W;
   return f * (size_t)B * W * 4          // Z1..Z4
          + h * (size_t)B * W * 3        // H1..H3 (BF16)
-         + f * (size_t)B * W * 2        // dZ, dH
-         + h * (size_t)B * W            // dZ (BF16)
+         + f * (size_t)B * W            // dZ
+         + h * (size_t)B * W * 2        // dZ (BF16), two buffers: a layer's dZ_prev GEMM reads one, writes the other
          + f * (size_t)B * (2 + 2 * MLP_OUT)   // X batch, T batch, dY
          + f * (size_t)B * MLP_OUT      // Y
+         + f * (size_t)col_blocks(B) * W * COL_Q   // column partial sums
          + 64;
 }
 
 struct MlpWs {
-  float *Z[4], *dZ, *dH, *Xb, *Tb, *dY, *Y, *loss;
-  uint16_t *H[3], *dZbf;
+  float *Z[4], *dZ, *Xb, *Tb, *dY, *Y, *part, *loss;
+  uint16_t *H[3], *dZbf[2];
   MlpWs(void* base, int B) {
     char* p = (char*)base;
     const size_t W = MLP_W;
     for (int l = 0; l < 4; l++) { Z[l] = (float*)p; p += sizeof(float) * B * W; }
     for (int l = 0; l < 3; l++) { H[l] = (uint16_t*)p; p += sizeof(uint16_t) * B * W; }
     dZ = (float*)p; p += sizeof(float) * B * W;
-    dH = (float*)p; p += sizeof(float) * B * W;
-    dZbf = (uint16_t*)p; p += sizeof(uint16_t) * B * W;
+    for (int l = 0; l < 2; l++) { dZbf[l] = (uint16_t*)p; p += sizeof(uint16_t) * B * W; }
     Xb = (float*)p; p += sizeof(float) * B * 2;
     Tb = (float*)p; p += sizeof(float) * B * MLP_OUT;
     dY = (float*)p; p += sizeof(float) * B * MLP_OUT;
     Y = (float*)p; p += sizeof(float) * B * MLP_OUT;
+    part = (float*)p; p += sizeof(float) * (size_t)col_blocks(B) * W * COL_Q;
     loss = (float*)p;
   }
 };
+
+static void col_sums(int B, int mode, const float* dY, const float* Z4, const float* dZ, const float* X, float* part, float* G,
+                     int bias_off, hipStream_t st) {
+  hipLaunchKernelGGL(k_col_partials, dim3(col_blocks(B)), dim3(256), 0, st, B, mode, dY, Z4, dZ, X, part);
+  hipLaunchKernelGGL(k_col_reduce, dim3(1), dim3(MLP_W), 0, st, col_blocks(B), mode, part, G, bias_off);
+}
 
 static hipError_t forward(const float* P, const uint16_t* Wbf, const float* X, int B, MlpWs& w, hipStream_t st) {
   const int g = cdivi((long)B * MLP_W, 256);
   hipLaunchKernelGGL(k_mlp_in, dim3(g), dim3(256), 0, st, B, X, P, w.Z[0], w.H[0]);
   const int boff[3] = {MLP_OFF_B2, MLP_OFF_B3, MLP_OFF_B4};
   for (int l = 0; l < 3; l++) {
-    // Z_{l+2} = H_{l+1} W_{l+2}^T: A = H (B x W, row-major), B(k, n) = W[n][k]
-    hipError_t e = gemm_bf16(B, MLP_W, MLP_W, w.H[l], MLP_W, 1, Wbf + (size_t)l * MLP_W * MLP_W, 1, MLP_W, w.Z[l + 1], MLP_W, 0, 1, st);
+    // Z_{l+2} = H_{l+1} W_{l+2}^T + b: A = H (B x W, row-major), B(k, n) = W[n][k]; bias and
+    // SiLU in the epilogue (layer 4's BF16 activations are not used: they land in dZbf[0])
+    uint16_t* Hout = l < 2 ? w.H[l + 1] : w.dZbf[0];
+    hipError_t e = gemm_bf16_epi(B, MLP_W, MLP_W, w.H[l], MLP_W, 1, Wbf + (size_t)l * MLP_W * MLP_W, 1, MLP_W, w.Z[l + 1], MLP_W, 0,
+                                 1, EPI_BIAS_SILU, P + boff[l], Hout, st);
     if (e != hipSuccess) return e;
-    uint16_t* Hout = l < 2 ? w.H[l + 1] : w.dZbf;   // layer 4's BF16 activations are not used
-    hipLaunchKernelGGL(k_bias_silu, dim3(g), dim3(256), 0, st, B, w.Z[l + 1], P + boff[l], Hout);
   }
   return hipGetLastError();
 }
@@ -365,31 +427,35 @@ hipError_t mlp_grad(const float* P, const uint16_t* Wbf, const float* X, const f
   MlpWs w(ws, B);
   hipError_t e = forward(P, Wbf, X, B, w, st);
   if (e != hipSuccess) return e;
-  e = hipMemsetAsync(loss, 0, sizeof(float), st);
-  if (e != hipSuccess) return e;
+  // G (the hidden weight gradients add up over K splits) and the loss start at zero
+  hipLaunchKernelGGL(k_zero_f32, dim3(cdivi(MLP_NPARAM, 256)), dim3(256), 0, st, G, (long)MLP_NPARAM);
+  hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(64), 0, st, loss, 1l);
   const int g = cdivi((long)B * MLP_W, 256);
   hipLaunchKernelGGL(k_mlp_out, dim3(cdivi(B, 256)), dim3(256), 0, st, B, w.Z[3], P, w.Y, T, w.dY, loss);
   // layer 5
-  hipLaunchKernelGGL(k_small_grads, dim3(MLP_W), dim3(256), 0, st, B, 0, w.dY, w.Z[3], (const float*)nullptr, (const float*)nullptr, G, 0);
+  col_sums(B, 0, w.dY, w.Z[3], nullptr, nullptr, w.part, G, 0, st);
   hipLaunchKernelGGL(k_out_bias_grad, dim3(1), dim3(256), 0, st, B, w.dY, G);
-  hipLaunchKernelGGL(k_out_bwd, dim3(g), dim3(256), 0, st, B, w.dY, w.Z[3], P, w.dZ, w.dZbf);
+  hipLaunchKernelGGL(k_out_bwd, dim3(g), dim3(256), 0, st, B, w.dY, w.Z[3], P, w.dZ, w.dZbf[0]);
   // layers 4, 3, 2
   const int woff[3] = {MLP_OFF_W2, MLP_OFF_W3, MLP_OFF_W4};
   const int boff[3] = {MLP_OFF_B2, MLP_OFF_B3, MLP_OFF_B4};
   const int ksplit = cdivi(B, 512);
+  int cur = 0;
   for (int l = 2; l >= 0; l--) {
     // dW = dZ^T H_prev: A(m, k) = dZ[k][m], B(k, n) = H_prev[k][n]; K = batch, split over workgroups
-    e = gemm_bf16(MLP_W, MLP_W, B, w.dZbf, 1, MLP_W, w.H[l], MLP_W, 1, G + woff[l], MLP_W, 0, ksplit, st);
+    e = gemm_bf16_epi(MLP_W, MLP_W, B, w.dZbf[cur], 1, MLP_W, w.H[l], MLP_W, 1, G + woff[l], MLP_W, 1, ksplit, EPI_STORE, nullptr,
+                      nullptr, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_small_grads, dim3(MLP_W), dim3(256), 0, st, B, 2, (const float*)nullptr, (const float*)nullptr, w.dZ,
-                       (const float*)nullptr, G, boff[l]);
-    // dH_prev = dZ W: A = dZ (row-major), B(k, n) = W[k][n]
-    e = gemm_bf16(B, MLP_W, MLP_W, w.dZbf, MLP_W, 1, Wbf + (size_t)l * MLP_W * MLP_W, MLP_W, 1, w.dH, MLP_W, 0, 1, st);
+    col_sums(B, 2, nullptr, nullptr, w.dZ, nullptr, w.part, G, boff[l], st);
+    // dZ_prev = (dZ W) * silu'(Z_prev): A = dZ (row-major), B(k, n) = W[k][n], the SiLU derivative in
+    // the epilogue; it reads dZbf[cur] and writes dZbf[1 - cur]
+    e = gemm_bf16_epi(B, MLP_W, MLP_W, w.dZbf[cur], MLP_W, 1, Wbf + (size_t)l * MLP_W * MLP_W, MLP_W, 1, w.dZ, MLP_W, 0, 1,
+                      EPI_DSILU, w.Z[l], w.dZbf[1 - cur], st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_silu_bwd, dim3(g), dim3(256), 0, st, B, w.dH, w.Z[l], w.dZ, w.dZbf);
+    cur = 1 - cur;
   }
   // layer 1
-  hipLaunchKernelGGL(k_small_grads, dim3(MLP_W), dim3(256), 0, st, B, 1, (const float*)nullptr, (const float*)nullptr, w.dZ, X, G, 0);
+  col_sums(B, 1, nullptr, nullptr, w.dZ, X, w.part, G, 0, st);
   return hipGetLastError();
 }
 
