@@ -324,6 +324,20 @@ def test_split_tracer_matches_persistent_tracer(small):
     assert np.nanmax(np.abs(la - lb)[:, :3]) < 1e-11
     assert same_steps > 0.99
     assert (a.status == b.status).all()
+    # the split tracer replays one captured HIP graph per batch of attempted steps (with a host
+    # read of the active count between replays); the same kernels launched directly give
+    # bit-identical results, before and after other work on the stream
+    import os
+    os.environ["TRACEON_SPLIT_GRAPH"] = "0"
+    try:
+        c = tracing.run_trace(prog, np.hstack([P, V]), QM, bounds, plane=plane, split=True)
+    finally:
+        del os.environ["TRACEON_SPLIT_GRAPH"]
+    d = tracing.run_trace(prog, np.hstack([P, V]), QM, bounds, plane=plane, split=True)
+    for x in (a, d):
+        assert np.array_equal(x.landing.cpu().numpy(), c.landing.cpu().numpy(), equal_nan=True)
+        assert np.array_equal(x.final.cpu().numpy(), c.final.cpu().numpy())
+        assert (x.n_attempted == c.n_attempted).all()
 
 
 def test_cylindrical_permanent_magnet_axis_field(gpu):

@@ -16,6 +16,7 @@
 // backwards; each crossing writes its interpolated state to the particle's landing row
 // (NaN from the particle's start), so the row ends with the last crossing.  An optional
 // second plane is tracked the same way.
+#include <cstdlib>
 #include <mutex>
 #include "radial_math.h"
 #include "trace_kernels.h"
@@ -779,34 +780,72 @@ int split_chunks(int n_particles, long long n_sources) {
 
 size_t split_workspace_bytes(int n_particles, int nchunk) { return SplitWs::bytes(n_particles, nchunk); }
 
+__global__ void k_split_zero_count(int* c) {
+  if (threadIdx.x == 0) c[0] = 0;
+}
+
+// SPLIT_BATCH attempted steps (13 launches each) and the count of particles still active
+static void split_batch(const TraceParams& P, const SplitWs& W, int nchunk, hipStream_t st) {
+  const dim3 gp((P.n + 255) / 256), bp(256), gf(nchunk, P.n);
+  const int count_blocks = (P.n + 255) / 256 < 64 ? (P.n + 255) / 256 : 64;
+  for (int a = 0; a < SPLIT_BATCH; a++) {
+    for (int s = 0; s < 6; s++) {
+      hipLaunchKernelGGL(k_split_field, gf, dim3(SPLIT_THREADS), 0, st, P, W, nchunk);
+      hipLaunchKernelGGL(k_split_stage, gp, bp, 0, st, P, W, nchunk, s);
+    }
+    hipLaunchKernelGGL(k_split_control, gp, bp, 0, st, P, W);
+  }
+  hipLaunchKernelGGL(k_split_zero_count, dim3(1), dim3(64), 0, st, W.count);   // a kernel: graph-safe
+  hipLaunchKernelGGL(k_split_count, dim3(count_blocks), dim3(256), 0, st, W.active, P.n, W.count);
+}
+
+// One batch captured once into a HIP graph (on a stream of this call, default priority) and
+// replayed per batch on the caller's stream: one launch instead of 106.  TRACEON_SPLIT_GRAPH=0
+// launches the kernels directly instead; the results are the same.  Returns null when
+// capture is off or fails (the batches are then launched directly).
+static hipGraphExec_t capture_split_batch(const TraceParams& P, const SplitWs& W, int nchunk) {
+  const char* env = getenv("TRACEON_SPLIT_GRAPH");
+  if (env && env[0] == '0') return nullptr;
+  hipStream_t cs = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) { (void)hipGetLastError(); return nullptr; }
+  if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+    split_batch(P, W, nchunk, cs);
+    if (hipStreamEndCapture(cs, &graph) == hipSuccess && graph) {
+      if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) exec = nullptr;
+      hipGraphDestroy(graph);
+    }
+  }
+  hipStreamDestroy(cs);
+  (void)hipGetLastError();   // a failed capture leaves no error behind: the direct launches follow
+  return exec;
+}
+
 hipError_t trace_split(const TraceParams& P, int nchunk, void* ws, hipStream_t st) {
   if (P.n <= 0) return hipSuccess;
   if (nchunk < 1 || P.n > 65535) return hipErrorInvalidValue;
   SplitWs W(ws, P.n, nchunk);
-  const dim3 gp((P.n + 255) / 256), bp(256), gf(nchunk, P.n);
-  hipLaunchKernelGGL(k_split_init, gp, bp, 0, st, P, W);
-  const int count_blocks = (P.n + 255) / 256 < 64 ? (P.n + 255) / 256 : 64;
+  hipLaunchKernelGGL(k_split_init, dim3((P.n + 255) / 256), dim3(256), 0, st, P, W);
+  hipGraphExec_t exec = capture_split_batch(P, W, nchunk);
+  hipError_t e = hipSuccess;
   for (long long attempts = 0;; attempts += SPLIT_BATCH) {
     // a trace that needs more attempted steps than this has a non-finite field or step
-    if (attempts > SPLIT_MAX_ATTEMPTS) return hipErrorLaunchTimeOut;
-    for (int a = 0; a < SPLIT_BATCH; a++) {
-      for (int s = 0; s < 6; s++) {
-        hipLaunchKernelGGL(k_split_field, gf, dim3(SPLIT_THREADS), 0, st, P, W, nchunk);
-        hipLaunchKernelGGL(k_split_stage, gp, bp, 0, st, P, W, nchunk, s);
-      }
-      hipLaunchKernelGGL(k_split_control, gp, bp, 0, st, P, W);
+    if (attempts > SPLIT_MAX_ATTEMPTS) { e = hipErrorLaunchTimeOut; break; }
+    if (exec) {
+      e = hipGraphLaunch(exec, st);
+      if (e != hipSuccess) break;
+    } else {
+      split_batch(P, W, nchunk, st);
     }
-    hipError_t e = hipMemsetAsync(W.count, 0, sizeof(int), st);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_split_count, dim3(count_blocks), dim3(256), 0, st, W.active, P.n, W.count);
     int remaining = 0;
     e = hipMemcpyAsync(&remaining, W.count, sizeof(int), hipMemcpyDeviceToHost, st);
-    if (e != hipSuccess) return e;
+    if (e != hipSuccess) break;
     e = hipStreamSynchronize(st);
-    if (e != hipSuccess) return e;
-    if (remaining == 0) break;
+    if (e != hipSuccess || remaining == 0) break;
   }
-  return hipGetLastError();
+  if (exec) hipGraphExecDestroy(exec);
+  return e != hipSuccess ? e : hipGetLastError();
 }
 
 // single field evaluation at a batch of points through a field program
