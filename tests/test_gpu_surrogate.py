@@ -215,6 +215,14 @@ def test_zero_trust_region_is_direct_bem(lens):
           "max landing diff", np.max(np.abs(la[ok] - lb[ok])), flush=True)
     assert np.array_equal(np.isfinite(la[:, 0]), ok)
     assert np.max(np.abs(la[ok, :3] - lb[ok, :3])) < 1e-12
+    # the persistent tracer: every off-axis point deferred and summed by the whole wave
+    QM = np.full(n, -tracing.E_CHARGE / tracing.M_E)
+    c = tracing.run_trace(learned0.get_tracer(bounds).program(), np.hstack([P, V]), QM, bounds, plane=plane, split=False)
+    lc = c.landing.cpu().numpy()
+    print("zero trust, persistent: steps equal", np.array_equal(c.n_attempted.cpu().numpy(), b.n_attempted.cpu().numpy()),
+          "max landing diff", np.max(np.abs(lc[ok] - lb[ok])), flush=True)
+    assert np.array_equal(np.isfinite(lc[:, 0]), ok)
+    assert np.max(np.abs(lc[ok, :3] - lb[ok, :3])) < 1e-12
 
 
 def test_error_check_routes_points_to_bem(lens):
@@ -257,3 +265,65 @@ def test_error_check_routes_points_to_bem(lens):
     frac_bounded = float(np.mean(actual[checked] <= pe[checked] + 1e-15))
     print("fraction of corrected points whose error is within the prediction:", frac_bounded, flush=True)
     assert frac_bounded > 0.9
+
+
+def test_split_learned_tracer_matches_persistent(lens):
+    """Few particles in a learned field go to the stage-synchronous tracer (the lead terms one
+    wave per 64 particles, the BEM fallback's sources spread over the chip); it must take the
+    persistent tracer's steps.  A point that keeps the learned field gets the same value in
+    both; a point sent to the BEM sum gets it in another summation order (1e-16 relative),
+    which can flip an accept/reject decision at the atol boundary for the odd particle."""
+    import time
+    import torch
+    from traceon_amd import tracing
+    from traceon_amd.surrogate import FieldRadialAxialLearned
+    field, axial, sur, _ = lens
+    learned = FieldRadialAxialLearned(sur, error_tolerance=1e-7)
+    rng = np.random.default_rng(21)
+    n = 96
+    r = rng.uniform(0.2 * MM, 0.55 * MM, n)
+    phi = rng.uniform(0, 2 * np.pi, n)
+    P = np.column_stack([r * np.cos(phi), r * np.sin(phi), np.full(n, -4.9 * MM)])
+    V = np.tile(tracing.velocity_vec(1000.0, [0, 0, 1]), (n, 1))
+    bounds = np.array([[-0.95 * MM, 0.95 * MM], [-0.95 * MM, 0.95 * MM], [-5 * MM, 5 * MM]])
+    plane = (np.array([0.0, 0.0, 4.5 * MM]), np.array([0.0, 0.0, 1.0]))
+    prog = learned.get_tracer(bounds).program()
+    QM = np.full(n, -tracing.E_CHARGE / tracing.M_E)
+    out = {}
+    for split in (True, False):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r_ = tracing.run_trace(prog, np.hstack([P, V]), QM, bounds, plane=plane, split=split)
+        torch.cuda.synchronize()
+        out[split] = (r_.landing.cpu().numpy(), r_.n_attempted.cpu().numpy(), time.perf_counter() - t0)
+    (la, sa, ta), (lb, sb, tb_) = out[True], out[False]
+    fate = np.isfinite(la[:, 0])
+    same = sa == sb
+    print(f"learned split vs persistent: {n} particles, landed {fate.sum()}, equal steps {same.mean():.3f}, "
+          f"max landing diff {np.max(np.abs(la[same & fate, :3] - lb[same & fate, :3])):.2e} m; "
+          f"split {ta:.2f} s, persistent {tb_:.2f} s", flush=True)
+    assert fate.mean() > 0.9
+    assert np.array_equal(fate, np.isfinite(lb[:, 0]))
+    assert same.mean() >= 0.95
+    assert np.max(np.abs(la[same & fate, :3] - lb[same & fate, :3])) < 1e-12
+
+
+def test_training_repeats_bit_for_bit(lens):
+    """Two surrogates with the same seed on the same field: the same samples, scales, mini-batches
+    and, with the weight gradients' K splits and the loss added in a fixed order, the same
+    parameters bit for bit after training."""
+    import torch
+    from traceon_amd.surrogate import FieldSurrogate
+    field, axial, _, _ = lens
+    runs = []
+    for _ in range(2):
+        s = FieldSurrogate(field, axial, r_max=0.6 * MM, z_min=-5 * MM, z_max=5 * MM, seed=7)
+        s.generate(1 << 16)
+        hist = s.train(steps=300, batch=8192, log_every=100)
+        torch.cuda.synchronize()
+        runs.append((s.P.clone(), s.zscale.copy(), hist))
+    (p0, z0, h0), (p1, z1, h1) = runs
+    print("training twice: loss", h0, h1, flush=True)
+    assert np.array_equal(z0, z1)
+    assert h0 == h1
+    assert torch.equal(p0, p1)

@@ -25,8 +25,10 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 // 64 x 64 tile per workgroup of 4 waves (32 x 32 each = 2 x 2 MFMA 16x16x16), K staged in
 // chunks of 32 through LDS.  An operand that is contiguous along k or along its row/column
 // index is staged with one 16-byte load per thread (8 elements) where the group of 8 lies
-// inside the matrix and is 16-byte aligned, else element by element.  gridDim.z splits K;
-// split results are added atomically into C, which the caller has zeroed.  Operand layout of
+// inside the matrix and is 16-byte aligned, else element by element.  gridDim.z splits K:
+// with `part` each split stores its M x N result at part + z M N (k_ksplit_reduce then adds
+// them in split order: the sum does not depend on timing), without it the split results are
+// added atomically into C, which the caller has zeroed.  Operand layout of
 // v_mfma_f32_16x16x16bf16_1k: lane l holds A[l%16][4(l/16)+i] and B[4(l/16)+i][l%16],
 // i = 0..3; the result D[4(l/16)+v][l%16].
 // Epilogues (one K split only): EPI_STORE C = acc (or C += acc); EPI_BIAS_SILU
@@ -71,7 +73,7 @@ __device__ __forceinline__ void stage_tile(uint16_t (*S)[MG_K + MG_PAD], const u
 __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, const uint16_t* __restrict__ A, long sam, long sak,
                                                    const uint16_t* __restrict__ B, long sbk, long sbn, float* __restrict__ C,
                                                    long ldc, int accumulate, int kchunk, int epi, const float* __restrict__ aux,
-                                                   uint16_t* __restrict__ outbf) {
+                                                   uint16_t* __restrict__ outbf, float* __restrict__ part) {
   __shared__ __align__(16) uint16_t As[MG_T][MG_K + MG_PAD];   // [m][k]
   __shared__ __align__(16) uint16_t Bs[MG_T][MG_K + MG_PAD];   // [n][k]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -117,7 +119,8 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, const ui
           const long o = (long)row * ldc + col;
           const float x = acc[i][j][v];
           if (split) {
-            atomicAdd(C + o, x);
+            if (part) part[((long)blockIdx.z * M + row) * N + col] = x;
+            else atomicAdd(C + o, x);
           } else if (epi == EPI_BIAS_SILU) {
             const float z = x + aux[col];
             C[o] = z;
@@ -133,6 +136,18 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, const ui
       }
 }
 
+// C = (accumulate ? C : 0) + part[0] + part[1] + ... + part[kz - 1], element by element in
+// that order
+__global__ __launch_bounds__(256) void k_ksplit_reduce(int M, int N, int kz, const float* __restrict__ part, float* __restrict__ C,
+                                                       long ldc, int accumulate) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)M * N) return;
+  const int row = (int)(e / N), col = (int)(e % N);
+  float x = accumulate ? C[(long)row * ldc + col] : 0.f;
+  for (int z = 0; z < kz; z++) x += part[(long)z * M * N + e];
+  C[(long)row * ldc + col] = x;
+}
+
 // first layer: Z1 = X W1^T + b1, H1 = silu(Z1) (BF16)
 __global__ __launch_bounds__(256) void k_mlp_in(int B, const float* __restrict__ X, const float* __restrict__ P,
                                                 float* __restrict__ Z, uint16_t* __restrict__ H) {
@@ -146,7 +161,8 @@ __global__ __launch_bounds__(256) void k_mlp_in(int B, const float* __restrict__
 }
 
 // last layer from the FP32 pre-activations of layer 4: Y = silu(Z4) W5^T + b5.
-// With targets T: dY = 2 (Y - T) / (5 B) and the summed squared error into *loss.
+// With targets T: dY = 2 (Y - T) / (5 B) and the squared error summed over the block's rows
+// into loss[blockIdx.x].
 __global__ __launch_bounds__(256) void k_mlp_out(int B, const float* __restrict__ Z4, const float* __restrict__ P,
                                                  float* __restrict__ Y, const float* __restrict__ T, float* __restrict__ dY,
                                                  float* __restrict__ loss) {
@@ -181,7 +197,15 @@ __global__ __launch_bounds__(256) void k_mlp_out(int B, const float* __restrict_
     if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) atomicAdd(loss, red[0]);
+  if (threadIdx.x == 0) loss[blockIdx.x] = red[0];
+}
+
+// *loss = the n block sums of k_mlp_out, added in block order
+__global__ void k_loss_reduce(int n, const float* __restrict__ part, float* __restrict__ loss) {
+  if (threadIdx.x != 0) return;
+  float x = 0.f;
+  for (int i = 0; i < n; i++) x += part[i];
+  *loss = x;
 }
 
 // backward through the last layer and the layer-4 activation:
@@ -328,18 +352,31 @@ __global__ __launch_bounds__(256) void k_gather(int B, const int* __restrict__ i
 // ---------------------------------------------------------------------------------
 static inline int cdivi(long a, long b) { return (int)((a + b - 1) / b); }
 
-// epilogue epi with aux / outbf (EPI_STORE: accumulate selects C += ...; a K split adds
-// atomically into C, which must then hold zeros or the sum to add to)
+// K splits of a product: ksplit rounded so that every split is a whole number of K chunks
+static inline int ksplit_count(int K, int ksplit) {
+  int kchunk = cdivi(K, ksplit < 1 ? 1 : ksplit);
+  kchunk = ((kchunk + MG_K - 1) / MG_K) * MG_K;
+  return cdivi(K, kchunk);
+}
+
+// epilogue epi with aux / outbf (EPI_STORE: accumulate selects C += ...).  A K split
+// (EPI_STORE only) stores its partial products in `part` (ksplit_count(K, ksplit) x M x N
+// floats) and adds them to C in split order; with part null they are added atomically into
+// C, which must then hold zeros or the sum to add to.
 static hipError_t gemm_bf16_epi(int M, int N, int K, const uint16_t* A, long sam, long sak, const uint16_t* B, long sbk, long sbn,
                                 float* C, long ldc, int accumulate, int ksplit, int epi, const float* aux, uint16_t* outbf,
-                                hipStream_t st) {
+                                hipStream_t st, float* part = nullptr) {
   if (M <= 0 || N <= 0) return hipSuccess;
   if (ksplit < 1 || epi != EPI_STORE) ksplit = 1;
   int kchunk = cdivi(K, ksplit);
   kchunk = ((kchunk + MG_K - 1) / MG_K) * MG_K;
   const int kz = cdivi(K, kchunk);
+  if (kz == 1) part = nullptr;
   hipLaunchKernelGGL(k_gemm_bf16, dim3(cdivi(M, MG_T), cdivi(N, MG_T), kz), dim3(256), 0, st, M, N, K, A, sam, sak, B, sbk, sbn, C,
-                     ldc, accumulate, kchunk, epi, aux, outbf);
+                     ldc, accumulate, kchunk, epi, aux, outbf, part);
+  if (part)
+    hipLaunchKernelGGL(k_ksplit_reduce, dim3(cdivi((long)M * N, 256)), dim3(256), 0, st, M, N, kz, (const float*)part, C, ldc,
+                       accumulate);
   return hipGetLastError();
 }
 
@@ -358,6 +395,8 @@ hipError_t gemm_bf16(int M, int N, int K, const uint16_t* A, long sam, long sak,
 }
 
 static inline int col_blocks(int B) { return cdivi(B, COL_ROWS); }
+// K splits of the weight-gradient products dW = dZ^T H (K = the batch)
+static inline int grad_ksplit(int B) { return cdivi(B, 512); }
 
 size_t mlp_workspace_bytes(int B) {
   const size_t f = sizeof(float), h = sizeof(uint16_t), W = MLP_W;
@@ -368,11 +407,13 @@ size_t mlp_workspace_bytes(int B) {
          + f * (size_t)B * (2 + 2 * MLP_OUT)   // X batch, T batch, dY
          + f * (size_t)B * MLP_OUT      // Y
          + f * (size_t)col_blocks(B) * W * COL_Q   // column partial sums
+         + f * (size_t)ksplit_count(B, grad_ksplit(B)) * W * W   // K-split partials of a weight gradient
+         + f * (size_t)cdivi(B, 256)    // loss per block of k_mlp_out
          + 64;
 }
 
 struct MlpWs {
-  float *Z[4], *dZ, *Xb, *Tb, *dY, *Y, *part, *loss;
+  float *Z[4], *dZ, *Xb, *Tb, *dY, *Y, *part, *kpart, *lpart, *loss;
   uint16_t *H[3], *dZbf[2];
   MlpWs(void* base, int B) {
     char* p = (char*)base;
@@ -386,6 +427,8 @@ struct MlpWs {
     dY = (float*)p; p += sizeof(float) * B * MLP_OUT;
     Y = (float*)p; p += sizeof(float) * B * MLP_OUT;
     part = (float*)p; p += sizeof(float) * (size_t)col_blocks(B) * W * COL_Q;
+    kpart = (float*)p; p += sizeof(float) * (size_t)ksplit_count(B, grad_ksplit(B)) * W * W;
+    lpart = (float*)p; p += sizeof(float) * (size_t)cdivi(B, 256);
     loss = (float*)p;
   }
 };
@@ -427,11 +470,11 @@ hipError_t mlp_grad(const float* P, const uint16_t* Wbf, const float* X, const f
   MlpWs w(ws, B);
   hipError_t e = forward(P, Wbf, X, B, w, st);
   if (e != hipSuccess) return e;
-  // G (the hidden weight gradients add up over K splits) and the loss start at zero
+  // G starts at zero (the hidden weight gradients are added into it)
   hipLaunchKernelGGL(k_zero_f32, dim3(cdivi(MLP_NPARAM, 256)), dim3(256), 0, st, G, (long)MLP_NPARAM);
-  hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(64), 0, st, loss, 1l);
   const int g = cdivi((long)B * MLP_W, 256);
-  hipLaunchKernelGGL(k_mlp_out, dim3(cdivi(B, 256)), dim3(256), 0, st, B, w.Z[3], P, w.Y, T, w.dY, loss);
+  hipLaunchKernelGGL(k_mlp_out, dim3(cdivi(B, 256)), dim3(256), 0, st, B, w.Z[3], P, w.Y, T, w.dY, w.lpart);
+  hipLaunchKernelGGL(k_loss_reduce, dim3(1), dim3(64), 0, st, cdivi(B, 256), (const float*)w.lpart, loss);
   // layer 5
   col_sums(B, 0, w.dY, w.Z[3], nullptr, nullptr, w.part, G, 0, st);
   hipLaunchKernelGGL(k_out_bias_grad, dim3(1), dim3(256), 0, st, B, w.dY, G);
@@ -439,12 +482,13 @@ hipError_t mlp_grad(const float* P, const uint16_t* Wbf, const float* X, const f
   // layers 4, 3, 2
   const int woff[3] = {MLP_OFF_W2, MLP_OFF_W3, MLP_OFF_W4};
   const int boff[3] = {MLP_OFF_B2, MLP_OFF_B3, MLP_OFF_B4};
-  const int ksplit = cdivi(B, 512);
+  const int ksplit = grad_ksplit(B);
   int cur = 0;
   for (int l = 2; l >= 0; l--) {
-    // dW = dZ^T H_prev: A(m, k) = dZ[k][m], B(k, n) = H_prev[k][n]; K = batch, split over workgroups
+    // dW = dZ^T H_prev: A(m, k) = dZ[k][m], B(k, n) = H_prev[k][n]; K = batch, split over
+    // workgroups, the splits added in a fixed order (training runs repeat bit for bit)
     e = gemm_bf16_epi(MLP_W, MLP_W, B, w.dZbf[cur], 1, MLP_W, w.H[l], MLP_W, 1, G + woff[l], MLP_W, 1, ksplit, EPI_STORE, nullptr,
-                      nullptr, st);
+                      nullptr, st, w.kpart);
     if (e != hipSuccess) return e;
     col_sums(B, 2, nullptr, nullptr, w.dZ, nullptr, w.part, G, boff[l], st);
     // dZ_prev = (dZ W) * silu'(Z_prev): A = dZ (row-major), B(k, n) = W[k][n], the SiLU derivative in

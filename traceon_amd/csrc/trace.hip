@@ -306,13 +306,19 @@ __device__ void mlp_wave(const FieldComp& c, const double q[3], bool need, int l
 // SOURCES = false compiles no source sums (programs flagged PROGRAM_NO_SOURCES); a BEM
 // component met there gives NaN.
 // WAVE (with MLP): the correction is left to mlp_wave; *mlp_need reports whether this
-// point gets it (one learned component, untransformed, as the tracer checks).
-template <bool MLP = false, bool SOURCES = true, bool WAVE = false>
+// point gets it (one learned component, untransformed, as the tracer checks).  With
+// bem_defer non-null, a point that the trust region or the error check sends to the ring
+// sources gets no source sum here: *bem_defer is set and the caller sums them (bem_wave).
+// LEAD = false compiles the lead terms out (lead is then false whatever is passed), so that
+// a kernel that only sums sources carries no learned-correction code or LDS.
+template <bool MLP = false, bool SOURCES = true, bool WAVE = false, bool LEAD = true>
 __device__ void eval_field(const FieldProgram& F, const double p[3], double E[3], double H[3], int s0 = 0, int ds = 1,
-                           bool lead = true, int lane = 0, bool* mlp_need = nullptr) {
+                           bool lead = true, int lane = 0, bool* mlp_need = nullptr, bool* bem_defer = nullptr) {
+  if (!LEAD) lead = false;
   E[0] = E[1] = E[2] = 0.0;
   H[0] = H[1] = H[2] = 0.0;
   if (WAVE) *mlp_need = false;
+  if (WAVE && bem_defer) *bem_defer = false;
   if (!in_bounds(F.bounds, F.bounds_mode, p)) return;
   for (int ci = 0; ci < F.ncomp; ci++) {
     const FieldComp& c = F.comp[ci];
@@ -334,16 +340,17 @@ __device__ void eval_field(const FieldProgram& F, const double p[3], double E[3]
         use_bem = !trust && (c.elec.n || c.mag.n || c.cur.n);
       }
       if (MLP && use_bem) {
-        bem_component(c, q, e, h, s0, ds);
+        if (WAVE && bem_defer) *bem_defer = true;
+        else bem_component(c, q, e, h, s0, ds);
       } else if (lead) {
         const int parts = c.axial_parts ? c.axial_parts : 3;
         if (parts & 1) axial_field(c.zs, c.nz, c.ce, q[0], q[1], q[2], e);
         if (parts & 2) axial_field(c.zs, c.nz, c.cm, q[0], q[1], q[2], h);
-        if (MLP && c.mlp) {
+        if (LEAD && MLP && c.mlp) {
           if constexpr (WAVE) {
             const double r = sqrt(q[0] * q[0] + q[1] * q[1]);
             *mlp_need = r >= c.mlp_geom[0] && q[2] >= c.mlp_geom[2] && q[2] <= c.mlp_geom[3];
-          } else {
+          } else if constexpr (LEAD) {
             mlp_correct(c, q, e, h, lane);
           }
         }
@@ -381,6 +388,29 @@ __device__ __forceinline__ void land_if_crossed(double kp, double kn, const doub
 __device__ __forceinline__ bool inside_trace_bounds(const double* b, const double y[6]) {
   // tracing.c:79-81, inclusive
   return b[0] <= y[0] && y[0] <= b[1] && b[2] <= y[1] && y[1] <= b[3] && b[4] <= y[2] && y[2] <= b[5];
+}
+
+// Ring-source sums of the points that the lanes in `mask` hold (component c, untransformed,
+// so the local point is the point itself), one point at a time by the whole wave: lane l
+// takes sources l, l + 64, ..., and the six partial sums meet in a butterfly over the wave.
+// The owning lane adds factor * sum to its E, H.  Every lane of the wave calls it with the
+// same mask (uniform control flow).  A lane's own sum would cost the wave a walk over every
+// source however few of its lanes need one; here each point costs 1/64 of that.
+__device__ void bem_wave(const FieldComp& c, unsigned long long mask, const double p[3], int lane, double E[3], double H[3]) {
+  while (mask) {
+    const int j = __ffsll((long long)mask) - 1;
+    mask &= mask - 1ull;
+    const double q[3] = {__shfl(p[0], j, 64), __shfl(p[1], j, 64), __shfl(p[2], j, 64)};
+    double e[3] = {0.0, 0.0, 0.0}, h[3] = {0.0, 0.0, 0.0};
+    bem_component(c, q, e, h, lane, 64);
+    double v[6] = {e[0], e[1], e[2], h[0], h[1], h[2]};
+#pragma unroll
+    for (int k = 0; k < 6; k++)
+      for (int o = 32; o >= 1; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
+    if (lane == j) {
+      for (int k = 0; k < 3; k++) { E[k] = fma(c.factor, v[k], E[k]); H[k] = fma(c.factor, v[3 + k], H[k]); }
+    }
+  }
 }
 
 template <int N>
@@ -477,9 +507,9 @@ __device__ __forceinline__ void trace_body(const TraceParams& P) {
         ys[c] = acc;
       }
       double E[3], H[3];
-      bool need = false;
+      bool need = false, bem = false;
       if (active) {
-        if (MLP && wave_mlp) eval_field<MLP, SOURCES, true>(P.field, ys, E, H, 0, 1, true, lane, &need);
+        if (MLP && wave_mlp) eval_field<MLP, SOURCES, true>(P.field, ys, E, H, 0, 1, true, lane, &need, &bem);
         else eval_field<MLP, SOURCES>(P.field, ys, E, H, 0, 1, true, lane);
       } else {   // a lane without a particle: no field work (its state is stale or out of range)
         E[0] = E[1] = E[2] = H[0] = H[1] = H[2] = 0.0;
@@ -489,6 +519,12 @@ __device__ __forceinline__ void trace_body(const TraceParams& P) {
         double ce[3] = {0.0, 0.0, 0.0}, ch[3] = {0.0, 0.0, 0.0};
         mlp_wave(c0, ys, need, lane, ce, ch);
         for (int i = 0; i < 3; i++) { E[i] = fma(c0.factor, ce[i], E[i]); H[i] = fma(c0.factor, ch[i], H[i]); }
+      }
+      if constexpr (MLP && SOURCES) {   // points sent to the ring sources, summed by the whole wave
+        if (wave_mlp) {
+          const unsigned long long bm = __ballot(bem);
+          if (bm) bem_wave(P.field.comp[0], bm, ys, lane, E, H);
+        }
       }
       const double vxh0 = ys[4] * H[2] - ys[5] * H[1];
       const double vxh1 = ys[5] * H[0] - ys[3] * H[2];
@@ -576,9 +612,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) void k_
 // per attempted step does the error norm, accept/reject, step-size update, landing and
 // trajectory bookkeeping with exactly the arithmetic of k_trace.  The host issues
 // SPLIT_BATCH attempts between reads of the active count.
+// Programs with a learned correction add a lead launch per stage, one wave per 64
+// particles: the terms without a source sum (axial series, learned correction on MFMA as
+// in k_trace, uniform fields) go to W.lead, and the chunk launch then sums the ring sources
+// only where the trust region or the error check sends a point to them.  A point that
+// keeps the learned field gets k_trace's value exactly; a point sent to the sources gets
+// the same sum in the chunked order.
 // ---------------------------------------------------------------------------------
 struct SplitWs {
-  double *y, *ys, *k, *part, *h, *hmax, *t, *kprev, *kprev2, *qmv;
+  double *y, *ys, *k, *part, *lead, *h, *hmax, *t, *kprev, *kprev2, *qmv;
   long long *nacc, *natt, *tlen;
   int *active, *count;
   size_t used = 0;   // bytes of the layout (a null base only measures it)
@@ -593,6 +635,7 @@ struct SplitWs {
     ys = (double*)take(sizeof(double) * 6 * P);
     k = (double*)take(sizeof(double) * 36 * P);
     part = (double*)take(sizeof(double) * 6 * (size_t)P * nchunk);
+    lead = (double*)take(sizeof(double) * 6 * (size_t)P);
     h = (double*)take(sizeof(double) * P);
     hmax = (double*)take(sizeof(double) * P);
     t = (double*)take(sizeof(double) * P);
@@ -608,6 +651,7 @@ struct SplitWs {
   static size_t bytes(int P, int nchunk) { return SplitWs(nullptr, P, nchunk).used; }
 };
 constexpr int SPLIT_THREADS = 256;
+constexpr int SPLIT_NO_SOURCE = 0x7fffffff;   // a first source index past every source set
 constexpr int SPLIT_BATCH = 8;
 constexpr long long SPLIT_MAX_ATTEMPTS = 1ll << 22;
 
@@ -649,7 +693,40 @@ __global__ __launch_bounds__(256) void k_split_init(TraceParams P, SplitWs W) {
   if (!inside_trace_bounds(P.bounds, y)) split_retire(P, W, i, STATUS_LEFT_BOUNDS);   // tracing.c:79: no step taken
 }
 
-// partial field sums: blockIdx.y = particle, blockIdx.x = source chunk
+// Lead terms of a program with learned corrections at each particle's stage point: every
+// term without a source sum (s0 past the last source), the learned corrections of the
+// wave's points on MFMA where k_trace takes that path, per lane otherwise.  64 threads, one
+// particle per lane; a wave without an active particle does nothing.
+__global__ __launch_bounds__(64) void k_split_lead(TraceParams P, SplitWs W) {
+  const int i = blockIdx.x * 64 + threadIdx.x, lane = threadIdx.x;
+  const bool active = i < P.n && W.active[i];
+  if (__ballot(active) == 0ull) return;
+  const bool wave_mlp = P.field.ncomp == 1 && P.field.comp[0].mlp && P.field.comp[0].mlp_wbf && !P.field.comp[0].transform;
+  double p[3] = {0.0, 0.0, 0.0};
+  if (active)
+    for (int c = 0; c < 3; c++) p[c] = W.ys[6 * (size_t)i + c];
+  double E[3] = {0.0, 0.0, 0.0}, H[3] = {0.0, 0.0, 0.0};
+  bool need = false;
+  if (active) {
+    if (wave_mlp) eval_field<true, true, true>(P.field, p, E, H, SPLIT_NO_SOURCE, 1, true, lane, &need);
+    else eval_field<true, true>(P.field, p, E, H, SPLIT_NO_SOURCE, 1, true, lane);
+  }
+  if (wave_mlp && __any(need)) {
+    const FieldComp& c0 = P.field.comp[0];
+    double ce[3] = {0.0, 0.0, 0.0}, ch[3] = {0.0, 0.0, 0.0};
+    mlp_wave(c0, p, need, lane, ce, ch);
+    for (int c = 0; c < 3; c++) { E[c] = fma(c0.factor, ce[c], E[c]); H[c] = fma(c0.factor, ch[c], H[c]); }
+  }
+  if (active) {
+    double* o = W.lead + 6 * (size_t)i;
+    o[0] = E[0]; o[1] = E[1]; o[2] = E[2]; o[3] = H[0]; o[4] = H[1]; o[5] = H[2];
+  }
+}
+
+// partial field sums: blockIdx.y = particle, blockIdx.x = source chunk.  LEARNED: the lead
+// terms come from k_split_lead, and a learned component adds its ring sources only at
+// points outside its trust region or error check.
+template <bool LEARNED>
 __global__ __launch_bounds__(SPLIT_THREADS) void k_split_field(TraceParams P, SplitWs W, int nchunk) {
   const int i = blockIdx.y;
   if (!W.active[i]) return;
@@ -657,7 +734,8 @@ __global__ __launch_bounds__(SPLIT_THREADS) void k_split_field(TraceParams P, Sp
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   double p[3] = {W.ys[6 * (size_t)i], W.ys[6 * (size_t)i + 1], W.ys[6 * (size_t)i + 2]};
   double E[3], H[3];
-  eval_field(P.field, p, E, H, blockIdx.x * SPLIT_THREADS + tid, nchunk * SPLIT_THREADS, blockIdx.x == 0 && tid == 0);
+  if (LEARNED) eval_field<true, true, false, false>(P.field, p, E, H, blockIdx.x * SPLIT_THREADS + tid, nchunk * SPLIT_THREADS, false);
+  else eval_field(P.field, p, E, H, blockIdx.x * SPLIT_THREADS + tid, nchunk * SPLIT_THREADS, blockIdx.x == 0 && tid == 0);
   double v[6] = {E[0], E[1], E[2], H[0], H[1], H[2]};
 #pragma unroll
   for (int c = 0; c < 6; c++) {
@@ -674,12 +752,14 @@ __global__ __launch_bounds__(SPLIT_THREADS) void k_split_field(TraceParams P, Sp
 }
 
 // k_s from the chunk partials, then the point of stage s + 1
-__global__ __launch_bounds__(256) void k_split_stage(TraceParams P, SplitWs W, int nchunk, int s) {
+__global__ __launch_bounds__(256) void k_split_stage(TraceParams P, SplitWs W, int nchunk, int s, int learned) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.n || !W.active[i]) return;
   double f[6] = {0, 0, 0, 0, 0, 0};
   for (int c = 0; c < nchunk; c++)
     for (int q = 0; q < 6; q++) f[q] += W.part[((size_t)i * nchunk + c) * 6 + q];
+  if (learned)
+    for (int q = 0; q < 6; q++) f[q] += W.lead[6 * (size_t)i + q];
   const double* ys = W.ys + 6 * (size_t)i;
   const double h = W.h[i], qm = W.qmv[i];
   const double vxh0 = ys[4] * f[5] - ys[5] * f[4];
@@ -788,10 +868,16 @@ __global__ void k_split_zero_count(int* c) {
 static void split_batch(const TraceParams& P, const SplitWs& W, int nchunk, hipStream_t st) {
   const dim3 gp((P.n + 255) / 256), bp(256), gf(nchunk, P.n);
   const int count_blocks = (P.n + 255) / 256 < 64 ? (P.n + 255) / 256 : 64;
+  const int learned = (P.field.flags & PROGRAM_HAS_MLP) ? 1 : 0;
   for (int a = 0; a < SPLIT_BATCH; a++) {
     for (int s = 0; s < 6; s++) {
-      hipLaunchKernelGGL(k_split_field, gf, dim3(SPLIT_THREADS), 0, st, P, W, nchunk);
-      hipLaunchKernelGGL(k_split_stage, gp, bp, 0, st, P, W, nchunk, s);
+      if (learned) {
+        hipLaunchKernelGGL(k_split_lead, dim3((P.n + 63) / 64), dim3(64), 0, st, P, W);
+        hipLaunchKernelGGL(k_split_field<true>, gf, dim3(SPLIT_THREADS), 0, st, P, W, nchunk);
+      } else {
+        hipLaunchKernelGGL(k_split_field<false>, gf, dim3(SPLIT_THREADS), 0, st, P, W, nchunk);
+      }
+      hipLaunchKernelGGL(k_split_stage, gp, bp, 0, st, P, W, nchunk, s, learned);
     }
     hipLaunchKernelGGL(k_split_control, gp, bp, 0, st, P, W);
   }
@@ -800,7 +886,8 @@ static void split_batch(const TraceParams& P, const SplitWs& W, int nchunk, hipS
 }
 
 // One batch captured once into a HIP graph (on a stream of this call, default priority) and
-// replayed per batch on the caller's stream: one launch instead of 106.  TRACEON_SPLIT_GRAPH=0
+// replayed per batch on the caller's stream: one launch instead of 106 (154 with lead
+// launches).  TRACEON_SPLIT_GRAPH=0
 // launches the kernels directly instead; the results are the same.  Returns null when
 // capture is off or fails (the batches are then launched directly).
 static hipGraphExec_t capture_split_batch(const TraceParams& P, const SplitWs& W, int nchunk) {

@@ -43,6 +43,21 @@ def _L():
     return backend.backend_lib
 
 
+def _bin_sums(b: torch.Tensor, v: torch.Tensor, nb: int) -> torch.Tensor:
+    """Sum of the rows of v (M, k) per bin b (M,) in [0, nb): (nb, k), the same bits on every
+    run (an atomic scatter-add would add in whatever order the hardware takes).  The rows are
+    stably sorted by bin and scattered into a dense (nb, largest bin, k) array, one slot
+    each, which a plain reduction then sums."""
+    order = torch.argsort(b, stable=True)
+    bs = b[order]
+    cnt = torch.bincount(b, minlength=nb)
+    start = torch.cumsum(cnt, 0) - cnt
+    slot = torch.arange(b.shape[0], device=b.device) - start[bs]
+    dense = torch.zeros((nb, int(cnt.max().item()), v.shape[1]), dtype=v.dtype, device=v.device)
+    dense[bs, slot] = v[order]
+    return dense.sum(dim=1)
+
+
 def _silu_he_init(rng, n_param):
     """He-style initialisation in the FP32 parameter layout of csrc/mlp_kernels.h."""
     W = 128
@@ -152,7 +167,7 @@ class FieldSurrogate:
         # s_k(z): RMS of D_k / w per z bin, spread to the neighbours (max over 3 bins) and
         # floored at 1e-6 of the largest, so that no bin divides by (almost) zero
         b = torch.clamp(((z - self.z_min) / (self.z_max - self.z_min) * self.nz).long(), 0, self.nz - 1)
-        ss = torch.zeros((self.nz, NOUT), dtype=torch.float64, device=dev).index_add_(0, b, Y * Y)
+        ss = _bin_sums(b, Y * Y, self.nz)
         cnt = torch.bincount(b, minlength=self.nz).double().clamp(min=1.0)
         rms = torch.sqrt(ss / cnt[:, None]).cpu().numpy()          # (nz, 5)
         pad = np.pad(rms, ((1, 1), (0, 0)), mode="edge")
@@ -222,7 +237,7 @@ class FieldSurrogate:
             sH = max(sH, float(Fb[:, 3:5].abs().max().item()))
             err = (self.correction(r, z) - (Fb - Fa)) / self._weight(r)[:, None]
             b = torch.clamp(((z - self.z_min) / (self.z_max - self.z_min) * self.nz).long(), 0, self.nz - 1)
-            ss.index_add_(0, b, err * err)
+            ss += _bin_sums(b, err * err, self.nz)
             cnt += torch.bincount(b, minlength=self.nz).double()
         rms = torch.sqrt(ss / cnt.clamp(min=1.0)[:, None]).cpu().numpy()
         pad = np.pad(rms, ((1, 1), (0, 0)), mode="edge")

@@ -247,8 +247,8 @@ def run_trace(program: DeviceFieldProgram, state0, qm, bounds, atol=1e-8, plane=
               max_attempts=1 << 62, waves=0, split=None, plane2=None, shape: LaunchShape | None = None):
     """GPU tracer over many particles (all arrays numpy or device tensors).
     plane, plane2: (p0, normal); the state at each one's last crossing (NaN if none).
-    split: None picks tb_trace_split for fields with ring sources and at most
-    SPLIT_MAX_PARTICLES particles, else the persistent tb_trace.  shape: launch shape of
+    split: None picks tb_trace_split for fields with ring sources (a learned field's are its
+    fallback) and at most SPLIT_MAX_PARTICLES particles, else the persistent tb_trace.  shape: launch shape of
     the persistent tracer (None: DEFAULT_SHAPE); waves > 0 overrides shape.waves."""
     import torch
     from . import device
@@ -257,9 +257,7 @@ def run_trace(program: DeviceFieldProgram, state0, qm, bounds, atol=1e-8, plane=
     L = backend.backend_lib
     order = None
     if split is None:
-        split = program.has_sources and n <= SPLIT_MAX_PARTICLES and not program.has_mlp
-    if split and program.has_mlp:
-        raise ValueError("fields with a learned correction are traced by the persistent tracer (split=False)")
+        split = program.has_sources and n <= SPLIT_MAX_PARTICLES
     if split:
         nchunk = L.tb_trace_split_chunks(n, program.n_sources)
         ws = torch.empty(L.tb_trace_split_workspace_bytes(n, nchunk), dtype=torch.uint8, device=dev)
