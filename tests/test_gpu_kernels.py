@@ -310,6 +310,44 @@ def test_field_points(dev, geo, oracle):
     assert rel(Fd, Fo) < 1e-11
 
 
+def test_field_points_few_and_many(dev, geo):
+    """The sums for a few points (a workgroup per point, or per four points from 4096 on)
+    against the lane-per-point kernel that serves 2^17 points and more, on the same points:
+    potential, field and both together; current-ring fields and axial potentials the same way.
+    The two paths differ only in summation order."""
+    import torch
+    from traceon_amd import backend
+    lines, types, values = geo
+    jac, pos = backend.fill_jacobian_buffer_radial(lines)
+    rng = np.random.default_rng(31)
+    q = rng.standard_normal(len(lines))
+    n = 1 << 17
+    pts = np.stack([rng.uniform(-0.9e-3, 0.9e-3, n), rng.uniform(-0.5e-3, 0.5e-3, n), rng.uniform(-3e-3, 3e-3, n)], 1)
+    pts[:3, :2] = 0.0
+    T = torch.tensor(pts, device="cuda")
+    S = dev.Sources(q, jac, pos)
+    P_all, F_all = (a.cpu().numpy() for a in S.field_points(T))
+    # synthetic current rings: 300 elements of 12 points, r in [1, 1.5] mm, z in [-0.5, 0.5] mm
+    nc = 300
+    cpos = np.zeros((nc, 12, 3))
+    cpos[..., 0] = rng.uniform(1.0e-3, 1.5e-3, (nc, 12))
+    cpos[..., 2] = rng.uniform(-0.5e-3, 0.5e-3, (nc, 12))
+    Sc = dev.Sources(rng.uniform(0.5, 1.5, nc), rng.uniform(1e-9, 2e-9, (nc, 12)), cpos, kind="current")
+    H_all = Sc.field_points(T, pot=False)[1].cpu().numpy()
+    zq = torch.tensor(rng.uniform(-3e-3, 3e-3, n), device="cuda")
+    A_all = Sc.current_axial_potential(zq).cpu().numpy()
+    for m in (1, 7, 4095, 4096, 5001):
+        Tm = T[:m].contiguous()
+        P, F = (a.cpu().numpy() for a in S.field_points(Tm))
+        Po = S.field_points(Tm, field=False)[0].cpu().numpy()
+        Fo = S.field_points(Tm, pot=False)[1].cpu().numpy()
+        H = Sc.field_points(Tm, pot=False)[1].cpu().numpy()
+        A = Sc.current_axial_potential(zq[:m].contiguous()).cpu().numpy()
+        e = (rel(P, P_all[:m]), rel(F, F_all[:m]), rel(Po, P_all[:m]), rel(Fo, F_all[:m]), rel(H, H_all[:m]), rel(A, A_all[:m]))
+        print("few points", m, ["%.1e" % x for x in e], flush=True)
+        assert max(e) < 1e-13, (m, e)
+
+
 def test_axial_derivatives_and_coefficients(dev, geo, oracle):
     import torch
     from oracle.oracle import quintic_spline_coefficients

@@ -231,6 +231,126 @@ __global__ __launch_bounds__(256) void k_field_points(const double* __restrict__
   }
 }
 
+// The same sums for a few points: PPW points per workgroup, whose 256 threads take the
+// sources s = tid, tid + 256, ... and meet in a fixed-order reduction (wave butterflies, then
+// the four waves in order).  With fewer points than the GPU has lanes, one lane per point
+// would leave most of the chip idle while each lane walks every source.
+template <int PPW, int NV>
+__device__ __forceinline__ void block_sums(double (&v)[PPW][NV], double (&red)[4][PPW][NV]) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < PPW; i++)
+#pragma unroll
+    for (int k = 0; k < NV; k++) v[i][k] = wave_sum(v[i][k]);
+  if (lane == 0)
+#pragma unroll
+    for (int i = 0; i < PPW; i++)
+#pragma unroll
+      for (int k = 0; k < NV; k++) red[wv][i][k] = v[i][k];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < PPW; i++)
+#pragma unroll
+    for (int k = 0; k < NV; k++) v[i][k] = ((red[0][i][k] + red[1][i][k]) + red[2][i][k]) + red[3][i][k];
+}
+
+template <bool POT, bool FIELD, int PPW>
+__global__ __launch_bounds__(256) void k_field_points_wg(const double* __restrict__ pts, int M, const double* __restrict__ sw,
+                                                         const double* __restrict__ sr, const double* __restrict__ sz, int Ns,
+                                                         double* __restrict__ pot, double* __restrict__ field, int accumulate) {
+  __shared__ double red[4][PPW][3];
+  const int p0 = blockIdx.x * PPW;
+  double x[PPW], y[PPW], z0[PPW], r0[PPW], v[PPW][3];
+#pragma unroll
+  for (int i = 0; i < PPW; i++) {
+    const int t = min(p0 + i, M - 1);
+    x[i] = pts[3 * (size_t)t]; y[i] = pts[3 * (size_t)t + 1]; z0[i] = pts[3 * (size_t)t + 2];
+    r0[i] = sqrt(x[i] * x[i] + y[i] * y[i]);
+    v[i][0] = v[i][1] = v[i][2] = 0.0;
+  }
+  for (int s = threadIdx.x; s < Ns; s += 256) {
+    const double w = sw[s], rs = sr[s], zz = sz[s];
+#pragma unroll
+    for (int i = 0; i < PPW; i++) {
+      const double dr = rs - r0[i], dz = zz - z0[i];
+      if (POT && FIELD) {
+        double ph, gr, gz;
+        ring_all(r0[i], dr, dz, ph, gr, gz);
+        v[i][0] = fma(w, ph, v[i][0]); v[i][1] = fma(-w, gr, v[i][1]); v[i][2] = fma(-w, gz, v[i][2]);
+      } else if (POT) {
+        v[i][0] = fma(w, ring_potential(r0[i], dr, dz), v[i][0]);
+      } else {
+        double gr, gz;
+        ring_grad(r0[i], dr, dz, gr, gz);
+        v[i][1] = fma(-w, gr, v[i][1]); v[i][2] = fma(-w, gz, v[i][2]);
+      }
+    }
+  }
+  block_sums<PPW, 3>(v, red);
+  if (threadIdx.x != 0) return;
+#pragma unroll
+  for (int i = 0; i < PPW; i++) {
+    const int t = p0 + i;
+    if (t >= M) break;
+    if (POT) pot[t] = accumulate ? pot[t] + v[i][0] : v[i][0];
+    if (FIELD) {
+      double fx = 0.0, fy = 0.0;
+      if (r0[i] >= AXIS_EPS) { fx = x[i] / r0[i] * v[i][1]; fy = y[i] / r0[i] * v[i][1]; }
+      double* f = field + 3 * (size_t)t;
+      if (accumulate) { f[0] += fx; f[1] += fy; f[2] += v[i][2]; }
+      else { f[0] = fx; f[1] = fy; f[2] = v[i][2]; }
+    }
+  }
+}
+
+template <int PPW>
+__global__ __launch_bounds__(256) void k_current_points_wg(const double* __restrict__ pts, int M, const double* __restrict__ sw,
+                                                           const double* __restrict__ sr, const double* __restrict__ sz, int Ns,
+                                                           double* __restrict__ field, int accumulate) {
+  __shared__ double red[4][PPW][2];
+  const int p0 = blockIdx.x * PPW;
+  double x[PPW], y[PPW], z0[PPW], r[PPW], v[PPW][2];
+#pragma unroll
+  for (int i = 0; i < PPW; i++) {
+    const int t = min(p0 + i, M - 1);
+    x[i] = pts[3 * (size_t)t]; y[i] = pts[3 * (size_t)t + 1]; z0[i] = pts[3 * (size_t)t + 2];
+    r[i] = sqrt(x[i] * x[i] + y[i] * y[i]);
+    v[i][0] = v[i][1] = 0.0;
+  }
+  for (int s = threadIdx.x; s < Ns; s += 256) {
+    const double w = sw[s], rs = sr[s], zz = sz[s];
+#pragma unroll
+    for (int i = 0; i < PPW; i++) {
+      double hr, hz;
+      current_ring_field(r[i], z0[i], rs, zz, hr, hz);
+      v[i][0] = fma(w, hr, v[i][0]); v[i][1] = fma(w, hz, v[i][1]);
+    }
+  }
+  block_sums<PPW, 2>(v, red);
+  if (threadIdx.x != 0) return;
+#pragma unroll
+  for (int i = 0; i < PPW; i++) {
+    const int t = p0 + i;
+    if (t >= M) break;
+    double fx = 0.0, fy = 0.0;
+    if (r[i] >= AXIS_EPS) { fx = x[i] / r[i] * v[i][0]; fy = y[i] / r[i] * v[i][0]; }
+    double* f = field + 3 * (size_t)t;
+    if (accumulate) { f[0] += fx; f[1] += fy; f[2] += v[i][1]; }
+    else { f[0] = fx; f[1] = fy; f[2] = v[i][1]; }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_current_axial_potential_wg(const double* __restrict__ zq, int M, const double* __restrict__ sw,
+                                                                    const double* __restrict__ sr, const double* __restrict__ sz,
+                                                                    int Ns, double* __restrict__ out) {
+  __shared__ double red[4][1][1];
+  const double z0 = zq[blockIdx.x];
+  double v[1][1] = {{0.0}};
+  for (int s = threadIdx.x; s < Ns; s += 256) v[0][0] = fma(sw[s], current_ring_axial_potential(z0, sr[s], sz[s]), v[0][0]);
+  block_sums<1, 1>(v, red);
+  if (threadIdx.x == 0) out[blockIdx.x] = v[0][0];
+}
+
 // current_field_radial (radial.c:56-83) at a batch of points; sources (w, r, z)
 __global__ __launch_bounds__(256) void k_current_points(const double* __restrict__ pts, int M, const double* __restrict__ sw,
                                                         const double* __restrict__ sr, const double* __restrict__ sz, int Ns,
@@ -596,9 +716,28 @@ hipError_t launch_sources(const double* q, const double* jac, const double* pos,
   return hipGetLastError();
 }
 
+// Below this many points the sums run a workgroup per point (or four), above it a lane per
+// point (the lane kernel fills the GPU from about 2^18 points on: 1024 workgroups, 4 waves per SIMD).
+constexpr int FEW_POINTS = 1 << 17;
+
 hipError_t launch_field_points(const double* pts, int M, const double* sw, const double* sr, const double* sz, int Ns, double* pot,
                                double* field, int accumulate, hipStream_t st) {
   if (M <= 0) return hipSuccess;
+  if (M < FEW_POINTS) {
+    const bool four = M >= 4096;
+    const dim3 g(four ? cdiv(M, 4) : (unsigned)M), b(256);
+    if (pot && field) {
+      if (four) hipLaunchKernelGGL((k_field_points_wg<true, true, 4>), g, b, 0, st, pts, M, sw, sr, sz, Ns, pot, field, accumulate);
+      else hipLaunchKernelGGL((k_field_points_wg<true, true, 1>), g, b, 0, st, pts, M, sw, sr, sz, Ns, pot, field, accumulate);
+    } else if (pot) {
+      if (four) hipLaunchKernelGGL((k_field_points_wg<true, false, 4>), g, b, 0, st, pts, M, sw, sr, sz, Ns, pot, field, accumulate);
+      else hipLaunchKernelGGL((k_field_points_wg<true, false, 1>), g, b, 0, st, pts, M, sw, sr, sz, Ns, pot, field, accumulate);
+    } else if (field) {
+      if (four) hipLaunchKernelGGL((k_field_points_wg<false, true, 4>), g, b, 0, st, pts, M, sw, sr, sz, Ns, pot, field, accumulate);
+      else hipLaunchKernelGGL((k_field_points_wg<false, true, 1>), g, b, 0, st, pts, M, sw, sr, sz, Ns, pot, field, accumulate);
+    }
+    return hipGetLastError();
+  }
   dim3 g(cdiv(M, 256)), b(256);
   if (pot && field) hipLaunchKernelGGL((k_field_points<true, true>), g, b, 0, st, pts, M, sw, sr, sz, Ns, pot, field, accumulate);
   else if (pot) hipLaunchKernelGGL((k_field_points<true, false>), g, b, 0, st, pts, M, sw, sr, sz, Ns, pot, field, accumulate);
@@ -609,6 +748,13 @@ hipError_t launch_field_points(const double* pts, int M, const double* sw, const
 hipError_t launch_current_points(const double* pts, int M, const double* sw, const double* sr, const double* sz, int Ns,
                                  double* field, int accumulate, hipStream_t st) {
   if (M <= 0) return hipSuccess;
+  if (M < FEW_POINTS) {
+    if (M >= 4096)
+      hipLaunchKernelGGL(k_current_points_wg<4>, dim3(cdiv(M, 4)), dim3(256), 0, st, pts, M, sw, sr, sz, Ns, field, accumulate);
+    else
+      hipLaunchKernelGGL(k_current_points_wg<1>, dim3(M), dim3(256), 0, st, pts, M, sw, sr, sz, Ns, field, accumulate);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(k_current_points, dim3(cdiv(M, 256)), dim3(256), 0, st, pts, M, sw, sr, sz, Ns, field, accumulate);
   return hipGetLastError();
 }
@@ -616,6 +762,10 @@ hipError_t launch_current_points(const double* pts, int M, const double* sw, con
 hipError_t launch_current_axial_potential(const double* zq, int M, const double* sw, const double* sr, const double* sz, int Ns,
                                           double* out, hipStream_t st) {
   if (M <= 0) return hipSuccess;
+  if (M < FEW_POINTS) {
+    hipLaunchKernelGGL(k_current_axial_potential_wg, dim3(M), dim3(256), 0, st, zq, M, sw, sr, sz, Ns, out);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(k_current_axial_potential, dim3(cdiv(M, 256)), dim3(256), 0, st, zq, M, sw, sr, sz, Ns, out);
   return hipGetLastError();
 }
