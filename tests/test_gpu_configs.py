@@ -202,6 +202,17 @@ def test_c1_mirror_full_size(gpu):
     e_p, e_f = rel(P.cpu().numpy(), f["pot"]), rel(Fd.cpu().numpy(), f["field"])
     print(f"C1 fields at {len(f['pts'])} grid points: potential {e_p:.2e}, field {e_f:.2e}", flush=True)
     assert e_p < 1e-10 and e_f < 1e-10
+    # the reference-named entry point, one point per call as the reference's field.py calls it
+    import time
+    from traceon_amd import backend
+    qh, jh, ph = q, E.jac.cpu().numpy(), E.pos.cpu().numpy()
+    backend.field_radial(f["pts"][0], qh, jh, ph)
+    t0 = time.perf_counter()
+    one = np.array([backend.field_radial(p, qh, jh, ph) for p in f["pts"][:8]])
+    ms = (time.perf_counter() - t0) / 8 * 1e3
+    e_one = rel(one, f["field"][:8])
+    print(f"C1 Tier R field_radial, one point per call: {ms:.2f} ms per call, field {e_one:.2e}", flush=True)
+    assert e_one < 1e-10
 
 
 # ---------------------------------------------------------------------------------------
