@@ -457,53 +457,94 @@ __global__ __launch_bounds__(256) void k_quintic(const double* __restrict__ d, c
 }
 
 // Orders 7, 8: not-a-knot cubic spline (scipy CubicSpline default) in slots 2..5.
-// Tridiagonal system for the knot slopes, solved by the Thomas algorithm; one
-// thread per order (Nz is a few thousand at most).  scratch: 3*Nz doubles per order.
-__global__ void k_cubic_notaknot(const double* __restrict__ d, const double* __restrict__ zq, int Nz,
-                                 double* __restrict__ C, double* __restrict__ scratch) {
-  const int l = 7 + threadIdx.x;
-  if (threadIdx.x >= 2) return;
-  double* cp = scratch + (size_t)threadIdx.x * 3 * Nz;   // modified super-diagonal
-  double* dp = cp + Nz;                                   // modified rhs
-  double* s = dp + Nz;                                    // slopes
-  auto y = [&](int i) { return d[(size_t)i * NDERIV + l]; };
-  auto dx = [&](int i) { return zq[i + 1] - zq[i]; };
-  auto sl = [&](int i) { return (y(i + 1) - y(i)) / dx(i); };
-  const int n = Nz;
-  // row 0: dx1 s0 + (dx0+dx1) s1 = ((dx0 + 2(dx0+dx1)) dx1 sl0 + dx0^2 sl1)/(dx0+dx1)
-  {
-    double a0 = dx(1), dd = dx(0) + dx(1);
-    double rhs = ((dx(0) + 2.0 * dd) * dx(1) * sl(0) + dx(0) * dx(0) * sl(1)) / dd;
-    cp[0] = dd / a0;
-    dp[0] = rhs / a0;
+// The knot slopes s solve a tridiagonal system (Thomas algorithm).  Rows 1..n-2:
+//   dx_i s_{i-1} + 2 (dx_{i-1} + dx_i) s_i + dx_{i-1} s_{i+1} = 3 (dx_i sl_{i-1} + dx_{i-1} sl_i),
+// row 0 and row n-1 the not-a-knot end conditions.  One wave per order walks the
+// elimination: its lanes form the rows of a block of 64 with coalesced loads, and the
+// wave-uniform recurrence takes them one by one through readlane, the lane of each row
+// keeping that row's result for one coalesced store per block.  scratch: 3 * Nz doubles per
+// order (cp, dp, then the slopes in place of dp).
+__device__ __forceinline__ double uniform_lane(double v, int l) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffull), l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// row i of the slope system for order l: (sub, diag, sup, rhs)
+__device__ __forceinline__ void spline_row(const double* __restrict__ d, const double* __restrict__ zq, int n, int l, int i,
+                                           double& sub, double& diag, double& sup, double& rhs) {
+  auto y = [&](int k) { return d[(size_t)k * NDERIV + l]; };
+  auto dx = [&](int k) { return zq[k + 1] - zq[k]; };
+  auto sl = [&](int k) { return (y(k + 1) - y(k)) / dx(k); };
+  if (i == 0) {
+    const double dd = dx(0) + dx(1);
+    sub = 0.0; diag = dx(1); sup = dd;
+    rhs = ((dx(0) + 2.0 * dd) * dx(1) * sl(0) + dx(0) * dx(0) * sl(1)) / dd;
+  } else if (i == n - 1) {
+    const double dd = dx(n - 3) + dx(n - 2);
+    sub = dd; diag = dx(n - 3); sup = 0.0;
+    rhs = (dx(n - 2) * dx(n - 2) * sl(n - 3) + (2.0 * dd + dx(n - 2)) * dx(n - 3) * sl(n - 2)) / dd;
+  } else {
+    sub = dx(i); diag = 2.0 * (dx(i - 1) + dx(i)); sup = dx(i - 1);
+    rhs = 3.0 * (dx(i) * sl(i - 1) + dx(i - 1) * sl(i));
   }
-  for (int i = 1; i < n - 1; i++) {
-    double sub = dx(i), diag = 2.0 * (dx(i - 1) + dx(i)), sup = dx(i - 1);
-    double rhs = 3.0 * (dx(i) * sl(i - 1) + dx(i - 1) * sl(i));
-    double den = diag - sub * cp[i - 1];
-    cp[i] = sup / den;
-    dp[i] = (rhs - sub * dp[i - 1]) / den;
+}
+
+// 64 threads, blockIdx.x = order - 7: forward elimination, then back substitution
+__global__ __launch_bounds__(64) void k_spline_slopes(const double* __restrict__ d, const double* __restrict__ zq, int Nz,
+                                                      double* __restrict__ scratch) {
+  const int l = 7 + blockIdx.x, lane = threadIdx.x, n = Nz;
+  double* cp = scratch + (size_t)blockIdx.x * 3 * Nz;
+  double* dp = cp + Nz;
+  double cprev = 0.0, dprev = 0.0;
+  for (int i0 = 0; i0 < n; i0 += 64) {
+    double sub = 0.0, diag = 1.0, sup = 0.0, rhs = 0.0;
+    if (i0 + lane < n) spline_row(d, zq, n, l, i0 + lane, sub, diag, sup, rhs);
+    double mc = 0.0, md = 0.0;
+    const int cnt = min(64, n - i0);
+#pragma unroll 8
+    for (int t = 0; t < cnt; t++) {
+      const double a = uniform_lane(sub, t), b = uniform_lane(diag, t), c = uniform_lane(sup, t), r = uniform_lane(rhs, t);
+      const double den = b - a * cprev;
+      cprev = c / den;
+      dprev = (r - a * dprev) / den;
+      if (lane == t) { mc = cprev; md = dprev; }
+    }
+    if (i0 + lane < n) { cp[i0 + lane] = mc; dp[i0 + lane] = md; }
   }
-  {
-    // last row: (dx_{n-3}+dx_{n-2}) s_{n-2} + dx_{n-3} s_{n-1} = (dx_{n-2}^2 sl_{n-3} + (2 dd + dx_{n-2}) dx_{n-3} sl_{n-2})/dd
-    double dd = dx(n - 3) + dx(n - 2);
-    double sub = dd, diag = dx(n - 3);
-    double rhs = (dx(n - 2) * dx(n - 2) * sl(n - 3) + (2.0 * dd + dx(n - 2)) * dx(n - 3) * sl(n - 2)) / dd;
-    double den = diag - sub * cp[n - 2];
-    dp[n - 1] = (rhs - sub * dp[n - 2]) / den;
+  // back substitution from the last row: s_{n-1} = dp_{n-1}, s_i = dp_i - cp_i s_{i+1}
+  double snext = 0.0;
+  for (int i1 = n; i1 > 0; i1 -= 64) {
+    const int i0 = max(0, i1 - 64), cnt = i1 - i0;
+    double c = 0.0, dv = 0.0;
+    if (lane < cnt) { c = cp[i0 + lane]; dv = dp[i0 + lane]; }
+    double ms = 0.0;
+#pragma unroll 8
+    for (int t = cnt - 1; t >= 0; t--) {
+      const double ct = uniform_lane(c, t), dt = uniform_lane(dv, t);
+      snext = (i0 + t == n - 1) ? dt : dt - ct * snext;
+      if (lane == t) ms = snext;
+    }
+    if (lane < cnt) dp[i0 + lane] = ms;   // the slopes replace dp
   }
-  s[n - 1] = dp[n - 1];
-  for (int i = n - 2; i >= 0; i--) s[i] = dp[i] - cp[i] * s[i + 1];
-  for (int i = 0; i < n - 1; i++) {
-    double h = dx(i), m = sl(i);
-    double t = (s[i] + s[i + 1] - 2.0 * m) / h;
-    double* c = C + ((size_t)i * NDERIV + l) * 6;
-    c[0] = 0.0; c[1] = 0.0;
-    c[2] = t / h;
-    c[3] = (m - s[i]) / h - t;
-    c[4] = s[i];
-    c[5] = y(i);
-  }
+}
+
+// slots 2..5 of orders 7, 8 on every interval from the slopes
+__global__ __launch_bounds__(256) void k_spline_coeffs(const double* __restrict__ d, const double* __restrict__ zq, int Nz,
+                                                       double* __restrict__ C, const double* __restrict__ scratch) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 2 * (Nz - 1)) return;
+  const int o = t / (Nz - 1), i = t % (Nz - 1), l = 7 + o;
+  const double* s = scratch + (size_t)o * 3 * Nz + Nz;
+  const double h = zq[i + 1] - zq[i], y0 = d[(size_t)i * NDERIV + l], m = (d[(size_t)(i + 1) * NDERIV + l] - y0) / h;
+  const double q = (s[i] + s[i + 1] - 2.0 * m) / h;
+  double* c = C + ((size_t)i * NDERIV + l) * 6;
+  c[0] = 0.0; c[1] = 0.0;
+  c[2] = q / h;
+  c[3] = (m - s[i]) / h - q;
+  c[4] = s[i];
+  c[5] = y0;
 }
 
 // potential_radial_derivs / field_radial_derivs (radial.c:313-384) at a batch of points
@@ -781,7 +822,8 @@ hipError_t launch_axial_derivs(double* out, const double* zq, int Nz, const doub
 hipError_t launch_axial_coeffs(const double* d, const double* zq, int Nz, double* C, double* scratch, hipStream_t st) {
   if (Nz < 4) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_quintic, dim3(cdiv((size_t)(Nz - 1) * 7, 256)), dim3(256), 0, st, d, zq, Nz, C);
-  hipLaunchKernelGGL(k_cubic_notaknot, dim3(1), dim3(64), 0, st, d, zq, Nz, C, scratch);
+  hipLaunchKernelGGL(k_spline_slopes, dim3(2), dim3(64), 0, st, d, zq, Nz, scratch);
+  hipLaunchKernelGGL(k_spline_coeffs, dim3(cdiv((size_t)(Nz - 1) * 2, 256)), dim3(256), 0, st, d, zq, Nz, C, (const double*)scratch);
   return hipGetLastError();
 }
 

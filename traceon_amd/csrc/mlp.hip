@@ -225,13 +225,14 @@ __global__ __launch_bounds__(256) void k_out_bwd(int B, const float* __restrict_
 }
 
 // small weight gradients, sums over the batch per column j of the width-128 layer:
-//   mode 0 (layer 5): dW5[o][j] = sum_b dY[b][o] silu(Z4[b][j])  (db5: k_out_bias_grad)
+//   mode 0 (layer 5): dW5[o][j] = sum_b dY[b][o] silu(Z4[b][j]),  db5[j] = sum_b dY[b][j] (j < 5)
 //   mode 1 (layer 1): dW1[j][c] = sum_b dZ1[b][j] X[b][c],        db1[j] = sum_b dZ1[b][j]
 //   mode 2 (bias):    db[j]     = sum_b dZ[b][j]
 // k_col_partials: workgroup g sums rows [g R, (g + 1) R) (R = COL_ROWS), thread t column
 // t % 128 over the rows of parity t / 128 (coalesced rows), into part[g][j][0..5];
-// k_col_reduce adds the partials of every column in workgroup order (deterministic).
-constexpr int COL_ROWS = 128, COL_Q = MLP_OUT + 1;
+// k_col_reduce adds the partials of every column in a fixed order (deterministic): eight
+// threads per column take every eighth workgroup's partial, then the eight sums in order.
+constexpr int COL_ROWS = 64, COL_Q = MLP_OUT + 1, COL_SPLIT = 8;
 __global__ __launch_bounds__(256) void k_col_partials(int B, int mode, const float* __restrict__ dY, const float* __restrict__ Z4,
                                                       const float* __restrict__ dZ, const float* __restrict__ X,
                                                       float* __restrict__ part) {
@@ -246,6 +247,7 @@ __global__ __launch_bounds__(256) void k_col_partials(int B, int mode, const flo
       const float h = silu(Z4[(long)b * MLP_W + j]);
 #pragma unroll
       for (int o = 0; o < MLP_OUT; o++) s[o] = fmaf(dY[(long)b * MLP_OUT + o], h, s[o]);
+      if (j < MLP_OUT) s[MLP_OUT] += dY[(long)b * MLP_OUT + j];
     } else {
       const float d = dZ[(long)b * MLP_W + j];
       if (mode == 1) {
@@ -264,18 +266,31 @@ __global__ __launch_bounds__(256) void k_col_partials(int B, int mode, const flo
     for (int q = 0; q < COL_Q; q++) part[((long)blockIdx.x * MLP_W + j) * COL_Q + q] = s[q] + red[j][q];
 }
 
-__global__ __launch_bounds__(128) void k_col_reduce(int nblk, int mode, const float* __restrict__ part, float* __restrict__ G,
-                                                    int bias_off) {
-  const int j = threadIdx.x;
+__global__ __launch_bounds__(MLP_W * COL_SPLIT) void k_col_reduce(int nblk, int mode, const float* __restrict__ part,
+                                                                    float* __restrict__ G, int bias_off) {
+  __shared__ float red[COL_SPLIT][MLP_W][COL_Q];
+  const int j = threadIdx.x % MLP_W, h = threadIdx.x / MLP_W;
   float s[COL_Q];
 #pragma unroll
   for (int q = 0; q < COL_Q; q++) s[q] = 0.f;
-  for (int g = 0; g < nblk; g++)
+  for (int g = h; g < nblk; g += COL_SPLIT)
 #pragma unroll
     for (int q = 0; q < COL_Q; q++) s[q] += part[((long)g * MLP_W + j) * COL_Q + q];
+#pragma unroll
+  for (int q = 0; q < COL_Q; q++) red[h][j][q] = s[q];
+  __syncthreads();
+  if (h != 0) return;
+#pragma unroll
+  for (int q = 0; q < COL_Q; q++) {
+    float x = red[0][j][q];
+#pragma unroll
+    for (int k = 1; k < COL_SPLIT; k++) x += red[k][j][q];
+    s[q] = x;
+  }
   if (mode == 0) {
 #pragma unroll
     for (int o = 0; o < MLP_OUT; o++) G[MLP_OFF_W5 + o * MLP_W + j] = s[o];
+    if (j < MLP_OUT) G[MLP_OFF_B5 + j] = s[MLP_OUT];
   } else if (mode == 1) {
     G[MLP_OFF_W1 + 2 * j] = s[0];
     G[MLP_OFF_W1 + 2 * j + 1] = s[1];
@@ -283,28 +298,6 @@ __global__ __launch_bounds__(128) void k_col_reduce(int nblk, int mode, const fl
   } else {
     G[bias_off + j] = s[2];
   }
-}
-
-// db5[o] = sum_b dY[b][o]: one workgroup
-__global__ __launch_bounds__(256) void k_out_bias_grad(int B, const float* __restrict__ dY, float* __restrict__ G) {
-  __shared__ float red[256][MLP_OUT];
-  const int tid = threadIdx.x;
-  float s[MLP_OUT] = {0.f, 0.f, 0.f, 0.f, 0.f};
-  for (int b = tid; b < B; b += 256)
-#pragma unroll
-    for (int o = 0; o < MLP_OUT; o++) s[o] += dY[(long)b * MLP_OUT + o];
-#pragma unroll
-  for (int o = 0; o < MLP_OUT; o++) red[tid][o] = s[o];
-  __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if (tid < st)
-#pragma unroll
-      for (int o = 0; o < MLP_OUT; o++) red[tid][o] += red[tid + st][o];
-    __syncthreads();
-  }
-  if (tid == 0)
-#pragma unroll
-    for (int o = 0; o < MLP_OUT; o++) G[MLP_OFF_B5 + o] = red[0][o];
 }
 
 // Adam on the FP32 master parameters; BF16 copies of the three hidden weight matrices.
@@ -436,7 +429,7 @@ struct MlpWs {
 static void col_sums(int B, int mode, const float* dY, const float* Z4, const float* dZ, const float* X, float* part, float* G,
                      int bias_off, hipStream_t st) {
   hipLaunchKernelGGL(k_col_partials, dim3(col_blocks(B)), dim3(256), 0, st, B, mode, dY, Z4, dZ, X, part);
-  hipLaunchKernelGGL(k_col_reduce, dim3(1), dim3(MLP_W), 0, st, col_blocks(B), mode, part, G, bias_off);
+  hipLaunchKernelGGL(k_col_reduce, dim3(1), dim3(MLP_W * COL_SPLIT), 0, st, col_blocks(B), mode, part, G, bias_off);
 }
 
 static hipError_t forward(const float* P, const uint16_t* Wbf, const float* X, int B, MlpWs& w, hipStream_t st) {
@@ -476,8 +469,7 @@ hipError_t mlp_grad(const float* P, const uint16_t* Wbf, const float* X, const f
   hipLaunchKernelGGL(k_mlp_out, dim3(cdivi(B, 256)), dim3(256), 0, st, B, w.Z[3], P, w.Y, T, w.dY, w.lpart);
   hipLaunchKernelGGL(k_loss_reduce, dim3(1), dim3(64), 0, st, cdivi(B, 256), (const float*)w.lpart, loss);
   // layer 5
-  col_sums(B, 0, w.dY, w.Z[3], nullptr, nullptr, w.part, G, 0, st);
-  hipLaunchKernelGGL(k_out_bias_grad, dim3(1), dim3(256), 0, st, B, w.dY, G);
+  col_sums(B, 0, w.dY, w.Z[3], nullptr, nullptr, w.part, G, 0, st);   // dW5 and db5
   hipLaunchKernelGGL(k_out_bwd, dim3(g), dim3(256), 0, st, B, w.dY, w.Z[3], P, w.dZ, w.dZbf[0]);
   // layers 4, 3, 2
   const int woff[3] = {MLP_OFF_W2, MLP_OFF_W3, MLP_OFF_W4};
