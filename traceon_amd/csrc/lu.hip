@@ -910,9 +910,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_num_vgpr(72))) void k_ge
 // B[8(l/16)+i][l%16], i = 0..7; the accumulators are those of v_mfma_f32_16x16x4f32.
 // Tile 128 x 128, 512 threads (4 x 2 waves of 32 x 64), K chunks of 32.  The split happens
 // when a chunk moves from registers to LDS: A as [part][row][k], B as [part][col][k], 64-byte
-// rows whose four 16-byte segments are XOR-swizzled by (row / 4) % 4, so the 16 rows a
-// fragment read touches fall in distinct banks.  48 KiB of LDS and 116 VGPRs: two workgroups
-// (16 waves) per CU, so one wave's split and stores overlap another's MFMAs.
+// rows whose four 16-byte segments are XOR-swizzled by (row / 2) % 4: any 8 consecutive rows
+// then put a given 16-byte segment in 8 distinct 16-byte bank groups, so the fragment reads
+// (lanes = 16 consecutive rows at one k group) and the B stores (lanes = consecutive columns,
+// 16 bytes of 8 consecutive k each) are free of bank conflicts, and an A row's 64 bytes are
+// written by 8 consecutive lanes.  48 KiB of LDS: two workgroups (16 waves) per CU, so one
+// wave's split and stores overlap another's MFMAs.
 // Fast shapes only (see gemm_tile): A rows and C rows 16-byte aligned, K % 32 == 0,
 // N % 4 == 0, M, N >= 128; B is read element-wise (its rows need no alignment).
 // ---------------------------------------------------------------------------------
@@ -926,7 +929,7 @@ __device__ __forceinline__ void split_bf16x3(float x, __bf16& p0, __bf16& p1, __
   p2 = (__bf16)(r1 - (float)p1);
 }
 // byte offset of (row, k) in one 128 x 32 BF16 part
-__device__ __forceinline__ int x6_off(int row, int k) { return row * 64 + 16 * ((k >> 3) ^ ((row >> 2) & 3)) + 2 * (k & 7); }
+__device__ __forceinline__ int x6_off(int row, int k) { return row * 64 + 16 * ((k >> 3) ^ ((row >> 1) & 3)) + 2 * (k & 7); }
 
 __global__ __launch_bounds__(512, 2) void k_gemm_x6(int M, int N, int K, float alpha, const float* __restrict__ A, size_t lda,
                                                     const float* __restrict__ B, size_t ldb, float beta, float* __restrict__ C,
@@ -962,7 +965,7 @@ __global__ __launch_bounds__(512, 2) void k_gemm_x6(int M, int N, int K, float a
 #pragma unroll
     for (int b = 0; b < MB; b++) acc[a][b] = f4v{0, 0, 0, 0};
   // per chunk, thread: A elements (tid + 512 i) * 4 .. + 3 of the 128 x 32 tile (k fastest);
-  // B column bn + (tid % 128), k = 4 (tid / 128 + 4 i) + 0..3
+  // B column bn + (tid % 128), k = 8 (tid / 128) + 0..7
   f4v ra[2];
   float rb[8];
   const int bc = tid & 127, bkg = tid >> 7;
@@ -973,9 +976,7 @@ __global__ __launch_bounds__(512, 2) void k_gemm_x6(int M, int N, int K, float a
       ra[i] = *reinterpret_cast<const f4v*>(A + (size_t)(bm + e / BK) * lda + k0 + e % BK);
     }
 #pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-      for (int j = 0; j < 4; j++) rb[4 * i + j] = B[(size_t)(k0 + 4 * (bkg + 4 * i) + j) * ldb + bn + bc];
+    for (int j = 0; j < 8; j++) rb[j] = B[(size_t)(k0 + 8 * bkg + j) * ldb + bn + bc];
   };
   auto stash = [&]() {
 #pragma unroll
@@ -993,19 +994,18 @@ __global__ __launch_bounds__(512, 2) void k_gemm_x6(int M, int N, int K, float a
       *reinterpret_cast<bf16x4_t*>(As + PART + off) = p1;
       *reinterpret_cast<bf16x4_t*>(As + 2 * PART + off) = p2;
     }
+    {
+      bf16x8_t p0, p1, p2;
 #pragma unroll
-    for (int i = 0; i < 2; i++) {
-      bf16x4_t p0, p1, p2;
-#pragma unroll
-      for (int v = 0; v < 4; v++) {
+      for (int v = 0; v < 8; v++) {
         __bf16 x0, x1, x2;
-        split_bf16x3(rb[4 * i + v], x0, x1, x2);
+        split_bf16x3(rb[v], x0, x1, x2);
         p0[v] = x0; p1[v] = x1; p2[v] = x2;
       }
-      const int off = x6_off(bc, 4 * (bkg + 4 * i));
-      *reinterpret_cast<bf16x4_t*>(Bs + off) = p0;
-      *reinterpret_cast<bf16x4_t*>(Bs + PART + off) = p1;
-      *reinterpret_cast<bf16x4_t*>(Bs + 2 * PART + off) = p2;
+      const int off = x6_off(bc, 8 * bkg);
+      *reinterpret_cast<bf16x8_t*>(Bs + off) = p0;
+      *reinterpret_cast<bf16x8_t*>(Bs + PART + off) = p1;
+      *reinterpret_cast<bf16x8_t*>(Bs + 2 * PART + off) = p2;
     }
   };
   const int nk = K / BK;

@@ -270,7 +270,9 @@ def solve_system(M64: torch.Tensor | None, F: np.ndarray | torch.Tensor, M32: to
         return X, SolveStats("mixed->fp64", it + st.iterations, st.residual, lu.info), lu
     if rlast is None:
         _, rlast = residual(M64, Bt, X)
-    return X, SolveStats(mode, it, float(rlast.max().item()), lu.info), lu
+    rstat = torch.zeros(1, dtype=torch.int64, device=X.device)
+    absmax_into(rlast, rstat)   # the largest residual over the right-hand sides (library kernel)
+    return X, SolveStats(mode, it, float(rstat.cpu().view(torch.float64)[0]), lu.info), lu
 
 
 class Sources:
