@@ -186,3 +186,14 @@ def test_solve_status_maps_to_an_error():
     device.check_solve_status(0)
     with pytest.raises(RuntimeError, match="spin limit"):
         device.check_solve_status(1)
+
+
+def test_split_chunk_count_is_defined_for_every_particle_count():
+    """tb_trace_split_chunks is host arithmetic: a chunk count in [1, 1024] for any particle
+    count, zero included (a trace of no particles launches nothing)."""
+    from traceon_amd import backend
+    L = backend.backend_lib
+    for n in (0, 1, 7, 64, 16384, 65535):
+        for ns in (0, 1, 256 * 8, 320000):
+            c = L.tb_trace_split_chunks(n, ns)
+            assert 1 <= c <= 1024, (n, ns, c)

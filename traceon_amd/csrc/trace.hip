@@ -849,6 +849,7 @@ __global__ __launch_bounds__(256) void k_split_count(const int* __restrict__ act
 }
 
 int split_chunks(int n_particles, long long n_sources) {
+  if (n_particles <= 0) return 1;   // nothing to trace; any valid chunk count
   // enough workgroups to cover the chip (~2048), at least 8 sources per thread
   long long by_src = n_sources / (SPLIT_THREADS * 8);
   long long by_occ = (2048 + n_particles - 1) / n_particles;
