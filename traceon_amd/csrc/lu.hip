@@ -1495,9 +1495,12 @@ __device__ __forceinline__ void dd_add(double& hi, double& lo, double a, double 
 __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ A, size_t lda, int N, const double* __restrict__ B,
                                                   const double* __restrict__ X, double* __restrict__ Rv, int nrhs,
                                                   unsigned long long* __restrict__ rmax) {
-  // one wave per row, four independent double-double partial sums per lane (four loads in
-  // flight); one atomic per workgroup and right-hand side for the maximum
+  // one wave per row, four independent double-double partial sums per lane; one atomic per
+  // workgroup and right-hand side for the maximum.  Rows that start 16-byte aligned are read
+  // as pairs of doubles, eight 16-byte loads in flight per lane (8 KB per wave, enough to keep
+  // HBM busy with the waves a CU holds); other rows element by element, four in flight.
   __shared__ double wm[4];
+  typedef double d2v __attribute__((ext_vector_type(2)));
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int row = blockIdx.x * 4 + wave;
   const double* a = A + (size_t)(row < N ? row : 0) * lda;
@@ -1507,6 +1510,26 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ A, 
     if (row < N) {
       double hi[4] = {0.0, 0.0, 0.0, 0.0}, lo[4] = {0.0, 0.0, 0.0, 0.0};
       int s = lane;
+      if ((((uintptr_t)a | (uintptr_t)x) & 15) == 0) {
+        // pair p of the row (elements 2p, 2p+1): lane takes pairs lane, lane + 64, ...
+        int pb = 0;
+        for (; pb + 8 * 64 <= N / 2; pb += 8 * 64) {
+          d2v pa[8], px[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) {
+            pa[u] = *reinterpret_cast<const d2v*>(a + 2 * (pb + 64 * u + lane));
+            px[u] = *reinterpret_cast<const d2v*>(x + 2 * (pb + 64 * u + lane));
+          }
+#pragma unroll
+          for (int u = 0; u < 8; u++)
+#pragma unroll
+            for (int v = 0; v < 2; v++) {
+              const double pr = pa[u][v] * px[u][v];
+              dd_add(hi[(2 * u + v) & 3], lo[(2 * u + v) & 3], pr, fma(pa[u][v], px[u][v], -pr));
+            }
+        }
+        s = 2 * pb + lane;   // the rest element by element
+      }
       for (; s + 192 < N; s += 256) {
         double pa[4], px[4];
 #pragma unroll
