@@ -928,6 +928,22 @@ __device__ __forceinline__ void split_bf16x3(float x, __bf16& p0, __bf16& p1, __
   p1 = (__bf16)r1;
   p2 = (__bf16)(r1 - (float)p1);
 }
+// The same split for two values at once: one packed conversion per part (v_cvt_pk_bf16_f32,
+// round to nearest even), the parts' values recovered exactly from the packed bits (BF16 is
+// the upper half of an FP32).  p0/p1/p2 hold the parts of a in bits 0-15 and of b in 16-31;
+// the arithmetic is that of split_bf16x3.
+__device__ __forceinline__ void split_pair_bf16x3(float a, float b, unsigned& p0, unsigned& p1, unsigned& p2) {
+  typedef float f2v __attribute__((ext_vector_type(2)));
+  typedef __bf16 h2v __attribute__((ext_vector_type(2)));
+  const h2v v0 = __builtin_convertvector((f2v{a, b}), h2v);
+  p0 = __builtin_bit_cast(unsigned, v0);
+  const float ra = a - __uint_as_float(p0 << 16), rb = b - __uint_as_float(p0 & 0xffff0000u);
+  const h2v v1 = __builtin_convertvector((f2v{ra, rb}), h2v);
+  p1 = __builtin_bit_cast(unsigned, v1);
+  const float sa = ra - __uint_as_float(p1 << 16), sb = rb - __uint_as_float(p1 & 0xffff0000u);
+  const h2v v2 = __builtin_convertvector((f2v{sa, sb}), h2v);
+  p2 = __builtin_bit_cast(unsigned, v2);
+}
 // byte offset of (row, k) in one 128 x 32 BF16 part
 __device__ __forceinline__ int x6_off(int row, int k) { return row * 64 + 16 * ((k >> 3) ^ ((row >> 1) & 3)) + 2 * (k & 7); }
 
@@ -969,43 +985,40 @@ __global__ __launch_bounds__(512, 2) void k_gemm_x6(int M, int N, int K, float a
   f4v ra[2];
   float rb[8];
   const int bc = tid & 127, bkg = tid >> 7;
+  // this thread's B column at k = 8 bkg; a chunk's rows follow at a stride of ldb
+  const float* const bcol = B + (size_t)(8 * bkg) * ldb + bn + bc;
   auto fetch = [&](int k0) {
 #pragma unroll
     for (int i = 0; i < 2; i++) {
       const int e = (tid + i * NT) * 4;
       ra[i] = *reinterpret_cast<const f4v*>(A + (size_t)(bm + e / BK) * lda + k0 + e % BK);
     }
+    const float* q = bcol + (size_t)k0 * ldb;
 #pragma unroll
-    for (int j = 0; j < 8; j++) rb[j] = B[(size_t)(k0 + 8 * bkg + j) * ldb + bn + bc];
+    for (int j = 0; j < 8; j++, q += ldb) rb[j] = *q;
   };
+  typedef unsigned u2v __attribute__((ext_vector_type(2)));
+  typedef unsigned u4v __attribute__((ext_vector_type(4)));
   auto stash = [&]() {
 #pragma unroll
     for (int i = 0; i < 2; i++) {
       const int e = (tid + i * NT) * 4;
-      bf16x4_t p0, p1, p2;
-#pragma unroll
-      for (int v = 0; v < 4; v++) {
-        __bf16 x0, x1, x2;
-        split_bf16x3(ra[i][v], x0, x1, x2);
-        p0[v] = x0; p1[v] = x1; p2[v] = x2;
-      }
+      unsigned p0[2], p1[2], p2[2];
+      split_pair_bf16x3(ra[i][0], ra[i][1], p0[0], p1[0], p2[0]);
+      split_pair_bf16x3(ra[i][2], ra[i][3], p0[1], p1[1], p2[1]);
       const int off = x6_off(e / BK, e % BK);
-      *reinterpret_cast<bf16x4_t*>(As + off) = p0;
-      *reinterpret_cast<bf16x4_t*>(As + PART + off) = p1;
-      *reinterpret_cast<bf16x4_t*>(As + 2 * PART + off) = p2;
+      *reinterpret_cast<u2v*>(As + off) = u2v{p0[0], p0[1]};
+      *reinterpret_cast<u2v*>(As + PART + off) = u2v{p1[0], p1[1]};
+      *reinterpret_cast<u2v*>(As + 2 * PART + off) = u2v{p2[0], p2[1]};
     }
     {
-      bf16x8_t p0, p1, p2;
+      unsigned p0[4], p1[4], p2[4];
 #pragma unroll
-      for (int v = 0; v < 8; v++) {
-        __bf16 x0, x1, x2;
-        split_bf16x3(rb[v], x0, x1, x2);
-        p0[v] = x0; p1[v] = x1; p2[v] = x2;
-      }
+      for (int v = 0; v < 4; v++) split_pair_bf16x3(rb[2 * v], rb[2 * v + 1], p0[v], p1[v], p2[v]);
       const int off = x6_off(bc, 8 * bkg);
-      *reinterpret_cast<bf16x8_t*>(Bs + off) = p0;
-      *reinterpret_cast<bf16x8_t*>(Bs + PART + off) = p1;
-      *reinterpret_cast<bf16x8_t*>(Bs + 2 * PART + off) = p2;
+      *reinterpret_cast<u4v*>(Bs + off) = u4v{p0[0], p0[1], p0[2], p0[3]};
+      *reinterpret_cast<u4v*>(Bs + PART + off) = u4v{p1[0], p1[1], p1[2], p1[3]};
+      *reinterpret_cast<u4v*>(Bs + 2 * PART + off) = u4v{p2[0], p2[1], p2[2], p2[3]};
     }
   };
   const int nk = K / BK;
