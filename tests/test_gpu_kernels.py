@@ -42,7 +42,9 @@ def test_gemm_mfma_layout(dev):
                                       (129, 257, 64, 1.0, False), (256, 384, 96, 1.0, False), (384, 256, 160, 0.0, False),
                                       (256, 256, 0, 1.0, False), (256, 128, 128, 1.0, True),
                                       # 512 or more 128 x 128 tiles: the large-tile kernel
-                                      (2048, 4096, 128, 1.0, False), (2100, 3970, 160, 0.0, False)):
+                                      (2048, 4096, 128, 1.0, False), (2100, 3970, 160, 0.0, False),
+                                      # K = 512 and 544 (an odd number of K chunks), edge tiles in both directions
+                                      (2300, 3900, 512, 1.0, False), (2100, 3972, 544, 0.0, False)):
             A = rng.standard_normal((M, K))
             Bm = rng.standard_normal((K, N))
             C0 = rng.standard_normal((M, N))
