@@ -160,7 +160,8 @@ def traces(tracer, P, V, exit_plane, back_plane=None, timer=False, atol=1e-8):
         back = (plane < 0) & np.isfinite(l2[:, 0])
         land[back] = l2[back]
         plane[back] = 1
-    out = {"landing": land, "plane": plane, "n_accepted": r.n_accepted.cpu().numpy(), "n_attempted": r.n_attempted.cpu().numpy()}
+    out = {"landing": land, "plane": plane, "n_accepted": r.n_accepted.cpu().numpy(), "n_attempted": r.n_attempted.cpu().numpy(),
+           "parked": r.n_parked}
     if t is not None:
         out["ms"] = t.ms
     return out
@@ -357,6 +358,7 @@ def c4_leg(ctx, n_samples=1 << 22, steps=8000):
            "train_steps": steps, "batch": 16384, "label_s": t_gen, "train_s": sur.train_seconds, "loss_history": hist,
            "field_error_by_radius": bands, "n": len(P), "ms": res["ms"], "electrons_per_s": len(P) / (res["ms"] * 1e-3),
            "vs_gpu_bem": landing_error(res, ctx["ref"]), "launch_shape": shape_str(shape),
+           "parked": res["parked"],   # finished on the stage-synchronous tracer (ring-source sums)
            "tuning_ms": {shape_str(k): round(v, 3) for k, v in tuning.items()}}
     # Off-axis beam: RK45's own error at the workload's atol = 1e-8 is about 400 nm here (the
     # oracle's landings at 1e-8 and 1e-10 differ by that much), so the fields are compared at

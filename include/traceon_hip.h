@@ -227,7 +227,7 @@ int tb_charge_radial(const double *lines, const double *q, int N, double *out, v
 /* Field programs and batched tracing (tracing.c:60-174 for many particles at once). */
 enum { TB_COMP_BEM = 0, TB_COMP_AXIAL = 1, TB_COMP_UNIFORM = 2 };
 enum { TB_BOUNDS_NONE = 0, TB_BOUNDS_STRICT = 1, TB_BOUNDS_INCLUSIVE = 2 };
-enum { TB_STATUS_LEFT_BOUNDS = 0, TB_STATUS_BLOCK_FULL = 1, TB_STATUS_STEP_CAP = 2 };
+enum { TB_STATUS_LEFT_BOUNDS = 0, TB_STATUS_BLOCK_FULL = 1, TB_STATUS_STEP_CAP = 2, TB_STATUS_PARKED = 3 };
 
 typedef struct { const double *w; const double *r; const double *z; int n, pad; } tb_sources_t;
 typedef struct {
@@ -278,6 +278,17 @@ typedef struct {
      None of them changes any particle's result. */
   const int *order;
   int batch, occupancy;
+  /* hmax0 (n, or null): each particle's step cap (null or <= 0: TRACING_STEP_MAX / |v| of
+     state0, tracing.c:68-70); hmax_final (n, or null): the cap each particle traced with, so
+     that a trace resumed from final_state, t_final, h_final and hmax_final continues with the
+     same steps. */
+  const double *hmax0; double *hmax_final;
+  /* park = 1 (tb_trace, a program with one learned component): a particle whose step needs a
+     ring-source sum (outside the trust region, or the error check fails) is retired at the
+     start of that step with TB_STATUS_PARKED, its state, t, h and counts at that point,
+     instead of its wave summing the sources; the caller finishes such particles with
+     tb_trace_split, which spreads every sum over the GPU.  tb_trace_split ignores it. */
+  int park;
 } tb_trace_params_t;
 
 /* Persistent RK45 tracer over params->n particles (all pointers device pointers).

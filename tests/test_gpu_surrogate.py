@@ -293,7 +293,7 @@ def test_split_learned_tracer_matches_persistent(lens):
     for split in (True, False):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        r_ = tracing.run_trace(prog, np.hstack([P, V]), QM, bounds, plane=plane, split=split)
+        r_ = tracing.run_trace(prog, np.hstack([P, V]), QM, bounds, plane=plane, split=split, park=False)
         torch.cuda.synchronize()
         out[split] = (r_.landing.cpu().numpy(), r_.n_attempted.cpu().numpy(), time.perf_counter() - t0)
     (la, sa, ta), (lb, sb, tb_) = out[True], out[False]
@@ -306,6 +306,50 @@ def test_split_learned_tracer_matches_persistent(lens):
     assert np.array_equal(fate, np.isfinite(lb[:, 0]))
     assert same.mean() >= 0.95
     assert np.max(np.abs(la[same & fate, :3] - lb[same & fate, :3])) < 1e-12
+
+
+def test_parked_particles_resume_on_split_tracer(lens):
+    """The persistent tracer parks a particle at the start of a step that needs the ring sources
+    and the stage-synchronous tracer finishes it from that state, step size and step cap: the
+    fates, step counts and landings are those of the persistent tracer that sums the sources in
+    the wave (up to the BEM sums' summation order, as above).  The entrance plane lies 0.1 mm
+    past the start, so most particles cross it before they are parked, and the exit plane after
+    the hand-over: both landings survive it."""
+    from traceon_amd import tracing
+    from traceon_amd.surrogate import FieldRadialAxialLearned
+    field, axial, sur, _ = lens
+    learned = FieldRadialAxialLearned(sur, error_tolerance=1e-7)
+    rng = np.random.default_rng(22)
+    n = 256
+    r = rng.uniform(0.05 * MM, 0.55 * MM, n)
+    phi = rng.uniform(0, 2 * np.pi, n)
+    P = np.column_stack([r * np.cos(phi), r * np.sin(phi), np.full(n, -4.9 * MM)])
+    V = np.tile(tracing.velocity_vec(1000.0, [0, 0, 1]), (n, 1))
+    bounds = np.array([[-0.95 * MM, 0.95 * MM], [-0.95 * MM, 0.95 * MM], [-5 * MM, 5 * MM]])
+    exit_plane = (np.array([0.0, 0.0, 4.5 * MM]), np.array([0.0, 0.0, 1.0]))
+    entry_plane = (np.array([0.0, 0.0, -4.8 * MM]), np.array([0.0, 0.0, 1.0]))
+    prog = learned.get_tracer(bounds).program()
+    QM = np.full(n, -tracing.E_CHARGE / tracing.M_E)
+    res = {}
+    for park in (True, False):
+        r_ = tracing.run_trace(prog, np.hstack([P, V]), QM, bounds, plane=exit_plane, plane2=entry_plane, split=False, park=park)
+        res[park] = r_
+        res[park].np = {k: getattr(r_, k).cpu().numpy() for k in ("landing", "landing2", "n_attempted", "n_accepted", "status", "final")}
+    a, b = res[True].np, res[False].np
+    npk = res[True].n_parked
+    fate = np.isfinite(a["landing"][:, 0])
+    same = a["n_attempted"] == b["n_attempted"]
+    print(f"parked {npk} of {n}; landed {fate.sum()}; equal steps {same.mean():.3f}; max landing diff "
+          f"{np.max(np.abs(a['landing'][same & fate, :3] - b['landing'][same & fate, :3])):.2e} m", flush=True)
+    assert npk > 0 and res[False].n_parked == 0
+    assert not np.any(a["status"] == tracing.STATUS_PARKED)
+    assert np.array_equal(a["status"], b["status"])
+    assert np.array_equal(fate, np.isfinite(b["landing"][:, 0])) and fate.mean() > 0.9
+    assert same.mean() >= 0.95 and np.array_equal(a["n_accepted"][same], b["n_accepted"][same])
+    assert np.max(np.abs(a["landing"][same & fate, :3] - b["landing"][same & fate, :3])) < 1e-12
+    assert np.all(np.isfinite(a["landing2"][:, 0]))
+    assert np.max(np.abs(a["landing2"][:, :3] - b["landing2"][:, :3])) < 1e-12
+    assert np.max(np.abs(a["landing2"][:, 3:] - b["landing2"][:, 3:])) < 1e-12 * np.max(np.abs(b["landing2"][:, 3:]))
 
 
 def test_training_repeats_bit_for_bit(lens):

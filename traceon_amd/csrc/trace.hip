@@ -448,7 +448,7 @@ __device__ __forceinline__ void trace_body(const TraceParams& P) {
         t = P.t0 ? P.t0[id] : 0.0;
         qm = P.qm[P.qm_stride ? id : 0];
         const double speed = sqrt(y[3] * y[3] + y[4] * y[4] + y[5] * y[5]);
-        hmax = STEP_MAX / speed;   // TRACING_STEP_MAX, tracing.c:5, 68-70
+        hmax = (P.hmax0 && P.hmax0[id] > 0.0) ? P.hmax0[id] : STEP_MAX / speed;   // TRACING_STEP_MAX, tracing.c:5, 68-70
         h = (P.h0 && P.h0[id] > 0.0) ? P.h0[id] : hmax;
         nacc = 0; natt = 0;
         kprev = signed_dist(P, y);
@@ -478,6 +478,7 @@ __device__ __forceinline__ void trace_body(const TraceParams& P) {
     if (P.n_attempted) P.n_attempted[id] = natt;
     if (P.t_final) P.t_final[id] = t;
     if (P.h_final) P.h_final[id] = h;
+    if (P.hmax_final) P.hmax_final[id] = hmax;
     if (P.traj_len) P.traj_len[id] = traj_n;
   };
 
@@ -495,6 +496,7 @@ __device__ __forceinline__ void trace_body(const TraceParams& P) {
     if (__ballot(pid >= 0) == 0ull) break;
     const bool active = pid >= 0;
     double k[6][6];
+    unsigned long long parked = 0ull;   // park = 1: the lanes whose step needs a ring-source sum
     // the six stages, each inlined with its index a constant (k stays in registers)
     auto stage = [&](auto stage_index) __attribute__((always_inline)) {
       constexpr int s = decltype(stage_index)::value;
@@ -523,7 +525,8 @@ __device__ __forceinline__ void trace_body(const TraceParams& P) {
       if constexpr (MLP && SOURCES) {   // points sent to the ring sources, summed by the whole wave
         if (wave_mlp) {
           const unsigned long long bm = __ballot(bem);
-          if (bm) bem_wave(P.field.comp[0], bm, ys, lane, E, H);
+          if (P.park) parked |= bm;   // their steps are dropped below; their k are not used
+          else if (bm) bem_wave(P.field.comp[0], bm, ys, lane, E, H);
         }
       }
       const double vxh0 = ys[4] * H[2] - ys[5] * H[1];
@@ -548,7 +551,10 @@ __device__ __forceinline__ void trace_body(const TraceParams& P) {
     }
     const double err = ep + h * ev;
     int st = -1;
-    if (active) {
+    if (active && ((parked >> lane) & 1ull)) {   // retired at the start of the step, to be resumed
+      st = STATUS_PARKED;
+      retire(st);
+    } else if (active) {
       natt++;
       if (err <= P.atol) {
         double yn[6];
@@ -662,6 +668,7 @@ __device__ __forceinline__ void split_retire(const TraceParams& P, const SplitWs
   if (P.n_attempted) P.n_attempted[i] = W.natt[i];
   if (P.t_final) P.t_final[i] = W.t[i];
   if (P.h_final) P.h_final[i] = W.h[i];
+  if (P.hmax_final) P.hmax_final[i] = W.hmax[i];
   if (P.traj_len) P.traj_len[i] = W.tlen[i];
   W.active[i] = 0;
 }
@@ -679,7 +686,7 @@ __global__ __launch_bounds__(256) void k_split_init(TraceParams P, SplitWs W) {
   }
   W.t[i] = P.t0 ? P.t0[i] : 0.0;
   W.qmv[i] = P.qm[P.qm_stride ? i : 0];
-  const double hm = STEP_MAX / sqrt(y[3] * y[3] + y[4] * y[4] + y[5] * y[5]);   // tracing.c:68-70
+  const double hm = (P.hmax0 && P.hmax0[i] > 0.0) ? P.hmax0[i] : STEP_MAX / sqrt(y[3] * y[3] + y[4] * y[4] + y[5] * y[5]);   // tracing.c:68-70
   W.hmax[i] = hm;
   W.h[i] = (P.h0 && P.h0[i] > 0.0) ? P.h0[i] : hm;
   W.kprev[i] = signed_dist(P, y);

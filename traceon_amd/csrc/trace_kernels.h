@@ -9,7 +9,7 @@ constexpr double STEP_MAX = 0.01;   // TRACING_STEP_MAX, tracing.c:5
 
 enum { COMP_BEM = 0, COMP_AXIAL = 1, COMP_UNIFORM = 2 };
 enum { BOUNDS_NONE = 0, BOUNDS_STRICT = 1, BOUNDS_INCLUSIVE = 2 };
-enum { STATUS_LEFT_BOUNDS = 0, STATUS_BLOCK_FULL = 1, STATUS_STEP_CAP = 2 };
+enum { STATUS_LEFT_BOUNDS = 0, STATUS_BLOCK_FULL = 1, STATUS_STEP_CAP = 2, STATUS_PARKED = 3 };
 
 // Flattened ring sources: weight (charge * Gauss weight * jacobian, or current density
 // * weight * area), ring radius, ring height.
@@ -95,6 +95,17 @@ struct TraceParams {
   // occupancy: waves per SIMD the axial-only tracer is compiled for (1-4; 0 picks 2).
   const int* order;
   int batch, occupancy;
+  // hmax0: each particle's step cap (null or <= 0: STEP_MAX / |v| of state0, tracing.c:68-70);
+  // hmax_final: the cap each particle traced with (null: not written), so that a trace
+  // resumed from final_state, t_final and h_final keeps it.
+  const double* hmax0;
+  double* hmax_final;
+  // park = 1 (persistent tracer, a field with one learned component): a particle whose step
+  // needs a ring-source sum (outside the trust region, or where the error check fails) is
+  // retired at the start of that step with STATUS_PARKED (its state, t, h and counts at that
+  // point), instead of its wave summing the sources; the caller finishes the parked particles
+  // on the stage-synchronous tracer, which spreads each sum over the GPU.
+  int park;
 };
 
 // One lane per particle; `waves` 64-lane workgroups drain the particle queue.

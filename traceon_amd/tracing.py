@@ -78,12 +78,14 @@ class TraceParamsT(C.Structure):
                 ("n_attempted", C.c_void_p), ("t_final", C.c_void_p), ("h_final", C.c_void_p), ("traj", C.c_void_p),
                 ("times", C.c_void_p), ("traj_cap", C.c_longlong), ("traj_len", C.c_void_p),
                 ("plane2_p", C.c_double * 3), ("plane2_n", C.c_double * 3), ("plane2_nn", C.c_double), ("landing2", C.c_void_p),
-                ("order", C.c_void_p), ("batch", C.c_int), ("occupancy", C.c_int)]
+                ("order", C.c_void_p), ("batch", C.c_int), ("occupancy", C.c_int),
+                ("hmax0", C.c_void_p), ("hmax_final", C.c_void_p), ("park", C.c_int)]
 
 
 COMP_KIND = {"bem": 0, "axial": 1, "uniform": 2, "learned": 1}
 BOUNDS_NONE, BOUNDS_STRICT, BOUNDS_INCLUSIVE = 0, 1, 2
 STATUS = {0: "left bounds", 1: "block full", 2: "step cap"}
+STATUS_PARKED = 3   # tb_trace with park = 1 only; run_trace resumes such particles, so callers never see it
 
 
 def _set_sources(dst: SourcesT, charges):
@@ -205,6 +207,7 @@ class BatchResult:
         self.n_accepted = n_accepted
         self.n_attempted = n_attempted
         self.t_final = t_final
+        self.n_parked = 0   # particles the persistent tracer handed to the stage-synchronous one
 
 
 # Below this many particles a field with ring sources is traced stage-synchronously
@@ -244,12 +247,16 @@ def queue_order(S0):
 
 
 def run_trace(program: DeviceFieldProgram, state0, qm, bounds, atol=1e-8, plane=None, traj_cap=0, t0=None,
-              max_attempts=1 << 62, waves=0, split=None, plane2=None, shape: LaunchShape | None = None):
+              max_attempts=1 << 62, waves=0, split=None, plane2=None, shape: LaunchShape | None = None,
+              park: bool | None = None):
     """GPU tracer over many particles (all arrays numpy or device tensors).
     plane, plane2: (p0, normal); the state at each one's last crossing (NaN if none).
     split: None picks tb_trace_split for fields with ring sources (a learned field's are its
     fallback) and at most SPLIT_MAX_PARTICLES particles, else the persistent tb_trace.  shape: launch shape of
-    the persistent tracer (None: DEFAULT_SHAPE); waves > 0 overrides shape.waves."""
+    the persistent tracer (None: DEFAULT_SHAPE); waves > 0 overrides shape.waves.  On the persistent
+    tracer, particles of a learned field whose step needs the ring sources are parked there and
+    finished on tb_trace_split (result.n_parked of them); park=False sums the sources in the
+    persistent kernel instead (None: park where it applies)."""
     import torch
     from . import device
     P, out = trace_params(program, state0, qm, bounds, atol, plane, traj_cap, t0, max_attempts, plane2)
@@ -259,9 +266,7 @@ def run_trace(program: DeviceFieldProgram, state0, qm, bounds, atol=1e-8, plane=
     if split is None:
         split = program.has_sources and n <= SPLIT_MAX_PARTICLES
     if split:
-        nchunk = L.tb_trace_split_chunks(n, program.n_sources)
-        ws = torch.empty(L.tb_trace_split_workspace_bytes(n, nchunk), dtype=torch.uint8, device=dev)
-        backend.check(L.tb_trace_split(C.byref(P), nchunk, ws.data_ptr(), device.stream_ptr()), "trace_split")
+        _trace_split(program, P, n, dev)
     else:
         shape = shape or DEFAULT_SHAPE
         if shape.ordered and n >= ORDER_MIN_PARTICLES:
@@ -269,9 +274,58 @@ def run_trace(program: DeviceFieldProgram, state0, qm, bounds, atol=1e-8, plane=
             P.order = order.data_ptr()
         P.batch = int(shape.batch)
         P.occupancy = int(shape.occupancy)
+        # a learned field's fallback sums: the persistent kernel parks the particles that need
+        # one and the stage-synchronous tracer finishes them (not for recorded trajectories or
+        # a step cap, which a resumed trace would count from zero)
+        can_park = program.has_mlp and program.has_sources and not traj_cap and max_attempts >= (1 << 62)
+        park = can_park if park is None else (park and can_park)
+        if park:
+            hfin = torch.empty(n, dtype=torch.float64, device=dev)
+            hmaxf = torch.empty(n, dtype=torch.float64, device=dev)
+            P.h_final, P.hmax_final, P.park = hfin.data_ptr(), hmaxf.data_ptr(), 1
         backend.check(L.tb_trace(C.byref(P), int(waves or shape.waves), device.stream_ptr()), "trace")
+        if park:
+            _resume_parked(program, out, hfin, hmaxf, qm, bounds, atol, plane, plane2, max_attempts)
     out._keep = out._keep + (order,)
     return out
+
+
+def _trace_split(program: DeviceFieldProgram, P, n, dev):
+    import torch
+    from . import device
+    L = backend.backend_lib
+    nchunk = L.tb_trace_split_chunks(n, program.n_sources)
+    ws = torch.empty(L.tb_trace_split_workspace_bytes(n, nchunk), dtype=torch.uint8, device=dev)
+    backend.check(L.tb_trace_split(C.byref(P), nchunk, ws.data_ptr(), device.stream_ptr()), "trace_split")
+    return ws
+
+
+def _resume_parked(program, out, hfin, hmaxf, qm, bounds, atol, plane, plane2, max_attempts):
+    """Finish the particles that tb_trace parked (status STATUS_PARKED: state, t, h and counts
+    at the start of the step that needed ring sources) on the stage-synchronous tracer, from
+    that state with the same step size and step cap, in batches of SPLIT_MAX_PARTICLES; their
+    counts add up and a plane crossed after the resume replaces one crossed before it."""
+    import torch
+    idx = torch.nonzero(out.status == STATUS_PARKED).flatten()
+    out.n_parked = int(idx.numel())
+    qm_arr = np.atleast_1d(np.asarray(qm, dtype=np.float64))
+    for c0 in range(0, out.n_parked, SPLIT_MAX_PARTICLES):
+        sel = idx[c0:c0 + SPLIT_MAX_PARTICLES]
+        m = int(sel.numel())
+        qm_sel = qm_arr[sel.cpu().numpy()] if qm_arr.size > 1 else qm_arr
+        S, H, HM = out.final[sel].contiguous(), hfin[sel].contiguous(), hmaxf[sel].contiguous()
+        P2, o2 = trace_params(program, S, qm_sel, bounds, atol, plane, 0, out.t_final[sel].cpu().numpy(), max_attempts, plane2)
+        P2.h0, P2.hmax0 = H.data_ptr(), HM.data_ptr()
+        ws = _trace_split(program, P2, m, S.device)
+        out.final[sel] = o2.final
+        out.status[sel] = o2.status
+        out.t_final[sel] = o2.t_final
+        out.n_accepted[sel] += o2.n_accepted
+        out.n_attempted[sel] += o2.n_attempted
+        for mine, theirs in ((out.landing, o2.landing), (out.landing2, o2.landing2)):
+            if mine is not None:
+                mine[sel] = torch.where(torch.isfinite(theirs[:, :1]), theirs, mine[sel])
+        del ws, S, H, HM, o2
 
 
 def trace_params(program: DeviceFieldProgram, state0, qm, bounds, atol=1e-8, plane=None, traj_cap=0, t0=None,
