@@ -82,24 +82,41 @@ __global__ __launch_bounds__(256) void k_fill(double* __restrict__ M, size_t ld,
 #pragma unroll
   for (int t = 0; t < ROWS; t++) acc[t] = 0.0;
 
-#pragma unroll 2
-  for (int k = 0; k < NQ; k++) {
-    const double r = sr[(size_t)k * N + jj], z = sz[(size_t)k * N + jj], w = sw[(size_t)k * N + jj];
+  // a block of potential rows only (every row of a Dirichlet geometry) runs a loop without
+  // the per-row branches, whose joins cost register copies of the accumulators in every pair
+  bool pot_only = ib + ROWS - 1 <= row1;
 #pragma unroll
-    for (int t = 0; t < ROWS; t++) {
-      const int i = ib + t;
-      if (i <= row1) {
-        const RowInfo R = rows[i];
-        const double dr = r - R.r0, dz = z - R.z0;
-        if (R.kind == 0) {
-          acc[t] = fma(w, ring_potential(R.r0, dr, dz), acc[t]);
-        } else if (R.kind == 1) {
-          double gr, gz;
-          ring_grad(R.r0, dr, dz, gr, gz);
-          // field_dot_normal_radial: factor * (n . E), E = -grad
-          acc[t] = fma(w, -R.fac * (R.nr * gr + R.nz * gz), acc[t]);
-        } else {
-          acc[t] = __builtin_nan("");
+  for (int t = 0; t < ROWS; t++) pot_only = pot_only && rows[min(ib + t, row1)].kind == 0;
+  if (pot_only) {
+    double r0[ROWS], z0[ROWS];
+#pragma unroll
+    for (int t = 0; t < ROWS; t++) { r0[t] = rows[ib + t].r0; z0[t] = rows[ib + t].z0; }
+#pragma unroll 2
+    for (int k = 0; k < NQ; k++) {
+      const double r = sr[(size_t)k * N + jj], z = sz[(size_t)k * N + jj], w = sw[(size_t)k * N + jj];
+#pragma unroll
+      for (int t = 0; t < ROWS; t++) acc[t] = fma(w, ring_potential(r0[t], r - r0[t], z - z0[t]), acc[t]);
+    }
+  } else {
+#pragma unroll 2
+    for (int k = 0; k < NQ; k++) {
+      const double r = sr[(size_t)k * N + jj], z = sz[(size_t)k * N + jj], w = sw[(size_t)k * N + jj];
+#pragma unroll
+      for (int t = 0; t < ROWS; t++) {
+        const int i = ib + t;
+        if (i <= row1) {
+          const RowInfo R = rows[i];
+          const double dr = r - R.r0, dz = z - R.z0;
+          if (R.kind == 0) {
+            acc[t] = fma(w, ring_potential(R.r0, dr, dz), acc[t]);
+          } else if (R.kind == 1) {
+            double gr, gz;
+            ring_grad(R.r0, dr, dz, gr, gz);
+            // field_dot_normal_radial: factor * (n . E), E = -grad
+            acc[t] = fma(w, -R.fac * (R.nr * gr + R.nz * gz), acc[t]);
+          } else {
+            acc[t] = __builtin_nan("");
+          }
         }
       }
     }

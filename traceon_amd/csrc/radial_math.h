@@ -86,6 +86,16 @@ TB_HD double horner_eb(double p) {
   return s * p;
 }
 
+// A wave-uniform FP64 constant held in a scalar register pair where it is used: the FMA of a
+// Horner step then takes it as its addend operand directly, where a constant kept in vector
+// registers makes the compiler copy it into the accumulating FMA's destination at every step.
+TB_HD double sconst(double c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm("" : "+s"(c));
+#endif
+  return c;
+}
+
 // ln(1/p) for p > 0 (finite), from the mathematics: p = m 2^e with m in [1/sqrt2, sqrt2),
 // ln m = 2 atanh(s), s = (m-1)/(m+1), |s| <= 0.1716, and the atanh series
 // s + s^3/3 + ... + s^19/19 in Horner form over s^2 (the first omitted term, s^21/21, is
@@ -112,10 +122,10 @@ TB_HD double neg_log(double p) {
 #endif
   const double s2 = s * s;
   double q = 1.0 / 19.0;
-  q = fma(q, s2, 1.0 / 17.0);
-  q = fma(q, s2, 1.0 / 15.0); q = fma(q, s2, 1.0 / 13.0); q = fma(q, s2, 1.0 / 11.0);
-  q = fma(q, s2, 1.0 / 9.0);  q = fma(q, s2, 1.0 / 7.0);  q = fma(q, s2, 1.0 / 5.0);
-  q = fma(q, s2, 1.0 / 3.0);
+  q = fma(q, s2, sconst(1.0 / 17.0));
+  q = fma(q, s2, sconst(1.0 / 15.0)); q = fma(q, s2, sconst(1.0 / 13.0)); q = fma(q, s2, sconst(1.0 / 11.0));
+  q = fma(q, s2, sconst(1.0 / 9.0));  q = fma(q, s2, sconst(1.0 / 7.0));  q = fma(q, s2, sconst(1.0 / 5.0));
+  q = fma(q, s2, sconst(1.0 / 3.0));
   const double lnm = fma(2.0 * s * s2, q, 2.0 * s);
   const double de = (double)e;
   // ln 2 split in a high part exact in its leading bits and a low correction
