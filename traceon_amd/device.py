@@ -216,8 +216,9 @@ def solve_system(M64: torch.Tensor | None, F: np.ndarray | torch.Tensor, M32: to
                   cond(M) * eps32 < 1; when it does not reach the rounding level of x (it
                   stalls or diverges) and ``fallback`` is set, the system is solved again
                   with an FP64 LU, and the stats report mode "mixed->fp64".
-    Refinement stops when the update is at the rounding level of x (2 eps |x|) or stops
-    shrinking.  gemm_timing: see lu_factor.  Returns (X (nrhs,N) FP64 tensor, SolveStats, LU)."""
+    Refinement stops when the update is at the rounding level of x (2 eps |x|), stops
+    shrinking, or shrinks by a factor rho per step such that the rest of the geometric series,
+    dmax rho / (1 - rho), is below that level.  gemm_timing: see lu_factor.  Returns (X (nrhs,N) FP64 tensor, SolveStats, LU)."""
     Bt = F if isinstance(F, torch.Tensor) else to_dev(np.atleast_2d(F))
     Bt = Bt.reshape(-1, Bt.shape[-1]).contiguous()
     if mode == "fp64":
@@ -259,7 +260,12 @@ def solve_system(M64: torch.Tensor | None, F: np.ndarray | torch.Tensor, M32: to
             if not math.isfinite(dmax) or (dmax >= 0.5 * last and dmax > 64.0 * eps * xmax):
                 break                      # diverging or stalled above the rounding level
             check(_L.tb_axpy(ptr(X), ptr(R), X.numel(), stream_ptr()), "axpy")
-            if dmax <= 2.0 * eps * xmax or dmax >= 0.5 * last:
+            # converged: the update is at the rounding level, or it stopped shrinking, or the
+            # updates contract by rho = dmax / last so that the next would be below the
+            # rounding level (dmax rho / (1 - rho) bounds the rest of the series)
+            rho = dmax / last if math.isfinite(last) and last > 0.0 else 1.0
+            tail_below = rho < 1.0 and dmax * rho / (1.0 - rho) <= 2.0 * eps * xmax
+            if dmax <= 2.0 * eps * xmax or dmax >= 0.5 * last or tail_below:
                 converged = True
                 break
             last = dmax
