@@ -108,12 +108,13 @@ TB_HD double neg_log(double p) {
   double m = frexp(p, &e);            // m in [0.5, 1)
   if (m < 0.70710678118654752440) { m *= 2.0; e -= 1; }
 #if defined(__HIP_DEVICE_COMPILE__)
-  // (m - 1) / (m + 1) with the denominator in [1.7, 2.5]: the hardware reciprocal refined by
-  // two Newton steps and the quotient corrected once (no scaling or special cases in this
-  // range, which the general division sequence spends four instructions on)
+  // (m - 1) / (m + 1) with the denominator in [1.7, 2.5]: the hardware reciprocal (relative
+  // error below 2^-24 here, measured over 2^24 points of the range) refined by one Newton step
+  // (error below 2^-48) and the quotient corrected once, which squares that error again (no
+  // scaling or special cases in this range, which the general division sequence spends four
+  // instructions on; a second Newton step changed none of the 2^24 quotients)
   const double dn = m + 1.0, nm = m - 1.0;
   double y = __builtin_amdgcn_rcp(dn);
-  y = fma(fma(-dn, y, 1.0), y, y);
   y = fma(fma(-dn, y, 1.0), y, y);
   double s = nm * y;
   s = fma(fma(-dn, s, nm), y, s);
