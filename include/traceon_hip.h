@@ -252,7 +252,9 @@ typedef struct {
 } tb_field_comp_t;
 /* flags: 1 = some component carries a learned correction; 2 = no component has ring sources
    (axial and uniform fields only: the tracer runs its variant without source sums, and a BEM
-   component in such a program evaluates to NaN) */
+   component in such a program evaluates to NaN); 4 = the program is one untransformed learned
+   component with BF16 weights (its ring sources serve only its fallback): with park = 1,
+   tb_trace runs a variant without source sums and parks every particle that needs them */
 typedef struct { const tb_field_comp_t *comp; int ncomp, bounds_mode; int flags, pad; double bounds[6]; } tb_field_program_t;
 typedef struct {
   tb_field_program_t field;

@@ -113,9 +113,11 @@ constexpr int TMLP_B1 = 256, TMLP_W2 = 384, TMLP_W3 = TMLP_W2 + 16512, TMLP_W4 =
 // LDS of the learned correction (the workgroups that evaluate corrections are one wave):
 // mlp_correct: g_mlp_act[layer parity][neuron][lane], BF16; mlp_wave: activations A0 and
 // A1 (64 points x 136 BF16, the row stride that keeps the MFMA operand reads conflict-free)
-// and, in place of A1, the FP32 output of the last hidden layer (64 x 129 floats).
-constexpr int MW_S = 136, MW_SF = 129;
-__shared__ __attribute__((aligned(16))) unsigned char g_mlp_lds[64 * MW_S * 2 + 64 * MW_SF * 4];
+// and, in place of A1, the FP32 output of the last hidden layer, 64 of its neurons at a time
+// (64 x 65 floats).  34 KiB: four one-wave workgroups per CU.
+constexpr int MW_S = 136, MW_SF = 65;
+static_assert(64 * MW_SF * 4 <= 64 * MW_S * 2, "the last layer's half fits in place of A1");
+__shared__ __attribute__((aligned(16))) unsigned char g_mlp_lds[64 * MW_S * 2 * 2];
 #define g_mlp_act (reinterpret_cast<uint16_t*>(g_mlp_lds))
 typedef const __attribute__((address_space(4))) float cfloat_t;   // read-only weights: scalar loads
 
@@ -237,12 +239,19 @@ __device__ void mlp_wave(const FieldComp& c, const double q[3], bool need, int l
   }
   __syncthreads();
   const int lr = lane & 15, lq = lane >> 4;
-  for (int layer = 0; layer < 3; layer++) {
+  float y[5];
+#pragma unroll
+  for (int o = 0; o < 5; o++) y[o] = W[TMLP_W5 + 640 + o];
+  // layers 0-1: all 128 neurons; layer 2 in two passes of 64 neurons, each followed by its
+  // part of the output layer (per lane on its own point, neurons in order 0..127)
+  for (int pass = 0; pass < 4; pass++) {
+    const int layer = pass < 2 ? pass : 2;
     const uint16_t* Ain = layer == 1 ? A1 : A0;
     uint16_t* Aout = layer == 0 ? A1 : A0;
     const uint16_t* Wl = c.mlp_wbf + (size_t)layer * 16384;
     const float* bias = c.mlp + TMLP_W2 + layer * 16512 + 16384;
-    for (int nb = 0; nb < 8; nb++) {
+    const int nb0 = pass == 3 ? 4 : 0, nb1 = pass == 2 ? 4 : 8;
+    for (int nb = nb0; nb < nb1; nb++) {
       mw_f4 acc[4];
 #pragma unroll
       for (int mb = 0; mb < 4; mb++) acc[mb] = mw_f4{0.f, 0.f, 0.f, 0.f};
@@ -264,21 +273,20 @@ __device__ void mlp_wave(const FieldComp& c, const double q[3], bool need, int l
           const int pt = 16 * mb + 4 * lq + vv;
           const float a = silu(acc[mb][vv] + bn);
           if (layer < 2) Aout[pt * MW_S + n] = f2bf(a);
-          else Fo[pt * MW_SF + n] = a;
+          else Fo[pt * MW_SF + n - 16 * nb0] = a;
         }
     }
     __syncthreads();
-  }
-  // output layer, per lane on its own point
-  float y[5];
+    if (layer == 2) {   // output layer over this pass's 64 neurons
+      const int n0 = 16 * nb0;
+      for (int n = 0; n < 64; n++) {
+        const float hv = Fo[lane * MW_SF + n];
 #pragma unroll
-  for (int o = 0; o < 5; o++) y[o] = W[TMLP_W5 + 640 + o];
-  for (int n = 0; n < 128; n++) {
-    const float hv = Fo[lane * MW_SF + n];
-#pragma unroll
-    for (int o = 0; o < 5; o++) y[o] = fmaf(W[TMLP_W5 + o * 128 + n], hv, y[o]);
+        for (int o = 0; o < 5; o++) y[o] = fmaf(W[TMLP_W5 + o * 128 + n0 + n], hv, y[o]);
+      }
+      __syncthreads();   // the LDS is reused by the next pass or evaluation
+    }
   }
-  __syncthreads();   // the LDS is reused by the next evaluation
   if (!need) return;
   // s_k(z): linear between bin centres; w(r)
   const int nz = c.mlp_nz;
@@ -341,7 +349,8 @@ __device__ void eval_field(const FieldProgram& F, const double p[3], double E[3]
       }
       if (MLP && use_bem) {
         if (WAVE && bem_defer) *bem_defer = true;
-        else bem_component(c, q, e, h, s0, ds);
+        else if constexpr (SOURCES) bem_component(c, q, e, h, s0, ds);
+        else e[0] = e[1] = e[2] = h[0] = h[1] = h[2] = __builtin_nan("");
       } else if (lead) {
         const int parts = c.axial_parts ? c.axial_parts : 3;
         if (parts & 1) axial_field(c.zs, c.nz, c.ce, q[0], q[1], q[2], e);
@@ -528,6 +537,8 @@ __device__ __forceinline__ void trace_body(const TraceParams& P) {
           if (P.park) parked |= bm;   // their steps are dropped below; their k are not used
           else if (bm) bem_wave(P.field.comp[0], bm, ys, lane, E, H);
         }
+      } else if constexpr (MLP) {   // no source-sum code: such points always park
+        if (wave_mlp) parked |= __ballot(bem);
       }
       const double vxh0 = ys[4] * H[2] - ys[5] * H[1];
       const double vxh1 = ys[5] * H[0] - ys[3] * H[2];
@@ -605,6 +616,14 @@ __global__ __launch_bounds__(64) void k_trace(TraceParams P) { trace_body<MLP, t
 template <int W>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) void k_trace_axial(TraceParams P) {
   trace_body<false, false>(P);
+}
+
+// A learned field whose ring sources serve only its fallback (PROGRAM_LEARNED_ONLY), traced
+// with park = 1: the points that need them park their particles, so the kernel carries no
+// source-sum code, and W waves per SIMD as the axial-only kernel.
+template <int W>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) void k_trace_learned(TraceParams P) {
+  trace_body<true, false>(P);
 }
 
 // ---------------------------------------------------------------------------------
@@ -978,7 +997,10 @@ hipError_t launch_trace(const TraceParams& P, int waves, hipStream_t st) {
   int nw = waves > 0 ? waves : 1;
   const int need = (P.n + 63) / 64;
   if (nw > need) nw = need;
-  if (P.field.flags & PROGRAM_HAS_MLP) hipLaunchKernelGGL(k_trace<true>, dim3(nw), dim3(64), 0, st, P);
+  if ((P.field.flags & PROGRAM_HAS_MLP) && (P.field.flags & PROGRAM_LEARNED_ONLY) && P.park) {
+    if (P.occupancy == 1) hipLaunchKernelGGL(k_trace_learned<1>, dim3(nw), dim3(64), 0, st, P);
+    else hipLaunchKernelGGL(k_trace_learned<2>, dim3(nw), dim3(64), 0, st, P);
+  } else if (P.field.flags & PROGRAM_HAS_MLP) hipLaunchKernelGGL(k_trace<true>, dim3(nw), dim3(64), 0, st, P);
   else if (P.field.flags & PROGRAM_NO_SOURCES) {
     switch (P.occupancy) {
       case 1: hipLaunchKernelGGL(k_trace_axial<1>, dim3(nw), dim3(64), 0, st, P); break;

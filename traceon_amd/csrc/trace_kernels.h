@@ -51,12 +51,15 @@ struct FieldComp {
 // PROGRAM_HAS_MLP: some component carries a learned correction.  PROGRAM_NO_SOURCES: no
 // component has ring sources (axial and uniform fields only): the tracer then runs a
 // variant without the source-sum code (far fewer registers); a BEM component in such a
-// program evaluates to NaN rather than being skipped.
-enum { PROGRAM_HAS_MLP = 1, PROGRAM_NO_SOURCES = 2 };
+// program evaluates to NaN rather than being skipped.  PROGRAM_LEARNED_ONLY: the program is
+// one untransformed learned component with BF16 weights, whose ring sources serve only its
+// fallback; with TraceParams::park set the persistent tracer parks every particle that needs
+// them and runs a variant without the source-sum code.
+enum { PROGRAM_HAS_MLP = 1, PROGRAM_NO_SOURCES = 2, PROGRAM_LEARNED_ONLY = 4 };
 struct FieldProgram {
   const FieldComp* comp;   // device array
   int ncomp, bounds_mode;
-  int flags, pad;          // PROGRAM_HAS_MLP, PROGRAM_NO_SOURCES
+  int flags, pad;          // PROGRAM_HAS_MLP, PROGRAM_NO_SOURCES, PROGRAM_LEARNED_ONLY
   double bounds[6];
 };
 

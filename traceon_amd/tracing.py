@@ -67,7 +67,7 @@ class FieldProgramT(C.Structure):
                 ("bounds", C.c_double * 6)]
 
 
-PROGRAM_HAS_MLP, PROGRAM_NO_SOURCES = 1, 2
+PROGRAM_HAS_MLP, PROGRAM_NO_SOURCES, PROGRAM_LEARNED_ONLY = 1, 2, 4
 
 
 class TraceParamsT(C.Structure):
@@ -177,6 +177,10 @@ class DeviceFieldProgram:
         self.prog.ncomp = n
         # a program without ring sources runs the tracer variant without source-sum code
         self.prog.flags = (PROGRAM_HAS_MLP if self.has_mlp else 0) | (0 if self.has_sources else PROGRAM_NO_SOURCES)
+        c0 = comps[0][0] if len(comps) == 1 else None
+        if c0 is not None and c0.mlp and c0.mlp_wbf and not c0.transform:
+            # one learned component: its sources serve only the fallback, which parks
+            self.prog.flags |= PROGRAM_LEARNED_ONLY
         # the superposition's own bounds (field.py:285-290 on the top-level object)
         if isinstance(field, FieldSuperposition) and field.field_bounds is not None:
             self.prog.bounds_mode = BOUNDS_INCLUSIVE
