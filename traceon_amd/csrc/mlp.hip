@@ -23,7 +23,7 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 // C (+)= op(A) op(B), A (M,K) and B (K,N) in BF16 through strides:
 //   A(m,k) = A[m*sam + k*sak],  B(k,n) = B[k*sbk + n*sbn],  C[m*ldc + n] FP32.
 // 64 x 64 tile per workgroup of 4 waves (32 x 32 each = 2 x 2 MFMA 16x16x16), K staged in
-// chunks of 32 through LDS.  An operand that is contiguous along k or along its row/column
+// chunks of 128 through LDS (a hidden layer's whole K in one chunk).  An operand that is contiguous along k or along its row/column
 // index is staged with one 16-byte load per thread (8 elements) where the group of 8 lies
 // inside the matrix and is 16-byte aligned, else element by element.  gridDim.z splits K:
 // with `part` each split stores its M x N result at part + z M N (k_ksplit_reduce then adds
@@ -35,32 +35,51 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 // C = acc + aux[col] (aux: the layer's bias), outbf = bf16(silu(C)); EPI_DSILU
 // C = acc * silu'(aux[row][col]) (aux: the layer's pre-activations), outbf = bf16(C).
 // ---------------------------------------------------------------------------------
-constexpr int MG_T = 64, MG_K = 32, MG_PAD = 8;
+constexpr int MG_T = 64, MG_K = 128, MG_PAD = 8;
 enum { EPI_STORE = 0, EPI_BIAS_SILU = 1, EPI_DSILU = 2 };
 
-// one 64 x 32 operand tile into S[64][MG_K + MG_PAD] (S[r][k] = X(r0 + r, k0 + k)), X(r, k) =
-// X[r * sr + k * sk], rows below R and k below kend, zero elsewhere
+// eight 16-bit values f(0..7) packed into 16 bytes, element i in halfword i
+template <class F>
+__device__ __forceinline__ uint4 pack8(F f) {
+  unsigned w[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) w[i] = (unsigned)f(2 * i) | ((unsigned)f(2 * i + 1) << 16);
+  return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// one 64 x MG_K operand tile into S[64][MG_K + MG_PAD] (S[r][k] = X(r0 + r, k0 + k)), X(r, k) =
+// X[r * sr + k * sk], rows below R and k below kend, zero elsewhere.  The tile is staged in
+// MG_K / 32 passes of 64 x 32; the vector paths issue every pass's 16-byte load before the
+// first LDS store, so one memory latency covers the whole tile.
 __device__ __forceinline__ void stage_tile(uint16_t (*S)[MG_K + MG_PAD], const uint16_t* __restrict__ X, long sr, long sk, int r0,
                                            int R, int k0, int kend, int tid) {
+  constexpr int NP = MG_K / 32;
   const bool vec_ok = ((reinterpret_cast<uintptr_t>(X) & 15) == 0);
-  if (sk == 1 && vec_ok && (sr & 7) == 0) {          // rows contiguous in k: thread = (row, 8 k)
-    const int r = tid >> 2, kq = (tid & 3) * 8, gr = r0 + r, gk = k0 + kq;
-    if (gr < R && gk + 8 <= kend && (gk & 7) == 0) {
-      *reinterpret_cast<uint4*>(&S[r][kq]) = *reinterpret_cast<const uint4*>(X + gr * sr + gk);
-    } else {
+  if (sk == 1 && vec_ok && (sr & 7) == 0) {          // rows contiguous in k: thread = (row, 8 k) per pass
+    const int r = tid >> 2, kq = (tid & 3) * 8, gr = r0 + r;
+    uint4 v[NP];
 #pragma unroll
-      for (int i = 0; i < 8; i++) S[r][kq + i] = (gr < R && gk + i < kend) ? X[gr * sr + gk + i] : (uint16_t)0;
+    for (int p = 0; p < NP; p++) {
+      const int gk = k0 + 32 * p + kq;
+      if (gr < R && gk + 8 <= kend && (gk & 7) == 0) v[p] = *reinterpret_cast<const uint4*>(X + gr * sr + gk);
+      else v[p] = pack8([&](int i) { return (gr < R && gk + i < kend) ? X[gr * sr + gk + i] : (uint16_t)0; });
     }
-  } else if (sr == 1 && vec_ok && (sk & 7) == 0) {   // columns contiguous in r: thread = (k, 8 rows)
-    const int k = tid >> 3, rq = (tid & 7) * 8, gr = r0 + rq, gk = k0 + k;
-    if (gk < kend && gr + 8 <= R && (gr & 7) == 0) {
-      const uint4 v = *reinterpret_cast<const uint4*>(X + gk * sk + gr);
-      const uint16_t* e = reinterpret_cast<const uint16_t*>(&v);
 #pragma unroll
-      for (int i = 0; i < 8; i++) S[rq + i][k] = e[i];
-    } else {
+    for (int p = 0; p < NP; p++) *reinterpret_cast<uint4*>(&S[r][32 * p + kq]) = v[p];
+  } else if (sr == 1 && vec_ok && (sk & 7) == 0) {   // columns contiguous in r: thread = (k, 8 rows) per pass
+    const int kl = tid >> 3, rq = (tid & 7) * 8, gr = r0 + rq;
+    uint4 v[NP];
 #pragma unroll
-      for (int i = 0; i < 8; i++) S[rq + i][k] = (gk < kend && gr + i < R) ? X[gk * sk + gr + i] : (uint16_t)0;
+    for (int p = 0; p < NP; p++) {
+      const int gk = k0 + 32 * p + kl;
+      if (gk < kend && gr + 8 <= R && (gr & 7) == 0) v[p] = *reinterpret_cast<const uint4*>(X + gk * sk + gr);
+      else v[p] = pack8([&](int i) { return (gk < kend && gr + i < R) ? X[gk * sk + gr + i] : (uint16_t)0; });
+    }
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+      const unsigned w[4] = {v[p].x, v[p].y, v[p].z, v[p].w};
+#pragma unroll
+      for (int i = 0; i < 8; i++) S[rq + i][32 * p + kl] = (uint16_t)(w[i >> 1] >> (16 * (i & 1)));
     }
   } else {
     for (int e = tid; e < MG_T * MG_K; e += 256) {
@@ -86,6 +105,21 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, const ui
   for (int i = 0; i < 2; i++)
 #pragma unroll
     for (int j = 0; j < 2; j++) acc[i][j] = f4v{0.f, 0.f, 0.f, 0.f};
+  // the epilogue's operand (bias or pre-activation) loaded before the products, so its memory
+  // latency overlaps the operand staging
+  float pre[2][2][4];
+  if (gridDim.z == 1 && epi != EPI_STORE) {
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int row = m0 + wm + i * 16 + 4 * (lane >> 4) + v;
+          const int col = n0 + wn + j * 16 + (lane & 15);
+          pre[i][j][v] = (row < M && col < N) ? (epi == EPI_DSILU ? aux[(long)row * ldc + col] : aux[col]) : 0.f;
+        }
+  }
   for (int k0 = kbeg; k0 < kend; k0 += MG_K) {
     stage_tile(As, A, sam, sak, m0, M, k0, kend, tid);
     stage_tile(Bs, B, sbn, sbk, n0, N, k0, kend, tid);     // B^T rows: n, contiguous along k when sbk == 1
@@ -122,11 +156,11 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, const ui
             if (part) part[((long)blockIdx.z * M + row) * N + col] = x;
             else atomicAdd(C + o, x);
           } else if (epi == EPI_BIAS_SILU) {
-            const float z = x + aux[col];
+            const float z = x + pre[i][j][v];
             C[o] = z;
             outbf[o] = f2bf(silu(z));
           } else if (epi == EPI_DSILU) {
-            const float dz = x * dsilu(aux[o]);
+            const float dz = x * dsilu(pre[i][j][v]);
             C[o] = dz;
             outbf[o] = f2bf(dz);
           } else {
@@ -144,7 +178,17 @@ __global__ __launch_bounds__(256) void k_ksplit_reduce(int M, int N, int kz, con
   if (e >= (long)M * N) return;
   const int row = (int)(e / N), col = (int)(e % N);
   float x = accumulate ? C[(long)row * ldc + col] : 0.f;
-  for (int z = 0; z < kz; z++) x += part[(long)z * M * N + e];
+  // eight loads in flight, then their sums in order (a load-add chain waits one memory latency
+  // per split)
+  int z = 0;
+  for (; z + 8 <= kz; z += 8) {
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = part[(long)(z + u) * M * N + e];
+#pragma unroll
+    for (int u = 0; u < 8; u++) x += v[u];
+  }
+  for (; z < kz; z++) x += part[(long)z * M * N + e];
   C[(long)row * ldc + col] = x;
 }
 
@@ -230,9 +274,10 @@ __global__ __launch_bounds__(256) void k_out_bwd(int B, const float* __restrict_
 //   mode 2 (bias):    db[j]     = sum_b dZ[b][j]
 // k_col_partials: workgroup g sums rows [g R, (g + 1) R) (R = COL_ROWS), thread t column
 // t % 128 over the rows of parity t / 128 (coalesced rows), into part[g][j][0..5];
-// k_col_reduce adds the partials of every column in a fixed order (deterministic): eight
-// threads per column take every eighth workgroup's partial, then the eight sums in order.
-constexpr int COL_ROWS = 64, COL_Q = MLP_OUT + 1, COL_SPLIT = 8;
+// k_col_reduce adds the partials of every column in a fixed order (deterministic): a
+// workgroup per 16 columns, 64 threads per column each taking every 64th workgroup's partial,
+// then the 64 sums in a fixed tree (eight groups of eight).
+constexpr int COL_ROWS = 64, COL_Q = MLP_OUT + 1, COL_SPLIT = 64, COL_RJ = 16, COL_RB = 8;
 __global__ __launch_bounds__(256) void k_col_partials(int B, int mode, const float* __restrict__ dY, const float* __restrict__ Z4,
                                                       const float* __restrict__ dZ, const float* __restrict__ X,
                                                       float* __restrict__ part) {
@@ -242,7 +287,51 @@ __global__ __launch_bounds__(256) void k_col_partials(int B, int mode, const flo
   float s[COL_Q];
 #pragma unroll
   for (int q = 0; q < COL_Q; q++) s[q] = 0.f;
-  for (int b = b0 + par; b < b1; b += 2) {
+  // rows in batches of COL_RB: the batch's loads issued together, then added row by row in
+  // order (the same sums as the plain loop, without a memory latency per row)
+  int b = b0 + par;
+  if (mode == 0) {
+    for (; b + 2 * (COL_RB - 1) < b1; b += 2 * COL_RB) {
+      float h[COL_RB], y[COL_RB][MLP_OUT], yj[COL_RB];
+#pragma unroll
+      for (int u = 0; u < COL_RB; u++) {
+        const long bb = b + 2 * u;
+        h[u] = Z4[bb * MLP_W + j];
+#pragma unroll
+        for (int o = 0; o < MLP_OUT; o++) y[u][o] = dY[bb * MLP_OUT + o];
+        yj[u] = j < MLP_OUT ? dY[bb * MLP_OUT + j] : 0.f;
+      }
+#pragma unroll
+      for (int u = 0; u < COL_RB; u++) {
+        const float hs = silu(h[u]);
+#pragma unroll
+        for (int o = 0; o < MLP_OUT; o++) s[o] = fmaf(y[u][o], hs, s[o]);
+        if (j < MLP_OUT) s[MLP_OUT] += yj[u];
+      }
+    }
+  } else {
+    for (; b + 2 * (COL_RB - 1) < b1; b += 2 * COL_RB) {
+      float d[COL_RB], x0[COL_RB], x1[COL_RB];
+#pragma unroll
+      for (int u = 0; u < COL_RB; u++) {
+        const long bb = b + 2 * u;
+        d[u] = dZ[bb * MLP_W + j];
+        if (mode == 1) {
+          x0[u] = X[2 * bb];
+          x1[u] = X[2 * bb + 1];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < COL_RB; u++) {
+        if (mode == 1) {
+          s[0] = fmaf(d[u], x0[u], s[0]);
+          s[1] = fmaf(d[u], x1[u], s[1]);
+        }
+        s[2] += d[u];
+      }
+    }
+  }
+  for (; b < b1; b += 2) {
     if (mode == 0) {
       const float h = silu(Z4[(long)b * MLP_W + j]);
 #pragma unroll
@@ -266,25 +355,50 @@ __global__ __launch_bounds__(256) void k_col_partials(int B, int mode, const flo
     for (int q = 0; q < COL_Q; q++) part[((long)blockIdx.x * MLP_W + j) * COL_Q + q] = s[q] + red[j][q];
 }
 
-__global__ __launch_bounds__(MLP_W * COL_SPLIT) void k_col_reduce(int nblk, int mode, const float* __restrict__ part,
-                                                                    float* __restrict__ G, int bias_off) {
-  __shared__ float red[COL_SPLIT][MLP_W][COL_Q];
-  const int j = threadIdx.x % MLP_W, h = threadIdx.x / MLP_W;
+__global__ __launch_bounds__(COL_RJ * COL_SPLIT) void k_col_reduce(int nblk, int mode, const float* __restrict__ part,
+                                                                     float* __restrict__ G, int bias_off) {
+  __shared__ float red[COL_SPLIT][COL_RJ][COL_Q];
+  const int jl = threadIdx.x % COL_RJ, h = threadIdx.x / COL_RJ, j = blockIdx.x * COL_RJ + jl;
   float s[COL_Q];
 #pragma unroll
   for (int q = 0; q < COL_Q; q++) s[q] = 0.f;
-  for (int g = h; g < nblk; g += COL_SPLIT)
+  // chain h adds the partials of workgroups h, h + COL_SPLIT, ... in that order, four loaded
+  // together
+  int g = h;
+  for (; g + 3 * COL_SPLIT < nblk; g += 4 * COL_SPLIT) {
+    float v[4][COL_Q];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int q = 0; q < COL_Q; q++) v[u][q] = part[((long)(g + u * COL_SPLIT) * MLP_W + j) * COL_Q + q];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int q = 0; q < COL_Q; q++) s[q] += v[u][q];
+  }
+  for (; g < nblk; g += COL_SPLIT)
 #pragma unroll
     for (int q = 0; q < COL_Q; q++) s[q] += part[((long)g * MLP_W + j) * COL_Q + q];
 #pragma unroll
-  for (int q = 0; q < COL_Q; q++) red[h][j][q] = s[q];
+  for (int q = 0; q < COL_Q; q++) red[h][jl][q] = s[q];
+  __syncthreads();
+  // the chains' sums in a fixed tree: eight groups of eight in order, then the eight groups
+  if (h < COL_SPLIT / 8) {
+#pragma unroll
+    for (int q = 0; q < COL_Q; q++) {
+      float x = red[8 * h][jl][q];
+#pragma unroll
+      for (int k = 1; k < 8; k++) x += red[8 * h + k][jl][q];
+      red[8 * h][jl][q] = x;
+    }
+  }
   __syncthreads();
   if (h != 0) return;
 #pragma unroll
   for (int q = 0; q < COL_Q; q++) {
-    float x = red[0][j][q];
+    float x = red[0][jl][q];
 #pragma unroll
-    for (int k = 1; k < COL_SPLIT; k++) x += red[k][j][q];
+    for (int k = 1; k < COL_SPLIT / 8; k++) x += red[8 * k][jl][q];
     s[q] = x;
   }
   if (mode == 0) {
@@ -389,7 +503,7 @@ hipError_t gemm_bf16(int M, int N, int K, const uint16_t* A, long sam, long sak,
 
 static inline int col_blocks(int B) { return cdivi(B, COL_ROWS); }
 // K splits of the weight-gradient products dW = dZ^T H (K = the batch)
-static inline int grad_ksplit(int B) { return cdivi(B, 512); }
+static inline int grad_ksplit(int B) { return cdivi(B, 256); }
 
 size_t mlp_workspace_bytes(int B) {
   const size_t f = sizeof(float), h = sizeof(uint16_t), W = MLP_W;
@@ -429,7 +543,7 @@ struct MlpWs {
 static void col_sums(int B, int mode, const float* dY, const float* Z4, const float* dZ, const float* X, float* part, float* G,
                      int bias_off, hipStream_t st) {
   hipLaunchKernelGGL(k_col_partials, dim3(col_blocks(B)), dim3(256), 0, st, B, mode, dY, Z4, dZ, X, part);
-  hipLaunchKernelGGL(k_col_reduce, dim3(1), dim3(MLP_W * COL_SPLIT), 0, st, col_blocks(B), mode, part, G, bias_off);
+  hipLaunchKernelGGL(k_col_reduce, dim3(MLP_W / COL_RJ), dim3(COL_RJ * COL_SPLIT), 0, st, col_blocks(B), mode, part, G, bias_off);
 }
 
 static hipError_t forward(const float* P, const uint16_t* Wbf, const float* X, int B, MlpWs& w, hipStream_t st) {
