@@ -321,6 +321,18 @@ size_t tb_trace_noalloc_scratch_bytes(void);
    closed form of the reference's numpy.linalg.lstsq system (centred sums, fixed summation
    order).  Rays that are all parallel have no focus: the result is then not finite.
    Workspace of tb_focus_workspace_bytes() bytes. */
+/* No counterpart in the reference (which takes start states from the caller; the beams of its
+   examples are drawn with numpy on the host).  A seeded beam drawn on the device, so that a
+   sample -> trace sequence can be captured once as a HIP graph and replayed per seed:
+   state0 (n, 6) = (x, y, z0, vx, vy, vz) with (x, y) uniform in the disc of radius r_max,
+   kinetic energy e_median * exp(e_sigma * g) eV (g ~ N(0, 1)), the direction (tan ax, tan ay, 1)
+   normalised with ax, ay ~ N(0, angle_sigma), speed sqrt(2 E charge / mass) (charge: the
+   magnitude in C).  Draw k of particle i is output 8 i + k + 1 of SplitMix64 seeded with *seed,
+   read from device memory when the launch runs.  Allocates nothing. */
+typedef struct {
+  double z0, r_max, e_median, e_sigma, angle_sigma, mass, charge;
+} tb_beam_t;
+int tb_sample_beam(const tb_beam_t *beam, const unsigned long long *seed, int n, double *state0, void *stream);
 size_t tb_focus_workspace_bytes(void);
 int tb_focus_position(const double *states, int n, long long stride, double *out, void *workspace, void *stream);
 /* Stage-synchronous tracer for a few particles (n <= 65535) in fields with ring-source

@@ -132,6 +132,7 @@ _SIG = {
     "tb_trace_noalloc": (i32, [vp, i32, i32, vp]),
     "tb_focus_workspace_bytes": (sz, []),
     "tb_focus_position": (i32, [vp, i32, C.c_longlong, vp, vp, vp]),
+    "tb_sample_beam": (i32, [vp, vp, i32, vp, vp]),
     "tb_trace_noalloc_max_particles": (i32, []),
     "tb_trace_noalloc_scratch_bytes": (sz, []),
     "tb_trace_split_chunks": (i32, [i32, C.c_longlong]),

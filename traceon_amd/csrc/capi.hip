@@ -666,6 +666,16 @@ int tb_trace_noalloc(const tb_trace_params_t* params, int waves, int ordered, vo
   if (nw < 0) return -1;
   TB_CALL(tb::trace_noalloc(*reinterpret_cast<const tb::TraceParams*>(params), nw, ordered, S(stream)), "trace_noalloc");
 }
+int tb_sample_beam(const tb_beam_t* beam, const unsigned long long* seed, int n, double* state0, void* stream) {
+  if (!beam || !seed || (n > 0 && !state0) || n < 0) { set_err("tb_sample_beam: null beam, seed or state0, or n < 0"); return -1; }
+  if (!(beam->mass > 0.0) || !(beam->charge > 0.0) || !(beam->e_median > 0.0) || !(beam->r_max >= 0.0) ||
+      !(beam->e_sigma >= 0.0) || !(beam->angle_sigma >= 0.0)) {
+    set_err("tb_sample_beam: mass, |charge| and e_median must be > 0; r_max, e_sigma and angle_sigma >= 0");
+    return -1;
+  }
+  static_assert(sizeof(tb_beam_t) == sizeof(tb::BeamSpec), "tb_beam_t layout");
+  TB_CALL(tb::sample_beam(*reinterpret_cast<const tb::BeamSpec*>(beam), seed, n, state0, S(stream)), "sample_beam");
+}
 size_t tb_focus_workspace_bytes(void) { return tb::focus_workspace_bytes(); }
 int tb_focus_position(const double* states, int n, long long stride, double* out, void* workspace, void* stream) {
   if (n < 2 || stride < 6 || !states || !out || !workspace) { set_err("tb_focus_position: needs n >= 2 rays, stride >= 6"); return -1; }

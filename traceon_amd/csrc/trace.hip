@@ -1250,6 +1250,51 @@ __global__ void k_zero_counter(int* c) {
 
 size_t trace_static_scratch_bytes() { return STATIC_SCRATCH_BYTES; }
 
+// SplitMix64 (Steele, Lea and Flood 2014): output j of the generator seeded with s is
+// mix(s + j * golden), j = 1, 2, ...; the mixer is Stafford's variant 13.
+__device__ __forceinline__ unsigned long long splitmix64_at(unsigned long long s, unsigned long long j) {
+  unsigned long long z = s + j * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+// uniform in [0, 1): the top 53 bits
+__device__ __forceinline__ double unit_draw(unsigned long long s, long i, int k) {
+  return (double)(splitmix64_at(s, 8ull * (unsigned long long)i + (unsigned long long)k + 1ull) >> 11) * 0x1.0p-53;
+}
+
+__global__ __launch_bounds__(256) void k_sample_beam(BeamSpec B, const unsigned long long* __restrict__ seed, int n,
+                                                     double* __restrict__ S) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const unsigned long long s = *seed;
+  const double two_pi = 6.283185307179586476925;
+  // draws 0, 1: the start point, r^2 uniform over the disc
+  const double r = B.r_max * sqrt(unit_draw(s, i, 0));
+  const double phi = two_pi * unit_draw(s, i, 1);
+  // draws 2, 3 and 4, 5: two Box-Muller pairs (1 - u keeps the log finite); the second pair's
+  // cosine branch is the energy's normal deviate
+  const double m1 = sqrt(-2.0 * log(1.0 - unit_draw(s, i, 2))), a1 = two_pi * unit_draw(s, i, 3);
+  const double m2 = sqrt(-2.0 * log(1.0 - unit_draw(s, i, 4))), a2 = two_pi * unit_draw(s, i, 5);
+  const double tx = tan(B.angle_sigma * m1 * cos(a1)), ty = tan(B.angle_sigma * m1 * sin(a1));
+  const double energy = B.e_median * exp(B.e_sigma * m2 * cos(a2));
+  const double speed = sqrt(2.0 * energy * B.charge / B.mass);
+  const double inv = speed / sqrt(tx * tx + ty * ty + 1.0);
+  double* o = S + 6 * i;
+  o[0] = r * cos(phi);
+  o[1] = r * sin(phi);
+  o[2] = B.z0;
+  o[3] = tx * inv;
+  o[4] = ty * inv;
+  o[5] = inv;
+}
+
+hipError_t sample_beam(const BeamSpec& B, const unsigned long long* seed, int n, double* state0, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sample_beam, dim3((n + 255) / 256), dim3(256), 0, st, B, seed, n, state0);
+  return hipGetLastError();
+}
+
 hipError_t trace_noalloc(const TraceParams& P, int waves, int ordered, hipStream_t st) {
   if (P.n > TRACE_STATIC_MAX_PARTICLES) return hipErrorInvalidValue;
   void* base = nullptr;

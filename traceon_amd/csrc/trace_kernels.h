@@ -126,6 +126,15 @@ hipError_t trace_order(const double* state0, int n, int* order, void* ws, hipStr
 constexpr int TRACE_STATIC_MAX_PARTICLES = 1 << 21;
 size_t trace_static_scratch_bytes();
 hipError_t trace_noalloc(const TraceParams& P, int waves, int ordered, hipStream_t st);
+// Seeded beam of n particles on the device (layout of tb_beam_t): start points uniform in a
+// disc of radius r_max on the plane z = z0, kinetic energy e_median exp(e_sigma g), angles
+// tan^-1 of the direction's x/z and y/z slopes N(0, angle_sigma), non-relativistic speed.
+// Draw k of particle i is output 8 i + k + 1 of SplitMix64 seeded with *seed (device memory,
+// so a captured launch takes a new seed per replay).
+struct BeamSpec {
+  double z0, r_max, e_median, e_sigma, angle_sigma, mass, charge;
+};
+hipError_t sample_beam(const BeamSpec& B, const unsigned long long* seed, int n, double* state0, hipStream_t st);
 // Least-squares focus of n rays given by states (x, y, z, vx, vy, vz) `stride` doubles apart
 // (focus.py:9-43): out = (X, Y, Z).  ws: focus_workspace_bytes().
 size_t focus_workspace_bytes();

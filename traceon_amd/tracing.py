@@ -294,6 +294,91 @@ def run_trace(program: DeviceFieldProgram, state0, qm, bounds, atol=1e-8, plane=
     return out
 
 
+class BeamT(C.Structure):
+    """tb_beam_t: a seeded beam drawn on the device (tb_sample_beam)."""
+    _fields_ = [(name, C.c_double) for name in ("z0", "r_max", "e_median", "e_sigma", "angle_sigma", "mass", "charge")]
+
+
+def sample_beam(n: int, seed, z0: float, r_max: float, e_median: float, e_sigma: float = 0.0, angle_sigma: float = 0.0,
+                mass: float = M_E, charge: float = E_CHARGE, out=None):
+    """(n, 6) start states drawn on the device (tb_sample_beam): (x, y) uniform in the disc of
+    radius r_max on z = z0, kinetic energy e_median exp(e_sigma g) eV, the direction
+    (tan ax, tan ay, 1) normalised with ax, ay ~ N(0, angle_sigma), non-relativistic speed.
+    seed: an int or a one-element uint64/int64 device tensor (read when the launch runs)."""
+    import torch
+    from . import device
+    dev = device.device()
+    if not torch.is_tensor(seed):
+        seed = torch.tensor([int(seed) & ((1 << 63) - 1)], dtype=torch.int64, device=dev)
+    if out is None:
+        out = torch.empty((n, 6), dtype=torch.float64, device=dev)
+    assert out.shape == (n, 6) and out.dtype == torch.float64 and out.is_contiguous()
+    spec = BeamT(z0, r_max, e_median, e_sigma, angle_sigma, mass, abs(charge))
+    backend.check(backend.backend_lib.tb_sample_beam(C.byref(spec), seed.data_ptr(), n, out.data_ptr(), device.stream_ptr()),
+                  "sample_beam")
+    return out
+
+
+class BeamPipeline:
+    """Sample -> trace captured once as a HIP graph and replayed per seed: tb_sample_beam
+    writes the start states, tb_trace_noalloc (the persistent tracer with its counter, queue
+    order and sort workspace in the library's static scratch array, so nothing is allocated
+    inside the capture) traces them.  Each call writes a new seed into device memory, replays
+    the graph on the current stream and returns the same result buffers (copy what must be
+    kept).  One pipeline at a time per GPU: every tb_trace_noalloc shares the scratch array.
+    Particles of a learned field whose step needs ring sources are summed in the persistent
+    kernel (no parking), as tb_trace with park = 0."""
+
+    def __init__(self, tracer, n: int, beam: dict, atol: float = 1e-8, plane=None, plane2=None,
+                 charge: float = -E_CHARGE, mass: float = M_E, shape: LaunchShape | None = None):
+        import torch
+        from . import device
+        L = backend.backend_lib
+        if n > L.tb_trace_noalloc_max_particles():
+            raise ValueError(f"{n} particles exceed the static maximum of {L.tb_trace_noalloc_max_particles()}")
+        dev = device.device()
+        self.n = n
+        self.beam = dict(beam)
+        self.beam.setdefault("mass", mass)
+        self.beam.setdefault("charge", abs(charge))
+        self.seed = torch.zeros(1, dtype=torch.int64, device=dev)
+        self.state0 = torch.empty((n, 6), dtype=torch.float64, device=dev)
+        program = tracer.program()
+        self.P, self.out = trace_params(program, self.state0, charge / mass, tracer.bounds, atol, plane, 0, None, 1 << 62, plane2)
+        shape = shape or DEFAULT_SHAPE
+        self.P.batch, self.P.occupancy, self.P.park = int(shape.batch), int(shape.occupancy), 0
+        occ = shape.occupancy if 1 <= shape.occupancy <= 4 else 2
+        cus = torch.cuda.get_device_properties(dev).multi_processor_count
+        self.waves = int(shape.waves) or cus * (4 * occ if not program.has_sources else 8)
+        self.ordered = int(bool(shape.ordered))
+        self._program = program
+        self.graph = None
+        self._run_eager()   # first launches outside the capture (the scratch array's set-up)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self._launch()
+        self.graph = g
+
+    def _launch(self):
+        from . import device
+        sample_beam(self.n, self.seed, out=self.state0, **self.beam)
+        backend.check(backend.backend_lib.tb_trace_noalloc(C.byref(self.P), self.waves, self.ordered, device.stream_ptr()),
+                      "trace_noalloc")
+
+    def _run_eager(self):
+        self._launch()
+
+    def __call__(self, seed: int, graph: bool = True) -> BatchResult:
+        """Trace the beam of this seed; graph=False launches the same kernels one by one."""
+        self.seed.fill_(int(seed) & ((1 << 63) - 1))
+        if graph:
+            self.graph.replay()
+        else:
+            self._run_eager()
+        return self.out
+
+
 def _trace_split(program: DeviceFieldProgram, P, n, dev):
     import torch
     from . import device
