@@ -255,6 +255,29 @@ def tracing_leg(n_particles, n_sample, n_axial=7000):
            "n_axial": len(axial.z), "axial_build_s": t_axial_build, "N_elements": len(field.electrostatic_point_charges),
            "launch_shape": shape_str(shape), "tuning_ms": {shape_str(k): round(v, 3) for k, v in tuning.items()}}
     out["roofline"] = roofline_trace(out["steps_attempted"], res["ms"], 1)
+    # The same workload as one captured HIP graph per beam (tracing.BeamPipeline): the start
+    # states drawn on the device (tb_sample_beam: the same distributions, another generator)
+    # and traced by the no-alloc tracer in the autotuned shape; one replay timed with HIP
+    # events on the stream it runs on, the sampling included.
+    if 0 < n_particles <= 1 << 21:
+        from traceon_amd.tracing import BeamPipeline
+        pipe = BeamPipeline(tr_ax, n_particles, dict(z0=-5.0 * mm, r_max=0.1 * mm, e_median=1000.0, e_sigma=0.5,
+                                                     angle_sigma=5e-3), atol=1e-8, plane=exit_plane, plane2=back_plane,
+                            shape=shape)
+        pipe(1)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        r = pipe(2)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        exit_ok = torch.isfinite(r.landing[:, 0])
+        back_only = torch.isfinite(r.landing2[:, 0]) & ~exit_ok
+        out["graph_pipeline"] = {"n": n_particles, "ms": ms, "electrons_per_s": n_particles / (ms * 1e-3),
+                                 "reflected_fraction": float(back_only.double().mean().item()),
+                                 "steps_attempted": int(r.n_attempted.sum().item())}
+        del pipe, r
     # accuracy: against the FP64 oracle's direct BEM traces (reference C backend, converged
     # diagonal) of the beam's first particles, and against the GPU's direct FP64 BEM tracer
     # (itself within 1e-9 m of the oracle) on a larger sample
