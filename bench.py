@@ -35,6 +35,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16": 2500e12, "hbm": 8.0e12}
+# MI355X: 256 CUs of 4 SIMDs, 16 lanes each, at 2.4 GHz (the FP64 vector peak above is
+# 256 * 4 * 16 * 2 FLOP * 2.4e9)
+PEAK_CUS, PEAK_CLOCK_HZ = 256, 2.4e9
+# VALU instructions per potential pair of the fill's branch-free loop (gfx950 assembly)
+FILL_VALU_PER_PAIR = 82
 # HBM bytes per launch of the LU's bulk-update GEMM (k_gemm_x6) in this workload
 # (N = 20,000): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE averaged over its 59 launches in one
 # factorisation, each scaled by what the counter reads for a 256 MiB device-to-device copy
@@ -570,6 +575,13 @@ def main():
     rf = {"kernel": "k_fill (dense FP64 fill)", "bound": "fp64-valu", "achieved": fill_flops / (fill_ms * 1e-3) / 1e12,
           "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s", "flop_per_launch": fill_flops, "launch_ms": fill_ms}
     rf["frac"] = rf["achieved"] / rf["peak"]
+    # the fill issues more VALU instructions than FLOPs: 82 per potential pair in the gfx950
+    # assembly of its branch-free loop (register copies, conversions, compares besides the FP64
+    # arithmetic), each wave64 instruction 4 cycles on a 16-lane SIMD
+    if fill_flops == float(N) * N * 16.0 * C_POT:   # potential rows only (the mirror)
+        issue_peak = PEAK_CUS * 4 * 16 * PEAK_CLOCK_HZ
+        rf["valu_instr_per_pair"] = FILL_VALU_PER_PAIR
+        rf["valu_issue_frac"] = FILL_VALU_PER_PAIR * N * N * 16.0 / (fill_ms * 1e-3) / issue_peak
     result["roofline_fill"] = rf
     if not args.no_field:
         ms, pairs, P, Fd, pts = field_grid_leg(X, lines, types, values)
