@@ -207,49 +207,81 @@ __global__ __launch_bounds__(256) void k_mlp_in(int B, const float* __restrict__
 // last layer from the FP32 pre-activations of layer 4: Y = silu(Z4) W5^T + b5.
 // With targets T: dY = 2 (Y - T) / (5 B) and the squared error summed over the block's rows
 // into loss[blockIdx.x].
+// output layer Y = silu(Z4) W5^T + b5 (5 outputs) and, with targets T, dY = 2 (Y - T) / (5 B)
+// and the squared error of each workgroup's samples.  A wave takes OUT_SPW samples one after
+// the other: lane l reads z[l] and z[l + 64] (two coalesced 256-byte rows), and the five dot
+// products are added over the lanes by a fixed butterfly.
+constexpr int OUT_SPW = 4, OUT_SPB = 4 * OUT_SPW;   // samples per wave, per 256-thread workgroup
 __global__ __launch_bounds__(256) void k_mlp_out(int B, const float* __restrict__ Z4, const float* __restrict__ P,
                                                  float* __restrict__ Y, const float* __restrict__ T, float* __restrict__ dY,
                                                  float* __restrict__ loss) {
-  __shared__ float red[256];
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ float red[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const float* W5 = P + MLP_OFF_W5;
+  float w0[MLP_OUT], w1[MLP_OUT];
+#pragma unroll
+  for (int o = 0; o < MLP_OUT; o++) {
+    w0[o] = W5[o * MLP_W + lane];
+    w1[o] = W5[o * MLP_W + 64 + lane];
+  }
+  const int b0 = blockIdx.x * OUT_SPB + wave * OUT_SPW;
+  float zv[OUT_SPW][2];
+#pragma unroll
+  for (int u = 0; u < OUT_SPW; u++) {
+    const int b = b0 + u;
+    zv[u][0] = b < B ? Z4[(long)b * MLP_W + lane] : 0.f;
+    zv[u][1] = b < B ? Z4[(long)b * MLP_W + 64 + lane] : 0.f;
+  }
   float se = 0.f;
-  if (b < B) {
-    const float* W5 = P + MLP_OFF_W5;
+#pragma unroll
+  for (int u = 0; u < OUT_SPW; u++) {
+    const int b = b0 + u;
+    const float h0 = silu(zv[u][0]), h1 = silu(zv[u][1]);
     float y[MLP_OUT];
 #pragma unroll
-    for (int o = 0; o < MLP_OUT; o++) y[o] = P[MLP_OFF_B5 + o];
-    const float* z = Z4 + (long)b * MLP_W;
-    for (int j = 0; j < MLP_W; j++) {
-      const float h = silu(z[j]);
+    for (int o = 0; o < MLP_OUT; o++) y[o] = fmaf(w1[o], h1, w0[o] * h0);
 #pragma unroll
-      for (int o = 0; o < MLP_OUT; o++) y[o] = fmaf(W5[o * MLP_W + j], h, y[o]);
-    }
+    for (int m = 32; m >= 1; m >>= 1)
 #pragma unroll
-    for (int o = 0; o < MLP_OUT; o++) {
-      if (Y) Y[(long)b * MLP_OUT + o] = y[o];
-      if (T) {
-        const float d = y[o] - T[(long)b * MLP_OUT + o];
-        se = fmaf(d, d, se);
-        dY[(long)b * MLP_OUT + o] = 2.0f * d / (float)(MLP_OUT * (long)B);
+      for (int o = 0; o < MLP_OUT; o++) y[o] += __shfl_xor(y[o], m);
+    if (b < B) {
+#pragma unroll
+      for (int o = 0; o < MLP_OUT; o++) {
+        const float yo = y[o] + P[MLP_OFF_B5 + o];
+        if (lane == o) {
+          if (Y) Y[(long)b * MLP_OUT + o] = yo;
+          if (T) {
+            const float d = yo - T[(long)b * MLP_OUT + o];
+            se = fmaf(d, d, se);
+            dY[(long)b * MLP_OUT + o] = 2.0f * d / (float)(MLP_OUT * (long)B);
+          }
+        }
       }
     }
   }
   if (!T) return;
-  red[threadIdx.x] = se;
+  // the block's squared error: lanes 0-4 of each wave, then the four waves, in order
+  float x = 0.f;
+#pragma unroll
+  for (int o = 0; o < MLP_OUT; o++) x += __shfl(se, o);
+  if (lane == 0) red[wave] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) loss[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
+// *loss = the n block sums of k_mlp_out in a fixed order: thread t adds blocks t, t + 256, ...,
+// then a fixed tree over the 256 threads
+__global__ __launch_bounds__(256) void k_loss_reduce(int n, const float* __restrict__ part, float* __restrict__ loss) {
+  __shared__ float red[256];
+  float x = 0.f;
+  for (int i = threadIdx.x; i < n; i += 256) x += part[i];
+  red[threadIdx.x] = x;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
     if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) loss[blockIdx.x] = red[0];
-}
-
-// *loss = the n block sums of k_mlp_out, added in block order
-__global__ void k_loss_reduce(int n, const float* __restrict__ part, float* __restrict__ loss) {
-  if (threadIdx.x != 0) return;
-  float x = 0.f;
-  for (int i = 0; i < n; i++) x += part[i];
-  *loss = x;
+  if (threadIdx.x == 0) *loss = red[0];
 }
 
 // backward through the last layer and the layer-4 activation:
@@ -515,7 +547,7 @@ size_t mlp_workspace_bytes(int B) {
          + f * (size_t)B * MLP_OUT      // Y
          + f * (size_t)col_blocks(B) * W * COL_Q   // column partial sums
          + f * (size_t)ksplit_count(B, grad_ksplit(B)) * W * W   // K-split partials of a weight gradient
-         + f * (size_t)cdivi(B, 256)    // loss per block of k_mlp_out
+         + f * (size_t)cdivi(B, OUT_SPB)    // loss per block of k_mlp_out
          + 64;
 }
 
@@ -535,7 +567,7 @@ struct MlpWs {
     Y = (float*)p; p += sizeof(float) * B * MLP_OUT;
     part = (float*)p; p += sizeof(float) * (size_t)col_blocks(B) * W * COL_Q;
     kpart = (float*)p; p += sizeof(float) * (size_t)ksplit_count(B, grad_ksplit(B)) * W * W;
-    lpart = (float*)p; p += sizeof(float) * (size_t)cdivi(B, 256);
+    lpart = (float*)p; p += sizeof(float) * (size_t)cdivi(B, OUT_SPB);
     loss = (float*)p;
   }
 };
@@ -566,7 +598,7 @@ hipError_t mlp_forward(const float* P, const uint16_t* Wbf, const float* X, int 
   MlpWs w(ws, B);
   hipError_t e = forward(P, Wbf, X, B, w, st);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_mlp_out, dim3(cdivi(B, 256)), dim3(256), 0, st, B, w.Z[3], P, Y, (const float*)nullptr, (float*)nullptr,
+  hipLaunchKernelGGL(k_mlp_out, dim3(cdivi(B, OUT_SPB)), dim3(256), 0, st, B, w.Z[3], P, Y, (const float*)nullptr, (float*)nullptr,
                      (float*)nullptr);
   return hipGetLastError();
 }
@@ -580,8 +612,8 @@ hipError_t mlp_grad(const float* P, const uint16_t* Wbf, const float* X, const f
   // G starts at zero (the hidden weight gradients are added into it)
   hipLaunchKernelGGL(k_zero_f32, dim3(cdivi(MLP_NPARAM, 256)), dim3(256), 0, st, G, (long)MLP_NPARAM);
   const int g = cdivi((long)B * MLP_W, 256);
-  hipLaunchKernelGGL(k_mlp_out, dim3(cdivi(B, 256)), dim3(256), 0, st, B, w.Z[3], P, w.Y, T, w.dY, w.lpart);
-  hipLaunchKernelGGL(k_loss_reduce, dim3(1), dim3(64), 0, st, cdivi(B, 256), (const float*)w.lpart, loss);
+  hipLaunchKernelGGL(k_mlp_out, dim3(cdivi(B, OUT_SPB)), dim3(256), 0, st, B, w.Z[3], P, w.Y, T, w.dY, w.lpart);
+  hipLaunchKernelGGL(k_loss_reduce, dim3(1), dim3(256), 0, st, cdivi(B, OUT_SPB), (const float*)w.lpart, loss);
   // layer 5
   col_sums(B, 0, w.dY, w.Z[3], nullptr, nullptr, w.part, G, 0, st);   // dW5 and db5
   hipLaunchKernelGGL(k_out_bwd, dim3(g), dim3(256), 0, st, B, w.dY, w.Z[3], P, w.dZ, w.dZbf[0]);
