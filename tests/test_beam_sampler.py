@@ -65,3 +65,23 @@ def test_beam_states_shapes_and_ranges():
     E = 0.5 * 9.1093837015e-31 * np.sum(S[:, 3:] ** 2, axis=1) / 1.602176634e-19
     assert abs(np.mean(np.log(E / 1000.0))) < 0.02 and abs(np.std(np.log(E)) - 0.5) < 0.01
     assert abs(np.std(np.arctan(S[:, 3] / S[:, 5])) - 5e-3) < 1e-4
+
+
+def test_sample_beam_rejects_bad_arguments_without_a_gpu():
+    """tb_sample_beam checks its arguments on the host before any launch; BeamPipeline refuses
+    more particles than the no-alloc tracer's static maximum before touching the device."""
+    import ctypes as C
+    import pytest
+    from traceon_amd import backend, tracing
+    L = backend.backend_lib
+    seed = C.c_ulonglong(1)
+    good = tracing.BeamT(-5e-3, 1e-4, 1000.0, 0.5, 5e-3, 9.1093837015e-31, 1.602176634e-19)
+    for field, value in (("mass", 0.0), ("charge", -1.0), ("e_median", 0.0), ("r_max", -1.0), ("e_sigma", float("nan"))):
+        bad = tracing.BeamT(*[getattr(good, f) for f, _ in tracing.BeamT._fields_])
+        setattr(bad, field, value)
+        assert L.tb_sample_beam(C.byref(bad), C.byref(seed), 4, C.c_void_p(16), None) == -1
+        assert b"tb_sample_beam" in L.tb_last_error()
+    assert L.tb_sample_beam(None, C.byref(seed), 4, C.c_void_p(16), None) == -1
+    assert L.tb_sample_beam(C.byref(good), C.byref(seed), -1, C.c_void_p(16), None) == -1
+    with pytest.raises(ValueError, match="static maximum"):
+        tracing.BeamPipeline(None, L.tb_trace_noalloc_max_particles() + 1, {})
