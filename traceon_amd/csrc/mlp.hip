@@ -207,14 +207,16 @@ __global__ __launch_bounds__(256) void k_mlp_in(int B, const float* __restrict__
 // last layer from the FP32 pre-activations of layer 4: Y = silu(Z4) W5^T + b5.
 // With targets T: dY = 2 (Y - T) / (5 B) and the squared error summed over the block's rows
 // into loss[blockIdx.x].
-// output layer Y = silu(Z4) W5^T + b5 (5 outputs) and, with targets T, dY = 2 (Y - T) / (5 B)
-// and the squared error of each workgroup's samples.  A wave takes OUT_SPW samples one after
+// output layer Y = silu(Z4) W5^T + b5 (5 outputs) and, with targets T, dY = 2 (Y - T) / (5 B),
+// the squared error of each workgroup's samples and, with dZ, the backward step through the
+// last layer and the layer-4 activation, dZ4 = (dY W5) * silu'(Z4) in FP32 and BF16 (from the
+// Z4 rows and W5 columns the wave already holds).  A wave takes OUT_SPW samples one after
 // the other: lane l reads z[l] and z[l + 64] (two coalesced 256-byte rows), and the five dot
 // products are added over the lanes by a fixed butterfly.
 constexpr int OUT_SPW = 4, OUT_SPB = 4 * OUT_SPW;   // samples per wave, per 256-thread workgroup
 __global__ __launch_bounds__(256) void k_mlp_out(int B, const float* __restrict__ Z4, const float* __restrict__ P,
                                                  float* __restrict__ Y, const float* __restrict__ T, float* __restrict__ dY,
-                                                 float* __restrict__ loss) {
+                                                 float* __restrict__ loss, float* __restrict__ dZ, uint16_t* __restrict__ dZbf) {
   __shared__ float red[4];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const float* W5 = P + MLP_OFF_W5;
@@ -245,6 +247,7 @@ __global__ __launch_bounds__(256) void k_mlp_out(int B, const float* __restrict_
 #pragma unroll
       for (int o = 0; o < MLP_OUT; o++) y[o] += __shfl_xor(y[o], m);
     if (b < B) {
+      float dyv = 0.f;   // lane o: dY[b][o]
 #pragma unroll
       for (int o = 0; o < MLP_OUT; o++) {
         const float yo = y[o] + P[MLP_OFF_B5 + o];
@@ -253,9 +256,26 @@ __global__ __launch_bounds__(256) void k_mlp_out(int B, const float* __restrict_
           if (T) {
             const float d = yo - T[(long)b * MLP_OUT + o];
             se = fmaf(d, d, se);
-            dY[(long)b * MLP_OUT + o] = 2.0f * d / (float)(MLP_OUT * (long)B);
+            dyv = 2.0f * d / (float)(MLP_OUT * (long)B);
+            dY[(long)b * MLP_OUT + o] = dyv;
           }
         }
+      }
+      if (T && dZ) {
+        // the stored dY values (lane o's) to every lane, then dZ4 for columns lane and lane + 64
+        float dh0 = 0.f, dh1 = 0.f;
+#pragma unroll
+        for (int o = 0; o < MLP_OUT; o++) {
+          const float dyo = __shfl(dyv, o);
+          dh0 = fmaf(dyo, w0[o], dh0);
+          dh1 = fmaf(dyo, w1[o], dh1);
+        }
+        const float dz0 = dh0 * dsilu(zv[u][0]), dz1 = dh1 * dsilu(zv[u][1]);
+        const long e = (long)b * MLP_W + lane;
+        dZ[e] = dz0;
+        dZ[e + 64] = dz1;
+        dZbf[e] = f2bf(dz0);
+        dZbf[e + 64] = f2bf(dz1);
       }
     }
   }
@@ -282,22 +302,6 @@ __global__ __launch_bounds__(256) void k_loss_reduce(int n, const float* __restr
     __syncthreads();
   }
   if (threadIdx.x == 0) *loss = red[0];
-}
-
-// backward through the last layer and the layer-4 activation:
-// dZ4 = (dY W5) * silu'(Z4), in FP32 and BF16
-__global__ __launch_bounds__(256) void k_out_bwd(int B, const float* __restrict__ dY, const float* __restrict__ Z4,
-                                                 const float* __restrict__ P, float* __restrict__ dZ, uint16_t* __restrict__ dZbf) {
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (long)B * MLP_W) return;
-  const int b = (int)(e / MLP_W), j = (int)(e % MLP_W);
-  const float* W5 = P + MLP_OFF_W5;
-  float dh = 0.f;
-#pragma unroll
-  for (int o = 0; o < MLP_OUT; o++) dh = fmaf(dY[(long)b * MLP_OUT + o], W5[o * MLP_W + j], dh);
-  const float dz = dh * dsilu(Z4[e]);
-  dZ[e] = dz;
-  dZbf[e] = f2bf(dz);
 }
 
 // small weight gradients, sums over the batch per column j of the width-128 layer:
@@ -599,7 +603,7 @@ hipError_t mlp_forward(const float* P, const uint16_t* Wbf, const float* X, int 
   hipError_t e = forward(P, Wbf, X, B, w, st);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_mlp_out, dim3(cdivi(B, OUT_SPB)), dim3(256), 0, st, B, w.Z[3], P, Y, (const float*)nullptr, (float*)nullptr,
-                     (float*)nullptr);
+                     (float*)nullptr, (float*)nullptr, (uint16_t*)nullptr);
   return hipGetLastError();
 }
 
@@ -611,12 +615,11 @@ hipError_t mlp_grad(const float* P, const uint16_t* Wbf, const float* X, const f
   if (e != hipSuccess) return e;
   // G starts at zero (the hidden weight gradients are added into it)
   hipLaunchKernelGGL(k_zero_f32, dim3(cdivi(MLP_NPARAM, 256)), dim3(256), 0, st, G, (long)MLP_NPARAM);
-  const int g = cdivi((long)B * MLP_W, 256);
-  hipLaunchKernelGGL(k_mlp_out, dim3(cdivi(B, OUT_SPB)), dim3(256), 0, st, B, w.Z[3], P, w.Y, T, w.dY, w.lpart);
+  // layer 5 forward, loss, and backward to dZ4 (dZbf[0])
+  hipLaunchKernelGGL(k_mlp_out, dim3(cdivi(B, OUT_SPB)), dim3(256), 0, st, B, w.Z[3], P, w.Y, T, w.dY, w.lpart, w.dZ, w.dZbf[0]);
   hipLaunchKernelGGL(k_loss_reduce, dim3(1), dim3(256), 0, st, cdivi(B, OUT_SPB), (const float*)w.lpart, loss);
   // layer 5
   col_sums(B, 0, w.dY, w.Z[3], nullptr, nullptr, w.part, G, 0, st);   // dW5 and db5
-  hipLaunchKernelGGL(k_out_bwd, dim3(g), dim3(256), 0, st, B, w.dY, w.Z[3], P, w.dZ, w.dZbf[0]);
   // layers 4, 3, 2
   const int woff[3] = {MLP_OFF_W2, MLP_OFF_W3, MLP_OFF_W4};
   const int boff[3] = {MLP_OFF_B2, MLP_OFF_B3, MLP_OFF_B4};
