@@ -209,23 +209,20 @@ __global__ __launch_bounds__(256) void k_mlp_in(int B, const float* __restrict__
 // into loss[blockIdx.x].
 // output layer Y = silu(Z4) W5^T + b5 (5 outputs) and, with targets T, dY = 2 (Y - T) / (5 B),
 // the squared error of each workgroup's samples and, with dZ, the backward step through the
-// last layer and the layer-4 activation, dZ4 = (dY W5) * silu'(Z4) in FP32 and BF16 (from the
-// Z4 rows and W5 columns the wave already holds).  A wave takes OUT_SPW samples one after
-// the other: lane l reads z[l] and z[l + 64] (two coalesced 256-byte rows), and the five dot
-// products are added over the lanes by a fixed butterfly.
+// last layer and the layer-4 activation, dZ4 = (dY W5) * silu'(Z4) in FP32 and BF16.
+// A wave takes OUT_SPW samples: lane l reads z[l] and z[l + 64] of each (coalesced 256-byte
+// rows) and stages silu(z) in LDS; lane 5u + o then forms output o of sample u as
+// b5[o] + sum_n W5[o][n] h[n], one FMA per neuron in the order n = 0..127 (the order of the
+// tracer's in-wave evaluation, so that both give the same correction); dZ4 comes from the
+// z and W5 values the lanes already hold.
 constexpr int OUT_SPW = 4, OUT_SPB = 4 * OUT_SPW;   // samples per wave, per 256-thread workgroup
 __global__ __launch_bounds__(256) void k_mlp_out(int B, const float* __restrict__ Z4, const float* __restrict__ P,
                                                  float* __restrict__ Y, const float* __restrict__ T, float* __restrict__ dY,
                                                  float* __restrict__ loss, float* __restrict__ dZ, uint16_t* __restrict__ dZbf) {
+  __shared__ float hs[4][OUT_SPW][MLP_W];
   __shared__ float red[4];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const float* W5 = P + MLP_OFF_W5;
-  float w0[MLP_OUT], w1[MLP_OUT];
-#pragma unroll
-  for (int o = 0; o < MLP_OUT; o++) {
-    w0[o] = W5[o * MLP_W + lane];
-    w1[o] = W5[o * MLP_W + 64 + lane];
-  }
   const int b0 = blockIdx.x * OUT_SPB + wave * OUT_SPW;
   float zv[OUT_SPW][2];
 #pragma unroll
@@ -233,43 +230,53 @@ __global__ __launch_bounds__(256) void k_mlp_out(int B, const float* __restrict_
     const int b = b0 + u;
     zv[u][0] = b < B ? Z4[(long)b * MLP_W + lane] : 0.f;
     zv[u][1] = b < B ? Z4[(long)b * MLP_W + 64 + lane] : 0.f;
+    hs[wave][u][lane] = silu(zv[u][0]);
+    hs[wave][u][64 + lane] = silu(zv[u][1]);
   }
-  float se = 0.f;
+  __syncthreads();
+  // lane 5u + o: output o of sample u
+  const int tu = lane / MLP_OUT, to = lane % MLP_OUT, tb = b0 + tu;
+  const bool active = lane < OUT_SPW * MLP_OUT && tb < B;
+  float se = 0.f, dyv = 0.f;
+  if (active) {
+    float y = P[MLP_OFF_B5 + to];
+    const float* w = W5 + to * MLP_W;
+    const float* h = hs[wave][tu];
+    for (int n = 0; n < MLP_W; n += 4) {
+      const float4 h4 = *reinterpret_cast<const float4*>(h + n);   // LDS, 16-byte aligned
+      y = fmaf(w[n], h4.x, y);
+      y = fmaf(w[n + 1], h4.y, y);
+      y = fmaf(w[n + 2], h4.z, y);
+      y = fmaf(w[n + 3], h4.w, y);
+    }
+    if (Y) Y[(long)tb * MLP_OUT + to] = y;
+    if (T) {
+      const float d = y - T[(long)tb * MLP_OUT + to];
+      se = d * d;
+      dyv = 2.0f * d / (float)(MLP_OUT * (long)B);
+      dY[(long)tb * MLP_OUT + to] = dyv;
+    }
+  }
+  if (!T) return;
+  if (dZ) {
+    float w0[MLP_OUT], w1[MLP_OUT];
 #pragma unroll
-  for (int u = 0; u < OUT_SPW; u++) {
-    const int b = b0 + u;
-    const float h0 = silu(zv[u][0]), h1 = silu(zv[u][1]);
-    float y[MLP_OUT];
+    for (int o = 0; o < MLP_OUT; o++) {
+      w0[o] = W5[o * MLP_W + lane];
+      w1[o] = W5[o * MLP_W + 64 + lane];
+    }
 #pragma unroll
-    for (int o = 0; o < MLP_OUT; o++) y[o] = fmaf(w1[o], h1, w0[o] * h0);
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1)
-#pragma unroll
-      for (int o = 0; o < MLP_OUT; o++) y[o] += __shfl_xor(y[o], m);
-    if (b < B) {
-      float dyv = 0.f;   // lane o: dY[b][o]
+    for (int u = 0; u < OUT_SPW; u++) {
+      // sample u's dY from lanes 5u..5u+4, then dZ4 for columns lane and lane + 64
+      float dh0 = 0.f, dh1 = 0.f;
 #pragma unroll
       for (int o = 0; o < MLP_OUT; o++) {
-        const float yo = y[o] + P[MLP_OFF_B5 + o];
-        if (lane == o) {
-          if (Y) Y[(long)b * MLP_OUT + o] = yo;
-          if (T) {
-            const float d = yo - T[(long)b * MLP_OUT + o];
-            se = fmaf(d, d, se);
-            dyv = 2.0f * d / (float)(MLP_OUT * (long)B);
-            dY[(long)b * MLP_OUT + o] = dyv;
-          }
-        }
+        const float dyo = __shfl(dyv, MLP_OUT * u + o);
+        dh0 = fmaf(dyo, w0[o], dh0);
+        dh1 = fmaf(dyo, w1[o], dh1);
       }
-      if (T && dZ) {
-        // the stored dY values (lane o's) to every lane, then dZ4 for columns lane and lane + 64
-        float dh0 = 0.f, dh1 = 0.f;
-#pragma unroll
-        for (int o = 0; o < MLP_OUT; o++) {
-          const float dyo = __shfl(dyv, o);
-          dh0 = fmaf(dyo, w0[o], dh0);
-          dh1 = fmaf(dyo, w1[o], dh1);
-        }
+      const int b = b0 + u;
+      if (b < B) {
         const float dz0 = dh0 * dsilu(zv[u][0]), dz1 = dh1 * dsilu(zv[u][1]);
         const long e = (long)b * MLP_W + lane;
         dZ[e] = dz0;
@@ -279,11 +286,10 @@ __global__ __launch_bounds__(256) void k_mlp_out(int B, const float* __restrict_
       }
     }
   }
-  if (!T) return;
-  // the block's squared error: lanes 0-4 of each wave, then the four waves, in order
+  // the block's squared error: lanes 0..19 of each wave in order, then the four waves
   float x = 0.f;
 #pragma unroll
-  for (int o = 0; o < MLP_OUT; o++) x += __shfl(se, o);
+  for (int t = 0; t < OUT_SPW * MLP_OUT; t++) x += __shfl(se, t);
   if (lane == 0) red[wave] = x;
   __syncthreads();
   if (threadIdx.x == 0) loss[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];

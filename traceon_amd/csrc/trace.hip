@@ -214,7 +214,10 @@ typedef float mw_f4 __attribute__((ext_vector_type(4)));
 // rounding points): the input layer per lane in FP32 with BF16 activations, three 64 x 128
 // x 128 products with FP32 accumulation, bias and SiLU, BF16 activations in between, the last
 // SiLU kept in FP32 and the output layer per lane.  Must be called by every lane of the wave
-// (uniform control flow); lanes with need = false compute on a dummy point and add nothing.
+// (uniform control flow); lanes with need = false add nothing.  The points that need the
+// correction are packed into the first rows (a lane's row is its rank among them), and only
+// the 16-row tiles that hold them are multiplied: each row's result does not depend on its
+// position, so a wave with 16 needing lanes or fewer runs a quarter of the products.
 // Operand layout of the instruction: lane l holds A[l%16][4(l/16)+i] (points x inputs),
 // B[4(l/16)+i][l%16] (inputs x outputs: W[out][in] read row-wise) and D[4(l/16)+v][l%16].
 __device__ void mlp_wave(const FieldComp& c, const double q[3], bool need, int lane, double e[3], double h[3]) {
@@ -226,16 +229,22 @@ __device__ void mlp_wave(const FieldComp& c, const double q[3], bool need, int l
   const float u = need ? (float)(2.0 * (r - r_lo) / (r_max - r_lo) - 1.0) : 0.f;
   const float v = need ? (float)(2.0 * (q[2] - z0) / (z1 - z0) - 1.0) : 0.f;
   cfloat_t* W = uniform_weights(c.mlp);
+  const unsigned long long needers = __ballot(need);
+  const int count = __popcll(needers);
+  const int row = need ? __popcll(needers & ((1ull << lane) - 1ull)) : 0;   // rank among the needing lanes
+  const int mt = (count + 15) >> 4;                                          // 16-row tiles in use
   // input layer: this lane's row of A0
+  if (need) {
 #pragma unroll 4
-  for (int j = 0; j < 128; j += 4) {
-    uint64_t pk = 0;
+    for (int j = 0; j < 128; j += 4) {
+      uint64_t pk = 0;
 #pragma unroll
-    for (int t = 0; t < 4; t++) {
-      const float z = fmaf(W[2 * (j + t)], u, fmaf(W[2 * (j + t) + 1], v, W[TMLP_B1 + j + t]));
-      pk |= (uint64_t)f2bf(silu(z)) << (16 * t);
+      for (int t = 0; t < 4; t++) {
+        const float z = fmaf(W[2 * (j + t)], u, fmaf(W[2 * (j + t) + 1], v, W[TMLP_B1 + j + t]));
+        pk |= (uint64_t)f2bf(silu(z)) << (16 * t);
+      }
+      *reinterpret_cast<uint64_t*>(&A0[row * MW_S + j]) = pk;
     }
-    *reinterpret_cast<uint64_t*>(&A0[lane * MW_S + j]) = pk;
   }
   __syncthreads();
   const int lr = lane & 15, lq = lane >> 4;
@@ -260,27 +269,30 @@ __device__ void mlp_wave(const FieldComp& c, const double q[3], bool need, int l
         const mw_s4 bfr = *reinterpret_cast<const mw_s4*>(Wl + (16 * nb + lr) * 128 + 16 * kb + 4 * lq);
 #pragma unroll
         for (int mb = 0; mb < 4; mb++) {
-          const mw_s4 afr = *reinterpret_cast<const mw_s4*>(Ain + (16 * mb + lr) * MW_S + 16 * kb + 4 * lq);
-          acc[mb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(afr, bfr, acc[mb], 0, 0, 0);
+          if (mb < mt) {   // wave-uniform
+            const mw_s4 afr = *reinterpret_cast<const mw_s4*>(Ain + (16 * mb + lr) * MW_S + 16 * kb + 4 * lq);
+            acc[mb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(afr, bfr, acc[mb], 0, 0, 0);
+          }
         }
       }
       const int n = 16 * nb + lr;
       const float bn = bias[n];
 #pragma unroll
       for (int mb = 0; mb < 4; mb++)
+        if (mb < mt)
 #pragma unroll
-        for (int vv = 0; vv < 4; vv++) {
-          const int pt = 16 * mb + 4 * lq + vv;
-          const float a = silu(acc[mb][vv] + bn);
-          if (layer < 2) Aout[pt * MW_S + n] = f2bf(a);
-          else Fo[pt * MW_SF + n - 16 * nb0] = a;
-        }
+          for (int vv = 0; vv < 4; vv++) {
+            const int pt = 16 * mb + 4 * lq + vv;
+            const float a = silu(acc[mb][vv] + bn);
+            if (layer < 2) Aout[pt * MW_S + n] = f2bf(a);
+            else Fo[pt * MW_SF + n - 16 * nb0] = a;
+          }
     }
     __syncthreads();
     if (layer == 2) {   // output layer over this pass's 64 neurons
       const int n0 = 16 * nb0;
       for (int n = 0; n < 64; n++) {
-        const float hv = Fo[lane * MW_SF + n];
+        const float hv = Fo[row * MW_SF + n];
 #pragma unroll
         for (int o = 0; o < 5; o++) y[o] = fmaf(W[TMLP_W5 + o * 128 + n0 + n], hv, y[o]);
       }
